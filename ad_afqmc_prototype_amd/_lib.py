@@ -1,0 +1,87 @@
+"""Loader for libafqmc_hip.so (the gfx950 kernels behind include/afqmc_hip.h).
+
+There is no CPU fallback: if the library is missing or no HIP device is
+present, every entry point raises.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import threading
+
+_LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libafqmc_hip.so")
+_lock = threading.Lock()
+_lib = None
+
+c_int = ctypes.c_int
+c_ll = ctypes.c_longlong
+c_double = ctypes.c_double
+c_vp = ctypes.c_void_p
+
+# name -> argtypes (restype is int for every compute entry point)
+_SIGNATURES = {
+    "afq_dgemm_nt": [c_int, c_int, c_int, c_double, c_vp, c_ll, c_vp, c_ll, c_double, c_vp, c_ll, c_vp],
+    "afq_overlap_greens": [c_int, c_int, c_int, c_int, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp],
+    "afq_force_bias": [c_int, c_int, c_int, c_double, c_vp, c_vp, c_vp, c_vp],
+    "afq_field_shifts": [c_int, c_int, c_double, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp],
+    "afq_vhs": [c_int, c_int, c_int, c_int, c_vp, c_vp, c_vp, c_vp],
+    "afq_trotter": [c_int, c_int, c_int, c_int, c_int, c_int, c_double, c_vp, c_vp, c_vp, c_vp,
+                    c_vp, c_vp, c_vp],
+    "afq_phaseless_update": [c_int, c_int, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_double, c_double,
+                             c_double, c_double, c_double, c_double, c_double, c_double, c_vp,
+                             c_vp, c_vp, c_vp],
+    "afq_orthonormalize": [c_int, c_int, c_int, c_vp, c_vp, c_vp],
+    "afq_energy": [c_int, c_int, c_int, c_int, c_int, c_double, c_vp, c_vp, c_int, c_vp, c_vp, c_vp],
+    "afq_block_energy": [c_int, c_vp, c_vp, c_double, c_double, c_double, c_vp, c_vp, c_vp],
+    "afq_comb_indices": [c_int, c_vp, c_double, c_vp, c_vp, c_vp, c_vp],
+    "afq_gather_rows": [c_int, c_ll, c_int, c_ll, c_ll, c_vp, c_vp, c_vp, c_vp],
+}
+
+ABI_VERSION = 1
+
+
+class HipLibraryError(RuntimeError):
+    pass
+
+
+def library_path() -> str:
+    return _LIB_PATH
+
+
+def load() -> ctypes.CDLL:
+    """Load and type the library (no device work)."""
+    global _lib
+    with _lock:
+        if _lib is not None:
+            return _lib
+        if not os.path.exists(_LIB_PATH):
+            raise HipLibraryError(
+                f"{_LIB_PATH} is missing: build it with "
+                "`python -m ad_afqmc_prototype_amd.csrc.build` (there is no CPU fallback)")
+        lib = ctypes.CDLL(_LIB_PATH)
+        lib.afq_last_error.restype = ctypes.c_char_p
+        lib.afq_last_error.argtypes = []
+        lib.afq_abi_version.restype = c_int
+        lib.afq_abi_version.argtypes = []
+        if lib.afq_abi_version() != ABI_VERSION:
+            raise HipLibraryError(f"libafqmc_hip ABI {lib.afq_abi_version()} != {ABI_VERSION}: rebuild")
+        for name, args in _SIGNATURES.items():
+            fn = getattr(lib, name)
+            fn.argtypes = args
+            fn.restype = c_int
+        _lib = lib
+        return lib
+
+
+def exported_symbols() -> list[str]:
+    return ["afq_last_error", "afq_abi_version", *_SIGNATURES.keys()]
+
+
+def call(name: str, *args) -> None:
+    lib = load()
+    rc = getattr(lib, name)(*args)
+    if rc != 0:
+        msg = lib.afq_last_error().decode(errors="replace")
+        if rc == -1:
+            raise ValueError(msg)
+        raise HipLibraryError(msg)

@@ -1,0 +1,34 @@
+// Error reporting shared by every entry point of libafqmc_hip.so.
+#include <cstdarg>
+#include <cstdio>
+
+#include "afqmc_hip.h"
+#include "common.h"
+
+namespace {
+thread_local char g_last_error[512] = "";
+}
+
+namespace afq {
+
+void set_error(const char* fmt, ...) {
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_last_error, sizeof(g_last_error), fmt, ap);
+    va_end(ap);
+}
+
+int check_launch(const char* what) {
+    hipError_t err = hipGetLastError();
+    if (err != hipSuccess) {
+        set_error("%s: launch failed: %s", what, hipGetErrorString(err));
+        return -2;
+    }
+    return 0;
+}
+
+}  // namespace afq
+
+extern "C" const char* afq_last_error(void) { return g_last_error; }
+
+extern "C" int afq_abi_version(void) { return 1; }

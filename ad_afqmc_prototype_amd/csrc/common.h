@@ -1,0 +1,55 @@
+// Shared definitions for the gfx950 phaseless-AFQMC kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+typedef double f64x4 __attribute__((ext_vector_type(4)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+namespace afq {
+
+// Thread-local last-error text, read back through afq_last_error().
+void set_error(const char* fmt, ...);
+// Converts a pending launch error into a return code (0 ok, -2 launch failure).
+int check_launch(const char* what);
+
+constexpr int kWave = 64;
+constexpr int kXcds = 8;  // MI355X: 8 XCDs, workgroups dispatched round-robin
+
+__host__ __device__ inline int ceil_div(int a, int b) { return (a + b - 1) / b; }
+__host__ __device__ inline int round_up(int a, int b) { return ceil_div(a, b) * b; }
+
+// Bijective remap of a linear workgroup id so that each XCD receives a
+// contiguous range of tiles (blocks b, b+8, b+16, ... run on one XCD).
+__device__ inline int xcd_contiguous(int b, int total) {
+    const int x = b % kXcds;
+    const int local = b / kXcds;
+    const int base = total / kXcds;
+    const int rem = total % kXcds;
+    const int offset = x * base + (x < rem ? x : rem);
+    return offset + local;
+}
+
+// v_mfma_f64_16x16x4_f64.  Lane l supplies A[l%16][l/16] and B[l/16][l%16];
+// the accumulator holds D[4*(l/16)+r][l%16] in element r.
+__device__ __forceinline__ f64x4 mfma_f64(double a, double b, f64x4 c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+}
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+}  // namespace afq
+
+#define AFQ_REQUIRE(cond, ...)         \
+    do {                               \
+        if (!(cond)) {                 \
+            afq::set_error(__VA_ARGS__); \
+            return -1;                 \
+        }                              \
+    } while (0)

@@ -1,0 +1,215 @@
+// Overlap matrix, LU log-determinant and half-rotated Green's function.
+//
+// One 256-thread workgroup per (walker, spin):
+//   1. m = C^T w (o x o complex) into LDS;
+//   2. LU with partial pivoting on |re| + |im| (the LAPACK zgetrf pivot rule
+//      used by jnp.linalg.det / jnp.linalg.solve), multipliers formed with the
+//      pivot's reciprocal;
+//   3. log det m = sum log(U_kk) + i pi (#row swaps);
+//   4. each thread owns rows p of w and solves x m = w_p in registers:
+//      u U = w_p (forward), v L = u (backward), x[perm[k]] = v[k];
+//      G[i][p] = x_i is written to the split-plane greens buffer, coalesced
+//      along p.
+#include <cmath>
+
+#include "afqmc_hip.h"
+#include "common.h"
+
+namespace {
+
+constexpr int THREADS = 256;
+
+__device__ __forceinline__ double2 cmul(double2 a, double2 b) {
+    return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+
+__device__ __forceinline__ double2 crecip(double2 a) {
+    // 1/a by the scaled formula (Smith's method) to avoid overflow.
+    if (fabs(a.x) >= fabs(a.y)) {
+        const double r = a.y / a.x;
+        const double d = a.x + a.y * r;
+        return make_double2(1.0 / d, -r / d);
+    }
+    const double r = a.x / a.y;
+    const double d = a.x * r + a.y;
+    return make_double2(r / d, -1.0 / d);
+}
+
+template <int MAXO>
+__global__ __launch_bounds__(THREADS) void greens_kernel(
+    int nw, int n, int o, int nspin, const double2* __restrict__ wup,
+    const double2* __restrict__ wdn, const double* __restrict__ cup, const double* __restrict__ cdn,
+    double* __restrict__ greens, double2* __restrict__ logdet) {
+    __shared__ double2 m[MAXO][MAXO + 1];
+    __shared__ double2 dinv[MAXO];
+    __shared__ int perm[MAXO];
+    __shared__ int piv_s;
+
+    const int b = blockIdx.x;
+    const int w = b / nspin;
+    const int s = b - w * nspin;
+    const double2* W = (s == 0 ? wup : wdn) + (long long)w * n * o;
+    const double* Cm = (s == 0 ? cup : cdn);
+    const int tid = threadIdx.x;
+    const int lane = tid & 63;
+    const int wave = tid >> 6;
+
+    // 1. m = C^T w
+    for (int e = tid; e < o * o; e += THREADS) {
+        const int i = e / o;
+        const int j = e - i * o;
+        double re = 0.0, im = 0.0;
+        for (int p = 0; p < n; ++p) {
+            const double c = Cm[p * o + i];
+            const double2 x = W[p * o + j];
+            re += c * x.x;
+            im += c * x.y;
+        }
+        m[i][j] = make_double2(re, im);
+    }
+    if (tid < o) perm[tid] = tid;
+    __syncthreads();
+
+    // 2. LU with partial pivoting
+    int nswap = 0;
+    for (int k = 0; k < o; ++k) {
+        if (wave == 0) {
+            double best = -1.0;
+            int bi = k;
+            for (int r = k + lane; r < o; r += 64) {
+                const double v = fabs(m[r][k].x) + fabs(m[r][k].y);
+                if (v > best) {
+                    best = v;
+                    bi = r;
+                }
+            }
+#pragma unroll
+            for (int off = 32; off > 0; off >>= 1) {
+                const double ob = __shfl_xor(best, off, 64);
+                const int oi = __shfl_xor(bi, off, 64);
+                if (ob > best || (ob == best && oi < bi)) {
+                    best = ob;
+                    bi = oi;
+                }
+            }
+            if (lane == 0) piv_s = bi;
+        }
+        __syncthreads();
+        const int piv = piv_s;
+        if (piv != k) {
+            for (int c = tid; c < o; c += THREADS) {
+                const double2 t = m[k][c];
+                m[k][c] = m[piv][c];
+                m[piv][c] = t;
+            }
+            if (tid == 0) {
+                const int t = perm[k];
+                perm[k] = perm[piv];
+                perm[piv] = t;
+            }
+            ++nswap;
+        }
+        __syncthreads();
+        const double2 inv = crecip(m[k][k]);
+        for (int r = k + 1 + tid; r < o; r += THREADS) m[r][k] = cmul(m[r][k], inv);
+        __syncthreads();
+        const int rem = o - k - 1;
+        for (int e = tid; e < rem * rem; e += THREADS) {
+            const int r = k + 1 + e / rem;
+            const int c = k + 1 + e % rem;
+            const double2 t = cmul(m[r][k], m[k][c]);
+            m[r][c].x -= t.x;
+            m[r][c].y -= t.y;
+        }
+        __syncthreads();
+    }
+
+    // 3. log det and pivot reciprocals
+    if (tid < o) dinv[tid] = crecip(m[tid][tid]);
+    if (tid == 0) {
+        double lr = 0.0, li = 0.0;
+        for (int k = 0; k < o; ++k) {
+            lr += log(hypot(m[k][k].x, m[k][k].y));
+            li += atan2(m[k][k].y, m[k][k].x);
+        }
+        if (nswap & 1) li += M_PI;
+        logdet[(long long)w * nspin + s] = make_double2(lr, li);
+    }
+    __syncthreads();
+
+    // 4. row solves x m = w_p
+    const long long ldg = (long long)nspin * o * n;
+    double* gre = greens + (long long)w * ldg + (long long)s * o * n;
+    double* gim = gre + (long long)nw * ldg;
+    for (int p = tid; p < n; p += THREADS) {
+        // keep the LU factors in LDS: without this barrier the compiler hoists
+        // all o^2 loads out of the row loop into registers and spills
+        asm volatile("" ::: "memory");
+        double2 x[MAXO];
+#pragma unroll
+        for (int j = 0; j < MAXO; ++j) x[j] = (j < o) ? W[p * o + j] : make_double2(0.0, 0.0);
+        // u U = w
+#pragma unroll
+        for (int j = 0; j < MAXO; ++j) {
+            if (j < o) {
+                x[j] = cmul(x[j], dinv[j]);
+#pragma unroll
+                for (int c = j + 1; c < MAXO; ++c) {
+                    if (c < o) {
+                        const double2 t = cmul(x[j], m[j][c]);
+                        x[c].x -= t.x;
+                        x[c].y -= t.y;
+                    }
+                }
+            }
+        }
+        // v L = u
+#pragma unroll
+        for (int j = MAXO - 1; j >= 0; --j) {
+            if (j < o) {
+#pragma unroll
+                for (int c = 0; c < j; ++c) {
+                    const double2 t = cmul(x[j], m[j][c]);
+                    x[c].x -= t.x;
+                    x[c].y -= t.y;
+                }
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < MAXO; ++k) {
+            if (k < o) {
+                const int i = perm[k];
+                gre[(long long)i * n + p] = x[k].x;
+                gim[(long long)i * n + p] = x[k].y;
+            }
+        }
+    }
+}
+
+}  // namespace
+
+extern "C" int afq_overlap_greens(int nw, int n, int o, int nspin, const double* walkers_up,
+                                  const double* walkers_dn, const double* trial_up,
+                                  const double* trial_dn, double* greens, double* logdet,
+                                  void* stream) {
+    AFQ_REQUIRE(nw > 0 && n > 0 && o > 0 && o <= n, "afq_overlap_greens: bad sizes nw=%d n=%d o=%d", nw, n, o);
+    AFQ_REQUIRE(nspin == 1 || nspin == 2, "afq_overlap_greens: nspin must be 1 or 2");
+    AFQ_REQUIRE(walkers_up && trial_up && greens && logdet, "afq_overlap_greens: null pointer");
+    AFQ_REQUIRE(nspin == 1 || (walkers_dn && trial_dn), "afq_overlap_greens: null spin-down pointer");
+    AFQ_REQUIRE(o <= 64, "afq_overlap_greens: o=%d exceeds the supported maximum of 64", o);
+    const dim3 grid((unsigned)(nw * nspin));
+    hipStream_t st = (hipStream_t)stream;
+    const double2* wu = (const double2*)walkers_up;
+    const double2* wd = (const double2*)(nspin == 2 ? walkers_dn : walkers_up);
+    const double* cd = nspin == 2 ? trial_dn : trial_up;
+    double2* ld = (double2*)logdet;
+    if (o <= 8)
+        hipLaunchKernelGGL(greens_kernel<8>, grid, dim3(THREADS), 0, st, nw, n, o, nspin, wu, wd, trial_up, cd, greens, ld);
+    else if (o <= 16)
+        hipLaunchKernelGGL(greens_kernel<16>, grid, dim3(THREADS), 0, st, nw, n, o, nspin, wu, wd, trial_up, cd, greens, ld);
+    else if (o <= 32)
+        hipLaunchKernelGGL(greens_kernel<32>, grid, dim3(THREADS), 0, st, nw, n, o, nspin, wu, wd, trial_up, cd, greens, ld);
+    else
+        hipLaunchKernelGGL(greens_kernel<64>, grid, dim3(THREADS), 0, st, nw, n, o, nspin, wu, wd, trial_up, cd, greens, ld);
+    return afq::check_launch("afq_overlap_greens");
+}

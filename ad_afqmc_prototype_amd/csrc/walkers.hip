@@ -1,0 +1,187 @@
+// Walker-container kernels: orthonormalization (QR), comb reconfiguration
+// indices and the row gather.
+#include <cmath>
+
+#include "afqmc_hip.h"
+#include "common.h"
+
+namespace {
+
+constexpr int THREADS = 256;
+
+// ---------------------------------------------------------------------------
+// Reduced QR by classical Gram-Schmidt with one re-orthogonalisation pass
+// (CGS2).  R_jj = ||v_j|| is real and positive, so Q is the unique Q of the
+// diag(R) >= 0 convention of walkers.py:169-180 (LAPACK geqrf + phase fix),
+// equal to it to rounding for numerically full-rank walkers.  One workgroup per
+// walker; the walker (n x o complex) sits in LDS with an odd row pitch.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(THREADS) void qr_kernel(int n, int o, const double2* __restrict__ win,
+                                                      double2* __restrict__ wout) {
+    extern __shared__ double2 q[];
+    __shared__ double2 r[64];
+    __shared__ double scale_s;
+    const int pitch = o + 1;
+    const int w = blockIdx.x;
+    const double2* W = win + (long long)w * n * o;
+    double2* Q = wout + (long long)w * n * o;
+    const int tid = threadIdx.x;
+    const int lane = tid & 63;
+    const int wave = tid >> 6;
+
+    for (int e = tid; e < n * o; e += THREADS) {
+        const int p = e / o;
+        const int j = e - p * o;
+        q[p * pitch + j] = W[e];
+    }
+    __syncthreads();
+
+    for (int j = 0; j < o; ++j) {
+        for (int pass = 0; pass < 2; ++pass) {
+            for (int i = wave; i < j; i += 4) {
+                double sr = 0.0, si = 0.0;
+                for (int p = lane; p < n; p += 64) {
+                    const double2 a = q[p * pitch + i];
+                    const double2 b = q[p * pitch + j];
+                    sr += a.x * b.x + a.y * b.y;  // conj(a) b
+                    si += a.x * b.y - a.y * b.x;
+                }
+                sr = afq::wave_sum(sr);
+                si = afq::wave_sum(si);
+                if (lane == 0) r[i] = make_double2(sr, si);
+            }
+            __syncthreads();
+            for (int p = tid; p < n; p += THREADS) {
+                double2 v = q[p * pitch + j];
+                for (int i = 0; i < j; ++i) {
+                    const double2 a = q[p * pitch + i];
+                    const double2 c = r[i];
+                    v.x -= a.x * c.x - a.y * c.y;
+                    v.y -= a.x * c.y + a.y * c.x;
+                }
+                q[p * pitch + j] = v;
+            }
+            __syncthreads();
+        }
+        if (wave == 0) {
+            double s = 0.0;
+            for (int p = lane; p < n; p += 64) {
+                const double2 b = q[p * pitch + j];
+                s += b.x * b.x + b.y * b.y;
+            }
+            s = afq::wave_sum(s);
+            if (lane == 0) scale_s = 1.0 / sqrt(s);
+        }
+        __syncthreads();
+        const double sc = scale_s;
+        for (int p = tid; p < n; p += THREADS) {
+            double2 v = q[p * pitch + j];
+            v.x *= sc;
+            v.y *= sc;
+            q[p * pitch + j] = v;
+        }
+        __syncthreads();
+    }
+
+    for (int e = tid; e < n * o; e += THREADS) {
+        const int p = e / o;
+        const int j = e - p * o;
+        Q[e] = q[p * pitch + j];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Comb indices.  The cumulative sum is one sequential left-to-right fp64 pass
+// (numpy.cumsum order) and z_k = (tot * (k + zeta)) / nw is evaluated without
+// contraction, so the indices are reproducible bit for bit.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(1024) void comb_kernel(int nw, const double* __restrict__ weights,
+                                                     double zeta, double* __restrict__ cw,
+                                                     int* __restrict__ idx,
+                                                     double* __restrict__ wnew) {
+#pragma clang fp contract(off)
+    __shared__ double tot_s;
+    if (threadIdx.x == 0) {
+        double s = 0.0;
+        for (int i = 0; i < nw; ++i) {
+            s = s + fabs(weights[i]);
+            cw[i] = s;
+        }
+        tot_s = s;
+        __threadfence();
+    }
+    __syncthreads();
+    const double tot = tot_s;
+    const double avg = tot / (double)nw;
+    for (int k = threadIdx.x; k < nw; k += blockDim.x) {
+        const double kz = (double)k + zeta;
+        const double prod = tot * kz;
+        const double z = prod / (double)nw;
+        int lo = 0, hi = nw;
+        while (lo < hi) {
+            const int mid = (lo + hi) >> 1;
+            if (cw[mid] < z)
+                lo = mid + 1;
+            else
+                hi = mid;
+        }
+        idx[k] = lo < nw ? lo : nw - 1;
+        wnew[k] = avg;
+    }
+}
+
+__global__ void gather_kernel(int nrows, long long row_len, int nplanes, long long sps,
+                              long long dps, const int* __restrict__ idx,
+                              const double* __restrict__ src, double* __restrict__ dst) {
+    const long long per_plane = (long long)nrows * row_len;
+    const long long total = per_plane * nplanes;
+    for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < total;
+         e += (long long)gridDim.x * blockDim.x) {
+        const long long p = e / per_plane;
+        const long long rem = e - p * per_plane;
+        const long long k = rem / row_len;
+        const long long c = rem - k * row_len;
+        dst[p * dps + k * row_len + c] = src[p * sps + (long long)idx[k] * row_len + c];
+    }
+}
+
+}  // namespace
+
+extern "C" int afq_orthonormalize(int nw, int n, int o, const double* win, double* wout,
+                                  void* stream) {
+    AFQ_REQUIRE(nw > 0 && n > 0 && o > 0 && o <= n, "afq_orthonormalize: bad sizes nw=%d n=%d o=%d", nw, n, o);
+    AFQ_REQUIRE(o <= 64, "afq_orthonormalize: o=%d exceeds the supported maximum of 64", o);
+    AFQ_REQUIRE(win && wout && win != wout, "afq_orthonormalize: null or aliased pointer");
+    const size_t lds = (size_t)n * (o + 1) * sizeof(double2);
+    AFQ_REQUIRE(lds + 64 * sizeof(double2) + 64 <= 160 * 1024,
+                "afq_orthonormalize: walker n=%d o=%d does not fit in LDS", n, o);
+    if (lds > 64 * 1024) {
+        (void)hipFuncSetAttribute((const void*)qr_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        (void)hipGetLastError();
+    }
+    hipLaunchKernelGGL(qr_kernel, dim3((unsigned)nw), dim3(THREADS), lds, (hipStream_t)stream, n, o,
+                       (const double2*)win, (double2*)wout);
+    return afq::check_launch("afq_orthonormalize");
+}
+
+extern "C" int afq_comb_indices(int nw, const double* weights, double zeta, double* cumsum,
+                                int* idx, double* weights_new, void* stream) {
+    AFQ_REQUIRE(nw > 0, "afq_comb_indices: nw=%d", nw);
+    AFQ_REQUIRE(weights && cumsum && idx && weights_new, "afq_comb_indices: null pointer");
+    hipLaunchKernelGGL(comb_kernel, dim3(1), dim3(1024), 0, (hipStream_t)stream, nw, weights, zeta,
+                       cumsum, idx, weights_new);
+    return afq::check_launch("afq_comb_indices");
+}
+
+extern "C" int afq_gather_rows(int nrows, long long row_len, int nplanes, long long src_plane_stride,
+                               long long dst_plane_stride, const int* idx, const double* src,
+                               double* dst, void* stream) {
+    AFQ_REQUIRE(nrows > 0 && row_len > 0 && nplanes > 0, "afq_gather_rows: bad sizes");
+    AFQ_REQUIRE(idx && src && dst && src != dst, "afq_gather_rows: null or aliased pointer");
+    const long long total = (long long)nrows * row_len * nplanes;
+    long long blocks = (total + 255) / 256;
+    if (blocks > 65536) blocks = 65536;
+    hipLaunchKernelGGL(gather_kernel, dim3((unsigned)blocks), dim3(256), 0, (hipStream_t)stream,
+                       nrows, row_len, nplanes, src_plane_stride, dst_plane_stride, idx, src, dst);
+    return afq::check_launch("afq_gather_rows");
+}

@@ -1,0 +1,289 @@
+"""Typed torch-tensor wrappers around the C ABI of libafqmc_hip.so.
+
+Every function checks device, dtype, contiguity and shape on the host, then
+enqueues the kernel on the current HIP stream of the tensors' device.  There is
+no fallback: a tensor that is not on a HIP device raises.
+"""
+from __future__ import annotations
+
+import torch
+
+from . import _lib
+
+f64 = torch.float64
+c128 = torch.complex128
+
+
+def _stream(t: torch.Tensor) -> int:
+    return torch.cuda.current_stream(t.device).cuda_stream
+
+
+# Optional per-entry-point timing with HIP events recorded on the launch stream
+# (used by the benchmark; off by default).
+_timing = None
+
+
+class timing:
+    """Context manager: records (start, end) HIP events around every library
+    call, on the stream the kernel is enqueued on."""
+
+    def __init__(self):
+        self.events = {}
+
+    def __enter__(self):
+        global _timing
+        _timing = self
+        return self
+
+    def __exit__(self, *exc):
+        global _timing
+        _timing = None
+        return False
+
+    def summary(self):
+        torch.cuda.synchronize()
+        out = {}
+        for name, evs in self.events.items():
+            ms = [a.elapsed_time(b) for a, b in evs]
+            out[name] = {"count": len(ms), "avg_ms": sum(ms) / len(ms), "total_ms": sum(ms)}
+        return out
+
+
+def _call(name, stream_tensor, *args):
+    if _timing is None:
+        _lib.call(name, *args)
+        return
+    st = torch.cuda.current_stream(stream_tensor.device)
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record(st)
+    _lib.call(name, *args)
+    e1.record(st)
+    _timing.events.setdefault(name, []).append((e0, e1))
+
+
+def _ptr(t):
+    return None if t is None else t.data_ptr()
+
+
+def _check(t: torch.Tensor, dtype, shape, name: str, device=None):
+    if not isinstance(t, torch.Tensor):
+        raise TypeError(f"{name}: expected a torch.Tensor")
+    if t.device.type != "cuda":
+        raise ValueError(f"{name}: tensor must live on a HIP device (got {t.device}); "
+                         "the HIP path has no CPU fallback")
+    if device is not None and t.device != device:
+        raise ValueError(f"{name}: tensor on {t.device}, expected {device}")
+    if t.dtype != dtype:
+        raise TypeError(f"{name}: dtype {t.dtype}, expected {dtype}")
+    if shape is not None and tuple(t.shape) != tuple(shape):
+        raise ValueError(f"{name}: shape {tuple(t.shape)}, expected {tuple(shape)}")
+    if not t.is_contiguous():
+        raise ValueError(f"{name}: tensor must be contiguous")
+
+
+def ldv_for(n: int) -> int:
+    """Row pitch of VHS / exp_h1_half / rotated Cholesky rows (32-byte rows)."""
+    return (n + 3) // 4 * 4
+
+
+def dgemm_nt(A, B, C, alpha=1.0, beta=0.0):
+    """C = alpha * A @ B.T + beta * C (fp64 MFMA)."""
+    M, K = A.shape
+    N = B.shape[0]
+    _check(A, f64, (M, K), "A")
+    _check(B, f64, (N, K), "B", A.device)
+    _check(C, f64, (M, N), "C", A.device)
+    _call("afq_dgemm_nt", A, M, N, K, float(alpha), A.data_ptr(), K, B.data_ptr(), K, float(beta),
+              C.data_ptr(), N, _stream(A))
+    return C
+
+
+def overlap_greens(walkers, trial_up, trial_dn=None, greens=None, logdet=None):
+    """Returns (greens planes (2, nw, nspin*o*n), logdet complex128 (nw, nspin))."""
+    wu, wd = (walkers if isinstance(walkers, tuple) else (walkers, None))
+    nspin = 1 if wd is None else 2
+    nw, n, o = wu.shape
+    _check(wu, c128, (nw, n, o), "walkers")
+    dev = wu.device
+    if wd is not None:
+        _check(wd, c128, (nw, n, o), "walkers_dn", dev)
+    _check(trial_up, f64, (n, o), "trial_up", dev)
+    if nspin == 2:
+        trial_dn = trial_up if trial_dn is None else trial_dn
+        _check(trial_dn, f64, (n, o), "trial_dn", dev)
+    if greens is None:
+        greens = torch.empty((2, nw, nspin * o * n), dtype=f64, device=dev)
+    if logdet is None:
+        logdet = torch.empty((nw, nspin), dtype=c128, device=dev)
+    _check(greens, f64, (2, nw, nspin * o * n), "greens", dev)
+    _check(logdet, c128, (nw, nspin), "logdet", dev)
+    _call("afq_overlap_greens", wu, nw, n, o, nspin, wu.data_ptr(), _ptr(wd), trial_up.data_ptr(),
+              _ptr(trial_dn if nspin == 2 else None), greens.data_ptr(), logdet.data_ptr(),
+              _stream(wu))
+    return greens, logdet
+
+
+def force_bias(greens, rot, alpha, fb=None):
+    """fb planes (2, nw, naux) = alpha * greens_rows . rot^T."""
+    _, nw, k = greens.shape
+    naux = rot.shape[0]
+    _check(greens, f64, (2, nw, k), "greens")
+    _check(rot, f64, (naux, k), "rot", greens.device)
+    if fb is None:
+        fb = torch.empty((2, nw, naux), dtype=f64, device=greens.device)
+    _check(fb, f64, (2, nw, naux), "fb", greens.device)
+    _call("afq_force_bias", greens, nw, naux, k, float(alpha), greens.data_ptr(), rot.data_ptr(),
+              fb.data_ptr(), _stream(greens))
+    return fb
+
+
+def field_shifts(fields, fb, mf, sqrt_dt, xbar=None, shift_term=None, fb_term=None):
+    nw, naux = fields.shape
+    dev = fields.device
+    _check(fields, f64, (nw, naux), "fields")
+    _check(fb, f64, (2, nw, naux), "fb", dev)
+    _check(mf, c128, (naux,), "mf_shifts", dev)
+    if xbar is None:
+        xbar = torch.empty((2, nw, naux), dtype=f64, device=dev)
+    if shift_term is None:
+        shift_term = torch.empty((nw,), dtype=c128, device=dev)
+    if fb_term is None:
+        fb_term = torch.empty((nw,), dtype=c128, device=dev)
+    _call("afq_field_shifts", fields, nw, naux, float(sqrt_dt), fields.data_ptr(), fb.data_ptr(),
+              mf.data_ptr(), xbar.data_ptr(), shift_term.data_ptr(), fb_term.data_ptr(),
+              _stream(fields))
+    return xbar, shift_term, fb_term
+
+
+def vhs(xbar, chol_t, n, vhs_out=None):
+    """VHS planes (2, nw, n, ldv) from shifted-field planes (2, nw, naux)."""
+    _, nw, naux = xbar.shape
+    ldv = ldv_for(n)
+    _check(xbar, f64, (2, nw, naux), "xbar")
+    _check(chol_t, f64, (n * ldv, naux), "chol_t", xbar.device)
+    if vhs_out is None:
+        vhs_out = torch.empty((2, nw, n, ldv), dtype=f64, device=xbar.device)
+    _check(vhs_out, f64, (2, nw, n, ldv), "vhs", xbar.device)
+    _call("afq_vhs", xbar, nw, n, ldv, naux, xbar.data_ptr(), chol_t.data_ptr(), vhs_out.data_ptr(),
+              _stream(xbar))
+    return vhs_out
+
+
+def trotter(walkers, vhs_planes, exp_h1_half_pad, sqrt_dt, n_terms, out=None):
+    wu, wd = (walkers if isinstance(walkers, tuple) else (walkers, None))
+    nspin = 1 if wd is None else 2
+    nw, n, o = wu.shape
+    ldv = ldv_for(n)
+    dev = wu.device
+    _check(wu, c128, (nw, n, o), "walkers")
+    if wd is not None:
+        _check(wd, c128, (nw, n, o), "walkers_dn", dev)
+    _check(vhs_planes, f64, (2, nw, n, ldv), "vhs", dev)
+    _check(exp_h1_half_pad, f64, (n, ldv), "exp_h1_half", dev)
+    if out is None:
+        out = (torch.empty_like(wu), torch.empty_like(wd)) if nspin == 2 else torch.empty_like(wu)
+    ou, od = (out if isinstance(out, tuple) else (out, None))
+    _call("afq_trotter", wu, nw, n, o, nspin, ldv, int(n_terms), float(sqrt_dt),
+              exp_h1_half_pad.data_ptr(), vhs_planes.data_ptr(), wu.data_ptr(), _ptr(wd),
+              ou.data_ptr(), _ptr(od), _stream(wu))
+    return out
+
+
+def phaseless_update(logdet_new, logovlp_old, shift_term, fb_term, weights, pop_shift, *,
+                     e_estimate, h0_prop, dt, sqrt_dt, weight_floor, weight_cap, damping):
+    nw, nspin = logdet_new.shape
+    dev = logdet_new.device
+    _check(logdet_new, c128, (nw, nspin), "logdet_new")
+    _check(logovlp_old, c128, (nw,), "log_overlaps", dev)
+    _check(shift_term, c128, (nw,), "shift_term", dev)
+    _check(fb_term, c128, (nw,), "fb_term", dev)
+    _check(weights, f64, (nw,), "weights", dev)
+    _check(pop_shift, f64, (), "pop_control_ene_shift", dev)
+    lo_new = torch.empty((nw,), dtype=c128, device=dev)
+    w_new = torch.empty((nw,), dtype=f64, device=dev)
+    pop_new = torch.empty((), dtype=f64, device=dev)
+    h0 = complex(h0_prop)
+    _call("afq_phaseless_update", logdet_new, nw, nspin, logdet_new.data_ptr(), logovlp_old.data_ptr(),
+              shift_term.data_ptr(), fb_term.data_ptr(), weights.data_ptr(), pop_shift.data_ptr(),
+              float(e_estimate), h0.real, h0.imag, float(dt), float(sqrt_dt), float(weight_floor),
+              float(weight_cap), float(damping), lo_new.data_ptr(), w_new.data_ptr(),
+              pop_new.data_ptr(), _stream(logdet_new))
+    return lo_new, w_new, pop_new
+
+
+def orthonormalize(w, out=None):
+    nw, n, o = w.shape
+    _check(w, c128, (nw, n, o), "walkers")
+    if out is None:
+        out = torch.empty_like(w)
+    _call("afq_orthonormalize", w, nw, n, o, w.data_ptr(), out.data_ptr(), _stream(w))
+    return out
+
+
+def energy(greens, nw, n, o, nspin, rot_chol_pad, rot_h1, h0, out=None):
+    naux = rot_chol_pad.shape[1]
+    dev = greens.device
+    ldr = ldv_for(n)
+    opad = (o + 15) // 16 * 16
+    _check(greens, f64, (2, nw, nspin * o * n), "greens")
+    _check(rot_chol_pad, f64, (nspin, naux, opad, ldr), "rot_chol_pad", dev)
+    _check(rot_h1, f64, (nspin, o, n), "rot_h1", dev)
+    if out is None:
+        out = torch.empty((nw,), dtype=c128, device=dev)
+    _check(out, c128, (nw,), "energy", dev)
+    _call("afq_energy", greens, nw, n, o, nspin, naux, float(h0), greens.data_ptr(),
+              rot_chol_pad.data_ptr(), ldr, rot_h1.data_ptr(), out.data_ptr(), _stream(greens))
+    return out
+
+
+def block_energy(energies, weights, pop_shift, *, e_ref, thresh, shift_ema):
+    nw = energies.shape[0]
+    dev = energies.device
+    _check(energies, c128, (nw,), "energies")
+    _check(weights, f64, (nw,), "weights", dev)
+    _check(pop_shift, f64, (), "pop_control_ene_shift", dev)
+    out = torch.empty((3,), dtype=f64, device=dev)
+    _call("afq_block_energy", energies, nw, energies.data_ptr(), weights.data_ptr(), float(e_ref),
+              float(thresh), float(shift_ema), pop_shift.data_ptr(), out.data_ptr(),
+              _stream(energies))
+    return out
+
+
+def comb_indices(weights, zeta):
+    nw = weights.shape[0]
+    dev = weights.device
+    _check(weights, f64, (nw,), "weights")
+    cw = torch.empty((nw,), dtype=f64, device=dev)
+    idx = torch.empty((nw,), dtype=torch.int32, device=dev)
+    w_new = torch.empty((nw,), dtype=f64, device=dev)
+    _call("afq_comb_indices", weights, nw, weights.data_ptr(), float(zeta), cw.data_ptr(),
+              idx.data_ptr(), w_new.data_ptr(), _stream(weights))
+    return idx, w_new
+
+
+def gather_rows(src, idx, nplanes=1, out=None):
+    """out[p, k] = src[p, idx[k]] over the leading (plane, row) axes; idx int32
+    with every entry in [0, rows)."""
+    if nplanes == 1:
+        rows = src.shape[0]
+        row_len = src[0].numel()
+    else:
+        rows = src.shape[1]
+        row_len = src[0, 0].numel()
+    if src.dtype == c128:
+        row_len *= 2
+    elif src.dtype != f64:
+        raise TypeError("gather_rows: float64 or complex128 rows only")
+    if src.device.type != "cuda" or not src.is_contiguous():
+        raise ValueError("gather_rows: contiguous HIP tensor required")
+    _check(idx, torch.int32, (idx.shape[0],), "idx", src.device)
+    nrows = idx.shape[0]
+    if out is None:
+        shape = (nrows,) + tuple(src.shape[1:]) if nplanes == 1 else (nplanes, nrows) + tuple(src.shape[2:])
+        out = torch.empty(shape, dtype=src.dtype, device=src.device)
+    plane_src = rows * row_len
+    plane_dst = nrows * row_len
+    _call("afq_gather_rows", src, nrows, row_len, nplanes, plane_src, plane_dst, idx.data_ptr(),
+              src.data_ptr(), out.data_ptr(), _stream(src))
+    return out

@@ -1,0 +1,137 @@
+"""Half-rotated measurements for RHF/UHF trials (meas/rhf.py:15-176 of the
+reference), batched over walkers on the HIP kernels.
+
+Force bias: fb = alpha * rot_chol_flat . vec(G) as one fp64 MFMA GEMM over the
+whole walker batch (alpha = 2 for restricted walkers, meas/rhf.py:33;
+spin sectors side by side along K for unrestricted walkers, meas/rhf.py:71-72).
+Energy: meas/rhf.py:36-50 / 75-105 with the (naux, o, o) intermediate reduced
+inside the kernel.
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass, field
+
+import torch
+
+from .. import kernels
+from ..core.ops import k_energy, k_force_bias, meas_ops
+from ..core.system import system
+from ..ham.chol import ham_chol
+from ..trial import rhf as trial_rhf
+
+
+@dataclass(frozen=True)
+class rhf_meas_ctx:
+    """Half-rotated integrals per trial spin sector: rot_h1 (2, o, n),
+    rot_chol (2, naux, o, n) (both sectors alias one another for an RHF trial),
+    rot_chol_flat = rot_chol[0] viewed as (naux, o*n), plus device layouts
+    built on first use (force-bias operand per walker kind, padded energy
+    operand)."""
+
+    rot_h1: torch.Tensor
+    rot_chol: torch.Tensor
+    rot_chol_flat: torch.Tensor
+    _cache: dict = field(default_factory=dict, compare=False, repr=False)
+
+    def fb_operand(self, nspin: int) -> torch.Tensor:
+        key = ("fb", nspin)
+        if key not in self._cache:
+            if nspin == 1:
+                self._cache[key] = self.rot_chol_flat
+            else:
+                g = self.rot_chol.shape[1]
+                self._cache[key] = torch.cat([self.rot_chol[0].reshape(g, -1),
+                                              self.rot_chol[1].reshape(g, -1)], dim=1).contiguous()
+        return self._cache[key]
+
+    def energy_operands(self, nspin: int):
+        key = ("energy", nspin)
+        if key not in self._cache:
+            _, g, o, n = self.rot_chol.shape
+            opad = (o + 15) // 16 * 16
+            ldr = kernels.ldv_for(n)
+            pad = torch.zeros((nspin, g, opad, ldr), dtype=torch.float64, device=self.rot_chol.device)
+            pad[:, :, :o, :n] = self.rot_chol[:nspin]
+            self._cache[key] = (pad, self.rot_h1[:nspin].contiguous())
+        return self._cache[key]
+
+
+def build_meas_ctx(ham_data: ham_chol, trial_data) -> rhf_meas_ctx:
+    """meas/rhf.py:141-150 (per spin sector for a UHF trial)."""
+    if ham_data.basis != "restricted":
+        raise ValueError("RHF meas_ops currently assumes ham_chol.basis == 'restricted'.")
+    cs = [trial_data.mo_up] if trial_data.restricted else [trial_data.mo_up, trial_data.mo_dn]
+    rot_h1 = [c.T @ ham_data.h1 for c in cs]
+    rot_chol = [torch.einsum("pi,gpj->gij", c, ham_data.chol).contiguous() for c in cs]
+    if trial_data.restricted:
+        rh = torch.stack([rot_h1[0], rot_h1[0]])
+        rc = rot_chol[0].unsqueeze(0).expand(2, *rot_chol[0].shape)
+    else:
+        rh = torch.stack(rot_h1)
+        rc = torch.stack(rot_chol)
+    flat = rc[0].reshape(rc.shape[1], -1)
+    return rhf_meas_ctx(rot_h1=rh.contiguous(), rot_chol=rc, rot_chol_flat=flat)
+
+
+def _nspin(walkers) -> int:
+    return 2 if isinstance(walkers, tuple) else 1
+
+
+def force_bias_planes(walkers, meas_ctx: rhf_meas_ctx, trial_data, greens=None) -> torch.Tensor:
+    """Force bias as split planes (2, nw, naux)."""
+    nspin = _nspin(walkers)
+    if greens is None:
+        greens = trial_rhf.greens(walkers, trial_data)
+    alpha = 2.0 if nspin == 1 else 1.0
+    return kernels.force_bias(greens.greens, meas_ctx.fb_operand(nspin), alpha)
+
+
+def force_bias_kernel_r(walkers, _ham, meas_ctx: rhf_meas_ctx, trial_data, greens=None):
+    """meas/rhf.py:27-33, batched: (nw, naux) complex."""
+    fb = force_bias_planes(walkers, meas_ctx, trial_data, greens)
+    return torch.complex(fb[0], fb[1])
+
+
+def force_bias_kernel_u(walkers, _ham, meas_ctx: rhf_meas_ctx, trial_data, greens=None):
+    """meas/rhf.py:60-72, batched: (nw, naux) complex."""
+    fb = force_bias_planes(walkers, meas_ctx, trial_data, greens)
+    return torch.complex(fb[0], fb[1])
+
+
+def _energy(walkers, ham_data: ham_chol, meas_ctx: rhf_meas_ctx, trial_data, greens=None):
+    nspin = _nspin(walkers)
+    w0 = walkers[0] if nspin == 2 else walkers
+    nw, n, o = w0.shape
+    if greens is None:
+        greens = trial_rhf.greens(walkers, trial_data)
+    rot_pad, rot_h1 = meas_ctx.energy_operands(nspin)
+    return kernels.energy(greens.greens, nw, n, o, nspin, rot_pad, rot_h1, float(ham_data.h0))
+
+
+def energy_kernel_r(walkers, ham_data, meas_ctx, trial_data, greens=None):
+    """meas/rhf.py:36-50, batched: (nw,) complex."""
+    return _energy(walkers, ham_data, meas_ctx, trial_data, greens)
+
+
+def energy_kernel_u(walkers, ham_data, meas_ctx, trial_data, greens=None):
+    """meas/rhf.py:75-105 (per-spin integrals for a UHF trial), batched: (nw,) complex."""
+    return _energy(walkers, ham_data, meas_ctx, trial_data, greens)
+
+
+def make_rhf_meas_ops(sys: system) -> meas_ops:
+    """meas/rhf.py:153-176; also serves UHF trials (per-spin integrals)."""
+    wk = sys.walker_kind.lower()
+    if wk == "restricted":
+        return meas_ops(overlap=trial_rhf.overlap_r, build_meas_ctx=build_meas_ctx,
+                        kernels={k_force_bias: force_bias_kernel_r, k_energy: energy_kernel_r},
+                        log_overlap=trial_rhf.log_overlap, greens=trial_rhf.greens)
+    if wk == "unrestricted":
+        return meas_ops(overlap=trial_rhf.overlap_u, build_meas_ctx=build_meas_ctx,
+                        kernels={k_force_bias: force_bias_kernel_u, k_energy: energy_kernel_u},
+                        log_overlap=trial_rhf.log_overlap, greens=trial_rhf.greens)
+    if wk == "generalized":
+        raise NotImplementedError("generalized walkers are not implemented on the HIP path")
+    raise ValueError(f"unknown walker_kind: {sys.walker_kind}")
+
+
+make_uhf_meas_ops = make_rhf_meas_ops

@@ -1,0 +1,126 @@
+"""Phaseless AFQMC step (prop/afqmc.py:17-147 of the reference) on HIP kernels.
+
+One step enqueues, on the current stream and without host synchronisation:
+force-bias GEMM (from the cached Green's function of the current walkers),
+field shifts, VHS GEMM, fused Trotter step, Green's function / log-overlap of
+the new walkers (cached for the next step) and the phaseless weight update.
+"""
+from __future__ import annotations
+
+from typing import Any
+
+import torch
+
+from .. import kernels
+from .. import walkers as wk
+from ..core.ops import k_energy, k_force_bias, meas_ops, trial_ops
+from ..core.system import system
+from ..ham.chol import ham_chol
+from .chol_afqmc_ops import chol_afqmc_ctx, chol_afqmc_ops
+from .types import afqmc_params, prop_state
+
+
+def make_key(seed: int) -> torch.Tensor:
+    """Host key (seed, counter)."""
+    return torch.tensor([int(seed), 0], dtype=torch.int64)
+
+
+def split_key(key: torch.Tensor):
+    """(next key, subkey): advancing the counter; the subkey seeds one draw."""
+    seed, ctr = int(key[0]), int(key[1])
+    sub = (seed * 0x9E3779B97F4A7C15 + ctr * 0xBF58476D1CE4E5B9 + 1) % (1 << 63)
+    return torch.tensor([seed, ctr + 1], dtype=torch.int64), sub
+
+
+def draw_fields(key: torch.Tensor, nw: int, n_fields: int, device):
+    """x ~ N(0,1) of shape (nw, n_fields) on the device, from the key's subkey
+    (torch Philox; not the reference's jax threefry stream)."""
+    key_next, sub = split_key(key)
+    gen = torch.Generator(device=device)
+    gen.manual_seed(sub)
+    return torch.randn((nw, n_fields), dtype=torch.float64, device=device, generator=gen), key_next
+
+
+def _log_overlap_and_greens(walkers, meas_ops_: meas_ops, trial_data):
+    if meas_ops_.greens is not None:
+        g = meas_ops_.greens(walkers, trial_data)
+        return g.log_overlap, g
+    if meas_ops_.log_overlap is not None:
+        return meas_ops_.log_overlap(walkers, trial_data), None
+    return torch.log(meas_ops_.overlap(walkers, trial_data).to(torch.complex128)), None
+
+
+def init_prop_state(*, sys: system, n_walkers: int, seed: int, ham_data: ham_chol,
+                    trial_ops_: trial_ops, trial_data: Any, meas_ops: meas_ops,
+                    initial_walkers: Any | None = None,
+                    initial_e_estimate: float | None = None) -> prop_state:
+    """prop/afqmc.py:17-71."""
+    device = ham_data.device
+    weights = torch.ones((n_walkers,), dtype=torch.float64, device=device)
+    if initial_walkers is None:
+        initial_walkers = wk.init_walkers(sys=sys, rdm1=trial_ops_.get_rdm1(trial_data),
+                                          n_walkers=n_walkers)
+    log_ov, greens = _log_overlap_and_greens(initial_walkers, meas_ops, trial_data)
+    if initial_e_estimate is not None:
+        e_est = float(initial_e_estimate)
+    else:
+        meas_ctx = meas_ops.build_meas_ctx(ham_data, trial_data)
+        e_kernel = meas_ops.require_kernel(k_energy)
+        e_samples = torch.real(e_kernel(initial_walkers, ham_data, meas_ctx, trial_data, greens=greens))
+        e_est = float(torch.mean(e_samples).item())
+    pop_shift = torch.tensor(e_est, dtype=torch.float64, device=device)
+    return prop_state(walkers=initial_walkers, weights=weights, log_overlaps=log_ov,
+                      rng_key=make_key(seed), pop_control_ene_shift=pop_shift,
+                      e_estimate=e_est, greens=greens)
+
+
+def _force_bias_planes(state: prop_state, ham_data, trial_data, meas_ops_: meas_ops, meas_ctx,
+                       n_chunks: int):
+    fb_kernel = meas_ops_.require_kernel(k_force_bias)
+    greens = state.greens
+    if greens is None and meas_ops_.greens is not None:
+        greens = meas_ops_.greens(state.walkers, trial_data)
+    fb = fb_kernel(state.walkers, ham_data, meas_ctx, trial_data, greens=greens)
+    if fb.is_complex():
+        return torch.stack([fb.real, fb.imag]).to(torch.float64).contiguous()
+    return fb
+
+
+def afqmc_step(state: prop_state, *, params: afqmc_params, ham_data: ham_chol, trial_data: Any,
+               meas_ops: meas_ops, prop_ops: chol_afqmc_ops, prop_ctx: chol_afqmc_ctx,
+               meas_ctx: Any, fields: torch.Tensor | None = None) -> prop_state:
+    """prop/afqmc.py:74-147.  ``fields`` (nw, n_fields) float64 may be injected;
+    otherwise they are drawn from ``state.rng_key``.  Pure: ``state`` is not
+    modified, every output is a new tensor."""
+    nw = wk.n_walkers(state.walkers)
+    device = state.weights.device
+    if fields is None:
+        fields, key = draw_fields(state.rng_key, nw, prop_ops.n_fields(), device)
+    else:
+        key, _ = split_key(state.rng_key)
+        fields = torch.as_tensor(fields, device=device)
+        if fields.is_complex():
+            fields = fields.real
+        fields = fields.to(torch.float64).contiguous()
+
+    fb = _force_bias_planes(state, ham_data, trial_data, meas_ops, meas_ctx, params.n_chunks)
+    xbar, shift_term, fb_term = kernels.field_shifts(fields, fb, prop_ctx.mf_shifts, prop_ctx.sqrt_dt)
+    walkers_new = prop_ops.apply_trotter_planes(state.walkers, xbar, prop_ctx, params.n_exp_terms,
+                                                params.n_chunks)
+    if meas_ops.greens is not None:
+        greens_new = meas_ops.greens(walkers_new, trial_data)
+        logdet = greens_new.logdet
+    else:
+        greens_new = None
+        lo = _log_overlap_and_greens(walkers_new, meas_ops, trial_data)[0]
+        logdet = lo.reshape(-1, 1) / 2.0  # restricted convention: log ovlp = 2 logdet
+    lo_new, w_new, pop_new = kernels.phaseless_update(
+        logdet, state.log_overlaps, shift_term, fb_term, state.weights,
+        state.pop_control_ene_shift, e_estimate=state.e_estimate, h0_prop=prop_ctx.h0_prop,
+        dt=prop_ctx.dt, sqrt_dt=prop_ctx.sqrt_dt,
+        weight_floor=float(getattr(params, "weight_floor", 1.0e-3)),
+        weight_cap=float(getattr(params, "weight_cap", 100.0)),
+        damping=float(getattr(params, "pop_control_damping", 0.1)))
+    return prop_state(walkers=walkers_new, weights=w_new, log_overlaps=lo_new, rng_key=key,
+                      pop_control_ene_shift=pop_new, e_estimate=state.e_estimate,
+                      greens=greens_new)

@@ -1,0 +1,130 @@
+"""Cholesky-AFQMC propagator (prop/chol_afqmc_ops.py:15-262 of the reference).
+
+apply_trotter is batched over walkers: one fp64 MFMA GEMM builds every
+walker's VHS = sum_g x_g L_g (prop/chol_afqmc_ops.py:70-74), then the fused
+Trotter kernel applies exp(-dt/2 h1eff) . Taylor_n(i sqrt_dt VHS) . exp(-dt/2 h1eff)
+(prop/chol_afqmc_ops.py:173-196, prop/utils.py:20-31).  n_chunks bounds the
+VHS workspace (2 * chunk * n * n doubles) exactly as the reference's chunked
+vmap bounds memory; results do not depend on it.
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass
+from typing import Any, Callable, NamedTuple
+
+import torch
+
+from .. import kernels
+from ..ham.chol import ham_chol
+
+
+@dataclass(frozen=True)
+class chol_afqmc_ctx:
+    dt: float
+    sqrt_dt: float
+    exp_h1_half: torch.Tensor       # (n, n)
+    mf_shifts: torch.Tensor         # (n_fields,) complex128, purely imaginary
+    h0_prop: complex
+    exp_h1_half_pad: torch.Tensor   # (n, ldv) device layout of exp_h1_half
+
+
+class chol_afqmc_ops(NamedTuple):
+    n_fields: Callable[[], int]
+    build_prop_ctx: Callable[[Any, float], chol_afqmc_ctx]
+    apply_trotter: Callable[..., Any]       # (walkers, fields (nw, G), ctx, n_terms) -> walkers
+    apply_trotter_planes: Callable[..., Any]  # (walkers, xbar planes (2, nw, G), ctx, n_terms)
+
+
+def _as_total_rdm1_restricted(dm: torch.Tensor) -> torch.Tensor:
+    if dm.ndim == 3 and dm.shape[0] == 2:
+        return dm[0] + dm[1]
+    return dm
+
+
+def _mf_shifts(ham_data: ham_chol, trial_rdm1: torch.Tensor) -> torch.Tensor:
+    """prop/chol_afqmc_ops.py:59-63: 1j * tr(L_g dm)."""
+    dm = _as_total_rdm1_restricted(torch.as_tensor(trial_rdm1, device=ham_data.device,
+                                                   dtype=torch.float64))
+    return 1.0j * torch.einsum("gij,ji->g", ham_data.chol, dm).to(torch.complex128)
+
+
+def _build_prop_ctx(ham_data: ham_chol, trial_rdm1, dt: float) -> chol_afqmc_ctx:
+    """prop/chol_afqmc_ops.py:77-103."""
+    mf = _mf_shifts(ham_data, trial_rdm1)
+    h0_prop = complex(-ham_data.h0 - 0.5 * torch.sum(mf ** 2).item())
+    chol = ham_data.chol
+    g, n, _ = chol.shape
+    v0m = 0.5 * torch.einsum("gik,gkj->ij", chol, chol)
+    mf_r = (1.0j * mf).real
+    v1m = torch.einsum("g,gik->ik", mf_r, chol)
+    h1_eff = ham_data.h1 - v0m - v1m
+    exp_h1_half = torch.linalg.matrix_exp(-0.5 * dt * h1_eff)
+    ldv = kernels.ldv_for(n)
+    pad = torch.zeros((n, ldv), dtype=torch.float64, device=chol.device)
+    pad[:, :n] = exp_h1_half
+    return chol_afqmc_ctx(dt=float(dt), sqrt_dt=float(dt) ** 0.5, exp_h1_half=exp_h1_half,
+                          mf_shifts=mf.contiguous(), h0_prop=h0_prop, exp_h1_half_pad=pad)
+
+
+def chol_transposed(chol: torch.Tensor) -> torch.Tensor:
+    """L^T (n*ldv, n_fields) with zero rows at q >= n: the B operand of the VHS GEMM."""
+    g, n, _ = chol.shape
+    ldv = kernels.ldv_for(n)
+    pad = torch.zeros((g, n, ldv), dtype=torch.float64, device=chol.device)
+    pad[:, :, :n] = chol
+    return pad.reshape(g, n * ldv).T.contiguous()
+
+
+def _chunks(nw: int, n_chunks: int):
+    if nw % n_chunks != 0:
+        raise ValueError(f"n_walkers={nw} is not divisible into n_chunks={n_chunks}")
+    cs = nw // n_chunks
+    return [(i * cs, (i + 1) * cs) for i in range(n_chunks)]
+
+
+def make_chol_afqmc_ops(ham_data: ham_chol, walker_kind: str) -> chol_afqmc_ops:
+    """prop/chol_afqmc_ops.py:215-262 (restricted basis, restricted/unrestricted walkers)."""
+    walker_kind = walker_kind.lower()
+    nf = int(ham_data.chol.shape[0])
+    n = int(ham_data.chol.shape[1])
+    if ham_data.basis == "generalized" and walker_kind != "generalized":
+        raise ValueError("ham_data.basis='generalized' requires walker_kind='generalized'")
+    if ham_data.basis == "generalized" or walker_kind == "generalized":
+        raise NotImplementedError("generalized walkers / basis are not implemented on the HIP path")
+    if walker_kind not in ("restricted", "unrestricted"):
+        raise ValueError(f"unknown walker_kind: {walker_kind}")
+
+    chol_t = chol_transposed(ham_data.chol)
+
+    def n_fields_() -> int:
+        return nf
+
+    def build_prop_ctx(trial_rdm1, dt: float) -> chol_afqmc_ctx:
+        return _build_prop_ctx(ham_data, trial_rdm1, dt)
+
+    def apply_trotter_planes(w, xbar: torch.Tensor, ctx: chol_afqmc_ctx, n_terms: int,
+                             n_chunks: int = 1):
+        unr = isinstance(w, tuple)
+        w0 = w[0] if unr else w
+        nw = w0.shape[0]
+        if n_chunks == 1:
+            v = kernels.vhs(xbar, chol_t, n)
+            return kernels.trotter(w, v, ctx.exp_h1_half_pad, ctx.sqrt_dt, n_terms)
+        out = (torch.empty_like(w[0]), torch.empty_like(w[1])) if unr else torch.empty_like(w0)
+        for a, b in _chunks(nw, n_chunks):
+            v = kernels.vhs(xbar[:, a:b].contiguous(), chol_t, n)
+            wc = (w[0][a:b], w[1][a:b]) if unr else w0[a:b]
+            oc = (out[0][a:b], out[1][a:b]) if unr else out[a:b]
+            kernels.trotter(wc, v, ctx.exp_h1_half_pad, ctx.sqrt_dt, n_terms, out=oc)
+        return out
+
+    def apply_trotter(w, fields, ctx: chol_afqmc_ctx, n_terms: int, n_chunks: int = 1):
+        """fields (nw, n_fields), real or complex."""
+        f = torch.as_tensor(fields, device=ham_data.device)
+        if f.is_complex():
+            xbar = torch.stack([f.real, f.imag]).to(torch.float64).contiguous()
+        else:
+            xbar = torch.stack([f, torch.zeros_like(f)]).to(torch.float64).contiguous()
+        return apply_trotter_planes(w, xbar, ctx, n_terms, n_chunks)
+
+    return chol_afqmc_ops(n_fields_, build_prop_ctx, apply_trotter, apply_trotter_planes)

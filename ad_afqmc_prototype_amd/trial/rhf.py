@@ -1,0 +1,138 @@
+"""RHF / UHF trial determinants (trial/rhf.py:12-71 of the reference).
+
+Overlaps are computed by the HIP Green's-function kernel (LU of C^T w), which
+carries the determinant as a complex logarithm; ``overlap_*`` return
+exp(log-overlap), identical to the reference's raw det(C^T w)^2 (restricted) or
+det(C^T wu) det(C^T wd) (unrestricted) wherever those are finite.
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass
+from typing import NamedTuple
+
+import torch
+
+from .. import kernels
+from ..core.ops import trial_ops
+from ..core.system import system
+from ..ham.chol import as_device_f64
+
+
+@dataclass(frozen=True)
+class rhf_trial:
+    mo_coeff: torch.Tensor  # (norb, nocc), real
+
+    def __post_init__(self):
+        object.__setattr__(self, "mo_coeff", as_device_f64(self.mo_coeff))
+
+    @property
+    def norb(self) -> int:
+        return int(self.mo_coeff.shape[0])
+
+    @property
+    def nocc(self) -> int:
+        return int(self.mo_coeff.shape[1])
+
+    @property
+    def mo_up(self) -> torch.Tensor:
+        return self.mo_coeff
+
+    @property
+    def mo_dn(self) -> torch.Tensor:
+        return self.mo_coeff
+
+    @property
+    def restricted(self) -> bool:
+        return True
+
+
+@dataclass(frozen=True)
+class uhf_trial:
+    """Spin-unrestricted trial (no counterpart in the reference, which only has
+    rhf_trial): one (norb, nocc) real orbital block per spin."""
+
+    mo_up: torch.Tensor
+    mo_dn: torch.Tensor
+
+    def __post_init__(self):
+        object.__setattr__(self, "mo_up", as_device_f64(self.mo_up))
+        object.__setattr__(self, "mo_dn", as_device_f64(self.mo_dn, self.mo_up.device))
+        if self.mo_up.shape != self.mo_dn.shape:
+            raise ValueError("uhf_trial: both spins need the same (norb, nocc)")
+
+    @property
+    def norb(self) -> int:
+        return int(self.mo_up.shape[0])
+
+    @property
+    def nocc(self) -> int:
+        return int(self.mo_up.shape[1])
+
+    @property
+    def restricted(self) -> bool:
+        return False
+
+
+class greens_cache(NamedTuple):
+    """Half-rotated Green's functions (split planes (2, nw, nspin*o*n)) and
+    log det(C_s^T w_s) (nw, nspin) of one walker batch."""
+
+    greens: torch.Tensor
+    logdet: torch.Tensor
+
+    @property
+    def log_overlap(self) -> torch.Tensor:
+        if self.logdet.shape[1] == 1:
+            return 2.0 * self.logdet[:, 0]
+        return self.logdet[:, 0] + self.logdet[:, 1]
+
+
+def get_rdm1(trial_data) -> torch.Tensor:
+    """trial/rhf.py:29-32: stack of C C^T per spin, (2, n, n)."""
+    cu, cd = trial_data.mo_up, trial_data.mo_dn
+    return torch.stack([cu @ cu.T, cd @ cd.T], dim=0)
+
+
+def greens(walkers, trial_data) -> greens_cache:
+    if isinstance(walkers, tuple):
+        g, ld = kernels.overlap_greens(walkers, trial_data.mo_up, trial_data.mo_dn)
+    else:
+        if not trial_data.restricted:
+            raise ValueError("restricted walkers need an RHF trial")
+        g, ld = kernels.overlap_greens(walkers, trial_data.mo_up)
+    return greens_cache(g, ld)
+
+
+def log_overlap(walkers, trial_data) -> torch.Tensor:
+    return greens(walkers, trial_data).log_overlap
+
+
+def overlap_r(walkers, trial_data) -> torch.Tensor:
+    """det(C^T w)^2 per walker (trial/rhf.py:35-37)."""
+    return torch.exp(log_overlap(walkers, trial_data))
+
+
+def overlap_u(walkers, trial_data) -> torch.Tensor:
+    """det(C_up^T wu) det(C_dn^T wd) per walker (trial/rhf.py:40-44)."""
+    return torch.exp(log_overlap(walkers, trial_data))
+
+
+def make_rhf_trial_ops(sys: system) -> trial_ops:
+    """trial/rhf.py:56-71.  Generalized walkers are not supported by the HIP path."""
+    if sys.nup != sys.ndn:
+        raise ValueError("RHF requires nelec[0] == nelec[1].")
+    wk = sys.walker_kind.lower()
+    if wk == "restricted":
+        return trial_ops(overlap=overlap_r, get_rdm1=get_rdm1, greens=greens, log_overlap=log_overlap)
+    if wk == "unrestricted":
+        return trial_ops(overlap=overlap_u, get_rdm1=get_rdm1, greens=greens, log_overlap=log_overlap)
+    if wk == "generalized":
+        raise NotImplementedError("generalized walkers are not implemented on the HIP path")
+    raise ValueError(f"unknown walker_kind: {sys.walker_kind}")
+
+
+def make_uhf_trial_ops(sys: system) -> trial_ops:
+    """UHF trial with unrestricted walkers."""
+    if sys.walker_kind.lower() != "unrestricted":
+        raise ValueError("a UHF trial needs walker_kind='unrestricted'")
+    return trial_ops(overlap=overlap_u, get_rdm1=get_rdm1, greens=greens, log_overlap=log_overlap)

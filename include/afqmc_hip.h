@@ -1,0 +1,169 @@
+/*
+ * C ABI of the gfx950 phaseless-AFQMC walker step (libafqmc_hip.so).
+ *
+ * Conventions
+ *  - every pointer argument is a device pointer owned by the caller
+ *    (except where noted); nothing is allocated inside the library;
+ *  - row-major storage; complex numbers are complex128 interleaved (re, im)
+ *    where the reference holds complex arrays, or split real planes
+ *    ("planes (2, ...)": all real parts, then all imaginary parts) for the
+ *    internal GEMM operands;
+ *  - `stream` is a hipStream_t; all work is enqueued on it, nothing synchronizes;
+ *  - return 0 on success, -1 on an invalid argument, -2 on a launch failure;
+ *    afq_last_error() gives the message (thread-local).
+ *
+ * Reference paths are relative to the reference package ad_afqmc_prototype/.
+ * Sizes: nw walkers, n orbitals, o occupied orbitals per spin, naux Cholesky
+ * vectors, nspin = 1 (restricted walkers) or 2 (unrestricted walkers).
+ */
+#ifndef AFQMC_HIP_H
+#define AFQMC_HIP_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Last error message raised on the calling thread ("" if none). */
+const char* afq_last_error(void);
+
+/* ABI version of this header (bumped on any signature change). */
+int afq_abi_version(void);
+
+/*
+ * fp64 GEMM  C = alpha * A * B^T + beta * C  on v_mfma_f64_16x16x4.
+ * A (M x K, lda), B (N x K, ldb), C (M x N, ldc).  Building block of the force
+ * bias and VHS contractions below; the reference has no standalone GEMM (XLA
+ * lowers jnp.matmul, meas/rhf.py:33 and prop/chol_afqmc_ops.py:72-73).
+ */
+int afq_dgemm_nt(int M, int N, int K, double alpha, const double* A, long long lda,
+                 const double* B, long long ldb, double beta, double* C, long long ldc,
+                 void* stream);
+
+/*
+ * Overlap matrix m = C^T w, LU with partial pivoting, log det(m), and the
+ * half-rotated Green's function G = (w m^-1)^T, per walker and spin.
+ * Replaces _half_green_from_overlap_matrix (meas/rhf.py:15-19), overlap_r
+ * (meas/rhf.py:22-24, trial/rhf.py:35-37), overlap_u (meas/rhf.py:53-57,
+ * trial/rhf.py:40-44) and _det (trial/rhf.py:25-26), with the determinant
+ * carried as a complex logarithm.
+ *   walkers_s : complex128 (nw, n, o)      (walkers_dn NULL when nspin == 1)
+ *   trial_s   : float64 (n, o), real trial orbitals
+ *   greens    : planes (2, nw, nspin*o*n); element (s, i, p) at (s*o + i)*n + p
+ *   logdet    : complex128 (nw, nspin), log det(C_s^T w_s)
+ */
+int afq_overlap_greens(int nw, int n, int o, int nspin,
+                       const double* walkers_up, const double* walkers_dn,
+                       const double* trial_up, const double* trial_dn,
+                       double* greens, double* logdet, void* stream);
+
+/*
+ * Force bias fb = alpha * rot . vec(G): planes (2, nw, naux).
+ * rot (naux, k) holds the half-rotated Cholesky vectors of every spin sector
+ * side by side (k = nspin*o*n).  alpha = 2 for restricted walkers.
+ * Replaces force_bias_kernel_r / _u (meas/rhf.py:27-33, 60-72).
+ */
+int afq_force_bias(int nw, int naux, int k, double alpha, const double* greens,
+                   const double* rot, double* fb, void* stream);
+
+/*
+ * Field shifts, shifted fields and the two per-walker sums of
+ * prop/afqmc.py:97-101:
+ *   fs = -sqrt_dt (i fb - mf);  xbar = x - fs;
+ *   shift_term = sum_g xbar_g mf_g;  fb_term = sum_g (x_g fs_g - fs_g^2 / 2).
+ *   fields (nw, naux) float64; fb planes (2, nw, naux); mf complex128 (naux);
+ *   xbar planes (2, nw, naux); shift_term, fb_term complex128 (nw).
+ */
+int afq_field_shifts(int nw, int naux, double sqrt_dt, const double* fields,
+                     const double* fb, const double* mf, double* xbar,
+                     double* shift_term, double* fb_term, void* stream);
+
+/*
+ * VHS = sum_g xbar_g L_g, planes (2, nw, n, ldv) (ldv = n rounded up to 4).
+ * chol_t (n*ldv, naux) is L transposed with zero rows at q >= n.
+ * Replaces _make_vhs_split_flat (prop/chol_afqmc_ops.py:70-74).
+ */
+int afq_vhs(int nw, int n, int ldv, int naux, const double* xbar, const double* chol_t,
+            double* vhs, void* stream);
+
+/*
+ * Trotter step w' = E . sum_{k<=n_terms} (i sqrt_dt VHS)^k / k! . E w per walker
+ * and spin (both spins share the walker's VHS), E = exp(-dt/2 h1_eff) (n, ldv).
+ * Replaces _apply_trotter_r / _u (prop/chol_afqmc_ops.py:106-116, 127-154,
+ * 173-196) and taylor_expm_action (prop/utils.py:20-31).
+ *   win_s, wout_s : complex128 (nw, n, o); wout must not alias win.
+ */
+int afq_trotter(int nw, int n, int o, int nspin, int ldv, int n_terms, double sqrt_dt,
+                const double* exp_h1_half, const double* vhs,
+                const double* win_up, const double* win_dn,
+                double* wout_up, double* wout_dn, void* stream);
+
+/*
+ * Phaseless importance, cos-phase cut, floor/cap and population-control shift
+ * of prop/afqmc.py:116-138, on log-overlaps:
+ *   log ovlp_new = (nspin == 1 ? 2 : 1) * sum_s logdet_new[w, s]
+ *   ratio = exp(log ovlp_new - log ovlp_old) ...
+ *   pop_shift_new = e_estimate - damping * log(max(mean w_new, 1e-300)) / dt.
+ * logdet_new complex128 (nw, nspin); logovlp_* complex128 (nw); shift_term,
+ * fb_term complex128 (nw); weights_* float64 (nw); pop_shift_* device float64[1].
+ */
+int afq_phaseless_update(int nw, int nspin, const double* logdet_new,
+                         const double* logovlp_old, const double* shift_term,
+                         const double* fb_term, const double* weights_old,
+                         const double* pop_shift_old, double e_estimate,
+                         double h0_prop_re, double h0_prop_im, double dt, double sqrt_dt,
+                         double weight_floor, double weight_cap, double damping,
+                         double* logovlp_new, double* weights_new, double* pop_shift_new,
+                         void* stream);
+
+/*
+ * Reduced QR with diag(R) real and positive, keeping Q (the norm is dropped),
+ * per walker: complex128 (nw, n, o) -> (nw, n, o).
+ * Replaces _qr / orthogonalize / orthonormalize (walkers.py:169-211).
+ */
+int afq_orthonormalize(int nw, int n, int o, const double* win, double* wout, void* stream);
+
+/*
+ * Local energy per walker, complex128 (nw):
+ *   restricted:   h0 + 2 sum G.rot_h1 + 2 sum_g c_g^2 - sum_g tr(f_g f_g)
+ *   unrestricted: h0 + sum_s G_s.rot_h1_s + [sum_g (c_up+c_dn)^2 - x_up - x_dn] / 2
+ * with f_g = rot_chol_g G^T (o x o) and c_g = tr f_g, never materialised.
+ *   rot_chol_pad (nspin, naux, opad, ldr) float64, opad = o rounded up to 16,
+ *   ldr = n rounded up to 4, zero padded; rot_h1 (nspin, o, n) float64.
+ * Replaces energy_kernel_r / _u (meas/rhf.py:36-50, 75-105).
+ */
+int afq_energy(int nw, int n, int o, int nspin, int naux, double h0, const double* greens,
+               const double* rot_chol_pad, int ldr, const double* rot_h1, double* energy,
+               void* stream);
+
+/*
+ * Block energy of prop/blocks.py:59-79: real part, outlier clip to e_ref when
+ * |e - e_ref| > thresh, e_block = sum w e / sum w (e_ref when sum w == 0), and
+ * pop_shift_new = (1 - shift_ema) pop_shift + shift_ema e_block.
+ *   out (device float64[3]) = {e_block, sum w, pop_shift_new}.
+ */
+int afq_block_energy(int nw, const double* energy, const double* weights, double e_ref,
+                     double thresh, double shift_ema, const double* pop_shift, double* out,
+                     void* stream);
+
+/*
+ * Comb reconfiguration indices of _sr_indices / stochastic_reconfiguration
+ * (walkers.py:233-265): sequential left-to-right cumulative sum of |w|,
+ * z_k = tot * (k + zeta) / nw, idx_k = first i with cw_i >= z_k,
+ * weights_new = tot / nw.  cumsum (nw) is caller scratch; idx int32 (nw).
+ */
+int afq_comb_indices(int nw, const double* weights, double zeta, double* cumsum, int* idx,
+                     double* weights_new, void* stream);
+
+/*
+ * dst[p][k][:] = src[p][idx[k]][:] for p < nplanes, rows of row_len doubles.
+ * The walker gather of stochastic_reconfiguration (walkers.py:258-263).
+ */
+int afq_gather_rows(int nrows, long long row_len, int nplanes, long long src_plane_stride,
+                    long long dst_plane_stride, const int* idx, const double* src, double* dst,
+                    void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* AFQMC_HIP_H */

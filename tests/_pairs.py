@@ -1,0 +1,80 @@
+"""Builds the same synthetic system for the CPU checker and for the HIP path."""
+from __future__ import annotations
+
+from dataclasses import dataclass
+
+import numpy as np
+
+from ad_afqmc_prototype_amd import synthetic as syn
+from oracle import afqmc_oracle as orc
+
+
+@dataclass
+class pair:
+    sd: object          # oracle system_data
+    prm: object         # oracle params
+    s: object           # synthetic_system
+    kind: str
+    dev: object = None
+    g: dict = None      # HIP-side objects
+
+
+def rel(a, b) -> float:
+    a = np.asarray(a)
+    b = np.asarray(b)
+    return float(np.linalg.norm((a - b).ravel()) / max(np.linalg.norm(b.ravel()), 1e-300))
+
+
+def make_pair(seed, norb, nocc, naux, *, uhf=False, kind="restricted", dt=0.01,
+              chol_scale=1.0, device=None) -> pair:
+    s = syn.make_system(seed, norb, nocc, naux, uhf=uhf, chol_scale=chol_scale)
+    sd = orc.make_system_data(s.h0, s.h1, s.chol, (s.mo_up, s.mo_dn), kind, dt)
+    prm = orc.params(dt=dt)
+    p = pair(sd=sd, prm=prm, s=s, kind=kind, dev=device)
+    if device is not None:
+        p.g = build_gpu(s, kind, dt, device)
+    return p
+
+
+def build_gpu(s, kind, dt, device):
+    from ad_afqmc_prototype_amd.core.system import system
+    from ad_afqmc_prototype_amd.ham.chol import ham_chol
+    from ad_afqmc_prototype_amd.meas import rhf as meas_rhf
+    from ad_afqmc_prototype_amd.prop.chol_afqmc_ops import make_chol_afqmc_ops
+    from ad_afqmc_prototype_amd.prop.types import afqmc_params
+    from ad_afqmc_prototype_amd.trial import rhf as trial_rhf
+
+    sys_ = system(norb=s.norb, nelec=(s.nocc, s.nocc), walker_kind=kind)
+    ham = ham_chol.from_arrays(s.h0, s.h1, s.chol, device=device)
+    trial = trial_rhf.rhf_trial(s.mo_up) if s.restricted_trial else trial_rhf.uhf_trial(s.mo_up, s.mo_dn)
+    t_ops = trial_rhf.make_rhf_trial_ops(sys_) if s.restricted_trial else trial_rhf.make_uhf_trial_ops(sys_)
+    m_ops = meas_rhf.make_rhf_meas_ops(sys_)
+    p_ops = make_chol_afqmc_ops(ham, kind)
+    p_ctx = p_ops.build_prop_ctx(t_ops.get_rdm1(trial), dt)
+    m_ctx = m_ops.build_meas_ctx(ham, trial)
+    params = afqmc_params(dt=dt)
+    return dict(sys=sys_, ham=ham, trial=trial, t_ops=t_ops, m_ops=m_ops, p_ops=p_ops, p_ctx=p_ctx,
+                m_ctx=m_ctx, params=params)
+
+
+def random_walkers(rng, kind, nw, n, o, c_up, c_dn, spread=0.3):
+    """Walkers near the trial (so overlaps are well conditioned), complex."""
+    def one(c):
+        w = c[None] + spread * (rng.standard_normal((nw, n, o)) + 1j * rng.standard_normal((nw, n, o)))
+        return w.astype(np.complex128)
+    if kind == "restricted":
+        return one(c_up)
+    return (one(c_up), one(c_dn))
+
+
+def to_dev(w, device):
+    import torch
+    if isinstance(w, tuple):
+        return tuple(torch.as_tensor(x, device=device).contiguous() for x in w)
+    return torch.as_tensor(w, device=device).contiguous()
+
+
+def to_np(w):
+    if isinstance(w, tuple):
+        return tuple(x.detach().cpu().numpy() for x in w)
+    return w.detach().cpu().numpy()

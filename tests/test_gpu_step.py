@@ -1,0 +1,174 @@
+"""Composed step / block on the HIP path vs the CPU checker, with injected
+fields and comb uniforms (fp64: 1e-10 relative; reconfiguration indices
+bit-exact)."""
+import numpy as np
+import pytest
+
+from ad_afqmc_prototype_amd import synthetic as syn
+from oracle import afqmc_oracle as orc
+
+from ._pairs import make_pair, rel, to_dev, to_np
+
+pytestmark = pytest.mark.gpu
+TOL = 1e-10
+
+
+def _gpu_state(p, st, device):
+    """HIP prop_state equal to the oracle state ``st`` (log-overlap mode)."""
+    import torch
+    from ad_afqmc_prototype_amd.prop.afqmc import make_key
+    from ad_afqmc_prototype_amd.prop.types import prop_state
+    from ad_afqmc_prototype_amd.trial import rhf as trial_rhf
+    w = to_dev(st.walkers, device)
+    g = trial_rhf.greens(w, p.g["trial"])
+    return prop_state(walkers=w, weights=torch.tensor(st.weights, device=device),
+                      log_overlaps=torch.tensor(st.overlaps, device=device), rng_key=make_key(0),
+                      pop_control_ene_shift=torch.tensor(st.pop_control_ene_shift, dtype=torch.float64, device=device),
+                      e_estimate=st.e_estimate, greens=g)
+
+
+def _step(p, state, fields, params=None):
+    import torch
+    from ad_afqmc_prototype_amd.prop.afqmc import afqmc_step
+    g = p.g
+    return afqmc_step(state, params=params or g["params"], ham_data=g["ham"], trial_data=g["trial"],
+                      meas_ops=g["m_ops"], prop_ops=g["p_ops"], prop_ctx=g["p_ctx"], meas_ctx=g["m_ctx"],
+                      fields=torch.tensor(fields, device=p.dev))
+
+
+def _cmp_walkers(a, b):
+    if isinstance(b, tuple):
+        return max(rel(a[0], b[0]), rel(a[1], b[1]))
+    return rel(a, b)
+
+
+def _cmp_state(gs, st, wtol=TOL):
+    assert _cmp_walkers(to_np(gs.walkers), st.walkers) < wtol
+    w = gs.weights.cpu().numpy()
+    np.testing.assert_array_equal(w == 0, st.weights == 0)
+    assert rel(w, st.weights) < wtol
+    lo = gs.log_overlaps.cpu().numpy()
+    np.testing.assert_allclose(lo.real, st.overlaps.real, rtol=1e-11, atol=1e-9)
+    np.testing.assert_allclose(np.exp(1j * lo.imag), np.exp(1j * st.overlaps.imag), atol=1e-9)
+    np.testing.assert_allclose(gs.pop_control_ene_shift.item(), st.pop_control_ene_shift, rtol=1e-11)
+
+
+@pytest.mark.parametrize("case", [
+    dict(seed=0, n=24, o=6, g=96, nw=128, uhf=False, kind="restricted", dt=0.01, steps=5),
+    dict(seed=1, n=120, o=20, g=600, nw=48, uhf=True, kind="unrestricted", dt=0.005, steps=2),
+    dict(seed=4, n=60, o=10, g=240, nw=40, uhf=False, kind="unrestricted", dt=0.01, steps=3),
+    dict(seed=2, n=200, o=32, g=1000, nw=24, uhf=False, kind="restricted", dt=0.005, steps=1),
+])
+def test_step_trajectory_matches_checker(gpu_device, case):
+    p = make_pair(case["seed"], case["n"], case["o"], case["g"], uhf=case["uhf"], kind=case["kind"],
+                  dt=case["dt"], device=gpu_device)
+    st = orc.init_state(p.sd, case["nw"], overlap_mode="log")
+    gs = _gpu_state(p, st, gpu_device)
+    for t in range(case["steps"]):
+        f = syn.fields_for_step(case["seed"], t, case["nw"], case["g"])
+        st = orc.afqmc_step(st, f, p.sd, p.prm, overlap_mode="log")
+        gs = _step(p, gs, f)
+        _cmp_state(gs, st)
+
+
+def test_init_state_energy_estimate(gpu_device):
+    from ad_afqmc_prototype_amd.prop.afqmc import init_prop_state
+    p = make_pair(0, 24, 6, 96, device=gpu_device)
+    g = p.g
+    s = init_prop_state(sys=g["sys"], n_walkers=16, seed=0, ham_data=g["ham"], trial_ops_=g["t_ops"],
+                        trial_data=g["trial"], meas_ops=g["m_ops"])
+    ref = orc.init_state(p.sd, 16, overlap_mode="log")
+    np.testing.assert_allclose(s.e_estimate, ref.e_estimate, rtol=1e-12)
+    np.testing.assert_allclose(s.log_overlaps.cpu().numpy().real, ref.overlaps.real, atol=1e-10)
+
+
+def test_weight_update_closed_form(gpu_device):
+    """h1 = chol = 0, h0 = 1: weights -> exp(-dt), pop shift -> 0.1
+    (reference tests/test_prop_afqmc_step.py:46-94)."""
+    import torch
+    from ad_afqmc_prototype_amd.core.system import system
+    from ad_afqmc_prototype_amd.ham.chol import ham_chol
+    from ad_afqmc_prototype_amd.meas.rhf import make_rhf_meas_ops
+    from ad_afqmc_prototype_amd.prop.afqmc import afqmc_step, make_key
+    from ad_afqmc_prototype_amd.prop.chol_afqmc_ops import make_chol_afqmc_ops
+    from ad_afqmc_prototype_amd.prop.types import afqmc_params, prop_state
+    from ad_afqmc_prototype_amd.trial.rhf import greens, make_rhf_trial_ops, rhf_trial
+    norb, nocc, nw, nf = 4, 2, 8, 3
+    d = gpu_device
+    ham = ham_chol.from_arrays(1.0, np.zeros((norb, norb)), np.zeros((nf, norb, norb)), device=d)
+    sys_ = system(norb=norb, nelec=(nocc, nocc), walker_kind="restricted")
+    params = afqmc_params(dt=0.2, n_chunks=2, n_exp_terms=4, pop_control_damping=0.1)
+    trial = rhf_trial(np.eye(norb)[:, :nocc])
+    m_ops = make_rhf_meas_ops(sys_)
+    t_ops = make_rhf_trial_ops(sys_)
+    prop_ops = make_chol_afqmc_ops(ham, "restricted")
+    prop_ctx = prop_ops.build_prop_ctx(t_ops.get_rdm1(trial), params.dt)
+    w = torch.tensor(np.broadcast_to(np.eye(norb)[:, :nocc] + 0j, (nw, norb, nocc)).copy(), device=d)
+    gc = greens(w, trial)
+    state = prop_state(walkers=w, weights=torch.ones(nw, dtype=torch.float64, device=d),
+                       log_overlaps=gc.log_overlap, rng_key=make_key(0),
+                       pop_control_ene_shift=torch.tensor(0.0, dtype=torch.float64, device=d),
+                       e_estimate=0.0, greens=gc)
+    out = afqmc_step(state, params=params, ham_data=ham, trial_data=trial, meas_ops=m_ops,
+                     prop_ops=prop_ops, prop_ctx=prop_ctx, meas_ctx=m_ops.build_meas_ctx(ham, trial))
+    np.testing.assert_allclose(out.weights.cpu().numpy(), np.exp(-0.2) * np.ones(nw), rtol=1e-14)
+    np.testing.assert_allclose(out.pop_control_ene_shift.item(), 0.1, rtol=1e-12)
+    assert out.e_estimate == 0.0
+
+
+def test_step_is_pure_and_chunk_invariant(gpu_device):
+    """Calling the step twice on one state gives identical results; n_chunks
+    does not change the result; the key advances once
+    (reference tests/test_prop_afqmc_step.py:97-173)."""
+    import dataclasses
+    import torch
+    from ad_afqmc_prototype_amd.prop.afqmc import afqmc_step, split_key
+    p = make_pair(0, 24, 6, 96, device=gpu_device)
+    st = orc.init_state(p.sd, 12, overlap_mode="log")
+    gs = _gpu_state(p, st, gpu_device)
+    w_before = to_np(gs.walkers).copy()
+    g = p.g
+    kw = dict(ham_data=g["ham"], trial_data=g["trial"], meas_ops=g["m_ops"], prop_ops=g["p_ops"],
+              prop_ctx=g["p_ctx"], meas_ctx=g["m_ctx"])
+    out1 = afqmc_step(gs, params=g["params"], **kw)
+    out2 = afqmc_step(gs, params=dataclasses.replace(g["params"], n_chunks=3), **kw)
+    np.testing.assert_array_equal(to_np(gs.walkers), w_before)
+    np.testing.assert_array_equal(to_np(out1.walkers), to_np(out2.walkers))
+    np.testing.assert_array_equal(out1.weights.cpu().numpy(), out2.weights.cpu().numpy())
+    assert torch.equal(out1.rng_key, split_key(gs.rng_key)[0])
+    assert torch.equal(out1.rng_key, out2.rng_key)
+
+
+def _gpu_block(p, gs, fields, zeta, n_prop, n_ortho):
+    import dataclasses
+    from ad_afqmc_prototype_amd.prop.blocks import afqmc_block
+    g = p.g
+    params = dataclasses.replace(g["params"], n_prop_steps=n_prop, n_ortho=n_ortho)
+    return afqmc_block(gs, sys=g["sys"], params=params, ham_data=g["ham"], trial_data=g["trial"],
+                       meas_ops=g["m_ops"], prop_ops=g["p_ops"], prop_ctx=g["p_ctx"], meas_ctx=g["m_ctx"],
+                       fields=fields, zeta=zeta, return_indices=True)
+
+
+@pytest.mark.parametrize("case", [
+    dict(seed=0, n=24, o=6, g=96, nw=128, uhf=False, kind="restricted", dt=0.01, steps=40, qr=5, sr=10),
+    dict(seed=6, n=30, o=5, g=90, nw=64, uhf=True, kind="unrestricted", dt=0.01, steps=20, qr=5, sr=10),
+])
+def test_blocks_match_checker(gpu_device, case):
+    import torch
+    p = make_pair(case["seed"], case["n"], case["o"], case["g"], uhf=case["uhf"], kind=case["kind"],
+                  dt=case["dt"], device=gpu_device)
+    st0 = orc.init_state(p.sd, case["nw"], overlap_mode="log")
+    fields = [syn.fields_for_step(case["seed"], t, case["nw"], case["g"]) for t in range(case["steps"])]
+    zetas = [syn.zeta_for_event(case["seed"], k) for k in range(case["steps"] // case["sr"])]
+    st, blocks = orc.run_cadenced(st0, p.sd, p.prm, n_steps=case["steps"], qr_every=case["qr"],
+                                  sr_every=case["sr"], fields_fn=lambda t: fields[t],
+                                  zeta_fn=lambda k: zetas[k], overlap_mode="log")
+    gs = _gpu_state(p, st0, gpu_device)
+    for k in range(case["steps"] // case["sr"]):
+        f = [torch.tensor(fields[k * case["sr"] + t], device=gpu_device) for t in range(case["sr"])]
+        gs, obs, idx = _gpu_block(p, gs, f, zetas[k], case["sr"], case["qr"])
+        ref = blocks[k]
+        np.testing.assert_allclose(obs.scalars["energy"].item(), ref["energy"], rtol=1e-10)
+        np.testing.assert_allclose(obs.scalars["weight"].item(), ref["weight"], rtol=1e-10)
+        np.testing.assert_array_equal(idx.cpu().numpy(), ref["sr_indices"])
+    _cmp_state(gs, st, wtol=1e-9)
