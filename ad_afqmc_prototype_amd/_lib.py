@@ -21,6 +21,14 @@ c_vp = ctypes.c_void_p
 # name -> argtypes (restype is int for every compute entry point)
 _SIGNATURES = {
     "afq_dgemm_nt": [c_int, c_int, c_int, c_double, c_vp, c_ll, c_vp, c_ll, c_double, c_vp, c_ll, c_vp],
+    "afq_sgemm_nt": [c_int, c_int, c_int, ctypes.c_float, c_vp, c_ll, c_vp, c_ll, ctypes.c_float, c_vp,
+                     c_ll, c_int, c_vp],
+    "afq_gemm_bf16x3_nt": [c_int, c_int, c_int, ctypes.c_float, c_vp, c_vp, c_ll, c_vp, c_vp, c_ll, c_vp,
+                           c_ll, c_int, c_vp],
+    "afq_split_bf16": [c_int, c_int, c_vp, c_ll, c_vp, c_vp, c_ll, c_vp],
+    "afq_convert_f32": [c_int, c_int, c_vp, c_ll, c_vp, c_ll, c_vp],
+    "afq_trotter_f32": [c_int, c_int, c_int, c_int, c_int, c_int, c_double, c_vp, c_vp, c_vp, c_vp,
+                        c_vp, c_vp, c_vp],
     "afq_overlap_greens": [c_int, c_int, c_int, c_int, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp],
     "afq_force_bias": [c_int, c_int, c_int, c_double, c_vp, c_vp, c_vp, c_vp],
     "afq_field_shifts": [c_int, c_int, c_double, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp],

@@ -33,10 +33,47 @@ __device__ inline int xcd_contiguous(int b, int total) {
 }
 
 // v_mfma_f64_16x16x4_f64.  Lane l supplies A[l%16][l/16] and B[l/16][l%16];
-// the accumulator holds D[4*(l/16)+r][l%16] in element r.
+// the accumulator holds D[4*r + l/16][l%16] in element r (measured on gfx950).
 __device__ __forceinline__ f64x4 mfma_f64(double a, double b, f64x4 c) {
     return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
 }
+
+// v_mfma_f32_16x16x4_f32: same operand layout as the fp64 variant, but the
+// accumulator holds D[4*(l/16) + r][l%16] in element r (as do the bf16 MFMAs).
+__device__ __forceinline__ f32x4 mfma_f32(float a, float b, f32x4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+}
+
+// Scalar-type dispatch for kernels templated on fp64 / fp32 arithmetic.
+template <typename T>
+struct mfma16;
+template <>
+struct mfma16<double> {
+    using acc = f64x4;
+    __device__ static __forceinline__ acc zero() { return acc{0.0, 0.0, 0.0, 0.0}; }
+    __device__ static __forceinline__ acc fma(double a, double b, acc c) { return mfma_f64(a, b, c); }
+    // row within the 16x16 tile of accumulator element r of lane `lane`
+    __device__ static __forceinline__ int row(int lane, int r) { return 4 * r + (lane >> 4); }
+};
+template <>
+struct mfma16<float> {
+    using acc = f32x4;
+    __device__ static __forceinline__ acc zero() { return acc{0.0f, 0.0f, 0.0f, 0.0f}; }
+    __device__ static __forceinline__ acc fma(float a, float b, acc c) { return mfma_f32(a, b, c); }
+    __device__ static __forceinline__ int row(int lane, int r) { return 4 * (lane >> 4) + r; }
+};
+
+// Four consecutive elements (16 or 32 bytes, naturally aligned).
+template <typename T>
+struct vec4;
+template <>
+struct vec4<double> {
+    using type = double4;
+};
+template <>
+struct vec4<float> {
+    using type = float4;
+};
 
 __device__ __forceinline__ double wave_sum(double v) {
 #pragma unroll

@@ -111,13 +111,13 @@ __global__ __launch_bounds__(THREADS, 1) void energy_kernel(
                 }
             }
             // Coulomb: c_g = tr f_g  (element r of a diagonal tile sits on the
-            // diagonal when lane%16 == 4*(lane/16) + r)
+            // diagonal when its row, 4*r + lane/16, equals its column lane%16)
             double cr = 0.0, ci = 0.0;
 #pragma unroll
             for (int t = 0; t < NT; ++t)
 #pragma unroll
                 for (int r = 0; r < 4; ++r)
-                    if (lr == (lane >> 4) * 4 + r) {
+                    if (lr == afq::mfma16<double>::row(lane, r)) {
                         cr += F[t][t][0][r];
                         ci += F[t][t][1][r];
                     }
@@ -146,7 +146,7 @@ __global__ __launch_bounds__(THREADS, 1) void energy_kernel(
                     for (int tj = 0; tj < NT; ++tj)
 #pragma unroll
                         for (int r = 0; r < 4; ++r) {
-                            const int row = ti * 16 + (lane >> 4) * 4 + r;
+                            const int row = ti * 16 + afq::mfma16<double>::row(lane, r);
                             const int col = tj * 16 + lr;
                             Sw[row * SP + col] = F[ti][tj][pl][r];
                         }
@@ -157,7 +157,7 @@ __global__ __launch_bounds__(THREADS, 1) void energy_kernel(
                     for (int tj = 0; tj < NT; ++tj)
 #pragma unroll
                         for (int r = 0; r < 4; ++r) {
-                            const int row = ti * 16 + (lane >> 4) * 4 + r;
+                            const int row = ti * 16 + afq::mfma16<double>::row(lane, r);
                             const int col = tj * 16 + lr;
                             const double t = Sw[col * SP + row];
                             if (pl == 0) {

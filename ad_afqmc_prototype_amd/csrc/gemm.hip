@@ -1,4 +1,9 @@
-// fp64 GEMM  C = alpha * A * B^T + beta * C  on v_mfma_f64_16x16x4_f64 (gfx950).
+// GEMMs  C = alpha * A * B^T + beta * C  on gfx950 matrix cores:
+//   fp64   v_mfma_f64_16x16x4_f64           (plain fp64 step)
+//   fp32   v_mfma_f32_16x16x4_f32           (mixed step, fp32 contractions)
+//   bf16x3 3 x v_mfma_f32_16x16x32_bf16    (mixed step, split-bf16 contractions:
+//          A B ~ Ahi Bhi + Ahi Blo + Alo Bhi, fp32 accumulation; Henry, Tang &
+//          Heinecke, ARITH 2019)
 //
 // Both operands are K-contiguous (row-major A: M x K, B: N x K), which is the
 // natural layout of the force-bias contraction (greens rows . rotated Cholesky
@@ -25,12 +30,14 @@ constexpr int THREADS = 256;
 constexpr int GROUP_M = 8;
 constexpr int LOADS = BM * BK / THREADS;  // 8 elements of A and of B per thread
 
-__global__ __launch_bounds__(THREADS, 2) void dgemm_nt_kernel(
-    int M, int N, int K, double alpha, const double* __restrict__ A, long long lda,
-    const double* __restrict__ B, long long ldb, double beta, double* __restrict__ C,
+template <typename T, typename TOut>
+__global__ __launch_bounds__(THREADS, 2) void gemm_nt_kernel(
+    int M, int N, int K, T alpha, const T* __restrict__ A, long long lda,
+    const T* __restrict__ B, long long ldb, T beta, TOut* __restrict__ C,
     long long ldc, int tiles_m, int tiles_n) {
-    __shared__ double As[2][BM * PITCH];
-    __shared__ double Bs[2][BN * PITCH];
+    using MF = afq::mfma16<T>;
+    __shared__ T As[2][BM * PITCH];
+    __shared__ T Bs[2][BN * PITCH];
 
     const int total = tiles_m * tiles_n;
     const int t = afq::xcd_contiguous(blockIdx.x, total);
@@ -50,7 +57,7 @@ __global__ __launch_bounds__(THREADS, 2) void dgemm_nt_kernel(
     const int wm = wave >> 1;
     const int wn = wave & 1;
 
-    double ra[LOADS], rb[LOADS];
+    T ra[LOADS], rb[LOADS];
     auto load_tile = [&](int k0) {
 #pragma unroll
         for (int i = 0; i < LOADS; ++i) {
@@ -60,8 +67,8 @@ __global__ __launch_bounds__(THREADS, 2) void dgemm_nt_kernel(
             const int gk = k0 + c;
             const int gm = m0 + r;
             const int gn = n0 + r;
-            ra[i] = (gm < M && gk < K) ? A[(long long)gm * lda + gk] : 0.0;
-            rb[i] = (gn < N && gk < K) ? B[(long long)gn * ldb + gk] : 0.0;
+            ra[i] = (gm < M && gk < K) ? A[(long long)gm * lda + gk] : T(0);
+            rb[i] = (gn < N && gk < K) ? B[(long long)gn * ldb + gk] : T(0);
         }
     };
     auto store_tile = [&](int buf) {
@@ -75,11 +82,11 @@ __global__ __launch_bounds__(THREADS, 2) void dgemm_nt_kernel(
         }
     };
 
-    f64x4 acc[4][4];
+    typename MF::acc acc[4][4];
 #pragma unroll
     for (int i = 0; i < 4; ++i)
 #pragma unroll
-        for (int j = 0; j < 4; ++j) acc[i][j] = f64x4{0.0, 0.0, 0.0, 0.0};
+        for (int j = 0; j < 4; ++j) acc[i][j] = MF::zero();
 
     const int nk = afq::ceil_div(K, BK);
     load_tile(0);
@@ -94,7 +101,7 @@ __global__ __launch_bounds__(THREADS, 2) void dgemm_nt_kernel(
 
 #pragma unroll
         for (int kk = 0; kk < 4; ++kk) {
-            double a[4], b[4];
+            T a[4], b[4];
 #pragma unroll
             for (int i = 0; i < 4; ++i) a[i] = As[buf][(wm * 64 + i * 16 + lr) * PITCH + lk + kk];
 #pragma unroll
@@ -102,7 +109,7 @@ __global__ __launch_bounds__(THREADS, 2) void dgemm_nt_kernel(
 #pragma unroll
             for (int i = 0; i < 4; ++i)
 #pragma unroll
-                for (int j = 0; j < 4; ++j) acc[i][j] = afq::mfma_f64(a[i], b[j], acc[i][j]);
+                for (int j = 0; j < 4; ++j) acc[i][j] = MF::fma(a[i], b[j], acc[i][j]);
         }
 
         if (kt + 1 < nk) store_tile(buf ^ 1);
@@ -116,12 +123,12 @@ __global__ __launch_bounds__(THREADS, 2) void dgemm_nt_kernel(
             const int col = n0 + wn * 64 + j * 16 + lr;
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
-                const int row = m0 + wm * 64 + i * 16 + (lane >> 4) * 4 + r;
+                const int row = m0 + wm * 64 + i * 16 + MF::row(lane, r);
                 if (row < M && col < N) {
-                    double* cp = C + (long long)row * ldc + col;
-                    double v = alpha * acc[i][j][r];
-                    if (beta != 0.0) v += beta * *cp;
-                    *cp = v;
+                    TOut* cp = C + (long long)row * ldc + col;
+                    T v = alpha * acc[i][j][r];
+                    if (beta != T(0)) v += beta * (T)*cp;
+                    *cp = (TOut)v;
                 }
             }
         }
@@ -132,22 +139,39 @@ __global__ __launch_bounds__(THREADS, 2) void dgemm_nt_kernel(
 
 namespace afq {
 
-int dgemm_nt(int M, int N, int K, double alpha, const double* A, long long lda, const double* B,
-             long long ldb, double beta, double* C, long long ldc, hipStream_t stream) {
-    AFQ_REQUIRE(M >= 0 && N >= 0 && K >= 0, "afq_dgemm_nt: negative size (M=%d N=%d K=%d)", M, N, K);
+template <typename T, typename TOut>
+int gemm_nt(const char* name, int M, int N, int K, T alpha, const T* A, long long lda, const T* B,
+            long long ldb, T beta, TOut* C, long long ldc, hipStream_t stream) {
+    AFQ_REQUIRE(M >= 0 && N >= 0 && K >= 0, "%s: negative size (M=%d N=%d K=%d)", name, M, N, K);
     if (M == 0 || N == 0) return 0;
-    AFQ_REQUIRE(A && B && C, "afq_dgemm_nt: null pointer");
-    AFQ_REQUIRE(lda >= K && ldb >= K && ldc >= N, "afq_dgemm_nt: leading dimension too small");
+    AFQ_REQUIRE(A && B && C, "%s: null pointer", name);
+    AFQ_REQUIRE(lda >= K && ldb >= K && ldc >= N, "%s: leading dimension too small", name);
     const int tiles_m = ceil_div(M, BM);
     const int tiles_n = ceil_div(N, BN);
     const long long total = (long long)tiles_m * tiles_n;
-    AFQ_REQUIRE(total < (1LL << 31), "afq_dgemm_nt: problem too large");
-    hipLaunchKernelGGL(dgemm_nt_kernel, dim3((unsigned)total), dim3(THREADS), 0, stream, M, N, K,
+    AFQ_REQUIRE(total < (1LL << 31), "%s: problem too large", name);
+    hipLaunchKernelGGL((gemm_nt_kernel<T, TOut>), dim3((unsigned)total), dim3(THREADS), 0, stream, M, N, K,
                        alpha, A, lda, B, ldb, beta, C, ldc, tiles_m, tiles_n);
-    return check_launch("afq_dgemm_nt");
+    return check_launch(name);
+}
+
+int dgemm_nt(int M, int N, int K, double alpha, const double* A, long long lda, const double* B,
+             long long ldb, double beta, double* C, long long ldc, hipStream_t stream) {
+    return gemm_nt<double, double>("afq_dgemm_nt", M, N, K, alpha, A, lda, B, ldb, beta, C, ldc,
+                                   stream);
 }
 
 }  // namespace afq
+
+extern "C" int afq_sgemm_nt(int M, int N, int K, float alpha, const float* A, long long lda,
+                            const float* B, long long ldb, float beta, void* C, long long ldc,
+                            int c_fp64, void* stream) {
+    if (c_fp64)
+        return afq::gemm_nt<float, double>("afq_sgemm_nt", M, N, K, alpha, A, lda, B, ldb, beta,
+                                           (double*)C, ldc, (hipStream_t)stream);
+    return afq::gemm_nt<float, float>("afq_sgemm_nt", M, N, K, alpha, A, lda, B, ldb, beta,
+                                      (float*)C, ldc, (hipStream_t)stream);
+}
 
 extern "C" int afq_dgemm_nt(int M, int N, int K, double alpha, const double* A, long long lda,
                             const double* B, long long ldb, double beta, double* C, long long ldc,

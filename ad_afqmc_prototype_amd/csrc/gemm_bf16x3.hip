@@ -1,0 +1,234 @@
+// Split-bf16 ("bf16x3") GEMM and the operand converters of the mixed step.
+//
+// An fp64 operand x is carried as two bf16 planes, hi = bf16(x) and
+// lo = bf16(x - hi), so hi + lo holds ~16 significant bits.  The product
+// A B^T is accumulated in fp32 as  Ahi Bhi + Ahi Blo + Alo Bhi  (the dropped
+// Alo Blo term is ~2^-16 relative), three v_mfma_f32_16x16x32_bf16 per
+// 16x16x32 step.
+//
+// Tiling: 128 x 128 x 32 per 256-thread workgroup (2 x 2 waves of 64 x 64),
+// all four operand planes staged through double-buffered LDS (80 KB, two
+// workgroups per CU).  Each lane reads 8 consecutive k of one row (16 bytes)
+// for the A and B fragments, so A and B see the same k order.  K must be a
+// multiple of 8 (operand planes are zero-padded to that by the converter).
+#include <cstdint>
+
+#include "afqmc_hip.h"
+#include "common.h"
+
+namespace {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+
+constexpr int BM = 128;
+constexpr int BN = 128;
+constexpr int BK = 32;
+constexpr int PITCH = 40;  // 80-byte rows: conflict-free 16-byte LDS reads
+constexpr int THREADS = 256;
+constexpr int GROUP_M = 8;
+
+__device__ __forceinline__ uint16_t f32_to_bf16_rne(float f) {
+    uint32_t u = __float_as_uint(f);
+    if ((u & 0x7fffffffu) > 0x7f800000u) return 0x7fc0;  // NaN
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return (uint16_t)(u >> 16);
+}
+
+__device__ __forceinline__ double bf16_to_f64(uint16_t h) {
+    return (double)__uint_as_float(((uint32_t)h) << 16);
+}
+
+template <typename TOut>
+__global__ __launch_bounds__(THREADS, 2) void gemm_bf16x3_nt_kernel(
+    int M, int N, int K, float alpha, const uint16_t* __restrict__ Ahi,
+    const uint16_t* __restrict__ Alo, long long lda, const uint16_t* __restrict__ Bhi,
+    const uint16_t* __restrict__ Blo, long long ldb, TOut* __restrict__ C, long long ldc,
+    int tiles_m, int tiles_n) {
+    __shared__ __attribute__((aligned(16))) uint16_t S[2][4][BM * PITCH];
+
+    const int total = tiles_m * tiles_n;
+    const int t = afq::xcd_contiguous(blockIdx.x, total);
+    const int per_group = GROUP_M * tiles_n;
+    const int group = t / per_group;
+    const int first_m = group * GROUP_M;
+    const int gsize = min(tiles_m - first_m, GROUP_M);
+    const int in_group = t - group * per_group;
+    const int m0 = (first_m + in_group % gsize) * BM;
+    const int n0 = (in_group / gsize) * BN;
+
+    const int tid = threadIdx.x;
+    const int lane = tid & 63;
+    const int wave = tid >> 6;
+    const int wm = wave >> 1;
+    const int wn = wave & 1;
+
+    uint4 reg[4][2];
+    auto load_tile = [&](int k0) {
+#pragma unroll
+        for (int a = 0; a < 4; ++a) {
+            const uint16_t* src = a == 0 ? Ahi : a == 1 ? Alo : a == 2 ? Bhi : Blo;
+            const int base = a < 2 ? m0 : n0;
+            const int lim = a < 2 ? M : N;
+            const long long ld = a < 2 ? lda : ldb;
+#pragma unroll
+            for (int j = 0; j < 2; ++j) {
+                const int chunk = tid + THREADS * j;
+                const int row = chunk >> 2;
+                const int kc = (chunk & 3) * 8;
+                const int grow = base + row;
+                const int gk = k0 + kc;
+                if (grow < lim && gk < K)
+                    reg[a][j] = *reinterpret_cast<const uint4*>(src + (long long)grow * ld + gk);
+                else
+                    reg[a][j] = make_uint4(0u, 0u, 0u, 0u);
+            }
+        }
+    };
+    auto store_tile = [&](int buf) {
+#pragma unroll
+        for (int a = 0; a < 4; ++a)
+#pragma unroll
+            for (int j = 0; j < 2; ++j) {
+                const int chunk = tid + THREADS * j;
+                const int row = chunk >> 2;
+                const int kc = (chunk & 3) * 8;
+                *reinterpret_cast<uint4*>(&S[buf][a][row * PITCH + kc]) = reg[a][j];
+            }
+    };
+
+    f32x4 acc[4][4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
+
+    const int nk = afq::ceil_div(K, BK);
+    load_tile(0);
+    store_tile(0);
+    __syncthreads();
+
+    const int lr = lane & 15;
+    const int lk = (lane >> 4) * 8;
+    for (int kt = 0; kt < nk; ++kt) {
+        const int buf = kt & 1;
+        if (kt + 1 < nk) load_tile((kt + 1) * BK);
+        bf16x8 ah[4], al[4], bh[4], bl[4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const int off = (wm * 64 + i * 16 + lr) * PITCH + lk;
+            ah[i] = *reinterpret_cast<const bf16x8*>(&S[buf][0][off]);
+            al[i] = *reinterpret_cast<const bf16x8*>(&S[buf][1][off]);
+        }
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int off = (wn * 64 + j * 16 + lr) * PITCH + lk;
+            bh[j] = *reinterpret_cast<const bf16x8*>(&S[buf][2][off]);
+            bl[j] = *reinterpret_cast<const bf16x8*>(&S[buf][3][off]);
+        }
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[i], bh[j], acc[i][j], 0, 0, 0);
+                acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bl[j], acc[i][j], 0, 0, 0);
+                acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bh[j], acc[i][j], 0, 0, 0);
+            }
+        if (kt + 1 < nk) store_tile(buf ^ 1);
+        __syncthreads();
+    }
+
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int col = n0 + wn * 64 + j * 16 + lr;
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int row = m0 + wm * 64 + i * 16 + (lane >> 4) * 4 + r;
+                if (row < M && col < N) C[(long long)row * ldc + col] = (TOut)(alpha * acc[i][j][r]);
+            }
+        }
+}
+
+__global__ void split_bf16_kernel(int rows, int cols, const double* __restrict__ src,
+                                  long long ld_src, uint16_t* __restrict__ hi,
+                                  uint16_t* __restrict__ lo, long long ld_dst) {
+    const long long total = (long long)rows * ld_dst;
+    for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < total;
+         e += (long long)gridDim.x * blockDim.x) {
+        const long long r = e / ld_dst;
+        const long long c = e - r * ld_dst;
+        const double x = c < cols ? src[r * ld_src + c] : 0.0;
+        const uint16_t h = f32_to_bf16_rne((float)x);
+        const uint16_t l = f32_to_bf16_rne((float)(x - bf16_to_f64(h)));
+        hi[e] = h;
+        lo[e] = l;
+    }
+}
+
+__global__ void convert_f32_kernel(int rows, int cols, const double* __restrict__ src,
+                                   long long ld_src, float* __restrict__ dst, long long ld_dst) {
+    const long long total = (long long)rows * ld_dst;
+    for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < total;
+         e += (long long)gridDim.x * blockDim.x) {
+        const long long r = e / ld_dst;
+        const long long c = e - r * ld_dst;
+        dst[e] = c < cols ? (float)src[r * ld_src + c] : 0.0f;
+    }
+}
+
+unsigned grid_for(long long total) {
+    long long b = (total + 255) / 256;
+    return (unsigned)(b > 65536 ? 65536 : (b < 1 ? 1 : b));
+}
+
+}  // namespace
+
+extern "C" int afq_gemm_bf16x3_nt(int M, int N, int K, float alpha, const unsigned short* A_hi,
+                                  const unsigned short* A_lo, long long lda,
+                                  const unsigned short* B_hi, const unsigned short* B_lo,
+                                  long long ldb, void* C, long long ldc, int c_fp64, void* stream) {
+    AFQ_REQUIRE(M >= 0 && N >= 0 && K >= 0, "afq_gemm_bf16x3_nt: negative size");
+    if (M == 0 || N == 0) return 0;
+    AFQ_REQUIRE(A_hi && A_lo && B_hi && B_lo && C, "afq_gemm_bf16x3_nt: null pointer");
+    AFQ_REQUIRE(K % 8 == 0 && lda % 8 == 0 && ldb % 8 == 0,
+                "afq_gemm_bf16x3_nt: K=%d, lda and ldb must be multiples of 8", K);
+    AFQ_REQUIRE(lda >= K && ldb >= K && ldc >= N, "afq_gemm_bf16x3_nt: leading dimension too small");
+    const int tiles_m = afq::ceil_div(M, BM);
+    const int tiles_n = afq::ceil_div(N, BN);
+    const long long total = (long long)tiles_m * tiles_n;
+    AFQ_REQUIRE(total < (1LL << 31), "afq_gemm_bf16x3_nt: problem too large");
+    hipStream_t st = (hipStream_t)stream;
+    if (c_fp64)
+        hipLaunchKernelGGL(gemm_bf16x3_nt_kernel<double>, dim3((unsigned)total), dim3(THREADS), 0, st,
+                           M, N, K, alpha, A_hi, A_lo, lda, B_hi, B_lo, ldb, (double*)C, ldc,
+                           tiles_m, tiles_n);
+    else
+        hipLaunchKernelGGL(gemm_bf16x3_nt_kernel<float>, dim3((unsigned)total), dim3(THREADS), 0, st,
+                           M, N, K, alpha, A_hi, A_lo, lda, B_hi, B_lo, ldb, (float*)C, ldc,
+                           tiles_m, tiles_n);
+    return afq::check_launch("afq_gemm_bf16x3_nt");
+}
+
+extern "C" int afq_split_bf16(int rows, int cols, const double* src, long long ld_src,
+                              unsigned short* hi, unsigned short* lo, long long ld_dst,
+                              void* stream) {
+    AFQ_REQUIRE(rows >= 0 && cols >= 0 && ld_src >= cols && ld_dst >= cols,
+                "afq_split_bf16: bad sizes rows=%d cols=%d", rows, cols);
+    if (rows == 0 || ld_dst == 0) return 0;
+    AFQ_REQUIRE(src && hi && lo, "afq_split_bf16: null pointer");
+    hipLaunchKernelGGL(split_bf16_kernel, dim3(grid_for((long long)rows * ld_dst)), dim3(256), 0,
+                       (hipStream_t)stream, rows, cols, src, ld_src, hi, lo, ld_dst);
+    return afq::check_launch("afq_split_bf16");
+}
+
+extern "C" int afq_convert_f32(int rows, int cols, const double* src, long long ld_src, float* dst,
+                               long long ld_dst, void* stream) {
+    AFQ_REQUIRE(rows >= 0 && cols >= 0 && ld_src >= cols && ld_dst >= cols,
+                "afq_convert_f32: bad sizes rows=%d cols=%d", rows, cols);
+    if (rows == 0 || ld_dst == 0) return 0;
+    AFQ_REQUIRE(src && dst, "afq_convert_f32: null pointer");
+    hipLaunchKernelGGL(convert_f32_kernel, dim3(grid_for((long long)rows * ld_dst)), dim3(256), 0,
+                       (hipStream_t)stream, rows, cols, src, ld_src, dst, ld_dst);
+    return afq::check_launch("afq_convert_f32");
+}

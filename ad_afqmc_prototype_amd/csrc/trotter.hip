@@ -1,13 +1,15 @@
-// Fused Trotter step  w' = E . sum_{k<=K} (i sqrt_dt V)^k / k! . E w  (fp64).
+// Fused Trotter step  w' = E . sum_{k<=K} (i sqrt_dt V)^k / k! . E w, in fp64
+// (plain step) or fp32 (mixed step: E, V and the Taylor terms in fp32, walkers
+// read and written as complex128).
 //
 // One 256-thread workgroup per (walker, spin).  The running Taylor term T
 // (n x o complex, split planes) lives in LDS (2 * ceil16(n) * 16*NC doubles;
-// 106 KB at n=200, o=32); the running sum lives in MFMA accumulators.  Every
+// 106 KB at n=200, o=32 in fp64); the running sum lives in MFMA accumulators.  Every
 // product is  P = A . T  with A either the real one-body propagator E or the
 // walker's complex VHS, streamed from global memory (L2/MALL) straight into
 // MFMA A-operand registers, one k-block ahead of use.  Wave wv owns the row
 // tiles wv, wv+4, ...; each lane feeds four consecutive k of one row to four
-// v_mfma_f64_16x16x4 (the B operand from LDS uses the same k order).
+// v_mfma_{f64,f32}_16x16x4 (the B operand from LDS uses the same k order).
 //
 // Sequence per workgroup (prop/chol_afqmc_ops.py:173-183, prop/utils.py:20-31):
 //   T = E w;  out = T;  for k: T = (i sqrt_dt / k) V T, out += T;  w' = E out.
@@ -19,16 +21,18 @@ namespace {
 // P[i][plane] = A . T for the row tiles (rg + 4 i) and column tile ct of this
 // wave.  The A operand of (k-block, row tile) step q+1 is loaded while the
 // MFMAs of step q run.
-template <int NC, int RPW, bool CPLX>
-__device__ __forceinline__ void tile_product(const double* __restrict__ Ar,
-                                             const double* __restrict__ Ai, int lda, int n,
-                                             int ntile, const double* Tr, const double* Ti,
-                                             int rg, int ct, int lane, f64x4 (&P)[RPW][2]) {
+template <typename R, int NC, int RPW, bool CPLX>
+__device__ __forceinline__ void tile_product(const R* __restrict__ Ar, const R* __restrict__ Ai,
+                                             int lda, int n, int ntile, const R* Tr, const R* Ti,
+                                             int rg, int ct, int lane,
+                                             typename afq::mfma16<R>::acc (&P)[RPW][2]) {
+    using MF = afq::mfma16<R>;
+    using V4 = typename afq::vec4<R>::type;
     constexpr int TC = NC * 16;
 #pragma unroll
     for (int i = 0; i < RPW; ++i) {
-        P[i][0] = f64x4{0.0, 0.0, 0.0, 0.0};
-        P[i][1] = f64x4{0.0, 0.0, 0.0, 0.0};
+        P[i][0] = MF::zero();
+        P[i][1] = MF::zero();
     }
     const int lr = lane & 15;
     const int lk = (lane >> 4) * 4;
@@ -37,27 +41,27 @@ __device__ __forceinline__ void tile_product(const double* __restrict__ Ar,
 
     // Branch-free operand load: out-of-range rows / k-blocks read a clamped,
     // valid address and are zeroed by a select.
-    auto load_a = [&](int kb, int i, double (&xr)[4], double (&xi)[4]) {
+    auto load_a = [&](int kb, int i, R (&xr)[4], R (&xi)[4]) {
         const int k0 = kb * 16 + lk;
         const int rt = rg + 4 * i;
         const int row = rt * 16 + lr;
         const bool ok = rt < ntile && row < n && k0 < n;
         const int rowc = ok ? row : 0;
         const int k0c = ok ? k0 : 0;
-        const double4 v = *reinterpret_cast<const double4*>(Ar + (long long)rowc * lda + k0c);
-        xr[0] = ok ? v.x : 0.0; xr[1] = ok ? v.y : 0.0; xr[2] = ok ? v.z : 0.0; xr[3] = ok ? v.w : 0.0;
+        const V4 v = *reinterpret_cast<const V4*>(Ar + (long long)rowc * lda + k0c);
+        xr[0] = ok ? v.x : R(0); xr[1] = ok ? v.y : R(0); xr[2] = ok ? v.z : R(0); xr[3] = ok ? v.w : R(0);
         if (CPLX) {
-            const double4 u = *reinterpret_cast<const double4*>(Ai + (long long)rowc * lda + k0c);
-            xi[0] = ok ? u.x : 0.0; xi[1] = ok ? u.y : 0.0; xi[2] = ok ? u.z : 0.0; xi[3] = ok ? u.w : 0.0;
+            const V4 u = *reinterpret_cast<const V4*>(Ai + (long long)rowc * lda + k0c);
+            xi[0] = ok ? u.x : R(0); xi[1] = ok ? u.y : R(0); xi[2] = ok ? u.z : R(0); xi[3] = ok ? u.w : R(0);
         }
     };
 
-    double ar[4], ai[4];
+    R ar[4], ai[4];
     load_a(0, 0, ar, ai);
 #pragma unroll 1
     for (int kb = 0; kb < nkb; ++kb) {
         const int k0 = kb * 16 + lk;
-        double br[4], bi[4], nbi[4];
+        R br[4], bi[4], nbi[4];
 #pragma unroll
         for (int kk = 0; kk < 4; ++kk) {
             br[kk] = Tr[(k0 + kk) * TC + col];
@@ -66,7 +70,7 @@ __device__ __forceinline__ void tile_product(const double* __restrict__ Ar,
         }
 #pragma unroll
         for (int i = 0; i < RPW; ++i) {
-            double nr[4], ni[4];
+            R nr[4], ni[4];
             const int nkb_i = (i + 1 < RPW) ? kb : kb + 1;
             const int ni_i = (i + 1 < RPW) ? i + 1 : 0;
             // the last step prefetches a clamped (discarded) address
@@ -74,12 +78,12 @@ __device__ __forceinline__ void tile_product(const double* __restrict__ Ar,
             // tiles past the last row tile multiply zeros (no divergent accumulators)
 #pragma unroll
             for (int kk = 0; kk < 4; ++kk) {
-                P[i][0] = afq::mfma_f64(ar[kk], br[kk], P[i][0]);
-                P[i][1] = afq::mfma_f64(ar[kk], bi[kk], P[i][1]);
+                P[i][0] = MF::fma(ar[kk], br[kk], P[i][0]);
+                P[i][1] = MF::fma(ar[kk], bi[kk], P[i][1]);
                 if (CPLX) {
                     // re -= Vi Ti ;  im += Vi Tr
-                    P[i][0] = afq::mfma_f64(ai[kk], nbi[kk], P[i][0]);
-                    P[i][1] = afq::mfma_f64(ai[kk], br[kk], P[i][1]);
+                    P[i][0] = MF::fma(ai[kk], nbi[kk], P[i][0]);
+                    P[i][1] = MF::fma(ai[kk], br[kk], P[i][1]);
                 }
             }
 #pragma unroll
@@ -91,9 +95,9 @@ __device__ __forceinline__ void tile_product(const double* __restrict__ Ar,
     }
 }
 
-template <int NC, int RPW>
-__device__ __forceinline__ void store_T(double* Tr, double* Ti, int rg, int ct, int lane, int ntile,
-                                        const f64x4 (&X)[RPW][2]) {
+template <typename R, int NC, int RPW>
+__device__ __forceinline__ void store_T(R* Tr, R* Ti, int rg, int ct, int lane, int ntile,
+                                        const typename afq::mfma16<R>::acc (&X)[RPW][2]) {
     constexpr int TC = NC * 16;
     const int col = ct * 16 + (lane & 15);
 #pragma unroll
@@ -102,7 +106,7 @@ __device__ __forceinline__ void store_T(double* Tr, double* Ti, int rg, int ct, 
         if (rt < ntile) {
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
-                const int row = rt * 16 + (lane >> 4) * 4 + r;
+                const int row = rt * 16 + afq::mfma16<R>::row(lane, r);
                 Tr[row * TC + col] = X[i][0][r];
                 Ti[row * TC + col] = X[i][1][r];
             }
@@ -111,19 +115,21 @@ __device__ __forceinline__ void store_T(double* Tr, double* Ti, int rg, int ct, 
 }
 
 // 4 * NC waves: wave -> (row group rg = wave % 4, column tile ct = wave / 4).
-template <int NC, int RPW>
+template <typename R, int NC, int RPW>
 __global__ __launch_bounds__(256 * NC, 1) void trotter_kernel(
     int nw, int n, int o, int nspin, int ldv, int n_terms, double sqrt_dt,
-    const double* __restrict__ E, const double* __restrict__ vhs, long long vplane,
+    const R* __restrict__ E, const R* __restrict__ vhs, long long vplane,
     const double2* __restrict__ win_up, const double2* __restrict__ win_dn,
     double2* __restrict__ wout_up, double2* __restrict__ wout_dn, int pair_map) {
-    extern __shared__ double lds[];
+    using ACC = typename afq::mfma16<R>::acc;
+    extern __shared__ __attribute__((aligned(16))) unsigned char lds_raw[];
+    R* lds = reinterpret_cast<R*>(lds_raw);
     constexpr int TC = NC * 16;
     constexpr int THREADS = 256 * NC;
     const int NP = afq::round_up(n, 16);
     const int ntile = NP / 16;
-    double* Tr = lds;
-    double* Ti = lds + NP * TC;
+    R* Tr = lds;
+    R* Ti = lds + NP * TC;
 
     // Both spins of a walker go to the same XCD, close together in time, so
     // the second reads the walker's VHS from L2.
@@ -142,8 +148,8 @@ __global__ __launch_bounds__(256 * NC, 1) void trotter_kernel(
     const long long woff = (long long)w * n * o;
     const double2* Win = (s == 0 ? win_up : win_dn) + woff;
     double2* Wout = (s == 0 ? wout_up : wout_dn) + woff;
-    const double* Vr = vhs + (long long)w * n * ldv;
-    const double* Vi = Vr + vplane;
+    const R* Vr = vhs + (long long)w * n * ldv;
+    const R* Vi = Vr + vplane;
 
     const int tid = threadIdx.x;
     const int lane = tid & 63;
@@ -155,18 +161,18 @@ __global__ __launch_bounds__(256 * NC, 1) void trotter_kernel(
         const int p = e / TC;
         const int j = e - p * TC;
         const double2 x = (p < n && j < o) ? Win[p * o + j] : make_double2(0.0, 0.0);
-        Tr[e] = x.x;
-        Ti[e] = x.y;
+        Tr[e] = (R)x.x;
+        Ti[e] = (R)x.y;
     }
     __syncthreads();
 
-    f64x4 P[RPW][2];
-    f64x4 out[RPW][2];
+    ACC P[RPW][2];
+    ACC out[RPW][2];
 
     // T = E w
-    tile_product<NC, RPW, false>(E, nullptr, ldv, n, ntile, Tr, Ti, rg, ct, lane, P);
+    tile_product<R, NC, RPW, false>(E, nullptr, ldv, n, ntile, Tr, Ti, rg, ct, lane, P);
     __syncthreads();
-    store_T<NC, RPW>(Tr, Ti, rg, ct, lane, ntile, P);
+    store_T<R, NC, RPW>(Tr, Ti, rg, ct, lane, ntile, P);
 #pragma unroll
     for (int i = 0; i < RPW; ++i) {
         out[i][0] = P[i][0];
@@ -176,27 +182,27 @@ __global__ __launch_bounds__(256 * NC, 1) void trotter_kernel(
 
 #pragma unroll 1
     for (int k = 1; k <= n_terms; ++k) {
-        tile_product<NC, RPW, true>(Vr, Vi, ldv, n, ntile, Tr, Ti, rg, ct, lane, P);
-        const double ck = sqrt_dt / (double)k;
+        tile_product<R, NC, RPW, true>(Vr, Vi, ldv, n, ntile, Tr, Ti, rg, ct, lane, P);
+        const R ck = (R)(sqrt_dt / (double)k);
         // (i ck) * P
 #pragma unroll
         for (int i = 0; i < RPW; ++i) {
-            const f64x4 re = -ck * P[i][1];
-            const f64x4 im = ck * P[i][0];
+            const ACC re = -ck * P[i][1];
+            const ACC im = ck * P[i][0];
             P[i][0] = re;
             P[i][1] = im;
             out[i][0] += re;
             out[i][1] += im;
         }
         __syncthreads();
-        store_T<NC, RPW>(Tr, Ti, rg, ct, lane, ntile, P);
+        store_T<R, NC, RPW>(Tr, Ti, rg, ct, lane, ntile, P);
         __syncthreads();
     }
 
     // w' = E out
-    store_T<NC, RPW>(Tr, Ti, rg, ct, lane, ntile, out);
+    store_T<R, NC, RPW>(Tr, Ti, rg, ct, lane, ntile, out);
     __syncthreads();
-    tile_product<NC, RPW, false>(E, nullptr, ldv, n, ntile, Tr, Ti, rg, ct, lane, P);
+    tile_product<R, NC, RPW, false>(E, nullptr, ldv, n, ntile, Tr, Ti, rg, ct, lane, P);
     const int col = ct * 16 + (lane & 15);
 #pragma unroll
     for (int i = 0; i < RPW; ++i) {
@@ -204,60 +210,58 @@ __global__ __launch_bounds__(256 * NC, 1) void trotter_kernel(
         if (rt < ntile && col < o) {
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
-                const int row = rt * 16 + (lane >> 4) * 4 + r;
-                if (row < n) Wout[row * o + col] = make_double2(P[i][0][r], P[i][1][r]);
+                const int row = rt * 16 + afq::mfma16<R>::row(lane, r);
+                if (row < n) Wout[row * o + col] = make_double2((double)P[i][0][r], (double)P[i][1][r]);
             }
         }
     }
 }
 
-template <int NC, int RPW>
+template <typename R, int NC, int RPW>
 int launch_trotter(int nw, int n, int o, int nspin, int ldv, int n_terms, double sqrt_dt,
-                   const double* E, const double* vhs, const double2* wu, const double2* wd,
+                   const R* E, const R* vhs, const double2* wu, const double2* wd,
                    double2* ou, double2* od, hipStream_t st) {
     const int NP = afq::round_up(n, 16);
-    const size_t lds = (size_t)2 * NP * NC * 16 * sizeof(double);
+    const size_t lds = (size_t)2 * NP * NC * 16 * sizeof(R);
     const int pair_map = (nspin == 2 && nw % 8 == 0) ? 1 : 0;
     if (lds > 64 * 1024) {
-        (void)hipFuncSetAttribute((const void*)trotter_kernel<NC, RPW>,
+        (void)hipFuncSetAttribute((const void*)trotter_kernel<R, NC, RPW>,
                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
         (void)hipGetLastError();
     }
-    hipLaunchKernelGGL((trotter_kernel<NC, RPW>), dim3((unsigned)(nw * nspin)), dim3(256 * NC), lds,
+    hipLaunchKernelGGL((trotter_kernel<R, NC, RPW>), dim3((unsigned)(nw * nspin)), dim3(256 * NC), lds,
                        st, nw, n, o, nspin, ldv, n_terms, sqrt_dt, E, vhs,
                        (long long)nw * n * ldv, wu, wd, ou, od, pair_map);
     return afq::check_launch("afq_trotter");
 }
 
-}  // namespace
-
-extern "C" int afq_trotter(int nw, int n, int o, int nspin, int ldv, int n_terms, double sqrt_dt,
-                           const double* exp_h1_half, const double* vhs, const double* win_up,
-                           const double* win_dn, double* wout_up, double* wout_dn, void* stream) {
-    AFQ_REQUIRE(nw > 0 && n > 0 && o > 0 && o <= n, "afq_trotter: bad sizes nw=%d n=%d o=%d", nw, n, o);
-    AFQ_REQUIRE(nspin == 1 || nspin == 2, "afq_trotter: nspin must be 1 or 2");
-    AFQ_REQUIRE(ldv >= n && ldv % 4 == 0, "afq_trotter: ldv=%d must be >= n and a multiple of 4", ldv);
-    AFQ_REQUIRE(n_terms >= 0, "afq_trotter: n_terms < 0");
-    AFQ_REQUIRE(exp_h1_half && vhs && win_up && wout_up, "afq_trotter: null pointer");
-    AFQ_REQUIRE(nspin == 1 || (win_dn && wout_dn), "afq_trotter: null spin-down pointer");
-    AFQ_REQUIRE(win_up != wout_up && (nspin == 1 || win_dn != wout_dn), "afq_trotter: output aliases input");
+template <typename R>
+int trotter_entry(const char* name, int nw, int n, int o, int nspin, int ldv, int n_terms,
+                  double sqrt_dt, const R* exp_h1_half, const R* vhs, const double* win_up,
+                  const double* win_dn, double* wout_up, double* wout_dn, hipStream_t st) {
+    AFQ_REQUIRE(nw > 0 && n > 0 && o > 0 && o <= n, "%s: bad sizes nw=%d n=%d o=%d", name, nw, n, o);
+    AFQ_REQUIRE(nspin == 1 || nspin == 2, "%s: nspin must be 1 or 2", name);
+    AFQ_REQUIRE(ldv >= n && ldv % 4 == 0, "%s: ldv=%d must be >= n and a multiple of 4", name, ldv);
+    AFQ_REQUIRE(n_terms >= 0, "%s: n_terms < 0", name);
+    AFQ_REQUIRE(exp_h1_half && vhs && win_up && wout_up, "%s: null pointer", name);
+    AFQ_REQUIRE(nspin == 1 || (win_dn && wout_dn), "%s: null spin-down pointer", name);
+    AFQ_REQUIRE(win_up != wout_up && (nspin == 1 || win_dn != wout_dn), "%s: output aliases input", name);
     const int NP = afq::round_up(n, 16);
     const int nc = afq::ceil_div(o, 16);
     const int ntile = NP / 16;
     const int rpw = afq::ceil_div(ntile, 4);
-    AFQ_REQUIRE(nc <= 2, "afq_trotter: o=%d exceeds the supported maximum of 32", o);
-    AFQ_REQUIRE(rpw <= 5, "afq_trotter: n=%d exceeds the supported maximum of 320", n);
-    AFQ_REQUIRE((size_t)2 * NP * nc * 16 * sizeof(double) <= 160 * 1024,
-                "afq_trotter: n=%d o=%d needs more than 160 KB of LDS", n, o);
-    hipStream_t st = (hipStream_t)stream;
+    AFQ_REQUIRE(nc <= 2, "%s: o=%d exceeds the supported maximum of 32", name, o);
+    AFQ_REQUIRE(rpw <= 5, "%s: n=%d exceeds the supported maximum of 320", name, n);
+    AFQ_REQUIRE((size_t)2 * NP * nc * 16 * sizeof(R) <= 160 * 1024,
+                "%s: n=%d o=%d needs more than 160 KB of LDS", name, n, o);
     const double2* wu = (const double2*)win_up;
     const double2* wd = (const double2*)(nspin == 2 ? win_dn : win_up);
     double2* ou = (double2*)wout_up;
     double2* od = (double2*)(nspin == 2 ? wout_dn : wout_up);
 #define AFQ_TROTTER_CASE(NC_, RPW_)                                                            \
     if (nc == NC_ && rpw == RPW_)                                                              \
-        return launch_trotter<NC_, RPW_>(nw, n, o, nspin, ldv, n_terms, sqrt_dt, exp_h1_half,  \
-                                         vhs, wu, wd, ou, od, st);
+        return launch_trotter<R, NC_, RPW_>(nw, n, o, nspin, ldv, n_terms, sqrt_dt, exp_h1_half, \
+                                            vhs, wu, wd, ou, od, st);
     AFQ_TROTTER_CASE(1, 1)
     AFQ_TROTTER_CASE(1, 2)
     AFQ_TROTTER_CASE(1, 3)
@@ -269,6 +273,24 @@ extern "C" int afq_trotter(int nw, int n, int o, int nspin, int ldv, int n_terms
     AFQ_TROTTER_CASE(2, 4)
     AFQ_TROTTER_CASE(2, 5)
 #undef AFQ_TROTTER_CASE
-    afq::set_error("afq_trotter: no kernel for n=%d o=%d", n, o);
+    afq::set_error("%s: no kernel for n=%d o=%d", name, n, o);
     return -1;
+}
+
+}  // namespace
+
+extern "C" int afq_trotter(int nw, int n, int o, int nspin, int ldv, int n_terms, double sqrt_dt,
+                           const double* exp_h1_half, const double* vhs, const double* win_up,
+                           const double* win_dn, double* wout_up, double* wout_dn, void* stream) {
+    return trotter_entry<double>("afq_trotter", nw, n, o, nspin, ldv, n_terms, sqrt_dt, exp_h1_half,
+                                 vhs, win_up, win_dn, wout_up, wout_dn, (hipStream_t)stream);
+}
+
+extern "C" int afq_trotter_f32(int nw, int n, int o, int nspin, int ldv, int n_terms,
+                               double sqrt_dt, const float* exp_h1_half, const float* vhs,
+                               const double* win_up, const double* win_dn, double* wout_up,
+                               double* wout_dn, void* stream) {
+    return trotter_entry<float>("afq_trotter_f32", nw, n, o, nspin, ldv, n_terms, sqrt_dt,
+                                exp_h1_half, vhs, win_up, win_dn, wout_up, wout_dn,
+                                (hipStream_t)stream);
 }

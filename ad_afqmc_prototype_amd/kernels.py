@@ -11,6 +11,7 @@ import torch
 from . import _lib
 
 f64 = torch.float64
+f32 = torch.float32
 c128 = torch.complex128
 
 
@@ -49,7 +50,7 @@ class timing:
         return out
 
 
-def _call(name, stream_tensor, *args):
+def _call(name, stream_tensor, *args, tag=None):
     if _timing is None:
         _lib.call(name, *args)
         return
@@ -59,7 +60,7 @@ def _call(name, stream_tensor, *args):
     e0.record(st)
     _lib.call(name, *args)
     e1.record(st)
-    _timing.events.setdefault(name, []).append((e0, e1))
+    _timing.events.setdefault(name if tag is None else f"{name}:{tag}", []).append((e0, e1))
 
 
 def _ptr(t):
@@ -96,6 +97,67 @@ def dgemm_nt(A, B, C, alpha=1.0, beta=0.0):
     _check(C, f64, (M, N), "C", A.device)
     _call("afq_dgemm_nt", A, M, N, K, float(alpha), A.data_ptr(), K, B.data_ptr(), K, float(beta),
               C.data_ptr(), N, _stream(A))
+    return C
+
+
+def sgemm_nt(A, B, C, alpha=1.0, beta=0.0, tag=None):
+    """C = alpha * A @ B.T + beta * C, fp32 operands, C float32 or float64."""
+    M, K = A.shape
+    N = B.shape[0]
+    _check(A, f32, (M, K), "A")
+    _check(B, f32, (N, K), "B", A.device)
+    if C.dtype not in (f32, f64):
+        raise TypeError("C must be float32 or float64")
+    _check(C, C.dtype, (M, N), "C", A.device)
+    _call("afq_sgemm_nt", A, M, N, K, float(alpha), A.data_ptr(), K, B.data_ptr(), K, float(beta),
+          C.data_ptr(), N, int(C.dtype == f64), _stream(A), tag=tag)
+    return C
+
+
+def pad8(k: int) -> int:
+    return (k + 7) // 8 * 8
+
+
+def split_bf16(x: torch.Tensor, ld: int | None = None):
+    """float64 (rows, cols) -> (hi, lo) bf16 bit planes (uint16 as int16) of
+    shape (rows, ld), ld = cols rounded up to 8 by default (zero padding)."""
+    if x.dim() != 2:
+        x = x.reshape(-1, x.shape[-1])
+    rows, cols = x.shape
+    _check(x, f64, (rows, cols), "x")
+    ld = pad8(cols) if ld is None else ld
+    hi = torch.empty((rows, ld), dtype=torch.int16, device=x.device)
+    lo = torch.empty((rows, ld), dtype=torch.int16, device=x.device)
+    _call("afq_split_bf16", x, rows, cols, x.data_ptr(), cols, hi.data_ptr(), lo.data_ptr(), ld,
+          _stream(x))
+    return hi, lo
+
+
+def convert_f32(x: torch.Tensor, ld: int | None = None):
+    """float64 (rows, cols) -> float32 (rows, ld) with zero padding."""
+    if x.dim() != 2:
+        x = x.reshape(-1, x.shape[-1])
+    rows, cols = x.shape
+    _check(x, f64, (rows, cols), "x")
+    ld = cols if ld is None else ld
+    out = torch.empty((rows, ld), dtype=f32, device=x.device)
+    _call("afq_convert_f32", x, rows, cols, x.data_ptr(), cols, out.data_ptr(), ld, _stream(x))
+    return out
+
+
+def gemm_bf16x3_nt(a_hi, a_lo, b_hi, b_lo, C, alpha=1.0, tag=None):
+    """C (M, N) = alpha * (Ahi Bhi^T + Ahi Blo^T + Alo Bhi^T); planes from split_bf16."""
+    M, K = a_hi.shape
+    N = b_hi.shape[0]
+    for t, nm, sh in ((a_hi, "a_hi", (M, K)), (a_lo, "a_lo", (M, K)), (b_hi, "b_hi", (N, K)),
+                      (b_lo, "b_lo", (N, K))):
+        _check(t, torch.int16, sh, nm, a_hi.device)
+    if C.dtype not in (f32, f64):
+        raise TypeError("C must be float32 or float64")
+    _check(C, C.dtype, (M, N), "C", a_hi.device)
+    _call("afq_gemm_bf16x3_nt", a_hi, M, N, K, float(alpha), a_hi.data_ptr(), a_lo.data_ptr(), K,
+          b_hi.data_ptr(), b_lo.data_ptr(), K, C.data_ptr(), N, int(C.dtype == f64), _stream(a_hi),
+          tag=tag)
     return C
 
 
@@ -171,6 +233,8 @@ def vhs(xbar, chol_t, n, vhs_out=None):
 
 
 def trotter(walkers, vhs_planes, exp_h1_half_pad, sqrt_dt, n_terms, out=None):
+    """Fused Trotter step; fp64 when vhs / exp_h1_half are float64, fp32 when
+    both are float32 (walkers are complex128 either way)."""
     wu, wd = (walkers if isinstance(walkers, tuple) else (walkers, None))
     nspin = 1 if wd is None else 2
     nw, n, o = wu.shape
@@ -179,12 +243,16 @@ def trotter(walkers, vhs_planes, exp_h1_half_pad, sqrt_dt, n_terms, out=None):
     _check(wu, c128, (nw, n, o), "walkers")
     if wd is not None:
         _check(wd, c128, (nw, n, o), "walkers_dn", dev)
-    _check(vhs_planes, f64, (2, nw, n, ldv), "vhs", dev)
-    _check(exp_h1_half_pad, f64, (n, ldv), "exp_h1_half", dev)
+    rt = vhs_planes.dtype
+    if rt not in (f64, f32):
+        raise TypeError("vhs must be float64 or float32")
+    _check(vhs_planes, rt, (2, nw, n, ldv), "vhs", dev)
+    _check(exp_h1_half_pad, rt, (n, ldv), "exp_h1_half", dev)
     if out is None:
         out = (torch.empty_like(wu), torch.empty_like(wd)) if nspin == 2 else torch.empty_like(wu)
     ou, od = (out if isinstance(out, tuple) else (out, None))
-    _call("afq_trotter", wu, nw, n, o, nspin, ldv, int(n_terms), float(sqrt_dt),
+    _call("afq_trotter" if rt == f64 else "afq_trotter_f32", wu, nw, n, o, nspin, ldv,
+          int(n_terms), float(sqrt_dt),
               exp_h1_half_pad.data_ptr(), vhs_planes.data_ptr(), wu.data_ptr(), _ptr(wd),
               ou.data_ptr(), _ptr(od), _stream(wu))
     return out

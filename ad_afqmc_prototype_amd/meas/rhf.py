@@ -31,17 +31,25 @@ class rhf_meas_ctx:
     rot_h1: torch.Tensor
     rot_chol: torch.Tensor
     rot_chol_flat: torch.Tensor
+    precision: str = "fp64"
     _cache: dict = field(default_factory=dict, compare=False, repr=False)
 
-    def fb_operand(self, nspin: int) -> torch.Tensor:
+    def fb_operand(self, nspin: int):
+        """B operand of the force-bias GEMM: float64 (naux, k); float32 copy in
+        fp32 mode; (hi, lo) bf16 planes (naux, k padded to 8) in bf16x3 mode."""
         key = ("fb", nspin)
         if key not in self._cache:
             if nspin == 1:
-                self._cache[key] = self.rot_chol_flat
+                op = self.rot_chol_flat
             else:
                 g = self.rot_chol.shape[1]
-                self._cache[key] = torch.cat([self.rot_chol[0].reshape(g, -1),
-                                              self.rot_chol[1].reshape(g, -1)], dim=1).contiguous()
+                op = torch.cat([self.rot_chol[0].reshape(g, -1),
+                                self.rot_chol[1].reshape(g, -1)], dim=1).contiguous()
+            if self.precision == "fp32":
+                op = kernels.convert_f32(op)
+            elif self.precision == "bf16x3":
+                op = kernels.split_bf16(op)
+            self._cache[key] = op
         return self._cache[key]
 
     def energy_operands(self, nspin: int):
@@ -56,7 +64,7 @@ class rhf_meas_ctx:
         return self._cache[key]
 
 
-def build_meas_ctx(ham_data: ham_chol, trial_data) -> rhf_meas_ctx:
+def build_meas_ctx(ham_data: ham_chol, trial_data, precision: str = "fp64") -> rhf_meas_ctx:
     """meas/rhf.py:141-150 (per spin sector for a UHF trial)."""
     if ham_data.basis != "restricted":
         raise ValueError("RHF meas_ops currently assumes ham_chol.basis == 'restricted'.")
@@ -70,7 +78,7 @@ def build_meas_ctx(ham_data: ham_chol, trial_data) -> rhf_meas_ctx:
         rh = torch.stack(rot_h1)
         rc = torch.stack(rot_chol)
     flat = rc[0].reshape(rc.shape[1], -1)
-    return rhf_meas_ctx(rot_h1=rh.contiguous(), rot_chol=rc, rot_chol_flat=flat)
+    return rhf_meas_ctx(rot_h1=rh.contiguous(), rot_chol=rc, rot_chol_flat=flat, precision=precision)
 
 
 def _nspin(walkers) -> int:
@@ -83,7 +91,22 @@ def force_bias_planes(walkers, meas_ctx: rhf_meas_ctx, trial_data, greens=None) 
     if greens is None:
         greens = trial_rhf.greens(walkers, trial_data)
     alpha = 2.0 if nspin == 1 else 1.0
-    return kernels.force_bias(greens.greens, meas_ctx.fb_operand(nspin), alpha)
+    op = meas_ctx.fb_operand(nspin)
+    if meas_ctx.precision == "fp64":
+        return kernels.force_bias(greens.greens, op, alpha)
+    g = greens.greens
+    _, nw, k = g.shape
+    naux = meas_ctx.rot_chol.shape[1]
+    fb = torch.empty((2, nw, naux), dtype=torch.float64, device=g.device)
+    flat = g.reshape(2 * nw, k)
+    if meas_ctx.precision == "fp32":
+        kernels.sgemm_nt(kernels.convert_f32(flat), op, fb.view(2 * nw, naux), alpha=alpha,
+                         tag="force_bias")
+    else:
+        gh, gl = kernels.split_bf16(flat)
+        kernels.gemm_bf16x3_nt(gh, gl, op[0], op[1], fb.view(2 * nw, naux), alpha=alpha,
+                               tag="force_bias")
+    return fb
 
 
 def force_bias_kernel_r(walkers, _ham, meas_ctx: rhf_meas_ctx, trial_data, greens=None):
@@ -118,15 +141,20 @@ def energy_kernel_u(walkers, ham_data, meas_ctx, trial_data, greens=None):
     return _energy(walkers, ham_data, meas_ctx, trial_data, greens)
 
 
-def make_rhf_meas_ops(sys: system) -> meas_ops:
-    """meas/rhf.py:153-176; also serves UHF trials (per-spin integrals)."""
+def make_rhf_meas_ops(sys: system, precision: str = "fp64") -> meas_ops:
+    """meas/rhf.py:153-176; also serves UHF trials (per-spin integrals).
+    precision selects the force-bias contraction: "fp64", "fp32" or "bf16x3"."""
+    from functools import partial
+    if precision not in ("fp64", "fp32", "bf16x3"):
+        raise ValueError(f"unknown precision {precision!r}")
+    build_meas_ctx_p = partial(build_meas_ctx, precision=precision)
     wk = sys.walker_kind.lower()
     if wk == "restricted":
-        return meas_ops(overlap=trial_rhf.overlap_r, build_meas_ctx=build_meas_ctx,
+        return meas_ops(overlap=trial_rhf.overlap_r, build_meas_ctx=build_meas_ctx_p,
                         kernels={k_force_bias: force_bias_kernel_r, k_energy: energy_kernel_r},
                         log_overlap=trial_rhf.log_overlap, greens=trial_rhf.greens)
     if wk == "unrestricted":
-        return meas_ops(overlap=trial_rhf.overlap_u, build_meas_ctx=build_meas_ctx,
+        return meas_ops(overlap=trial_rhf.overlap_u, build_meas_ctx=build_meas_ctx_p,
                         kernels={k_force_bias: force_bias_kernel_u, k_energy: energy_kernel_u},
                         log_overlap=trial_rhf.log_overlap, greens=trial_rhf.greens)
     if wk == "generalized":

@@ -6,6 +6,9 @@ Trotter kernel applies exp(-dt/2 h1eff) . Taylor_n(i sqrt_dt VHS) . exp(-dt/2 h1
 (prop/chol_afqmc_ops.py:173-196, prop/utils.py:20-31).  n_chunks bounds the
 VHS workspace (2 * chunk * n * n doubles) exactly as the reference's chunked
 vmap bounds memory; results do not depend on it.
+
+precision: "fp64" (plain step), "fp32" (VHS GEMM and Trotter step in fp32) or
+"bf16x3" (VHS GEMM in split bf16 with fp32 accumulation, Trotter step in fp32).
 """
 from __future__ import annotations
 
@@ -26,6 +29,7 @@ class chol_afqmc_ctx:
     mf_shifts: torch.Tensor         # (n_fields,) complex128, purely imaginary
     h0_prop: complex
     exp_h1_half_pad: torch.Tensor   # (n, ldv) device layout of exp_h1_half
+    exp_h1_half_pad32: torch.Tensor = None  # float32 copy for the mixed step
 
 
 class chol_afqmc_ops(NamedTuple):
@@ -33,6 +37,7 @@ class chol_afqmc_ops(NamedTuple):
     build_prop_ctx: Callable[[Any, float], chol_afqmc_ctx]
     apply_trotter: Callable[..., Any]       # (walkers, fields (nw, G), ctx, n_terms) -> walkers
     apply_trotter_planes: Callable[..., Any]  # (walkers, xbar planes (2, nw, G), ctx, n_terms)
+    precision: str = "fp64"
 
 
 def _as_total_rdm1_restricted(dm: torch.Tensor) -> torch.Tensor:
@@ -63,7 +68,8 @@ def _build_prop_ctx(ham_data: ham_chol, trial_rdm1, dt: float) -> chol_afqmc_ctx
     pad = torch.zeros((n, ldv), dtype=torch.float64, device=chol.device)
     pad[:, :n] = exp_h1_half
     return chol_afqmc_ctx(dt=float(dt), sqrt_dt=float(dt) ** 0.5, exp_h1_half=exp_h1_half,
-                          mf_shifts=mf.contiguous(), h0_prop=h0_prop, exp_h1_half_pad=pad)
+                          mf_shifts=mf.contiguous(), h0_prop=h0_prop, exp_h1_half_pad=pad,
+                          exp_h1_half_pad32=pad.to(torch.float32))
 
 
 def chol_transposed(chol: torch.Tensor) -> torch.Tensor:
@@ -82,8 +88,14 @@ def _chunks(nw: int, n_chunks: int):
     return [(i * cs, (i + 1) * cs) for i in range(n_chunks)]
 
 
-def make_chol_afqmc_ops(ham_data: ham_chol, walker_kind: str) -> chol_afqmc_ops:
+PRECISIONS = ("fp64", "fp32", "bf16x3")
+
+
+def make_chol_afqmc_ops(ham_data: ham_chol, walker_kind: str,
+                        precision: str = "fp64") -> chol_afqmc_ops:
     """prop/chol_afqmc_ops.py:215-262 (restricted basis, restricted/unrestricted walkers)."""
+    if precision not in PRECISIONS:
+        raise ValueError(f"unknown precision {precision!r}; expected one of {PRECISIONS}")
     walker_kind = walker_kind.lower()
     nf = int(ham_data.chol.shape[0])
     n = int(ham_data.chol.shape[1])
@@ -95,9 +107,33 @@ def make_chol_afqmc_ops(ham_data: ham_chol, walker_kind: str) -> chol_afqmc_ops:
         raise ValueError(f"unknown walker_kind: {walker_kind}")
 
     chol_t = chol_transposed(ham_data.chol)
+    ldv = kernels.ldv_for(n)
+    if precision == "fp32":
+        chol_t32 = kernels.convert_f32(chol_t)
+        del chol_t
+    elif precision == "bf16x3":
+        chol_hi, chol_lo = kernels.split_bf16(chol_t)
+        del chol_t
 
     def n_fields_() -> int:
         return nf
+
+    def make_vhs(xbar: torch.Tensor) -> torch.Tensor:
+        """VHS planes (2, nw, n, ldv) in the step's precision."""
+        if precision == "fp64":
+            return kernels.vhs(xbar, chol_t, n)
+        nwc = xbar.shape[1]
+        out = torch.empty((2, nwc, n, ldv), dtype=torch.float32, device=xbar.device)
+        flat = xbar.reshape(2 * nwc, nf)
+        if precision == "fp32":
+            kernels.sgemm_nt(kernels.convert_f32(flat), chol_t32, out.view(2 * nwc, n * ldv), tag="vhs")
+        else:
+            xh, xl = kernels.split_bf16(flat)
+            kernels.gemm_bf16x3_nt(xh, xl, chol_hi, chol_lo, out.view(2 * nwc, n * ldv), tag="vhs")
+        return out
+
+    def e_half(ctx: chol_afqmc_ctx) -> torch.Tensor:
+        return ctx.exp_h1_half_pad if precision == "fp64" else ctx.exp_h1_half_pad32
 
     def build_prop_ctx(trial_rdm1, dt: float) -> chol_afqmc_ctx:
         return _build_prop_ctx(ham_data, trial_rdm1, dt)
@@ -108,14 +144,13 @@ def make_chol_afqmc_ops(ham_data: ham_chol, walker_kind: str) -> chol_afqmc_ops:
         w0 = w[0] if unr else w
         nw = w0.shape[0]
         if n_chunks == 1:
-            v = kernels.vhs(xbar, chol_t, n)
-            return kernels.trotter(w, v, ctx.exp_h1_half_pad, ctx.sqrt_dt, n_terms)
+            return kernels.trotter(w, make_vhs(xbar), e_half(ctx), ctx.sqrt_dt, n_terms)
         out = (torch.empty_like(w[0]), torch.empty_like(w[1])) if unr else torch.empty_like(w0)
         for a, b in _chunks(nw, n_chunks):
-            v = kernels.vhs(xbar[:, a:b].contiguous(), chol_t, n)
+            v = make_vhs(xbar[:, a:b].contiguous())
             wc = (w[0][a:b], w[1][a:b]) if unr else w0[a:b]
             oc = (out[0][a:b], out[1][a:b]) if unr else out[a:b]
-            kernels.trotter(wc, v, ctx.exp_h1_half_pad, ctx.sqrt_dt, n_terms, out=oc)
+            kernels.trotter(wc, v, e_half(ctx), ctx.sqrt_dt, n_terms, out=oc)
         return out
 
     def apply_trotter(w, fields, ctx: chol_afqmc_ctx, n_terms: int, n_chunks: int = 1):
@@ -127,4 +162,4 @@ def make_chol_afqmc_ops(ham_data: ham_chol, walker_kind: str) -> chol_afqmc_ops:
             xbar = torch.stack([f, torch.zeros_like(f)]).to(torch.float64).contiguous()
         return apply_trotter_planes(w, xbar, ctx, n_terms, n_chunks)
 
-    return chol_afqmc_ops(n_fields_, build_prop_ctx, apply_trotter, apply_trotter_planes)
+    return chol_afqmc_ops(n_fields_, build_prop_ctx, apply_trotter, apply_trotter_planes, precision)

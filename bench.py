@@ -21,7 +21,7 @@ import time
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-PEAK_FP64_TFLOPS = 78.6
+PEAK_TFLOPS = {"fp64": 78.6, "fp32": 157.3, "bf16": 2500.0}
 PEAK_HBM_GBS = 8000.0
 
 
@@ -38,7 +38,7 @@ def flops_per_walker_step(n, o, g, nspin, rhf_trial):
                 total=vhs + fb + taylor + one_body + greens)
 
 
-def build(workload: str, device):
+def build(workload: str, device, precision: str = "fp64"):
     import torch
 
     from ad_afqmc_prototype_amd import synthetic as syn
@@ -60,8 +60,8 @@ def build(workload: str, device):
         trial, t_ops = uhf_trial(s.mo_up, s.mo_dn), make_uhf_trial_ops(sys_)
     else:
         trial, t_ops = rhf_trial(s.mo_up), make_rhf_trial_ops(sys_)
-    m_ops = make_rhf_meas_ops(sys_)
-    p_ops = make_chol_afqmc_ops(ham, kind)
+    m_ops = make_rhf_meas_ops(sys_, precision=precision)
+    p_ops = make_chol_afqmc_ops(ham, kind, precision=precision)
     params = afqmc_params(dt=cfg["dt"], n_walkers=cfg["n_walkers"], n_prop_steps=cfg["sr_every"],
                           n_ortho=cfg["qr_every"])
     p_ctx = p_ops.build_prop_ctx(t_ops.get_rdm1(trial), params.dt)
@@ -73,9 +73,10 @@ def build(workload: str, device):
                 p_ops=p_ops, params=params, p_ctx=p_ctx, m_ctx=m_ctx, state=state)
 
 
-def run_steps(ctx, state, fields, t0, k, cadence=True):
+def run_steps(ctx, state, fields, t0, k, cadence=True, energies=None):
     """k steps starting at global step index t0 with the QR / measurement +
-    reconfiguration cadence of the workload."""
+    reconfiguration cadence of the workload (block energies appended to
+    ``energies`` as device scalars)."""
     from ad_afqmc_prototype_amd import synthetic as syn
     from ad_afqmc_prototype_amd.prop.afqmc import afqmc_step
     from ad_afqmc_prototype_amd.prop.blocks import measure, orthonormalize_state, reconfigure
@@ -91,8 +92,10 @@ def run_steps(ctx, state, fields, t0, k, cadence=True):
         if (t + 1) % cfg["sr_every"] == 0:
             state = orthonormalize_state(state, sys=ctx["sys"], trial_data=ctx["trial"],
                                          meas_ops_=ctx["m_ops"])
-            state, _, _ = measure(state, params=ctx["params"], ham_data=ctx["ham"],
-                                  trial_data=ctx["trial"], meas_ops=ctx["m_ops"], meas_ctx=ctx["m_ctx"])
+            state, sc, _ = measure(state, params=ctx["params"], ham_data=ctx["ham"],
+                                   trial_data=ctx["trial"], meas_ops=ctx["m_ops"], meas_ctx=ctx["m_ctx"])
+            if energies is not None:
+                energies.append(sc["energy"])
             state, _ = reconfigure(state, sys=ctx["sys"], trial_data=ctx["trial"],
                                    meas_ops_=ctx["m_ops"],
                                    zeta=syn.zeta_for_event(cfg["seed"], (t + 1) // cfg["sr_every"]))
@@ -139,13 +142,75 @@ def cpu_baseline(workload: str, budget_s: float = 20.0):
                       f"log-overlaps, {el:.1f} s, {threads} BLAS threads)"}
 
 
+def kernel_table(ktimes, n, o, g, nw, nspin, rhf):
+    """Algorithmic flops per launch and peak of every timed entry point."""
+    fl = flops_per_walker_step(n, o, g, nspin, rhf)
+    taylor = (fl["taylor"] + fl["one_body"]) * nw
+    spec = {
+        "afq_vhs": (fl["vhs"] * nw, "fp64"),
+        "afq_sgemm_nt:vhs": (fl["vhs"] * nw, "fp32"),
+        "afq_gemm_bf16x3_nt:vhs": (fl["vhs"] * nw, "bf16x3"),
+        "afq_force_bias": (fl["force_bias"] * nw, "fp64"),
+        "afq_sgemm_nt:force_bias": (fl["force_bias"] * nw, "fp32"),
+        "afq_gemm_bf16x3_nt:force_bias": (fl["force_bias"] * nw, "bf16x3"),
+        "afq_trotter": (taylor, "fp64"),
+        "afq_trotter_f32": (taylor, "fp32"),
+        "afq_overlap_greens": (fl["greens"] * nw, "fp64"),
+        "afq_energy": (4.0 * g * o * o * n * nspin * nw, "fp64"),
+    }
+    out = {}
+    for name, t in ktimes.items():
+        row = {"count": t["count"], "avg_ms": round(t["avg_ms"], 4), "total_ms": round(t["total_ms"], 3)}
+        if name in spec:
+            flops, prec = spec[name]
+            # bf16x3 issues three bf16 MFMA products per useful product
+            peak = PEAK_TFLOPS["bf16"] / 3.0 if prec == "bf16x3" else PEAK_TFLOPS[prec]
+            ach = flops / (t["avg_ms"] * 1e-3) / 1e12
+            row.update({"tflops": round(ach, 2), "peak_tflops": round(peak, 1), "frac": round(ach / peak, 4),
+                        "precision": prec, "gflop_per_launch": round(flops / 1e9, 3)})
+        out[name] = row
+    return out
+
+
+def timed_run(args, precision, device):
+    import torch
+
+    from ad_afqmc_prototype_amd import kernels
+
+    ctx = build(args.workload, device, precision)
+    cfg = ctx["cfg"]
+    nw, g = cfg["n_walkers"], cfg["naux"]
+    gen = torch.Generator(device=device)
+    gen.manual_seed(cfg["seed"])
+    n_fields = min(args.warmup + args.steps, 40)
+    fields = [torch.randn((nw, g), dtype=torch.float64, device=device, generator=gen)
+              for _ in range(n_fields)]
+    state = run_steps(ctx, ctx["state"], fields, 0, args.warmup, cadence=not args.no_cadence)
+    torch.cuda.synchronize()
+    energies = []
+    with kernels.timing() as tm:
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        state = run_steps(ctx, state, fields, args.warmup, args.steps, cadence=not args.no_cadence,
+                          energies=energies)
+        torch.cuda.synchronize()
+        elapsed = time.perf_counter() - t0
+    ktimes = tm.summary()
+    e_block = [float(e.item()) for e in energies]
+    del ctx, state, fields
+    torch.cuda.empty_cache()
+    return elapsed, ktimes, e_block
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="n200_o32_g1000_w2048_rhf")
-    ap.add_argument("--precision", default="fp64", choices=["fp64"])
+    ap.add_argument("--precision", default="bf16x3", choices=["fp64", "fp32", "bf16x3"])
+    ap.add_argument("--no-fp64-compare", action="store_true",
+                    help="skip the fp64 run that the mixed throughput is compared with")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-cadence", action="store_true",
                     help="time afqmc_step alone (no QR / measurement / reconfiguration)")
@@ -159,35 +224,21 @@ def main():
 
     import torch
 
-    from ad_afqmc_prototype_amd import kernels
+    from ad_afqmc_prototype_amd import synthetic as syn
 
     device = torch.device("cuda", 0)
-    ctx = build(args.workload, device)
-    cfg = ctx["cfg"]
+    cfg = syn.WORKLOADS[args.workload]
     n, o, g, nw = cfg["norb"], cfg["nocc"], cfg["naux"], cfg["n_walkers"]
     nspin = 2 if cfg["walker_kind"] == "unrestricted" else 1
 
-    gen = torch.Generator(device=device)
-    gen.manual_seed(cfg["seed"])
-    n_fields = min(args.warmup + args.steps, 40)
-    fields = [torch.randn((nw, g), dtype=torch.float64, device=device, generator=gen)
-              for _ in range(n_fields)]
-
-    state = run_steps(ctx, ctx["state"], fields, 0, args.warmup, cadence=not args.no_cadence)
-    torch.cuda.synchronize()
-    with kernels.timing() as tm:
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        state = run_steps(ctx, state, fields, args.warmup, args.steps, cadence=not args.no_cadence)
-        torch.cuda.synchronize()
-        elapsed = time.perf_counter() - t0
-    ktimes = tm.summary()
-
-    flops = flops_per_walker_step(n, o, g, nspin, not cfg["uhf"])
-    vhs = ktimes["afq_vhs"]
-    vhs_flops = flops["vhs"] * nw
-    achieved = vhs_flops / (vhs["avg_ms"] * 1e-3) / 1e12
+    elapsed, ktimes, e_block = timed_run(args, args.precision, device)
+    table = kernel_table(ktimes, n, o, g, nw, nspin, not cfg["uhf"])
+    dominant = max((k for k in table if "tflops" in table[k]), key=lambda k: table[k]["total_ms"])
+    dom = table[dominant]
     value = nw * args.steps / elapsed
+    flops = flops_per_walker_step(n, o, g, nspin, not cfg["uhf"])
+    dtype = {"fp64": "f64", "fp32": "f32 contractions + f32 Trotter, f64 LU/QR/weights/energy",
+             "bf16x3": "bf16x3 contractions + f32 Trotter, f64 LU/QR/weights/energy"}[args.precision]
     out = {
         "metric": "walker-steps/s at N=200,naux=1000: mixed-prec vs FP64 GPU, |dE|<0.1 mHa",
         "value": value,
@@ -199,20 +250,26 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": dtype,
         "data": "synthetic",
         "config": {"workload": args.workload, "norb": n, "nocc": o, "naux": g, "n_walkers": nw,
-                   "walker_kind": cfg["walker_kind"], "dt": cfg["dt"],
+                   "walker_kind": cfg["walker_kind"], "dt": cfg["dt"], "precision": args.precision,
                    "qr_every": None if args.no_cadence else cfg["qr_every"],
-                   "measure_sr_every": None if args.no_cadence else cfg["sr_every"],
-                   "precision": args.precision},
-        "roofline": {"bound": "mfma", "kernel": "afq_vhs (fp64 MFMA GEMM)",
-                     "achieved": achieved, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / PEAK_FP64_TFLOPS, "traffic": None},
-        "step_tflops": flops["total"] * nw * args.steps / elapsed / 1e12,
-        "kernels_ms": {k: round(v["avg_ms"], 4) for k, v in ktimes.items()},
-        "kernels_count": {k: v["count"] for k, v in ktimes.items()},
+                   "measure_sr_every": None if args.no_cadence else cfg["sr_every"]},
+        "roofline": {"bound": "mfma", "kernel": dominant, "achieved": dom["tflops"],
+                     "peak": dom["peak_tflops"], "unit": "TFLOP/s", "frac": dom["frac"],
+                     "traffic": None},
+        "step_tflops_algorithmic": round(flops["total"] * nw * args.steps / elapsed / 1e12, 2),
+        "kernels": table,
+        "block_energies": e_block,
     }
+    if args.precision != "fp64" and not args.no_fp64_compare:
+        el64, kt64, e64 = timed_run(args, "fp64", device)
+        out["fp64_value"] = nw * args.steps / el64
+        out["fp64_ms_per_step"] = 1e3 * el64 / args.steps
+        out["speedup_vs_fp64"] = out["value"] / out["fp64_value"]
+        out["fp64_kernels"] = kernel_table(kt64, n, o, g, nw, nspin, not cfg["uhf"])
+        out["block_energy_abs_diff_mHa"] = [1e3 * abs(a - b) for a, b in zip(e_block, e64)]
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(args.workload)
     print(json.dumps(out), flush=True)

@@ -40,6 +40,44 @@ int afq_dgemm_nt(int M, int N, int K, double alpha, const double* A, long long l
                  void* stream);
 
 /*
+ * fp32 GEMM  C = alpha * A * B^T + beta * C  on v_mfma_f32_16x16x4_f32, same
+ * layout as afq_dgemm_nt; C is float32, or float64 when c_fp64 != 0.
+ * Force-bias / VHS contractions of the mixed step in fp32 mode (meas/rhf.py:33,
+ * prop/chol_afqmc_ops.py:72-73 in fp32).
+ */
+int afq_sgemm_nt(int M, int N, int K, float alpha, const float* A, long long lda,
+                 const float* B, long long ldb, float beta, void* C, long long ldc,
+                 int c_fp64, void* stream);
+
+/*
+ * Split-bf16 GEMM  C = alpha * (Ahi Bhi^T + Ahi Blo^T + Alo Bhi^T)  with fp32
+ * accumulation (3 x v_mfma_f32_16x16x32_bf16 per step).  A (M x K) and B
+ * (N x K) are given as bf16 hi/lo planes (uint16 bit patterns) from
+ * afq_split_bf16; K, lda, ldb multiples of 8.  C is float32, or float64 when
+ * c_fp64 != 0.  Force-bias / VHS contractions of the mixed step in bf16x3
+ * mode (meas/rhf.py:33, prop/chol_afqmc_ops.py:72-73).
+ */
+int afq_gemm_bf16x3_nt(int M, int N, int K, float alpha, const unsigned short* A_hi,
+                       const unsigned short* A_lo, long long lda, const unsigned short* B_hi,
+                       const unsigned short* B_lo, long long ldb, void* C, long long ldc,
+                       int c_fp64, void* stream);
+
+/*
+ * x (rows x cols, float64, row pitch ld_src) -> hi = bf16(x), lo = bf16(x - hi)
+ * (rows x ld_dst, zero in columns >= cols).  Operand preparation for
+ * afq_gemm_bf16x3_nt; no counterpart in the reference (it computes in fp64).
+ */
+int afq_split_bf16(int rows, int cols, const double* src, long long ld_src,
+                   unsigned short* hi, unsigned short* lo, long long ld_dst, void* stream);
+
+/*
+ * float64 (rows x cols, pitch ld_src) -> float32 (rows x ld_dst, zero padded).
+ * Operand preparation for the fp32 kernels; no counterpart in the reference.
+ */
+int afq_convert_f32(int rows, int cols, const double* src, long long ld_src, float* dst,
+                    long long ld_dst, void* stream);
+
+/*
  * Overlap matrix m = C^T w, LU with partial pivoting, log det(m), and the
  * half-rotated Green's function G = (w m^-1)^T, per walker and spin.
  * Replaces _half_green_from_overlap_matrix (meas/rhf.py:15-19), overlap_r
@@ -96,6 +134,16 @@ int afq_trotter(int nw, int n, int o, int nspin, int ldv, int n_terms, double sq
                 const double* exp_h1_half, const double* vhs,
                 const double* win_up, const double* win_dn,
                 double* wout_up, double* wout_dn, void* stream);
+
+/*
+ * afq_trotter with E, VHS and the Taylor recursion in fp32 (walkers still read
+ * and written as complex128): the Trotter step of the mixed step.
+ * exp_h1_half (n, ldv) float32; vhs planes (2, nw, n, ldv) float32.
+ */
+int afq_trotter_f32(int nw, int n, int o, int nspin, int ldv, int n_terms, double sqrt_dt,
+                    const float* exp_h1_half, const float* vhs,
+                    const double* win_up, const double* win_dn,
+                    double* wout_up, double* wout_dn, void* stream);
 
 /*
  * Phaseless importance, cos-phase cut, floor/cap and population-control shift

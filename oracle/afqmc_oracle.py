@@ -321,17 +321,24 @@ def field_shift_terms(fields: np.ndarray, fb: np.ndarray, pctx: prop_ctx):
 
 def phaseless_update(ov_new, ov_old, shift_term, fb_term, weights, pop_shift, e_estimate,
                      pctx: prop_ctx, prm: params, overlap_mode: str = "raw"):
-    """prop/afqmc.py:116-138.  Returns (weights_new, pop_shift_new)."""
-    if overlap_mode == "log":
-        ratio = np.exp(ov_new - ov_old)
-    else:
-        ratio = ov_new / ov_old
+    """prop/afqmc.py:116-138.  Returns (weights_new, pop_shift_new).
+
+    overlap_mode="raw" is the reference formula verbatim.  In "log" mode the
+    importance magnitude is exp(Re(exponent + dlog)) and the phase
+    Im(-sqrt_dt shift_term + dlog): the same numbers wherever the raw
+    exp(exponent) * ratio is finite, without its overflow."""
     with np.errstate(all="ignore"):
         exponent = (-pctx.sqrt_dt * shift_term + fb_term
                     + pctx.dt * (pop_shift + pctx.h0_prop))
-        imp = np.exp(exponent) * ratio
-        theta = np.angle(np.exp(-pctx.sqrt_dt * shift_term) * ratio)
-        imp_ph = np.abs(imp) * np.cos(theta)
+        if overlap_mode == "log":
+            dlog = ov_new - ov_old
+            mag = np.exp(np.real(exponent + dlog))
+            theta = np.imag(-pctx.sqrt_dt * shift_term + dlog)
+        else:
+            ratio = ov_new / ov_old
+            mag = np.abs(np.exp(exponent) * ratio)
+            theta = np.angle(np.exp(-pctx.sqrt_dt * shift_term) * ratio)
+        imp_ph = mag * np.cos(theta)
         imp_ph = np.where(imp_ph < prm.weight_floor, 0.0, imp_ph)
         imp_ph = np.where(imp_ph > prm.weight_cap, 0.0, imp_ph)
         w_new = weights * imp_ph

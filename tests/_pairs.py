@@ -25,6 +25,13 @@ def rel(a, b) -> float:
     return float(np.linalg.norm((a - b).ravel()) / max(np.linalg.norm(b.ravel()), 1e-300))
 
 
+def assert_rel(a, b, tol, what=""):
+    """Norm-wise relative error check with a one-line failure message."""
+    err = rel(a, b)
+    if not err < tol:
+        raise AssertionError(f"{what} relative error {err:.3e} exceeds {tol:.1e}")
+
+
 def make_pair(seed, norb, nocc, naux, *, uhf=False, kind="restricted", dt=0.01,
               chol_scale=1.0, device=None) -> pair:
     s = syn.make_system(seed, norb, nocc, naux, uhf=uhf, chol_scale=chol_scale)

@@ -5,7 +5,7 @@ import pytest
 
 from oracle import afqmc_oracle as orc
 
-from ._pairs import make_pair, rel, random_walkers, to_dev, to_np
+from ._pairs import assert_rel, make_pair, rel, random_walkers, to_dev, to_np
 
 pytestmark = pytest.mark.gpu
 TOL = 1e-10
@@ -36,7 +36,8 @@ def test_dgemm_matches_torch_fp64(gpu_device):
         C = C0.to(gpu_device)
         kernels.dgemm_nt(A.to(gpu_device), B.to(gpu_device), C, alpha=0.5, beta=-1.5)
         ref = 0.5 * (A @ B.T) - 1.5 * C0
-        assert rel(C.cpu().numpy(), ref.numpy()) < 1e-13, (M, N, K)
+        err = rel(C.cpu().numpy(), ref.numpy())
+        assert err < 1e-13, f"M={M} N={N} K={K} rel err {err:.3e}"
 
 
 @pytest.mark.parametrize("seed,n,o,naux,uhf,kind", SHAPES)
@@ -57,7 +58,7 @@ def test_greens_and_log_overlap(gpu_device, seed, n, o, naux, uhf, kind):
     ws = (w,) if kind == "restricted" else w
     cs = (p.s.mo_up,) if kind == "restricted" else (p.s.mo_up, p.s.mo_dn)
     for s_ in range(nspin):
-        assert rel(Gc[:, s_], orc.half_green(ws[s_], cs[s_])) < TOL
+        assert_rel(Gc[:, s_], orc.half_green(ws[s_], cs[s_]), TOL)
 
 
 @pytest.mark.parametrize("seed,n,o,naux,uhf,kind", SHAPES)
@@ -67,7 +68,7 @@ def test_force_bias(gpu_device, seed, n, o, naux, uhf, kind):
     rng = np.random.default_rng(seed + 11)
     w = random_walkers(rng, kind, 24, n, o, p.s.mo_up, p.s.mo_dn)
     fb = p.g["m_ops"].require_kernel(k_force_bias)(to_dev(w, gpu_device), p.g["ham"], p.g["m_ctx"], p.g["trial"])
-    assert rel(fb.cpu().numpy(), orc.force_bias(w, p.sd.trial, p.sd.mctx, kind)) < TOL
+    assert_rel(fb.cpu().numpy(), orc.force_bias(w, p.sd.trial, p.sd.mctx, kind), TOL)
 
 
 @pytest.mark.parametrize("seed,n,o,naux,uhf,kind", SHAPES)
@@ -84,9 +85,9 @@ def test_field_shifts(gpu_device, seed, n, o, naux, uhf, kind):
                                         p.g["p_ctx"].sqrt_dt)
     sh, st_ref, ft_ref = orc.field_shift_terms(x, fb, p.sd.pctx)
     xb = xbar.cpu().numpy()
-    assert rel(xb[0] + 1j * xb[1], sh) < 1e-13
-    assert rel(st.cpu().numpy(), st_ref) < 1e-12
-    assert rel(ft.cpu().numpy(), ft_ref) < 1e-12
+    assert_rel(xb[0] + 1j * xb[1], sh, 1e-13)
+    assert_rel(st.cpu().numpy(), st_ref, 1e-12)
+    assert_rel(ft.cpu().numpy(), ft_ref, 1e-12)
 
 
 @pytest.mark.parametrize("seed,n,o,naux,uhf,kind", SHAPES)
@@ -101,13 +102,14 @@ def test_vhs_and_trotter(gpu_device, seed, n, o, naux, uhf, kind):
     x = 0.3 * (rng.standard_normal((nw, naux)) + 0.5j * rng.standard_normal((nw, naux)))
     xp = torch.tensor(np.stack([x.real, x.imag]), device=gpu_device)
     v = kernels.vhs(xp, chol_transposed(p.g["ham"].chol), n).cpu().numpy()[..., :n]
-    assert rel(v[0] + 1j * v[1], orc.make_vhs(p.sd.chol, x)) < TOL
+    assert_rel(v[0] + 1j * v[1], orc.make_vhs(p.sd.chol, x), TOL)
     out = p.g["p_ops"].apply_trotter_planes(to_dev(w, gpu_device), xp, p.g["p_ctx"], 6)
     ref = orc.apply_trotter(w, x, p.sd.pctx, p.sd.chol, 6, kind)
     if kind == "restricted":
-        assert rel(to_np(out), ref) < TOL
+        assert_rel(to_np(out), ref, TOL)
     else:
-        assert rel(to_np(out)[0], ref[0]) < TOL and rel(to_np(out)[1], ref[1]) < TOL
+        assert_rel(to_np(out)[0], ref[0], TOL)
+        assert_rel(to_np(out)[1], ref[1], TOL)
     # n_chunks does not change the result
     out2 = p.g["p_ops"].apply_trotter_planes(to_dev(w, gpu_device), xp, p.g["p_ctx"], 6, n_chunks=2)
     a, b = (to_np(out), to_np(out2))
@@ -131,7 +133,7 @@ def test_trotter_zero_terms_and_identity(gpu_device):
     v = torch.zeros((2, nw, n, ldv), dtype=torch.float64, device=gpu_device)
     wd = torch.tensor(w, device=gpu_device)
     out = kernels.trotter(wd, v, torch.tensor(ep, device=gpu_device), 0.1, 0).cpu().numpy()
-    assert rel(out, e @ (e @ w)) < 1e-13
+    assert_rel(out, e @ (e @ w), 1e-13)
     ep_i = np.zeros((n, ldv)); ep_i[:, :n] = np.eye(n)
     out = kernels.trotter(wd, v, torch.tensor(ep_i, device=gpu_device), 0.1, 6).cpu().numpy()
     np.testing.assert_array_equal(out, w)
@@ -146,7 +148,7 @@ def test_energy(gpu_device, seed, n, o, naux, uhf, kind):
     w = random_walkers(rng, kind, nw, n, o, p.s.mo_up, p.s.mo_dn)
     e = p.g["m_ops"].require_kernel(k_energy)(to_dev(w, gpu_device), p.g["ham"], p.g["m_ctx"], p.g["trial"])
     e_ref = orc.energy(w, p.sd.h0, p.sd.trial, p.sd.mctx, kind)
-    assert rel(e.cpu().numpy(), e_ref) < TOL
+    assert_rel(e.cpu().numpy(), e_ref, TOL)
 
 
 @pytest.mark.parametrize("n,o", [(24, 6), (120, 20), (200, 32), (64, 64)])
@@ -158,7 +160,7 @@ def test_orthonormalize(gpu_device, n, o):
     w = rng.standard_normal((nw, n, o)) + 1j * rng.standard_normal((nw, n, o))
     q = kernels.orthonormalize(torch.tensor(w, device=gpu_device)).cpu().numpy()
     q_ref = orc.orthonormalize(w, "restricted")
-    assert rel(q, q_ref) < 1e-12
+    assert_rel(q, q_ref, 1e-12)
 
 
 def test_comb_indices_bit_exact(gpu_device):

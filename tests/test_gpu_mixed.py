@@ -1,0 +1,133 @@
+"""Mixed-precision kernels (fp32 and split-bf16 contractions, fp32 Trotter
+step) against fp64.  Tolerances are stated per test: fp32 ~ 1e-6 relative,
+bf16x3 (hi/lo split, fp32 accumulation) ~ 1e-4 relative."""
+import numpy as np
+import pytest
+
+from ad_afqmc_prototype_amd import synthetic as syn
+from oracle import afqmc_oracle as orc
+
+from ._pairs import assert_rel, make_pair, rel, to_dev, to_np
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("M,N,K", [(1, 1, 8), (17, 33, 40), (256, 130, 1000), (300, 257, 512)])
+def test_sgemm_and_bf16x3_gemm(gpu_device, M, N, K):
+    import torch
+    from ad_afqmc_prototype_amd import kernels
+    g = torch.Generator().manual_seed(M + N + K)
+    A = torch.randn((M, K), dtype=torch.float64, generator=g)
+    B = torch.randn((N, K), dtype=torch.float64, generator=g)
+    ref = (A @ B.T).numpy()
+    Ad, Bd = A.to(gpu_device), B.to(gpu_device)
+    C = torch.empty((M, N), dtype=torch.float64, device=gpu_device)
+    kernels.sgemm_nt(kernels.convert_f32(Ad), kernels.convert_f32(Bd), C)
+    assert_rel(C.cpu().numpy(), ref, 2e-6)
+    C32 = torch.empty((M, N), dtype=torch.float32, device=gpu_device)
+    kernels.sgemm_nt(kernels.convert_f32(Ad), kernels.convert_f32(Bd), C32, alpha=0.5)
+    assert_rel(C32.cpu().numpy(), 0.5 * ref, 2e-6)
+    ah, al = kernels.split_bf16(Ad)
+    bh, bl = kernels.split_bf16(Bd)
+    kernels.gemm_bf16x3_nt(ah, al, bh, bl, C, alpha=2.0)
+    err = rel(C.cpu().numpy(), 2.0 * ref)
+    assert err < 1e-4
+    # the split is exact to ~2^-16: hi + lo reproduces x to that precision
+    hi = (ah.cpu().to(torch.int32) & 0xFFFF) << 16
+    lo = (al.cpu().to(torch.int32) & 0xFFFF) << 16
+    x = hi.view(torch.float32).double() + lo.view(torch.float32).double()
+    assert_rel(x[:, :K].numpy(), A.numpy(), 2e-5)
+
+
+def test_split_bf16_zero_padding_and_plain_bf16(gpu_device):
+    import torch
+    from ad_afqmc_prototype_amd import kernels
+    x = torch.tensor([[1.0, -2.5, 3.0e-3]], dtype=torch.float64, device=gpu_device)
+    hi, lo = kernels.split_bf16(x)
+    assert hi.shape == (1, 8)
+    assert torch.all(hi[0, 3:] == 0) and torch.all(lo[0, 3:] == 0)
+    # values exactly representable in bf16 have a zero low part
+    assert int(lo[0, 0]) == 0 and int(lo[0, 1]) == 0
+    assert int(hi[0, 0]) == 0x3F80 and (int(hi[0, 1]) & 0xFFFF) == 0xC020
+
+
+@pytest.mark.parametrize("precision,tol", [("fp32", 2e-5), ("bf16x3", 5e-4)])
+@pytest.mark.parametrize("case", [
+    dict(seed=0, n=24, o=6, g=96, uhf=False, kind="restricted"),
+    dict(seed=2, n=200, o=32, g=1000, uhf=False, kind="restricted"),
+    dict(seed=1, n=120, o=20, g=600, uhf=True, kind="unrestricted"),
+])
+def test_mixed_trotter_matches_fp64(gpu_device, precision, tol, case):
+    import torch
+    from ad_afqmc_prototype_amd.prop.chol_afqmc_ops import make_chol_afqmc_ops
+    p = make_pair(case["seed"], case["n"], case["o"], case["g"], uhf=case["uhf"], kind=case["kind"],
+                  dt=0.005, chol_scale=0.2, device=gpu_device)
+    rng = np.random.default_rng(5)
+    nw = 8
+    st = orc.init_state(p.sd, nw, overlap_mode="log")
+    w = st.walkers
+    x = rng.standard_normal((nw, case["g"])) + 0.3j * rng.standard_normal((nw, case["g"]))
+    xp = torch.tensor(np.stack([x.real, x.imag]), device=gpu_device)
+    ops = make_chol_afqmc_ops(p.g["ham"], case["kind"], precision=precision)
+    ctx = ops.build_prop_ctx(p.g["t_ops"].get_rdm1(p.g["trial"]), 0.005)
+    out = to_np(ops.apply_trotter_planes(to_dev(w, gpu_device), xp, ctx, 6))
+    ref = orc.apply_trotter(w, x, p.sd.pctx, p.sd.chol, 6, case["kind"])
+    if case["kind"] == "restricted":
+        assert_rel(out, ref, tol)
+    else:
+        assert max(rel(out[0], ref[0]), rel(out[1], ref[1])) < tol
+
+
+@pytest.mark.parametrize("precision,tol", [("fp32", 1e-5), ("bf16x3", 3e-4)])
+def test_mixed_force_bias(gpu_device, precision, tol):
+    from ad_afqmc_prototype_amd.core.ops import k_force_bias
+    from ad_afqmc_prototype_amd.meas.rhf import make_rhf_meas_ops
+    p = make_pair(2, 200, 32, 1000, device=gpu_device)
+    m_ops = make_rhf_meas_ops(p.g["sys"], precision=precision)
+    ctx = m_ops.build_meas_ctx(p.g["ham"], p.g["trial"])
+    st = orc.init_state(p.sd, 16, overlap_mode="log")
+    rng = np.random.default_rng(1)
+    w = st.walkers + 0.1 * (rng.standard_normal(st.walkers.shape) + 1j * rng.standard_normal(st.walkers.shape))
+    fb = m_ops.require_kernel(k_force_bias)(to_dev(w, gpu_device), p.g["ham"], ctx, p.g["trial"])
+    assert_rel(fb.cpu().numpy(), orc.force_bias(w, p.sd.trial, p.sd.mctx, "restricted"), tol)
+
+
+@pytest.mark.parametrize("precision", ["fp32", "bf16x3"])
+def test_mixed_block_energy_close_to_fp64(gpu_device, precision):
+    """Common random numbers: the mixed run and the fp64 checker see the same
+    fields and comb uniforms; block energies agree to well below 0.1 mHa on the
+    24-orbital workload (where the phaseless walk is in its working regime)."""
+    import dataclasses
+    import torch
+    from ad_afqmc_prototype_amd.meas.rhf import make_rhf_meas_ops
+    from ad_afqmc_prototype_amd.prop.afqmc import make_key
+    from ad_afqmc_prototype_amd.prop.blocks import afqmc_block
+    from ad_afqmc_prototype_amd.prop.chol_afqmc_ops import make_chol_afqmc_ops
+    from ad_afqmc_prototype_amd.prop.types import prop_state
+    from ad_afqmc_prototype_amd.trial import rhf as trial_rhf
+    seed, n, o, g, nw, dt = 0, 24, 6, 96, 128, 0.01
+    p = make_pair(seed, n, o, g, dt=dt, device=gpu_device)
+    st0 = orc.init_state(p.sd, nw, overlap_mode="log")
+    fields = [syn.fields_for_step(seed, t, nw, g) for t in range(40)]
+    zetas = [syn.zeta_for_event(seed, k) for k in range(4)]
+    _, blocks = orc.run_cadenced(st0, p.sd, p.prm, n_steps=40, qr_every=5, sr_every=10,
+                                 fields_fn=lambda t: fields[t], zeta_fn=lambda k: zetas[k],
+                                 overlap_mode="log")
+    m_ops = make_rhf_meas_ops(p.g["sys"], precision=precision)
+    p_ops = make_chol_afqmc_ops(p.g["ham"], "restricted", precision=precision)
+    p_ctx = p_ops.build_prop_ctx(p.g["t_ops"].get_rdm1(p.g["trial"]), dt)
+    m_ctx = m_ops.build_meas_ctx(p.g["ham"], p.g["trial"])
+    w = to_dev(st0.walkers, gpu_device)
+    gs = prop_state(walkers=w, weights=torch.ones(nw, dtype=torch.float64, device=gpu_device),
+                    log_overlaps=torch.tensor(st0.overlaps, device=gpu_device), rng_key=make_key(0),
+                    pop_control_ene_shift=torch.tensor(st0.e_estimate, dtype=torch.float64, device=gpu_device),
+                    e_estimate=st0.e_estimate, greens=trial_rhf.greens(w, p.g["trial"]))
+    params = dataclasses.replace(p.g["params"], n_prop_steps=10, n_ortho=5)
+    worst = 0.0
+    for k in range(4):
+        f = [torch.tensor(fields[10 * k + t], device=gpu_device) for t in range(10)]
+        gs, obs = afqmc_block(gs, sys=p.g["sys"], params=params, ham_data=p.g["ham"],
+                              trial_data=p.g["trial"], meas_ops=m_ops, prop_ops=p_ops, prop_ctx=p_ctx,
+                              meas_ctx=m_ctx, fields=f, zeta=zetas[k])
+        worst = max(worst, abs(obs.scalars["energy"].item() - blocks[k]["energy"]))
+    assert worst < 1e-4, worst  # Ha

