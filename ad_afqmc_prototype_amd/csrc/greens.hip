@@ -54,18 +54,36 @@ __global__ __launch_bounds__(THREADS) void greens_kernel(
     const int lane = tid & 63;
     const int wave = tid >> 6;
 
-    // 1. m = C^T w
-    for (int e = tid; e < o * o; e += THREADS) {
-        const int i = e / o;
-        const int j = e - i * o;
-        double re = 0.0, im = 0.0;
-        for (int p = 0; p < n; ++p) {
-            const double c = Cm[p * o + i];
-            const double2 x = W[p * o + j];
-            re += c * x.x;
-            im += c * x.y;
+    // 1. m = C^T w on v_mfma_f64_16x16x4: each wave owns 16x16 tiles of m;
+    //    A = C^T (lane: row i, four consecutive p), B = w (same p, column j).
+    {
+        const int ntile = (o + 15) >> 4;
+        const int lr = lane & 15;
+        const int lk = (lane >> 4) * 4;
+        const int nkb = (n + 15) >> 4;
+        for (int t = wave; t < ntile * ntile; t += THREADS / 64) {
+            const int it = t / ntile;
+            const int jt = t - it * ntile;
+            const int i = it * 16 + lr;
+            const int j = jt * 16 + lr;
+            f64x4 ar = {0.0, 0.0, 0.0, 0.0}, ai = {0.0, 0.0, 0.0, 0.0};
+            for (int kb = 0; kb < nkb; ++kb) {
+#pragma unroll
+                for (int kk = 0; kk < 4; ++kk) {
+                    const int p = kb * 16 + lk + kk;
+                    const bool okp = p < n;
+                    const double a = (okp && i < o) ? Cm[p * o + i] : 0.0;
+                    const double2 x = (okp && j < o) ? W[p * o + j] : make_double2(0.0, 0.0);
+                    ar = afq::mfma_f64(a, x.x, ar);
+                    ai = afq::mfma_f64(a, x.y, ai);
+                }
+            }
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int ii = it * 16 + afq::mfma16<double>::row(lane, r);
+                if (ii < o && j < o) m[ii][j] = make_double2(ar[r], ai[r]);
+            }
         }
-        m[i][j] = make_double2(re, im);
     }
     if (tid < o) perm[tid] = tid;
     __syncthreads();

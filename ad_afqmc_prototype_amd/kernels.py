@@ -295,12 +295,16 @@ def energy(greens, nw, n, o, nspin, rot_chol_pad, rot_h1, h0, out=None):
     ldr = ldv_for(n)
     opad = (o + 15) // 16 * 16
     _check(greens, f64, (2, nw, nspin * o * n), "greens")
-    _check(rot_chol_pad, f64, (nspin, naux, opad, ldr), "rot_chol_pad", dev)
+    rt = rot_chol_pad.dtype
+    if rt not in (f64, f32):
+        raise TypeError("rot_chol_pad must be float64 or float32")
+    _check(rot_chol_pad, rt, (nspin, naux, opad, ldr), "rot_chol_pad", dev)
     _check(rot_h1, f64, (nspin, o, n), "rot_h1", dev)
     if out is None:
         out = torch.empty((nw,), dtype=c128, device=dev)
     _check(out, c128, (nw,), "energy", dev)
-    _call("afq_energy", greens, nw, n, o, nspin, naux, float(h0), greens.data_ptr(),
+    _call("afq_energy" if rt == f64 else "afq_energy_f32", greens, nw, n, o, nspin, naux, float(h0),
+          greens.data_ptr(),
               rot_chol_pad.data_ptr(), ldr, rot_h1.data_ptr(), out.data_ptr(), _stream(greens))
     return out
 

@@ -53,12 +53,16 @@ class rhf_meas_ctx:
         return self._cache[key]
 
     def energy_operands(self, nspin: int):
+        """(rot_chol_pad (nspin, naux, opad, ldr), rot_h1 (nspin, o, n)) for afq_energy."""
         key = ("energy", nspin)
         if key not in self._cache:
             _, g, o, n = self.rot_chol.shape
             opad = (o + 15) // 16 * 16
             ldr = kernels.ldv_for(n)
-            pad = torch.zeros((nspin, g, opad, ldr), dtype=torch.float64, device=self.rot_chol.device)
+            # fp32 contractions (fp64 sums) for the mixed modes: split bf16 is
+            # too coarse for the exchange term
+            dt = torch.float64 if self.precision == "fp64" else torch.float32
+            pad = torch.zeros((nspin, g, opad, ldr), dtype=dt, device=self.rot_chol.device)
             pad[:, :, :o, :n] = self.rot_chol[:nspin]
             self._cache[key] = (pad, self.rot_h1[:nspin].contiguous())
         return self._cache[key]

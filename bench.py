@@ -38,7 +38,7 @@ def flops_per_walker_step(n, o, g, nspin, rhf_trial):
                 total=vhs + fb + taylor + one_body + greens)
 
 
-def build(workload: str, device, precision: str = "fp64"):
+def build(workload: str, device, precision: str = "fp64", chol_scale: float = 1.0):
     import torch
 
     from ad_afqmc_prototype_amd import synthetic as syn
@@ -52,7 +52,8 @@ def build(workload: str, device, precision: str = "fp64"):
                                                   rhf_trial, uhf_trial)
 
     cfg = syn.WORKLOADS[workload]
-    s = syn.make_system(cfg["seed"], cfg["norb"], cfg["nocc"], cfg["naux"], uhf=cfg["uhf"])
+    s = syn.make_system(cfg["seed"], cfg["norb"], cfg["nocc"], cfg["naux"], uhf=cfg["uhf"],
+                        chol_scale=chol_scale)
     kind = cfg["walker_kind"]
     sys_ = system(norb=s.norb, nelec=(s.nocc, s.nocc), walker_kind=kind)
     ham = ham_chol.from_arrays(s.h0, s.h1, s.chol, device=device)
@@ -142,6 +143,33 @@ def cpu_baseline(workload: str, budget_s: float = 20.0):
                       f"log-overlaps, {el:.1f} s, {threads} BLAS threads)"}
 
 
+def accuracy_check(workload: str, precision: str, device, chol_scale: float, n_blocks: int):
+    """Block energies of the mixed and the fp64 step under common random
+    numbers (same fields, same comb uniforms, same initial state) on the
+    workload's shapes with the Cholesky vectors scaled by chol_scale, so that
+    the walk stays in the working regime of the order-6 Taylor step."""
+    import torch
+
+    res = {}
+    for prec in ("fp64", precision):
+        ctx = build(workload, device, prec, chol_scale)
+        cfg = ctx["cfg"]
+        gen = torch.Generator(device=device)
+        gen.manual_seed(cfg["seed"] + 1000)
+        steps = n_blocks * cfg["sr_every"]
+        fields = [torch.randn((cfg["n_walkers"], cfg["naux"]), dtype=torch.float64, device=device,
+                              generator=gen) for _ in range(steps)]
+        energies = []
+        run_steps(ctx, ctx["state"], fields, 0, steps, cadence=True, energies=energies)
+        res[prec] = [float(e.item()) for e in energies]
+        del ctx, fields
+        torch.cuda.empty_cache()
+    diffs = [1e3 * abs(a - b) for a, b in zip(res[precision], res["fp64"])]
+    return {"chol_scale": chol_scale, "n_blocks": n_blocks, "precision": precision,
+            "block_energies_fp64_Ha": res["fp64"], "block_energies_mixed_Ha": res[precision],
+            "abs_diff_mHa": diffs, "max_abs_diff_mHa": max(diffs) if diffs else None}
+
+
 def kernel_table(ktimes, n, o, g, nw, nspin, rhf):
     """Algorithmic flops per launch and peak of every timed entry point."""
     fl = flops_per_walker_step(n, o, g, nspin, rhf)
@@ -212,6 +240,9 @@ def main():
     ap.add_argument("--no-fp64-compare", action="store_true",
                     help="skip the fp64 run that the mixed throughput is compared with")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--accuracy-scale", type=float, default=0.03,
+                    help="Cholesky scale of the common-random-number energy check (0 skips it)")
+    ap.add_argument("--accuracy-blocks", type=int, default=4)
     ap.add_argument("--no-cadence", action="store_true",
                     help="time afqmc_step alone (no QR / measurement / reconfiguration)")
     args = ap.parse_args()
@@ -270,6 +301,9 @@ def main():
         out["speedup_vs_fp64"] = out["value"] / out["fp64_value"]
         out["fp64_kernels"] = kernel_table(kt64, n, o, g, nw, nspin, not cfg["uhf"])
         out["block_energy_abs_diff_mHa"] = [1e3 * abs(a - b) for a, b in zip(e_block, e64)]
+    if args.precision != "fp64" and args.accuracy_scale > 0:
+        out["energy_check"] = accuracy_check(args.workload, args.precision, device,
+                                             args.accuracy_scale, args.accuracy_blocks)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(args.workload)
     print(json.dumps(out), flush=True)

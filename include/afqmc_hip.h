@@ -184,6 +184,16 @@ int afq_energy(int nw, int n, int o, int nspin, int naux, double h0, const doubl
                void* stream);
 
 /*
+ * afq_energy with the f_g contractions on fp32 MFMA (rot_chol_pad float32) and
+ * every sum over g (Coulomb squares, exchange, one-body) in fp64; two walkers
+ * share each streamed rot_chol_g tile when they fit in LDS.  The measurement
+ * of the mixed step.
+ */
+int afq_energy_f32(int nw, int n, int o, int nspin, int naux, double h0, const double* greens,
+                   const float* rot_chol_pad, int ldr, const double* rot_h1, double* energy,
+                   void* stream);
+
+/*
  * Block energy of prop/blocks.py:59-79: real part, outlier clip to e_ref when
  * |e - e_ref| > thresh, e_block = sum w e / sum w (e_ref when sum w == 0), and
  * pop_shift_new = (1 - shift_ema) pop_shift + shift_ema e_block.

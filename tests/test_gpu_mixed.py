@@ -131,3 +131,28 @@ def test_mixed_block_energy_close_to_fp64(gpu_device, precision):
                               meas_ctx=m_ctx, fields=f, zeta=zetas[k])
         worst = max(worst, abs(obs.scalars["energy"].item() - blocks[k]["energy"]))
     assert worst < 1e-4, worst  # Ha
+
+
+@pytest.mark.parametrize("case", [
+    dict(seed=2, n=200, o=32, g=1000, uhf=False, kind="restricted", nw=5),
+    dict(seed=1, n=120, o=20, g=600, uhf=True, kind="unrestricted", nw=7),
+    dict(seed=0, n=24, o=6, g=96, uhf=False, kind="restricted", nw=9),
+])
+def test_energy_fp32_contractions(gpu_device, case):
+    """fp32 f_g contractions with fp64 sums: per-walker energies within 2e-5 of
+    |E| of the fp64 checker (two walkers per workgroup, odd walker count).  The
+    random walkers here are far from the trial (|E| up to 2e4 Ha, large
+    imaginary parts), the hardest case for fp32."""
+    from ad_afqmc_prototype_amd.core.ops import k_energy
+    from ad_afqmc_prototype_amd.meas.rhf import make_rhf_meas_ops
+    p = make_pair(case["seed"], case["n"], case["o"], case["g"], uhf=case["uhf"], kind=case["kind"],
+                  device=gpu_device)
+    m_ops = make_rhf_meas_ops(p.g["sys"], precision="fp32")
+    ctx = m_ops.build_meas_ctx(p.g["ham"], p.g["trial"])
+    rng = np.random.default_rng(3)
+    from ._pairs import random_walkers
+    w = random_walkers(rng, case["kind"], case["nw"], case["n"], case["o"], p.s.mo_up, p.s.mo_dn)
+    e = m_ops.require_kernel(k_energy)(to_dev(w, gpu_device), p.g["ham"], ctx, p.g["trial"]).cpu().numpy()
+    e_ref = orc.energy(w, p.sd.h0, p.sd.trial, p.sd.mctx, case["kind"])
+    err = np.abs(e - e_ref) / np.abs(e_ref)
+    assert err.max() < 2e-5, err.max()
