@@ -170,6 +170,24 @@ def accuracy_check(workload: str, precision: str, device, chol_scale: float, n_b
             "abs_diff_mHa": diffs, "max_abs_diff_mHa": max(diffs) if diffs else None}
 
 
+def measured_traffic(workload: str, kernel: str):
+    """HBM bytes per launch of ``kernel`` from the committed rocprofv3 FETCH_SIZE /
+    WRITE_SIZE summaries (profiles/), each scaled by its copy-kernel
+    calibration; None when not measured."""
+    short = workload.rsplit("_", 1)[0]
+    path = os.path.join(ROOT, "profiles", f"traffic_{short}.json")
+    try:
+        data = json.load(open(path))[workload]
+    except (OSError, KeyError, ValueError):
+        return None
+    row = data.get(kernel)
+    cal = data.get("calibration", {})
+    if not row or row.get("fetch_kib_raw") is None or row.get("write_kib_raw") is None:
+        return None
+    return 1024.0 * (row["fetch_kib_raw"] * cal.get("fetch_factor", 1.0)
+                     + row["write_kib_raw"] * cal.get("write_factor", 1.0))
+
+
 def kernel_table(ktimes, n, o, g, nw, nspin, rhf):
     """Algorithmic flops per launch and peak of every timed entry point."""
     fl = flops_per_walker_step(n, o, g, nspin, rhf)
@@ -243,6 +261,9 @@ def main():
     ap.add_argument("--accuracy-scale", type=float, default=0.03,
                     help="Cholesky scale of the common-random-number energy check (0 skips it)")
     ap.add_argument("--accuracy-blocks", type=int, default=4)
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
+                    help="only run a plain device copy of BYTES bytes 5 times (the known-size "
+                         "kernel that calibrates rocprofv3 FETCH_SIZE / WRITE_SIZE)")
     ap.add_argument("--no-cadence", action="store_true",
                     help="time afqmc_step alone (no QR / measurement / reconfiguration)")
     args = ap.parse_args()
@@ -258,6 +279,15 @@ def main():
     from ad_afqmc_prototype_amd import synthetic as syn
 
     device = torch.device("cuda", 0)
+    if args.copy_calibration:
+        n = args.copy_calibration // 8
+        src = torch.ones(n, dtype=torch.float64, device=device)
+        dst = torch.empty_like(src)
+        for _ in range(5):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_calibration_bytes": n * 8, "launches": 5}))
+        return
     cfg = syn.WORKLOADS[args.workload]
     n, o, g, nw = cfg["norb"], cfg["nocc"], cfg["naux"], cfg["n_walkers"]
     nspin = 2 if cfg["walker_kind"] == "unrestricted" else 1
@@ -289,7 +319,7 @@ def main():
                    "measure_sr_every": None if args.no_cadence else cfg["sr_every"]},
         "roofline": {"bound": "mfma", "kernel": dominant, "achieved": dom["tflops"],
                      "peak": dom["peak_tflops"], "unit": "TFLOP/s", "frac": dom["frac"],
-                     "traffic": None},
+                     "traffic": measured_traffic(args.workload, dominant)},
         "step_tflops_algorithmic": round(flops["total"] * nw * args.steps / elapsed / 1e12, 2),
         "kernels": table,
         "block_energies": e_block,
