@@ -235,6 +235,237 @@ int launch_trotter(int nw, int n, int o, int nspin, int ldv, int n_terms, double
     return afq::check_launch("afq_trotter");
 }
 
+// ---------------------------------------------------------------------------
+// fp32 Trotter step with the walker's VHS resident on chip for all Taylor
+// terms.  One wave per 16-row tile of V (ceil16(n)/16 waves, 13 at n = 200):
+// each wave loads its V rows once; k-blocks (16 k) 0..KRB-1 stay in registers
+// as MFMA A fragments, the remaining k-blocks sit in LDS in fragment order
+// (lane-contiguous, conflict-free).  The Taylor recursion never mixes columns
+// of T, so each workgroup owns one 16-column tile of one (walker, spin): T
+// (n x 16, split planes) is the B operand in LDS and every wave computes its
+// whole row tile, so no partial sums cross waves.  The workgroups of one
+// walker run back to back on one XCD and share V through L2; V crosses HBM
+// once per walker instead of once per Taylor term.
+// ---------------------------------------------------------------------------
+template <int KRB>
+__global__ __launch_bounds__(1024, 1) void trotter_resident_kernel(
+    int nw, int n, int o, int nspin, int nct, int ldv, int n_terms, double sqrt_dt,
+    const float* __restrict__ E, const float* __restrict__ vhs, long long vplane,
+    const double2* __restrict__ win_up, const double2* __restrict__ win_dn,
+    double2* __restrict__ wout_up, double2* __restrict__ wout_dn, int group_map) {
+    constexpr int NC = 1;
+    using MF = afq::mfma16<float>;
+    using ACC = MF::acc;
+    extern __shared__ __attribute__((aligned(16))) unsigned char lds_raw[];
+    constexpr int TC = NC * 16;
+    const int NP = afq::round_up(n, 16);
+    const int ntile = NP / 16;
+    const int nkb = NP / 16;
+    const int nkl = nkb > KRB ? nkb - KRB : 0;
+    float* Tr = reinterpret_cast<float*>(lds_raw);
+    float* Ti = Tr + NP * TC;
+    float* VL = Ti + NP * TC;  // [wave][nkl][4 kk][2 planes][64 lanes]
+
+    // block -> (walker, spin, column tile); the G = nspin * nct blocks of a
+    // walker land on one XCD (blocks b, b+8, ... share an XCD) back to back
+    const int G = nspin * nct;
+    const int b = blockIdx.x;
+    int w, sub;
+    if (group_map) {
+        w = (b / (8 * G)) * 8 + (b % 8);
+        sub = (b / 8) % G;
+    } else {
+        w = b / G;
+        sub = b % G;
+    }
+    const int s = sub / nct;
+    const int c0 = (sub - s * nct) * 16;  // first walker column of this tile
+    const long long woff = (long long)w * n * o;
+    const double2* Win = (s == 0 ? win_up : win_dn) + woff;
+    double2* Wout = (s == 0 ? wout_up : wout_dn) + woff;
+    const float* Vr = vhs + (long long)w * n * ldv;
+    const float* Vi = Vr + vplane;
+
+    const int tid = threadIdx.x;
+    const int nthreads = ntile * 64;
+    const int lane = tid & 63;
+    const int wave = tid >> 6;
+    const int lr = lane & 15;
+    const int lk = (lane >> 4) * 4;
+    const int rt = wave;
+    const int row = rt * 16 + lr;
+
+    // branch-free fragment load of 4 consecutive k of this lane's row
+    auto load4 = [&](const float* A, int kb, float (&x)[4]) {
+        const int k0 = kb * 16 + lk;
+        const bool ok = row < n && k0 < n;
+        const float4 v = *reinterpret_cast<const float4*>(A + (long long)(ok ? row : 0) * ldv + (ok ? k0 : 0));
+        x[0] = ok ? v.x : 0.f; x[1] = ok ? v.y : 0.f; x[2] = ok ? v.z : 0.f; x[3] = ok ? v.w : 0.f;
+    };
+
+    for (int e = tid; e < NP * TC; e += nthreads) {
+        const int p = e / TC;
+        const int j = c0 + (e - p * TC);
+        const double2 x = (p < n && j < o) ? Win[p * o + j] : make_double2(0.0, 0.0);
+        Tr[e] = (float)x.x;
+        Ti[e] = (float)x.y;
+    }
+    __syncthreads();
+
+    ACC P[NC][2], out[NC][2];
+
+    // P = E . T (E real, from L2): all of this wave's E fragments are loaded
+    // first (nkb <= 16), then consumed, so the loads overlap
+    auto e_product = [&]() {
+        float a[16][4];
+#pragma unroll
+        for (int kb = 0; kb < 16; ++kb)
+            if (kb < nkb) load4(E, kb, a[kb]);
+#pragma unroll
+        for (int c = 0; c < NC; ++c) P[c][0] = P[c][1] = MF::zero();
+#pragma unroll
+        for (int kb = 0; kb < 16; ++kb) {
+            if (kb < nkb) {
+                const int k0 = kb * 16 + lk;
+#pragma unroll
+                for (int kk = 0; kk < 4; ++kk)
+#pragma unroll
+                    for (int c = 0; c < NC; ++c) {
+                        const int col = c * 16 + lr;
+                        P[c][0] = MF::fma(a[kb][kk], Tr[(k0 + kk) * TC + col], P[c][0]);
+                        P[c][1] = MF::fma(a[kb][kk], Ti[(k0 + kk) * TC + col], P[c][1]);
+                    }
+            }
+        }
+    };
+    auto store_T = [&](const ACC (&X)[NC][2]) {
+#pragma unroll
+        for (int c = 0; c < NC; ++c)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int rr = rt * 16 + MF::row(lane, r);
+                Tr[rr * TC + c * 16 + lr] = X[c][0][r];
+                Ti[rr * TC + c * 16 + lr] = X[c][1][r];
+            }
+    };
+    // complex V . T step for one (k-block, kk) with A = (ar, ai)
+    auto vt_step = [&](int k, float ar, float ai) {
+#pragma unroll
+        for (int c = 0; c < NC; ++c) {
+            const int col = c * 16 + lr;
+            const float br = Tr[k * TC + col];
+            const float bi = Ti[k * TC + col];
+            P[c][0] = MF::fma(ar, br, P[c][0]);
+            P[c][1] = MF::fma(ar, bi, P[c][1]);
+            P[c][0] = MF::fma(ai, -bi, P[c][0]);
+            P[c][1] = MF::fma(ai, br, P[c][1]);
+        }
+    };
+
+    // T = E w, before V occupies the registers
+    e_product();
+
+    float vr[KRB][4], vi[KRB][4];
+#pragma unroll
+    for (int kb = 0; kb < KRB; ++kb) {
+        load4(Vr, kb < nkb ? kb : 0, vr[kb]);
+        load4(Vi, kb < nkb ? kb : 0, vi[kb]);
+    }
+    for (int kb = KRB; kb < nkb; ++kb) {
+        float xr[4], xi[4];
+        load4(Vr, kb, xr);
+        load4(Vi, kb, xi);
+        float* dst = VL + ((size_t)(wave * nkl + (kb - KRB)) * 8) * 64 + lane;
+#pragma unroll
+        for (int kk = 0; kk < 4; ++kk) {
+            dst[(kk * 2 + 0) * 64] = xr[kk];
+            dst[(kk * 2 + 1) * 64] = xi[kk];
+        }
+    }
+
+    __syncthreads();
+    store_T(P);
+#pragma unroll
+    for (int c = 0; c < NC; ++c) {
+        out[c][0] = P[c][0];
+        out[c][1] = P[c][1];
+    }
+    __syncthreads();
+
+#pragma unroll 1
+    for (int k = 1; k <= n_terms; ++k) {
+#pragma unroll
+        for (int c = 0; c < NC; ++c) P[c][0] = P[c][1] = MF::zero();
+#pragma unroll
+        for (int kb = 0; kb < KRB; ++kb) {
+            if (kb < nkb) {
+#pragma unroll
+                for (int kk = 0; kk < 4; ++kk) vt_step(kb * 16 + lk + kk, vr[kb][kk], vi[kb][kk]);
+            }
+        }
+        for (int kb = KRB; kb < nkb; ++kb) {
+            const float* src = VL + ((size_t)(wave * nkl + (kb - KRB)) * 8) * 64 + lane;
+#pragma unroll
+            for (int kk = 0; kk < 4; ++kk)
+                vt_step(kb * 16 + lk + kk, src[(kk * 2 + 0) * 64], src[(kk * 2 + 1) * 64]);
+        }
+        const float ck = (float)(sqrt_dt / (double)k);
+#pragma unroll
+        for (int c = 0; c < NC; ++c) {
+            const ACC re = -ck * P[c][1];
+            const ACC im = ck * P[c][0];
+            P[c][0] = re;
+            P[c][1] = im;
+            out[c][0] += re;
+            out[c][1] += im;
+        }
+        __syncthreads();
+        store_T(P);
+        __syncthreads();
+    }
+
+    store_T(out);
+    __syncthreads();
+    e_product();
+    {
+        const int col = c0 + lr;
+        if (col < o) {
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int rr = rt * 16 + MF::row(lane, r);
+                if (rr < n) Wout[rr * o + col] = make_double2((double)P[0][0][r], (double)P[0][1][r]);
+            }
+        }
+    }
+}
+
+constexpr int RESIDENT_KRB = 9;
+
+size_t resident_lds(int n) {
+    const int NP = afq::round_up(n, 16);
+    const int nkb = NP / 16;
+    const int nkl = nkb > RESIDENT_KRB ? nkb - RESIDENT_KRB : 0;
+    return sizeof(float) * ((size_t)2 * NP * 16 + (size_t)(NP / 16) * nkl * 8 * 64);
+}
+
+int launch_trotter_resident(int nw, int n, int o, int nspin, int ldv, int n_terms, double sqrt_dt,
+                            const float* E, const float* vhs, const double2* wu, const double2* wd,
+                            double2* ou, double2* od, hipStream_t st) {
+    const int NP = afq::round_up(n, 16);
+    const int nct = afq::ceil_div(o, 16);
+    const size_t lds = resident_lds(n);
+    const int group_map = (nw % 8 == 0) ? 1 : 0;
+    if (lds > 64 * 1024) {
+        (void)hipFuncSetAttribute((const void*)trotter_resident_kernel<RESIDENT_KRB>,
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        (void)hipGetLastError();
+    }
+    hipLaunchKernelGGL((trotter_resident_kernel<RESIDENT_KRB>), dim3((unsigned)(nw * nspin * nct)),
+                       dim3(64 * (NP / 16)), lds, st, nw, n, o, nspin, nct, ldv, n_terms, sqrt_dt, E,
+                       vhs, (long long)nw * n * ldv, wu, wd, ou, od, group_map);
+    return afq::check_launch("afq_trotter_f32");
+}
+
 template <typename R>
 int trotter_entry(const char* name, int nw, int n, int o, int nspin, int ldv, int n_terms,
                   double sqrt_dt, const R* exp_h1_half, const R* vhs, const double* win_up,
@@ -258,6 +489,13 @@ int trotter_entry(const char* name, int nw, int n, int o, int nspin, int ldv, in
     const double2* wd = (const double2*)(nspin == 2 ? win_dn : win_up);
     double2* ou = (double2*)wout_up;
     double2* od = (double2*)(nspin == 2 ? wout_dn : wout_up);
+    if constexpr (sizeof(R) == sizeof(float)) {
+        // VHS resident on chip when one wave per row tile fits (<= 16 waves)
+        // and the LDS share of V plus T fits in 160 KB
+        if (ntile <= 16 && resident_lds(n) <= 160 * 1024)
+            return launch_trotter_resident(nw, n, o, nspin, ldv, n_terms, sqrt_dt, exp_h1_half,
+                                           vhs, wu, wd, ou, od, st);
+    }
 #define AFQ_TROTTER_CASE(NC_, RPW_)                                                            \
     if (nc == NC_ && rpw == RPW_)                                                              \
         return launch_trotter<R, NC_, RPW_>(nw, n, o, nspin, ldv, n_terms, sqrt_dt, exp_h1_half, \
