@@ -29,6 +29,8 @@ _SIGNATURES = {
     "afq_convert_f32": [c_int, c_int, c_vp, c_ll, c_vp, c_ll, c_vp],
     "afq_trotter_f32": [c_int, c_int, c_int, c_int, c_int, c_int, c_double, c_vp, c_vp, c_vp, c_vp,
                         c_vp, c_vp, c_vp],
+    "afq_trotter_bf16x3": [c_int, c_int, c_int, c_int, c_int, c_int, c_double, c_vp, c_vp, c_vp,
+                           c_vp, c_vp, c_vp, c_vp],
     "afq_overlap_greens": [c_int, c_int, c_int, c_int, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp],
     "afq_force_bias": [c_int, c_int, c_int, c_double, c_vp, c_vp, c_vp, c_vp],
     "afq_field_shifts": [c_int, c_int, c_double, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp],

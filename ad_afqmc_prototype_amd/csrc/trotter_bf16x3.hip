@@ -1,0 +1,494 @@
+// Split-bf16 ("bf16x3") Trotter step of the mixed-precision walker step.
+//
+//   W <- E . Taylor_m(i sqrt(dt) V) . E . W,  E = exp(-dt h1 / 2)
+//
+// (reference: prop/chol_afqmc_ops.py:147-166 and :115-128).  Inputs and
+// outputs are those of the fp32 step: fp32 E (n x ldv) and VHS planes, fp64
+// complex walkers.  Arithmetic:
+//
+// * Every matrix product runs on v_mfma_f32_16x16x32_bf16 with both operands
+//   split as x = hi + lo (hi = bf16(x), lo = bf16(x - hi), ~16 significant
+//   bits) and accumulated in fp32 as al.bh + ah.bl + ah.bh.
+// * The Taylor products V.T all carry the factor sqrt(dt) / k, so the split
+//   error (~2^-17 relative) enters the walker scaled by sqrt(dt) ||V||.
+// * E.X is formed as X + (E - I).X: X is added exactly in fp32 (it is the
+//   accumulator's initial value) and only (E - I), of norm ~dt ||h1|| / 2,
+//   goes through the split product.
+// * The running sum and every Taylor term are fp32 accumulators.
+//
+// Layout: one workgroup per (walker, spin, 16-column tile of the walker),
+// ceil(ntile / 2) waves, each owning two 16-row tiles of V.  The walker's V
+// is resident on chip for all Taylor terms: k-blocks (32 wide) 0..KR-1 as
+// split MFMA A fragments in registers (160 VGPRs per lane), the remaining NL
+// k-blocks in LDS in fragment order (lane-contiguous 16-byte slots).  The
+// current Taylor term T (n x 16) sits in LDS as four bf16 planes (re hi/lo,
+// im hi/lo), column-major so that each lane's B fragment (8 consecutive k of
+// one column) is one 16-byte read.  The workgroups of one walker run back to
+// back on one XCD and share V through L2.
+#include <cstdint>
+
+#include "afqmc_hip.h"
+#include "common.h"
+
+namespace {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+
+constexpr int RT = 2;  // 16-row tiles per wave
+constexpr int KR = 5;  // 32-wide k-blocks of V held in registers
+constexpr int NL_MAX = 2;
+
+__device__ __forceinline__ f32x4 mfma_bf16(bf16x8 a, bf16x8 b, f32x4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
+}
+
+__device__ __forceinline__ bf16x8 neg8(bf16x8 x) {
+    uint4 u = __builtin_bit_cast(uint4, x);
+    u.x ^= 0x80008000u;
+    u.y ^= 0x80008000u;
+    u.z ^= 0x80008000u;
+    u.w ^= 0x80008000u;
+    return __builtin_bit_cast(bf16x8, u);
+}
+
+struct split8 {
+    bf16x8 hi, lo;
+};
+
+__device__ __forceinline__ split8 split_f32x8(const float (&x)[8]) {
+    split8 s;
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+        const __bf16 h = (__bf16)x[j];
+        s.hi[j] = h;
+        s.lo[j] = (__bf16)(x[j] - (float)h);
+    }
+    return s;
+}
+
+// Raw buffer loads: a 32-bit byte offset from a per-array base in SGPRs, and
+// zeros for offsets at or past the array's size (gfx9 descriptor word 3).
+typedef __amdgpu_buffer_rsrc_t rsrc_t;
+constexpr unsigned kOutside = 0x80000000u;
+
+__device__ __forceinline__ rsrc_t make_rsrc(const void* p, unsigned bytes) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), (short)0, (int)bytes, 0x00020000);
+}
+
+__device__ __forceinline__ float4 load_f4(rsrc_t r, unsigned off) {
+    return __builtin_bit_cast(float4, __builtin_amdgcn_raw_buffer_load_b128(r, off, 0, 0));
+}
+
+__device__ __forceinline__ double2 load_d2(rsrc_t r, unsigned off) {
+    return __builtin_bit_cast(double2, __builtin_amdgcn_raw_buffer_load_b128(r, off, 0, 0));
+}
+
+// Eight consecutive k (k0 .. k0+7, k0 a multiple of 8) of row `row` of an
+// n x n fp32 matrix stored with leading dimension ld (a multiple of 4) in the
+// buffer r of n * ld floats: rows >= n read as zero (outside the buffer), k
+// blocks starting at or past n are skipped; columns n..ld-1 inside a row are
+// read as stored.  Nothing here waits on the loaded data.
+__device__ __forceinline__ void load_row8(rsrc_t r, int ld, int n, int row, int k0, float (&x)[8]) {
+    const unsigned base = (unsigned)(row * ld + k0) * 4u;
+    const float4 a = load_f4(r, k0 < n ? base : kOutside);
+    const float4 c = load_f4(r, k0 + 4 < n ? base + 16u : kOutside);
+    x[0] = a.x; x[1] = a.y; x[2] = a.z; x[3] = a.w;
+    x[4] = c.x; x[5] = c.y; x[6] = c.z; x[7] = c.w;
+}
+
+// x through an empty asm: the compiler recomputes what depends on it instead
+// of keeping (and spilling) values from before a long loop
+__device__ __forceinline__ int opaque(int x) {
+    asm volatile("" : "+v"(x));
+    return x;
+}
+
+// split of x - I restricted to row `row` (< n), columns k0..k0+7; branch-free
+__device__ __forceinline__ split8 split_minus_identity(const float (&x)[8], int row, int n, int k0) {
+    float y[8];
+    const int d = row < n ? row - k0 : -1;
+#pragma unroll
+    for (int j = 0; j < 8; ++j) y[j] = x[j] - (d == j ? 1.f : 0.f);
+    return split_f32x8(y);
+}
+
+template <int NL>
+__global__ __launch_bounds__(512, 1) void trotter_bf16x3_kernel(
+    int nw, int n, int o, int nspin, int nct, int ldv, int n_terms, double sqrt_dt,
+    const float* __restrict__ E, const float* __restrict__ vhs, long long vplane,
+    const double2* __restrict__ win_up, const double2* __restrict__ win_dn,
+    double2* __restrict__ wout_up, double2* __restrict__ wout_dn, int group_map) {
+    constexpr int KMAX = KR + NL;
+    extern __shared__ __attribute__((aligned(16))) unsigned char lds_raw[];
+    const int NP = afq::round_up(n, 16);
+    const int nk = afq::ceil_div(NP, 32);  // == KMAX when NL > 0, <= KR otherwise
+    const int TP = nk * 32 + 8;            // bf16 pitch of one column of T
+    __bf16* Tt = reinterpret_cast<__bf16*>(lds_raw);  // [4 planes][16 cols][TP]
+    uint4* VL = reinterpret_cast<uint4*>(lds_raw + (size_t)4 * 16 * TP * sizeof(__bf16));
+
+    const int G = nspin * nct;
+    const int b = blockIdx.x;
+    int w, sub;
+    if (group_map) {
+        w = (b / (8 * G)) * 8 + (b % 8);
+        sub = (b / 8) % G;
+    } else {
+        w = b / G;
+        sub = b % G;
+    }
+    const int s = sub / nct;
+    const int c0 = (sub - s * nct) * 16;
+    const long long woff = (long long)w * n * o;
+    double2* Wout = (s == 0 ? wout_up : wout_dn) + woff;
+    const rsrc_t Wr = make_rsrc((s == 0 ? win_up : win_dn) + woff, (unsigned)n * o * 16u);
+    const unsigned mat_bytes = (unsigned)n * ldv * 4u;
+    const rsrc_t Er = make_rsrc(E, mat_bytes);
+    const rsrc_t Vr = make_rsrc(vhs + (long long)w * n * ldv, mat_bytes);
+    const rsrc_t Vi = make_rsrc(vhs + vplane + (long long)w * n * ldv, mat_bytes);
+
+    // blockDim.x = 64 * nk: one wave per pair of 16-row tiles
+    const int tid = threadIdx.x;
+    const int nthreads = blockDim.x;
+    const int lane = tid & 63;
+    const int wave = tid >> 6;
+    const int lr = lane & 15;
+    const int q = lane >> 4;
+    const int kq = 8 * q;
+    const int col = c0 + lr;
+    const int row0 = wave * RT * 16 + lr;  // A-fragment row of tile 0 (tile t: + 16 t)
+
+    auto read_b = [&](int kb, int plane) {
+        return *reinterpret_cast<const bf16x8*>(Tt + (plane * 16 + lr) * TP + kb * 32 + kq);
+    };
+    // W[p][j], zero outside n x o (rows p >= n fall outside the buffer)
+    auto load_w = [&](int p, int j) {
+        return load_d2(Wr, j < o ? (unsigned)(p * o + j) * 16u : kOutside);
+    };
+    // all E fragments of this wave's rows, as raw fp32 (one round trip)
+    auto load_e = [&](float (&er)[RT][KMAX][8], int r0, int k8) {
+#pragma unroll
+        for (int t = 0; t < RT; ++t)
+#pragma unroll
+            for (int kb = 0; kb < KMAX; ++kb)
+                load_row8(Er, ldv, n, r0 + 16 * t, kb * 32 + k8, er[t][kb]);
+    };
+    // acc += (E - I) . T
+    auto e_product = [&](f32x4 (&acc)[RT][2], const float (&er)[RT][KMAX][8], int r0, int k8) {
+#pragma unroll
+        for (int kb = 0; kb < KMAX; ++kb) {
+            if (kb < nk) {
+                split8 ef[RT];
+#pragma unroll
+                for (int t = 0; t < RT; ++t) ef[t] = split_minus_identity(er[t][kb], r0 + 16 * t, n, kb * 32 + k8);
+                const bf16x8 brh = read_b(kb, 0), brl = read_b(kb, 1);
+                const bf16x8 bih = read_b(kb, 2), bil = read_b(kb, 3);
+#pragma unroll
+                for (int t = 0; t < RT; ++t) {
+                    acc[t][0] = mfma_bf16(ef[t].lo, brh, acc[t][0]);
+                    acc[t][1] = mfma_bf16(ef[t].lo, bih, acc[t][1]);
+                }
+#pragma unroll
+                for (int t = 0; t < RT; ++t) {
+                    acc[t][0] = mfma_bf16(ef[t].hi, brl, acc[t][0]);
+                    acc[t][1] = mfma_bf16(ef[t].hi, bil, acc[t][1]);
+                }
+#pragma unroll
+                for (int t = 0; t < RT; ++t) {
+                    acc[t][0] = mfma_bf16(ef[t].hi, brh, acc[t][0]);
+                    acc[t][1] = mfma_bf16(ef[t].hi, bih, acc[t][1]);
+                }
+            }
+        }
+    };
+    // T <- split(X): lane writes rows 4q..4q+3 of its tiles in column lr
+    auto store_t = [&](const f32x4 (&X)[RT][2]) {
+#pragma unroll
+        for (int t = 0; t < RT; ++t)
+#pragma unroll
+            for (int c = 0; c < 2; ++c) {
+                bf16x8 hl;  // [0..3] hi, [4..7] lo
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const __bf16 h = (__bf16)X[t][c][r];
+                    hl[r] = h;
+                    hl[4 + r] = (__bf16)(X[t][c][r] - (float)h);
+                }
+                const uint4 u = __builtin_bit_cast(uint4, hl);
+                const int k = (wave * RT + t) * 16 + 4 * q;
+                *reinterpret_cast<uint2*>(Tt + ((2 * c + 0) * 16 + lr) * TP + k) = make_uint2(u.x, u.y);
+                *reinterpret_cast<uint2*>(Tt + ((2 * c + 1) * 16 + lr) * TP + k) = make_uint2(u.z, u.w);
+            }
+    };
+
+    // ---- prologue: walker columns, accumulator rows and E in flight together ----
+    f32x4 acc[RT][2];
+    {
+        // T fill: 16 * nk * 32 elements = 8 per thread, element e = tid + i * nthreads
+        double2 wf[8];
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+            const int e = tid + i * nthreads;
+            const int p = e >> 4;
+            const int j = c0 + (e & 15);
+            wf[i] = load_w(p, j);
+        }
+        // accumulator element r of tile t: row (wave*RT + t)*16 + 4q + r, column col
+        double2 wa[RT][4];
+#pragma unroll
+        for (int t = 0; t < RT; ++t)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                wa[t][r] = load_w((wave * RT + t) * 16 + 4 * q + r, col);
+            }
+        float er[RT][KMAX][8];
+        load_e(er, row0, kq);
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+            const int e = tid + i * nthreads;
+            const int p = e >> 4;
+            const int jj = e & 15;
+            const float xr = (float)wf[i].x, xi = (float)wf[i].y;
+            const __bf16 hr = (__bf16)xr, hi = (__bf16)xi;
+            Tt[(0 * 16 + jj) * TP + p] = hr;
+            Tt[(1 * 16 + jj) * TP + p] = (__bf16)(xr - (float)hr);
+            Tt[(2 * 16 + jj) * TP + p] = hi;
+            Tt[(3 * 16 + jj) * TP + p] = (__bf16)(xi - (float)hi);
+        }
+#pragma unroll
+        for (int t = 0; t < RT; ++t)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                acc[t][0][r] = (float)wa[t][r].x;
+                acc[t][1][r] = (float)wa[t][r].y;
+            }
+        __syncthreads();
+        // ---- T0 = E W ----
+        e_product(acc, er, row0, kq);
+    }
+    f32x4 out[RT][2];
+#pragma unroll
+    for (int t = 0; t < RT; ++t) {
+        out[t][0] = acc[t][0];
+        out[t][1] = acc[t][1];
+    }
+    __syncthreads();
+    store_t(acc);
+
+    // ---- V resident: the LDS k-blocks first (one round trip, registers
+    // released after the LDS store), then the register k-blocks ----
+    if constexpr (NL > 0) {
+        float xl[RT][NL][2][8];
+#pragma unroll
+        for (int t = 0; t < RT; ++t)
+#pragma unroll
+            for (int l = 0; l < NL; ++l) {
+                const int k0 = (KR + l) * 32 + kq;
+                load_row8(Vr, ldv, n, row0 + 16 * t, k0, xl[t][l][0]);
+                load_row8(Vi, ldv, n, row0 + 16 * t, k0, xl[t][l][1]);
+            }
+#pragma unroll
+        for (int t = 0; t < RT; ++t)
+#pragma unroll
+            for (int l = 0; l < NL; ++l) {
+                uint4* dst = VL + ((size_t)((wave * RT + t) * NL + l) * 4) * 64 + lane;
+                const split8 a = split_f32x8(xl[t][l][0]);
+                const split8 c = split_f32x8(xl[t][l][1]);
+                dst[0 * 64] = __builtin_bit_cast(uint4, a.hi);
+                dst[1 * 64] = __builtin_bit_cast(uint4, a.lo);
+                dst[2 * 64] = __builtin_bit_cast(uint4, c.hi);
+                dst[3 * 64] = __builtin_bit_cast(uint4, c.lo);
+            }
+    }
+    split8 vr[RT][KR], vi[RT][KR];
+    {
+        float xv[RT][KR][2][8];
+#pragma unroll
+        for (int t = 0; t < RT; ++t)
+#pragma unroll
+            for (int kb = 0; kb < KR; ++kb) {
+                const int k0 = kb * 32 + kq;
+                load_row8(Vr, ldv, n, row0 + 16 * t, k0, xv[t][kb][0]);
+                load_row8(Vi, ldv, n, row0 + 16 * t, k0, xv[t][kb][1]);
+            }
+#pragma unroll
+        for (int t = 0; t < RT; ++t)
+#pragma unroll
+            for (int kb = 0; kb < KR; ++kb) {
+                vr[t][kb] = split_f32x8(xv[t][kb][0]);
+                vi[t][kb] = split_f32x8(xv[t][kb][1]);
+            }
+    }
+    __syncthreads();
+
+    // P += V . T for one k-block, A fragments (ar, ai) per row tile
+    auto vt_block = [&](f32x4 (&P)[RT][2], int kb, const split8 (&ar)[RT], const split8 (&ai)[RT]) {
+        const bf16x8 brh = read_b(kb, 0), brl = read_b(kb, 1);
+        bf16x8 bih = read_b(kb, 2), bil = read_b(kb, 3);
+        // re += Vr Tr, im += Vr Ti
+#pragma unroll
+        for (int t = 0; t < RT; ++t) {
+            P[t][0] = mfma_bf16(ar[t].lo, brh, P[t][0]);
+            P[t][1] = mfma_bf16(ar[t].lo, bih, P[t][1]);
+        }
+#pragma unroll
+        for (int t = 0; t < RT; ++t) {
+            P[t][0] = mfma_bf16(ar[t].hi, brl, P[t][0]);
+            P[t][1] = mfma_bf16(ar[t].hi, bil, P[t][1]);
+        }
+#pragma unroll
+        for (int t = 0; t < RT; ++t) {
+            P[t][0] = mfma_bf16(ar[t].hi, brh, P[t][0]);
+            P[t][1] = mfma_bf16(ar[t].hi, bih, P[t][1]);
+        }
+        // im += Vi Tr, re += Vi (-Ti)
+        bih = neg8(bih);
+        bil = neg8(bil);
+#pragma unroll
+        for (int t = 0; t < RT; ++t) {
+            P[t][1] = mfma_bf16(ai[t].lo, brh, P[t][1]);
+            P[t][0] = mfma_bf16(ai[t].lo, bih, P[t][0]);
+        }
+#pragma unroll
+        for (int t = 0; t < RT; ++t) {
+            P[t][1] = mfma_bf16(ai[t].hi, brl, P[t][1]);
+            P[t][0] = mfma_bf16(ai[t].hi, bil, P[t][0]);
+        }
+#pragma unroll
+        for (int t = 0; t < RT; ++t) {
+            P[t][1] = mfma_bf16(ai[t].hi, brh, P[t][1]);
+            P[t][0] = mfma_bf16(ai[t].hi, bih, P[t][0]);
+        }
+    };
+
+    // ---- Taylor terms: T_k = (i sqrt(dt) / k) V T_{k-1}, out += T_k ----
+#pragma unroll 1
+    for (int k = 1; k <= n_terms; ++k) {
+        f32x4 P[RT][2];
+#pragma unroll
+        for (int t = 0; t < RT; ++t) P[t][0] = P[t][1] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int kb = 0; kb < KR; ++kb) {
+            if (kb < nk) {
+                split8 ar[RT], ai[RT];
+#pragma unroll
+                for (int t = 0; t < RT; ++t) {
+                    ar[t] = vr[t][kb];
+                    ai[t] = vi[t][kb];
+                }
+                vt_block(P, kb, ar, ai);
+            }
+        }
+#pragma unroll
+        for (int l = 0; l < NL; ++l) {
+            split8 ar[RT], ai[RT];
+#pragma unroll
+            for (int t = 0; t < RT; ++t) {
+                const uint4* src = VL + ((size_t)((wave * RT + t) * NL + l) * 4) * 64 + lane;
+                ar[t].hi = __builtin_bit_cast(bf16x8, src[0 * 64]);
+                ar[t].lo = __builtin_bit_cast(bf16x8, src[1 * 64]);
+                ai[t].hi = __builtin_bit_cast(bf16x8, src[2 * 64]);
+                ai[t].lo = __builtin_bit_cast(bf16x8, src[3 * 64]);
+            }
+            vt_block(P, KR + l, ar, ai);
+        }
+        const float ck = (float)(sqrt_dt / (double)k);
+#pragma unroll
+        for (int t = 0; t < RT; ++t) {
+            const f32x4 re = -ck * P[t][1];
+            const f32x4 im = ck * P[t][0];
+            P[t][0] = re;
+            P[t][1] = im;
+            out[t][0] += re;
+            out[t][1] += im;
+        }
+        __syncthreads();
+        store_t(P);
+        __syncthreads();
+    }
+
+    // ---- W_out = E out = out + (E - I) out ----
+    // (indices recomputed from an opaque lane id: nothing address-like stays
+    // live across the Taylor loop)
+    const int lane2 = opaque(lane);
+    const int wave2 = opaque(wave);
+    {
+        float er[RT][KMAX][8];
+        load_e(er, wave2 * RT * 16 + (lane2 & 15), 8 * (lane2 >> 4));
+        __syncthreads();
+        store_t(out);
+        __syncthreads();
+        e_product(out, er, wave2 * RT * 16 + (lane2 & 15), 8 * (lane2 >> 4));
+    }
+    const int col2 = c0 + (lane2 & 15);
+    if (col2 < o) {
+#pragma unroll
+        for (int t = 0; t < RT; ++t)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int row = (wave2 * RT + t) * 16 + 4 * (lane2 >> 4) + r;
+                if (row < n) Wout[row * o + col2] = make_double2((double)out[t][0][r], (double)out[t][1][r]);
+            }
+    }
+}
+
+size_t bf16x3_lds(int n, int nl) {
+    const int NP = afq::round_up(n, 16);
+    const int nk = afq::ceil_div(NP, 32);
+    const int nwave = afq::ceil_div(NP / 16, RT);
+    return (size_t)4 * 16 * (nk * 32 + 8) * 2 + (size_t)nwave * RT * nl * 4 * 64 * 16;
+}
+
+template <int NL>
+int launch_bf16x3(int nw, int n, int o, int nspin, int ldv, int n_terms, double sqrt_dt,
+                  const float* E, const float* vhs, const double2* wu, const double2* wd, double2* ou,
+                  double2* od, hipStream_t st) {
+    const int NP = afq::round_up(n, 16);
+    const int nct = afq::ceil_div(o, 16);
+    const int nwave = afq::ceil_div(NP / 16, RT);
+    const size_t lds = bf16x3_lds(n, NL);
+    const int group_map = (nw % 8 == 0) ? 1 : 0;
+    if (lds > 64 * 1024) {
+        (void)hipFuncSetAttribute((const void*)trotter_bf16x3_kernel<NL>,
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        (void)hipGetLastError();
+    }
+    hipLaunchKernelGGL((trotter_bf16x3_kernel<NL>), dim3((unsigned)(nw * nspin * nct)), dim3(64 * nwave),
+                       lds, st, nw, n, o, nspin, nct, ldv, n_terms, sqrt_dt, E, vhs,
+                       (long long)nw * n * ldv, wu, wd, ou, od, group_map);
+    return afq::check_launch("afq_trotter_bf16x3");
+}
+
+}  // namespace
+
+// Declared in afqmc_hip.h.  Sizes beyond the on-chip capacity of the split
+// kernel (n > 224) go to the fp32 step, which is more accurate.
+extern "C" int afq_trotter_bf16x3(int nw, int n, int o, int nspin, int ldv, int n_terms,
+                                  double sqrt_dt, const float* exp_h1_half, const float* vhs,
+                                  const double* win_up, const double* win_dn, double* wout_up,
+                                  double* wout_dn, void* stream) {
+    const char* name = "afq_trotter_bf16x3";
+    AFQ_REQUIRE(nw > 0 && n > 0 && o > 0 && o <= n, "%s: bad sizes nw=%d n=%d o=%d", name, nw, n, o);
+    AFQ_REQUIRE(nspin == 1 || nspin == 2, "%s: nspin must be 1 or 2", name);
+    AFQ_REQUIRE(ldv >= n && ldv % 4 == 0, "%s: ldv=%d must be >= n and a multiple of 4", name, ldv);
+    AFQ_REQUIRE(n_terms >= 0, "%s: n_terms < 0", name);
+    AFQ_REQUIRE(exp_h1_half && vhs && win_up && wout_up, "%s: null pointer", name);
+    AFQ_REQUIRE(nspin == 1 || (win_dn && wout_dn), "%s: null spin-down pointer", name);
+    AFQ_REQUIRE(win_up != wout_up && (nspin == 1 || win_dn != wout_dn), "%s: output aliases input", name);
+    AFQ_REQUIRE((long long)nw * nspin * afq::ceil_div(o, 16) < (1LL << 31), "%s: too many walkers", name);
+    const int NP = afq::round_up(n, 16);
+    const int nk = afq::ceil_div(NP, 32);
+    const int nl = nk > KR ? nk - KR : 0;
+    if (nl > NL_MAX || bf16x3_lds(n, nl) > 160 * 1024)
+        return afq_trotter_f32(nw, n, o, nspin, ldv, n_terms, sqrt_dt, exp_h1_half, vhs, win_up, win_dn,
+                               wout_up, wout_dn, stream);
+    const double2* wu = (const double2*)win_up;
+    const double2* wd = (const double2*)win_dn;
+    double2* ou = (double2*)wout_up;
+    double2* od = (double2*)wout_dn;
+    hipStream_t st = (hipStream_t)stream;
+    if (nl == 0)
+        return launch_bf16x3<0>(nw, n, o, nspin, ldv, n_terms, sqrt_dt, exp_h1_half, vhs, wu, wd, ou, od, st);
+    if (nl == 1)
+        return launch_bf16x3<1>(nw, n, o, nspin, ldv, n_terms, sqrt_dt, exp_h1_half, vhs, wu, wd, ou, od, st);
+    return launch_bf16x3<2>(nw, n, o, nspin, ldv, n_terms, sqrt_dt, exp_h1_half, vhs, wu, wd, ou, od, st);
+}

@@ -232,9 +232,10 @@ def vhs(xbar, chol_t, n, vhs_out=None):
     return vhs_out
 
 
-def trotter(walkers, vhs_planes, exp_h1_half_pad, sqrt_dt, n_terms, out=None):
-    """Fused Trotter step; fp64 when vhs / exp_h1_half are float64, fp32 when
-    both are float32 (walkers are complex128 either way)."""
+def trotter(walkers, vhs_planes, exp_h1_half_pad, sqrt_dt, n_terms, out=None, arith=None):
+    """Fused Trotter step; fp64 when vhs / exp_h1_half are float64.  With
+    float32 operands ``arith`` picks the products: "fp32" (default) or
+    "bf16x3" (split-bf16 MFMAs, fp32 sums).  Walkers are complex128 either way."""
     wu, wd = (walkers if isinstance(walkers, tuple) else (walkers, None))
     nspin = 1 if wd is None else 2
     nw, n, o = wu.shape
@@ -246,12 +247,17 @@ def trotter(walkers, vhs_planes, exp_h1_half_pad, sqrt_dt, n_terms, out=None):
     rt = vhs_planes.dtype
     if rt not in (f64, f32):
         raise TypeError("vhs must be float64 or float32")
+    if arith is None:
+        arith = "fp64" if rt == f64 else "fp32"
+    if (rt == f64) != (arith == "fp64") or arith not in ("fp64", "fp32", "bf16x3"):
+        raise ValueError(f"arith={arith!r} does not match {rt} operands")
     _check(vhs_planes, rt, (2, nw, n, ldv), "vhs", dev)
     _check(exp_h1_half_pad, rt, (n, ldv), "exp_h1_half", dev)
     if out is None:
         out = (torch.empty_like(wu), torch.empty_like(wd)) if nspin == 2 else torch.empty_like(wu)
     ou, od = (out if isinstance(out, tuple) else (out, None))
-    _call("afq_trotter" if rt == f64 else "afq_trotter_f32", wu, nw, n, o, nspin, ldv,
+    entry = {"fp64": "afq_trotter", "fp32": "afq_trotter_f32", "bf16x3": "afq_trotter_bf16x3"}[arith]
+    _call(entry, wu, nw, n, o, nspin, ldv,
           int(n_terms), float(sqrt_dt),
               exp_h1_half_pad.data_ptr(), vhs_planes.data_ptr(), wu.data_ptr(), _ptr(wd),
               ou.data_ptr(), _ptr(od), _stream(wu))

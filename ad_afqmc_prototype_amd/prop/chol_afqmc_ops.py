@@ -144,13 +144,13 @@ def make_chol_afqmc_ops(ham_data: ham_chol, walker_kind: str,
         w0 = w[0] if unr else w
         nw = w0.shape[0]
         if n_chunks == 1:
-            return kernels.trotter(w, make_vhs(xbar), e_half(ctx), ctx.sqrt_dt, n_terms)
+            return kernels.trotter(w, make_vhs(xbar), e_half(ctx), ctx.sqrt_dt, n_terms, arith=precision)
         out = (torch.empty_like(w[0]), torch.empty_like(w[1])) if unr else torch.empty_like(w0)
         for a, b in _chunks(nw, n_chunks):
             v = make_vhs(xbar[:, a:b].contiguous())
             wc = (w[0][a:b], w[1][a:b]) if unr else w0[a:b]
             oc = (out[0][a:b], out[1][a:b]) if unr else out[a:b]
-            kernels.trotter(wc, v, e_half(ctx), ctx.sqrt_dt, n_terms, out=oc)
+            kernels.trotter(wc, v, e_half(ctx), ctx.sqrt_dt, n_terms, out=oc, arith=precision)
         return out
 
     def apply_trotter(w, fields, ctx: chol_afqmc_ctx, n_terms: int, n_chunks: int = 1):

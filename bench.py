@@ -38,7 +38,7 @@ def flops_per_walker_step(n, o, g, nspin, rhf_trial):
                 total=vhs + fb + taylor + one_body + greens)
 
 
-def build(workload: str, device, precision: str = "fp64", chol_scale: float = 1.0):
+def build(workload: str, device, precision: str = "fp64", chol_scale: float | None = None):
     import torch
 
     from ad_afqmc_prototype_amd import synthetic as syn
@@ -53,7 +53,7 @@ def build(workload: str, device, precision: str = "fp64", chol_scale: float = 1.
 
     cfg = syn.WORKLOADS[workload]
     s = syn.make_system(cfg["seed"], cfg["norb"], cfg["nocc"], cfg["naux"], uhf=cfg["uhf"],
-                        chol_scale=chol_scale)
+                        chol_scale=cfg["chol_scale"] if chol_scale is None else chol_scale)
     kind = cfg["walker_kind"]
     sys_ = system(norb=s.norb, nelec=(s.nocc, s.nocc), walker_kind=kind)
     ham = ham_chol.from_arrays(s.h0, s.h1, s.chol, device=device)
@@ -121,7 +121,8 @@ def cpu_baseline(workload: str, budget_s: float = 20.0):
     except Exception:  # pragma: no cover
         threads = os.cpu_count() or 1
     cfg = syn.WORKLOADS[workload]
-    s = syn.make_system(cfg["seed"], cfg["norb"], cfg["nocc"], cfg["naux"], uhf=cfg["uhf"])
+    s = syn.make_system(cfg["seed"], cfg["norb"], cfg["nocc"], cfg["naux"], uhf=cfg["uhf"],
+                        chol_scale=cfg["chol_scale"])
     kind = cfg["walker_kind"]
     sd = orc.make_system_data(s.h0, s.h1, s.chol, (s.mo_up, s.mo_dn), kind, cfg["dt"])
     prm = orc.params(dt=cfg["dt"])
@@ -145,29 +146,43 @@ def cpu_baseline(workload: str, budget_s: float = 20.0):
 
 def accuracy_check(workload: str, precision: str, device, chol_scale: float, n_blocks: int):
     """Block energies of the mixed and the fp64 step under common random
-    numbers (same fields, same comb uniforms, same initial state) on the
-    workload's shapes with the Cholesky vectors scaled by chol_scale, so that
-    the walk stays in the working regime of the order-6 Taylor step."""
+    numbers (same fields, same comb uniforms, same initial state) over
+    n_blocks blocks of the workload, its Cholesky vectors scaled by chol_scale.
+
+    ``resync``: every mixed block starts from the fp64 state at the start of
+    that block, so the difference is the precision error of one block (the
+    20 steps, QR, and the measurement).  ``free``: the mixed run continues
+    from its own state.  The comb reconfiguration is discontinuous in the
+    weights: once a weight differing in the last bits lands on the other
+    side of a comb tooth, a different walker is copied, the two runs become
+    different samples and their difference grows to the statistical noise of
+    a block."""
     import torch
 
-    res = {}
-    for prec in ("fp64", precision):
-        ctx = build(workload, device, prec, chol_scale)
-        cfg = ctx["cfg"]
-        gen = torch.Generator(device=device)
-        gen.manual_seed(cfg["seed"] + 1000)
-        steps = n_blocks * cfg["sr_every"]
-        fields = [torch.randn((cfg["n_walkers"], cfg["naux"]), dtype=torch.float64, device=device,
-                              generator=gen) for _ in range(steps)]
-        energies = []
-        run_steps(ctx, ctx["state"], fields, 0, steps, cadence=True, energies=energies)
-        res[prec] = [float(e.item()) for e in energies]
-        del ctx, fields
-        torch.cuda.empty_cache()
-    diffs = [1e3 * abs(a - b) for a, b in zip(res[precision], res["fp64"])]
+    ctx64 = build(workload, device, "fp64", chol_scale)
+    ctxmx = build(workload, device, precision, chol_scale)
+    cfg = ctx64["cfg"]
+    gen = torch.Generator(device=device)
+    gen.manual_seed(cfg["seed"] + 1000)
+    sr = cfg["sr_every"]
+    fields = [torch.randn((cfg["n_walkers"], cfg["naux"]), dtype=torch.float64, device=device,
+                          generator=gen) for _ in range(n_blocks * sr)]
+    st64, stfree = ctx64["state"], ctxmx["state"]
+    e64, eres, efree = [], [], []
+    for b in range(n_blocks):
+        start = st64
+        st64 = run_steps(ctx64, st64, fields, b * sr, sr, cadence=True, energies=e64)
+        run_steps(ctxmx, start, fields, b * sr, sr, cadence=True, energies=eres)
+        stfree = run_steps(ctxmx, stfree, fields, b * sr, sr, cadence=True, energies=efree)
+    e64, eres, efree = ([float(e.item()) for e in x] for x in (e64, eres, efree))
+    del ctx64, ctxmx, fields, st64, stfree, start
+    torch.cuda.empty_cache()
+    d_res = [1e3 * abs(a - b) for a, b in zip(eres, e64)]
+    d_free = [1e3 * abs(a - b) for a, b in zip(efree, e64)]
     return {"chol_scale": chol_scale, "n_blocks": n_blocks, "precision": precision,
-            "block_energies_fp64_Ha": res["fp64"], "block_energies_mixed_Ha": res[precision],
-            "abs_diff_mHa": diffs, "max_abs_diff_mHa": max(diffs) if diffs else None}
+            "block_energies_fp64_Ha": e64, "block_energies_mixed_resync_Ha": eres,
+            "resync_abs_diff_mHa": d_res, "max_resync_abs_diff_mHa": max(d_res) if d_res else None,
+            "block_energies_mixed_free_Ha": efree, "free_abs_diff_mHa": d_free}
 
 
 def measured_traffic(workload: str, kernel: str):
@@ -201,6 +216,7 @@ def kernel_table(ktimes, n, o, g, nw, nspin, rhf):
         "afq_gemm_bf16x3_nt:force_bias": (fl["force_bias"] * nw, "bf16x3"),
         "afq_trotter": (taylor, "fp64"),
         "afq_trotter_f32": (taylor, "fp32"),
+        "afq_trotter_bf16x3": (taylor, "bf16x3"),
         "afq_overlap_greens": (fl["greens"] * nw, "fp64"),
         "afq_energy": (4.0 * g * o * o * n * nspin * nw, "fp64"),
     }
@@ -258,8 +274,9 @@ def main():
     ap.add_argument("--no-fp64-compare", action="store_true",
                     help="skip the fp64 run that the mixed throughput is compared with")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--accuracy-scale", type=float, default=0.03,
-                    help="Cholesky scale of the common-random-number energy check (0 skips it)")
+    ap.add_argument("--accuracy-scale", type=float, default=None,
+                    help="Cholesky scale of the common-random-number energy check (default: the "
+                         "workload's own; 0 skips it)")
     ap.add_argument("--accuracy-blocks", type=int, default=4)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run a plain device copy of BYTES bytes 5 times (the known-size "
@@ -299,7 +316,8 @@ def main():
     value = nw * args.steps / elapsed
     flops = flops_per_walker_step(n, o, g, nspin, not cfg["uhf"])
     dtype = {"fp64": "f64", "fp32": "f32 contractions + f32 Trotter, f64 LU/QR/weights/energy",
-             "bf16x3": "bf16x3 contractions + f32 Trotter, f64 LU/QR/weights/energy"}[args.precision]
+             "bf16x3": "bf16x3 contractions and Trotter products (f32 sums), f64 LU/QR/weights/"
+                       "energy sums"}[args.precision]
     out = {
         "metric": "walker-steps/s at N=200,naux=1000: mixed-prec vs FP64 GPU, |dE|<0.1 mHa",
         "value": value,
@@ -314,7 +332,8 @@ def main():
         "dtype": dtype,
         "data": "synthetic",
         "config": {"workload": args.workload, "norb": n, "nocc": o, "naux": g, "n_walkers": nw,
-                   "walker_kind": cfg["walker_kind"], "dt": cfg["dt"], "precision": args.precision,
+                   "walker_kind": cfg["walker_kind"], "dt": cfg["dt"], "chol_scale": cfg["chol_scale"],
+                   "precision": args.precision,
                    "qr_every": None if args.no_cadence else cfg["qr_every"],
                    "measure_sr_every": None if args.no_cadence else cfg["sr_every"]},
         "roofline": {"bound": "mfma", "kernel": dominant, "achieved": dom["tflops"],
@@ -331,9 +350,10 @@ def main():
         out["speedup_vs_fp64"] = out["value"] / out["fp64_value"]
         out["fp64_kernels"] = kernel_table(kt64, n, o, g, nw, nspin, not cfg["uhf"])
         out["block_energy_abs_diff_mHa"] = [1e3 * abs(a - b) for a, b in zip(e_block, e64)]
-    if args.precision != "fp64" and args.accuracy_scale > 0:
-        out["energy_check"] = accuracy_check(args.workload, args.precision, device,
-                                             args.accuracy_scale, args.accuracy_blocks)
+    if args.precision != "fp64" and args.accuracy_scale != 0:
+        scale = cfg["chol_scale"] if args.accuracy_scale is None else args.accuracy_scale
+        out["energy_check"] = accuracy_check(args.workload, args.precision, device, scale,
+                                             args.accuracy_blocks)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(args.workload)
     print(json.dumps(out), flush=True)

@@ -146,6 +146,17 @@ int afq_trotter_f32(int nw, int n, int o, int nspin, int ldv, int n_terms, doubl
                     double* wout_up, double* wout_dn, void* stream);
 
 /*
+ * afq_trotter_f32 with every product on split-bf16 MFMAs (x = hi + lo, three
+ * bf16 products accumulated in fp32), E applied as X + (E - I) X with X added
+ * exactly; running sums in fp32.  Same arguments as afq_trotter_f32, same
+ * reference functions.  n > 224 runs the fp32 step instead.
+ */
+int afq_trotter_bf16x3(int nw, int n, int o, int nspin, int ldv, int n_terms, double sqrt_dt,
+                       const float* exp_h1_half, const float* vhs,
+                       const double* win_up, const double* win_dn,
+                       double* wout_up, double* wout_dn, void* stream);
+
+/*
  * Phaseless importance, cos-phase cut, floor/cap and population-control shift
  * of prop/afqmc.py:116-138, on log-overlaps:
  *   log ovlp_new = (nspin == 1 ? 2 : 1) * sum_s logdet_new[w, s]

@@ -156,3 +156,79 @@ def test_energy_fp32_contractions(gpu_device, case):
     e_ref = orc.energy(w, p.sd.h0, p.sd.trial, p.sd.mctx, case["kind"])
     err = np.abs(e - e_ref) / np.abs(e_ref)
     assert err.max() < 2e-5, err.max()
+
+
+def _trotter_inputs(rng, nw, n, o, nspin, dt, vnorm):
+    """Complex-symmetric VHS (as sum_g xbar_g L_g is) with sqrt(dt) ||V_w||_2 =
+    vnorm, E = expm(-dt h1 / 2) for a symmetric h1 of norm ~2, walkers."""
+    import scipy.linalg
+    h1 = rng.standard_normal((n, n)) / np.sqrt(n)
+    h1 = h1 + h1.T
+    e = scipy.linalg.expm(-0.5 * dt * h1)
+    a = rng.standard_normal((nw, n, n)) + 1j * rng.standard_normal((nw, n, n))
+    v = 0.5 * (a + np.transpose(a, (0, 2, 1)))
+    v *= vnorm / (np.sqrt(dt) * np.linalg.norm(v, 2, axis=(1, 2)))[:, None, None]
+    ws = [rng.standard_normal((nw, n, o)) + 1j * rng.standard_normal((nw, n, o)) for _ in range(nspin)]
+    return e, v, ws
+
+
+def _planes(e, v, ldv):
+    n = e.shape[0]
+    ep = np.zeros((n, ldv))
+    ep[:, :n] = e
+    vp = np.zeros((2, v.shape[0], n, ldv))
+    vp[0, :, :, :n] = v.real
+    vp[1, :, :, :n] = v.imag
+    return ep, vp
+
+
+@pytest.mark.parametrize("nw,n,o,nspin", [
+    (8, 24, 6, 1),     # all of V in registers
+    (5, 100, 20, 2),   # odd row-tile count, o not a multiple of 16, nw not a multiple of 8
+    (8, 180, 32, 1),   # one k-block of V in LDS
+    (16, 200, 32, 1),  # two k-blocks in LDS (the 200-orbital shape)
+    (3, 224, 17, 2),   # largest on-chip size
+    (2, 250, 8, 1),    # beyond on-chip capacity: the fp32 step runs
+])
+def test_trotter_bf16x3_vs_fp64(gpu_device, nw, n, o, nspin):
+    """Split-bf16 Trotter step against the fp64 step (pinned to the checker)
+    on the same E, VHS and walkers, sqrt(dt) ||V||_2 = 0.3, dt = 0.005.  The
+    split (~2^-17 relative) enters through sqrt(dt) V / k and E - I, so each
+    walker is within 2e-5 (norm-wise relative); the fp32 step within 2e-6."""
+    import torch
+    from ad_afqmc_prototype_amd import kernels
+    rng = np.random.default_rng(n + 7 * o + nspin)
+    dt = 0.005
+    e, v, ws = _trotter_inputs(rng, nw, n, o, nspin, dt, 0.3)
+    ldv = kernels.ldv_for(n)
+    ep, vp = _planes(e, v, ldv)
+    ep64, vp64 = torch.tensor(ep, device=gpu_device), torch.tensor(vp, device=gpu_device)
+    ep32, vp32 = ep64.float(), vp64.float()
+    wd = tuple(torch.tensor(x, device=gpu_device) for x in ws)
+    wd = wd if nspin == 2 else wd[0]
+    ref = to_np(kernels.trotter(wd, vp64, ep64, dt ** 0.5, 6))
+    got = to_np(kernels.trotter(wd, vp32, ep32, dt ** 0.5, 6, arith="bf16x3"))
+    f32 = to_np(kernels.trotter(wd, vp32, ep32, dt ** 0.5, 6, arith="fp32"))
+    refs, gots, f32s = [x if nspin == 2 else (x,) for x in (ref, got, f32)]
+    for r, g, f in zip(refs, gots, f32s):
+        err = max(rel(g[i], r[i]) for i in range(nw))
+        assert err < 2e-5, err
+        err32 = max(rel(f[i], r[i]) for i in range(nw))
+        assert err32 < 2e-6, err32
+
+
+def test_trotter_bf16x3_exact_identity(gpu_device):
+    """E = I and V = 0: E - I is exactly zero, so the step returns the fp32
+    rounding of the walkers bit for bit (X is added exactly, not split)."""
+    import torch
+    from ad_afqmc_prototype_amd import kernels
+    rng = np.random.default_rng(11)
+    nw, n, o = 8, 200, 32
+    ldv = kernels.ldv_for(n)
+    w = rng.standard_normal((nw, n, o)) + 1j * rng.standard_normal((nw, n, o))
+    ep = np.zeros((n, ldv), dtype=np.float32)
+    ep[:, :n] = np.eye(n)
+    v = torch.zeros((2, nw, n, ldv), dtype=torch.float32, device=gpu_device)
+    out = kernels.trotter(torch.tensor(w, device=gpu_device), v, torch.tensor(ep, device=gpu_device),
+                          0.07, 6, arith="bf16x3").cpu().numpy()
+    np.testing.assert_array_equal(out, w.astype(np.complex64).astype(np.complex128))
