@@ -150,6 +150,143 @@ __global__ __launch_bounds__(THREADS, 2) void gemm_bf16x3_nt_kernel(
         }
 }
 
+// Large-problem variant: 256 x 256 x 32 per 512-thread workgroup (4 x 2
+// waves of 64 x 128), so each LDS fragment feeds twice the MFMAs of the
+// 128 x 128 tile.  The four operand planes of a k-step (64 KB) go from global
+// memory straight to LDS (buffer_load ... lds, 16 bytes per lane, no staging
+// registers), double-buffered (128 KB, one workgroup per CU).  Rows are 64
+// bytes, unpadded; the 16-byte chunk c of row r is stored at position
+// c ^ ((r >> 1) & 3), which makes the fragment reads conflict-free.  Rows past
+// M / N and k-chunks past K fall outside the buffer descriptors and read as
+// zero.
+constexpr int BM2 = 256;
+constexpr int BN2 = 256;
+constexpr int THREADS2 = 512;
+constexpr int GROUP_M2 = 8;
+constexpr unsigned kOutside = 0x80000000u;
+
+typedef __amdgpu_buffer_rsrc_t rsrc_t;
+typedef __attribute__((address_space(3))) void* lds_ptr_t;
+
+__device__ __forceinline__ rsrc_t make_rsrc(const void* p, unsigned long long bytes) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), (short)0,
+                                            (int)(bytes > 0x7fffffffull ? 0x7fffffffull : bytes), 0x00020000);
+}
+
+template <typename TOut>
+__global__ __launch_bounds__(THREADS2, 1) void gemm_bf16x3_nt_256_kernel(
+    int M, int N, int K, float alpha, const uint16_t* __restrict__ Ahi,
+    const uint16_t* __restrict__ Alo, long long lda, const uint16_t* __restrict__ Bhi,
+    const uint16_t* __restrict__ Blo, long long ldb, TOut* __restrict__ C, long long ldc,
+    int tiles_m, int tiles_n) {
+    // [stage][plane: A hi, A lo, B hi, B lo][256 rows x 64 bytes]
+    __shared__ __attribute__((aligned(1024))) uint4 S[2][4][BM2 * 4];
+
+    const int total = tiles_m * tiles_n;
+    const int t = afq::xcd_contiguous(blockIdx.x, total);
+    const int per_group = GROUP_M2 * tiles_n;
+    const int group = t / per_group;
+    const int first_m = group * GROUP_M2;
+    const int gsize = min(tiles_m - first_m, GROUP_M2);
+    const int in_group = t - group * per_group;
+    const int m0 = (first_m + in_group % gsize) * BM2;
+    const int n0 = (in_group / gsize) * BN2;
+
+    const int tid = threadIdx.x;
+    const int lane = tid & 63;
+    // wave-uniform (scalar): the descriptors and LDS bases below must be
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int wm = wave & 3;   // 64-row slice of the tile
+    const int wn = wave >> 2;  // 128-column slice
+
+    // copy assignment: wave w moves plane w / 2 (A hi, A lo, B hi, B lo), 1 KB
+    // blocks 8 (w % 2) + i, i < 8 (rows 16 blk ..); lane L fills slot L of a
+    // block: row L / 4, position L % 4, i.e. chunk (L % 4) ^ ((L / 8) & 3)
+    const int plane = wave >> 1;
+    const uint16_t* src = plane == 0 ? Ahi + (long long)m0 * lda
+                        : plane == 1 ? Alo + (long long)m0 * lda
+                        : plane == 2 ? Bhi + (long long)n0 * ldb
+                                     : Blo + (long long)n0 * ldb;
+    const long long ld = plane < 2 ? lda : ldb;
+    const int rows_left = max(plane < 2 ? M - m0 : N - n0, 0);
+    const rsrc_t R = make_rsrc(src, (unsigned long long)rows_left * ld * 2);
+    const int r_loc = lane >> 2;
+    const int chunk = (lane & 3) ^ ((lane >> 3) & 3);
+    auto issue_stage = [&](int kt, int buf) {
+        const int k = kt * 32 + chunk * 8;
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+            const int blk = (wave & 1) * 8 + i;
+            const int row = blk * 16 + r_loc;
+            // k >= K sets the top bit: the offset lands outside the descriptor
+            const unsigned past = (unsigned)((K - 1 - k) >> 31) & kOutside;
+            const unsigned off = (unsigned)(((long long)row * ld + k) * 2) | past;
+            __builtin_amdgcn_raw_ptr_buffer_load_lds(R, (lds_ptr_t)&S[buf][plane][blk * 64], 16, off, 0, 0, 0);
+        }
+    };
+
+    f32x4 acc[4][8];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 8; ++j) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
+
+    // fragment read: row lr of a 16-row tile, chunk q at its swizzled position
+    const int lr = lane & 15;
+    const int q = lane >> 4;
+    const int lane_slot = lr * 4 + (q ^ ((lr >> 1) & 3));
+
+    const int nk = afq::ceil_div(K, 32);
+    issue_stage(0, 0);
+    asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+    __syncthreads();
+    for (int kt = 0; kt < nk; ++kt) {
+        const int buf = kt & 1;
+        if (kt + 1 < nk) issue_stage(kt + 1, buf ^ 1);
+        bf16x8 ah[4], al[4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const int slot = (wm * 64 + i * 16) * 4 + lane_slot;
+            ah[i] = __builtin_bit_cast(bf16x8, S[buf][0][slot]);
+            al[i] = __builtin_bit_cast(bf16x8, S[buf][1][slot]);
+        }
+#pragma unroll
+        for (int jh = 0; jh < 2; ++jh) {
+            bf16x8 bh[4], bl[4];
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const int slot = (wn * 128 + (jh * 4 + j) * 16) * 4 + lane_slot;
+                bh[j] = __builtin_bit_cast(bf16x8, S[buf][2][slot]);
+                bl[j] = __builtin_bit_cast(bf16x8, S[buf][3][slot]);
+            }
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    f32x4& c = acc[i][jh * 4 + j];
+                    c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[i], bh[j], c, 0, 0, 0);
+                    c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bl[j], c, 0, 0, 0);
+                    c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bh[j], c, 0, 0, 0);
+                }
+        }
+        // the next stage has landed in LDS and every wave is done with this one
+        asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+        __syncthreads();
+    }
+
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            const int col = n0 + wn * 128 + j * 16 + lr;
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int row = m0 + wm * 64 + i * 16 + q * 4 + r;
+                if (row < M && col < N) C[(long long)row * ldc + col] = (TOut)(alpha * acc[i][j][r]);
+            }
+        }
+}
+
 __global__ void split_bf16_kernel(int rows, int cols, const double* __restrict__ src,
                                   long long ld_src, uint16_t* __restrict__ hi,
                                   uint16_t* __restrict__ lo, long long ld_dst) {
@@ -194,11 +331,26 @@ extern "C" int afq_gemm_bf16x3_nt(int M, int N, int K, float alpha, const unsign
     AFQ_REQUIRE(K % 8 == 0 && lda % 8 == 0 && ldb % 8 == 0,
                 "afq_gemm_bf16x3_nt: K=%d, lda and ldb must be multiples of 8", K);
     AFQ_REQUIRE(lda >= K && ldb >= K && ldc >= N, "afq_gemm_bf16x3_nt: leading dimension too small");
+    hipStream_t st = (hipStream_t)stream;
+    // 256 x 256 tiles once they give at least two workgroups per CU
+    const int tiles_m2 = afq::ceil_div(M, BM2);
+    const int tiles_n2 = afq::ceil_div(N, BN2);
+    const long long total2 = (long long)tiles_m2 * tiles_n2;
+    if (total2 >= 512 && total2 < (1LL << 31)) {
+        if (c_fp64)
+            hipLaunchKernelGGL(gemm_bf16x3_nt_256_kernel<double>, dim3((unsigned)total2), dim3(THREADS2), 0, st,
+                               M, N, K, alpha, A_hi, A_lo, lda, B_hi, B_lo, ldb, (double*)C, ldc, tiles_m2,
+                               tiles_n2);
+        else
+            hipLaunchKernelGGL(gemm_bf16x3_nt_256_kernel<float>, dim3((unsigned)total2), dim3(THREADS2), 0, st,
+                               M, N, K, alpha, A_hi, A_lo, lda, B_hi, B_lo, ldb, (float*)C, ldc, tiles_m2,
+                               tiles_n2);
+        return afq::check_launch("afq_gemm_bf16x3_nt");
+    }
     const int tiles_m = afq::ceil_div(M, BM);
     const int tiles_n = afq::ceil_div(N, BN);
     const long long total = (long long)tiles_m * tiles_n;
     AFQ_REQUIRE(total < (1LL << 31), "afq_gemm_bf16x3_nt: problem too large");
-    hipStream_t st = (hipStream_t)stream;
     if (c_fp64)
         hipLaunchKernelGGL(gemm_bf16x3_nt_kernel<double>, dim3((unsigned)total), dim3(THREADS), 0, st,
                            M, N, K, alpha, A_hi, A_lo, lda, B_hi, B_lo, ldb, (double*)C, ldc,

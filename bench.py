@@ -74,10 +74,10 @@ def build(workload: str, device, precision: str = "fp64", chol_scale: float | No
                 p_ops=p_ops, params=params, p_ctx=p_ctx, m_ctx=m_ctx, state=state)
 
 
-def run_steps(ctx, state, fields, t0, k, cadence=True, energies=None):
+def run_steps(ctx, state, fields, t0, k, cadence=True, energies=None, comb=None):
     """k steps starting at global step index t0 with the QR / measurement +
     reconfiguration cadence of the workload (block energies appended to
-    ``energies`` as device scalars)."""
+    ``energies`` as device scalars, comb indices to ``comb``)."""
     from ad_afqmc_prototype_amd import synthetic as syn
     from ad_afqmc_prototype_amd.prop.afqmc import afqmc_step
     from ad_afqmc_prototype_amd.prop.blocks import measure, orthonormalize_state, reconfigure
@@ -97,9 +97,11 @@ def run_steps(ctx, state, fields, t0, k, cadence=True, energies=None):
                                    trial_data=ctx["trial"], meas_ops=ctx["m_ops"], meas_ctx=ctx["m_ctx"])
             if energies is not None:
                 energies.append(sc["energy"])
-            state, _ = reconfigure(state, sys=ctx["sys"], trial_data=ctx["trial"],
-                                   meas_ops_=ctx["m_ops"],
-                                   zeta=syn.zeta_for_event(cfg["seed"], (t + 1) // cfg["sr_every"]))
+            state, idx = reconfigure(state, sys=ctx["sys"], trial_data=ctx["trial"],
+                                     meas_ops_=ctx["m_ops"],
+                                     zeta=syn.zeta_for_event(cfg["seed"], (t + 1) // cfg["sr_every"]))
+            if comb is not None:
+                comb.append(idx)
         elif (t + 1) % cfg["qr_every"] == 0:
             state = orthonormalize_state(state, sys=ctx["sys"], trial_data=ctx["trial"],
                                          meas_ops_=ctx["m_ops"])
@@ -168,21 +170,24 @@ def accuracy_check(workload: str, precision: str, device, chol_scale: float, n_b
     fields = [torch.randn((cfg["n_walkers"], cfg["naux"]), dtype=torch.float64, device=device,
                           generator=gen) for _ in range(n_blocks * sr)]
     st64, stfree = ctx64["state"], ctxmx["state"]
-    e64, eres, efree = [], [], []
+    e64, eres, efree, c64, cfree = [], [], [], [], []
     for b in range(n_blocks):
         start = st64
-        st64 = run_steps(ctx64, st64, fields, b * sr, sr, cadence=True, energies=e64)
+        st64 = run_steps(ctx64, st64, fields, b * sr, sr, cadence=True, energies=e64, comb=c64)
         run_steps(ctxmx, start, fields, b * sr, sr, cadence=True, energies=eres)
-        stfree = run_steps(ctxmx, stfree, fields, b * sr, sr, cadence=True, energies=efree)
+        stfree = run_steps(ctxmx, stfree, fields, b * sr, sr, cadence=True, energies=efree, comb=cfree)
     e64, eres, efree = ([float(e.item()) for e in x] for x in (e64, eres, efree))
-    del ctx64, ctxmx, fields, st64, stfree, start
+    # walkers whose comb pick differs between the fp64 and the free-running mixed run
+    comb_diff = [int((a != b).sum().item()) for a, b in zip(c64, cfree)]
+    del ctx64, ctxmx, fields, st64, stfree, start, c64, cfree
     torch.cuda.empty_cache()
     d_res = [1e3 * abs(a - b) for a, b in zip(eres, e64)]
     d_free = [1e3 * abs(a - b) for a, b in zip(efree, e64)]
     return {"chol_scale": chol_scale, "n_blocks": n_blocks, "precision": precision,
             "block_energies_fp64_Ha": e64, "block_energies_mixed_resync_Ha": eres,
             "resync_abs_diff_mHa": d_res, "max_resync_abs_diff_mHa": max(d_res) if d_res else None,
-            "block_energies_mixed_free_Ha": efree, "free_abs_diff_mHa": d_free}
+            "block_energies_mixed_free_Ha": efree, "free_abs_diff_mHa": d_free,
+            "free_comb_picks_differing": comb_diff}
 
 
 def measured_traffic(workload: str, kernel: str):

@@ -39,6 +39,27 @@ def test_sgemm_and_bf16x3_gemm(gpu_device, M, N, K):
     assert_rel(x[:, :K].numpy(), A.numpy(), 2e-5)
 
 
+@pytest.mark.parametrize("M,N,K,c64", [(4100, 8200, 1000, False), (4096, 8192, 968, True)])
+def test_bf16x3_gemm_large_tiles(gpu_device, M, N, K, c64):
+    """The 256 x 256 tile path (taken at >= 512 tiles) with partial M / N tiles
+    and a partial last k-step, against fp64: same 1e-4 bound as above."""
+    import torch
+    from ad_afqmc_prototype_amd import kernels
+    g = torch.Generator(device=gpu_device).manual_seed(M + K)
+    A = torch.randn((M, K), dtype=torch.float64, device=gpu_device, generator=g)
+    B = torch.randn((N, K), dtype=torch.float64, device=gpu_device, generator=g)
+    ref = A @ B.T
+    ah, al = kernels.split_bf16(A)
+    bh, bl = kernels.split_bf16(B)
+    C = torch.full((M, N), float("nan"), dtype=torch.float64 if c64 else torch.float32, device=gpu_device)
+    kernels.gemm_bf16x3_nt(ah, al, bh, bl, C, alpha=0.5)
+    err = (torch.linalg.norm(C.double() - 0.5 * ref) / torch.linalg.norm(0.5 * ref)).item()
+    assert err < 1e-4, err
+    # every element written, and the largest single-element error is small too
+    assert torch.isfinite(C).all()
+    assert ((C.double() - 0.5 * ref).abs().max() / (0.5 * ref).abs().max()).item() < 1e-4
+
+
 def test_split_bf16_zero_padding_and_plain_bf16(gpu_device):
     import torch
     from ad_afqmc_prototype_amd import kernels
