@@ -43,6 +43,8 @@ _SIGNATURES = {
     "afq_orthonormalize": [c_int, c_int, c_int, c_vp, c_vp, c_vp],
     "afq_energy": [c_int, c_int, c_int, c_int, c_int, c_double, c_vp, c_vp, c_int, c_vp, c_vp, c_vp],
     "afq_energy_f32": [c_int, c_int, c_int, c_int, c_int, c_double, c_vp, c_vp, c_int, c_vp, c_vp, c_vp],
+    "afq_energy_bf16x3": [c_int, c_int, c_int, c_int, c_int, c_double, c_vp, c_vp, c_vp, c_int, c_vp,
+                          c_vp, c_vp],
     "afq_block_energy": [c_int, c_vp, c_vp, c_double, c_double, c_double, c_vp, c_vp, c_vp],
     "afq_comb_indices": [c_int, c_vp, c_double, c_vp, c_vp, c_vp, c_vp],
     "afq_gather_rows": [c_int, c_ll, c_int, c_ll, c_ll, c_vp, c_vp, c_vp, c_vp],

@@ -45,6 +45,10 @@ class meas_ops:
     observables: Mapping[str, meas_kernel] = field(default_factory=dict)
     log_overlap: Optional[Callable[[Any, Any], Any]] = None
     greens: Optional[Callable[[Any, Any], Any]] = None
+    # (walkers, meas_ctx, trial_data, greens) -> force bias as float64 planes
+    # (2, nw, naux): the same values as the force_bias kernel without the
+    # complex tensor in between (the step consumes planes)
+    force_bias_planes: Optional[Callable[..., Any]] = None
 
     def has_kernel(self, name: str) -> bool:
         return name in self.kernels

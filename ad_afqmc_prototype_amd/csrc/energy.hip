@@ -284,6 +284,263 @@ int energy_entry(const char* name, int nw, int n, int o, int nspin, int naux, do
     }
 }
 
+// ---------------------------------------------------------------------------
+// Split-bf16 variant: f_g = R_g G^T with both operands carried as bf16
+// hi + lo and three v_mfma_f32_16x16x32_bf16 per k-block (fp32 sums within a
+// g, fp64 sums over g).  G of WPB walkers sits in LDS as four bf16 planes
+// (re hi/lo, im hi/lo) in its stored order (occupied index j, then orbital p),
+// which is the B-fragment order (8 consecutive p of one column j per lane).
+// R_g (hi/lo, rows padded to TC, pitch ldr a multiple of 32) is read from
+// L2 as A fragments, one k-block ahead.  Eight waves, each on its own g; the
+// exchange transpose goes through a per-wave LDS tile, ordered within the wave
+// (no workgroup barrier inside the g loop).
+// ---------------------------------------------------------------------------
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+constexpr int THREADS_B = 512;
+constexpr int KMAX_E = 8;  // k-blocks of 32 orbitals: n <= 256
+
+__device__ __forceinline__ f32x4 mfma_bf16(bf16x8 a, bf16x8 b, f32x4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
+}
+
+template <int NT, int WPB>
+size_t energy_bf16x3_lds(int n, int nspin, int naux) {
+    constexpr int TC = NT * 16;
+    const int pp = afq::round_up(n, 32) + 8;
+    size_t b = (size_t)WPB * 4 * TC * pp * 2 + (size_t)8 * TC * (TC + 1) * 4;
+    b = (b + 15) & ~(size_t)15;
+    if (nspin == 2) b += sizeof(double2) * (size_t)WPB * naux;
+    return b;
+}
+
+template <int NT, int WPB>
+__global__ __launch_bounds__(THREADS_B, 1) void energy_bf16x3_kernel(
+    int nw, int n, int o, int nspin, int naux, double h0, const double* __restrict__ greens,
+    const uint16_t* __restrict__ rot_hi, const uint16_t* __restrict__ rot_lo, int ldr,
+    const double* __restrict__ rot_h1, double2* __restrict__ eout) {
+    using MF = afq::mfma16<float>;
+    constexpr int TC = NT * 16;
+    constexpr int SP = TC + 1;
+    extern __shared__ __attribute__((aligned(16))) unsigned char lds_raw[];
+    __shared__ double red[8][6 * WPB];
+    const int nk = ldr / 32;
+    const int PP = ldr + 8;  // bf16 pitch of one column of G
+    __bf16* X = reinterpret_cast<__bf16*>(lds_raw);               // [WPB][4][TC][PP]
+    float* S = reinterpret_cast<float*>(lds_raw + (size_t)WPB * 4 * TC * PP * 2);  // [8][TC][SP]
+    size_t cs_off = (size_t)WPB * 4 * TC * PP * 2 + (size_t)8 * TC * SP * 4;
+    cs_off = (cs_off + 15) & ~(size_t)15;
+    double2* cs = reinterpret_cast<double2*>(lds_raw + cs_off);   // [WPB][naux] (nspin == 2)
+
+    const int w0 = blockIdx.x * WPB;
+    const int nwb = min(WPB, nw - w0);
+    const long long ldg = (long long)nspin * o * n;
+    const int tid = threadIdx.x;
+    const int lane = tid & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int lr = lane & 15;
+    const int q = lane >> 4;
+    float* Sw = S + wave * TC * SP;
+
+    double e1r[WPB], e1i[WPB], exr[WPB], exi[WPB], c2r[WPB], c2i[WPB];
+#pragma unroll
+    for (int b = 0; b < WPB; ++b) e1r[b] = e1i[b] = exr[b] = exi[b] = c2r[b] = c2i[b] = 0.0;
+
+    for (int s = 0; s < nspin; ++s) {
+        __syncthreads();
+        // G of each walker -> four bf16 planes; zero for j >= o and p >= n
+#pragma unroll
+        for (int b = 0; b < WPB; ++b) {
+            const bool have = b < nwb;
+            const double* gre = greens + (long long)(w0 + (have ? b : 0)) * ldg + (long long)s * o * n;
+            const double* gim = gre + (long long)nw * ldg;
+            __bf16* Xb = X + (size_t)b * 4 * TC * PP;
+            for (int e = tid; e < TC * ldr; e += THREADS_B) {
+                const int j = e / ldr;
+                const int p = e - j * ldr;
+                const bool ok = have && j < o && p < n;
+                const int idx = ok ? j * n + p : 0;
+                const double vr = ok ? gre[idx] : 0.0;
+                const double vi = ok ? gim[idx] : 0.0;
+                const float fr = (float)vr, fi = (float)vi;
+                const __bf16 hr = (__bf16)fr, hi = (__bf16)fi;
+                Xb[(0 * TC + j) * PP + p] = hr;
+                Xb[(1 * TC + j) * PP + p] = (__bf16)(fr - (float)hr);
+                Xb[(2 * TC + j) * PP + p] = hi;
+                Xb[(3 * TC + j) * PP + p] = (__bf16)(fi - (float)hi);
+            }
+            if (have) {
+                const double* rh = rot_h1 + (long long)s * o * n;
+                for (int e = tid; e < o * n; e += THREADS_B) {
+                    const double h = rh[e];
+                    e1r[b] += gre[e] * h;
+                    e1i[b] += gim[e] * h;
+                }
+            }
+        }
+        __syncthreads();
+
+        const long long rs = (long long)s * naux * TC * ldr;
+        auto read_b = [&](int b, int plane, int tj, int kb) {
+            return *reinterpret_cast<const bf16x8*>(X + ((size_t)(b * 4 + plane) * TC + tj * 16 + lr) * PP +
+                                                   kb * 32 + 8 * q);
+        };
+        for (int g = wave; g < naux; g += 8) {
+            const uint16_t* Ah = rot_hi + rs + (long long)g * TC * ldr;
+            const uint16_t* Al = rot_lo + rs + (long long)g * TC * ldr;
+            f32x4 F[WPB][NT][NT][2];
+#pragma unroll
+            for (int b = 0; b < WPB; ++b)
+#pragma unroll
+                for (int ti = 0; ti < NT; ++ti)
+#pragma unroll
+                    for (int tj = 0; tj < NT; ++tj) F[b][ti][tj][0] = F[b][ti][tj][1] = MF::zero();
+            // A fragments of R_g (rows ti*16 + lr, k = kb*32 + 8q..), one k-block ahead
+            bf16x8 ring[2][NT][2];
+            auto issue = [&](int kb) {
+#pragma unroll
+                for (int ti = 0; ti < NT; ++ti) {
+                    const long long off = (long long)(ti * 16 + lr) * ldr + (kb < nk ? kb : 0) * 32 + 8 * q;
+                    ring[kb % 2][ti][0] = *reinterpret_cast<const bf16x8*>(Ah + off);
+                    ring[kb % 2][ti][1] = *reinterpret_cast<const bf16x8*>(Al + off);
+                }
+            };
+            issue(0);
+            // fully unrolled to KMAX_E k-blocks so that every ring index is a constant
+#pragma unroll
+            for (int kb = 0; kb < KMAX_E; ++kb) {
+                if (kb >= nk) break;
+                if (kb + 1 < nk) issue(kb + 1);
+                bf16x8 ah[NT], al[NT];
+#pragma unroll
+                for (int ti = 0; ti < NT; ++ti) {
+                    ah[ti] = ring[kb % 2][ti][0];
+                    al[ti] = ring[kb % 2][ti][1];
+                }
+#pragma unroll
+                for (int b = 0; b < WPB; ++b)
+#pragma unroll
+                    for (int tj = 0; tj < NT; ++tj)
+#pragma unroll
+                        for (int c = 0; c < 2; ++c) {
+                            const bf16x8 bh = read_b(b, 2 * c, tj, kb);
+                            const bf16x8 bl = read_b(b, 2 * c + 1, tj, kb);
+#pragma unroll
+                            for (int ti = 0; ti < NT; ++ti) {
+                                f32x4& acc = F[b][ti][tj][c];
+                                acc = mfma_bf16(al[ti], bh, acc);
+                                acc = mfma_bf16(ah[ti], bl, acc);
+                                acc = mfma_bf16(ah[ti], bh, acc);
+                            }
+                        }
+                // keeps the next k-block's fragment reads out of this one
+                __builtin_amdgcn_sched_barrier(0);
+            }
+#pragma unroll
+            for (int b = 0; b < WPB; ++b) {
+                // Coulomb: element r of a diagonal tile is on the diagonal when
+                // its row MF::row(lane, r) equals its column lane % 16
+                double cr = 0.0, ci = 0.0;
+#pragma unroll
+                for (int t = 0; t < NT; ++t)
+#pragma unroll
+                    for (int r = 0; r < 4; ++r)
+                        if (lr == MF::row(lane, r)) {
+                            cr += (double)F[b][t][t][0][r];
+                            ci += (double)F[b][t][t][1][r];
+                        }
+                cr = afq::wave_sum(cr);
+                ci = afq::wave_sum(ci);
+                if (b < nwb) {
+                    if (nspin == 1) {
+                        c2r[b] += cr * cr - ci * ci;
+                        c2i[b] += 2.0 * cr * ci;
+                    } else if (s == 0) {
+                        if (lane == 0) cs[b * naux + g] = make_double2(cr, ci);
+                    } else {
+                        const double2 cu = cs[b * naux + g];
+                        const double tr = cu.x + cr;
+                        const double tim = cu.y + ci;
+                        c2r[b] += tr * tr - tim * tim;
+                        c2i[b] += 2.0 * tr * tim;
+                    }
+                }
+                // exchange sum_ij f_ij f_ji through this wave's LDS tile; the
+                // wave's LDS accesses execute in program order, the asm keeps
+                // the compiler from moving them across one another
+#pragma unroll
+                for (int pl = 0; pl < 2; ++pl) {
+#pragma unroll
+                    for (int ti = 0; ti < NT; ++ti)
+#pragma unroll
+                        for (int tj = 0; tj < NT; ++tj)
+#pragma unroll
+                            for (int r = 0; r < 4; ++r)
+                                Sw[(ti * 16 + MF::row(lane, r)) * SP + tj * 16 + lr] = F[b][ti][tj][pl][r];
+                    asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
+#pragma unroll
+                    for (int ti = 0; ti < NT; ++ti)
+#pragma unroll
+                        for (int tj = 0; tj < NT; ++tj)
+#pragma unroll
+                            for (int r = 0; r < 4; ++r) {
+                                const double t = (double)Sw[(tj * 16 + lr) * SP + ti * 16 + MF::row(lane, r)];
+                                if (pl == 0) {
+                                    exr[b] += (double)F[b][ti][tj][0][r] * t;
+                                    exi[b] += (double)F[b][ti][tj][1][r] * t;
+                                } else {
+                                    exr[b] -= (double)F[b][ti][tj][1][r] * t;
+                                    exi[b] += (double)F[b][ti][tj][0][r] * t;
+                                }
+                            }
+                    asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
+                }
+            }
+        }
+    }
+
+#pragma unroll
+    for (int b = 0; b < WPB; ++b) {
+        const double v[6] = {afq::wave_sum(e1r[b]), afq::wave_sum(e1i[b]), afq::wave_sum(exr[b]),
+                             afq::wave_sum(exi[b]), c2r[b], c2i[b]};
+        if (lane == 0)
+            for (int k = 0; k < 6; ++k) red[wave][6 * b + k] = v[k];
+    }
+    __syncthreads();
+    if (tid < nwb) {
+        const int b = tid;
+        double v[6] = {0, 0, 0, 0, 0, 0};
+        for (int w8 = 0; w8 < 8; ++w8)
+            for (int k = 0; k < 6; ++k) v[k] += red[w8][6 * b + k];
+        double er, ei;
+        if (nspin == 1) {
+            er = h0 + 2.0 * v[0] + 2.0 * v[4] - v[2];
+            ei = 2.0 * v[1] + 2.0 * v[5] - v[3];
+        } else {
+            er = h0 + v[0] + 0.5 * (v[4] - v[2]);
+            ei = v[1] + 0.5 * (v[5] - v[3]);
+        }
+        eout[w0 + b] = make_double2(er, ei);
+    }
+}
+
+template <int NT, int WPB>
+int launch_energy_bf16x3(int nw, int n, int o, int nspin, int naux, double h0, const double* greens,
+                         const uint16_t* rh, const uint16_t* rl, int ldr, const double* rot_h1,
+                         double* energy, hipStream_t st) {
+    const size_t lds = energy_bf16x3_lds<NT, WPB>(ldr, nspin, naux);
+    AFQ_REQUIRE(lds + 8 * 6 * WPB * sizeof(double) <= 160 * 1024,
+                "afq_energy_bf16x3: n=%d o=%d naux=%d needs more than 160 KB of LDS", n, o, naux);
+    if (lds > 64 * 1024) {
+        (void)hipFuncSetAttribute((const void*)energy_bf16x3_kernel<NT, WPB>,
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        (void)hipGetLastError();
+    }
+    hipLaunchKernelGGL((energy_bf16x3_kernel<NT, WPB>), dim3((unsigned)afq::ceil_div(nw, WPB)),
+                       dim3(THREADS_B), lds, st, nw, n, o, nspin, naux, h0, greens, rh, rl, ldr, rot_h1,
+                       (double2*)energy);
+    return afq::check_launch("afq_energy_bf16x3");
+}
+
 }  // namespace
 
 extern "C" int afq_energy(int nw, int n, int o, int nspin, int naux, double h0, const double* greens,
@@ -298,4 +555,30 @@ extern "C" int afq_energy_f32(int nw, int n, int o, int nspin, int naux, double 
                               const double* rot_h1, double* energy, void* stream) {
     return energy_entry<float>("afq_energy_f32", nw, n, o, nspin, naux, h0, greens, rot_chol_pad,
                                ldr, rot_h1, energy, (hipStream_t)stream);
+}
+
+extern "C" int afq_energy_bf16x3(int nw, int n, int o, int nspin, int naux, double h0,
+                                 const double* greens, const unsigned short* rot_hi,
+                                 const unsigned short* rot_lo, int ldr, const double* rot_h1,
+                                 double* energy, void* stream) {
+    AFQ_REQUIRE(nw > 0 && n > 0 && o > 0 && o <= n && naux > 0, "afq_energy_bf16x3: bad sizes");
+    AFQ_REQUIRE(nspin == 1 || nspin == 2, "afq_energy_bf16x3: nspin must be 1 or 2");
+    AFQ_REQUIRE(ldr >= n && ldr % 32 == 0, "afq_energy_bf16x3: ldr=%d must be >= n and a multiple of 32", ldr);
+    AFQ_REQUIRE(ldr <= 32 * KMAX_E, "afq_energy_bf16x3: n=%d exceeds the supported maximum of %d", n, 32 * KMAX_E);
+    AFQ_REQUIRE(greens && rot_hi && rot_lo && rot_h1 && energy, "afq_energy_bf16x3: null pointer");
+    AFQ_REQUIRE(o <= 32, "afq_energy_bf16x3: o=%d exceeds the supported maximum of 32", o);
+    hipStream_t st = (hipStream_t)stream;
+    const uint16_t* rh = (const uint16_t*)rot_hi;
+    const uint16_t* rl = (const uint16_t*)rot_lo;
+    // two walkers per workgroup (halves the R_g traffic) when they fit in LDS
+    // (the per-g Coulomb store of the unrestricted form takes room too)
+    auto fits2 = [&](size_t lds) { return lds + 8 * 6 * 2 * sizeof(double) <= 160 * 1024; };
+    if (o <= 16) {
+        if (fits2(energy_bf16x3_lds<1, 2>(ldr, nspin, naux)))
+            return launch_energy_bf16x3<1, 2>(nw, n, o, nspin, naux, h0, greens, rh, rl, ldr, rot_h1, energy, st);
+        return launch_energy_bf16x3<1, 1>(nw, n, o, nspin, naux, h0, greens, rh, rl, ldr, rot_h1, energy, st);
+    }
+    if (fits2(energy_bf16x3_lds<2, 2>(ldr, nspin, naux)))
+        return launch_energy_bf16x3<2, 2>(nw, n, o, nspin, naux, h0, greens, rh, rl, ldr, rot_h1, energy, st);
+    return launch_energy_bf16x3<2, 1>(nw, n, o, nspin, naux, h0, greens, rh, rl, ldr, rot_h1, energy, st);
 }

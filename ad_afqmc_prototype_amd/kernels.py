@@ -296,22 +296,38 @@ def orthonormalize(w, out=None):
 
 
 def energy(greens, nw, n, o, nspin, rot_chol_pad, rot_h1, h0, out=None):
-    naux = rot_chol_pad.shape[1]
+    """Local energies (nw,) complex128.  rot_chol_pad: (nspin, naux, opad,
+    ldr) float64 or float32 (fp64 / fp32 contractions), or a (hi, lo) pair of
+    bf16 planes (nspin * naux * opad, ldr32), ldr32 = n rounded up to 32
+    (split-bf16 contractions); sums over g are fp64 in every mode."""
     dev = greens.device
-    ldr = ldv_for(n)
     opad = (o + 15) // 16 * 16
     _check(greens, f64, (2, nw, nspin * o * n), "greens")
-    rt = rot_chol_pad.dtype
-    if rt not in (f64, f32):
-        raise TypeError("rot_chol_pad must be float64 or float32")
-    _check(rot_chol_pad, rt, (nspin, naux, opad, ldr), "rot_chol_pad", dev)
     _check(rot_h1, f64, (nspin, o, n), "rot_h1", dev)
     if out is None:
         out = torch.empty((nw,), dtype=c128, device=dev)
     _check(out, c128, (nw,), "energy", dev)
+    if isinstance(rot_chol_pad, tuple):
+        hi, lo = rot_chol_pad
+        rows, ldr = hi.shape
+        naux = rows // (nspin * opad)
+        if naux * nspin * opad != rows or ldr % 32 or ldr < n:
+            raise ValueError(f"bf16 energy operand {tuple(hi.shape)} does not match nspin={nspin}, "
+                             f"o={o}, n={n}")
+        _check(hi, torch.int16, (rows, ldr), "rot_chol_hi", dev)
+        _check(lo, torch.int16, (rows, ldr), "rot_chol_lo", dev)
+        _call("afq_energy_bf16x3", greens, nw, n, o, nspin, naux, float(h0), greens.data_ptr(),
+              hi.data_ptr(), lo.data_ptr(), ldr, rot_h1.data_ptr(), out.data_ptr(), _stream(greens))
+        return out
+    naux = rot_chol_pad.shape[1]
+    ldr = ldv_for(n)
+    rt = rot_chol_pad.dtype
+    if rt not in (f64, f32):
+        raise TypeError("rot_chol_pad must be float64 or float32")
+    _check(rot_chol_pad, rt, (nspin, naux, opad, ldr), "rot_chol_pad", dev)
     _call("afq_energy" if rt == f64 else "afq_energy_f32", greens, nw, n, o, nspin, naux, float(h0),
-          greens.data_ptr(),
-              rot_chol_pad.data_ptr(), ldr, rot_h1.data_ptr(), out.data_ptr(), _stream(greens))
+          greens.data_ptr(), rot_chol_pad.data_ptr(), ldr, rot_h1.data_ptr(), out.data_ptr(),
+          _stream(greens))
     return out
 
 

@@ -53,18 +53,26 @@ class rhf_meas_ctx:
         return self._cache[key]
 
     def energy_operands(self, nspin: int):
-        """(rot_chol_pad (nspin, naux, opad, ldr), rot_h1 (nspin, o, n)) for afq_energy."""
+        """(rot_chol_pad, rot_h1 (nspin, o, n)) for the energy kernels:
+        rot_chol_pad is (nspin, naux, opad, ldr) float64 / float32 in fp64 /
+        fp32 mode, and a (hi, lo) pair of bf16 planes (nspin * naux * opad,
+        ldr32) with ldr32 = n rounded up to 32 in bf16x3 mode."""
         key = ("energy", nspin)
         if key not in self._cache:
             _, g, o, n = self.rot_chol.shape
             opad = (o + 15) // 16 * 16
-            ldr = kernels.ldv_for(n)
-            # fp32 contractions (fp64 sums) for the mixed modes: split bf16 is
-            # too coarse for the exchange term
-            dt = torch.float64 if self.precision == "fp64" else torch.float32
-            pad = torch.zeros((nspin, g, opad, ldr), dtype=dt, device=self.rot_chol.device)
-            pad[:, :, :o, :n] = self.rot_chol[:nspin]
-            self._cache[key] = (pad, self.rot_h1[:nspin].contiguous())
+            if self.precision == "bf16x3":
+                ldr = (n + 31) // 32 * 32
+                pad = torch.zeros((nspin, g, opad, ldr), dtype=torch.float64, device=self.rot_chol.device)
+                pad[:, :, :o, :n] = self.rot_chol[:nspin]
+                op = kernels.split_bf16(pad.reshape(-1, ldr))
+                del pad
+            else:
+                ldr = kernels.ldv_for(n)
+                dt = torch.float64 if self.precision == "fp64" else torch.float32
+                op = torch.zeros((nspin, g, opad, ldr), dtype=dt, device=self.rot_chol.device)
+                op[:, :, :o, :n] = self.rot_chol[:nspin]
+            self._cache[key] = (op, self.rot_h1[:nspin].contiguous())
         return self._cache[key]
 
 
@@ -147,7 +155,8 @@ def energy_kernel_u(walkers, ham_data, meas_ctx, trial_data, greens=None):
 
 def make_rhf_meas_ops(sys: system, precision: str = "fp64") -> meas_ops:
     """meas/rhf.py:153-176; also serves UHF trials (per-spin integrals).
-    precision selects the force-bias contraction: "fp64", "fp32" or "bf16x3"."""
+    precision selects the force-bias and energy contractions: "fp64", "fp32"
+    or "bf16x3"."""
     from functools import partial
     if precision not in ("fp64", "fp32", "bf16x3"):
         raise ValueError(f"unknown precision {precision!r}")
@@ -156,11 +165,13 @@ def make_rhf_meas_ops(sys: system, precision: str = "fp64") -> meas_ops:
     if wk == "restricted":
         return meas_ops(overlap=trial_rhf.overlap_r, build_meas_ctx=build_meas_ctx_p,
                         kernels={k_force_bias: force_bias_kernel_r, k_energy: energy_kernel_r},
-                        log_overlap=trial_rhf.log_overlap, greens=trial_rhf.greens)
+                        log_overlap=trial_rhf.log_overlap, greens=trial_rhf.greens,
+                        force_bias_planes=force_bias_planes)
     if wk == "unrestricted":
         return meas_ops(overlap=trial_rhf.overlap_u, build_meas_ctx=build_meas_ctx_p,
                         kernels={k_force_bias: force_bias_kernel_u, k_energy: energy_kernel_u},
-                        log_overlap=trial_rhf.log_overlap, greens=trial_rhf.greens)
+                        log_overlap=trial_rhf.log_overlap, greens=trial_rhf.greens,
+                        force_bias_planes=force_bias_planes)
     if wk == "generalized":
         raise NotImplementedError("generalized walkers are not implemented on the HIP path")
     raise ValueError(f"unknown walker_kind: {sys.walker_kind}")

@@ -76,10 +76,12 @@ def init_prop_state(*, sys: system, n_walkers: int, seed: int, ham_data: ham_cho
 
 def _force_bias_planes(state: prop_state, ham_data, trial_data, meas_ops_: meas_ops, meas_ctx,
                        n_chunks: int):
-    fb_kernel = meas_ops_.require_kernel(k_force_bias)
     greens = state.greens
     if greens is None and meas_ops_.greens is not None:
         greens = meas_ops_.greens(state.walkers, trial_data)
+    if meas_ops_.force_bias_planes is not None:
+        return meas_ops_.force_bias_planes(state.walkers, meas_ctx, trial_data, greens=greens)
+    fb_kernel = meas_ops_.require_kernel(k_force_bias)
     fb = fb_kernel(state.walkers, ham_data, meas_ctx, trial_data, greens=greens)
     if fb.is_complex():
         return torch.stack([fb.real, fb.imag]).to(torch.float64).contiguous()

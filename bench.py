@@ -224,6 +224,8 @@ def kernel_table(ktimes, n, o, g, nw, nspin, rhf):
         "afq_trotter_bf16x3": (taylor, "bf16x3"),
         "afq_overlap_greens": (fl["greens"] * nw, "fp64"),
         "afq_energy": (4.0 * g * o * o * n * nspin * nw, "fp64"),
+        "afq_energy_f32": (4.0 * g * o * o * n * nspin * nw, "fp32"),
+        "afq_energy_bf16x3": (4.0 * g * o * o * n * nspin * nw, "bf16x3"),
     }
     out = {}
     for name, t in ktimes.items():
@@ -321,8 +323,8 @@ def main():
     value = nw * args.steps / elapsed
     flops = flops_per_walker_step(n, o, g, nspin, not cfg["uhf"])
     dtype = {"fp64": "f64", "fp32": "f32 contractions + f32 Trotter, f64 LU/QR/weights/energy",
-             "bf16x3": "bf16x3 contractions and Trotter products (f32 sums), f64 LU/QR/weights/"
-                       "energy sums"}[args.precision]
+             "bf16x3": "bf16x3 contractions, Trotter and energy products (f32 sums), f64 LU/QR/"
+                       "weights/energy sums"}[args.precision]
     out = {
         "metric": "walker-steps/s at N=200,naux=1000: mixed-prec vs FP64 GPU, |dE|<0.1 mHa",
         "value": value,

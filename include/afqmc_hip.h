@@ -198,11 +198,22 @@ int afq_energy(int nw, int n, int o, int nspin, int naux, double h0, const doubl
  * afq_energy with the f_g contractions on fp32 MFMA (rot_chol_pad float32) and
  * every sum over g (Coulomb squares, exchange, one-body) in fp64; two walkers
  * share each streamed rot_chol_g tile when they fit in LDS.  The measurement
- * of the mixed step.
+ * of the fp32 mixed step.
  */
 int afq_energy_f32(int nw, int n, int o, int nspin, int naux, double h0, const double* greens,
                    const float* rot_chol_pad, int ldr, const double* rot_h1, double* energy,
                    void* stream);
+
+/*
+ * afq_energy with the f_g contractions as split-bf16 products (rot_chol and G
+ * both carried as bf16 hi + lo, three bf16 MFMAs, fp32 sums within a g) and
+ * every sum over g in fp64.  rot_hi / rot_lo: bf16 bit planes (nspin, naux,
+ * opad, ldr) with opad = o rounded up to 16, zero beyond (o, n); ldr a
+ * multiple of 32, n <= ldr <= 256.  Same reference functions as afq_energy.
+ */
+int afq_energy_bf16x3(int nw, int n, int o, int nspin, int naux, double h0, const double* greens,
+                      const unsigned short* rot_hi, const unsigned short* rot_lo, int ldr,
+                      const double* rot_h1, double* energy, void* stream);
 
 /*
  * Block energy of prop/blocks.py:59-79: real part, outlier clip to e_ref when

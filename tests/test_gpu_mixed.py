@@ -253,3 +253,34 @@ def test_trotter_bf16x3_exact_identity(gpu_device):
     out = kernels.trotter(torch.tensor(w, device=gpu_device), v, torch.tensor(ep, device=gpu_device),
                           0.07, 6, arith="bf16x3").cpu().numpy()
     np.testing.assert_array_equal(out, w.astype(np.complex64).astype(np.complex128))
+
+
+@pytest.mark.parametrize("case,tol", [
+    (dict(seed=2, n=200, o=32, g=1000, uhf=False, kind="restricted", nw=5, spread=0.05), 2e-6),
+    (dict(seed=1, n=120, o=20, g=600, uhf=True, kind="unrestricted", nw=3, spread=0.05), 2e-6),
+    (dict(seed=0, n=24, o=6, g=96, uhf=False, kind="restricted", nw=9, spread=0.05), 1e-5),
+    (dict(seed=2, n=200, o=32, g=1000, uhf=False, kind="restricted", nw=4, spread=0.3), 1e-4),
+])
+def test_energy_bf16x3_contractions(gpu_device, case, tol):
+    """Split-bf16 f_g contractions with fp64 sums against the fp64 checker.
+    The split keeps ~2^-17 of each operand, and that error enters the
+    Coulomb square and the exchange coherently, so the bound is relative to
+    |E|: 2e-6 for walkers near the trial at the working Cholesky scale, 1e-5
+    for the 24-orbital system at scale 1 (two-body parts a larger share of
+    |E|; still below one unit of 2^-17), 1e-4 for walkers far from the trial
+    (|E| ~ 1e3, large Coulomb and exchange parts that cancel).  Odd walker
+    counts leave a half-filled two-walker workgroup."""
+    from ad_afqmc_prototype_amd.core.ops import k_energy
+    from ad_afqmc_prototype_amd.meas.rhf import make_rhf_meas_ops
+    from ._pairs import random_walkers
+    p = make_pair(case["seed"], case["n"], case["o"], case["g"], uhf=case["uhf"], kind=case["kind"],
+                  chol_scale=0.03 if case["n"] == 200 else 1.0, device=gpu_device)
+    m_ops = make_rhf_meas_ops(p.g["sys"], precision="bf16x3")
+    ctx = m_ops.build_meas_ctx(p.g["ham"], p.g["trial"])
+    rng = np.random.default_rng(4)
+    w = random_walkers(rng, case["kind"], case["nw"], case["n"], case["o"], p.s.mo_up, p.s.mo_dn,
+                       spread=case["spread"])
+    e = m_ops.require_kernel(k_energy)(to_dev(w, gpu_device), p.g["ham"], ctx, p.g["trial"]).cpu().numpy()
+    e_ref = orc.energy(w, p.sd.h0, p.sd.trial, p.sd.mctx, case["kind"])
+    err = np.abs(e - e_ref) / np.abs(e_ref)
+    assert err.max() < tol, err.max()
