@@ -47,6 +47,11 @@ _SIGNATURES = {
                           c_vp, c_vp],
     "afq_block_energy": [c_int, c_vp, c_vp, c_double, c_double, c_double, c_vp, c_vp, c_vp],
     "afq_comb_indices": [c_int, c_vp, c_double, c_vp, c_vp, c_vp, c_vp],
+    "afq_comb_indices_dev": [c_int, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp],
+    "afq_rng_split": [c_vp, c_vp, c_vp, c_vp],
+    "afq_rng_normal": [c_vp, c_ll, c_vp, c_vp],
+    "afq_rng_uniform": [c_vp, c_vp, c_vp],
+    "afq_threefry2x32": [ctypes.c_uint, ctypes.c_uint, c_ll, c_vp, c_vp, c_vp, c_vp, c_vp],
     "afq_gather_rows": [c_int, c_ll, c_int, c_ll, c_ll, c_vp, c_vp, c_vp, c_vp],
 }
 

@@ -1,6 +1,8 @@
 // Error reporting shared by every entry point of libafqmc_hip.so.
 #include <cstdarg>
 #include <cstdio>
+#include <mutex>
+#include <unordered_set>
 
 #include "afqmc_hip.h"
 #include "common.h"
@@ -25,6 +27,16 @@ int check_launch(const char* what) {
         return -2;
     }
     return 0;
+}
+
+void allow_full_lds(const void* fn) {
+    static std::mutex mu;
+    static std::unordered_set<const void*> done;
+    std::lock_guard<std::mutex> lock(mu);
+    if (done.count(fn)) return;
+    (void)hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+    (void)hipGetLastError();
+    done.insert(fn);
 }
 
 }  // namespace afq

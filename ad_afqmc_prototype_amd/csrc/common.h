@@ -12,6 +12,10 @@ namespace afq {
 
 // Thread-local last-error text, read back through afq_last_error().
 void set_error(const char* fmt, ...);
+// Raises the dynamic-LDS limit of kernel `fn` to the 160 KB of a gfx950 CU,
+// once per kernel and process (so no such call happens inside a stream capture
+// after the first, eager launch).
+void allow_full_lds(const void* fn);
 // Converts a pending launch error into a return code (0 ok, -2 launch failure).
 int check_launch(const char* what);
 

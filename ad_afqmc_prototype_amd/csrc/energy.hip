@@ -246,11 +246,7 @@ int launch_energy(const char* name, int nw, int n, int o, int nspin, int naux, d
     const size_t lds = energy_lds<R, NT, WPB>(n, nspin, naux);
     AFQ_REQUIRE((energy_fits<R, NT, WPB>(n, nspin, naux)),
                 "%s: n=%d o=%d naux=%d needs more than 160 KB of LDS", name, n, o, naux);
-    if (lds > 64 * 1024) {
-        (void)hipFuncSetAttribute((const void*)energy_kernel<R, NT, WPB>,
-                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-        (void)hipGetLastError();
-    }
+    if (lds > 64 * 1024) afq::allow_full_lds((const void*)energy_kernel<R, NT, WPB>);
     hipLaunchKernelGGL((energy_kernel<R, NT, WPB>), dim3((unsigned)afq::ceil_div(nw, WPB)),
                        dim3(THREADS), lds, st, nw, n, o, nspin, naux, h0, greens, rotp, ldr,
                        rot_h1, (double2*)energy);
@@ -530,11 +526,7 @@ int launch_energy_bf16x3(int nw, int n, int o, int nspin, int naux, double h0, c
     const size_t lds = energy_bf16x3_lds<NT, WPB>(ldr, nspin, naux);
     AFQ_REQUIRE(lds + 8 * 6 * WPB * sizeof(double) <= 160 * 1024,
                 "afq_energy_bf16x3: n=%d o=%d naux=%d needs more than 160 KB of LDS", n, o, naux);
-    if (lds > 64 * 1024) {
-        (void)hipFuncSetAttribute((const void*)energy_bf16x3_kernel<NT, WPB>,
-                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-        (void)hipGetLastError();
-    }
+    if (lds > 64 * 1024) afq::allow_full_lds((const void*)energy_bf16x3_kernel<NT, WPB>);
     hipLaunchKernelGGL((energy_bf16x3_kernel<NT, WPB>), dim3((unsigned)afq::ceil_div(nw, WPB)),
                        dim3(THREADS_B), lds, st, nw, n, o, nspin, naux, h0, greens, rh, rl, ldr, rot_h1,
                        (double2*)energy);

@@ -1,0 +1,160 @@
+// Device random stream of the step: keys split and normals / uniforms drawn
+// on the GPU, so a step or a whole block has no host round trip (and can be
+// captured in a graph).  The stream follows the reference's calls
+// (prop/afqmc.py:32,86-88, prop/blocks.py:81-82) under jax's original
+// (non-partitionable) Threefry scheme, config.py:27:
+//
+//   split(key)       = threefry(key, [0,1,2,3]) as (2,2): new key row 0, subkey row 1
+//   (the caller splits, then draws from the subkey, as the reference does)
+//   bits64[i] of n   = (A[i] << 32) | B[i],  (A[i], B[i]) = threefry(key, (i, n + i))
+//   uniform(lo, hi)  = max(lo, (bitcast((bits >> 12) | bits(1.0)) - 1) * (hi - lo) + lo)
+//   normal           = sqrt(2) erfinv(uniform(nextafter(-1, 0), 1))
+//
+// Threefry-2x32 with 20 rounds: Salmon, Moraes, Dror & Shaw, SC 2011.  erfinv is
+// computed here by Halley iterations on erf / erfc, so normals agree with
+// another erfinv implementation to a few ulp, not bit for bit.
+#include <cmath>
+#include <cstdint>
+
+#include "afqmc_hip.h"
+#include "common.h"
+
+namespace {
+
+__device__ __forceinline__ uint32_t rotl32(uint32_t x, int r) { return (x << r) | (x >> (32 - r)); }
+
+__device__ __forceinline__ void threefry2x32(uint32_t k0, uint32_t k1, uint32_t& x0, uint32_t& x1) {
+    const uint32_t k2 = k0 ^ k1 ^ 0x1BD11BDAu;
+    const uint32_t ks[3] = {k0, k1, k2};
+    constexpr int R[8] = {13, 15, 26, 6, 17, 29, 16, 24};
+    x0 += k0;
+    x1 += k1;
+#pragma unroll
+    for (int r = 0; r < 20; ++r) {
+        x0 += x1;
+        x1 = rotl32(x1, R[r & 7]);
+        x1 ^= x0;
+        if ((r & 3) == 3) {
+            const int s = (r + 1) >> 2;
+            x0 += ks[s % 3];
+            x1 += ks[(s + 1) % 3] + (uint32_t)s;
+        }
+    }
+}
+
+// (new key, subkey) = split(key)
+__device__ __forceinline__ void split_key(const uint32_t* key, uint32_t (&nk)[2], uint32_t (&sk)[2]) {
+    uint32_t a0 = 0, b0 = 2, a1 = 1, b1 = 3;
+    threefry2x32(key[0], key[1], a0, b0);
+    threefry2x32(key[0], key[1], a1, b1);
+    nk[0] = a0;
+    nk[1] = a1;
+    sk[0] = b0;
+    sk[1] = b1;
+}
+
+// [0, 1) from 64 random bits: the top 52 bits as the mantissa of [1, 2), minus 1
+__device__ __forceinline__ double unit_from_bits(uint32_t hi, uint32_t lo) {
+    const uint64_t bits = ((uint64_t)hi << 32) | lo;
+    return __longlong_as_double((long long)((bits >> 12) | 0x3FF0000000000000ull)) - 1.0;
+}
+
+// sqrt(2) erfinv(u) for u in (-1, 1): closed-form start (|rel err| < 4e-3),
+// then Halley steps on erf(y) = |u| (|u| <= 0.5) or erfc(y) = 1 - |u| (tails,
+// where 1 - |u| is exact and erfc keeps full relative precision).
+__device__ double sqrt2_erfinv(double u) {
+    const double a = fabs(u);
+    const double t = log((1.0 - a) * (1.0 + a));
+    const double c = 2.0 / (M_PI * 0.147) + 0.5 * t;
+    double y = sqrt(sqrt(c * c - t / 0.147) - c);
+    const double two_over_sqrt_pi = 1.1283791670955126;
+    for (int it = 0; it < 3; ++it) {
+        const double r = a <= 0.5 ? erf(y) - a : (1.0 - a) - erfc(y);
+        const double d = two_over_sqrt_pi * exp(-y * y);  // d/dy erf(y)
+        const double step = r / d;
+        y -= step / (1.0 + y * step);  // Halley: f'' / f' = -2 y
+    }
+    return copysign(1.4142135623730951 * y, u);
+}
+
+__global__ void split_kernel(const uint32_t* __restrict__ key, uint32_t* __restrict__ new_key,
+                             uint32_t* __restrict__ subkey) {
+    uint32_t nk[2], sk[2];
+    split_key(key, nk, sk);
+    new_key[0] = nk[0];
+    new_key[1] = nk[1];
+    subkey[0] = sk[0];
+    subkey[1] = sk[1];
+}
+
+__global__ void normal_kernel(const uint32_t* __restrict__ key, long long n, double* __restrict__ out) {
+    const uint32_t k0 = key[0], k1 = key[1];
+    const double lo = -1.0 + 0x1p-53;  // nextafter(-1, 0)
+    const double scale = 1.0 - lo;           // rounds to 2
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (long long)gridDim.x * blockDim.x) {
+        uint32_t a = (uint32_t)i, b = (uint32_t)(n + i);
+        threefry2x32(k0, k1, a, b);
+        const double f = unit_from_bits(a, b);
+        const double v = fmax(lo, f * scale + lo);
+        out[i] = sqrt2_erfinv(v);
+    }
+}
+
+__global__ void uniform_kernel(const uint32_t* __restrict__ key, double* __restrict__ out) {
+    uint32_t a = 0, b = 1;
+    threefry2x32(key[0], key[1], a, b);
+    out[0] = fmax(0.0, unit_from_bits(a, b));
+}
+
+__global__ void threefry_kernel(uint32_t k0, uint32_t k1, long long n, const uint32_t* __restrict__ c0,
+                                const uint32_t* __restrict__ c1, uint32_t* __restrict__ o0,
+                                uint32_t* __restrict__ o1) {
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (long long)gridDim.x * blockDim.x) {
+        uint32_t a = c0[i], b = c1[i];
+        threefry2x32(k0, k1, a, b);
+        o0[i] = a;
+        o1[i] = b;
+    }
+}
+
+unsigned grid_for(long long n) {
+    long long b = (n + 255) / 256;
+    return (unsigned)(b > 8192 ? 8192 : (b < 1 ? 1 : b));
+}
+
+}  // namespace
+
+extern "C" int afq_rng_split(const unsigned* key, unsigned* new_key, unsigned* subkey, void* stream) {
+    AFQ_REQUIRE(key && new_key && subkey, "afq_rng_split: null pointer");
+    AFQ_REQUIRE(key != new_key && key != subkey && new_key != subkey, "afq_rng_split: aliased keys");
+    hipLaunchKernelGGL(split_kernel, dim3(1), dim3(1), 0, (hipStream_t)stream, (const uint32_t*)key,
+                       (uint32_t*)new_key, (uint32_t*)subkey);
+    return afq::check_launch("afq_rng_split");
+}
+
+extern "C" int afq_rng_normal(const unsigned* key, long long n, double* out, void* stream) {
+    AFQ_REQUIRE(n >= 0 && n < (1LL << 31), "afq_rng_normal: n=%lld out of range", n);
+    if (n == 0) return 0;
+    AFQ_REQUIRE(key && out, "afq_rng_normal: null pointer");
+    hipLaunchKernelGGL(normal_kernel, dim3(grid_for(n)), dim3(256), 0, (hipStream_t)stream,
+                       (const uint32_t*)key, n, out);
+    return afq::check_launch("afq_rng_normal");
+}
+
+extern "C" int afq_rng_uniform(const unsigned* key, double* out, void* stream) {
+    AFQ_REQUIRE(key && out, "afq_rng_uniform: null pointer");
+    hipLaunchKernelGGL(uniform_kernel, dim3(1), dim3(1), 0, (hipStream_t)stream, (const uint32_t*)key, out);
+    return afq::check_launch("afq_rng_uniform");
+}
+
+extern "C" int afq_threefry2x32(unsigned k0, unsigned k1, long long n, const unsigned* c0,
+                                const unsigned* c1, unsigned* o0, unsigned* o1, void* stream) {
+    AFQ_REQUIRE(n >= 0, "afq_threefry2x32: n < 0");
+    if (n == 0) return 0;
+    AFQ_REQUIRE(c0 && c1 && o0 && o1, "afq_threefry2x32: null pointer");
+    hipLaunchKernelGGL(threefry_kernel, dim3(grid_for(n)), dim3(256), 0, (hipStream_t)stream, k0, k1, n,
+                       (const uint32_t*)c0, (const uint32_t*)c1, (uint32_t*)o0, (uint32_t*)o1);
+    return afq::check_launch("afq_threefry2x32");
+}

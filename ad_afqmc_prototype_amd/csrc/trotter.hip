@@ -224,11 +224,7 @@ int launch_trotter(int nw, int n, int o, int nspin, int ldv, int n_terms, double
     const int NP = afq::round_up(n, 16);
     const size_t lds = (size_t)2 * NP * NC * 16 * sizeof(R);
     const int pair_map = (nspin == 2 && nw % 8 == 0) ? 1 : 0;
-    if (lds > 64 * 1024) {
-        (void)hipFuncSetAttribute((const void*)trotter_kernel<R, NC, RPW>,
-                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-        (void)hipGetLastError();
-    }
+    if (lds > 64 * 1024) afq::allow_full_lds((const void*)trotter_kernel<R, NC, RPW>);
     hipLaunchKernelGGL((trotter_kernel<R, NC, RPW>), dim3((unsigned)(nw * nspin)), dim3(256 * NC), lds,
                        st, nw, n, o, nspin, ldv, n_terms, sqrt_dt, E, vhs,
                        (long long)nw * n * ldv, wu, wd, ou, od, pair_map);
@@ -455,11 +451,7 @@ int launch_trotter_resident(int nw, int n, int o, int nspin, int ldv, int n_term
     const int nct = afq::ceil_div(o, 16);
     const size_t lds = resident_lds(n);
     const int group_map = (nw % 8 == 0) ? 1 : 0;
-    if (lds > 64 * 1024) {
-        (void)hipFuncSetAttribute((const void*)trotter_resident_kernel<RESIDENT_KRB>,
-                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-        (void)hipGetLastError();
-    }
+    if (lds > 64 * 1024) afq::allow_full_lds((const void*)trotter_resident_kernel<RESIDENT_KRB>);
     hipLaunchKernelGGL((trotter_resident_kernel<RESIDENT_KRB>), dim3((unsigned)(nw * nspin * nct)),
                        dim3(64 * (NP / 16)), lds, st, nw, n, o, nspin, nct, ldv, n_terms, sqrt_dt, E,
                        vhs, (long long)nw * n * ldv, wu, wd, ou, od, group_map);

@@ -447,11 +447,7 @@ int launch_bf16x3(int nw, int n, int o, int nspin, int ldv, int n_terms, double 
     const int nwave = afq::ceil_div(NP / 16, RT);
     const size_t lds = bf16x3_lds(n, NL);
     const int group_map = (nw % 8 == 0) ? 1 : 0;
-    if (lds > 64 * 1024) {
-        (void)hipFuncSetAttribute((const void*)trotter_bf16x3_kernel<NL>,
-                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-        (void)hipGetLastError();
-    }
+    if (lds > 64 * 1024) afq::allow_full_lds((const void*)trotter_bf16x3_kernel<NL>);
     hipLaunchKernelGGL((trotter_bf16x3_kernel<NL>), dim3((unsigned)(nw * nspin * nct)), dim3(64 * nwave),
                        lds, st, nw, n, o, nspin, nct, ldv, n_terms, sqrt_dt, E, vhs,
                        (long long)nw * n * ldv, wu, wd, ou, od, group_map);

@@ -96,7 +96,8 @@ __global__ __launch_bounds__(THREADS) void qr_kernel(int n, int o, const double2
 // contraction, so the indices are reproducible bit for bit.
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(1024) void comb_kernel(int nw, const double* __restrict__ weights,
-                                                     double zeta, double* __restrict__ cw,
+                                                     double zeta_v, const double* __restrict__ zeta_p,
+                                                     double* __restrict__ cw,
                                                      int* __restrict__ idx,
                                                      double* __restrict__ wnew) {
 #pragma clang fp contract(off)
@@ -113,6 +114,7 @@ __global__ __launch_bounds__(1024) void comb_kernel(int nw, const double* __rest
     __syncthreads();
     const double tot = tot_s;
     const double avg = tot / (double)nw;
+    const double zeta = zeta_p ? zeta_p[0] : zeta_v;
     for (int k = threadIdx.x; k < nw; k += blockDim.x) {
         const double kz = (double)k + zeta;
         const double prod = tot * kz;
@@ -155,10 +157,7 @@ extern "C" int afq_orthonormalize(int nw, int n, int o, const double* win, doubl
     const size_t lds = (size_t)n * (o + 1) * sizeof(double2);
     AFQ_REQUIRE(lds + 64 * sizeof(double2) + 64 <= 160 * 1024,
                 "afq_orthonormalize: walker n=%d o=%d does not fit in LDS", n, o);
-    if (lds > 64 * 1024) {
-        (void)hipFuncSetAttribute((const void*)qr_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-        (void)hipGetLastError();
-    }
+    if (lds > 64 * 1024) afq::allow_full_lds((const void*)qr_kernel);
     hipLaunchKernelGGL(qr_kernel, dim3((unsigned)nw), dim3(THREADS), lds, (hipStream_t)stream, n, o,
                        (const double2*)win, (double2*)wout);
     return afq::check_launch("afq_orthonormalize");
@@ -169,8 +168,17 @@ extern "C" int afq_comb_indices(int nw, const double* weights, double zeta, doub
     AFQ_REQUIRE(nw > 0, "afq_comb_indices: nw=%d", nw);
     AFQ_REQUIRE(weights && cumsum && idx && weights_new, "afq_comb_indices: null pointer");
     hipLaunchKernelGGL(comb_kernel, dim3(1), dim3(1024), 0, (hipStream_t)stream, nw, weights, zeta,
-                       cumsum, idx, weights_new);
+                       (const double*)nullptr, cumsum, idx, weights_new);
     return afq::check_launch("afq_comb_indices");
+}
+
+extern "C" int afq_comb_indices_dev(int nw, const double* weights, const double* zeta,
+                                    double* cumsum, int* idx, double* weights_new, void* stream) {
+    AFQ_REQUIRE(nw > 0, "afq_comb_indices_dev: nw=%d", nw);
+    AFQ_REQUIRE(weights && zeta && cumsum && idx && weights_new, "afq_comb_indices_dev: null pointer");
+    hipLaunchKernelGGL(comb_kernel, dim3(1), dim3(1024), 0, (hipStream_t)stream, nw, weights, 0.0, zeta,
+                       cumsum, idx, weights_new);
+    return afq::check_launch("afq_comb_indices_dev");
 }
 
 extern "C" int afq_gather_rows(int nrows, long long row_len, int nplanes, long long src_plane_stride,

@@ -4,6 +4,9 @@ the reference), around the HIP block.
 Blocks of one chunk are enqueued back to back without host synchronisation;
 the block energies/weights of a chunk come back to the host once per chunk
 (the reference's jit + lax.scan over a chunk of blocks, driver.py:22-36).
+With use_graph the block is recorded once as a HIP graph (fields and comb
+uniforms drawn on the device from the state's key) and each block of a chunk
+is one graph launch.
 """
 from __future__ import annotations
 
@@ -19,7 +22,7 @@ from .core.ops import trial_ops as trial_ops_t
 from .core.system import system
 from .ham.chol import ham_chol
 from .prop.afqmc import init_prop_state
-from .prop.blocks import afqmc_block
+from .prop.blocks import afqmc_block, captured_block
 from .prop.chol_afqmc_ops import make_chol_afqmc_ops
 from .prop.types import afqmc_params
 from .stats import blocking_analysis_ratio, reject_outliers
@@ -44,9 +47,14 @@ def make_run_blocks(block: Callable):
 
 def run_afqmc_energy(*, sys: system, params: afqmc_params, ham_data: ham_chol, trial_data: Any,
                      meas_ops: meas_ops_t, trial_ops: trial_ops_t,
-                     block_fn: Callable = afqmc_block, prop_ops=None, verbose: bool = True):
+                     block_fn: Callable = afqmc_block, prop_ops=None, verbose: bool = True,
+                     use_graph: bool = False):
     """Returns (mean_energy, stderr, block_energies, block_weights) like
-    driver.py:39-176; block arrays include the initial estimate."""
+    driver.py:39-176; block arrays include the initial estimate.  use_graph
+    replays one captured afqmc_block per block (same results as the eager
+    block loop)."""
+    if use_graph and block_fn is not afqmc_block:
+        raise ValueError("use_graph records afqmc_block itself; a custom block_fn runs eagerly")
     if params.n_eql_blocks < 5 or params.n_blocks < 10:
         raise ValueError("run_afqmc_energy needs n_eql_blocks >= 5 and n_blocks >= 10 "
                          "(blocks are run in chunks of n_eql_blocks//5 and n_blocks//10)")
@@ -64,7 +72,19 @@ def run_afqmc_energy(*, sys: system, params: afqmc_params, ham_data: ham_chol, t
                         trial_data=trial_data, meas_ops=meas_ops, prop_ops=prop_ops,
                         prop_ctx=prop_ctx, meas_ctx=meas_ctx)
 
-    run_blocks = make_run_blocks(block)
+    if use_graph:
+        cb = captured_block(state, sys=sys, params=params, ham_data=ham_data, trial_data=trial_data,
+                            meas_ops=meas_ops, prop_ops=prop_ops, prop_ctx=prop_ctx, meas_ctx=meas_ctx)
+
+        def run_blocks(_state, *, n_blocks: int):
+            es, ws = [], []
+            for _ in range(n_blocks):
+                e, w = cb.replay()
+                es.append(e.clone())
+                ws.append(w.clone())
+            return cb.state, torch.stack(es), torch.stack(ws)
+    else:
+        run_blocks = make_run_blocks(block)
     t0 = time.perf_counter()
     t_mark = t0
 

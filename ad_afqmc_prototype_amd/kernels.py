@@ -6,6 +6,7 @@ no fallback: a tensor that is not on a HIP device raises.
 """
 from __future__ import annotations
 
+import numpy as np
 import torch
 
 from . import _lib
@@ -345,15 +346,70 @@ def block_energy(energies, weights, pop_shift, *, e_ref, thresh, shift_ema):
 
 
 def comb_indices(weights, zeta):
+    """Comb indices and new weights; zeta a float, or a 0-d float64 device
+    tensor (read on the device, no host round trip)."""
     nw = weights.shape[0]
     dev = weights.device
     _check(weights, f64, (nw,), "weights")
     cw = torch.empty((nw,), dtype=f64, device=dev)
     idx = torch.empty((nw,), dtype=torch.int32, device=dev)
     w_new = torch.empty((nw,), dtype=f64, device=dev)
-    _call("afq_comb_indices", weights, nw, weights.data_ptr(), float(zeta), cw.data_ptr(),
+    if isinstance(zeta, torch.Tensor):
+        _check(zeta, f64, (), "zeta", dev)
+        _call("afq_comb_indices_dev", weights, nw, weights.data_ptr(), zeta.data_ptr(), cw.data_ptr(),
+              idx.data_ptr(), w_new.data_ptr(), _stream(weights))
+    else:
+        _call("afq_comb_indices", weights, nw, weights.data_ptr(), float(zeta), cw.data_ptr(),
               idx.data_ptr(), w_new.data_ptr(), _stream(weights))
     return idx, w_new
+
+
+def rng_key(seed: int, device) -> torch.Tensor:
+    """jax.random.PRNGKey(seed) as a device uint32[2] (held in an int32 tensor)."""
+    s = int(seed) & 0xFFFFFFFFFFFFFFFF
+    words = np.array([s >> 32, s & 0xFFFFFFFF], dtype=np.uint32).view(np.int32)
+    return torch.tensor(words, dtype=torch.int32, device=device)
+
+
+def rng_split(key):
+    """(new key, subkey) = split(key), on the device."""
+    _check(key, torch.int32, (2,), "key")
+    nk = torch.empty_like(key)
+    sk = torch.empty_like(key)
+    _call("afq_rng_split", key, key.data_ptr(), nk.data_ptr(), sk.data_ptr(), _stream(key))
+    return nk, sk
+
+
+def rng_normal(key, shape, out=None):
+    """normal(key, shape) float64 on the device."""
+    _check(key, torch.int32, (2,), "key")
+    if out is None:
+        out = torch.empty(shape, dtype=f64, device=key.device)
+    _check(out, f64, tuple(shape), "out", key.device)
+    _call("afq_rng_normal", key, key.data_ptr(), out.numel(), out.data_ptr(), _stream(key))
+    return out
+
+
+def rng_uniform(key, out=None):
+    """uniform(key) in [0, 1): a 0-d float64 device tensor."""
+    _check(key, torch.int32, (2,), "key")
+    if out is None:
+        out = torch.empty((), dtype=f64, device=key.device)
+    _check(out, f64, (), "out", key.device)
+    _call("afq_rng_uniform", key, key.data_ptr(), out.data_ptr(), _stream(key))
+    return out
+
+
+def threefry2x32(k0: int, k1: int, c0, c1):
+    """Threefry-2x32-20 block function on uint32 counter pairs (int32 tensors)."""
+    n = c0.numel()
+    _check(c0, torch.int32, (n,), "c0")
+    _check(c1, torch.int32, (n,), "c1", c0.device)
+    o0 = torch.empty_like(c0)
+    o1 = torch.empty_like(c1)
+    _call("afq_threefry2x32", c0, int(k0) & 0xFFFFFFFF, int(k1) & 0xFFFFFFFF, n, c0.data_ptr(),
+          c1.data_ptr(), o0.data_ptr(), o1.data_ptr(), _stream(c0))
+    return o0, o1
 
 
 def gather_rows(src, idx, nplanes=1, out=None):

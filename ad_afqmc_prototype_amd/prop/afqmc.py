@@ -20,25 +20,23 @@ from .chol_afqmc_ops import chol_afqmc_ctx, chol_afqmc_ops
 from .types import afqmc_params, prop_state
 
 
-def make_key(seed: int) -> torch.Tensor:
-    """Host key (seed, counter)."""
-    return torch.tensor([int(seed), 0], dtype=torch.int64)
+def make_key(seed: int, device=None) -> torch.Tensor:
+    """jax.random.PRNGKey(seed) (prop/afqmc.py:32) as a device uint32[2]."""
+    if device is None:
+        device = torch.device("cuda", torch.cuda.current_device())
+    return kernels.rng_key(seed, device)
 
 
 def split_key(key: torch.Tensor):
-    """(next key, subkey): advancing the counter; the subkey seeds one draw."""
-    seed, ctr = int(key[0]), int(key[1])
-    sub = (seed * 0x9E3779B97F4A7C15 + ctr * 0xBF58476D1CE4E5B9 + 1) % (1 << 63)
-    return torch.tensor([seed, ctr + 1], dtype=torch.int64), sub
+    """(new key, subkey) = jax.random.split(key), on the device."""
+    return kernels.rng_split(key)
 
 
-def draw_fields(key: torch.Tensor, nw: int, n_fields: int, device):
-    """x ~ N(0,1) of shape (nw, n_fields) on the device, from the key's subkey
-    (torch Philox; not the reference's jax threefry stream)."""
+def draw_fields(key: torch.Tensor, nw: int, n_fields: int, device=None):
+    """prop/afqmc.py:86-88: key, subkey = split(key); x = normal(subkey, (nw, n_fields)),
+    drawn on the device from the Threefry stream (no host round trip)."""
     key_next, sub = split_key(key)
-    gen = torch.Generator(device=device)
-    gen.manual_seed(sub)
-    return torch.randn((nw, n_fields), dtype=torch.float64, device=device, generator=gen), key_next
+    return kernels.rng_normal(sub, (nw, n_fields)), key_next
 
 
 def _log_overlap_and_greens(walkers, meas_ops_: meas_ops, trial_data):
@@ -70,7 +68,7 @@ def init_prop_state(*, sys: system, n_walkers: int, seed: int, ham_data: ham_cho
         e_est = float(torch.mean(e_samples).item())
     pop_shift = torch.tensor(e_est, dtype=torch.float64, device=device)
     return prop_state(walkers=initial_walkers, weights=weights, log_overlaps=log_ov,
-                      rng_key=make_key(seed), pop_control_ene_shift=pop_shift,
+                      rng_key=make_key(seed, device), pop_control_ene_shift=pop_shift,
                       e_estimate=e_est, greens=greens)
 
 

@@ -26,11 +26,10 @@ class block_obs(NamedTuple):
 
 
 def draw_zeta(key: torch.Tensor):
-    """Comb uniform in [0, 1) from the key's subkey (host draw, no device sync)."""
+    """prop/blocks.py:81-82: key, subkey = split(key); zeta = uniform(subkey),
+    a 0-d device tensor (the comb reads it on the device)."""
     key_next, sub = split_key(key)
-    gen = torch.Generator()
-    gen.manual_seed(sub)
-    return float(torch.rand((), dtype=torch.float64, generator=gen)), key_next
+    return kernels.rng_uniform(sub), key_next
 
 
 def orthonormalize_state(state: prop_state, *, sys: system, trial_data, meas_ops_: meas_ops) -> prop_state:
@@ -55,16 +54,20 @@ def measure(state: prop_state, *, params: afqmc_params, ham_data: ham_chol, tria
 
 
 def reconfigure(state: prop_state, *, sys: system, trial_data, meas_ops_: meas_ops,
-                zeta: float | None = None) -> tuple[prop_state, torch.Tensor]:
+                zeta=None) -> tuple[prop_state, torch.Tensor]:
     """prop/blocks.py:81-92: comb reconfiguration; the Green's functions and
     log-overlaps of the selected walkers are gathered (the reference recomputes
-    the overlaps of the gathered walkers, which gives the same values)."""
+    the overlaps of the gathered walkers, which gives the same values).
+    zeta: a float, a 0-d float64 device tensor, or None (drawn on the device
+    from the state's key)."""
     if zeta is None:
         zeta, key = draw_zeta(state.rng_key)
     else:
         key = split_key(state.rng_key)[0]
+    if not isinstance(zeta, torch.Tensor):
+        zeta = float(zeta)
     w_sr, w_new, idx = wk.stochastic_reconfiguration_with_indices(state.walkers, state.weights,
-                                                                  float(zeta), sys.walker_kind)
+                                                                  zeta, sys.walker_kind)
     lo = kernels.gather_rows(state.log_overlaps, idx)
     greens = state.greens
     if greens is not None:
@@ -96,3 +99,68 @@ def afqmc_block(state: prop_state, *, sys: system, params: afqmc_params, ham_dat
     if return_indices:
         return state, obs, idx
     return state, obs
+
+
+def _clone_state(state: prop_state) -> prop_state:
+    def c(x):
+        if x is None:
+            return None
+        if isinstance(x, torch.Tensor):
+            return x.clone()
+        return type(x)(*(c(y) for y in x))
+    return state._replace(walkers=c(state.walkers), weights=c(state.weights),
+                          log_overlaps=c(state.log_overlaps), rng_key=c(state.rng_key),
+                          pop_control_ene_shift=c(state.pop_control_ene_shift), greens=c(state.greens))
+
+
+def _copy_state_(dst: prop_state, src: prop_state) -> None:
+    def cp(d, s):
+        if isinstance(d, torch.Tensor):
+            d.copy_(s)
+        else:
+            for a, b in zip(d, s):
+                cp(a, b)
+    for name in ("walkers", "weights", "log_overlaps", "rng_key", "pop_control_ene_shift", "greens"):
+        cp(getattr(dst, name), getattr(src, name))
+
+
+class captured_block:
+    """One afqmc_block recorded once as a HIP graph and replayed in place.
+
+    The fields and the comb uniform come from the state's key on the device
+    (no injected arrays, no host round trip), so every replay is a new block:
+    replay() advances ``self.state`` (the graph's static buffers) by one block
+    and returns the block's energy and weight as 0-d device tensors (the
+    graph's static outputs, overwritten by the next replay).  Results equal
+    those of calling afqmc_block eagerly from the same state.  e_estimate is a
+    host constant of the state, fixed at capture (the reference keeps it fixed
+    too, prop/afqmc.py:138,146)."""
+
+    def __init__(self, state: prop_state, *, sys: system, params: afqmc_params, ham_data: ham_chol,
+                 trial_data: Any, meas_ops: meas_ops, prop_ops: chol_afqmc_ops,
+                 prop_ctx: chol_afqmc_ctx, meas_ctx: Any):
+        self._kw = dict(sys=sys, params=params, ham_data=ham_data, trial_data=trial_data,
+                        meas_ops=meas_ops, prop_ops=prop_ops, prop_ctx=prop_ctx, meas_ctx=meas_ctx)
+        if state.greens is None and meas_ops.greens is not None:
+            state = state._replace(greens=meas_ops.greens(state.walkers, trial_data))
+        self.state = _clone_state(state)
+        # one eager block on a copy first (lazy operand layouts, kernel
+        # attributes), on a side stream as graph capture requires
+        torch.cuda.synchronize()
+        side = torch.cuda.Stream()
+        side.wait_stream(torch.cuda.current_stream())
+        with torch.cuda.stream(side):
+            afqmc_block(_clone_state(self.state), **self._kw)
+        torch.cuda.current_stream().wait_stream(side)
+        torch.cuda.synchronize()
+        self.graph = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(self.graph):
+            out, obs = afqmc_block(self.state, **self._kw)
+            _copy_state_(self.state, out)
+            self.energy = obs.scalars["energy"]
+            self.weight = obs.scalars["weight"]
+        del out, obs
+
+    def replay(self):
+        self.graph.replay()
+        return self.energy, self.weight

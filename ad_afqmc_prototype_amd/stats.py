@@ -43,9 +43,13 @@ def blocking_analysis_ratio(ene, wt, block_grid=None, *, min_blocks=20, min_rise
         if n // cand[-1] >= 5 and cand[-1] not in block_grid:
             block_grid.append(int(cand[-1]))
     sizes, ses, counts, loo = [], [], [], {}
+    if not any(n // bsz >= 5 for bsz in block_grid):
+        # too few samples for any block size (the reference raises IndexError
+        # here): the block-size-1 jackknife, NaN below two samples
+        block_grid = [1]
     for bsz in block_grid:
         nb = n // bsz
-        if nb < 5:
+        if nb < 5 and not (bsz == 1 and len(block_grid) == 1 and nb >= 2):
             continue
         used = nb * bsz
         sn = num[:used].reshape(nb, bsz).sum(axis=1)
@@ -60,11 +64,20 @@ def blocking_analysis_ratio(ene, wt, block_grid=None, *, min_blocks=20, min_rise
         ses.append(float(np.sqrt(max(var, 0.0))))
         counts.append(nb)
         loo[bsz] = (mu_i, mbar, nb)
+    if not sizes:
+        return {"mu": float(mu), "block_sizes": np.zeros(0, int), "se_curve": np.zeros(0),
+                "n_blocks": np.zeros(0, int), "B_star": 1, "se_star": float("nan"),
+                "ci95_star": (float("nan"), float("nan")), "estimator_scale_samples": np.zeros(0),
+                "bias": None, "z_score": None}
     b_star, se_star, _ = _plateau(sizes, ses, counts, min_blocks=min_blocks,
                                   min_rise=min_rise, flat_tol=flat_tol, k=k)
+    mu_i, mbar, nb = loo[b_star]
     out = {"mu": float(mu), "block_sizes": np.array(sizes, int), "se_curve": np.array(ses),
            "n_blocks": np.array(counts, int), "B_star": b_star, "se_star": se_star,
-           "ci95_star": (float(mu - 1.96 * se_star), float(mu + 1.96 * se_star))}
+           "ci95_star": (float(mu - 1.96 * se_star), float(mu + 1.96 * se_star)),
+           # leave-one-block-out estimates rescaled to the spread of the estimator
+           "estimator_scale_samples": mu + (nb - 1) / np.sqrt(nb) * (mu_i - mbar),
+           "bias": None, "z_score": None}
     if exact is not None and np.isfinite(se_star) and se_star > 0:
         out["bias"] = float(mu - exact)
         out["z_score"] = float((mu - exact) / se_star)
@@ -90,3 +103,14 @@ def reject_outliers(data, obs, m=10.0, min_threshold=1e-5):
     disp = max(np.median(dev), (q3 - q1) / 1.349, min_threshold)
     mask = dev / disp < m
     return data[mask], mask
+
+
+def jackknife_ratios(num, denom):
+    """Jackknife mean and standard deviation of mean(num) / mean(denom)
+    (stat_utils.py:187-213): leave-one-out ratios r_i, sigma = sqrt((n-1) var(r))."""
+    num = np.asarray(num)
+    denom = np.asarray(denom)
+    n = num.size
+    tot_n, tot_d = num.sum(), denom.sum()
+    r = ((tot_n - num) / (n - 1) / ((tot_d - denom) / (n - 1))).real
+    return np.mean(r), np.sqrt((n - 1) * np.var(r))

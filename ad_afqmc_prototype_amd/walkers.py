@@ -98,7 +98,9 @@ def sr_indices(weights: torch.Tensor, zeta: float) -> torch.Tensor:
     return kernels.comb_indices(weights, zeta)[0]
 
 
-def stochastic_reconfiguration_with_indices(w, weights: torch.Tensor, zeta: float, walker_kind: str):
+def stochastic_reconfiguration_with_indices(w, weights: torch.Tensor, zeta, walker_kind: str):
+    """Comb resampling with the selected indices; zeta a float or a 0-d float64
+    device tensor."""
     wk = walker_kind.lower()
     if wk not in ("restricted", "unrestricted"):
         raise ValueError(f"unknown walker_kind: {walker_kind}")
@@ -108,7 +110,7 @@ def stochastic_reconfiguration_with_indices(w, weights: torch.Tensor, zeta: floa
     return kernels.gather_rows(w, idx), w_new, idx
 
 
-def stochastic_reconfiguration(w, weights: torch.Tensor, zeta: float, walker_kind: str):
+def stochastic_reconfiguration(w, weights: torch.Tensor, zeta, walker_kind: str):
     """walkers.py:243-265: comb resampling, weights reset to the mean |w|."""
     w_sr, w_new, _ = stochastic_reconfiguration_with_indices(w, weights, zeta, walker_kind)
     return w_sr, w_new

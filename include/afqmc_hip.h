@@ -235,6 +235,39 @@ int afq_comb_indices(int nw, const double* weights, double zeta, double* cumsum,
                      double* weights_new, void* stream);
 
 /*
+ * afq_comb_indices with zeta read from device memory (zeta: one float64, e.g.
+ * written by afq_rng_uniform on the same stream), so that the draw and the
+ * comb need no host round trip.  Same reference functions.
+ */
+int afq_comb_indices_dev(int nw, const double* weights, const double* zeta, double* cumsum,
+                         int* idx, double* weights_new, void* stream);
+
+/*
+ * The step's random stream on the device (prop/afqmc.py:32,86-88,
+ * prop/blocks.py:81-82: key, subkey = split(key); normal(subkey, shape) /
+ * uniform(subkey)), in jax's original Threefry-2x32 scheme (config.py:27).
+ * Keys are device uint32[2].
+ *   afq_rng_split:   (new_key, subkey) = split(key); the three distinct.
+ *   afq_rng_normal:  out[0..n) = sqrt(2) erfinv(uniform(nextafter(-1, 0), 1)),
+ *                    value i from threefry(key, (i, n + i)); n < 2^31.
+ *   afq_rng_uniform: out[0] = uniform in [0, 1) from threefry(key, (0, 1)).
+ * The uniforms follow jax's bit recipe; erfinv is the library's own (Halley
+ * iterations on erf / erfc), so normals match another erfinv to a few ulp.
+ */
+int afq_rng_split(const unsigned* key, unsigned* new_key, unsigned* subkey, void* stream);
+int afq_rng_normal(const unsigned* key, long long n, double* out, void* stream);
+int afq_rng_uniform(const unsigned* key, double* out, void* stream);
+
+/*
+ * The Threefry-2x32-20 block function itself (Salmon et al., SC 2011) on n
+ * counter pairs (c0[i], c1[i]) under key (k0, k1): (o0[i], o1[i]).  No
+ * reference counterpart (the reference reaches it through jax.random); exposed
+ * for known-answer tests.
+ */
+int afq_threefry2x32(unsigned k0, unsigned k1, long long n, const unsigned* c0, const unsigned* c1,
+                     unsigned* o0, unsigned* o1, void* stream);
+
+/*
  * dst[p][k][:] = src[p][idx[k]][:] for p < nplanes, rows of row_len doubles.
  * The walker gather of stochastic_reconfiguration (walkers.py:258-263).
  */

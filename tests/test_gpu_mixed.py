@@ -117,7 +117,8 @@ def test_mixed_force_bias(gpu_device, precision, tol):
 def test_mixed_block_energy_close_to_fp64(gpu_device, precision):
     """Common random numbers: the mixed run and the fp64 checker see the same
     fields and comb uniforms; block energies agree to well below 0.1 mHa on the
-    24-orbital workload (where the phaseless walk is in its working regime)."""
+    24-orbital workload at Cholesky scale 0.25 (the phaseless walk's working
+    regime: no local energy is clipped)."""
     import dataclasses
     import torch
     from ad_afqmc_prototype_amd.meas.rhf import make_rhf_meas_ops
@@ -127,7 +128,7 @@ def test_mixed_block_energy_close_to_fp64(gpu_device, precision):
     from ad_afqmc_prototype_amd.prop.types import prop_state
     from ad_afqmc_prototype_amd.trial import rhf as trial_rhf
     seed, n, o, g, nw, dt = 0, 24, 6, 96, 128, 0.01
-    p = make_pair(seed, n, o, g, dt=dt, device=gpu_device)
+    p = make_pair(seed, n, o, g, dt=dt, chol_scale=0.25, device=gpu_device)
     st0 = orc.init_state(p.sd, nw, overlap_mode="log")
     fields = [syn.fields_for_step(seed, t, nw, g) for t in range(40)]
     zetas = [syn.zeta_for_event(seed, k) for k in range(4)]
@@ -152,6 +153,9 @@ def test_mixed_block_energy_close_to_fp64(gpu_device, precision):
                               meas_ctx=m_ctx, fields=f, zeta=zetas[k])
         worst = max(worst, abs(obs.scalars["energy"].item() - blocks[k]["energy"]))
     assert worst < 1e-4, worst  # Ha
+    # the walk is live: block energies move and none is the clipped estimate
+    e = np.array([b["energy"] for b in blocks])
+    assert np.ptp(e) > 1e-3 and np.all(np.abs(e - st0.e_estimate) > 1e-6), e
 
 
 @pytest.mark.parametrize("case", [

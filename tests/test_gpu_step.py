@@ -50,7 +50,10 @@ def _cmp_state(gs, st, wtol=TOL):
     lo = gs.log_overlaps.cpu().numpy()
     np.testing.assert_allclose(lo.real, st.overlaps.real, rtol=1e-11, atol=10 * wtol)
     np.testing.assert_allclose(np.exp(1j * lo.imag), np.exp(1j * st.overlaps.imag), atol=10 * wtol)
-    np.testing.assert_allclose(gs.pop_control_ene_shift.item(), st.pop_control_ene_shift, rtol=1e-11)
+    # pop shift = e_est - 0.1 log(mean w) / dt: a relative weight difference d
+    # moves it by ~0.1 d / dt absolute (10 d at dt = 0.01)
+    np.testing.assert_allclose(gs.pop_control_ene_shift.item(), st.pop_control_ene_shift,
+                               rtol=max(1e-11, wtol / 10))
 
 
 @pytest.mark.parametrize("case", [
@@ -150,13 +153,21 @@ def _gpu_block(p, gs, fields, zeta, n_prop, n_ortho):
 
 
 @pytest.mark.parametrize("case", [
-    dict(seed=0, n=24, o=6, g=96, nw=128, uhf=False, kind="restricted", dt=0.01, steps=40, qr=5, sr=10),
-    dict(seed=6, n=30, o=5, g=90, nw=64, uhf=True, kind="unrestricted", dt=0.01, steps=20, qr=5, sr=10),
+    # Cholesky scale 1: the literal generator of the 24-orbital config; its local
+    # energies leave the sqrt(2/dt) window, so the block energies are the clipped
+    # estimate and the comparison rests on weights, comb indices and walkers
+    dict(seed=0, n=24, o=6, g=96, nw=128, uhf=False, kind="restricted", dt=0.01, steps=40, qr=5, sr=10,
+         scale=1.0),
+    # the working regime: block energies move and are compared as values
+    dict(seed=0, n=24, o=6, g=96, nw=128, uhf=False, kind="restricted", dt=0.01, steps=40, qr=5, sr=10,
+         scale=0.25),
+    dict(seed=6, n=30, o=5, g=90, nw=64, uhf=True, kind="unrestricted", dt=0.01, steps=20, qr=5, sr=10,
+         scale=0.25),
 ])
 def test_blocks_match_checker(gpu_device, case):
     import torch
     p = make_pair(case["seed"], case["n"], case["o"], case["g"], uhf=case["uhf"], kind=case["kind"],
-                  dt=case["dt"], device=gpu_device)
+                  dt=case["dt"], chol_scale=case["scale"], device=gpu_device)
     st0 = orc.init_state(p.sd, case["nw"], overlap_mode="log")
     fields = [syn.fields_for_step(case["seed"], t, case["nw"], case["g"]) for t in range(case["steps"])]
     zetas = [syn.zeta_for_event(case["seed"], k) for k in range(case["steps"] // case["sr"])]
@@ -169,8 +180,14 @@ def test_blocks_match_checker(gpu_device, case):
         gs, obs, idx = _gpu_block(p, gs, f, zetas[k], case["sr"], case["qr"])
         ref = blocks[k]
         np.testing.assert_allclose(obs.scalars["energy"].item(), ref["energy"], rtol=1e-10)
-        np.testing.assert_allclose(obs.scalars["weight"].item(), ref["weight"], rtol=1e-10)
+        # the weight sum carries the per-step rounding differences (~1e-12) of
+        # every earlier step through the overlap ratios; after 30-40 steps of a
+        # live walk it agrees to ~1e-10
+        np.testing.assert_allclose(obs.scalars["weight"].item(), ref["weight"], rtol=1e-9)
         np.testing.assert_array_equal(idx.cpu().numpy(), ref["sr_indices"])
+    if case["scale"] < 1.0:
+        e = np.array([b["energy"] for b in blocks])
+        assert np.ptp(e) > 1e-3 and np.all(np.abs(e - st0.e_estimate) > 1e-6), e
     # 40 steps and 4 reconfigurations amplify rounding differences (the
     # per-step agreement is 1e-10); block energies above agree to 1e-10.
     _cmp_state(gs, st, wtol=1e-8)
