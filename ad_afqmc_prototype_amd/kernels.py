@@ -27,10 +27,30 @@ _timing = None
 
 class timing:
     """Context manager: records (start, end) HIP events around every library
-    call, on the stream the kernel is enqueued on."""
+    call, on the stream the kernel is enqueued on.
 
-    def __init__(self):
+    ``pool`` event pairs are created (and recorded once, then synchronized)
+    before the context is entered: a freshly created HIP event costs ~10x a
+    launch on its first record, which would otherwise land inside the timed
+    region.  Calls beyond the pool create events on the fly."""
+
+    def __init__(self, pool: int = 0):
         self.events = {}
+        self._pool = []
+        if pool > 0:
+            st = torch.cuda.current_stream()
+            for _ in range(pool):
+                pair = (torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+                pair[0].record(st)
+                pair[1].record(st)
+                self._pool.append(pair)
+            torch.cuda.synchronize()
+            self._pool.reverse()
+
+    def _pair(self):
+        if self._pool:
+            return self._pool.pop()
+        return torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
 
     def __enter__(self):
         global _timing
@@ -56,8 +76,7 @@ def _call(name, stream_tensor, *args, tag=None):
         _lib.call(name, *args)
         return
     st = torch.cuda.current_stream(stream_tensor.device)
-    e0 = torch.cuda.Event(enable_timing=True)
-    e1 = torch.cuda.Event(enable_timing=True)
+    e0, e1 = _timing._pair()
     e0.record(st)
     _lib.call(name, *args)
     e1.record(st)

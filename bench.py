@@ -190,6 +190,35 @@ def accuracy_check(workload: str, precision: str, device, chol_scale: float, n_b
             "free_comb_picks_differing": comb_diff}
 
 
+def end_to_end(workload: str, precision: str, device, n_eql: int, n_blocks: int):
+    """The driver (equilibration + sampling blocks, blocking analysis) on the
+    workload with the device random stream (fields and comb uniforms from the
+    seed's Threefry keys: the fp64 and the mixed run see the same stream), one
+    captured HIP graph per block."""
+    import dataclasses
+
+    import torch
+
+    from ad_afqmc_prototype_amd.driver import run_afqmc_energy
+
+    ctx = build(workload, device, precision)
+    cfg = ctx["cfg"]
+    params = dataclasses.replace(ctx["params"], n_eql_blocks=n_eql, n_blocks=n_blocks, seed=cfg["seed"])
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    mu, se, e_b, w_b = run_afqmc_energy(sys=ctx["sys"], params=params, ham_data=ctx["ham"],
+                                        trial_data=ctx["trial"], meas_ops=ctx["m_ops"],
+                                        trial_ops=ctx["t_ops"], prop_ops=ctx["p_ops"], verbose=False,
+                                        use_graph=True)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    del ctx
+    torch.cuda.empty_cache()
+    return {"mean_Ha": mu, "se_Ha": se, "n_eql_blocks": n_eql, "n_blocks": n_blocks,
+            "steps_per_block": params.n_prop_steps, "wall_s": round(wall, 2),
+            "block_energies_Ha": [float(x) for x in e_b]}
+
+
 def measured_traffic(workload: str, kernel: str):
     """HBM bytes per launch of ``kernel`` from the committed rocprofv3 FETCH_SIZE /
     WRITE_SIZE summaries (profiles/), each scaled by its copy-kernel
@@ -257,7 +286,8 @@ def timed_run(args, precision, device):
     state = run_steps(ctx, ctx["state"], fields, 0, args.warmup, cadence=not args.no_cadence)
     torch.cuda.synchronize()
     energies = []
-    with kernels.timing() as tm:
+    # ~10 library calls per step plus the QR / measurement cadence
+    with kernels.timing(pool=16 * args.steps + 64) as tm:
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         state = run_steps(ctx, state, fields, args.warmup, args.steps, cadence=not args.no_cadence,
@@ -288,6 +318,9 @@ def main():
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run a plain device copy of BYTES bytes 5 times (the known-size "
                          "kernel that calibrates rocprofv3 FETCH_SIZE / WRITE_SIZE)")
+    ap.add_argument("--end-to-end", type=int, nargs=2, default=(5, 20), metavar=("EQL", "SAMPLE"),
+                    help="driver run (equilibration and sampling blocks) for the end_to_end field; "
+                         "0 0 skips it")
     ap.add_argument("--no-cadence", action="store_true",
                     help="time afqmc_step alone (no QR / measurement / reconfiguration)")
     args = ap.parse_args()
@@ -361,6 +394,22 @@ def main():
         scale = cfg["chol_scale"] if args.accuracy_scale is None else args.accuracy_scale
         out["energy_check"] = accuracy_check(args.workload, args.precision, device, scale,
                                              args.accuracy_blocks)
+    if args.end_to_end[1] > 0:
+        e2e = {prec: end_to_end(args.workload, prec, device, args.end_to_end[0], args.end_to_end[1])
+               for prec in ([args.precision, "fp64"] if args.precision != "fp64" else ["fp64"])}
+        if args.precision != "fp64":
+            a, b = e2e[args.precision], e2e["fp64"]
+            e2e["mean_diff_mHa"] = 1e3 * (a["mean_Ha"] - b["mean_Ha"])
+            e2e["se_fp64_mHa"] = 1e3 * b["se_Ha"]
+            # Both runs draw the same fields and comb uniforms, so they follow one
+            # trajectory until a comb pick differs (a uniform within the weight
+            # perturbation of a cumulative-weight boundary); from then on they are
+            # independent chains and only agree statistically.
+            diffs = [1e3 * abs(x - y) for x, y in zip(a["block_energies_Ha"], b["block_energies_Ha"])]
+            same = next((i for i, d in enumerate(diffs) if d > 0.01), len(diffs))
+            e2e["blocks_on_common_trajectory"] = same
+            e2e["max_abs_diff_mHa_common_trajectory"] = max(diffs[:same], default=None)
+        out["end_to_end"] = e2e
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(args.workload)
     print(json.dumps(out), flush=True)
