@@ -53,9 +53,18 @@ _SIGNATURES = {
     "afq_rng_uniform": [c_vp, c_vp, c_vp],
     "afq_threefry2x32": [ctypes.c_uint, ctypes.c_uint, c_ll, c_vp, c_vp, c_vp, c_vp, c_vp],
     "afq_gather_rows": [c_int, c_ll, c_int, c_ll, c_ll, c_vp, c_vp, c_vp, c_vp],
+    # grouped launches over many systems (descriptor tables in device memory)
+    "afq_gemm_tile_shape": [c_int, ctypes.POINTER(c_int), ctypes.POINTER(c_int)],
+    "afq_sgemm_nt_grouped": [c_int, c_vp, c_int, c_int, c_vp],
+    "afq_rng_fields_grouped": [c_int, c_vp, c_int, c_vp],
+    "afq_field_shifts_grouped": [c_int, c_vp, c_int, c_vp],
+    "afq_trotter_f32_grouped": [c_int, c_vp, c_int, c_int, c_int, c_vp],
+    "afq_overlap_greens_grouped": [c_int, c_vp, c_int, c_int, c_vp],
+    "afq_phaseless_update_grouped": [c_int, c_vp, c_vp],
+    "afq_orthonormalize_grouped": [c_int, c_vp, c_int, c_int, c_int, c_vp],
 }
 
-ABI_VERSION = 1
+ABI_VERSION = 2
 
 
 class HipLibraryError(RuntimeError):

@@ -79,6 +79,22 @@ struct vec4<float> {
     using type = float4;
 };
 
+// Index of the last group whose first item start(i) is <= x, for ascending
+// starts with start(0) == 0 (binary search; x is uniform across the
+// workgroup, so the loads are scalar).
+template <typename F>
+__device__ __forceinline__ int find_group(int count, int x, F start) {
+    int lo = 0, hi = count - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (start(mid) <= x)
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    return lo;
+}
+
 __device__ __forceinline__ double wave_sum(double v) {
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);

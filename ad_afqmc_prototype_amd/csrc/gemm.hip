@@ -22,71 +22,73 @@
 
 namespace {
 
-constexpr int BM = 128;
-constexpr int BN = 128;
 constexpr int BK = 16;
 constexpr int PITCH = BK + 2;  // 144-byte rows: conflict-free 16-byte LDS reads
 constexpr int THREADS = 256;
 constexpr int GROUP_M = 8;
-constexpr int LOADS = BM * BK / THREADS;  // 8 elements of A and of B per thread
 
-template <typename T, typename TOut>
-__global__ __launch_bounds__(THREADS, 2) void gemm_nt_kernel(
-    int M, int N, int K, T alpha, const T* __restrict__ A, long long lda,
-    const T* __restrict__ B, long long ldb, T beta, TOut* __restrict__ C,
-    long long ldc, int tiles_m, int tiles_n) {
+// One workgroup computes a BM x BN tile of C, BM = 32 TM, BN = 32 TN: four
+// waves in 2 x 2, each TM x TN MFMA tiles of 16 x 16.
+template <typename T, int TM, int TN>
+struct tile_cfg {
+    static constexpr int BM = 32 * TM;
+    static constexpr int BN = 32 * TN;
+    static constexpr int LA = BM * BK / THREADS;  // A elements loaded per thread per k-block
+    static constexpr int LB = BN * BK / THREADS;
+    using acc_t = typename afq::mfma16<T>::acc;
+};
+
+// acc = A[m0:m0+BM, :] . B[n0:n0+BN, :]^T over all of K, through
+// double-buffered LDS (As: 2 * BM * PITCH, Bs: 2 * BN * PITCH elements).
+template <typename T, int TM, int TN>
+__device__ __forceinline__ void gemm_nt_tile(int M, int N, int K, const T* __restrict__ A,
+                                             long long lda, const T* __restrict__ B, long long ldb,
+                                             int m0, int n0, T* As, T* Bs,
+                                             typename tile_cfg<T, TM, TN>::acc_t (&acc)[TM][TN]) {
+    using C = tile_cfg<T, TM, TN>;
     using MF = afq::mfma16<T>;
-    __shared__ T As[2][BM * PITCH];
-    __shared__ T Bs[2][BN * PITCH];
-
-    const int total = tiles_m * tiles_n;
-    const int t = afq::xcd_contiguous(blockIdx.x, total);
-    const int per_group = GROUP_M * tiles_n;
-    const int group = t / per_group;
-    const int first_m = group * GROUP_M;
-    const int gsize = min(tiles_m - first_m, GROUP_M);
-    const int in_group = t - group * per_group;
-    const int tm = first_m + in_group % gsize;
-    const int tn = in_group / gsize;
-    const int m0 = tm * BM;
-    const int n0 = tn * BN;
-
     const int tid = threadIdx.x;
     const int lane = tid & 63;
     const int wave = tid >> 6;
     const int wm = wave >> 1;
     const int wn = wave & 1;
 
-    T ra[LOADS], rb[LOADS];
+    T ra[C::LA], rb[C::LB];
     auto load_tile = [&](int k0) {
 #pragma unroll
-        for (int i = 0; i < LOADS; ++i) {
+        for (int i = 0; i < C::LA; ++i) {
             const int e = tid + THREADS * i;
             const int r = e >> 4;
-            const int c = e & 15;
-            const int gk = k0 + c;
+            const int gk = k0 + (e & 15);
             const int gm = m0 + r;
-            const int gn = n0 + r;
             ra[i] = (gm < M && gk < K) ? A[(long long)gm * lda + gk] : T(0);
+        }
+#pragma unroll
+        for (int i = 0; i < C::LB; ++i) {
+            const int e = tid + THREADS * i;
+            const int r = e >> 4;
+            const int gk = k0 + (e & 15);
+            const int gn = n0 + r;
             rb[i] = (gn < N && gk < K) ? B[(long long)gn * ldb + gk] : T(0);
         }
     };
     auto store_tile = [&](int buf) {
 #pragma unroll
-        for (int i = 0; i < LOADS; ++i) {
+        for (int i = 0; i < C::LA; ++i) {
             const int e = tid + THREADS * i;
-            const int r = e >> 4;
-            const int c = e & 15;
-            As[buf][r * PITCH + c] = ra[i];
-            Bs[buf][r * PITCH + c] = rb[i];
+            As[buf * C::BM * PITCH + (e >> 4) * PITCH + (e & 15)] = ra[i];
+        }
+#pragma unroll
+        for (int i = 0; i < C::LB; ++i) {
+            const int e = tid + THREADS * i;
+            Bs[buf * C::BN * PITCH + (e >> 4) * PITCH + (e & 15)] = rb[i];
         }
     };
 
-    typename MF::acc acc[4][4];
 #pragma unroll
-    for (int i = 0; i < 4; ++i)
+    for (int i = 0; i < TM; ++i)
 #pragma unroll
-        for (int j = 0; j < 4; ++j) acc[i][j] = MF::zero();
+        for (int j = 0; j < TN; ++j) acc[i][j] = MF::zero();
 
     const int nk = afq::ceil_div(K, BK);
     load_tile(0);
@@ -98,32 +100,44 @@ __global__ __launch_bounds__(THREADS, 2) void gemm_nt_kernel(
     for (int kt = 0; kt < nk; ++kt) {
         const int buf = kt & 1;
         if (kt + 1 < nk) load_tile((kt + 1) * BK);
-
+        const T* as = As + buf * C::BM * PITCH;
+        const T* bs = Bs + buf * C::BN * PITCH;
 #pragma unroll
         for (int kk = 0; kk < 4; ++kk) {
-            T a[4], b[4];
+            T a[TM], b[TN];
 #pragma unroll
-            for (int i = 0; i < 4; ++i) a[i] = As[buf][(wm * 64 + i * 16 + lr) * PITCH + lk + kk];
+            for (int i = 0; i < TM; ++i) a[i] = as[(wm * 16 * TM + i * 16 + lr) * PITCH + lk + kk];
 #pragma unroll
-            for (int j = 0; j < 4; ++j) b[j] = Bs[buf][(wn * 64 + j * 16 + lr) * PITCH + lk + kk];
+            for (int j = 0; j < TN; ++j) b[j] = bs[(wn * 16 * TN + j * 16 + lr) * PITCH + lk + kk];
 #pragma unroll
-            for (int i = 0; i < 4; ++i)
+            for (int i = 0; i < TM; ++i)
 #pragma unroll
-                for (int j = 0; j < 4; ++j) acc[i][j] = MF::fma(a[i], b[j], acc[i][j]);
+                for (int j = 0; j < TN; ++j) acc[i][j] = MF::fma(a[i], b[j], acc[i][j]);
         }
-
         if (kt + 1 < nk) store_tile(buf ^ 1);
         __syncthreads();
     }
+}
 
+// C[m0.., n0..] = alpha acc + beta C
+template <typename T, typename TOut, int TM, int TN>
+__device__ __forceinline__ void gemm_store(int M, int N, T alpha, T beta, TOut* __restrict__ C,
+                                           long long ldc, int m0, int n0,
+                                           const typename tile_cfg<T, TM, TN>::acc_t (&acc)[TM][TN]) {
+    using MF = afq::mfma16<T>;
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    const int wm = wave >> 1;
+    const int wn = wave & 1;
+    const int lr = lane & 15;
 #pragma unroll
-    for (int i = 0; i < 4; ++i) {
+    for (int i = 0; i < TM; ++i) {
 #pragma unroll
-        for (int j = 0; j < 4; ++j) {
-            const int col = n0 + wn * 64 + j * 16 + lr;
+        for (int j = 0; j < TN; ++j) {
+            const int col = n0 + wn * 16 * TN + j * 16 + lr;
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
-                const int row = m0 + wm * 64 + i * 16 + MF::row(lane, r);
+                const int row = m0 + wm * 16 * TM + i * 16 + MF::row(lane, r);
                 if (row < M && col < N) {
                     TOut* cp = C + (long long)row * ldc + col;
                     T v = alpha * acc[i][j][r];
@@ -134,6 +148,58 @@ __global__ __launch_bounds__(THREADS, 2) void gemm_nt_kernel(
         }
     }
 }
+
+template <typename T, typename TOut>
+__global__ __launch_bounds__(THREADS, 2) void gemm_nt_kernel(
+    int M, int N, int K, T alpha, const T* __restrict__ A, long long lda,
+    const T* __restrict__ B, long long ldb, T beta, TOut* __restrict__ C,
+    long long ldc, int tiles_m, int tiles_n) {
+    using CF = tile_cfg<T, 4, 4>;
+    __shared__ T As[2 * CF::BM * PITCH];
+    __shared__ T Bs[2 * CF::BN * PITCH];
+
+    const int total = tiles_m * tiles_n;
+    const int t = afq::xcd_contiguous(blockIdx.x, total);
+    const int per_group = GROUP_M * tiles_n;
+    const int group = t / per_group;
+    const int first_m = group * GROUP_M;
+    const int gsize = min(tiles_m - first_m, GROUP_M);
+    const int in_group = t - group * per_group;
+    const int m0 = (first_m + in_group % gsize) * CF::BM;
+    const int n0 = (in_group / gsize) * CF::BN;
+
+    typename CF::acc_t acc[4][4];
+    gemm_nt_tile<T, 4, 4>(M, N, K, A, lda, B, ldb, m0, n0, As, Bs, acc);
+    gemm_store<T, TOut, 4, 4>(M, N, alpha, beta, C, ldc, m0, n0, acc);
+}
+
+// Grouped fp32 GEMM: workgroup -> (problem, tile) by the problems' tile_start
+// prefix sums; the linear workgroup id is first made XCD-contiguous, so each
+// XCD runs a contiguous range of tiles (of one or a few problems) and shares
+// their panels in its L2.
+template <int TM, int TN>
+__global__ __launch_bounds__(THREADS, 2) void sgemm_grouped_kernel(int count, int total,
+                                                                   const afq_gemm_problem* __restrict__ P) {
+    using CF = tile_cfg<float, TM, TN>;
+    __shared__ float As[2 * CF::BM * PITCH];
+    __shared__ float Bs[2 * CF::BN * PITCH];
+    const int t = afq::xcd_contiguous(blockIdx.x, total);
+    const int pi = afq::find_group(count, t, [&](int i) { return P[i].tile_start; });
+    const afq_gemm_problem& p = P[pi];
+    const int M = p.M, N = p.N;
+    const int tiles_m = afq::ceil_div(M, CF::BM);
+    const int local = t - p.tile_start;
+    const int m0 = (local % tiles_m) * CF::BM;
+    const int n0 = (local / tiles_m) * CF::BN;
+    typename CF::acc_t acc[TM][TN];
+    gemm_nt_tile<float, TM, TN>(M, N, p.K, p.A, p.lda, p.B, p.ldb, m0, n0, As, Bs, acc);
+    if (p.c_fp64)
+        gemm_store<float, double, TM, TN>(M, N, p.alpha, 0.0f, (double*)p.C, p.ldc, m0, n0, acc);
+    else
+        gemm_store<float, float, TM, TN>(M, N, p.alpha, 0.0f, (float*)p.C, p.ldc, m0, n0, acc);
+}
+
+constexpr int GROUPED_SHAPES[4][2] = {{4, 4}, {2, 4}, {4, 2}, {2, 2}};
 
 }  // namespace
 
@@ -146,8 +212,8 @@ int gemm_nt(const char* name, int M, int N, int K, T alpha, const T* A, long lon
     if (M == 0 || N == 0) return 0;
     AFQ_REQUIRE(A && B && C, "%s: null pointer", name);
     AFQ_REQUIRE(lda >= K && ldb >= K && ldc >= N, "%s: leading dimension too small", name);
-    const int tiles_m = ceil_div(M, BM);
-    const int tiles_n = ceil_div(N, BN);
+    const int tiles_m = ceil_div(M, tile_cfg<T, 4, 4>::BM);
+    const int tiles_n = ceil_div(N, tile_cfg<T, 4, 4>::BN);
     const long long total = (long long)tiles_m * tiles_n;
     AFQ_REQUIRE(total < (1LL << 31), "%s: problem too large", name);
     hipLaunchKernelGGL((gemm_nt_kernel<T, TOut>), dim3((unsigned)total), dim3(THREADS), 0, stream, M, N, K,
@@ -193,4 +259,28 @@ extern "C" int afq_vhs(int nw, int n, int ldv, int naux, const double* xbar, con
     AFQ_REQUIRE(ldv >= n && ldv % 4 == 0, "afq_vhs: ldv=%d must be >= n=%d and a multiple of 4", ldv, n);
     return afq::dgemm_nt(2 * nw, n * ldv, naux, 1.0, xbar, naux, chol_t, naux, 0.0, vhs,
                          (long long)n * ldv, (hipStream_t)stream);
+}
+
+extern "C" int afq_gemm_tile_shape(int shape, int* bm, int* bn) {
+    AFQ_REQUIRE(shape >= 0 && shape < 4, "afq_gemm_tile_shape: shape=%d not in 0..3", shape);
+    AFQ_REQUIRE(bm && bn, "afq_gemm_tile_shape: null pointer");
+    *bm = 32 * GROUPED_SHAPES[shape][0];
+    *bn = 32 * GROUPED_SHAPES[shape][1];
+    return 0;
+}
+
+extern "C" int afq_sgemm_nt_grouped(int count, const afq_gemm_problem* problems, int total_tiles,
+                                    int shape, void* stream) {
+    AFQ_REQUIRE(count > 0 && total_tiles > 0, "afq_sgemm_nt_grouped: count=%d total_tiles=%d",
+                count, total_tiles);
+    AFQ_REQUIRE(problems, "afq_sgemm_nt_grouped: null pointer");
+    AFQ_REQUIRE(shape >= 0 && shape < 4, "afq_sgemm_nt_grouped: shape=%d not in 0..3", shape);
+    hipStream_t st = (hipStream_t)stream;
+    switch (shape) {
+        case 0: hipLaunchKernelGGL((sgemm_grouped_kernel<4, 4>), dim3(total_tiles), dim3(THREADS), 0, st, count, total_tiles, problems); break;
+        case 1: hipLaunchKernelGGL((sgemm_grouped_kernel<2, 4>), dim3(total_tiles), dim3(THREADS), 0, st, count, total_tiles, problems); break;
+        case 2: hipLaunchKernelGGL((sgemm_grouped_kernel<4, 2>), dim3(total_tiles), dim3(THREADS), 0, st, count, total_tiles, problems); break;
+        default: hipLaunchKernelGGL((sgemm_grouped_kernel<2, 2>), dim3(total_tiles), dim3(THREADS), 0, st, count, total_tiles, problems); break;
+    }
+    return afq::check_launch("afq_sgemm_nt_grouped");
 }

@@ -35,21 +35,17 @@ __device__ __forceinline__ double2 crecip(double2 a) {
     return make_double2(r / d, -1.0 / d);
 }
 
+// One (walker, spin): W (n x o complex), Cm (n x o real) -> G^T rows into
+// gre / gim (o x n each), log det into *ld.
 template <int MAXO>
-__global__ __launch_bounds__(THREADS) void greens_kernel(
-    int nw, int n, int o, int nspin, const double2* __restrict__ wup,
-    const double2* __restrict__ wdn, const double* __restrict__ cup, const double* __restrict__ cdn,
-    double* __restrict__ greens, double2* __restrict__ logdet) {
+__device__ __forceinline__ void greens_body(int n, int o, const double2* __restrict__ W,
+                                            const double* __restrict__ Cm, double* __restrict__ gre,
+                                            double* __restrict__ gim, double2* __restrict__ ld) {
     __shared__ double2 m[MAXO][MAXO + 1];
     __shared__ double2 dinv[MAXO];
     __shared__ int perm[MAXO];
     __shared__ int piv_s;
 
-    const int b = blockIdx.x;
-    const int w = b / nspin;
-    const int s = b - w * nspin;
-    const double2* W = (s == 0 ? wup : wdn) + (long long)w * n * o;
-    const double* Cm = (s == 0 ? cup : cdn);
     const int tid = threadIdx.x;
     const int lane = tid & 63;
     const int wave = tid >> 6;
@@ -151,14 +147,11 @@ __global__ __launch_bounds__(THREADS) void greens_kernel(
             li += atan2(m[k][k].y, m[k][k].x);
         }
         if (nswap & 1) li += M_PI;
-        logdet[(long long)w * nspin + s] = make_double2(lr, li);
+        *ld = make_double2(lr, li);
     }
     __syncthreads();
 
     // 4. row solves x m = w_p
-    const long long ldg = (long long)nspin * o * n;
-    double* gre = greens + (long long)w * ldg + (long long)s * o * n;
-    double* gim = gre + (long long)nw * ldg;
     for (int p = tid; p < n; p += THREADS) {
         // keep the LU factors in LDS: without this barrier the compiler hoists
         // all o^2 loads out of the row loop into registers and spills
@@ -204,6 +197,35 @@ __global__ __launch_bounds__(THREADS) void greens_kernel(
     }
 }
 
+template <int MAXO>
+__global__ __launch_bounds__(THREADS) void greens_kernel(
+    int nw, int n, int o, int nspin, const double2* __restrict__ wup,
+    const double2* __restrict__ wdn, const double* __restrict__ cup, const double* __restrict__ cdn,
+    double* __restrict__ greens, double2* __restrict__ logdet) {
+    const int b = blockIdx.x;
+    const int w = b / nspin;
+    const int s = b - w * nspin;
+    const long long ldg = (long long)nspin * o * n;
+    double* gre = greens + (long long)w * ldg + (long long)s * o * n;
+    greens_body<MAXO>(n, o, (s == 0 ? wup : wdn) + (long long)w * n * o, s == 0 ? cup : cdn, gre,
+                      gre + (long long)nw * ldg, logdet + (long long)w * nspin + s);
+}
+
+// One workgroup per walker of every system: Green's function and log det of
+// the system's walkers_out.
+template <int MAXO>
+__global__ __launch_bounds__(THREADS) void greens_grouped_kernel(int count,
+                                                                 const afq_system_step* __restrict__ S) {
+    const int b = blockIdx.x;
+    const int si = afq::find_group(count, b, [&](int i) { return S[i].walker0; });
+    const afq_system_step& d = S[si];
+    const int w = b - d.walker0;
+    const int n = d.n, o = d.o;
+    double* gre = d.greens + (long long)w * o * n;
+    greens_body<MAXO>(n, o, (const double2*)d.walkers_out + (long long)w * n * o, d.trial, gre,
+                      gre + (long long)d.nw * o * n, (double2*)d.logdet + w);
+}
+
 }  // namespace
 
 extern "C" int afq_overlap_greens(int nw, int n, int o, int nspin, const double* walkers_up,
@@ -230,4 +252,21 @@ extern "C" int afq_overlap_greens(int nw, int n, int o, int nspin, const double*
     else
         hipLaunchKernelGGL(greens_kernel<64>, grid, dim3(THREADS), 0, st, nw, n, o, nspin, wu, wd, trial_up, cd, greens, ld);
     return afq::check_launch("afq_overlap_greens");
+}
+
+extern "C" int afq_overlap_greens_grouped(int count, const afq_system_step* sys, int total_walkers,
+                                          int max_o, void* stream) {
+    AFQ_REQUIRE(count > 0 && total_walkers > 0, "afq_overlap_greens_grouped: count=%d total_walkers=%d",
+                count, total_walkers);
+    AFQ_REQUIRE(sys, "afq_overlap_greens_grouped: null pointer");
+    AFQ_REQUIRE(max_o > 0 && max_o <= 32, "afq_overlap_greens_grouped: max_o=%d not in 1..32", max_o);
+    const dim3 grid((unsigned)total_walkers);
+    hipStream_t st = (hipStream_t)stream;
+    if (max_o <= 8)
+        hipLaunchKernelGGL(greens_grouped_kernel<8>, grid, dim3(THREADS), 0, st, count, sys);
+    else if (max_o <= 16)
+        hipLaunchKernelGGL(greens_grouped_kernel<16>, grid, dim3(THREADS), 0, st, count, sys);
+    else
+        hipLaunchKernelGGL(greens_grouped_kernel<32>, grid, dim3(THREADS), 0, st, count, sys);
+    return afq::check_launch("afq_overlap_greens_grouped");
 }

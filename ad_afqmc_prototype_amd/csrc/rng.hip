@@ -87,18 +87,40 @@ __global__ void split_kernel(const uint32_t* __restrict__ key, uint32_t* __restr
     subkey[1] = sk[1];
 }
 
+// value i of normal(key, n values)
+__device__ __forceinline__ double normal_value(uint32_t k0, uint32_t k1, long long n, long long i) {
+    const double lo = -1.0 + 0x1p-53;  // nextafter(-1, 0)
+    const double scale = 1.0 - lo;     // rounds to 2
+    uint32_t a = (uint32_t)i, b = (uint32_t)(n + i);
+    threefry2x32(k0, k1, a, b);
+    const double f = unit_from_bits(a, b);
+    return sqrt2_erfinv(fmax(lo, f * scale + lo));
+}
+
 __global__ void normal_kernel(const uint32_t* __restrict__ key, long long n, double* __restrict__ out) {
     const uint32_t k0 = key[0], k1 = key[1];
-    const double lo = -1.0 + 0x1p-53;  // nextafter(-1, 0)
-    const double scale = 1.0 - lo;           // rounds to 2
     for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-         i += (long long)gridDim.x * blockDim.x) {
-        uint32_t a = (uint32_t)i, b = (uint32_t)(n + i);
-        threefry2x32(k0, k1, a, b);
-        const double f = unit_from_bits(a, b);
-        const double v = fmax(lo, f * scale + lo);
-        out[i] = sqrt2_erfinv(v);
+         i += (long long)gridDim.x * blockDim.x)
+        out[i] = normal_value(k0, k1, n, i);
+}
+
+// One workgroup per walker row of every system: key_out, sub = split(key_in);
+// fields = normal(sub, (nw, naux)).
+__global__ __launch_bounds__(256) void fields_grouped_kernel(int count,
+                                                             const afq_system_step* __restrict__ S) {
+    const int b = blockIdx.x;
+    const int si = afq::find_group(count, b, [&](int i) { return S[i].walker0; });
+    const afq_system_step& d = S[si];
+    const int w = b - d.walker0;
+    uint32_t nk[2], sk[2];
+    split_key((const uint32_t*)d.key_in, nk, sk);
+    if (w == 0 && threadIdx.x == 0) {
+        d.key_out[0] = nk[0];
+        d.key_out[1] = nk[1];
     }
+    const long long n = (long long)d.nw * d.naux;
+    const long long row = (long long)w * d.naux;
+    for (int g = threadIdx.x; g < d.naux; g += 256) d.fields[row + g] = normal_value(sk[0], sk[1], n, row + g);
 }
 
 __global__ void uniform_kernel(const uint32_t* __restrict__ key, double* __restrict__ out) {
@@ -157,4 +179,14 @@ extern "C" int afq_threefry2x32(unsigned k0, unsigned k1, long long n, const uns
     hipLaunchKernelGGL(threefry_kernel, dim3(grid_for(n)), dim3(256), 0, (hipStream_t)stream, k0, k1, n,
                        (const uint32_t*)c0, (const uint32_t*)c1, (uint32_t*)o0, (uint32_t*)o1);
     return afq::check_launch("afq_threefry2x32");
+}
+
+extern "C" int afq_rng_fields_grouped(int count, const afq_system_step* sys, int total_walkers,
+                                      void* stream) {
+    AFQ_REQUIRE(count > 0 && total_walkers > 0, "afq_rng_fields_grouped: count=%d total_walkers=%d",
+                count, total_walkers);
+    AFQ_REQUIRE(sys, "afq_rng_fields_grouped: null pointer");
+    hipLaunchKernelGGL(fields_grouped_kernel, dim3((unsigned)total_walkers), dim3(256), 0,
+                       (hipStream_t)stream, count, sys);
+    return afq::check_launch("afq_rng_fields_grouped");
 }

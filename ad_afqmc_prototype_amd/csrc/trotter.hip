@@ -243,47 +243,28 @@ int launch_trotter(int nw, int n, int o, int nspin, int ldv, int n_terms, double
 // walker run back to back on one XCD and share V through L2; V crosses HBM
 // once per walker instead of once per Taylor term.
 // ---------------------------------------------------------------------------
+// One 16-column tile (first column c0) of one walker: Win -> Wout with the
+// walker's V planes Vr / Vi.  LDS holds T for NPL >= round16(n) rows; waves
+// beyond round16(n) / 16 (a workgroup sized for a larger system) compute on
+// zero rows, write only LDS rows no other wave reads, and store nothing.
 template <int KRB>
-__global__ __launch_bounds__(1024, 1) void trotter_resident_kernel(
-    int nw, int n, int o, int nspin, int nct, int ldv, int n_terms, double sqrt_dt,
-    const float* __restrict__ E, const float* __restrict__ vhs, long long vplane,
-    const double2* __restrict__ win_up, const double2* __restrict__ win_dn,
-    double2* __restrict__ wout_up, double2* __restrict__ wout_dn, int group_map) {
+__device__ __forceinline__ void trotter_resident_body(
+    int n, int o, int ldv, int n_terms, double sqrt_dt, const float* __restrict__ E,
+    const float* __restrict__ Vr, const float* __restrict__ Vi, const double2* __restrict__ Win,
+    double2* __restrict__ Wout, int c0, int NPL, unsigned char* lds_raw) {
     constexpr int NC = 1;
     using MF = afq::mfma16<float>;
     using ACC = MF::acc;
-    extern __shared__ __attribute__((aligned(16))) unsigned char lds_raw[];
     constexpr int TC = NC * 16;
     const int NP = afq::round_up(n, 16);
-    const int ntile = NP / 16;
     const int nkb = NP / 16;
     const int nkl = nkb > KRB ? nkb - KRB : 0;
     float* Tr = reinterpret_cast<float*>(lds_raw);
-    float* Ti = Tr + NP * TC;
-    float* VL = Ti + NP * TC;  // [wave][nkl][4 kk][2 planes][64 lanes]
-
-    // block -> (walker, spin, column tile); the G = nspin * nct blocks of a
-    // walker land on one XCD (blocks b, b+8, ... share an XCD) back to back
-    const int G = nspin * nct;
-    const int b = blockIdx.x;
-    int w, sub;
-    if (group_map) {
-        w = (b / (8 * G)) * 8 + (b % 8);
-        sub = (b / 8) % G;
-    } else {
-        w = b / G;
-        sub = b % G;
-    }
-    const int s = sub / nct;
-    const int c0 = (sub - s * nct) * 16;  // first walker column of this tile
-    const long long woff = (long long)w * n * o;
-    const double2* Win = (s == 0 ? win_up : win_dn) + woff;
-    double2* Wout = (s == 0 ? wout_up : wout_dn) + woff;
-    const float* Vr = vhs + (long long)w * n * ldv;
-    const float* Vi = Vr + vplane;
+    float* Ti = Tr + NPL * TC;
+    float* VL = Ti + NPL * TC;  // [wave][nkl][4 kk][2 planes][64 lanes]
 
     const int tid = threadIdx.x;
-    const int nthreads = ntile * 64;
+    const int nthreads = blockDim.x;
     const int lane = tid & 63;
     const int wave = tid >> 6;
     const int lr = lane & 15;
@@ -435,6 +416,57 @@ __global__ __launch_bounds__(1024, 1) void trotter_resident_kernel(
     }
 }
 
+template <int KRB>
+__global__ __launch_bounds__(1024, 1) void trotter_resident_kernel(
+    int nw, int n, int o, int nspin, int nct, int ldv, int n_terms, double sqrt_dt,
+    const float* __restrict__ E, const float* __restrict__ vhs, long long vplane,
+    const double2* __restrict__ win_up, const double2* __restrict__ win_dn,
+    double2* __restrict__ wout_up, double2* __restrict__ wout_dn, int group_map) {
+    extern __shared__ __attribute__((aligned(16))) unsigned char lds_raw[];
+    // block -> (walker, spin, column tile); the G = nspin * nct blocks of a
+    // walker land on one XCD (blocks b, b+8, ... share an XCD) back to back
+    const int G = nspin * nct;
+    const int b = blockIdx.x;
+    int w, sub;
+    if (group_map) {
+        w = (b / (8 * G)) * 8 + (b % 8);
+        sub = (b / 8) % G;
+    } else {
+        w = b / G;
+        sub = b % G;
+    }
+    const int s = sub / nct;
+    const int c0 = (sub - s * nct) * 16;  // first walker column of this tile
+    const long long woff = (long long)w * n * o;
+    const float* Vr = vhs + (long long)w * n * ldv;
+    trotter_resident_body<KRB>(n, o, ldv, n_terms, sqrt_dt, E, Vr, Vr + vplane,
+                               (s == 0 ? win_up : win_dn) + woff, (s == 0 ? wout_up : wout_dn) + woff,
+                               c0, afq::round_up(n, 16), lds_raw);
+}
+
+// Every system's walkers in one launch: workgroup -> (system, walker, column
+// tile) by the systems' tblock0 prefix sums, after an XCD-contiguous remap so
+// that the column tiles of one walker share an XCD (and its V in L2).  The
+// workgroup has 64 * round16(max_n) / 16 threads and LDS for max_n rows.
+template <int KRB>
+__global__ __launch_bounds__(1024, 1) void trotter_resident_grouped_kernel(
+    int count, int total, int NPL, int n_terms, const afq_system_step* __restrict__ S) {
+    extern __shared__ __attribute__((aligned(16))) unsigned char lds_raw[];
+    const int b = afq::xcd_contiguous(blockIdx.x, total);
+    const int si = afq::find_group(count, b, [&](int i) { return S[i].tblock0; });
+    const afq_system_step& d = S[si];
+    const int n = d.n, o = d.o, ldv = d.ldv;
+    const int nct = afq::ceil_div(o, 16);
+    const int lb = b - d.tblock0;
+    const int w = lb / nct;
+    const int c0 = (lb - w * nct) * 16;
+    const long long woff = (long long)w * n * o;
+    const float* Vr = d.vhs + (long long)w * n * ldv;
+    trotter_resident_body<KRB>(n, o, ldv, n_terms, d.sqrt_dt, d.exp_h1_half, Vr,
+                               Vr + (long long)d.nw * n * ldv, (const double2*)d.walkers_in + woff,
+                               (double2*)d.walkers_out + woff, c0, NPL, lds_raw);
+}
+
 constexpr int RESIDENT_KRB = 9;
 
 size_t resident_lds(int n) {
@@ -523,4 +555,21 @@ extern "C" int afq_trotter_f32(int nw, int n, int o, int nspin, int ldv, int n_t
     return trotter_entry<float>("afq_trotter_f32", nw, n, o, nspin, ldv, n_terms, sqrt_dt,
                                 exp_h1_half, vhs, win_up, win_dn, wout_up, wout_dn,
                                 (hipStream_t)stream);
+}
+
+extern "C" int afq_trotter_f32_grouped(int count, const afq_system_step* sys, int total_blocks,
+                                       int max_n, int n_terms, void* stream) {
+    AFQ_REQUIRE(count > 0 && total_blocks > 0, "afq_trotter_f32_grouped: count=%d total_blocks=%d",
+                count, total_blocks);
+    AFQ_REQUIRE(sys, "afq_trotter_f32_grouped: null pointer");
+    AFQ_REQUIRE(n_terms >= 0, "afq_trotter_f32_grouped: n_terms < 0");
+    AFQ_REQUIRE(max_n > 0 && max_n <= 256 && resident_lds(max_n) <= 160 * 1024,
+                "afq_trotter_f32_grouped: max_n=%d needs more than 160 KB of LDS", max_n);
+    const int NP = afq::round_up(max_n, 16);
+    const size_t lds = resident_lds(max_n);
+    if (lds > 64 * 1024) afq::allow_full_lds((const void*)trotter_resident_grouped_kernel<RESIDENT_KRB>);
+    hipLaunchKernelGGL((trotter_resident_grouped_kernel<RESIDENT_KRB>), dim3((unsigned)total_blocks),
+                       dim3(64 * (NP / 16)), lds, (hipStream_t)stream, count, total_blocks, NP, n_terms,
+                       sys);
+    return afq::check_launch("afq_trotter_f32_grouped");
 }

@@ -29,13 +29,12 @@ __device__ __forceinline__ void block_sum(double (&v)[4], double* sh) {
     }
 }
 
-// One workgroup per walker.
-__global__ __launch_bounds__(256) void field_shift_kernel(
-    int nw, int naux, double sqrt_dt, const double* __restrict__ x, const double* __restrict__ fb,
+// One walker: row w of the (nw, naux) arrays (planes nw * naux apart).
+__device__ __forceinline__ void field_shift_body(
+    int nw, int naux, int w, double sqrt_dt, const double* __restrict__ x, const double* __restrict__ fb,
     const double2* __restrict__ mf, double* __restrict__ xbar, double2* __restrict__ shift_term,
     double2* __restrict__ fb_term) {
     __shared__ double sh[16];
-    const int w = blockIdx.x;
     const long long row = (long long)w * naux;
     const long long plane = (long long)nw * naux;
     double acc[4] = {0.0, 0.0, 0.0, 0.0};
@@ -66,7 +65,25 @@ __global__ __launch_bounds__(256) void field_shift_kernel(
     }
 }
 
-__global__ __launch_bounds__(1024) void phaseless_kernel(
+// One workgroup per walker.
+__global__ __launch_bounds__(256) void field_shift_kernel(
+    int nw, int naux, double sqrt_dt, const double* __restrict__ x, const double* __restrict__ fb,
+    const double2* __restrict__ mf, double* __restrict__ xbar, double2* __restrict__ shift_term,
+    double2* __restrict__ fb_term) {
+    field_shift_body(nw, naux, blockIdx.x, sqrt_dt, x, fb, mf, xbar, shift_term, fb_term);
+}
+
+// One workgroup per walker of every system.
+__global__ __launch_bounds__(256) void field_shift_grouped_kernel(int count,
+                                                                  const afq_system_step* __restrict__ S) {
+    const int b = blockIdx.x;
+    const int si = afq::find_group(count, b, [&](int i) { return S[i].walker0; });
+    const afq_system_step& d = S[si];
+    field_shift_body(d.nw, d.naux, b - d.walker0, d.sqrt_dt, d.fields, d.fb, (const double2*)d.mf,
+                     d.xbar, (double2*)d.shift_term, (double2*)d.fb_term);
+}
+
+__device__ __forceinline__ void phaseless_body(
     int nw, int nspin, const double2* __restrict__ logdet, const double2* __restrict__ lo_old,
     const double2* __restrict__ st, const double2* __restrict__ ft, const double* __restrict__ w_old,
     const double* __restrict__ pop_old, double e_est, double h0r, double h0i, double dt,
@@ -111,6 +128,25 @@ __global__ __launch_bounds__(1024) void phaseless_kernel(
         const double avg = (mean < 1.0e-300) ? 1.0e-300 : mean;  // NaN propagates
         *pop_new = e_est - damping * (log(avg) / dt);
     }
+}
+
+__global__ __launch_bounds__(1024) void phaseless_kernel(
+    int nw, int nspin, const double2* __restrict__ logdet, const double2* __restrict__ lo_old,
+    const double2* __restrict__ st, const double2* __restrict__ ft, const double* __restrict__ w_old,
+    const double* __restrict__ pop_old, double e_est, double h0r, double h0i, double dt,
+    double sdt, double wfloor, double wcap, double damping, double2* __restrict__ lo_new,
+    double* __restrict__ w_new, double* __restrict__ pop_new) {
+    phaseless_body(nw, nspin, logdet, lo_old, st, ft, w_old, pop_old, e_est, h0r, h0i, dt, sdt,
+                   wfloor, wcap, damping, lo_new, w_new, pop_new);
+}
+
+// One workgroup per system (restricted walkers: log ovlp = 2 log det).
+__global__ __launch_bounds__(1024) void phaseless_grouped_kernel(const afq_system_step* __restrict__ S) {
+    const afq_system_step& d = S[blockIdx.x];
+    phaseless_body(d.nw, 1, (const double2*)d.logdet, (const double2*)d.log_overlaps_in,
+                   (const double2*)d.shift_term, (const double2*)d.fb_term, d.weights_in, d.pop_in,
+                   d.e_estimate, d.h0_prop, 0.0, d.dt, d.sqrt_dt, d.weight_floor, d.weight_cap,
+                   d.damping, (double2*)d.log_overlaps_out, d.weights_out, d.pop_out);
 }
 
 __global__ __launch_bounds__(1024) void block_energy_kernel(
@@ -176,4 +212,22 @@ extern "C" int afq_block_energy(int nw, const double* energy, const double* weig
     hipLaunchKernelGGL(block_energy_kernel, dim3(1), dim3(1024), 0, (hipStream_t)stream, nw,
                        (const double2*)energy, weights, e_ref, thresh, shift_ema, pop_shift, out);
     return afq::check_launch("afq_block_energy");
+}
+
+extern "C" int afq_field_shifts_grouped(int count, const afq_system_step* sys, int total_walkers,
+                                        void* stream) {
+    AFQ_REQUIRE(count > 0 && total_walkers > 0, "afq_field_shifts_grouped: count=%d total_walkers=%d",
+                count, total_walkers);
+    AFQ_REQUIRE(sys, "afq_field_shifts_grouped: null pointer");
+    hipLaunchKernelGGL(field_shift_grouped_kernel, dim3((unsigned)total_walkers), dim3(256), 0,
+                       (hipStream_t)stream, count, sys);
+    return afq::check_launch("afq_field_shifts_grouped");
+}
+
+extern "C" int afq_phaseless_update_grouped(int count, const afq_system_step* sys, void* stream) {
+    AFQ_REQUIRE(count > 0, "afq_phaseless_update_grouped: count=%d", count);
+    AFQ_REQUIRE(sys, "afq_phaseless_update_grouped: null pointer");
+    hipLaunchKernelGGL(phaseless_grouped_kernel, dim3((unsigned)count), dim3(1024), 0,
+                       (hipStream_t)stream, sys);
+    return afq::check_launch("afq_phaseless_update_grouped");
 }

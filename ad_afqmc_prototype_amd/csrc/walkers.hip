@@ -16,15 +16,11 @@ constexpr int THREADS = 256;
 // equal to it to rounding for numerically full-rank walkers.  One workgroup per
 // walker; the walker (n x o complex) sits in LDS with an odd row pitch.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(THREADS) void qr_kernel(int n, int o, const double2* __restrict__ win,
-                                                      double2* __restrict__ wout) {
-    extern __shared__ double2 q[];
+__device__ __forceinline__ void qr_body(int n, int o, const double2* __restrict__ W,
+                                        double2* __restrict__ Q, double2* q) {
     __shared__ double2 r[64];
     __shared__ double scale_s;
     const int pitch = o + 1;
-    const int w = blockIdx.x;
-    const double2* W = win + (long long)w * n * o;
-    double2* Q = wout + (long long)w * n * o;
     const int tid = threadIdx.x;
     const int lane = tid & 63;
     const int wave = tid >> 6;
@@ -88,6 +84,24 @@ __global__ __launch_bounds__(THREADS) void qr_kernel(int n, int o, const double2
         const int j = e - p * o;
         Q[e] = q[p * pitch + j];
     }
+}
+
+__global__ __launch_bounds__(THREADS) void qr_kernel(int n, int o, const double2* __restrict__ win,
+                                                      double2* __restrict__ wout) {
+    extern __shared__ double2 q_lds[];
+    const long long off = (long long)blockIdx.x * n * o;
+    qr_body(n, o, win + off, wout + off, q_lds);
+}
+
+// One workgroup per walker of every system: walkers_in -> walkers_out.
+__global__ __launch_bounds__(THREADS) void qr_grouped_kernel(int count,
+                                                             const afq_system_step* __restrict__ S) {
+    extern __shared__ double2 q_lds[];
+    const int b = blockIdx.x;
+    const int si = afq::find_group(count, b, [&](int i) { return S[i].walker0; });
+    const afq_system_step& d = S[si];
+    const long long off = (long long)(b - d.walker0) * d.n * d.o;
+    qr_body(d.n, d.o, (const double2*)d.walkers_in + off, (double2*)d.walkers_out + off, q_lds);
 }
 
 // ---------------------------------------------------------------------------
@@ -192,4 +206,21 @@ extern "C" int afq_gather_rows(int nrows, long long row_len, int nplanes, long l
     hipLaunchKernelGGL(gather_kernel, dim3((unsigned)blocks), dim3(256), 0, (hipStream_t)stream,
                        nrows, row_len, nplanes, src_plane_stride, dst_plane_stride, idx, src, dst);
     return afq::check_launch("afq_gather_rows");
+}
+
+extern "C" int afq_orthonormalize_grouped(int count, const afq_system_step* sys, int total_walkers,
+                                          int max_n, int max_o, void* stream) {
+    AFQ_REQUIRE(count > 0 && total_walkers > 0, "afq_orthonormalize_grouped: count=%d total_walkers=%d",
+                count, total_walkers);
+    AFQ_REQUIRE(sys, "afq_orthonormalize_grouped: null pointer");
+    AFQ_REQUIRE(max_o > 0 && max_o <= 64 && max_n >= max_o, "afq_orthonormalize_grouped: max_n=%d max_o=%d",
+                max_n, max_o);
+    const size_t lds = (size_t)max_n * (max_o + 1) * sizeof(double2);
+    AFQ_REQUIRE(lds + 64 * sizeof(double2) + 64 <= 160 * 1024,
+                "afq_orthonormalize_grouped: max_n=%d max_o=%d needs more than 160 KB of LDS",
+                max_n, max_o);
+    if (lds > 64 * 1024) afq::allow_full_lds((const void*)qr_grouped_kernel);
+    hipLaunchKernelGGL(qr_grouped_kernel, dim3((unsigned)total_walkers), dim3(THREADS), lds,
+                       (hipStream_t)stream, count, sys);
+    return afq::check_launch("afq_orthonormalize_grouped");
 }

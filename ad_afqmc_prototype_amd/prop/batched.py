@@ -1,0 +1,453 @@
+"""Many independent systems advancing together: uneven orbital, occupied,
+Cholesky and walker counts, each with its own Hamiltonian, RHF trial,
+restricted walkers and random stream.
+
+Every system is an ordinary run of the reference's step (prop/afqmc.py:74-147)
+and block (prop/blocks.py:22-95); the batch changes how the work is launched,
+not what is computed.  Each stage of the step is one grouped HIP launch over
+all systems (random fields, force-bias GEMM, field shifts, VHS GEMM, Trotter,
+Green's function / log-overlap, phaseless update); the fp64 -> fp32 operand
+conversions run once over packed buffers.  A system's walkers, weights and
+block energies equal those of running it alone with afqmc_step in fp32
+precision (the same kernel bodies, the same arithmetic).
+
+Layout: every per-system array lives at an aligned offset of one packed
+buffer per quantity; two descriptor tables (device memory, written once) name
+them, the second with input and output buffers swapped, so consecutive steps
+alternate tables and nothing is rewritten between steps.  A block (steps, QR,
+measurement, comb) can be captured once as a HIP graph and replayed.
+
+GEMM tiles: the grouped GEMM has four tile shapes (128x128, 64x128, 128x64,
+64x64 rows x columns of C per workgroup); tune() times each on the batch's own
+problem set with HIP events and keeps the fastest for the force-bias and the
+VHS contraction separately (optionally cached in a JSON file keyed by the
+problem shapes).
+"""
+from __future__ import annotations
+
+import ctypes
+import json
+import os
+from dataclasses import dataclass
+from typing import Any, Sequence
+
+import torch
+
+from .. import _lib
+from .. import kernels
+from ..core.system import system
+from ..ham.chol import ham_chol
+from ..meas.rhf import make_rhf_meas_ops
+from ..trial.rhf import make_rhf_trial_ops, rhf_trial
+from .afqmc import init_prop_state
+from .chol_afqmc_ops import chol_transposed, make_chol_afqmc_ops
+
+c128 = torch.complex128
+f64 = torch.float64
+f32 = torch.float32
+
+_vp = ctypes.c_void_p
+
+
+class _system_step(ctypes.Structure):
+    """Host mirror of afq_system_step (include/afqmc_hip.h)."""
+
+    _fields_ = [("n", ctypes.c_int), ("o", ctypes.c_int), ("naux", ctypes.c_int), ("nw", ctypes.c_int),
+                ("ldv", ctypes.c_int), ("walker0", ctypes.c_int), ("tblock0", ctypes.c_int),
+                ("reserved0", ctypes.c_int),
+                ("sqrt_dt", ctypes.c_double), ("dt", ctypes.c_double), ("e_estimate", ctypes.c_double),
+                ("h0_prop", ctypes.c_double), ("weight_floor", ctypes.c_double),
+                ("weight_cap", ctypes.c_double), ("damping", ctypes.c_double),
+                ("reserved1", ctypes.c_double)] + [(name, _vp) for name in (
+                    "trial", "walkers_in", "walkers_out", "greens", "logdet", "fields", "fb", "mf",
+                    "xbar", "shift_term", "fb_term", "exp_h1_half", "vhs", "log_overlaps_in",
+                    "log_overlaps_out", "weights_in", "weights_out", "pop_in", "pop_out", "key_in",
+                    "key_out")]
+
+
+class _gemm_problem(ctypes.Structure):
+    """Host mirror of afq_gemm_problem (include/afqmc_hip.h)."""
+
+    _fields_ = [("M", ctypes.c_int), ("N", ctypes.c_int), ("K", ctypes.c_int), ("tile_start", ctypes.c_int),
+                ("lda", ctypes.c_longlong), ("ldb", ctypes.c_longlong), ("ldc", ctypes.c_longlong),
+                ("A", _vp), ("B", _vp), ("C", _vp), ("alpha", ctypes.c_float), ("c_fp64", ctypes.c_int)]
+
+
+assert ctypes.sizeof(_system_step) == 8 * 4 + 8 * 8 + 21 * 8
+assert ctypes.sizeof(_gemm_problem) == 72
+
+GEMM_SHAPES = (0, 1, 2, 3)
+
+
+def gemm_tile_shape(shape: int) -> tuple[int, int]:
+    bm, bn = ctypes.c_int(), ctypes.c_int()
+    _lib.call("afq_gemm_tile_shape", shape, ctypes.byref(bm), ctypes.byref(bn))
+    return bm.value, bn.value
+
+
+@dataclass(frozen=True)
+class batch_system:
+    """One member of the batch: Hamiltonian (restricted basis, on the device),
+    RHF trial, walker count and seed of its random stream."""
+
+    ham: ham_chol
+    trial: rhf_trial
+    n_walkers: int
+    seed: int
+
+
+class _packer:
+    """Aligned sub-ranges of one flat buffer per quantity."""
+
+    ALIGN = 64  # elements
+
+    def __init__(self, sizes: Sequence[int], dtype, device):
+        offs, o = [], 0
+        for s in sizes:
+            offs.append(o)
+            o += (int(s) + self.ALIGN - 1) // self.ALIGN * self.ALIGN
+        self.buf = torch.zeros(max(o, 1), dtype=dtype, device=device)
+        self.offs = offs
+        self.sizes = [int(s) for s in sizes]
+
+    def part(self, i: int, shape=None):
+        v = self.buf[self.offs[i]:self.offs[i] + self.sizes[i]]
+        return v if shape is None else v.view(*shape)
+
+    def ptr(self, i: int) -> int:
+        return self.part(i).data_ptr()
+
+
+def _upload(structs) -> torch.Tensor:
+    raw = ctypes.string_at(ctypes.addressof(structs), ctypes.sizeof(structs))
+    return torch.frombuffer(bytearray(raw), dtype=torch.uint8).clone()
+
+
+class system_batch:
+    """The state of every system plus the grouped step.
+
+    Walkers, weights, log-overlaps, population shifts and keys are kept in two
+    buffer sets (A, B); steps alternate A -> B and B -> A, so after an even
+    number of steps the state is in A again.  QR (every ``qr_every`` steps),
+    measurement and comb (every ``sr_every`` steps, prop/blocks.py:53-92) run
+    on the A set; both cadences must be even."""
+
+    def __init__(self, systems: Sequence[batch_system], *, dt: float, qr_every: int = 10,
+                 sr_every: int = 20, n_exp_terms: int = 6, weight_floor: float = 1.0e-3,
+                 weight_cap: float = 100.0, damping: float = 0.1, shift_ema: float = 0.1,
+                 gemm_shapes: tuple[int, int] | None = None):
+        if not systems:
+            raise ValueError("empty batch")
+        if qr_every % 2 or sr_every % 2 or qr_every <= 0 or sr_every <= 0:
+            raise ValueError("qr_every and sr_every must be positive and even")
+        dev = systems[0].ham.h1.device
+        if dev.type != "cuda":
+            raise ValueError("the batch runs on a HIP device only")
+        self.device = dev
+        self.dt, self.sqrt_dt = float(dt), float(dt) ** 0.5
+        self.qr_every, self.sr_every = qr_every, sr_every
+        self.n_exp_terms = n_exp_terms
+        self.shift_ema = shift_ema
+        self.floor, self.cap, self.damping = weight_floor, weight_cap, damping
+        self.count = len(systems)
+        self.members = []
+        for s in systems:
+            n, o, g = s.ham.h1.shape[0], s.trial.nocc, s.ham.chol.shape[0]
+            if s.ham.basis != "restricted" or not s.trial.restricted:
+                raise ValueError("batched systems need a restricted basis and an RHF trial")
+            if o > 32 or n > 208 or o > n:
+                raise ValueError(f"batched system n={n} o={o} outside n <= 208, o <= 32")
+            if s.n_walkers * g >= 2 ** 31:
+                raise ValueError("n_walkers * naux must stay below 2^31")
+            self.members.append(self._member(s, n, o, g))
+        self.max_n = max(m["n"] for m in self.members)
+        self.max_o = max(m["o"] for m in self.members)
+        self.total_walkers = sum(m["nw"] for m in self.members)
+        self._allocate()
+        self._tables()
+        self.gemm_shapes = tuple(gemm_shapes) if gemm_shapes is not None else (0, 0)
+        self.steps_done = 0
+        self.block_stats = torch.zeros((self.count, 2), dtype=f64, device=dev)
+
+    # ------------------------------------------------------------------ setup
+    def _member(self, s: batch_system, n: int, o: int, g: int) -> dict:
+        sys_ = system(n, (o, o), "restricted")
+        t_ops = make_rhf_trial_ops(sys_)
+        m_ops = make_rhf_meas_ops(sys_, precision="fp32")
+        p_ops = make_chol_afqmc_ops(s.ham, "restricted", precision="fp32")
+        p_ctx = p_ops.build_prop_ctx(t_ops.get_rdm1(s.trial), self.dt)
+        m_ctx = m_ops.build_meas_ctx(s.ham, s.trial)
+        st = init_prop_state(sys=sys_, n_walkers=s.n_walkers, seed=s.seed, ham_data=s.ham,
+                             trial_ops_=t_ops, trial_data=s.trial, meas_ops=m_ops)
+        rot_pad, rot_h1 = m_ctx.energy_operands(1)
+        return dict(n=n, o=o, g=g, nw=s.n_walkers, ldv=kernels.ldv_for(n), h0=float(s.ham.h0),
+                    trial=s.trial.mo_up.contiguous(), chol_t32=kernels.convert_f32(chol_transposed(s.ham.chol)),
+                    rot32=m_ctx.fb_operand(1), rot_pad=rot_pad, rot_h1=rot_h1,
+                    e_half=p_ctx.exp_h1_half_pad32.contiguous(), mf=p_ctx.mf_shifts.contiguous(),
+                    h0_prop=float(complex(p_ctx.h0_prop).real), e_est=float(st.e_estimate), state0=st)
+
+    def _allocate(self):
+        dev, ms = self.device, self.members
+        P = lambda sizes, dt: _packer(sizes, dt, dev)  # noqa: E731
+        self.walkers = [P([m["nw"] * m["n"] * m["o"] for m in ms], c128) for _ in range(2)]
+        self.greens = P([2 * m["nw"] * m["o"] * m["n"] for m in ms], f64)
+        self.greens32 = P(self.greens.sizes, f32)
+        self.logdet = P([m["nw"] for m in ms], c128)
+        self.fields = P([m["nw"] * m["g"] for m in ms], f64)
+        self.fb = P([2 * m["nw"] * m["g"] for m in ms], f64)
+        self.xbar = P([2 * m["nw"] * m["g"] for m in ms], f64)
+        self.xbar32 = P(self.xbar.sizes, f32)
+        self.shift = P([m["nw"] for m in ms], c128)
+        self.fbt = P([m["nw"] for m in ms], c128)
+        self.vhs = P([2 * m["nw"] * m["n"] * m["ldv"] for m in ms], f32)
+        self.lo = [P([m["nw"] for m in ms], c128) for _ in range(2)]
+        self.weights = [P([m["nw"] for m in ms], f64) for _ in range(2)]
+        self.pop = [torch.zeros((self.count,), dtype=f64, device=dev) for _ in range(2)]
+        self.keys = [torch.zeros((self.count, 2), dtype=torch.int32, device=dev) for _ in range(2)]
+        self.e_half = P([m["n"] * m["ldv"] for m in ms], f32)
+        self.mf = P([m["g"] for m in ms], c128)
+        self.trials = P([m["n"] * m["o"] for m in ms], f64)
+        for i, m in enumerate(ms):
+            st = m.pop("state0")
+            self.walkers[0].part(i).copy_(st.walkers.reshape(-1))
+            self.greens.part(i).copy_(st.greens.greens.reshape(-1))
+            self.lo[0].part(i).copy_(st.log_overlaps)
+            self.weights[0].part(i).copy_(st.weights)
+            self.pop[0][i] = st.pop_control_ene_shift
+            self.keys[0][i] = st.rng_key
+            self.e_half.part(i).copy_(m["e_half"].reshape(-1))
+            self.mf.part(i).copy_(m["mf"])
+            self.trials.part(i).copy_(m["trial"].reshape(-1))
+
+    def _desc(self, parity: int):
+        a, b = parity, 1 - parity
+        arr = (_system_step * self.count)()
+        w0 = t0 = 0
+        for i, m in enumerate(self.members):
+            d = arr[i]
+            d.n, d.o, d.naux, d.nw, d.ldv = m["n"], m["o"], m["g"], m["nw"], m["ldv"]
+            d.walker0, d.tblock0 = w0, t0
+            w0 += m["nw"]
+            t0 += m["nw"] * ((m["o"] + 15) // 16)
+            d.sqrt_dt, d.dt, d.e_estimate, d.h0_prop = self.sqrt_dt, self.dt, m["e_est"], m["h0_prop"]
+            d.weight_floor, d.weight_cap, d.damping = self.floor, self.cap, self.damping
+            d.trial = self.trials.ptr(i)
+            d.walkers_in, d.walkers_out = self.walkers[a].ptr(i), self.walkers[b].ptr(i)
+            d.greens, d.logdet = self.greens.ptr(i), self.logdet.ptr(i)
+            d.fields, d.fb, d.mf, d.xbar = self.fields.ptr(i), self.fb.ptr(i), self.mf.ptr(i), self.xbar.ptr(i)
+            d.shift_term, d.fb_term = self.shift.ptr(i), self.fbt.ptr(i)
+            d.exp_h1_half, d.vhs = self.e_half.ptr(i), self.vhs.ptr(i)
+            d.log_overlaps_in, d.log_overlaps_out = self.lo[a].ptr(i), self.lo[b].ptr(i)
+            d.weights_in, d.weights_out = self.weights[a].ptr(i), self.weights[b].ptr(i)
+            d.pop_in, d.pop_out = self.pop[a][i:i + 1].data_ptr(), self.pop[b][i:i + 1].data_ptr()
+            d.key_in, d.key_out = self.keys[a][i].data_ptr(), self.keys[b][i].data_ptr()
+        self.total_tblocks = t0
+        return arr
+
+    def _problems(self, which: str, shape: int):
+        bm, bn = gemm_tile_shape(shape)
+        arr = (_gemm_problem * self.count)()
+        start = 0
+        for i, m in enumerate(self.members):
+            p = arr[i]
+            if which == "fb":
+                k = m["o"] * m["n"]
+                p.M, p.N, p.K = 2 * m["nw"], m["g"], k
+                p.lda, p.ldb, p.ldc = k, m["rot32"].shape[1], m["g"]
+                p.A, p.B, p.C = self.greens32.ptr(i), m["rot32"].data_ptr(), self.fb.ptr(i)
+                p.alpha, p.c_fp64 = 2.0, 1
+            else:
+                nc = m["n"] * m["ldv"]
+                p.M, p.N, p.K = 2 * m["nw"], nc, m["g"]
+                p.lda, p.ldb, p.ldc = m["g"], m["g"], nc
+                p.A, p.B, p.C = self.xbar32.ptr(i), m["chol_t32"].data_ptr(), self.vhs.ptr(i)
+                p.alpha, p.c_fp64 = 1.0, 0
+            p.tile_start = start
+            start += -(-p.M // bm) * -(-p.N // bn)
+        return arr, start
+
+    def _tables(self):
+        self.desc_host = [self._desc(0), self._desc(1)]
+        self.desc = [_upload(d).to(self.device) for d in self.desc_host]
+        self.gemm_tables = {}
+        for which in ("fb", "vhs"):
+            for shape in GEMM_SHAPES:
+                arr, tiles = self._problems(which, shape)
+                self.gemm_tables[(which, shape)] = (_upload(arr).to(self.device), tiles)
+        if self.greens.buf.numel() >= 2 ** 31 or self.xbar.buf.numel() >= 2 ** 31:
+            raise ValueError("batch too large for one conversion launch")
+
+    # ------------------------------------------------------------------ stages
+    def _grouped_gemm(self, which: str):
+        shape = self.gemm_shapes[0 if which == "fb" else 1]
+        tab, tiles = self.gemm_tables[(which, shape)]
+        kernels._call("afq_sgemm_nt_grouped", tab, self.count, tab.data_ptr(), tiles, shape,
+                      kernels._stream(tab), tag=which)
+
+    def _convert(self, src: _packer, dst: _packer):
+        n = src.buf.numel()
+        kernels._call("afq_convert_f32", src.buf, 1, n, src.buf.data_ptr(), n, dst.buf.data_ptr(), n,
+                      kernels._stream(src.buf))
+
+    def step(self):
+        """One step of every system (fields drawn from each system's key)."""
+        p = self.steps_done % 2
+        d = self.desc[p]
+        st = kernels._stream(d)
+        kernels._call("afq_rng_fields_grouped", d, self.count, d.data_ptr(), self.total_walkers, st)
+        self._convert(self.greens, self.greens32)
+        self._grouped_gemm("fb")
+        kernels._call("afq_field_shifts_grouped", d, self.count, d.data_ptr(), self.total_walkers, st)
+        self._convert(self.xbar, self.xbar32)
+        self._grouped_gemm("vhs")
+        kernels._call("afq_trotter_f32_grouped", d, self.count, d.data_ptr(), self.total_tblocks,
+                      self.max_n, self.n_exp_terms, st)
+        kernels._call("afq_overlap_greens_grouped", d, self.count, d.data_ptr(), self.total_walkers,
+                      self.max_o, st)
+        kernels._call("afq_phaseless_update_grouped", d, self.count, d.data_ptr(), st)
+        self.steps_done += 1
+
+    def _orthonormalize(self):
+        """QR of every system (walkers.py:169-211) and the overlap refresh
+        (prop/blocks.py:53-57); the state stays in set A."""
+        if self.steps_done % 2:
+            raise RuntimeError("QR needs the state in buffer set A (an even step count)")
+        d = self.desc[0]
+        st = kernels._stream(d)
+        kernels._call("afq_orthonormalize_grouped", d, self.count, d.data_ptr(), self.total_walkers,
+                      self.max_n, self.max_o, st)
+        kernels._call("afq_overlap_greens_grouped", d, self.count, d.data_ptr(), self.total_walkers,
+                      self.max_o, st)
+        self.walkers[0].buf.copy_(self.walkers[1].buf)
+        torch.mul(self.logdet.buf, 2.0, out=self.lo[0].buf)
+
+    def _measure_reconfigure(self):
+        """Per system: clipped local energies, block energy and shift EMA
+        (prop/blocks.py:59-79), comb with a uniform from the system's key
+        (prop/blocks.py:81-92).  Called right after _orthonormalize."""
+        thresh = (2.0 / self.dt) ** 0.5
+        for i, m in enumerate(self.members):
+            nw, n, o = m["nw"], m["n"], m["o"]
+            g = self.greens.part(i, (2, nw, o * n))
+            w = self.weights[0].part(i)
+            pop = self.pop[0][i]
+            e = kernels.energy(g, nw, n, o, 1, m["rot_pad"], m["rot_h1"], m["h0"])
+            out = kernels.block_energy(e, w, pop, e_ref=m["e_est"], thresh=thresh,
+                                       shift_ema=self.shift_ema)
+            self.block_stats[i].copy_(out[:2])
+            pop.copy_(out[2])
+            key_next, sub = kernels.rng_split(self.keys[0][i])
+            self.keys[0][i].copy_(key_next)
+            idx, w_new = kernels.comb_indices(w, kernels.rng_uniform(sub))
+            w.copy_(w_new)
+            kernels.gather_rows(self.walkers[1].part(i, (nw, n, o)), idx,
+                                out=self.walkers[0].part(i, (nw, n, o)))
+            g.copy_(kernels.gather_rows(g, idx, nplanes=2))
+            torch.mul(kernels.gather_rows(self.logdet.part(i), idx), 2.0, out=self.lo[0].part(i))
+
+    def advance(self, n_steps: int):
+        """n_steps steps of every system with the QR / measurement cadence
+        (counted from the batch's first step)."""
+        for _ in range(n_steps):
+            self.step()
+            t = self.steps_done
+            if t % self.sr_every == 0:
+                self._orthonormalize()
+                self._measure_reconfigure()
+            elif t % self.qr_every == 0:
+                self._orthonormalize()
+
+    # --------------------------------------------------------------- queries
+    def system_state(self, i: int) -> dict:
+        """Copies of system i's walkers, weights, log-overlaps, pop shift and key
+        (state in set A: call after an even number of steps)."""
+        m = self.members[i]
+        a = self.steps_done % 2
+        return dict(walkers=self.walkers[a].part(i, (m["nw"], m["n"], m["o"])).clone(),
+                    weights=self.weights[a].part(i).clone(), log_overlaps=self.lo[a].part(i).clone(),
+                    pop=self.pop[a][i].clone(), key=self.keys[a][i].clone())
+
+    def flops_per_step(self) -> float:
+        """Algorithmic flops of one step of every system (real flops; complex x
+        real MAC = 4, complex x complex MAC = 8; force bias, VHS, Taylor, the two
+        one-body halves, C^T w, LU and the Green's solve)."""
+        tot = 0.0
+        for m in self.members:
+            n, o, g, nw = m["n"], m["o"], m["g"], m["nw"]
+            tot += nw * (4 * g * n * n + 4 * g * o * n + self.n_exp_terms * 8 * n * n * o
+                         + 2 * 4 * n * n * o + 4 * o * o * n + 8 / 3 * o ** 3 + 8 * o * o * n)
+        return tot
+
+    # ------------------------------------------------------------- autotuner
+    def signature(self) -> str:
+        return ";".join(f"{m['n']},{m['o']},{m['g']},{m['nw']}" for m in self.members)
+
+    def tune(self, reps: int = 5, cache_path: str | None = None) -> dict:
+        """Pick the grouped-GEMM tile shape of the force-bias and the VHS
+        contraction by timing each shape on this batch (HIP events, ``reps``
+        launches after one warm-up).  With ``cache_path`` the choice is read
+        from / written to a JSON file keyed by the batch's problem shapes."""
+        key = self.signature()
+        cache = {}
+        if cache_path and os.path.exists(cache_path):
+            with open(cache_path) as f:
+                cache = json.load(f)
+            if key in cache:
+                self.gemm_shapes = tuple(cache[key]["shapes"])
+                return cache[key]
+        stream = torch.cuda.current_stream(self.device)
+        timings = {}
+        best = []
+        for which in ("fb", "vhs"):
+            ms = {}
+            for shape in GEMM_SHAPES:
+                tab, tiles = self.gemm_tables[(which, shape)]
+                _lib.call("afq_sgemm_nt_grouped", self.count, tab.data_ptr(), tiles, shape, stream.cuda_stream)
+                e0 = torch.cuda.Event(enable_timing=True)
+                e1 = torch.cuda.Event(enable_timing=True)
+                e0.record(stream)
+                for _ in range(reps):
+                    _lib.call("afq_sgemm_nt_grouped", self.count, tab.data_ptr(), tiles, shape,
+                              stream.cuda_stream)
+                e1.record(stream)
+                e1.synchronize()
+                ms[shape] = e0.elapsed_time(e1) / reps
+            timings[which] = ms
+            best.append(min(ms, key=ms.get))
+        self.gemm_shapes = tuple(best)
+        entry = {"shapes": list(best), "ms": {w: {str(k): v for k, v in t.items()} for w, t in timings.items()}}
+        if cache_path:
+            cache[key] = entry
+            os.makedirs(os.path.dirname(os.path.abspath(cache_path)), exist_ok=True)
+            with open(cache_path, "w") as f:
+                json.dump(cache, f, indent=1)
+        return entry
+
+
+class captured_batch_block:
+    """sr_every steps of the batch (one block: steps, QR, measurement, comb)
+    recorded once as a HIP graph; replay() advances every system by one block
+    and leaves the block energies and weights in batch.block_stats."""
+
+    def __init__(self, batch: system_batch, warmup_blocks: int = 1):
+        if batch.steps_done % batch.sr_every:
+            raise ValueError("capture at a block boundary")
+        self.batch = batch
+        side = torch.cuda.Stream(device=batch.device)
+        side.wait_stream(torch.cuda.current_stream(batch.device))
+        with torch.cuda.stream(side):
+            for _ in range(warmup_blocks):
+                batch.advance(batch.sr_every)
+        torch.cuda.current_stream(batch.device).wait_stream(side)
+        torch.cuda.synchronize(batch.device)
+        self.graph = torch.cuda.CUDAGraph()
+        start = batch.steps_done
+        with torch.cuda.graph(self.graph):
+            batch.advance(batch.sr_every)
+        # capture enqueued nothing: the counter must describe the state, not the recording
+        batch.steps_done = start
+
+    def replay(self):
+        self.graph.replay()
+        self.batch.steps_done += self.batch.sr_every
+        return self.batch.block_stats

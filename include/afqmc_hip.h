@@ -275,6 +275,117 @@ int afq_gather_rows(int nrows, long long row_len, int nplanes, long long src_pla
                     long long dst_plane_stride, const int* idx, const double* src, double* dst,
                     void* stream);
 
+/*
+ * ---------------------------------------------------------------------------
+ * Grouped launches: many independent systems (uneven n, o, naux and walker
+ * counts, each with its own Hamiltonian, RHF trial and restricted walkers)
+ * advance together, one launch per stage of the step for all of them.  The
+ * reference runs one system per call; these entry points batch that call
+ * (prop/afqmc.py:74-147 per system) and have no single counterpart there.
+ *
+ * The caller keeps a device array of descriptors, one per system, written
+ * once: they hold the system's sizes, scalars and device pointers into the
+ * caller's buffers.  Two tables (input and output buffers swapped) serve
+ * alternate steps, so a sequence of steps needs no descriptor rewrite and can
+ * be captured in a HIP graph.  The systems of one table must not share output
+ * buffers.  `count` systems; `total_walkers` = sum of nw; every system has
+ * n <= 256, o <= 32 and o <= n.
+ * ---------------------------------------------------------------------------
+ */
+typedef struct afq_system_step {
+    int n, o, naux, nw;          /* orbitals, occupied orbitals, Cholesky vectors, walkers */
+    int ldv;                     /* n rounded up to 4: row pitch of exp_h1_half and VHS */
+    int walker0;                 /* first walker of this system in the batch (prefix sum of nw) */
+    int tblock0;                 /* first Trotter workgroup: prefix sum of nw * ceil(o / 16) */
+    int reserved0;
+    double sqrt_dt, dt, e_estimate, h0_prop;
+    double weight_floor, weight_cap, damping, reserved1;
+    const double* trial;         /* (n, o) float64 RHF orbitals */
+    const double* walkers_in;    /* complex128 (nw, n, o) */
+    double* walkers_out;         /* complex128 (nw, n, o) */
+    double* greens;              /* planes (2, nw, o*n): G of walkers_out (Green's stage) */
+    double* logdet;              /* complex128 (nw): log det(C^T walkers_out) */
+    double* fields;              /* (nw, naux) float64 */
+    const double* fb;            /* force bias planes (2, nw, naux) */
+    const double* mf;            /* complex128 (naux) mean-field shifts */
+    double* xbar;                /* shifted-field planes (2, nw, naux) */
+    double* shift_term;          /* complex128 (nw) */
+    double* fb_term;             /* complex128 (nw) */
+    const float* exp_h1_half;    /* (n, ldv) float32 */
+    const float* vhs;            /* planes (2, nw, n, ldv) float32 */
+    const double* log_overlaps_in;  /* complex128 (nw) */
+    double* log_overlaps_out;       /* complex128 (nw) */
+    const double* weights_in;    /* (nw) */
+    double* weights_out;         /* (nw) */
+    const double* pop_in;        /* float64[1] population-control shift */
+    double* pop_out;             /* float64[1] */
+    const unsigned* key_in;      /* uint32[2] Threefry key */
+    unsigned* key_out;           /* uint32[2] */
+} afq_system_step;
+
+/* One problem of a grouped GEMM: C (M x N) = alpha A (M x K) B^T (N x K). */
+typedef struct afq_gemm_problem {
+    int M, N, K;
+    int tile_start;              /* first workgroup of this problem: prefix sum of its tiles */
+    long long lda, ldb, ldc;
+    const float* A;
+    const float* B;
+    void* C;                     /* float32, or float64 when c_fp64 != 0 */
+    float alpha;
+    int c_fp64;
+} afq_gemm_problem;
+
+/*
+ * Tile shape `shape` (0..3) of the grouped fp32 GEMM: *bm x *bn rows x
+ * columns of C per workgroup.  A problem has ceil(M/bm) * ceil(N/bn) tiles.
+ * Host-only query, no device work.
+ */
+int afq_gemm_tile_shape(int shape, int* bm, int* bn);
+
+/*
+ * Every problem's fp32 GEMM in one launch of total_tiles workgroups (the sum
+ * of the problems' tiles for `shape`, tile_start their prefix sums).  The
+ * force-bias and VHS contractions of every system of a grouped step
+ * (meas/rhf.py:33, prop/chol_afqmc_ops.py:72-73 per system).
+ */
+int afq_sgemm_nt_grouped(int count, const afq_gemm_problem* problems, int total_tiles, int shape,
+                         void* stream);
+
+/*
+ * Per system: key_out, sub = split(key_in); fields = normal(sub, (nw, naux))
+ * (prop/afqmc.py:86-88), one launch for all systems; nw * naux < 2^31 each.
+ */
+int afq_rng_fields_grouped(int count, const afq_system_step* sys, int total_walkers, void* stream);
+
+/* afq_field_shifts for every system (prop/afqmc.py:97-101). */
+int afq_field_shifts_grouped(int count, const afq_system_step* sys, int total_walkers, void* stream);
+
+/*
+ * afq_trotter_f32 for every system: walkers_in -> walkers_out with the
+ * system's exp_h1_half and VHS (prop/chol_afqmc_ops.py:173-196,
+ * prop/utils.py:20-31).  total_blocks = sum of nw * ceil(o / 16); max_n the
+ * largest n.
+ */
+int afq_trotter_f32_grouped(int count, const afq_system_step* sys, int total_blocks, int max_n,
+                            int n_terms, void* stream);
+
+/*
+ * afq_overlap_greens for every system, of walkers_out into greens / logdet
+ * (meas/rhf.py:15-24, trial/rhf.py:25-37).  max_o the largest o.
+ */
+int afq_overlap_greens_grouped(int count, const afq_system_step* sys, int total_walkers, int max_o,
+                               void* stream);
+
+/* afq_phaseless_update for every system, one workgroup each (prop/afqmc.py:116-138). */
+int afq_phaseless_update_grouped(int count, const afq_system_step* sys, void* stream);
+
+/*
+ * afq_orthonormalize for every system: walkers_in -> walkers_out
+ * (walkers.py:169-211).  max_n, max_o the largest n and o.
+ */
+int afq_orthonormalize_grouped(int count, const afq_system_step* sys, int total_walkers, int max_n,
+                               int max_o, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

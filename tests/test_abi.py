@@ -64,3 +64,46 @@ def test_invalid_arguments_rejected_on_host(name, args, msg):
     L = _lib()
     with pytest.raises(ValueError, match=msg):
         L.call(name, *args)
+
+
+def test_grouped_descriptor_layout_matches_header():
+    """The host mirrors of afq_system_step / afq_gemm_problem have the C layout
+    of include/afqmc_hip.h (compiled here with gcc)."""
+    import ctypes
+    import os
+    import shutil
+    import subprocess
+    import tempfile
+    from ad_afqmc_prototype_amd.prop import batched as B
+    gcc = shutil.which("gcc")
+    if gcc is None:
+        pytest.skip("gcc not available")
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    src = r'''
+#include <stdio.h>
+#include <stddef.h>
+#include "afqmc_hip.h"
+#define O(T, f) printf(#f " %zu\n", offsetof(T, f))
+int main(void) {
+  printf("size_step %zu\nsize_gemm %zu\n", sizeof(afq_system_step), sizeof(afq_gemm_problem));
+  O(afq_system_step, tblock0); O(afq_system_step, sqrt_dt); O(afq_system_step, damping);
+  O(afq_system_step, trial); O(afq_system_step, vhs); O(afq_system_step, pop_out);
+  O(afq_system_step, key_out); O(afq_gemm_problem, tile_start); O(afq_gemm_problem, ldc);
+  O(afq_gemm_problem, C); O(afq_gemm_problem, alpha); O(afq_gemm_problem, c_fp64);
+  return 0;
+}
+'''
+    with tempfile.TemporaryDirectory() as d:
+        with open(os.path.join(d, "l.c"), "w") as f:
+            f.write(src)
+        exe = os.path.join(d, "l")
+        subprocess.run([gcc, "-I", os.path.join(root, "include"), os.path.join(d, "l.c"), "-o", exe],
+                       check=True)
+        out = dict(line.split() for line in subprocess.run([exe], capture_output=True, text=True,
+                                                           check=True).stdout.splitlines())
+    assert int(out["size_step"]) == ctypes.sizeof(B._system_step)
+    assert int(out["size_gemm"]) == ctypes.sizeof(B._gemm_problem)
+    for f in ("tblock0", "sqrt_dt", "damping", "trial", "vhs", "pop_out", "key_out"):
+        assert int(out[f]) == getattr(B._system_step, f).offset, f
+    for f in ("tile_start", "ldc", "C", "alpha", "c_fp64"):
+        assert int(out[f]) == getattr(B._gemm_problem, f).offset, f

@@ -1,0 +1,169 @@
+"""Grouped launches over many independent systems (prop/batched.py):
+
+* the grouped fp32 GEMM equals the single-problem GEMM bit for bit, in every
+  tile shape;
+* a batch of uneven systems (different n, o, naux, walker counts) equals each
+  system run alone with afqmc_step / orthonormalize_state / measure /
+  reconfigure in fp32 precision: walkers, weights, log-overlaps, population
+  shifts, keys and block energies, bit for bit (the same kernel bodies);
+* a block captured as a HIP graph and replayed equals the eager batch.
+"""
+import numpy as np
+import pytest
+
+from ad_afqmc_prototype_amd import synthetic as syn
+
+pytestmark = pytest.mark.gpu
+
+SYSTEMS = [  # seed, n, o, naux, walkers
+    (101, 24, 4, 100, 16),
+    (102, 40, 6, 190, 40),
+    (103, 33, 9, 150, 24),
+    (104, 56, 12, 260, 64),
+]
+DT, QR, SR = 0.005, 2, 4
+
+
+def _members(device, systems=SYSTEMS, scale=0.25):
+    from ad_afqmc_prototype_amd.ham.chol import ham_chol
+    from ad_afqmc_prototype_amd.prop.batched import batch_system
+    from ad_afqmc_prototype_amd.trial.rhf import rhf_trial
+    out = []
+    for seed, n, o, g, nw in systems:
+        s = syn.make_system(seed, n, o, g, chol_scale=scale)
+        ham = ham_chol.from_arrays(s.h0, s.h1, s.chol, device=device)
+        out.append(batch_system(ham=ham, trial=rhf_trial(s.mo_up), n_walkers=nw, seed=seed))
+    return out
+
+
+def _alone(member, n_steps):
+    """The system run by itself through the single-system step and block functions."""
+    from ad_afqmc_prototype_amd.core.system import system
+    from ad_afqmc_prototype_amd.meas.rhf import make_rhf_meas_ops
+    from ad_afqmc_prototype_amd.prop.afqmc import afqmc_step, init_prop_state
+    from ad_afqmc_prototype_amd.prop.blocks import measure, orthonormalize_state, reconfigure
+    from ad_afqmc_prototype_amd.prop.chol_afqmc_ops import make_chol_afqmc_ops
+    from ad_afqmc_prototype_amd.prop.types import afqmc_params
+    from ad_afqmc_prototype_amd.trial.rhf import make_rhf_trial_ops
+    ham, trial = member.ham, member.trial
+    n, o = trial.norb, trial.nocc
+    sys_ = system(n, (o, o), "restricted")
+    t_ops = make_rhf_trial_ops(sys_)
+    m_ops = make_rhf_meas_ops(sys_, precision="fp32")
+    p_ops = make_chol_afqmc_ops(ham, "restricted", precision="fp32")
+    params = afqmc_params(dt=DT, n_walkers=member.n_walkers)
+    p_ctx = p_ops.build_prop_ctx(t_ops.get_rdm1(trial), DT)
+    m_ctx = m_ops.build_meas_ctx(ham, trial)
+    st = init_prop_state(sys=sys_, n_walkers=member.n_walkers, seed=member.seed, ham_data=ham,
+                         trial_ops_=t_ops, trial_data=trial, meas_ops=m_ops)
+    blocks = []
+    for t in range(1, n_steps + 1):
+        st = afqmc_step(st, params=params, ham_data=ham, trial_data=trial, meas_ops=m_ops,
+                        prop_ops=p_ops, prop_ctx=p_ctx, meas_ctx=m_ctx)
+        if t % SR == 0:
+            st = orthonormalize_state(st, sys=sys_, trial_data=trial, meas_ops_=m_ops)
+            st, sc, _ = measure(st, params=params, ham_data=ham, trial_data=trial, meas_ops=m_ops,
+                                meas_ctx=m_ctx)
+            blocks.append((float(sc["energy"]), float(sc["weight"])))
+            st, _ = reconfigure(st, sys=sys_, trial_data=trial, meas_ops_=m_ops)
+        elif t % QR == 0:
+            st = orthonormalize_state(st, sys=sys_, trial_data=trial, meas_ops_=m_ops)
+    return st, blocks
+
+
+def test_grouped_gemm_matches_single(gpu_device):
+    import torch
+    from ad_afqmc_prototype_amd import _lib, kernels
+    from ad_afqmc_prototype_amd.prop import batched as B
+    g = torch.Generator(device="cpu").manual_seed(3)
+    dims = [(1, 1, 1), (37, 70, 19), (130, 129, 260), (64, 4096, 96), (300, 17, 1001)]
+    ops = []
+    for i, (M, N, K) in enumerate(dims):
+        a = torch.randn((M, K), generator=g).to(gpu_device, torch.float32)
+        b = torch.randn((N, K), generator=g).to(gpu_device, torch.float32)
+        c64 = i % 2 == 0
+        ref = torch.empty((M, N), dtype=torch.float64 if c64 else torch.float32, device=gpu_device)
+        kernels.sgemm_nt(a, b, ref, alpha=0.75)
+        ops.append((a, b, ref, c64))
+    for shape in B.GEMM_SHAPES:
+        bm, bn = B.gemm_tile_shape(shape)
+        arr = (B._gemm_problem * len(ops))()
+        outs, start = [], 0
+        for p, (a, b, ref, c64) in zip(arr, ops):
+            out = torch.full_like(ref, float("nan"))
+            outs.append(out)
+            M, K = a.shape
+            N = b.shape[0]
+            p.M, p.N, p.K, p.tile_start = M, N, K, start
+            p.lda, p.ldb, p.ldc = K, K, N
+            p.A, p.B, p.C, p.alpha, p.c_fp64 = a.data_ptr(), b.data_ptr(), out.data_ptr(), 0.75, int(c64)
+            start += -(-M // bm) * -(-N // bn)
+        tab = B._upload(arr).to(gpu_device)
+        _lib.call("afq_sgemm_nt_grouped", len(ops), tab.data_ptr(), start, shape,
+                  torch.cuda.current_stream().cuda_stream)
+        torch.cuda.synchronize()
+        for out, (_, _, ref, _) in zip(outs, ops):
+            assert torch.equal(out, ref), f"shape {shape}: grouped GEMM differs"
+
+
+def _compare_system(batch, i, st, blocks_alone, blocks_batch):
+    import torch
+    got = batch.system_state(i)
+    assert torch.equal(got["walkers"], st.walkers), f"system {i}: walkers"
+    assert torch.equal(got["weights"], st.weights), f"system {i}: weights"
+    assert torch.equal(got["log_overlaps"], st.log_overlaps), f"system {i}: log-overlaps"
+    assert torch.equal(got["pop"], st.pop_control_ene_shift), f"system {i}: pop shift"
+    assert torch.equal(got["key"], st.rng_key), f"system {i}: key"
+    assert blocks_batch == blocks_alone, f"system {i}: block energies / weights"
+
+
+@pytest.mark.parametrize("shapes", [(0, 0), (3, 1), (2, 3)])
+def test_batch_matches_systems_run_alone(gpu_device, shapes):
+    from ad_afqmc_prototype_amd.prop.batched import system_batch
+    print(f"uneven batch of {len(SYSTEMS)} systems, GEMM shapes {shapes}", flush=True)
+    members = _members(gpu_device)
+    batch = system_batch(members, dt=DT, qr_every=QR, sr_every=SR, gemm_shapes=shapes)
+    n_steps = 2 * SR
+    per_block = []
+    for _ in range(n_steps // SR):
+        batch.advance(SR)
+        per_block.append(batch.block_stats.cpu().numpy().copy())
+    for i, m in enumerate(members):
+        st, blocks = _alone(m, n_steps)
+        bb = [(float(b[i, 0]), float(b[i, 1])) for b in per_block]
+        _compare_system(batch, i, st, blocks, bb)
+    # the run is live: weights moved and the comb copied walkers
+    w = batch.system_state(3)["weights"]
+    assert float(w.min()) > 0.0 and np.isfinite(per_block[-1]).all()
+
+
+def test_batch_graph_replay_matches_eager(gpu_device):
+    import torch
+    from ad_afqmc_prototype_amd.prop.batched import captured_batch_block, system_batch
+    print("captured batch block vs eager", flush=True)
+    eager = system_batch(_members(gpu_device), dt=DT, qr_every=QR, sr_every=SR)
+    graph_batch = system_batch(_members(gpu_device), dt=DT, qr_every=QR, sr_every=SR)
+    cap = captured_batch_block(graph_batch, warmup_blocks=1)   # advances one block eagerly
+    eager.advance(SR)
+    for rep in range(3):
+        stats = cap.replay().clone()
+        eager.advance(SR)
+        torch.cuda.synchronize()
+        assert torch.equal(stats, eager.block_stats), f"replay {rep}: block stats"
+        for i in range(len(SYSTEMS)):
+            a, b = graph_batch.system_state(i), eager.system_state(i)
+            for k in a:
+                assert torch.equal(a[k], b[k]), f"replay {rep}, system {i}: {k}"
+        # eager work on the program's own stream between replays
+        torch.zeros(1 << 20, device=gpu_device).sum().item()
+
+
+def test_tune_chooses_shapes_and_caches(gpu_device, tmp_path):
+    from ad_afqmc_prototype_amd.prop.batched import GEMM_SHAPES, system_batch
+    batch = system_batch(_members(gpu_device), dt=DT, qr_every=QR, sr_every=SR)
+    path = str(tmp_path / "tiles.json")
+    entry = batch.tune(reps=2, cache_path=path)
+    assert tuple(entry["shapes"]) == batch.gemm_shapes
+    assert all(s in GEMM_SHAPES for s in batch.gemm_shapes)
+    again = system_batch(_members(gpu_device), dt=DT, qr_every=QR, sr_every=SR)
+    assert again.tune(cache_path=path)["shapes"] == entry["shapes"]
