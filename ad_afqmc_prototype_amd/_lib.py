@@ -62,9 +62,10 @@ _SIGNATURES = {
     "afq_overlap_greens_grouped": [c_int, c_vp, c_int, c_int, c_vp],
     "afq_phaseless_update_grouped": [c_int, c_vp, c_vp],
     "afq_orthonormalize_grouped": [c_int, c_vp, c_int, c_int, c_int, c_vp],
+    "afq_energy_f32_grouped": [c_int, c_vp, c_int, c_int, c_int, c_vp],
 }
 
-ABI_VERSION = 2
+ABI_VERSION = 3
 
 
 class HipLibraryError(RuntimeError):

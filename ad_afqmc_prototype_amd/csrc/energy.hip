@@ -16,15 +16,15 @@ namespace {
 
 constexpr int THREADS = 256;
 
+// Walkers w0 .. w0 + WPB - 1 of one batch (nw walkers).
 template <typename R, int NT, int WPB>
-__global__ __launch_bounds__(THREADS, 1) void energy_kernel(
+__device__ __forceinline__ void energy_body(
     int nw, int n, int o, int nspin, int naux, double h0, const double* __restrict__ greens,
     const R* __restrict__ rotp, int ldr, const double* __restrict__ rot_h1,
-    double2* __restrict__ eout) {
+    double2* __restrict__ eout, int w0, unsigned char* lds_raw) {
     using MF = afq::mfma16<R>;
     using ACC = typename MF::acc;
     using V4 = typename afq::vec4<R>::type;
-    extern __shared__ __attribute__((aligned(16))) unsigned char lds_raw[];
     constexpr int TC = NT * 16;
     constexpr int XP = TC + 1;
     constexpr int SP = TC + 1;
@@ -36,7 +36,6 @@ __global__ __launch_bounds__(THREADS, 1) void energy_kernel(
     cs_off = (cs_off + 15) & ~(size_t)15;
     double2* cs = reinterpret_cast<double2*>(lds_raw + cs_off);  // [WPB][naux] (nspin == 2)
 
-    const int w0 = blockIdx.x * WPB;
     const int nwb = min(WPB, nw - w0);
     const long long ldg = (long long)nspin * o * n;
     const int tid = threadIdx.x;
@@ -222,6 +221,29 @@ __global__ __launch_bounds__(THREADS, 1) void energy_kernel(
         }
         eout[w0 + b] = make_double2(er, ei);
     }
+}
+
+template <typename R, int NT, int WPB>
+__global__ __launch_bounds__(THREADS, 1) void energy_kernel(
+    int nw, int n, int o, int nspin, int naux, double h0, const double* __restrict__ greens,
+    const R* __restrict__ rotp, int ldr, const double* __restrict__ rot_h1,
+    double2* __restrict__ eout) {
+    extern __shared__ __attribute__((aligned(16))) unsigned char lds_raw[];
+    energy_body<R, NT, WPB>(nw, n, o, nspin, naux, h0, greens, rotp, ldr, rot_h1, eout,
+                            blockIdx.x * WPB, lds_raw);
+}
+
+// One workgroup per walker of every system (restricted walkers, fp32
+// contractions); every system has ceil(o / 16) == NT.
+template <int NT>
+__global__ __launch_bounds__(THREADS, 1) void energy_f32_grouped_kernel(
+    int count, const afq_system_step* __restrict__ S) {
+    extern __shared__ __attribute__((aligned(16))) unsigned char lds_raw[];
+    const int b = blockIdx.x;
+    const int si = afq::find_group(count, b, [&](int i) { return S[i].walker0; });
+    const afq_system_step& d = S[si];
+    energy_body<float, NT, 1>(d.nw, d.n, d.o, 1, d.naux, d.h0, d.greens, d.rot_chol, d.ldr, d.rot_h1,
+                              (double2*)d.energy, b - d.walker0, lds_raw);
 }
 
 template <typename R, int NT, int WPB>
@@ -573,4 +595,28 @@ extern "C" int afq_energy_bf16x3(int nw, int n, int o, int nspin, int naux, doub
     if (fits2(energy_bf16x3_lds<2, 2>(ldr, nspin, naux)))
         return launch_energy_bf16x3<2, 2>(nw, n, o, nspin, naux, h0, greens, rh, rl, ldr, rot_h1, energy, st);
     return launch_energy_bf16x3<2, 1>(nw, n, o, nspin, naux, h0, greens, rh, rl, ldr, rot_h1, energy, st);
+}
+
+extern "C" int afq_energy_f32_grouped(int count, const afq_system_step* sys, int total_walkers,
+                                      int max_n, int nt, void* stream) {
+    AFQ_REQUIRE(count > 0 && total_walkers > 0, "afq_energy_f32_grouped: count=%d total_walkers=%d",
+                count, total_walkers);
+    AFQ_REQUIRE(sys, "afq_energy_f32_grouped: null pointer");
+    AFQ_REQUIRE(nt == 1 || nt == 2, "afq_energy_f32_grouped: nt=%d must be 1 or 2", nt);
+    AFQ_REQUIRE(max_n > 0 && max_n <= 256, "afq_energy_f32_grouped: max_n=%d", max_n);
+    hipStream_t st = (hipStream_t)stream;
+    if (nt == 1) {
+        const size_t lds = energy_lds<float, 1, 1>(max_n, 1, 0);
+        AFQ_REQUIRE((energy_fits<float, 1, 1>(max_n, 1, 0)), "afq_energy_f32_grouped: max_n=%d too large", max_n);
+        if (lds > 64 * 1024) afq::allow_full_lds((const void*)energy_f32_grouped_kernel<1>);
+        hipLaunchKernelGGL(energy_f32_grouped_kernel<1>, dim3((unsigned)total_walkers), dim3(THREADS), lds, st,
+                           count, sys);
+    } else {
+        const size_t lds = energy_lds<float, 2, 1>(max_n, 1, 0);
+        AFQ_REQUIRE((energy_fits<float, 2, 1>(max_n, 1, 0)), "afq_energy_f32_grouped: max_n=%d too large", max_n);
+        if (lds > 64 * 1024) afq::allow_full_lds((const void*)energy_f32_grouped_kernel<2>);
+        hipLaunchKernelGGL(energy_f32_grouped_kernel<2>, dim3((unsigned)total_walkers), dim3(THREADS), lds, st,
+                           count, sys);
+    }
+    return afq::check_launch("afq_energy_f32_grouped");
 }

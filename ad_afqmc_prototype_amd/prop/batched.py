@@ -54,15 +54,15 @@ class _system_step(ctypes.Structure):
 
     _fields_ = [("n", ctypes.c_int), ("o", ctypes.c_int), ("naux", ctypes.c_int), ("nw", ctypes.c_int),
                 ("ldv", ctypes.c_int), ("walker0", ctypes.c_int), ("tblock0", ctypes.c_int),
-                ("reserved0", ctypes.c_int),
+                ("ldr", ctypes.c_int),
                 ("sqrt_dt", ctypes.c_double), ("dt", ctypes.c_double), ("e_estimate", ctypes.c_double),
                 ("h0_prop", ctypes.c_double), ("weight_floor", ctypes.c_double),
                 ("weight_cap", ctypes.c_double), ("damping", ctypes.c_double),
-                ("reserved1", ctypes.c_double)] + [(name, _vp) for name in (
+                ("h0", ctypes.c_double)] + [(name, _vp) for name in (
                     "trial", "walkers_in", "walkers_out", "greens", "logdet", "fields", "fb", "mf",
                     "xbar", "shift_term", "fb_term", "exp_h1_half", "vhs", "log_overlaps_in",
                     "log_overlaps_out", "weights_in", "weights_out", "pop_in", "pop_out", "key_in",
-                    "key_out")]
+                    "key_out", "rot_chol", "rot_h1", "energy")]
 
 
 class _gemm_problem(ctypes.Structure):
@@ -73,7 +73,7 @@ class _gemm_problem(ctypes.Structure):
                 ("A", _vp), ("B", _vp), ("C", _vp), ("alpha", ctypes.c_float), ("c_fp64", ctypes.c_int)]
 
 
-assert ctypes.sizeof(_system_step) == 8 * 4 + 8 * 8 + 21 * 8
+assert ctypes.sizeof(_system_step) == 8 * 4 + 8 * 8 + 24 * 8
 assert ctypes.sizeof(_gemm_problem) == 72
 
 GEMM_SHAPES = (0, 1, 2, 3)
@@ -200,6 +200,7 @@ class system_batch:
         self.shift = P([m["nw"] for m in ms], c128)
         self.fbt = P([m["nw"] for m in ms], c128)
         self.vhs = P([2 * m["nw"] * m["n"] * m["ldv"] for m in ms], f32)
+        self.eloc = P([m["nw"] for m in ms], c128)
         self.lo = [P([m["nw"] for m in ms], c128) for _ in range(2)]
         self.weights = [P([m["nw"] for m in ms], f64) for _ in range(2)]
         self.pop = [torch.zeros((self.count,), dtype=f64, device=dev) for _ in range(2)]
@@ -219,18 +220,22 @@ class system_batch:
             self.mf.part(i).copy_(m["mf"])
             self.trials.part(i).copy_(m["trial"].reshape(-1))
 
-    def _desc(self, parity: int):
+    def _desc(self, parity: int, idx: Sequence[int] | None = None):
+        """Descriptor table of the systems ``idx`` (all by default) for steps
+        from buffer set ``parity``; walker0 / tblock0 are prefix sums over the
+        table.  Returns (array, walkers, Trotter workgroups)."""
         a, b = parity, 1 - parity
-        arr = (_system_step * self.count)()
+        idx = list(range(self.count)) if idx is None else list(idx)
+        arr = (_system_step * len(idx))()
         w0 = t0 = 0
-        for i, m in enumerate(self.members):
-            d = arr[i]
-            d.n, d.o, d.naux, d.nw, d.ldv = m["n"], m["o"], m["g"], m["nw"], m["ldv"]
+        for d, i in zip(arr, idx):
+            m = self.members[i]
+            d.n, d.o, d.naux, d.nw, d.ldv, d.ldr = m["n"], m["o"], m["g"], m["nw"], m["ldv"], m["ldv"]
             d.walker0, d.tblock0 = w0, t0
             w0 += m["nw"]
             t0 += m["nw"] * ((m["o"] + 15) // 16)
             d.sqrt_dt, d.dt, d.e_estimate, d.h0_prop = self.sqrt_dt, self.dt, m["e_est"], m["h0_prop"]
-            d.weight_floor, d.weight_cap, d.damping = self.floor, self.cap, self.damping
+            d.weight_floor, d.weight_cap, d.damping, d.h0 = self.floor, self.cap, self.damping, m["h0"]
             d.trial = self.trials.ptr(i)
             d.walkers_in, d.walkers_out = self.walkers[a].ptr(i), self.walkers[b].ptr(i)
             d.greens, d.logdet = self.greens.ptr(i), self.logdet.ptr(i)
@@ -241,8 +246,9 @@ class system_batch:
             d.weights_in, d.weights_out = self.weights[a].ptr(i), self.weights[b].ptr(i)
             d.pop_in, d.pop_out = self.pop[a][i:i + 1].data_ptr(), self.pop[b][i:i + 1].data_ptr()
             d.key_in, d.key_out = self.keys[a][i].data_ptr(), self.keys[b][i].data_ptr()
-        self.total_tblocks = t0
-        return arr
+            d.rot_chol, d.rot_h1, d.energy = (m["rot_pad"].data_ptr(), m["rot_h1"].data_ptr(),
+                                              self.eloc.ptr(i))
+        return arr, w0, t0
 
     def _problems(self, which: str, shape: int):
         bm, bn = gemm_tile_shape(shape)
@@ -267,8 +273,29 @@ class system_batch:
         return arr, start
 
     def _tables(self):
-        self.desc_host = [self._desc(0), self._desc(1)]
+        full = [self._desc(0), self._desc(1)]
+        self.desc_host = [f[0] for f in full]
         self.desc = [_upload(d).to(self.device) for d in self.desc_host]
+        # Trotter launches per bucket of equal row-tile count (the workgroup is
+        # sized for its bucket's largest n), energy launches per bucket of equal
+        # occupied-tile count (the rotated-Cholesky layout of the system)
+        def buckets(key):
+            out = {}
+            for i, m in enumerate(self.members):
+                out.setdefault(key(m), []).append(i)
+            return [out[k] for k in sorted(out)]
+        self.trotter_buckets = []
+        for idx in buckets(lambda m: (m["n"] + 15) // 16):
+            tabs = [self._desc(p, idx) for p in (0, 1)]
+            self.trotter_buckets.append(dict(
+                tables=[_upload(t[0]).to(self.device) for t in tabs], count=len(idx),
+                blocks=tabs[0][2], max_n=max(self.members[i]["n"] for i in idx)))
+        self.energy_buckets = []
+        for idx in buckets(lambda m: (m["o"] + 15) // 16):
+            arr, nw, _ = self._desc(0, idx)
+            self.energy_buckets.append(dict(table=_upload(arr).to(self.device), count=len(idx),
+                                            walkers=nw, max_n=max(self.members[i]["n"] for i in idx),
+                                            nt=(self.members[idx[0]]["o"] + 15) // 16))
         self.gemm_tables = {}
         for which in ("fb", "vhs"):
             for shape in GEMM_SHAPES:
@@ -300,8 +327,10 @@ class system_batch:
         kernels._call("afq_field_shifts_grouped", d, self.count, d.data_ptr(), self.total_walkers, st)
         self._convert(self.xbar, self.xbar32)
         self._grouped_gemm("vhs")
-        kernels._call("afq_trotter_f32_grouped", d, self.count, d.data_ptr(), self.total_tblocks,
-                      self.max_n, self.n_exp_terms, st)
+        for bk in self.trotter_buckets:
+            t = bk["tables"][p]
+            kernels._call("afq_trotter_f32_grouped", t, bk["count"], t.data_ptr(), bk["blocks"],
+                          bk["max_n"], self.n_exp_terms, st)
         kernels._call("afq_overlap_greens_grouped", d, self.count, d.data_ptr(), self.total_walkers,
                       self.max_o, st)
         kernels._call("afq_phaseless_update_grouped", d, self.count, d.data_ptr(), st)
@@ -326,12 +355,16 @@ class system_batch:
         (prop/blocks.py:59-79), comb with a uniform from the system's key
         (prop/blocks.py:81-92).  Called right after _orthonormalize."""
         thresh = (2.0 / self.dt) ** 0.5
+        for bk in self.energy_buckets:
+            t = bk["table"]
+            kernels._call("afq_energy_f32_grouped", t, bk["count"], t.data_ptr(), bk["walkers"],
+                          bk["max_n"], bk["nt"], kernels._stream(t))
         for i, m in enumerate(self.members):
             nw, n, o = m["nw"], m["n"], m["o"]
             g = self.greens.part(i, (2, nw, o * n))
             w = self.weights[0].part(i)
             pop = self.pop[0][i]
-            e = kernels.energy(g, nw, n, o, 1, m["rot_pad"], m["rot_h1"], m["h0"])
+            e = self.eloc.part(i)
             out = kernels.block_energy(e, w, pop, e_ref=m["e_est"], thresh=thresh,
                                        shift_ema=self.shift_ema)
             self.block_stats[i].copy_(out[:2])

@@ -219,6 +219,111 @@ def end_to_end(workload: str, precision: str, device, n_eql: int, n_blocks: int)
             "block_energies_Ha": [float(x) for x in e_b]}
 
 
+def batched_run(device, n_steps: int, n_systems: int = 64, chol_scale: float = 0.05,
+                baseline_steps: int = 20):
+    """Uneven-system batch (N ~ U[40,160], nocc ~ U[6,24], naux = round(N U[4,6]),
+    walkers ~ U[64,512], seeds 100..), fp32 contractions, dt 0.005, QR every 10
+    and measurement + comb every 20 steps: walker-steps/s of the grouped step
+    (eager, and one HIP graph per block) and of the same systems stepped one
+    after another through afqmc_step (``baseline_steps`` steps)."""
+    import numpy as np
+    import torch
+
+    from ad_afqmc_prototype_amd import synthetic as syn
+    from ad_afqmc_prototype_amd.ham.chol import ham_chol
+    from ad_afqmc_prototype_amd.prop.afqmc import afqmc_step
+    from ad_afqmc_prototype_amd.prop.batched import batch_system, captured_batch_block, system_batch
+    from ad_afqmc_prototype_amd.prop.blocks import measure, orthonormalize_state, reconfigure
+    from ad_afqmc_prototype_amd.prop.types import afqmc_params
+    from ad_afqmc_prototype_amd.trial.rhf import rhf_trial
+
+    specs = syn.uneven_batch_specs(n_systems)
+    members = []
+    for sp in specs:
+        s = syn.make_system(sp["seed"], sp["norb"], sp["nocc"], sp["naux"], chol_scale=chol_scale)
+        members.append(batch_system(ham=ham_chol.from_arrays(s.h0, s.h1, s.chol, device=device),
+                                    trial=rhf_trial(s.mo_up), n_walkers=sp["n_walkers"], seed=sp["seed"]))
+    dt, qr, sr = 0.005, 10, 20
+    batch = system_batch(members, dt=dt, qr_every=qr, sr_every=sr)
+    tuned = batch.tune(reps=5)
+    nw_tot = batch.total_walkers
+    n_steps = max(sr, n_steps // sr * sr)
+    out = {"workload": f"{n_systems} uneven systems (seeds {specs[0]['seed']}..{specs[-1]['seed']}), "
+                       f"{nw_tot} walkers, chol_scale {chol_scale}",
+           "n_systems": n_systems, "total_walkers": nw_tot, "steps": n_steps, "dt": dt,
+           "qr_every": qr, "measure_sr_every": sr, "gemm_shapes": list(batch.gemm_shapes),
+           "tune_ms": tuned["ms"], "gflop_per_step": round(batch.flops_per_step() / 1e9, 2)}
+    # eager grouped launches (one block of warm-up first)
+    batch.advance(sr)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    batch.advance(n_steps)
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    out["grouped_eager"] = {"walker_steps_per_s": nw_tot * n_steps / el, "ms_per_step": 1e3 * el / n_steps}
+    # one graph launch per block
+    cap = captured_batch_block(batch, warmup_blocks=1)
+    cap.replay()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(n_steps // sr):
+        cap.replay()
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    out["grouped_graph"] = {"walker_steps_per_s": nw_tot * n_steps / el, "ms_per_step": 1e3 * el / n_steps,
+                            "tflops_algorithmic": round(batch.flops_per_step() * n_steps / el / 1e12, 2)}
+    stats = batch.block_stats.cpu().numpy()
+    out["block_energy_finite"] = bool(np.isfinite(stats).all())
+    del cap, batch
+    torch.cuda.empty_cache()
+    # the same systems one after another through the single-system step
+    from ad_afqmc_prototype_amd.core.system import system
+    from ad_afqmc_prototype_amd.meas.rhf import make_rhf_meas_ops
+    from ad_afqmc_prototype_amd.prop.afqmc import init_prop_state
+    from ad_afqmc_prototype_amd.prop.chol_afqmc_ops import make_chol_afqmc_ops
+    from ad_afqmc_prototype_amd.trial.rhf import make_rhf_trial_ops
+    runs = []
+    for m in members:
+        n, o = m.trial.norb, m.trial.nocc
+        sys_ = system(n, (o, o), "restricted")
+        t_ops, m_ops = make_rhf_trial_ops(sys_), make_rhf_meas_ops(sys_, precision="fp32")
+        p_ops = make_chol_afqmc_ops(m.ham, "restricted", precision="fp32")
+        kw = dict(params=afqmc_params(dt=dt, n_walkers=m.n_walkers), ham_data=m.ham, trial_data=m.trial,
+                  meas_ops=m_ops, prop_ops=p_ops, prop_ctx=p_ops.build_prop_ctx(t_ops.get_rdm1(m.trial), dt),
+                  meas_ctx=m_ops.build_meas_ctx(m.ham, m.trial))
+        st = init_prop_state(sys=sys_, n_walkers=m.n_walkers, seed=m.seed, ham_data=m.ham, trial_ops_=t_ops,
+                             trial_data=m.trial, meas_ops=m_ops)
+        runs.append((sys_, kw, st))
+
+    def loop(k):
+        for j, (sys_, kw, st) in enumerate(runs):
+            for t in range(1, k + 1):
+                st = afqmc_step(st, **kw)
+                if t % sr == 0:
+                    st = orthonormalize_state(st, sys=sys_, trial_data=kw["trial_data"], meas_ops_=kw["meas_ops"])
+                    st, _, _ = measure(st, params=kw["params"], ham_data=kw["ham_data"],
+                                       trial_data=kw["trial_data"], meas_ops=kw["meas_ops"],
+                                       meas_ctx=kw["meas_ctx"])
+                    st, _ = reconfigure(st, sys=sys_, trial_data=kw["trial_data"], meas_ops_=kw["meas_ops"])
+                elif t % qr == 0:
+                    st = orthonormalize_state(st, sys=sys_, trial_data=kw["trial_data"], meas_ops_=kw["meas_ops"])
+            runs[j] = (sys_, kw, st)
+
+    loop(2)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    loop(baseline_steps)
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    out["one_system_at_a_time"] = {"walker_steps_per_s": nw_tot * baseline_steps / el,
+                                   "ms_per_step": 1e3 * el / baseline_steps, "steps": baseline_steps}
+    out["graph_speedup_vs_one_at_a_time"] = (out["grouped_graph"]["walker_steps_per_s"]
+                                             / out["one_system_at_a_time"]["walker_steps_per_s"])
+    del runs
+    torch.cuda.empty_cache()
+    return out
+
+
 def measured_traffic(workload: str, kernel: str):
     """HBM bytes per launch of ``kernel`` from the committed rocprofv3 FETCH_SIZE /
     WRITE_SIZE summaries (profiles/), each scaled by its copy-kernel
@@ -321,6 +426,8 @@ def main():
     ap.add_argument("--end-to-end", type=int, nargs=2, default=(5, 20), metavar=("EQL", "SAMPLE"),
                     help="driver run (equilibration and sampling blocks) for the end_to_end field; "
                          "0 0 skips it")
+    ap.add_argument("--batched-steps", type=int, default=100,
+                    help="steps of the 64-system uneven batch for the batched field; 0 skips it")
     ap.add_argument("--no-cadence", action="store_true",
                     help="time afqmc_step alone (no QR / measurement / reconfiguration)")
     args = ap.parse_args()
@@ -410,6 +517,8 @@ def main():
             e2e["blocks_on_common_trajectory"] = same
             e2e["max_abs_diff_mHa_common_trajectory"] = max(diffs[:same], default=None)
         out["end_to_end"] = e2e
+    if args.batched_steps > 0:
+        out["batched"] = batched_run(device, args.batched_steps)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(args.workload)
     print(json.dumps(out), flush=True)

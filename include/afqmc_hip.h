@@ -297,9 +297,9 @@ typedef struct afq_system_step {
     int ldv;                     /* n rounded up to 4: row pitch of exp_h1_half and VHS */
     int walker0;                 /* first walker of this system in the batch (prefix sum of nw) */
     int tblock0;                 /* first Trotter workgroup: prefix sum of nw * ceil(o / 16) */
-    int reserved0;
+    int ldr;                     /* n rounded up to 4: row pitch of rot_chol */
     double sqrt_dt, dt, e_estimate, h0_prop;
-    double weight_floor, weight_cap, damping, reserved1;
+    double weight_floor, weight_cap, damping, h0;
     const double* trial;         /* (n, o) float64 RHF orbitals */
     const double* walkers_in;    /* complex128 (nw, n, o) */
     double* walkers_out;         /* complex128 (nw, n, o) */
@@ -321,6 +321,9 @@ typedef struct afq_system_step {
     double* pop_out;             /* float64[1] */
     const unsigned* key_in;      /* uint32[2] Threefry key */
     unsigned* key_out;           /* uint32[2] */
+    const float* rot_chol;       /* (naux, round16(o), ldr) float32 half-rotated Cholesky, zero padded */
+    const double* rot_h1;        /* (o, n) float64 */
+    double* energy;              /* complex128 (nw) local energies of walkers_out */
 } afq_system_step;
 
 /* One problem of a grouped GEMM: C (M x N) = alpha A (M x K) B^T (N x K). */
@@ -378,6 +381,14 @@ int afq_overlap_greens_grouped(int count, const afq_system_step* sys, int total_
 
 /* afq_phaseless_update for every system, one workgroup each (prop/afqmc.py:116-138). */
 int afq_phaseless_update_grouped(int count, const afq_system_step* sys, void* stream);
+
+/*
+ * afq_energy_f32 (restricted walkers) for every system: local energies of the
+ * Green's functions in `greens` into `energy` (meas/rhf.py:36-50).  Every
+ * system of the table must have ceil(o / 16) == nt (1 or 2); max_n the largest n.
+ */
+int afq_energy_f32_grouped(int count, const afq_system_step* sys, int total_walkers, int max_n,
+                           int nt, void* stream);
 
 /*
  * afq_orthonormalize for every system: walkers_in -> walkers_out

@@ -15,11 +15,12 @@ from ad_afqmc_prototype_amd import synthetic as syn
 
 pytestmark = pytest.mark.gpu
 
-SYSTEMS = [  # seed, n, o, naux, walkers
-    (101, 24, 4, 100, 16),
+SYSTEMS = [  # seed, n, o, naux, walkers: three Trotter buckets (2, 3, 4 row tiles),
+    (101, 24, 4, 100, 16),   # two energy buckets (o <= 16, o > 16)
     (102, 40, 6, 190, 40),
     (103, 33, 9, 150, 24),
     (104, 56, 12, 260, 64),
+    (105, 48, 20, 220, 24),
 ]
 DT, QR, SR = 0.005, 2, 4
 
