@@ -63,9 +63,14 @@ _SIGNATURES = {
     "afq_phaseless_update_grouped": [c_int, c_vp, c_vp],
     "afq_orthonormalize_grouped": [c_int, c_vp, c_int, c_int, c_int, c_vp],
     "afq_energy_f32_grouped": [c_int, c_vp, c_int, c_int, c_int, c_vp],
+    # Hamiltonian ingestion (modified Cholesky)
+    "afq_mcd_workspace": [c_int, ctypes.POINTER(c_ll), ctypes.POINTER(c_ll)],
+    "afq_mcd_start": [c_int, c_vp, c_ll, c_double, c_int, c_vp, c_vp, c_vp, c_vp, c_vp],
+    "afq_mcd_iterate": [c_int, c_vp, c_ll, c_double, c_int, c_int, c_vp, c_vp, c_vp, c_vp, c_vp],
+    "afq_unpack_pairs": [c_int, c_int, c_vp, c_ll, c_vp, c_vp],
 }
 
-ABI_VERSION = 3
+ABI_VERSION = 4
 
 
 class HipLibraryError(RuntimeError):

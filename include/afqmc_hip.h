@@ -397,6 +397,36 @@ int afq_energy_f32_grouped(int count, const afq_system_step* sys, int total_walk
 int afq_orthonormalize_grouped(int count, const afq_system_step* sys, int total_walkers, int max_n,
                                int max_o, void* stream);
 
+/*
+ * ---------------------------------------------------------------------------
+ * Hamiltonian ingestion: modified (pivoted, incomplete) Cholesky decomposition
+ * of the two-electron integrals, replacing modified_cholesky
+ * (integrals.py:3-46) as called by the front end (default.py:18-21).
+ *   mat   : (size x size) float64, row pitch ldm: the 4-fold symmetric ERIs
+ *           (pq|rs) in packed pair indices, size = n (n + 1) / 2, pair (m, p)
+ *           with m >= p at m (m + 1) / 2 + p;
+ *   vecs  : (max_vectors x size) float64, the Cholesky vectors v_0, v_1, ...;
+ *   resid, partial : workspace of afq_mcd_workspace's sizes (doubles);
+ *   state : float64[4] = {pivot value, pivot index, vectors so far, stopped}.
+ * afq_mcd_start forms v_0 and the stop flag; afq_mcd_iterate enqueues
+ * `iterations` iterations (each a no-op once stopped), so the caller can
+ * enqueue a chunk, read state[3] and continue; the result is
+ * v_0 .. v_{state[2]-1}.  Iteration: M += v_k^2, nu = argmax |diag - M|
+ * (first index on ties), d = |diag - M|[nu],
+ * v_{k+1} = (mat[nu] - sum_{i<=k} v_i[nu] v_i) / sqrt(d + 1e-10), while
+ * |d| > max_error and fewer than max_vectors (<= size) vectors.
+ * afq_unpack_pairs: chol[i][m][p] = vecs[i][pair(max(m,p), min(m,p))],
+ * (nchol, n, n) float64 (integrals.py:39-45).
+ * ---------------------------------------------------------------------------
+ */
+int afq_mcd_workspace(int size, long long* resid_doubles, long long* partial_doubles);
+int afq_mcd_start(int size, const double* mat, long long ldm, double max_error, int max_vectors,
+                  double* vecs, double* resid, double* partial, double* state, void* stream);
+int afq_mcd_iterate(int size, const double* mat, long long ldm, double max_error, int max_vectors,
+                    int iterations, double* vecs, double* resid, double* partial, double* state,
+                    void* stream);
+int afq_unpack_pairs(int nchol, int n, const double* vecs, long long ldv, double* chol, void* stream);
+
 #ifdef __cplusplus
 }
 #endif
