@@ -102,11 +102,26 @@ def force_bias_planes(walkers, meas_ctx: rhf_meas_ctx, trial_data, greens=None) 
     nspin = _nspin(walkers)
     if greens is None:
         greens = trial_rhf.greens(walkers, trial_data)
-    alpha = 2.0 if nspin == 1 else 1.0
+    return _force_bias_rows(greens.greens, meas_ctx, nspin, 2.0 if nspin == 1 else 1.0)
+
+
+def force_bias_planes_g(walkers, meas_ctx: rhf_meas_ctx, trial_data, greens=None) -> torch.Tensor:
+    """force_bias_kernel_g (meas/rhf.py:117-130) as split planes (2, nw, naux):
+    rot_chol_flat @ vec(G_upup + G_dndn) from the (2o, 2n) Green's function."""
+    if greens is None:
+        greens = trial_rhf.greens_g(walkers, trial_data)
+    nw, n2, o2 = walkers.shape
+    n, o = n2 // 2, o2 // 2
+    gv = greens.greens.view(2, nw, o2, n2)
+    rows = (gv[:, :, :o, :n] + gv[:, :, o:, n:]).reshape(2, nw, o * n).contiguous()
+    return _force_bias_rows(rows, meas_ctx, 1, 1.0)
+
+
+def _force_bias_rows(g: torch.Tensor, meas_ctx: rhf_meas_ctx, nspin: int, alpha: float) -> torch.Tensor:
+    """fb planes (2, nw, naux) = alpha * rows of g (2, nw, k) . rot^T in the ctx's precision."""
     op = meas_ctx.fb_operand(nspin)
     if meas_ctx.precision == "fp64":
-        return kernels.force_bias(greens.greens, op, alpha)
-    g = greens.greens
+        return kernels.force_bias(g, op, alpha)
     _, nw, k = g.shape
     naux = meas_ctx.rot_chol.shape[1]
     fb = torch.empty((2, nw, naux), dtype=torch.float64, device=g.device)
@@ -124,6 +139,12 @@ def force_bias_planes(walkers, meas_ctx: rhf_meas_ctx, trial_data, greens=None) 
 def force_bias_kernel_r(walkers, _ham, meas_ctx: rhf_meas_ctx, trial_data, greens=None):
     """meas/rhf.py:27-33, batched: (nw, naux) complex."""
     fb = force_bias_planes(walkers, meas_ctx, trial_data, greens)
+    return torch.complex(fb[0], fb[1])
+
+
+def force_bias_kernel_g(walkers, _ham, meas_ctx: rhf_meas_ctx, trial_data, greens=None):
+    """meas/rhf.py:117-130, batched: (nw, naux) complex."""
+    fb = force_bias_planes_g(walkers, meas_ctx, trial_data, greens)
     return torch.complex(fb[0], fb[1])
 
 
@@ -173,7 +194,11 @@ def make_rhf_meas_ops(sys: system, precision: str = "fp64") -> meas_ops:
                         log_overlap=trial_rhf.log_overlap, greens=trial_rhf.greens,
                         force_bias_planes=force_bias_planes)
     if wk == "generalized":
-        raise NotImplementedError("generalized walkers are not implemented on the HIP path")
+        # meas/rhf.py:169-174: overlap and force bias, no energy kernel
+        return meas_ops(overlap=trial_rhf.overlap_g, build_meas_ctx=build_meas_ctx_p,
+                        kernels={k_force_bias: force_bias_kernel_g},
+                        log_overlap=trial_rhf.log_overlap_g, greens=trial_rhf.greens_g,
+                        force_bias_planes=force_bias_planes_g)
     raise ValueError(f"unknown walker_kind: {sys.walker_kind}")
 
 

@@ -93,7 +93,8 @@ PRECISIONS = ("fp64", "fp32", "bf16x3")
 
 def make_chol_afqmc_ops(ham_data: ham_chol, walker_kind: str,
                         precision: str = "fp64") -> chol_afqmc_ops:
-    """prop/chol_afqmc_ops.py:215-262 (restricted basis, restricted/unrestricted walkers)."""
+    """prop/chol_afqmc_ops.py:215-262 (restricted basis; restricted, unrestricted or
+    generalized walkers)."""
     if precision not in PRECISIONS:
         raise ValueError(f"unknown precision {precision!r}; expected one of {PRECISIONS}")
     walker_kind = walker_kind.lower()
@@ -101,10 +102,11 @@ def make_chol_afqmc_ops(ham_data: ham_chol, walker_kind: str,
     n = int(ham_data.chol.shape[1])
     if ham_data.basis == "generalized" and walker_kind != "generalized":
         raise ValueError("ham_data.basis='generalized' requires walker_kind='generalized'")
-    if ham_data.basis == "generalized" or walker_kind == "generalized":
-        raise NotImplementedError("generalized walkers / basis are not implemented on the HIP path")
-    if walker_kind not in ("restricted", "unrestricted"):
+    if ham_data.basis == "generalized":
+        raise NotImplementedError("generalized-basis Hamiltonians are not implemented on the HIP path")
+    if walker_kind not in ("restricted", "unrestricted", "generalized"):
         raise ValueError(f"unknown walker_kind: {walker_kind}")
+    generalized = walker_kind == "generalized"
 
     chol_t = chol_transposed(ham_data.chol)
     ldv = kernels.ldv_for(n)
@@ -140,6 +142,15 @@ def make_chol_afqmc_ops(ham_data: ham_chol, walker_kind: str,
 
     def apply_trotter_planes(w, xbar: torch.Tensor, ctx: chol_afqmc_ctx, n_terms: int,
                              n_chunks: int = 1):
+        if generalized:
+            # _apply_trotter_g_from_restricted (prop/chol_afqmc_ops.py:200-212): the
+            # spin blocks of each (2n, o') walker with the walker's one VHS
+            top, bot = apply_spin_blocks((w[:, :n, :].contiguous(), w[:, n:, :].contiguous()), xbar,
+                                         ctx, n_terms, n_chunks)
+            return torch.cat([top, bot], dim=1)
+        return apply_spin_blocks(w, xbar, ctx, n_terms, n_chunks)
+
+    def apply_spin_blocks(w, xbar: torch.Tensor, ctx: chol_afqmc_ctx, n_terms: int, n_chunks: int):
         unr = isinstance(w, tuple)
         w0 = w[0] if unr else w
         nw = w0.shape[0]

@@ -107,6 +107,37 @@ def log_overlap(walkers, trial_data) -> torch.Tensor:
     return greens(walkers, trial_data).log_overlap
 
 
+def block_trial(trial_data) -> torch.Tensor:
+    """diag(C, C) (2n, 2o): the RHF trial in the spin-orbital basis."""
+    c = trial_data.mo_coeff
+    n, o = c.shape
+    out = torch.zeros((2 * n, 2 * o), dtype=c.dtype, device=c.device)
+    out[:n, :o] = c
+    out[n:, o:] = c
+    return out
+
+
+def greens_g(walkers, trial_data) -> greens_cache:
+    """Generalized walkers (nw, 2n, 2o) against an RHF trial (trial/rhf.py:47-53,
+    meas/rhf.py:108-130): with the trial diag(C, C) the overlap matrix is the
+    stacked [C^T w_top; C^T w_bot] and G = (w m^-1)^T is (2o, 2n).  The overlap
+    is det m (not squared): logdet is carried as (nw, 2) = [log det m, 0], so
+    the spin sum of log_overlap and of the phaseless update gives log det m."""
+    if not trial_data.restricted:
+        raise ValueError("generalized walkers need an RHF trial")
+    g, ld = kernels.overlap_greens(walkers, block_trial(trial_data))
+    return greens_cache(g, torch.cat([ld, torch.zeros_like(ld)], dim=1))
+
+
+def log_overlap_g(walkers, trial_data) -> torch.Tensor:
+    return greens_g(walkers, trial_data).log_overlap
+
+
+def overlap_g(walkers, trial_data) -> torch.Tensor:
+    """det([C^T w_top; C^T w_bot]) per walker (trial/rhf.py:47-53)."""
+    return torch.exp(log_overlap_g(walkers, trial_data))
+
+
 def overlap_r(walkers, trial_data) -> torch.Tensor:
     """det(C^T w)^2 per walker (trial/rhf.py:35-37)."""
     return torch.exp(log_overlap(walkers, trial_data))
@@ -118,7 +149,7 @@ def overlap_u(walkers, trial_data) -> torch.Tensor:
 
 
 def make_rhf_trial_ops(sys: system) -> trial_ops:
-    """trial/rhf.py:56-71.  Generalized walkers are not supported by the HIP path."""
+    """trial/rhf.py:56-71."""
     if sys.nup != sys.ndn:
         raise ValueError("RHF requires nelec[0] == nelec[1].")
     wk = sys.walker_kind.lower()
@@ -127,7 +158,7 @@ def make_rhf_trial_ops(sys: system) -> trial_ops:
     if wk == "unrestricted":
         return trial_ops(overlap=overlap_u, get_rdm1=get_rdm1, greens=greens, log_overlap=log_overlap)
     if wk == "generalized":
-        raise NotImplementedError("generalized walkers are not implemented on the HIP path")
+        return trial_ops(overlap=overlap_g, get_rdm1=get_rdm1, greens=greens_g, log_overlap=log_overlap_g)
     raise ValueError(f"unknown walker_kind: {sys.walker_kind}")
 
 

@@ -1,6 +1,7 @@
 """Walker container operations (walkers.py:13-265 of the reference), batched
-on the HIP kernels.  Walkers are complex128 tensors (nw, n, o), or a tuple of
-two for unrestricted walkers."""
+on the HIP kernels.  Walkers are complex128 tensors (nw, n, o) (restricted),
+(nw, 2n, nup + ndn) (generalized, spin-orbital rows), or a tuple of two
+(nw, n, o) tensors (unrestricted)."""
 from __future__ import annotations
 
 from typing import Any, Callable
@@ -19,19 +20,31 @@ def _natorbs(dm: torch.Tensor, n_occ: int) -> torch.Tensor:
 
 
 def init_walkers(sys: system, rdm1: torch.Tensor, n_walkers: int):
-    """walkers.py:19-77 (restricted / unrestricted): natural orbitals of the
-    trial rdm1, identical for every walker."""
+    """walkers.py:19-77: natural orbitals of the trial rdm1, identical for every
+    walker.  Generalized walkers (2n, nup + ndn): from a (2n, 2n) rdm1, or from
+    the spin blocks as [[up, 0], [0, dn]]."""
     wk = sys.walker_kind.lower()
-    if wk == "generalized":
-        raise NotImplementedError("generalized walkers are not implemented on the HIP path")
-    if rdm1.ndim == 2:
-        raise ValueError("For walker_kind in {'restricted','unrestricted'}, rdm1 must be spin-block (2, norb, norb).")
-    if rdm1.ndim != 3 or rdm1.shape[0] != 2:
-        raise ValueError("Expected rdm1 with shape (2, norb, norb).")
 
     def bcast(c):
         c = c.to(torch.complex128)
         return c.unsqueeze(0).expand(n_walkers, *c.shape).contiguous()
+
+    if wk == "generalized":
+        n, nup, ndn = sys.norb, sys.nup, sys.ndn
+        if rdm1.ndim == 2:
+            if tuple(rdm1.shape) != (2 * n, 2 * n):
+                raise ValueError("For generalized walkers, a 2D rdm1 must have shape (2*norb, 2*norb).")
+            return bcast(_natorbs(rdm1, nup + ndn))
+        if rdm1.ndim != 3 or rdm1.shape[0] != 2:
+            raise ValueError("Expected rdm1 with shape (2, norb, norb) for generalized init from spin blocks.")
+        w0 = torch.zeros((2 * n, nup + ndn), dtype=torch.complex128, device=rdm1.device)
+        w0[:n, :nup] = _natorbs(rdm1[0], nup)
+        w0[n:, nup:] = _natorbs(rdm1[1], ndn)
+        return bcast(w0)
+    if rdm1.ndim == 2:
+        raise ValueError("For walker_kind in {'restricted','unrestricted'}, rdm1 must be spin-block (2, norb, norb).")
+    if rdm1.ndim != 3 or rdm1.shape[0] != 2:
+        raise ValueError("Expected rdm1 with shape (2, norb, norb).")
 
     if wk == "restricted":
         return bcast(_natorbs(rdm1[0] + rdm1[1], sys.nup))
@@ -86,10 +99,8 @@ def orthonormalize(w, walker_kind: str):
     wk = walker_kind.lower()
     if wk == "unrestricted":
         return (kernels.orthonormalize(w[0]), kernels.orthonormalize(w[1]))
-    if wk == "restricted":
+    if wk in ("restricted", "generalized"):  # one (n, o) or (2n, 2o) matrix per walker
         return kernels.orthonormalize(w)
-    if wk == "generalized":
-        raise NotImplementedError("generalized walkers are not implemented on the HIP path")
     raise ValueError(f"unknown walker_kind: {walker_kind}")
 
 
@@ -102,7 +113,7 @@ def stochastic_reconfiguration_with_indices(w, weights: torch.Tensor, zeta, walk
     """Comb resampling with the selected indices; zeta a float or a 0-d float64
     device tensor."""
     wk = walker_kind.lower()
-    if wk not in ("restricted", "unrestricted"):
+    if wk not in ("restricted", "unrestricted", "generalized"):
         raise ValueError(f"unknown walker_kind: {walker_kind}")
     idx, w_new = kernels.comb_indices(weights, zeta)
     if wk == "unrestricted":

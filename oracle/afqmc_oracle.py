@@ -78,12 +78,23 @@ def overlap(w, trial, kind: str, mode: str = "raw") -> np.ndarray:
         if mode == "log":
             return logdet(mu) + logdet(md)
         return det(mu) * det(md)
+    if kind == "generalized":
+        m = overlap_matrix_g(cu, w)
+        return logdet(m) if mode == "log" else det(m)
     raise ValueError(f"unknown walker_kind: {kind}")
 
 
-def half_green(w: np.ndarray, c: np.ndarray) -> np.ndarray:
-    """_half_green_from_overlap_matrix (meas/rhf.py:15-19): solve(m^T, w^T) -> (nw, o, n)."""
-    m = overlap_matrix(c, w)
+def overlap_matrix_g(c: np.ndarray, w: np.ndarray) -> np.ndarray:
+    """overlap_g (trial/rhf.py:47-53, meas/rhf.py:108-114): generalized walkers
+    (nw, 2n, 2o) against an RHF trial: m = [C^H w_top; C^H w_bot] (2o x 2o)."""
+    n = c.shape[0]
+    return np.concatenate([overlap_matrix(c, w[:, :n, :]), overlap_matrix(c, w[:, n:, :])], axis=1)
+
+
+def half_green(w: np.ndarray, c: np.ndarray, kind: str = "restricted") -> np.ndarray:
+    """_half_green_from_overlap_matrix (meas/rhf.py:15-19): solve(m^T, w^T) -> (nw, o, n);
+    generalized walkers: (nw, 2o, 2n) from the stacked overlap matrix."""
+    m = overlap_matrix_g(c, w) if kind == "generalized" else overlap_matrix(c, w)
     return np.linalg.solve(np.swapaxes(m, 1, 2), np.swapaxes(w, 1, 2))
 
 
@@ -120,6 +131,12 @@ def force_bias(w, trial, mctx: meas_ctx, kind: str) -> np.ndarray:
     if kind == "restricted":
         g = half_green(w, cu)
         return 2.0 * (g.reshape(g.shape[0], -1) @ mctx.rot_chol_flat[0].T)
+    if kind == "generalized":
+        # force_bias_kernel_g (meas/rhf.py:117-130): rot_chol_flat @ vec(G_upup + G_dndn)
+        n, o = cu.shape
+        g = half_green(w, cu, kind)
+        gs = g[:, :o, :n] + g[:, o:, n:]
+        return gs.reshape(gs.shape[0], -1) @ mctx.rot_chol_flat[0].T
     wu, wd = w
     gu = half_green(wu, cu)
     gd = half_green(wd, cd)
@@ -145,6 +162,8 @@ def energy(w, h0: float, trial, mctx: meas_ctx, kind: str) -> np.ndarray:
     """energy_kernel_r (meas/rhf.py:36-50) and energy_kernel_u (meas/rhf.py:75-105),
     the latter with per-spin half-rotated integrals (UHF generalisation)."""
     cu, cd = trial
+    if kind == "generalized":
+        raise KeyError("generalized walkers have no energy kernel (meas/rhf.py:169-174)")
     if kind == "restricted":
         g = half_green(w, cu)
         e1 = 2.0 * np.einsum("wij,ij->w", g, mctx.rot_h1[0])
@@ -230,6 +249,12 @@ def apply_trotter(w, fields: np.ndarray, pctx: prop_ctx, chol: np.ndarray,
         w1 = e @ w
         w2 = taylor_expm_action(a, vhs, w1, n_terms)
         return e @ w2
+    if kind == "generalized":
+        # _apply_trotter_g_from_restricted (prop/chol_afqmc_ops.py:118-124,157-170,200-212):
+        # both spin blocks of the spin-orbital walker with the same E and VHS
+        n = e.shape[0]
+        halves = [e @ taylor_expm_action(a, vhs, e @ w[:, s * n:(s + 1) * n, :], n_terms) for s in (0, 1)]
+        return np.concatenate(halves, axis=1)
     wu, wd = w
     return (e @ taylor_expm_action(a, vhs, e @ wu, n_terms),
             e @ taylor_expm_action(a, vhs, e @ wd, n_terms))
@@ -288,7 +313,14 @@ def natorbs(dm: np.ndarray, n_occ: int) -> np.ndarray:
 
 
 def init_walkers(rdm1: np.ndarray, nocc: int, nw: int, kind: str):
-    """init_walkers (walkers.py:19-77), restricted / unrestricted."""
+    """init_walkers (walkers.py:19-77), restricted / unrestricted / generalized
+    (from the spin blocks: [[up, 0], [0, dn]], walkers.py:27-49)."""
+    if kind == "generalized":
+        n = rdm1.shape[1]
+        top = np.concatenate([natorbs(rdm1[0], nocc), np.zeros((n, nocc))], axis=1)
+        bot = np.concatenate([np.zeros((n, nocc)), natorbs(rdm1[1], nocc)], axis=1)
+        w0 = np.concatenate([top, bot], axis=0) + 0.0j
+        return np.broadcast_to(w0, (nw,) + w0.shape).copy()
     if kind == "restricted":
         w0 = natorbs(rdm1[0] + rdm1[1], nocc) + 0.0j
         return np.broadcast_to(w0, (nw,) + w0.shape).copy()
@@ -298,14 +330,18 @@ def init_walkers(rdm1: np.ndarray, nocc: int, nw: int, kind: str):
             np.broadcast_to(wd, (nw,) + wd.shape).copy())
 
 
-def init_state(sd: system_data, nw: int, overlap_mode: str = "raw", walkers=None) -> state:
-    """init_prop_state (prop/afqmc.py:17-71) without the PRNG key."""
+def init_state(sd: system_data, nw: int, overlap_mode: str = "raw", walkers=None,
+               e_estimate: float | None = None) -> state:
+    """init_prop_state (prop/afqmc.py:17-71) without the PRNG key; e_estimate
+    given directly skips the energy (generalized walkers have none)."""
     nocc = sd.trial[0].shape[1]
     if walkers is None:
         walkers = init_walkers(get_rdm1(sd.trial), nocc, nw, sd.kind)
     ov = overlap(walkers, sd.trial, sd.kind, overlap_mode)
-    e = np.real(energy(walkers, sd.h0, sd.trial, sd.mctx, sd.kind))
-    e_est = float(np.mean(e))
+    if e_estimate is None:
+        e_est = float(np.mean(np.real(energy(walkers, sd.h0, sd.trial, sd.mctx, sd.kind))))
+    else:
+        e_est = float(e_estimate)
     return state(walkers, np.ones(nw), ov, e_est, e_est)
 
 
@@ -378,7 +414,8 @@ def qr_phase_fixed(mat: np.ndarray):
 
 
 def orthonormalize(w, kind: str):
-    """orthonormalize / orthogonalize (walkers.py:183-211)."""
+    """orthonormalize / orthogonalize (walkers.py:183-211); generalized walkers
+    are one (2n x 2o) matrix, as restricted ones."""
     if kind == "unrestricted":
         return (qr_phase_fixed(w[0])[0], qr_phase_fixed(w[1])[0])
     return qr_phase_fixed(w)[0]
@@ -465,7 +502,7 @@ def run_cadenced(st: state, sd: system_data, prm: params, *, n_steps: int, qr_ev
 # --------------------------------------------------------------------------
 
 def _ov_single(w, trial, kind):
-    if kind == "restricted":
+    if kind in ("restricted", "generalized"):
         return overlap(w[None], trial, kind)[0]
     return overlap((w[0][None], w[1][None]), trial, kind)[0]
 
@@ -473,11 +510,14 @@ def _ov_single(w, trial, kind):
 def _apply_single(mat, w, kind):
     if kind == "restricted":
         return mat @ w
+    if kind == "generalized":  # a spatial one-body operator acts on both spin blocks
+        n = mat.shape[0]
+        return np.concatenate([mat @ w[:n], mat @ w[n:]], axis=0)
     return (mat @ w[0], mat @ w[1])
 
 
 def _axpy(w, a, v, kind):
-    if kind == "restricted":
+    if kind in ("restricted", "generalized"):
         return w + a * v
     return (w[0] + a * v[0], w[1] + a * v[1])
 
