@@ -132,12 +132,40 @@ class _layouts:
         self.h0_prop = float(h0_prop)
         self.rot = rot.contiguous()
         self.rot_flat = self.rot.reshape(g, o * n)
+        self.rot_flat_t = self.rot_flat.T.contiguous()       # (o*n, naux): B operand of G_bar
+        self.chol_flat = self.chol.reshape(g, n * n)          # B operand of xbar_bar
         opad = (o + 15) // 16 * 16
         self.rot_pad = torch.zeros((1, g, opad, ldv), dtype=f64, device=chol.device)
         self.rot_pad[0, :, :o, :n] = rot
         self.rot_h1 = rot_h1.contiguous()
         self.rot_h1_k = self.rot_h1.reshape(1, o, n)
         self.trial = run.trial.contiguous()
+
+
+def _dgemm(A, B, C, alpha: float = 1.0, beta: float = 0.0):
+    """C = alpha A B^T + beta C on the library's fp64 MFMA GEMM, with K split
+    into independent slices (partial products summed afterwards) when the
+    128 x 128 tiles of C alone would leave the GPU mostly idle."""
+    M, K = A.shape
+    N = B.shape[0]
+    tiles = -(-M // 128) * -(-N // 128)
+    splits = 1
+    while tiles * splits < 512 and K // (2 * splits) >= 64:
+        splits *= 2
+    if splits == 1:
+        return kernels.dgemm_nt(A, B, C, alpha=alpha, beta=beta)
+    part = torch.empty((splits, M, N), dtype=f64, device=A.device)
+    step = -(-K // splits)
+    st = kernels._stream(A)
+    for s in range(splits):
+        k0, k1 = s * step, min(K, (s + 1) * step)
+        kernels._call("afq_dgemm_nt", A, M, N, k1 - k0, 1.0, A.data_ptr() + 8 * k0, K,
+                      B.data_ptr() + 8 * k0, K, 0.0, part[s].data_ptr(), N, st)
+    if beta == 0.0:
+        torch.mul(part.sum(0), alpha, out=C)
+    else:
+        C.mul_(beta).add_(part.sum(0), alpha=alpha)
+    return C
 
 
 def _gc(planes, nw, o, n):
@@ -328,13 +356,16 @@ def _step_adj(lay: _layouts, run: adjoint_run, tp, adj, e_est: float, acc):
     # Trotter
     vhs_c = torch.complex(tp["vhs"][0, :, :, :n], tp["vhs"][1, :, :, :n])
     w_bar_in, vhs_bar = _trotter_adj(lay, run, tp["w"], vhs_c, w_bar, acc)
-    # VHS = sum_g xbar_g L_g
-    lflat = lay.chol.reshape(g, n * n)
-    vr = vhs_bar.real.reshape(nw, n * n)
-    vi = vhs_bar.imag.reshape(nw, n * n)
-    xbar_bar = torch.complex(vr @ lflat.T, vi @ lflat.T)
+    # VHS = sum_g xbar_g L_g, on the library's fp64 MFMA GEMM (C = alpha A B^T + beta C):
+    # xbar_bar = VHS_bar . L (real and imaginary planes as rows), and
+    # L_bar += Re(xbar)^T Re(VHS_bar) + Im(xbar)^T Im(VHS_bar) as one GEMM over K = 2 nw
+    v2 = torch.stack([vhs_bar.real, vhs_bar.imag]).reshape(2 * nw, n * n)
+    xbb = torch.empty((2 * nw, g), dtype=f64, device=v2.device)
+    _dgemm(v2, lay.chol_flat, xbb)
+    xbar_bar = torch.complex(xbb[:nw], xbb[nw:])
     xbar_c = torch.complex(tp["xbar"][0], tp["xbar"][1])
-    acc["chol"] += (xbar_c.real.T @ vr + xbar_c.imag.T @ vi).view(g, n, n)
+    _dgemm(tp["xbar"].reshape(2 * nw, g).T.contiguous(), v2.T.contiguous(),
+           acc["chol"].view(g, n * n), alpha=1.0, beta=1.0)
     # field shifts: fs = -sqrt_dt (i fb - mf), xbar = x - fs
     mf = lay.mf
     xbar_bar = xbar_bar + mf.conj()[None, :] * shift_bar[:, None]
@@ -343,12 +374,14 @@ def _step_adj(lay: _layouts, run: adjoint_run, tp, adj, e_est: float, acc):
     fb_bar = 1.0j * sd * fs_bar
     mf_bar = mf_bar + sd * fs_bar.sum(0)
     acc["m"] += mf_bar.imag
-    # force bias fb = 2 rot . vec(G)
+    # force bias fb = 2 rot . vec(G): G_bar = 2 fb_bar . rot, rot_bar += 2 Re(fb_bar^H G)
     g_in = _gc(tp["g"], nw, o, n)
-    rflat = lay.rot_flat
-    g_bar = 2.0 * torch.complex(fb_bar.real @ rflat, fb_bar.imag @ rflat).view(nw, o, n)
-    acc["rot"] += 2.0 * (fb_bar.real.T @ g_in.real.reshape(nw, o * n)
-                         + fb_bar.imag.T @ g_in.imag.reshape(nw, o * n)).view(g, o, n)
+    fb2 = torch.stack([fb_bar.real, fb_bar.imag]).reshape(2 * nw, g)
+    gb = torch.empty((2 * nw, o * n), dtype=f64, device=fb2.device)
+    _dgemm(fb2, lay.rot_flat_t, gb, alpha=2.0)
+    g_bar = torch.complex(gb[:nw], gb[nw:]).view(nw, o, n)
+    _dgemm(fb2.T.contiguous(), tp["g"].reshape(2 * nw, o * n).T.contiguous(),
+           acc["rot"].view(g, o * n), alpha=2.0, beta=1.0)
     w_bar_in = w_bar_in + _greens_adj(tp["w"], lay.trial, g_c=g_in, g_bar=g_bar)
     return (w_bar_in, wt_bar_in, lo_bar_in, pop_bar_in)
 

@@ -324,6 +324,53 @@ def batched_run(device, n_steps: int, n_systems: int = 64, chol_scale: float = 0
     return out
 
 
+def adjoint_run_bench(device):
+    """d<E>/d lambda and d<E>/dL of the 96-orbital workload (16 occupied, 480
+    vectors, 512 walkers, dt 0.01, 50 steps in blocks of 10 = one checkpoint per
+    10 steps, seed 3, V = 0.01 sym(N(0,1))): wall time of the energy + gradient
+    (forward on the HIP kernels, hand-written reverse pass) and of a forward run
+    alone, and the agreement with central differences at lambda = +-1e-4."""
+    import numpy as np
+    import torch
+
+    from ad_afqmc_prototype_amd import synthetic as syn
+    from ad_afqmc_prototype_amd.prop import adjoint as adj
+
+    cfg = syn.WORKLOADS["n96_o16_g480_w512_rhf"]
+    s = syn.make_system(cfg["seed"], cfg["norb"], cfg["nocc"], cfg["naux"], chol_scale=cfg["chol_scale"])
+    n_blocks, n_prop = 5, 10
+    fields, zetas = adj.device_random_numbers(cfg["seed"], n_blocks, n_prop, cfg["n_walkers"], cfg["naux"],
+                                              device)
+    run = adj.adjoint_run(trial=torch.tensor(s.mo_up, device=device), h0=float(s.h0), dt=cfg["dt"],
+                          n_walkers=cfg["n_walkers"], n_blocks=n_blocks, n_prop_steps=n_prop,
+                          fields=fields, zetas=zetas)
+    rng = np.random.default_rng(cfg["seed"] + 1000)
+    m = rng.standard_normal((cfg["norb"], cfg["norb"]))
+    v = torch.tensor(0.01 * 0.5 * (m + m.T), device=device)
+    h1 = torch.tensor(s.h1, device=device)
+    chol = torch.tensor(s.chol, device=device)
+    adj.energy_gradients(h1, chol, run, v)  # warm-up
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    out = adj.energy_gradients(h1, chol, run, v)
+    torch.cuda.synchronize()
+    t_grad = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    jp = adj.run_blocks(h1 + 1e-4 * v, chol, run)
+    torch.cuda.synchronize()
+    t_fwd = time.perf_counter() - t0
+    jm = adj.run_blocks(h1 - 1e-4 * v, chol, run)
+    same = all(bool((a == b).all()) for a, b in zip(jp[3], jm[3]))
+    fd = (jp[0] - jm[0]) / 2e-4
+    return {"workload": "96 orbitals, 16 occupied, 480 Cholesky vectors, 512 walkers, dt 0.01, "
+                        "50 steps (5 blocks of 10, one checkpoint each), fp64",
+            "energy_Ha": out["energy"], "d_energy_d_lambda": out["d_lambda"],
+            "central_difference": fd, "rel_diff": abs(out["d_lambda"] - fd) / abs(fd),
+            "comb_picks_identical_at_pm_lambda": same,
+            "s_energy_and_gradient": round(t_grad, 3), "s_forward_only": round(t_fwd, 3),
+            "gradient_cost_in_forward_runs": round(t_grad / t_fwd, 2)}
+
+
 def measured_traffic(workload: str, kernel: str):
     """HBM bytes per launch of ``kernel`` from the committed rocprofv3 FETCH_SIZE /
     WRITE_SIZE summaries (profiles/), each scaled by its copy-kernel
@@ -428,6 +475,8 @@ def main():
                          "0 0 skips it")
     ap.add_argument("--batched-steps", type=int, default=100,
                     help="steps of the 64-system uneven batch for the batched field; 0 skips it")
+    ap.add_argument("--no-adjoint", action="store_true",
+                    help="skip the adjoint field (gradient of the 96-orbital run vs central differences)")
     ap.add_argument("--no-cadence", action="store_true",
                     help="time afqmc_step alone (no QR / measurement / reconfiguration)")
     args = ap.parse_args()
@@ -519,6 +568,8 @@ def main():
         out["end_to_end"] = e2e
     if args.batched_steps > 0:
         out["batched"] = batched_run(device, args.batched_steps)
+    if not args.no_adjoint:
+        out["adjoint"] = adjoint_run_bench(device)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(args.workload)
     print(json.dumps(out), flush=True)
