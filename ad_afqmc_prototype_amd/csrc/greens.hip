@@ -197,6 +197,217 @@ __device__ __forceinline__ void greens_body(int n, int o, const double2* __restr
     }
 }
 
+// ---------------------------------------------------------------------------
+// One wave per (walker, spin), four per workgroup, for o <= 32: the same
+// arithmetic as greens_body in the same order (m by the same MFMA sequence,
+// the same LU updates, log det summed sequentially over k, the same row
+// solves), with the LU driven by one wave in its own LDS region: no workgroup
+// barriers, only the wave's LDS ordering (s_waitcnt lgkmcnt(0)).
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void wave_lds_sync() { asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory"); }
+
+template <int MAXO>
+struct greens_wave_lds {
+    double2 m[MAXO][MAXO + 1];
+    double2 dinv[MAXO];
+    double2 logs[MAXO];  // (log |U_kk|, arg U_kk)
+    int perm[MAXO];
+};
+
+template <int MAXO>
+__device__ __forceinline__ void greens_wave_body(int n, int o, const double2* __restrict__ W,
+                                                 const double* __restrict__ Cm, double* __restrict__ gre,
+                                                 double* __restrict__ gim, double2* __restrict__ ld,
+                                                 greens_wave_lds<MAXO>& S) {
+    constexpr int NT = (MAXO + 15) / 16;
+    const int lane = threadIdx.x & 63;
+    const int lr = lane & 15;
+    const int lk = (lane >> 4) * 4;
+
+    // 1. m = C^T w: every 16 x 16 tile of m by this wave, the next k-block's
+    //    operands loaded before the current block's MFMAs
+    {
+        const int nkb = (n + 15) >> 4;
+        f64x4 acc[NT][NT][2];
+#pragma unroll
+        for (int ti = 0; ti < NT; ++ti)
+#pragma unroll
+            for (int tj = 0; tj < NT; ++tj) acc[ti][tj][0] = acc[ti][tj][1] = f64x4{0.0, 0.0, 0.0, 0.0};
+        double a[NT][4], an[NT][4];
+        double2 x[NT][4], xn[NT][4];
+        auto load = [&](int kb, double (&av)[NT][4], double2 (&xv)[NT][4]) {
+#pragma unroll
+            for (int kk = 0; kk < 4; ++kk) {
+                const int p = kb * 16 + lk + kk;
+                const bool okp = p < n;
+#pragma unroll
+                for (int t = 0; t < NT; ++t) {
+                    const int c = t * 16 + lr;
+                    av[t][kk] = (okp && c < o) ? Cm[p * o + c] : 0.0;
+                    xv[t][kk] = (okp && c < o) ? W[p * o + c] : make_double2(0.0, 0.0);
+                }
+            }
+        };
+        load(0, a, x);
+        for (int kb = 0; kb < nkb; ++kb) {
+            load(kb + 1, an, xn);  // past the end: all masked to zero
+#pragma unroll
+            for (int kk = 0; kk < 4; ++kk)
+#pragma unroll
+                for (int ti = 0; ti < NT; ++ti)
+#pragma unroll
+                    for (int tj = 0; tj < NT; ++tj) {
+                        acc[ti][tj][0] = afq::mfma_f64(a[ti][kk], x[tj][kk].x, acc[ti][tj][0]);
+                        acc[ti][tj][1] = afq::mfma_f64(a[ti][kk], x[tj][kk].y, acc[ti][tj][1]);
+                    }
+#pragma unroll
+            for (int t = 0; t < NT; ++t)
+#pragma unroll
+                for (int kk = 0; kk < 4; ++kk) {
+                    a[t][kk] = an[t][kk];
+                    x[t][kk] = xn[t][kk];
+                }
+        }
+#pragma unroll
+        for (int ti = 0; ti < NT; ++ti)
+#pragma unroll
+            for (int tj = 0; tj < NT; ++tj)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const int ii = ti * 16 + afq::mfma16<double>::row(lane, r);
+                    const int j = tj * 16 + lr;
+                    if (ii < o && j < o) S.m[ii][j] = make_double2(acc[ti][tj][0][r], acc[ti][tj][1][r]);
+                }
+    }
+    if (lane < o) S.perm[lane] = lane;
+    wave_lds_sync();
+
+    // 2. LU with partial pivoting (|re| + |im|, first index on ties)
+    int nswap = 0;
+    for (int k = 0; k < o; ++k) {
+        double best = -1.0;
+        int bi = k;
+        for (int r = k + lane; r < o; r += 64) {
+            const double v = fabs(S.m[r][k].x) + fabs(S.m[r][k].y);
+            if (v > best) {
+                best = v;
+                bi = r;
+            }
+        }
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) {
+            const double ob = __shfl_xor(best, off, 64);
+            const int oi = __shfl_xor(bi, off, 64);
+            if (ob > best || (ob == best && oi < bi)) {
+                best = ob;
+                bi = oi;
+            }
+        }
+        const int piv = __builtin_amdgcn_readfirstlane(bi);
+        if (piv != k) {
+            for (int c = lane; c < o; c += 64) {
+                const double2 t = S.m[k][c];
+                S.m[k][c] = S.m[piv][c];
+                S.m[piv][c] = t;
+            }
+            if (lane == 0) {
+                const int t = S.perm[k];
+                S.perm[k] = S.perm[piv];
+                S.perm[piv] = t;
+            }
+            ++nswap;
+        }
+        wave_lds_sync();
+        const double2 inv = crecip(S.m[k][k]);
+        for (int r = k + 1 + lane; r < o; r += 64) S.m[r][k] = cmul(S.m[r][k], inv);
+        wave_lds_sync();
+        const int rem = o - k - 1;
+        for (int e = lane; e < rem * rem; e += 64) {
+            const int r = k + 1 + e / rem;
+            const int c = k + 1 + e % rem;
+            const double2 t = cmul(S.m[r][k], S.m[k][c]);
+            S.m[r][c].x -= t.x;
+            S.m[r][c].y -= t.y;
+        }
+        wave_lds_sync();
+    }
+
+    // 3. log det (summed in k order) and pivot reciprocals
+    if (lane < o) {
+        const double2 d = S.m[lane][lane];
+        S.dinv[lane] = crecip(d);
+        S.logs[lane] = make_double2(log(hypot(d.x, d.y)), atan2(d.y, d.x));
+    }
+    wave_lds_sync();
+    if (lane == 0) {
+        double lre = 0.0, lim = 0.0;
+        for (int k = 0; k < o; ++k) {
+            lre += S.logs[k].x;
+            lim += S.logs[k].y;
+        }
+        if (nswap & 1) lim += M_PI;
+        *ld = make_double2(lre, lim);
+    }
+
+    // 4. row solves x m = w_p, one row per lane at a time
+    for (int p = lane; p < n; p += 64) {
+        asm volatile("" ::: "memory");
+        double2 xr[MAXO];
+#pragma unroll
+        for (int j = 0; j < MAXO; ++j) xr[j] = (j < o) ? W[p * o + j] : make_double2(0.0, 0.0);
+#pragma unroll
+        for (int j = 0; j < MAXO; ++j) {
+            if (j < o) {
+                xr[j] = cmul(xr[j], S.dinv[j]);
+#pragma unroll
+                for (int c = j + 1; c < MAXO; ++c) {
+                    if (c < o) {
+                        const double2 t = cmul(xr[j], S.m[j][c]);
+                        xr[c].x -= t.x;
+                        xr[c].y -= t.y;
+                    }
+                }
+            }
+        }
+#pragma unroll
+        for (int j = MAXO - 1; j >= 0; --j) {
+            if (j < o) {
+#pragma unroll
+                for (int c = 0; c < j; ++c) {
+                    const double2 t = cmul(xr[j], S.m[j][c]);
+                    xr[c].x -= t.x;
+                    xr[c].y -= t.y;
+                }
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < MAXO; ++k) {
+            if (k < o) {
+                const int i = S.perm[k];
+                gre[(long long)i * n + p] = xr[k].x;
+                gim[(long long)i * n + p] = xr[k].y;
+            }
+        }
+    }
+}
+
+template <int MAXO>
+__global__ __launch_bounds__(THREADS) void greens_wave_kernel(
+    int nw, int n, int o, int nspin, const double2* __restrict__ wup,
+    const double2* __restrict__ wdn, const double* __restrict__ cup, const double* __restrict__ cdn,
+    double* __restrict__ greens, double2* __restrict__ logdet) {
+    __shared__ greens_wave_lds<MAXO> L[THREADS / 64];
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int item = blockIdx.x * (THREADS / 64) + wave;
+    if (item >= nw * nspin) return;
+    const int w = item / nspin;
+    const int s = item - w * nspin;
+    const long long ldg = (long long)nspin * o * n;
+    double* gre = greens + (long long)w * ldg + (long long)s * o * n;
+    greens_wave_body<MAXO>(n, o, (s == 0 ? wup : wdn) + (long long)w * n * o, s == 0 ? cup : cdn, gre,
+                           gre + (long long)nw * ldg, logdet + (long long)w * nspin + s, L[wave]);
+}
+
 template <int MAXO>
 __global__ __launch_bounds__(THREADS) void greens_kernel(
     int nw, int n, int o, int nspin, const double2* __restrict__ wup,
@@ -211,19 +422,22 @@ __global__ __launch_bounds__(THREADS) void greens_kernel(
                       gre + (long long)nw * ldg, logdet + (long long)w * nspin + s);
 }
 
-// One workgroup per walker of every system: Green's function and log det of
-// the system's walkers_out.
+// One wave per walker of every system (four per workgroup): Green's function
+// and log det of the system's walkers_out.
 template <int MAXO>
-__global__ __launch_bounds__(THREADS) void greens_grouped_kernel(int count,
+__global__ __launch_bounds__(THREADS) void greens_grouped_kernel(int count, int total,
                                                                  const afq_system_step* __restrict__ S) {
-    const int b = blockIdx.x;
+    __shared__ greens_wave_lds<MAXO> L[THREADS / 64];
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int b = blockIdx.x * (THREADS / 64) + wave;
+    if (b >= total) return;
     const int si = afq::find_group(count, b, [&](int i) { return S[i].walker0; });
     const afq_system_step& d = S[si];
     const int w = b - d.walker0;
     const int n = d.n, o = d.o;
     double* gre = d.greens + (long long)w * o * n;
-    greens_body<MAXO>(n, o, (const double2*)d.walkers_out + (long long)w * n * o, d.trial, gre,
-                      gre + (long long)d.nw * o * n, (double2*)d.logdet + w);
+    greens_wave_body<MAXO>(n, o, (const double2*)d.walkers_out + (long long)w * n * o, d.trial, gre,
+                           gre + (long long)d.nw * o * n, (double2*)d.logdet + w, L[wave]);
 }
 
 }  // namespace
@@ -243,12 +457,13 @@ extern "C" int afq_overlap_greens(int nw, int n, int o, int nspin, const double*
     const double2* wd = (const double2*)(nspin == 2 ? walkers_dn : walkers_up);
     const double* cd = nspin == 2 ? trial_dn : trial_up;
     double2* ld = (double2*)logdet;
+    const dim3 wgrid((unsigned)afq::ceil_div(nw * nspin, THREADS / 64));
     if (o <= 8)
-        hipLaunchKernelGGL(greens_kernel<8>, grid, dim3(THREADS), 0, st, nw, n, o, nspin, wu, wd, trial_up, cd, greens, ld);
+        hipLaunchKernelGGL(greens_wave_kernel<8>, wgrid, dim3(THREADS), 0, st, nw, n, o, nspin, wu, wd, trial_up, cd, greens, ld);
     else if (o <= 16)
-        hipLaunchKernelGGL(greens_kernel<16>, grid, dim3(THREADS), 0, st, nw, n, o, nspin, wu, wd, trial_up, cd, greens, ld);
+        hipLaunchKernelGGL(greens_wave_kernel<16>, wgrid, dim3(THREADS), 0, st, nw, n, o, nspin, wu, wd, trial_up, cd, greens, ld);
     else if (o <= 32)
-        hipLaunchKernelGGL(greens_kernel<32>, grid, dim3(THREADS), 0, st, nw, n, o, nspin, wu, wd, trial_up, cd, greens, ld);
+        hipLaunchKernelGGL(greens_wave_kernel<32>, wgrid, dim3(THREADS), 0, st, nw, n, o, nspin, wu, wd, trial_up, cd, greens, ld);
     else
         hipLaunchKernelGGL(greens_kernel<64>, grid, dim3(THREADS), 0, st, nw, n, o, nspin, wu, wd, trial_up, cd, greens, ld);
     return afq::check_launch("afq_overlap_greens");
@@ -260,13 +475,13 @@ extern "C" int afq_overlap_greens_grouped(int count, const afq_system_step* sys,
                 count, total_walkers);
     AFQ_REQUIRE(sys, "afq_overlap_greens_grouped: null pointer");
     AFQ_REQUIRE(max_o > 0 && max_o <= 32, "afq_overlap_greens_grouped: max_o=%d not in 1..32", max_o);
-    const dim3 grid((unsigned)total_walkers);
+    const dim3 grid((unsigned)afq::ceil_div(total_walkers, THREADS / 64));
     hipStream_t st = (hipStream_t)stream;
     if (max_o <= 8)
-        hipLaunchKernelGGL(greens_grouped_kernel<8>, grid, dim3(THREADS), 0, st, count, sys);
+        hipLaunchKernelGGL(greens_grouped_kernel<8>, grid, dim3(THREADS), 0, st, count, total_walkers, sys);
     else if (max_o <= 16)
-        hipLaunchKernelGGL(greens_grouped_kernel<16>, grid, dim3(THREADS), 0, st, count, sys);
+        hipLaunchKernelGGL(greens_grouped_kernel<16>, grid, dim3(THREADS), 0, st, count, total_walkers, sys);
     else
-        hipLaunchKernelGGL(greens_grouped_kernel<32>, grid, dim3(THREADS), 0, st, count, sys);
+        hipLaunchKernelGGL(greens_grouped_kernel<32>, grid, dim3(THREADS), 0, st, count, total_walkers, sys);
     return afq::check_launch("afq_overlap_greens_grouped");
 }
