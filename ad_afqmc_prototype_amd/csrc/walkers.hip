@@ -105,6 +105,174 @@ __global__ __launch_bounds__(THREADS) void qr_grouped_kernel(int count,
 }
 
 // ---------------------------------------------------------------------------
+// Reduced QR by two Cholesky-QR passes (CholQR2) for o <= 32, one wave per
+// walker, four per workgroup: W^H W = R^H R (Gram matrix on fp64 MFMA,
+// Cholesky in the wave's LDS, R upper triangular with a real positive
+// diagonal), Q = W R^-1 row by row; the second pass on Q restores
+// orthogonality to rounding.  The Q of the diag(R) > 0 convention is unique,
+// so this is the Q of walkers.py:169-180 to rounding for walkers far from
+// rank deficiency (condition number << 1e8; walkers are re-orthonormalised
+// every few steps).  One pass per launch (the second in place: each lane reads
+// its rows before writing them, and the Gram matrix is formed before any row
+// is written).
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void wave_lds_sync() { asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory"); }
+
+template <int MAXO>
+struct cholqr_lds {
+    double2 g[MAXO][MAXO + 1];
+    double rinv[MAXO];
+};
+
+template <int MAXO>
+__device__ __forceinline__ void cholqr_pass(int n, int o, const double2* W, double2* Q, cholqr_lds<MAXO>& S) {
+    constexpr int NT = (MAXO + 15) / 16;
+    const int lane = threadIdx.x & 63;
+    const int lr = lane & 15;
+    const int lk = (lane >> 4) * 4;
+    // 1. Gram matrix g = W^H W
+    {
+        const int nkb = (n + 15) >> 4;
+        f64x4 acc[NT][NT][2];
+#pragma unroll
+        for (int ti = 0; ti < NT; ++ti)
+#pragma unroll
+            for (int tj = 0; tj < NT; ++tj) acc[ti][tj][0] = acc[ti][tj][1] = f64x4{0.0, 0.0, 0.0, 0.0};
+        double2 x[NT][4], xn[NT][4];
+        auto load = [&](int kb, double2 (&xv)[NT][4]) {
+#pragma unroll
+            for (int kk = 0; kk < 4; ++kk) {
+                const int p = kb * 16 + lk + kk;
+#pragma unroll
+                for (int t = 0; t < NT; ++t) {
+                    const int c = t * 16 + lr;
+                    xv[t][kk] = (p < n && c < o) ? W[p * o + c] : make_double2(0.0, 0.0);
+                }
+            }
+        };
+        load(0, x);
+        for (int kb = 0; kb < nkb; ++kb) {
+            load(kb + 1, xn);
+#pragma unroll
+            for (int kk = 0; kk < 4; ++kk)
+#pragma unroll
+                for (int ti = 0; ti < NT; ++ti)
+#pragma unroll
+                    for (int tj = 0; tj < NT; ++tj) {
+                        const double2 a = x[ti][kk], b = x[tj][kk];
+                        acc[ti][tj][0] = afq::mfma_f64(a.x, b.x, acc[ti][tj][0]);
+                        acc[ti][tj][0] = afq::mfma_f64(a.y, b.y, acc[ti][tj][0]);
+                        acc[ti][tj][1] = afq::mfma_f64(a.x, b.y, acc[ti][tj][1]);
+                        acc[ti][tj][1] = afq::mfma_f64(-a.y, b.x, acc[ti][tj][1]);
+                    }
+#pragma unroll
+            for (int t = 0; t < NT; ++t)
+#pragma unroll
+                for (int kk = 0; kk < 4; ++kk) x[t][kk] = xn[t][kk];
+        }
+#pragma unroll
+        for (int ti = 0; ti < NT; ++ti)
+#pragma unroll
+            for (int tj = 0; tj < NT; ++tj)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const int i = ti * 16 + afq::mfma16<double>::row(lane, r);
+                    const int j = tj * 16 + lr;
+                    if (i < o && j < o) S.g[i][j] = make_double2(acc[ti][tj][0][r], acc[ti][tj][1][r]);
+                }
+    }
+    wave_lds_sync();
+    // 2. Cholesky g = R^H R, R over the upper triangle (right-looking)
+    for (int k = 0; k < o; ++k) {
+        const double d = sqrt(S.g[k][k].x);
+        const double dinv = 1.0 / d;
+        for (int j = k + lane; j < o; j += 64) {
+            const double2 v = S.g[k][j];
+            S.g[k][j] = (j == k) ? make_double2(d, 0.0) : make_double2(v.x * dinv, v.y * dinv);
+        }
+        if (lane == 0) S.rinv[k] = dinv;
+        wave_lds_sync();
+        const int rem = o - k - 1;
+        for (int e = lane; e < rem * rem; e += 64) {
+            const int i = k + 1 + e / rem;
+            const int j = k + 1 + e % rem;
+            if (j >= i) {
+                const double2 a = S.g[k][i], b = S.g[k][j];  // conj(a) b
+                S.g[i][j].x -= a.x * b.x + a.y * b.y;
+                S.g[i][j].y -= a.x * b.y - a.y * b.x;
+            }
+        }
+        wave_lds_sync();
+    }
+    // 3. rows: x R = w_p
+    for (int p = lane; p < n; p += 64) {
+        asm volatile("" ::: "memory");
+        double2 xr[MAXO];
+#pragma unroll
+        for (int j = 0; j < MAXO; ++j) xr[j] = (j < o) ? W[p * o + j] : make_double2(0.0, 0.0);
+#pragma unroll
+        for (int i = 0; i < MAXO; ++i) {
+            if (i < o) {
+                const double ri = S.rinv[i];
+                xr[i].x *= ri;
+                xr[i].y *= ri;
+#pragma unroll
+                for (int j = i + 1; j < MAXO; ++j) {
+                    if (j < o) {
+                        const double2 r = S.g[i][j];
+                        xr[j].x -= xr[i].x * r.x - xr[i].y * r.y;
+                        xr[j].y -= xr[i].x * r.y + xr[i].y * r.x;
+                    }
+                }
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < MAXO; ++j)
+            if (j < o) Q[p * o + j] = xr[j];
+    }
+}
+
+template <int MAXO>
+__global__ __launch_bounds__(THREADS) void cholqr_kernel(int nw, int n, int o, const double2* src,
+                                                         double2* dst) {  // src may equal dst
+    __shared__ cholqr_lds<MAXO> L[THREADS / 64];
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int w = blockIdx.x * (THREADS / 64) + wave;
+    if (w >= nw) return;
+    const long long off = (long long)w * n * o;
+    cholqr_pass<MAXO>(n, o, src + off, dst + off, L[wave]);
+}
+
+// second == 0: walkers_in -> walkers_out; second == 1: walkers_out in place
+template <int MAXO>
+__global__ __launch_bounds__(THREADS) void cholqr_grouped_kernel(int count, int total, int second,
+                                                                 const afq_system_step* __restrict__ S) {
+    __shared__ cholqr_lds<MAXO> L[THREADS / 64];
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int b = blockIdx.x * (THREADS / 64) + wave;
+    if (b >= total) return;
+    const int si = afq::find_group(count, b, [&](int i) { return S[i].walker0; });
+    const afq_system_step& d = S[si];
+    const long long off = (long long)(b - d.walker0) * d.n * d.o;
+    const double2* src = (second ? (const double2*)d.walkers_out : (const double2*)d.walkers_in) + off;
+    cholqr_pass<MAXO>(d.n, d.o, src, (double2*)d.walkers_out + off, L[wave]);
+}
+
+template <int MAXO>
+void launch_cholqr(int nw, int n, int o, const double2* win, double2* wout, hipStream_t st) {
+    const dim3 grid((unsigned)afq::ceil_div(nw, THREADS / 64));
+    hipLaunchKernelGGL(cholqr_kernel<MAXO>, grid, dim3(THREADS), 0, st, nw, n, o, win, wout);
+    hipLaunchKernelGGL(cholqr_kernel<MAXO>, grid, dim3(THREADS), 0, st, nw, n, o, (const double2*)wout, wout);
+}
+
+template <int MAXO>
+void launch_cholqr_grouped(int count, int total, const afq_system_step* sys, hipStream_t st) {
+    const dim3 grid((unsigned)afq::ceil_div(total, THREADS / 64));
+    hipLaunchKernelGGL(cholqr_grouped_kernel<MAXO>, grid, dim3(THREADS), 0, st, count, total, 0, sys);
+    hipLaunchKernelGGL(cholqr_grouped_kernel<MAXO>, grid, dim3(THREADS), 0, st, count, total, 1, sys);
+}
+
+// ---------------------------------------------------------------------------
 // Comb indices.  The cumulative sum is one sequential left-to-right fp64 pass
 // (numpy.cumsum order) and z_k = (tot * (k + zeta)) / nw is evaluated without
 // contraction, so the indices are reproducible bit for bit.
@@ -168,6 +336,16 @@ extern "C" int afq_orthonormalize(int nw, int n, int o, const double* win, doubl
     AFQ_REQUIRE(nw > 0 && n > 0 && o > 0 && o <= n, "afq_orthonormalize: bad sizes nw=%d n=%d o=%d", nw, n, o);
     AFQ_REQUIRE(o <= 64, "afq_orthonormalize: o=%d exceeds the supported maximum of 64", o);
     AFQ_REQUIRE(win && wout && win != wout, "afq_orthonormalize: null or aliased pointer");
+    hipStream_t st = (hipStream_t)stream;
+    if (o <= 32) {
+        if (o <= 8)
+            launch_cholqr<8>(nw, n, o, (const double2*)win, (double2*)wout, st);
+        else if (o <= 16)
+            launch_cholqr<16>(nw, n, o, (const double2*)win, (double2*)wout, st);
+        else
+            launch_cholqr<32>(nw, n, o, (const double2*)win, (double2*)wout, st);
+        return afq::check_launch("afq_orthonormalize");
+    }
     const size_t lds = (size_t)n * (o + 1) * sizeof(double2);
     AFQ_REQUIRE(lds + 64 * sizeof(double2) + 64 <= 160 * 1024,
                 "afq_orthonormalize: walker n=%d o=%d does not fit in LDS", n, o);
@@ -215,6 +393,16 @@ extern "C" int afq_orthonormalize_grouped(int count, const afq_system_step* sys,
     AFQ_REQUIRE(sys, "afq_orthonormalize_grouped: null pointer");
     AFQ_REQUIRE(max_o > 0 && max_o <= 64 && max_n >= max_o, "afq_orthonormalize_grouped: max_n=%d max_o=%d",
                 max_n, max_o);
+    if (max_o <= 32) {
+        hipStream_t st = (hipStream_t)stream;
+        if (max_o <= 8)
+            launch_cholqr_grouped<8>(count, total_walkers, sys, st);
+        else if (max_o <= 16)
+            launch_cholqr_grouped<16>(count, total_walkers, sys, st);
+        else
+            launch_cholqr_grouped<32>(count, total_walkers, sys, st);
+        return afq::check_launch("afq_orthonormalize_grouped");
+    }
     const size_t lds = (size_t)max_n * (max_o + 1) * sizeof(double2);
     AFQ_REQUIRE(lds + 64 * sizeof(double2) + 64 <= 160 * 1024,
                 "afq_orthonormalize_grouped: max_n=%d max_o=%d needs more than 160 KB of LDS",
