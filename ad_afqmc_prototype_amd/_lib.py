@@ -35,6 +35,7 @@ _SIGNATURES = {
     "afq_force_bias": [c_int, c_int, c_int, c_double, c_vp, c_vp, c_vp, c_vp],
     "afq_field_shifts": [c_int, c_int, c_double, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp],
     "afq_vhs": [c_int, c_int, c_int, c_int, c_vp, c_vp, c_vp, c_vp],
+    "afq_vhs_sym": [c_int, c_int, c_int, c_int, c_int, c_vp, c_vp, c_ll, c_vp, c_vp, c_ll, c_vp, c_vp],
     "afq_trotter": [c_int, c_int, c_int, c_int, c_int, c_int, c_double, c_vp, c_vp, c_vp, c_vp,
                     c_vp, c_vp, c_vp],
     "afq_phaseless_update": [c_int, c_int, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_double, c_double,
@@ -70,7 +71,7 @@ _SIGNATURES = {
     "afq_unpack_pairs": [c_int, c_int, c_vp, c_ll, c_vp, c_vp],
 }
 
-ABI_VERSION = 4
+ABI_VERSION = 5
 
 
 class HipLibraryError(RuntimeError):

@@ -101,6 +101,65 @@ __device__ __forceinline__ double wave_sum(double v) {
     return v;
 }
 
+// ---------------------------------------------------------------------------
+// Symmetric VHS epilogue.  The VHS GEMM can run on the upper 16 x 16 blocks of
+// the n x n plane only (V = sum_g x_g L_g is symmetric): its column
+// c = blk * 256 + pi * 16 + qi is entry (bi * 16 + pi, bj * 16 + qi) of block
+// blk = (bi, bj), bi <= bj, numbered row by row.  The GEMM then stores every
+// entry twice into the full row-major (n, ldv) matrix of its output row: at
+// (p, q) and, off the diagonal blocks, at (q, p).
+struct sym_dst {
+    int n, ldv, nb;  // nb = ceil(n / 16)
+};
+
+// (bi, bj) of upper block blk (uniform per tile: a short scalar loop)
+__device__ __forceinline__ void sym_block(int blk, int nb, int& bi, int& bj) {
+    int b = blk, i = 0;
+    while (b >= nb - i) {
+        b -= nb - i;
+        ++i;
+    }
+    bi = i;
+    bj = i + b;
+}
+
+// One lane's values v[j] = entry (bi*16 + pi0 + j, bj*16 + qi), j < NPI, of one
+// output row (row points at that row's n x ldv matrix).  The mirror entries
+// (bj*16 + qi, bi*16 + pi0 + j) are consecutive and go out in 16-byte
+// stores: NPI * sizeof(T) is a multiple of 16, pi0 a multiple of 16 / sizeof(T),
+// and ldv and the row base are 16-byte aligned.  Entries past n in the stored
+// blocks are zero (zero operand rows), so the padding columns n..ldv-1 get
+// zeros from the direct stores; mirror rows past n are skipped.
+template <int NPI, typename T>
+__device__ __forceinline__ void sym_store(T* __restrict__ row, const sym_dst& d, int bi, int bj, int pi0, int qi,
+                                          const T (&v)[NPI]) {
+    const int q = bj * 16 + qi;
+    if (q < d.ldv) {
+#pragma unroll
+        for (int j = 0; j < NPI; ++j) {
+            const int p = bi * 16 + pi0 + j;
+            if (p < d.n) row[(long long)p * d.ldv + q] = v[j];
+        }
+    }
+    if (bi != bj && q < d.n) {
+        constexpr int W = 16 / sizeof(T);
+        typedef T vec_t __attribute__((ext_vector_type(W)));
+        vec_t* dst = reinterpret_cast<vec_t*>(row + (long long)q * d.ldv + bi * 16 + pi0);
+#pragma unroll
+        for (int c = 0; c < NPI / W; ++c) {
+            vec_t x;
+#pragma unroll
+            for (int e = 0; e < W; ++e) x[e] = v[c * W + e];
+            dst[c] = x;
+        }
+    }
+}
+
+// symmetric-VHS GEMM in fp64 / fp32 (defined with the GEMMs)
+template <typename T>
+int gemm_nt_sym(const char* name, int M, int K, const T* A, long long lda, const T* B, long long ldb,
+                sym_dst d, T* C, hipStream_t stream);
+
 }  // namespace afq
 
 #define AFQ_REQUIRE(cond, ...)         \

@@ -149,11 +149,41 @@ __device__ __forceinline__ void gemm_store(int M, int N, T alpha, T beta, TOut* 
     }
 }
 
-template <typename T, typename TOut>
+// The store of a tile whose columns are symmetric-VHS block entries
+// (afq::sym_store): a 128-column tile lies inside one 256-column block, and a
+// wave's TN column tiles are TN consecutive block rows pi.
+template <typename T, int TM, int TN>
+__device__ __forceinline__ void gemm_store_sym(int M, T alpha, T* __restrict__ C, const afq::sym_dst& d,
+                                               int m0, int n0,
+                                               const typename tile_cfg<T, TM, TN>::acc_t (&acc)[TM][TN]) {
+    using MF = afq::mfma16<T>;
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    const int wm = wave >> 1;
+    const int wn = wave & 1;
+    int bi, bj;
+    afq::sym_block(n0 >> 8, d.nb, bi, bj);
+    const int pi0 = ((n0 & 255) >> 4) + wn * TN;
+    const long long ldc = (long long)d.n * d.ldv;
+#pragma unroll
+    for (int i = 0; i < TM; ++i) {
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int row = m0 + wm * 16 * TM + i * 16 + MF::row(lane, r);
+            if (row >= M) continue;
+            T v[TN];
+#pragma unroll
+            for (int j = 0; j < TN; ++j) v[j] = alpha * acc[i][j][r];
+            afq::sym_store<TN>(C + row * ldc, d, bi, bj, pi0, lane & 15, v);
+        }
+    }
+}
+
+template <typename T, typename TOut, bool SYM = false>
 __global__ __launch_bounds__(THREADS, 2) void gemm_nt_kernel(
     int M, int N, int K, T alpha, const T* __restrict__ A, long long lda,
     const T* __restrict__ B, long long ldb, T beta, TOut* __restrict__ C,
-    long long ldc, int tiles_m, int tiles_n) {
+    long long ldc, int tiles_m, int tiles_n, afq::sym_dst sd = afq::sym_dst{0, 0, 0}) {
     using CF = tile_cfg<T, 4, 4>;
     __shared__ T As[2 * CF::BM * PITCH];
     __shared__ T Bs[2 * CF::BN * PITCH];
@@ -170,7 +200,10 @@ __global__ __launch_bounds__(THREADS, 2) void gemm_nt_kernel(
 
     typename CF::acc_t acc[4][4];
     gemm_nt_tile<T, 4, 4>(M, N, K, A, lda, B, ldb, m0, n0, As, Bs, acc);
-    gemm_store<T, TOut, 4, 4>(M, N, alpha, beta, C, ldc, m0, n0, acc);
+    if constexpr (SYM)
+        gemm_store_sym<T, 4, 4>(M, alpha, (T*)C, sd, m0, n0, acc);
+    else
+        gemm_store<T, TOut, 4, 4>(M, N, alpha, beta, C, ldc, m0, n0, acc);
 }
 
 // Grouped fp32 GEMM: workgroup -> (problem, tile) by the problems' tile_start
@@ -220,6 +253,26 @@ int gemm_nt(const char* name, int M, int N, int K, T alpha, const T* A, long lon
                        alpha, A, lda, B, ldb, beta, C, ldc, tiles_m, tiles_n);
     return check_launch(name);
 }
+
+// symmetric-VHS GEMM (alpha = 1): M rows of A (M x K) against the upper-block
+// rows of B (N = blocks * 256 rows), stored expanded into C (M x n x ldv)
+template <typename T>
+int gemm_nt_sym(const char* name, int M, int K, const T* A, long long lda, const T* B, long long ldb,
+                sym_dst d, T* C, hipStream_t stream) {
+    const int N = d.nb * (d.nb + 1) / 2 * 256;
+    const int tiles_m = ceil_div(M, tile_cfg<T, 4, 4>::BM);
+    const int tiles_n = ceil_div(N, tile_cfg<T, 4, 4>::BN);
+    const long long total = (long long)tiles_m * tiles_n;
+    AFQ_REQUIRE(total < (1LL << 31), "%s: problem too large", name);
+    hipLaunchKernelGGL((gemm_nt_kernel<T, T, true>), dim3((unsigned)total), dim3(THREADS), 0, stream, M, N, K,
+                       T(1), A, lda, B, ldb, T(0), C, 0LL, tiles_m, tiles_n, d);
+    return check_launch(name);
+}
+
+template int gemm_nt_sym<double>(const char*, int, int, const double*, long long, const double*, long long,
+                                 sym_dst, double*, hipStream_t);
+template int gemm_nt_sym<float>(const char*, int, int, const float*, long long, const float*, long long,
+                                sym_dst, float*, hipStream_t);
 
 int dgemm_nt(int M, int N, int K, double alpha, const double* A, long long lda, const double* B,
              long long ldb, double beta, double* C, long long ldc, hipStream_t stream) {

@@ -38,12 +38,14 @@ __device__ __forceinline__ double bf16_to_f64(uint16_t h) {
     return (double)__uint_as_float(((uint32_t)h) << 16);
 }
 
-template <typename TOut>
+// SYM: the columns are symmetric-VHS block entries and the tile is stored
+// expanded (afq::sym_store); C is then float, ldc unused.
+template <typename TOut, bool SYM = false>
 __global__ __launch_bounds__(THREADS, 2) void gemm_bf16x3_nt_kernel(
     int M, int N, int K, float alpha, const uint16_t* __restrict__ Ahi,
     const uint16_t* __restrict__ Alo, long long lda, const uint16_t* __restrict__ Bhi,
     const uint16_t* __restrict__ Blo, long long ldb, TOut* __restrict__ C, long long ldc,
-    int tiles_m, int tiles_n) {
+    int tiles_m, int tiles_n, afq::sym_dst sd = afq::sym_dst{0, 0, 0}) {
     __shared__ __attribute__((aligned(16))) uint16_t S[2][4][BM * PITCH];
 
     const int total = tiles_m * tiles_n;
@@ -137,6 +139,26 @@ __global__ __launch_bounds__(THREADS, 2) void gemm_bf16x3_nt_kernel(
         __syncthreads();
     }
 
+    if constexpr (SYM) {
+        // a 128-column tile lies inside one block; the wave's 4 column tiles
+        // are 4 consecutive block rows
+        int bi, bj;
+        afq::sym_block(n0 >> 8, sd.nb, bi, bj);
+        const int pi0 = ((n0 & 255) >> 4) + wn * 4;
+        const long long ldf = (long long)sd.n * sd.ldv;
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int row = m0 + wm * 64 + i * 16 + (lane >> 4) * 4 + r;
+                if (row >= M) continue;
+                float v[4];
+#pragma unroll
+                for (int j = 0; j < 4; ++j) v[j] = alpha * acc[i][j][r];
+                afq::sym_store<4>((float*)C + row * ldf, sd, bi, bj, pi0, lr, v);
+            }
+        return;
+    }
 #pragma unroll
     for (int i = 0; i < 4; ++i)
 #pragma unroll
@@ -173,12 +195,12 @@ __device__ __forceinline__ rsrc_t make_rsrc(const void* p, unsigned long long by
                                             (int)(bytes > 0x7fffffffull ? 0x7fffffffull : bytes), 0x00020000);
 }
 
-template <typename TOut>
+template <typename TOut, bool SYM = false>
 __global__ __launch_bounds__(THREADS2, 1) void gemm_bf16x3_nt_256_kernel(
     int M, int N, int K, float alpha, const uint16_t* __restrict__ Ahi,
     const uint16_t* __restrict__ Alo, long long lda, const uint16_t* __restrict__ Bhi,
     const uint16_t* __restrict__ Blo, long long ldb, TOut* __restrict__ C, long long ldc,
-    int tiles_m, int tiles_n) {
+    int tiles_m, int tiles_n, afq::sym_dst sd = afq::sym_dst{0, 0, 0}) {
     // [stage][plane: A hi, A lo, B hi, B lo][256 rows x 64 bytes]
     __shared__ __attribute__((aligned(1024))) uint4 S[2][4][BM2 * 4];
 
@@ -274,6 +296,51 @@ __global__ __launch_bounds__(THREADS2, 1) void gemm_bf16x3_nt_256_kernel(
         __syncthreads();
     }
 
+    if constexpr (SYM) {
+        // The 256-column tile is one block (bi, bj).  It leaves through LDS in
+        // four chunks of 64 rows, so that every store writes whole 64-byte row
+        // segments: the 16 entries of a block row at (16 bi + pi, 16 bj ..) and,
+        // off the diagonal, the 16 entries of a block column at
+        // (16 bj + qi, 16 bi ..).  Chunk layout: row c at c * RP floats, block
+        // row pi at pi * PP (both padded: the fills and the column reads hit 64
+        // distinct banks).
+        constexpr int RP = 324, PP = 20;
+        float* L = reinterpret_cast<float*>(&S[0][0][0]);
+        float* Cf = reinterpret_cast<float*>(C);
+        int bi, bj;
+        afq::sym_block(n0 >> 8, sd.nb, bi, bj);
+        const long long ldf = (long long)sd.n * sd.ldv;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+#pragma unroll
+            for (int j = 0; j < 8; ++j)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) L[(wm * 16 + q * 4 + r) * RP + (wn * 8 + j) * PP + lr] = alpha * acc[i][j][r];
+            __syncthreads();
+#pragma unroll
+            for (int s = 0; s < 8; ++s) {
+                const int u = tid + s * THREADS2;
+                const int c = u >> 6;        // chunk row: wm = c / 16, 4 q + r = c % 16
+                const int a = (u >> 2) & 15;  // block row (direct) / block column (mirror)
+                const int b = u & 3;          // 4-entry group of the 64-byte segment
+                const int row = m0 + (c >> 4) * 64 + i * 16 + (c & 15);
+                if (row >= M) continue;
+                float* dst = Cf + row * ldf;
+                const int p = bi * 16 + a, qd = bj * 16 + 4 * b;
+                if (p < sd.n && qd < sd.ldv)
+                    *reinterpret_cast<float4*>(dst + (long long)p * sd.ldv + qd) =
+                        *reinterpret_cast<const float4*>(&L[c * RP + a * PP + 4 * b]);
+                const int qm = bj * 16 + a;
+                if (bi != bj && qm < sd.n) {
+                    const float* col = &L[c * RP + 4 * b * PP + a];
+                    *reinterpret_cast<float4*>(dst + (long long)qm * sd.ldv + bi * 16 + 4 * b) =
+                        make_float4(col[0], col[PP], col[2 * PP], col[3 * PP]);
+                }
+            }
+            __syncthreads();
+        }
+        return;
+    }
 #pragma unroll
     for (int i = 0; i < 4; ++i)
 #pragma unroll
@@ -360,6 +427,46 @@ extern "C" int afq_gemm_bf16x3_nt(int M, int N, int K, float alpha, const unsign
                            M, N, K, alpha, A_hi, A_lo, lda, B_hi, B_lo, ldb, (float*)C, ldc,
                            tiles_m, tiles_n);
     return afq::check_launch("afq_gemm_bf16x3_nt");
+}
+
+extern "C" int afq_vhs_sym(int rows, int n, int ldv, int k, int arith, const void* a_hi, const void* a_lo,
+                           long long lda, const void* b_hi, const void* b_lo, long long ldb, void* vhs,
+                           void* stream) {
+    AFQ_REQUIRE(rows > 0 && n > 0 && k > 0, "afq_vhs_sym: bad sizes rows=%d n=%d k=%d", rows, n, k);
+    AFQ_REQUIRE(ldv >= n && ldv % 4 == 0, "afq_vhs_sym: ldv=%d must be >= n=%d and a multiple of 4", ldv, n);
+    AFQ_REQUIRE(arith >= 0 && arith <= 2, "afq_vhs_sym: arith=%d not in 0..2", arith);
+    AFQ_REQUIRE(a_hi && b_hi && vhs && (arith != 2 || (a_lo && b_lo)), "afq_vhs_sym: null pointer");
+    AFQ_REQUIRE(((uintptr_t)vhs & 15) == 0, "afq_vhs_sym: vhs must be 16-byte aligned");
+    AFQ_REQUIRE(lda >= k && ldb >= k, "afq_vhs_sym: leading dimension too small");
+    const int nb = afq::ceil_div(n, 16);
+    AFQ_REQUIRE((long long)nb * (nb + 1) / 2 * 256 < (1LL << 31), "afq_vhs_sym: n=%d too large", n);
+    const afq::sym_dst d{n, ldv, nb};
+    hipStream_t st = (hipStream_t)stream;
+    if (arith == 0)
+        return afq::gemm_nt_sym<double>("afq_vhs_sym", rows, k, (const double*)a_hi, lda, (const double*)b_hi, ldb,
+                                        d, (double*)vhs, st);
+    if (arith == 1)
+        return afq::gemm_nt_sym<float>("afq_vhs_sym", rows, k, (const float*)a_hi, lda, (const float*)b_hi, ldb,
+                                       d, (float*)vhs, st);
+    AFQ_REQUIRE(k % 8 == 0 && lda % 8 == 0 && ldb % 8 == 0,
+                "afq_vhs_sym: bf16x3 needs k=%d, lda and ldb multiples of 8", k);
+    const int N = nb * (nb + 1) / 2 * 256;
+    const uint16_t *ah = (const uint16_t*)a_hi, *al = (const uint16_t*)a_lo;
+    const uint16_t *bh = (const uint16_t*)b_hi, *bl = (const uint16_t*)b_lo;
+    const int tiles_m2 = afq::ceil_div(rows, BM2);
+    const int tiles_n2 = N / BN2;
+    if ((long long)tiles_m2 * tiles_n2 >= 512) {
+        hipLaunchKernelGGL((gemm_bf16x3_nt_256_kernel<float, true>), dim3((unsigned)(tiles_m2 * tiles_n2)),
+                           dim3(THREADS2), 0, st, rows, N, k, 1.0f, ah, al, lda, bh, bl, ldb, (float*)vhs, 0LL,
+                           tiles_m2, tiles_n2, d);
+    } else {
+        const int tiles_m = afq::ceil_div(rows, BM);
+        const int tiles_n = N / BN;
+        hipLaunchKernelGGL((gemm_bf16x3_nt_kernel<float, true>), dim3((unsigned)(tiles_m * tiles_n)),
+                           dim3(THREADS), 0, st, rows, N, k, 1.0f, ah, al, lda, bh, bl, ldb, (float*)vhs, 0LL,
+                           tiles_m, tiles_n, d);
+    }
+    return afq::check_launch("afq_vhs_sym");
 }
 
 extern "C" int afq_split_bf16(int rows, int cols, const double* src, long long ld_src,

@@ -104,11 +104,13 @@ def _check(t: torch.Tensor, dtype, shape, name: str, device=None):
 
 
 def ldv_for(n: int) -> int:
-    """Row pitch of VHS / exp_h1_half / rotated Cholesky rows (32-byte rows)."""
-    return (n + 3) // 4 * 4
+    """Row pitch of VHS / exp_h1_half / rotated Cholesky rows: a multiple of 16
+    entries, so fp32 rows start on 64-byte boundaries and the 64-byte row
+    segments the VHS contraction stores are whole memory sectors."""
+    return (n + 15) // 16 * 16
 
 
-def dgemm_nt(A, B, C, alpha=1.0, beta=0.0):
+def dgemm_nt(A, B, C, alpha=1.0, beta=0.0, tag=None):
     """C = alpha * A @ B.T + beta * C (fp64 MFMA)."""
     M, K = A.shape
     N = B.shape[0]
@@ -116,8 +118,44 @@ def dgemm_nt(A, B, C, alpha=1.0, beta=0.0):
     _check(B, f64, (N, K), "B", A.device)
     _check(C, f64, (M, N), "C", A.device)
     _call("afq_dgemm_nt", A, M, N, K, float(alpha), A.data_ptr(), K, B.data_ptr(), K, float(beta),
-              C.data_ptr(), N, _stream(A))
+              C.data_ptr(), N, _stream(A), tag=tag)
     return C
+
+
+def sym_blocks(n: int) -> int:
+    """Number of upper 16 x 16 blocks of an n x n symmetric matrix."""
+    nb = (n + 15) // 16
+    return nb * (nb + 1) // 2
+
+
+def vhs_sym(a, b, n, out=None, tag="vhs"):
+    """Symmetric VHS contraction: full VHS rows (rows, n, ldv) from shifted-field
+    rows ``a`` (rows, k) and the upper-block Cholesky rows ``b``
+    (sym_blocks(n) * 256, k).  ``a``, ``b`` float64 (fp64, float64 out),
+    float32 (fp32, float32 out) or (hi, lo) bf16 plane pairs from split_bf16
+    (bf16x3, float32 out)."""
+    if isinstance(a, tuple):
+        (ah, al), (bh, bl) = a, b
+        arith, out_dtype = 2, f32
+        rows, k = ah.shape
+        for t, nm, sh in ((ah, "a_hi", (rows, k)), (al, "a_lo", (rows, k)),
+                          (bh, "b_hi", (sym_blocks(n) * 256, k)), (bl, "b_lo", (sym_blocks(n) * 256, k))):
+            _check(t, torch.int16, sh, nm, ah.device)
+    else:
+        ah, al, bh, bl = a, None, b, None
+        if a.dtype not in (f64, f32):
+            raise TypeError("vhs_sym: float64, float32 or bf16 plane pairs")
+        arith, out_dtype = (0, f64) if a.dtype == f64 else (1, f32)
+        rows, k = a.shape
+        _check(a, a.dtype, (rows, k), "a")
+        _check(b, a.dtype, (sym_blocks(n) * 256, k), "b", a.device)
+    ldv = ldv_for(n)
+    if out is None:
+        out = torch.empty((rows, n, ldv), dtype=out_dtype, device=ah.device)
+    _check(out, out_dtype, (rows, n, ldv), "out", ah.device)
+    _call("afq_vhs_sym", ah, rows, n, ldv, k, arith, ah.data_ptr(), _ptr(al), k, bh.data_ptr(), _ptr(bl), k,
+          out.data_ptr(), _stream(ah), tag=tag)
+    return out
 
 
 def sgemm_nt(A, B, C, alpha=1.0, beta=0.0, tag=None):

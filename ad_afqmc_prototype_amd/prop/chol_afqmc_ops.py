@@ -81,6 +81,23 @@ def chol_transposed(chol: torch.Tensor) -> torch.Tensor:
     return pad.reshape(g, n * ldv).T.contiguous()
 
 
+def chol_sym_rows(chol: torch.Tensor) -> torch.Tensor:
+    """The rows of L^T for the upper 16 x 16 blocks of the (p, q) plane only:
+    (sym_blocks(n) * 256, n_fields), block (bi, bj), bi <= bj, in row-major
+    order, element (pi, qi) at pi * 16 + qi, zero rows past n.  The B operand
+    of the symmetric VHS contraction (V = sum_g x_g L_g is symmetric)."""
+    g, n, _ = chol.shape
+    nb = (n + 15) // 16
+    bi, bj = torch.triu_indices(nb, nb, device=chol.device)
+    ar = torch.arange(16, device=chol.device)
+    p = (bi[:, None, None] * 16 + ar[None, :, None]).expand(-1, 16, 16).reshape(-1)
+    q = (bj[:, None, None] * 16 + ar[None, None, :]).expand(-1, 16, 16).reshape(-1)
+    out = torch.zeros((p.numel(), g), dtype=torch.float64, device=chol.device)
+    ok = (p < n) & (q < n)
+    out[ok] = chol[:, p[ok], q[ok]].T
+    return out
+
+
 def _chunks(nw: int, n_chunks: int):
     if nw % n_chunks != 0:
         raise ValueError(f"n_walkers={nw} is not divisible into n_chunks={n_chunks}")
@@ -108,31 +125,31 @@ def make_chol_afqmc_ops(ham_data: ham_chol, walker_kind: str,
         raise ValueError(f"unknown walker_kind: {walker_kind}")
     generalized = walker_kind == "generalized"
 
-    chol_t = chol_transposed(ham_data.chol)
+    # the VHS contraction against the upper blocks only, each entry stored twice
+    chol_s = chol_sym_rows(ham_data.chol)
     ldv = kernels.ldv_for(n)
     if precision == "fp32":
-        chol_t32 = kernels.convert_f32(chol_t)
-        del chol_t
+        chol_s32 = kernels.convert_f32(chol_s)
+        del chol_s
     elif precision == "bf16x3":
-        chol_hi, chol_lo = kernels.split_bf16(chol_t)
-        del chol_t
+        chol_hi, chol_lo = kernels.split_bf16(chol_s)
+        del chol_s
 
     def n_fields_() -> int:
         return nf
 
     def make_vhs(xbar: torch.Tensor) -> torch.Tensor:
-        """VHS planes (2, nw, n, ldv) in the step's precision."""
-        if precision == "fp64":
-            return kernels.vhs(xbar, chol_t, n)
+        """VHS planes (2, nw, n, ldv) in the step's precision, from the
+        symmetric contraction (upper 16 x 16 blocks, stored expanded)."""
         nwc = xbar.shape[1]
-        out = torch.empty((2, nwc, n, ldv), dtype=torch.float32, device=xbar.device)
         flat = xbar.reshape(2 * nwc, nf)
-        if precision == "fp32":
-            kernels.sgemm_nt(kernels.convert_f32(flat), chol_t32, out.view(2 * nwc, n * ldv), tag="vhs")
+        if precision == "fp64":
+            out = kernels.vhs_sym(flat, chol_s, n)
+        elif precision == "fp32":
+            out = kernels.vhs_sym(kernels.convert_f32(flat), chol_s32, n)
         else:
-            xh, xl = kernels.split_bf16(flat)
-            kernels.gemm_bf16x3_nt(xh, xl, chol_hi, chol_lo, out.view(2 * nwc, n * ldv), tag="vhs")
-        return out
+            out = kernels.vhs_sym(kernels.split_bf16(flat), (chol_hi, chol_lo), n)
+        return out.view(2, nwc, n, ldv)
 
     def e_half(ctx: chol_afqmc_ctx) -> torch.Tensor:
         return ctx.exp_h1_half_pad if precision == "fp64" else ctx.exp_h1_half_pad32

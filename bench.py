@@ -27,9 +27,10 @@ PEAK_HBM_GBS = 8000.0
 
 def flops_per_walker_step(n, o, g, nspin, rhf_trial):
     """Algorithmic real flops of one walker through the step (complex x real
-    MAC = 4, complex x complex MAC = 8)."""
+    MAC = 4, complex x complex MAC = 8).  The VHS is symmetric, so only its
+    n (n + 1) / 2 distinct entries count."""
     s_fb = 1 if rhf_trial else nspin
-    vhs = 4.0 * g * n * n
+    vhs = 2.0 * g * n * (n + 1)
     fb = s_fb * 4.0 * g * o * n
     taylor = nspin * 6 * 8.0 * n * n * o
     one_body = nspin * 2 * 4.0 * n * n * o
@@ -394,7 +395,7 @@ def kernel_table(ktimes, n, o, g, nw, nspin, rhf):
     fl = flops_per_walker_step(n, o, g, nspin, rhf)
     taylor = (fl["taylor"] + fl["one_body"]) * nw
     spec = {
-        "afq_vhs": (fl["vhs"] * nw, "fp64"),
+        "afq_dgemm_nt:vhs": (fl["vhs"] * nw, "fp64"),
         "afq_sgemm_nt:vhs": (fl["vhs"] * nw, "fp32"),
         "afq_gemm_bf16x3_nt:vhs": (fl["vhs"] * nw, "bf16x3"),
         "afq_force_bias": (fl["force_bias"] * nw, "fp64"),

@@ -124,6 +124,24 @@ int afq_vhs(int nw, int n, int ldv, int naux, const double* xbar, const double* 
             double* vhs, void* stream);
 
 /*
+ * Symmetric VHS contraction, the VHS GEMM of _make_vhs_split_flat
+ * (prop/chol_afqmc_ops.py:70-74) on half the work: V = sum_g x_g L_g is
+ * symmetric, so A (rows x k: shifted-field rows) is contracted only against
+ * the Cholesky rows of the upper 16 x 16 blocks of the (p, q) plane, B
+ * (nb (nb + 1) / 2 * 256 rows x k, nb = ceil(n / 16)): block (bi, bj),
+ * bi <= bj, numbered row by row, element (pi, qi) at row 256 blk + 16 pi + qi,
+ * entry (16 bi + pi, 16 bj + qi), zero rows past n.  Each result goes straight
+ * to both of its places in vhs (rows, n, ldv), 16-byte aligned; columns
+ * n..ldv-1 receive zeros.
+ *   arith 0: fp64 (a_hi, b_hi double; vhs double)
+ *   arith 1: fp32 (a_hi, b_hi float; vhs float)
+ *   arith 2: bf16x3 (hi / lo planes of afq_split_bf16; k, lda, ldb multiples
+ *            of 8; vhs float)
+ */
+int afq_vhs_sym(int rows, int n, int ldv, int k, int arith, const void* a_hi, const void* a_lo,
+                long long lda, const void* b_hi, const void* b_lo, long long ldb, void* vhs, void* stream);
+
+/*
  * Trotter step w' = E . sum_{k<=n_terms} (i sqrt_dt VHS)^k / k! . E w per walker
  * and spin (both spins share the walker's VHS), E = exp(-dt/2 h1_eff) (n, ldv).
  * Replaces _apply_trotter_r / _u (prop/chol_afqmc_ops.py:106-116, 127-154,

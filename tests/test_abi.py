@@ -51,6 +51,10 @@ def test_library_loads_and_reports_abi():
     ("afq_overlap_greens", (4, 100, 70, 1, 1, 0, 1, 0, 1, 1, 0), "maximum of 64"),
     ("afq_overlap_greens", (4, 8, 2, 2, 1, 0, 1, 0, 1, 1, 0), "spin-down"),
     ("afq_vhs", (4, 10, 10, 5, 1, 1, 1, 0), "multiple of 4"),
+    ("afq_vhs_sym", (4, 10, 10, 5, 0, 16, 0, 5, 16, 0, 5, 16, 0), "multiple of 4"),
+    ("afq_vhs_sym", (4, 10, 12, 5, 3, 16, 0, 5, 16, 0, 5, 16, 0), "arith"),
+    ("afq_vhs_sym", (4, 10, 12, 5, 0, 16, 0, 5, 16, 0, 5, 8, 0), "16-byte"),
+    ("afq_vhs_sym", (4, 10, 12, 5, 2, 16, 16, 5, 16, 16, 5, 16, 0), "multiples of 8"),
     ("afq_trotter", (1, 400, 40, 1, 400, 6, 0.1, 1, 1, 2, 0, 3, 0, 0), "exceeds"),
     ("afq_trotter", (1, 8, 2, 1, 8, 6, 0.1, 1, 1, 2, 0, 2, 0, 0), "aliases"),
     ("afq_energy", (4, 50, 40, 1, 5, 0.0, 1, 1, 52, 1, 1, 0), "maximum of 32"),

@@ -244,3 +244,42 @@ def test_invalid_arguments_raise(gpu_device):
         _lib.call("afq_trotter", 1, 400, 40, 1, 400, 6, 0.1, 1, 1, 2, 0, 3, 0, 0)
     with pytest.raises(ValueError, match="aliases"):
         _lib.call("afq_trotter", 1, 8, 2, 1, 8, 6, 0.1, 1, 1, 2, 0, 2, 0, 0)
+
+
+@pytest.mark.parametrize("n,naux,nw,prec", [
+    (1, 3, 2, "fp64"), (17, 40, 5, "fp64"), (45, 130, 3, "fp64"), (200, 1000, 16, "fp64"),
+    (1, 3, 2, "fp32"), (17, 40, 5, "fp32"), (200, 1000, 16, "fp32"),
+    (1, 8, 2, "bf16x3"), (24, 96, 8, "bf16x3"), (45, 130, 3, "bf16x3"), (200, 1000, 16, "bf16x3"),
+    (200, 1000, 2048, "bf16x3"),  # the 256 x 256 tile kernel
+])
+def test_symmetric_vhs_matches_full_gemm(gpu_device, n, naux, nw, prec):
+    """The symmetric contraction (upper 16 x 16 blocks, each entry stored at
+    (p, q) and (q, p)) equals the full-row GEMM bit for bit (same k order per
+    entry; L_g exactly symmetric), padding columns zero, every entry written."""
+    import torch
+    from ad_afqmc_prototype_amd import kernels
+    from ad_afqmc_prototype_amd.prop.chol_afqmc_ops import chol_sym_rows, chol_transposed
+    print(f"symmetric VHS n={n} naux={naux} rows={2 * nw} {prec}", flush=True)
+    rng = np.random.default_rng(n)
+    a = rng.standard_normal((naux, n, n))
+    chol = torch.tensor(0.5 * (a + a.transpose(0, 2, 1)), device=gpu_device)
+    x = torch.tensor(rng.standard_normal((2 * nw, naux)), device=gpu_device)
+    ldv = kernels.ldv_for(n)
+    full_b, sym_b = chol_transposed(chol), chol_sym_rows(chol)
+    odt = torch.float64 if prec == "fp64" else torch.float32
+    ref = torch.empty((2 * nw, n * ldv), dtype=odt, device=gpu_device)
+    if prec == "fp64":
+        kernels.dgemm_nt(x, full_b, ref)
+        a_op, b_op = x, sym_b
+    elif prec == "fp32":
+        x32 = kernels.convert_f32(x)
+        kernels.sgemm_nt(x32, kernels.convert_f32(full_b), ref)
+        a_op, b_op = x32, kernels.convert_f32(sym_b)
+    else:
+        a_op = kernels.split_bf16(x)
+        kernels.gemm_bf16x3_nt(*a_op, *kernels.split_bf16(full_b), ref)
+        b_op = kernels.split_bf16(sym_b)
+    out = torch.full((2 * nw, n, ldv), float("nan"), dtype=odt, device=gpu_device)
+    kernels.vhs_sym(a_op, b_op, n, out=out)
+    assert torch.equal(out.view(2 * nw, n * ldv), ref)
+    assert torch.equal(out[..., n:], torch.zeros_like(out[..., n:]))
