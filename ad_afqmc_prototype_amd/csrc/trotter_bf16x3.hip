@@ -79,6 +79,15 @@ __device__ __forceinline__ float4 load_f4(rsrc_t r, unsigned off) {
     return __builtin_bit_cast(float4, __builtin_amdgcn_raw_buffer_load_b128(r, off, 0, 0));
 }
 
+typedef __attribute__((address_space(3))) void* lds_ptr_t;
+
+// 16 bytes per lane from buffer r at byte offset off straight into LDS at
+// slot + lane (slot wave-uniform); zeros for offsets past the buffer.  The data
+// is there once the issuing wave has passed s_waitcnt vmcnt(0).
+__device__ __forceinline__ void load_lds16(rsrc_t r, uint4* slot, unsigned off) {
+    __builtin_amdgcn_raw_ptr_buffer_load_lds(r, (lds_ptr_t)slot, 16, off, 0, 0, 0);
+}
+
 __device__ __forceinline__ double2 load_d2(rsrc_t r, unsigned off) {
     return __builtin_bit_cast(double2, __builtin_amdgcn_raw_buffer_load_b128(r, off, 0, 0));
 }
@@ -148,7 +157,6 @@ __global__ __launch_bounds__(512, 1) void trotter_bf16x3_kernel(
 
     // blockDim.x = 64 * nk: one wave per pair of 16-row tiles
     const int tid = threadIdx.x;
-    const int nthreads = blockDim.x;
     const int lane = tid & 63;
     const int wave = tid >> 6;
     const int lr = lane & 15;
@@ -223,16 +231,8 @@ __global__ __launch_bounds__(512, 1) void trotter_bf16x3_kernel(
     // ---- prologue: walker columns, accumulator rows and E in flight together ----
     f32x4 acc[RT][2];
     {
-        // T fill: 16 * nk * 32 elements = 8 per thread, element e = tid + i * nthreads
-        double2 wf[8];
-#pragma unroll
-        for (int i = 0; i < 8; ++i) {
-            const int e = tid + i * nthreads;
-            const int p = e >> 4;
-            const int j = c0 + (e & 15);
-            wf[i] = load_w(p, j);
-        }
-        // accumulator element r of tile t: row (wave*RT + t)*16 + 4q + r, column col
+        // accumulator element r of tile t: row (wave*RT + t)*16 + 4q + r, column
+        // col; the waves' accumulators cover all nk * 32 rows of T
         double2 wa[RT][4];
 #pragma unroll
         for (int t = 0; t < RT; ++t)
@@ -243,24 +243,13 @@ __global__ __launch_bounds__(512, 1) void trotter_bf16x3_kernel(
         float er[RT][KMAX][8];
         load_e(er, row0, kq);
 #pragma unroll
-        for (int i = 0; i < 8; ++i) {
-            const int e = tid + i * nthreads;
-            const int p = e >> 4;
-            const int jj = e & 15;
-            const float xr = (float)wf[i].x, xi = (float)wf[i].y;
-            const __bf16 hr = (__bf16)xr, hi = (__bf16)xi;
-            Tt[(0 * 16 + jj) * TP + p] = hr;
-            Tt[(1 * 16 + jj) * TP + p] = (__bf16)(xr - (float)hr);
-            Tt[(2 * 16 + jj) * TP + p] = hi;
-            Tt[(3 * 16 + jj) * TP + p] = (__bf16)(xi - (float)hi);
-        }
-#pragma unroll
         for (int t = 0; t < RT; ++t)
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
                 acc[t][0][r] = (float)wa[t][r].x;
                 acc[t][1][r] = (float)wa[t][r].y;
             }
+        store_t(acc);  // T = W
         __syncthreads();
         // ---- T0 = E W ----
         e_product(acc, er, row0, kq);

@@ -390,14 +390,12 @@ def measured_traffic(workload: str, kernel: str):
                      + row["write_kib_raw"] * cal.get("write_factor", 1.0))
 
 
-def kernel_table(ktimes, n, o, g, nw, nspin, rhf):
+def kernel_table(ktimes, n, o, g, nw, nspin, rhf, precision):
     """Algorithmic flops per launch and peak of every timed entry point."""
     fl = flops_per_walker_step(n, o, g, nspin, rhf)
     taylor = (fl["taylor"] + fl["one_body"]) * nw
     spec = {
-        "afq_dgemm_nt:vhs": (fl["vhs"] * nw, "fp64"),
-        "afq_sgemm_nt:vhs": (fl["vhs"] * nw, "fp32"),
-        "afq_gemm_bf16x3_nt:vhs": (fl["vhs"] * nw, "bf16x3"),
+        "afq_vhs_sym:vhs": (fl["vhs"] * nw, precision),
         "afq_force_bias": (fl["force_bias"] * nw, "fp64"),
         "afq_sgemm_nt:force_bias": (fl["force_bias"] * nw, "fp32"),
         "afq_gemm_bf16x3_nt:force_bias": (fl["force_bias"] * nw, "bf16x3"),
@@ -507,7 +505,7 @@ def main():
     nspin = 2 if cfg["walker_kind"] == "unrestricted" else 1
 
     elapsed, ktimes, e_block = timed_run(args, args.precision, device)
-    table = kernel_table(ktimes, n, o, g, nw, nspin, not cfg["uhf"])
+    table = kernel_table(ktimes, n, o, g, nw, nspin, not cfg["uhf"], args.precision)
     dominant = max((k for k in table if "tflops" in table[k]), key=lambda k: table[k]["total_ms"])
     dom = table[dominant]
     value = nw * args.steps / elapsed
@@ -545,7 +543,7 @@ def main():
         out["fp64_value"] = nw * args.steps / el64
         out["fp64_ms_per_step"] = 1e3 * el64 / args.steps
         out["speedup_vs_fp64"] = out["value"] / out["fp64_value"]
-        out["fp64_kernels"] = kernel_table(kt64, n, o, g, nw, nspin, not cfg["uhf"])
+        out["fp64_kernels"] = kernel_table(kt64, n, o, g, nw, nspin, not cfg["uhf"], "fp64")
         out["block_energy_abs_diff_mHa"] = [1e3 * abs(a - b) for a, b in zip(e_block, e64)]
     if args.precision != "fp64" and args.accuracy_scale != 0:
         scale = cfg["chol_scale"] if args.accuracy_scale is None else args.accuracy_scale
