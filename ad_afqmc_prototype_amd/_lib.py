@@ -32,6 +32,8 @@ _SIGNATURES = {
     "afq_trotter_bf16x3": [c_int, c_int, c_int, c_int, c_int, c_int, c_double, c_vp, c_vp, c_vp,
                            c_vp, c_vp, c_vp, c_vp],
     "afq_overlap_greens": [c_int, c_int, c_int, c_int, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp],
+    "afq_overlap_greens_split": [c_int, c_int, c_int, c_int, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp,
+                                 c_ll, c_vp],
     "afq_force_bias": [c_int, c_int, c_int, c_double, c_vp, c_vp, c_vp, c_vp],
     "afq_field_shifts": [c_int, c_int, c_double, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp],
     "afq_vhs": [c_int, c_int, c_int, c_int, c_vp, c_vp, c_vp, c_vp],

@@ -101,6 +101,23 @@ __device__ __forceinline__ double wave_sum(double v) {
     return v;
 }
 
+// bf16 bits of x, round to nearest even (NaN -> quiet NaN)
+__device__ __forceinline__ uint16_t f32_to_bf16_rne(float f) {
+    uint32_t u = __float_as_uint(f);
+    if ((u & 0x7fffffffu) > 0x7f800000u) return 0x7fc0;
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return (uint16_t)(u >> 16);
+}
+
+__device__ __forceinline__ double bf16_to_f64(uint16_t h) { return (double)__uint_as_float(((uint32_t)h) << 16); }
+
+// The split-bf16 pair of an fp64 value: hi = bf16(x), lo = bf16(x - hi), both
+// rounded to nearest even through fp32.
+__device__ __forceinline__ void split_bf16(double x, uint16_t& hi, uint16_t& lo) {
+    hi = f32_to_bf16_rne((float)x);
+    lo = f32_to_bf16_rne((float)(x - bf16_to_f64(hi)));
+}
+
 // ---------------------------------------------------------------------------
 // Symmetric VHS epilogue.  The VHS GEMM can run on the upper 16 x 16 blocks of
 // the n x n plane only (V = sum_g x_g L_g is symmetric): its column

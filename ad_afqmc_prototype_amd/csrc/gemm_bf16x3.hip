@@ -27,16 +27,6 @@ constexpr int PITCH = 40;  // 80-byte rows: conflict-free 16-byte LDS reads
 constexpr int THREADS = 256;
 constexpr int GROUP_M = 8;
 
-__device__ __forceinline__ uint16_t f32_to_bf16_rne(float f) {
-    uint32_t u = __float_as_uint(f);
-    if ((u & 0x7fffffffu) > 0x7f800000u) return 0x7fc0;  // NaN
-    u += 0x7fffu + ((u >> 16) & 1u);
-    return (uint16_t)(u >> 16);
-}
-
-__device__ __forceinline__ double bf16_to_f64(uint16_t h) {
-    return (double)__uint_as_float(((uint32_t)h) << 16);
-}
 
 // SYM: the columns are symmetric-VHS block entries and the tile is stored
 // expanded (afq::sym_store); C is then float, ldc unused.
@@ -363,10 +353,7 @@ __global__ void split_bf16_kernel(int rows, int cols, const double* __restrict__
         const long long r = e / ld_dst;
         const long long c = e - r * ld_dst;
         const double x = c < cols ? src[r * ld_src + c] : 0.0;
-        const uint16_t h = f32_to_bf16_rne((float)x);
-        const uint16_t l = f32_to_bf16_rne((float)(x - bf16_to_f64(h)));
-        hi[e] = h;
-        lo[e] = l;
+        afq::split_bf16(x, hi[e], lo[e]);
     }
 }
 

@@ -214,11 +214,17 @@ struct greens_wave_lds {
     int perm[MAXO];
 };
 
-template <int MAXO>
+// Optional second copy of the G rows as split-bf16 planes (afq::split_bf16):
+// hi / lo of the real row and of the imaginary row, same column order.
+struct split_rows {
+    uint16_t *hre, *lre, *him, *lim;
+};
+
+template <int MAXO, bool SPLIT = false>
 __device__ __forceinline__ void greens_wave_body(int n, int o, const double2* __restrict__ W,
                                                  const double* __restrict__ Cm, double* __restrict__ gre,
                                                  double* __restrict__ gim, double2* __restrict__ ld,
-                                                 greens_wave_lds<MAXO>& S) {
+                                                 greens_wave_lds<MAXO>& S, split_rows sp = split_rows{}) {
     constexpr int NT = (MAXO + 15) / 16;
     const int lane = threadIdx.x & 63;
     const int lr = lane & 15;
@@ -384,18 +390,26 @@ __device__ __forceinline__ void greens_wave_body(int n, int o, const double2* __
         for (int k = 0; k < MAXO; ++k) {
             if (k < o) {
                 const int i = S.perm[k];
-                gre[(long long)i * n + p] = xr[k].x;
-                gim[(long long)i * n + p] = xr[k].y;
+                const long long e = (long long)i * n + p;
+                gre[e] = xr[k].x;
+                gim[e] = xr[k].y;
+                if constexpr (SPLIT) {
+                    afq::split_bf16(xr[k].x, sp.hre[e], sp.lre[e]);
+                    afq::split_bf16(xr[k].y, sp.him[e], sp.lim[e]);
+                }
             }
         }
     }
 }
 
-template <int MAXO>
+// SPLIT: the G rows also go to split-bf16 planes hi / lo (2 nw rows of ldk,
+// real rows then imaginary rows, like greens), columns ldg..ldk-1 zeroed.
+template <int MAXO, bool SPLIT = false>
 __global__ __launch_bounds__(THREADS) void greens_wave_kernel(
     int nw, int n, int o, int nspin, const double2* __restrict__ wup,
     const double2* __restrict__ wdn, const double* __restrict__ cup, const double* __restrict__ cdn,
-    double* __restrict__ greens, double2* __restrict__ logdet) {
+    double* __restrict__ greens, double2* __restrict__ logdet, uint16_t* __restrict__ ghi = nullptr,
+    uint16_t* __restrict__ glo = nullptr, long long ldk = 0) {
     __shared__ greens_wave_lds<MAXO> L[THREADS / 64];
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int item = blockIdx.x * (THREADS / 64) + wave;
@@ -404,8 +418,22 @@ __global__ __launch_bounds__(THREADS) void greens_wave_kernel(
     const int s = item - w * nspin;
     const long long ldg = (long long)nspin * o * n;
     double* gre = greens + (long long)w * ldg + (long long)s * o * n;
-    greens_wave_body<MAXO>(n, o, (s == 0 ? wup : wdn) + (long long)w * n * o, s == 0 ? cup : cdn, gre,
-                           gre + (long long)nw * ldg, logdet + (long long)w * nspin + s, L[wave]);
+    split_rows sp{};
+    if constexpr (SPLIT) {
+        const long long rre = (long long)w * ldk + (long long)s * o * n;
+        const long long rim = rre + (long long)nw * ldk;
+        sp = split_rows{ghi + rre, glo + rre, ghi + rim, glo + rim};
+        const int lane = threadIdx.x & 63;
+        if (s == nspin - 1 && lane < ldk - ldg) {
+            const long long c = (long long)w * ldk + ldg + lane;
+            ghi[c] = 0;
+            glo[c] = 0;
+            ghi[c + (long long)nw * ldk] = 0;
+            glo[c + (long long)nw * ldk] = 0;
+        }
+    }
+    greens_wave_body<MAXO, SPLIT>(n, o, (s == 0 ? wup : wdn) + (long long)w * n * o, s == 0 ? cup : cdn, gre,
+                                  gre + (long long)nw * ldg, logdet + (long long)w * nspin + s, L[wave], sp);
 }
 
 template <int MAXO>
@@ -459,11 +487,14 @@ extern "C" int afq_overlap_greens(int nw, int n, int o, int nspin, const double*
     double2* ld = (double2*)logdet;
     const dim3 wgrid((unsigned)afq::ceil_div(nw * nspin, THREADS / 64));
     if (o <= 8)
-        hipLaunchKernelGGL(greens_wave_kernel<8>, wgrid, dim3(THREADS), 0, st, nw, n, o, nspin, wu, wd, trial_up, cd, greens, ld);
+        hipLaunchKernelGGL((greens_wave_kernel<8, false>), wgrid, dim3(THREADS), 0, st, nw, n, o, nspin, wu, wd, trial_up, cd, greens, ld,
+                           nullptr, nullptr, 0LL);
     else if (o <= 16)
-        hipLaunchKernelGGL(greens_wave_kernel<16>, wgrid, dim3(THREADS), 0, st, nw, n, o, nspin, wu, wd, trial_up, cd, greens, ld);
+        hipLaunchKernelGGL((greens_wave_kernel<16, false>), wgrid, dim3(THREADS), 0, st, nw, n, o, nspin, wu, wd, trial_up, cd, greens, ld,
+                           nullptr, nullptr, 0LL);
     else if (o <= 32)
-        hipLaunchKernelGGL(greens_wave_kernel<32>, wgrid, dim3(THREADS), 0, st, nw, n, o, nspin, wu, wd, trial_up, cd, greens, ld);
+        hipLaunchKernelGGL((greens_wave_kernel<32, false>), wgrid, dim3(THREADS), 0, st, nw, n, o, nspin, wu, wd, trial_up, cd, greens, ld,
+                           nullptr, nullptr, 0LL);
     else
         hipLaunchKernelGGL(greens_kernel<64>, grid, dim3(THREADS), 0, st, nw, n, o, nspin, wu, wd, trial_up, cd, greens, ld);
     return afq::check_launch("afq_overlap_greens");
@@ -484,4 +515,45 @@ extern "C" int afq_overlap_greens_grouped(int count, const afq_system_step* sys,
     else
         hipLaunchKernelGGL(greens_grouped_kernel<32>, grid, dim3(THREADS), 0, st, count, total_walkers, sys);
     return afq::check_launch("afq_overlap_greens_grouped");
+}
+
+extern "C" int afq_overlap_greens_split(int nw, int n, int o, int nspin, const double* walkers_up,
+                                        const double* walkers_dn, const double* trial_up,
+                                        const double* trial_dn, double* greens, double* logdet,
+                                        unsigned short* g_hi, unsigned short* g_lo, long long ldk,
+                                        void* stream) {
+    const long long ldg = (long long)nspin * o * n;
+    AFQ_REQUIRE(g_hi && g_lo, "afq_overlap_greens_split: null pointer");
+    AFQ_REQUIRE(ldk >= ldg && ldk - ldg < 64, "afq_overlap_greens_split: ldk=%lld not in [%lld, %lld)", ldk, ldg,
+                ldg + 64);
+    if (o > 32) {
+        // block kernel, then the split of its rows
+        const int rc = afq_overlap_greens(nw, n, o, nspin, walkers_up, walkers_dn, trial_up, trial_dn, greens,
+                                          logdet, stream);
+        if (rc != 0) return rc;
+        return afq_split_bf16(2 * nw, (int)ldg, greens, ldg, g_hi, g_lo, ldk, stream);
+    }
+    AFQ_REQUIRE(nw > 0 && n > 0 && o > 0 && o <= n, "afq_overlap_greens_split: bad sizes nw=%d n=%d o=%d", nw, n,
+                o);
+    AFQ_REQUIRE(nspin == 1 || nspin == 2, "afq_overlap_greens_split: nspin must be 1 or 2");
+    AFQ_REQUIRE(walkers_up && trial_up && greens && logdet, "afq_overlap_greens_split: null pointer");
+    AFQ_REQUIRE(nspin == 1 || (walkers_dn && trial_dn), "afq_overlap_greens_split: null spin-down pointer");
+    hipStream_t st = (hipStream_t)stream;
+    const double2* wu = (const double2*)walkers_up;
+    const double2* wd = (const double2*)(nspin == 2 ? walkers_dn : walkers_up);
+    const double* cd = nspin == 2 ? trial_dn : trial_up;
+    double2* ld = (double2*)logdet;
+    uint16_t* hi = (uint16_t*)g_hi;
+    uint16_t* lo = (uint16_t*)g_lo;
+    const dim3 wgrid((unsigned)afq::ceil_div(nw * nspin, THREADS / 64));
+    if (o <= 8)
+        hipLaunchKernelGGL((greens_wave_kernel<8, true>), wgrid, dim3(THREADS), 0, st, nw, n, o, nspin, wu, wd, trial_up,
+                           cd, greens, ld, hi, lo, ldk);
+    else if (o <= 16)
+        hipLaunchKernelGGL((greens_wave_kernel<16, true>), wgrid, dim3(THREADS), 0, st, nw, n, o, nspin, wu, wd,
+                           trial_up, cd, greens, ld, hi, lo, ldk);
+    else
+        hipLaunchKernelGGL((greens_wave_kernel<32, true>), wgrid, dim3(THREADS), 0, st, nw, n, o, nspin, wu, wd,
+                           trial_up, cd, greens, ld, hi, lo, ldk);
+    return afq::check_launch("afq_overlap_greens_split");
 }

@@ -219,8 +219,10 @@ def gemm_bf16x3_nt(a_hi, a_lo, b_hi, b_lo, C, alpha=1.0, tag=None):
     return C
 
 
-def overlap_greens(walkers, trial_up, trial_dn=None, greens=None, logdet=None):
-    """Returns (greens planes (2, nw, nspin*o*n), logdet complex128 (nw, nspin))."""
+def overlap_greens(walkers, trial_up, trial_dn=None, greens=None, logdet=None, split=False):
+    """Returns (greens planes (2, nw, nspin*o*n), logdet complex128 (nw, nspin)),
+    and with ``split`` also the (hi, lo) bf16 planes of the greens rows
+    (2 nw, pad8(nspin*o*n)) that split_bf16 would make, from the same pass."""
     wu, wd = (walkers if isinstance(walkers, tuple) else (walkers, None))
     nspin = 1 if wd is None else 2
     nw, n, o = wu.shape
@@ -238,6 +240,14 @@ def overlap_greens(walkers, trial_up, trial_dn=None, greens=None, logdet=None):
         logdet = torch.empty((nw, nspin), dtype=c128, device=dev)
     _check(greens, f64, (2, nw, nspin * o * n), "greens", dev)
     _check(logdet, c128, (nw, nspin), "logdet", dev)
+    if split:
+        ldk = pad8(nspin * o * n)
+        hi = torch.empty((2 * nw, ldk), dtype=torch.int16, device=dev)
+        lo = torch.empty((2 * nw, ldk), dtype=torch.int16, device=dev)
+        _call("afq_overlap_greens_split", wu, nw, n, o, nspin, wu.data_ptr(), _ptr(wd), trial_up.data_ptr(),
+              _ptr(trial_dn if nspin == 2 else None), greens.data_ptr(), logdet.data_ptr(), hi.data_ptr(),
+              lo.data_ptr(), ldk, _stream(wu))
+        return greens, logdet, (hi, lo)
     _call("afq_overlap_greens", wu, nw, n, o, nspin, wu.data_ptr(), _ptr(wd), trial_up.data_ptr(),
               _ptr(trial_dn if nspin == 2 else None), greens.data_ptr(), logdet.data_ptr(),
               _stream(wu))

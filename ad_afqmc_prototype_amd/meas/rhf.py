@@ -101,8 +101,8 @@ def force_bias_planes(walkers, meas_ctx: rhf_meas_ctx, trial_data, greens=None) 
     """Force bias as split planes (2, nw, naux)."""
     nspin = _nspin(walkers)
     if greens is None:
-        greens = trial_rhf.greens(walkers, trial_data)
-    return _force_bias_rows(greens.greens, meas_ctx, nspin, 2.0 if nspin == 1 else 1.0)
+        greens = trial_rhf.greens(walkers, trial_data, split=meas_ctx.precision == "bf16x3")
+    return _force_bias_rows(greens.greens, meas_ctx, nspin, 2.0 if nspin == 1 else 1.0, greens.split)
 
 
 def force_bias_planes_g(walkers, meas_ctx: rhf_meas_ctx, trial_data, greens=None) -> torch.Tensor:
@@ -117,8 +117,11 @@ def force_bias_planes_g(walkers, meas_ctx: rhf_meas_ctx, trial_data, greens=None
     return _force_bias_rows(rows, meas_ctx, 1, 1.0)
 
 
-def _force_bias_rows(g: torch.Tensor, meas_ctx: rhf_meas_ctx, nspin: int, alpha: float) -> torch.Tensor:
-    """fb planes (2, nw, naux) = alpha * rows of g (2, nw, k) . rot^T in the ctx's precision."""
+def _force_bias_rows(g: torch.Tensor, meas_ctx: rhf_meas_ctx, nspin: int, alpha: float,
+                     split: tuple | None = None) -> torch.Tensor:
+    """fb planes (2, nw, naux) = alpha * rows of g (2, nw, k) . rot^T in the ctx's precision
+    (bf16x3: ``split``, the bf16 planes of g's rows when the Green's function
+    kernel made them, else split here)."""
     op = meas_ctx.fb_operand(nspin)
     if meas_ctx.precision == "fp64":
         return kernels.force_bias(g, op, alpha)
@@ -130,7 +133,7 @@ def _force_bias_rows(g: torch.Tensor, meas_ctx: rhf_meas_ctx, nspin: int, alpha:
         kernels.sgemm_nt(kernels.convert_f32(flat), op, fb.view(2 * nw, naux), alpha=alpha,
                          tag="force_bias")
     else:
-        gh, gl = kernels.split_bf16(flat)
+        gh, gl = split if split is not None else kernels.split_bf16(flat)
         kernels.gemm_bf16x3_nt(gh, gl, op[0], op[1], fb.view(2 * nw, naux), alpha=alpha,
                                tag="force_bias")
     return fb
@@ -183,15 +186,18 @@ def make_rhf_meas_ops(sys: system, precision: str = "fp64") -> meas_ops:
         raise ValueError(f"unknown precision {precision!r}")
     build_meas_ctx_p = partial(build_meas_ctx, precision=precision)
     wk = sys.walker_kind.lower()
+    # the bf16x3 force bias takes the Green's function rows as bf16 planes,
+    # which the Green's function kernel writes in the same pass
+    greens_p = partial(trial_rhf.greens, split=precision == "bf16x3")
     if wk == "restricted":
         return meas_ops(overlap=trial_rhf.overlap_r, build_meas_ctx=build_meas_ctx_p,
                         kernels={k_force_bias: force_bias_kernel_r, k_energy: energy_kernel_r},
-                        log_overlap=trial_rhf.log_overlap, greens=trial_rhf.greens,
+                        log_overlap=trial_rhf.log_overlap, greens=greens_p,
                         force_bias_planes=force_bias_planes)
     if wk == "unrestricted":
         return meas_ops(overlap=trial_rhf.overlap_u, build_meas_ctx=build_meas_ctx_p,
                         kernels={k_force_bias: force_bias_kernel_u, k_energy: energy_kernel_u},
-                        log_overlap=trial_rhf.log_overlap, greens=trial_rhf.greens,
+                        log_overlap=trial_rhf.log_overlap, greens=greens_p,
                         force_bias_planes=force_bias_planes)
     if wk == "generalized":
         # meas/rhf.py:169-174: overlap and force bias, no energy kernel

@@ -71,8 +71,15 @@ def reconfigure(state: prop_state, *, sys: system, trial_data, meas_ops_: meas_o
     lo = kernels.gather_rows(state.log_overlaps, idx)
     greens = state.greens
     if greens is not None:
+        split = greens.split
+        if split is not None:
+            # the bf16 planes (2 nw rows of ldk, ldk a multiple of 8) move as
+            # float64 words: a bit copy
+            nw_old = greens.logdet.shape[0]
+            split = tuple(kernels.gather_rows(x.view(2, nw_old, -1).view(torch.float64), idx, nplanes=2)
+                          .view(torch.int16).reshape(2 * idx.shape[0], -1) for x in split)
         greens = type(greens)(kernels.gather_rows(greens.greens, idx, nplanes=2),
-                              kernels.gather_rows(greens.logdet, idx))
+                              kernels.gather_rows(greens.logdet, idx), split)
     return state._replace(walkers=w_sr, weights=w_new, log_overlaps=lo, rng_key=key,
                           greens=greens), idx
 
@@ -107,6 +114,8 @@ def _clone_state(state: prop_state) -> prop_state:
             return None
         if isinstance(x, torch.Tensor):
             return x.clone()
+        if type(x) is tuple:
+            return tuple(c(y) for y in x)
         return type(x)(*(c(y) for y in x))
     return state._replace(walkers=c(state.walkers), weights=c(state.weights),
                           log_overlaps=c(state.log_overlaps), rng_key=c(state.rng_key),
@@ -115,7 +124,10 @@ def _clone_state(state: prop_state) -> prop_state:
 
 def _copy_state_(dst: prop_state, src: prop_state) -> None:
     def cp(d, s):
-        if isinstance(d, torch.Tensor):
+        if d is None or s is None:
+            if d is not s:
+                raise ValueError("state layouts differ (a cached entry is missing on one side)")
+        elif isinstance(d, torch.Tensor):
             d.copy_(s)
         else:
             for a, b in zip(d, s):

@@ -75,10 +75,12 @@ class uhf_trial:
 
 class greens_cache(NamedTuple):
     """Half-rotated Green's functions (split planes (2, nw, nspin*o*n)) and
-    log det(C_s^T w_s) (nw, nspin) of one walker batch."""
+    log det(C_s^T w_s) (nw, nspin) of one walker batch; optionally the
+    (hi, lo) bf16 planes of the greens rows for the bf16x3 force bias."""
 
     greens: torch.Tensor
     logdet: torch.Tensor
+    split: tuple | None = None
 
     @property
     def log_overlap(self) -> torch.Tensor:
@@ -93,14 +95,15 @@ def get_rdm1(trial_data) -> torch.Tensor:
     return torch.stack([cu @ cu.T, cd @ cd.T], dim=0)
 
 
-def greens(walkers, trial_data) -> greens_cache:
+def greens(walkers, trial_data, split: bool = False) -> greens_cache:
+    """``split`` also returns the bf16 planes of the rows (same kernel pass)."""
     if isinstance(walkers, tuple):
-        g, ld = kernels.overlap_greens(walkers, trial_data.mo_up, trial_data.mo_dn)
+        out = kernels.overlap_greens(walkers, trial_data.mo_up, trial_data.mo_dn, split=split)
     else:
         if not trial_data.restricted:
             raise ValueError("restricted walkers need an RHF trial")
-        g, ld = kernels.overlap_greens(walkers, trial_data.mo_up)
-    return greens_cache(g, ld)
+        out = kernels.overlap_greens(walkers, trial_data.mo_up, split=split)
+    return greens_cache(*out)
 
 
 def log_overlap(walkers, trial_data) -> torch.Tensor:

@@ -95,6 +95,17 @@ int afq_overlap_greens(int nw, int n, int o, int nspin,
                        double* greens, double* logdet, void* stream);
 
 /*
+ * afq_overlap_greens that also writes the Green's function rows as split-bf16
+ * planes g_hi / g_lo (2 nw rows of ldk: real rows, then imaginary rows, the
+ * values afq_split_bf16 makes of greens; columns nspin o n .. ldk - 1 zeroed),
+ * the A operand of the bf16x3 force-bias contraction, without a second pass
+ * over greens.  nspin o n <= ldk < nspin o n + 64.
+ */
+int afq_overlap_greens_split(int nw, int n, int o, int nspin, const double* walkers_up, const double* walkers_dn,
+                             const double* trial_up, const double* trial_dn, double* greens, double* logdet,
+                             unsigned short* g_hi, unsigned short* g_lo, long long ldk, void* stream);
+
+/*
  * Force bias fb = alpha * rot . vec(G): planes (2, nw, naux).
  * rot (naux, k) holds the half-rotated Cholesky vectors of every spin sector
  * side by side (k = nspin*o*n).  alpha = 2 for restricted walkers.

@@ -50,6 +50,9 @@ def test_library_loads_and_reports_abi():
     ("afq_overlap_greens", (4, 8, 9, 1, 1, 0, 1, 0, 1, 1, 0), "bad sizes"),
     ("afq_overlap_greens", (4, 100, 70, 1, 1, 0, 1, 0, 1, 1, 0), "maximum of 64"),
     ("afq_overlap_greens", (4, 8, 2, 2, 1, 0, 1, 0, 1, 1, 0), "spin-down"),
+    ("afq_overlap_greens_split", (4, 8, 2, 1, 1, 0, 1, 0, 1, 1, 1, 1, 15, 0), "ldk"),
+    ("afq_overlap_greens_split", (4, 8, 2, 1, 1, 0, 1, 0, 1, 1, 1, 1, 80, 0), "ldk"),
+    ("afq_overlap_greens_split", (4, 8, 2, 1, 1, 0, 1, 0, 1, 1, 0, 1, 16, 0), "null"),
     ("afq_vhs", (4, 10, 10, 5, 1, 1, 1, 0), "multiple of 4"),
     ("afq_vhs_sym", (4, 10, 10, 5, 0, 16, 0, 5, 16, 0, 5, 16, 0), "multiple of 4"),
     ("afq_vhs_sym", (4, 10, 12, 5, 3, 16, 0, 5, 16, 0, 5, 16, 0), "arith"),

@@ -10,7 +10,7 @@ from ad_afqmc_prototype_amd import synthetic as syn
 pytestmark = pytest.mark.gpu
 
 
-def _setup(gpu_device, nw=64, steps=5, kind="restricted", seed=0, n=24, o=6, g=96):
+def _setup(gpu_device, nw=64, steps=5, kind="restricted", seed=0, n=24, o=6, g=96, precision="fp64"):
     from ad_afqmc_prototype_amd.core.system import system
     from ad_afqmc_prototype_amd.ham.chol import ham_chol
     from ad_afqmc_prototype_amd.meas.rhf import make_rhf_meas_ops
@@ -23,8 +23,8 @@ def _setup(gpu_device, nw=64, steps=5, kind="restricted", seed=0, n=24, o=6, g=9
     ham = ham_chol.from_arrays(s.h0, s.h1, s.chol, device=gpu_device)
     trial = rhf_trial(s.mo_up)
     t_ops = make_rhf_trial_ops(sys_)
-    m_ops = make_rhf_meas_ops(sys_)
-    p_ops = make_chol_afqmc_ops(ham, kind)
+    m_ops = make_rhf_meas_ops(sys_, precision=precision)
+    p_ops = make_chol_afqmc_ops(ham, kind, precision=precision)
     params = afqmc_params(dt=0.01, n_prop_steps=steps, n_walkers=nw, n_ortho=steps, seed=seed,
                           n_eql_blocks=5, n_blocks=10)
     kw = dict(sys=sys_, params=params, ham_data=ham, trial_data=trial, meas_ops=m_ops, prop_ops=p_ops,
@@ -49,10 +49,16 @@ def test_step_draws_fields_from_the_device_key(gpu_device):
     assert torch.equal(a.rng_key, nk) and torch.equal(b.rng_key, nk)
 
 
-def test_captured_block_replays_match_eager(gpu_device):
+@pytest.mark.parametrize("precision", ["fp64", "bf16x3"])
+def test_captured_block_replays_match_eager(gpu_device, precision):
+    """bf16x3: the state also carries the Green's function's bf16 planes (made
+    by the Green's function kernel, gathered by the comb), which the graph
+    copies like every other state buffer."""
     import torch
     from ad_afqmc_prototype_amd.prop.blocks import _clone_state, afqmc_block, captured_block
-    st, kw, _ = _setup(gpu_device)
+    st, kw, _ = _setup(gpu_device, precision=precision)
+    if precision == "bf16x3":
+        assert st.greens.split is not None
     ref, st_e = [], _clone_state(st)
     for _ in range(3):
         st_e, obs = afqmc_block(st_e, **kw)

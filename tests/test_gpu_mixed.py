@@ -288,3 +288,45 @@ def test_energy_bf16x3_contractions(gpu_device, case, tol):
     e_ref = orc.energy(w, p.sd.h0, p.sd.trial, p.sd.mctx, case["kind"])
     err = np.abs(e - e_ref) / np.abs(e_ref)
     assert err.max() < tol, err.max()
+
+
+@pytest.mark.parametrize("nw,n,o,nspin", [(8, 24, 6, 1), (5, 40, 7, 2), (6, 45, 11, 2), (16, 200, 32, 1),
+                                          (3, 60, 40, 1)])
+def test_greens_split_planes_equal_split_of_greens(gpu_device, nw, n, o, nspin):
+    """afq_overlap_greens_split: the same greens and logdet as afq_overlap_greens,
+    and bf16 planes equal to split_bf16 of the greens rows bit for bit (zero
+    padding columns included); o > 32 takes the block kernel and a split."""
+    import torch
+    from ad_afqmc_prototype_amd import kernels
+    print(f"greens split nw={nw} n={n} o={o} nspin={nspin}", flush=True)
+    rng = np.random.default_rng(n + o)
+    trial = torch.tensor(np.linalg.qr(rng.standard_normal((n, o)))[0], device=gpu_device)
+    ws = [torch.tensor(rng.standard_normal((nw, n, o)) + 1j * rng.standard_normal((nw, n, o)),
+                       device=gpu_device) for _ in range(nspin)]
+    walk = ws[0] if nspin == 1 else tuple(ws)
+    td = None if nspin == 1 else trial
+    g0, ld0 = kernels.overlap_greens(walk, trial, td)
+    g1, ld1, (hi, lo) = kernels.overlap_greens(walk, trial, td, split=True)
+    assert torch.equal(g0, g1) and torch.equal(ld0, ld1)
+    rh, rl = kernels.split_bf16(g0.reshape(2 * nw, -1))
+    assert hi.shape == rh.shape
+    assert torch.equal(hi, rh) and torch.equal(lo, rl)
+
+
+def test_reconfigure_gathers_greens_planes(gpu_device):
+    """After the comb the state's bf16 planes are those of its gathered greens."""
+    import torch
+    from ad_afqmc_prototype_amd import kernels
+    from ad_afqmc_prototype_amd.meas.rhf import make_rhf_meas_ops
+    from ad_afqmc_prototype_amd.prop.afqmc import init_prop_state
+    from ad_afqmc_prototype_amd.prop.blocks import reconfigure
+    from ad_afqmc_prototype_amd.trial.rhf import make_rhf_trial_ops
+    p = make_pair(0, 24, 6, 96, device=gpu_device)
+    m_ops = make_rhf_meas_ops(p.g["sys"], precision="bf16x3")
+    st = init_prop_state(sys=p.g["sys"], n_walkers=32, seed=3, ham_data=p.g["ham"],
+                         trial_ops_=make_rhf_trial_ops(p.g["sys"]), trial_data=p.g["trial"], meas_ops=m_ops)
+    gen = torch.Generator(device=gpu_device).manual_seed(4)
+    st = st._replace(weights=torch.rand(32, dtype=torch.float64, device=gpu_device, generator=gen) + 0.1)
+    st2, idx = reconfigure(st, sys=p.g["sys"], trial_data=p.g["trial"], meas_ops_=m_ops, zeta=0.3)
+    rh, rl = kernels.split_bf16(st2.greens.greens.reshape(64, -1))
+    assert torch.equal(st2.greens.split[0], rh) and torch.equal(st2.greens.split[1], rl)
