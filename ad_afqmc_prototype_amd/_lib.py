@@ -25,6 +25,8 @@ _SIGNATURES = {
                      c_ll, c_int, c_vp],
     "afq_gemm_bf16x3_nt": [c_int, c_int, c_int, ctypes.c_float, c_vp, c_vp, c_ll, c_vp, c_vp, c_ll, c_vp,
                            c_ll, c_int, c_vp],
+    "afq_gemm_bf16x3_nt_splitk": [c_int, c_int, c_int, ctypes.c_float, c_vp, c_vp, c_ll, c_vp, c_vp, c_ll, c_vp,
+                                  c_ll, c_int, c_int, c_vp, c_vp],
     "afq_split_bf16": [c_int, c_int, c_vp, c_ll, c_vp, c_vp, c_ll, c_vp],
     "afq_convert_f32": [c_int, c_int, c_vp, c_ll, c_vp, c_ll, c_vp],
     "afq_trotter_f32": [c_int, c_int, c_int, c_int, c_int, c_int, c_double, c_vp, c_vp, c_vp, c_vp,

@@ -30,12 +30,23 @@ constexpr int GROUP_M = 8;
 
 // SYM: the columns are symmetric-VHS block entries and the tile is stored
 // expanded (afq::sym_store); C is then float, ldc unused.
+// Split K: blockIdx.y = kz takes k in [kz kc, (kz + 1) kc) (kc a multiple of
+// 32) and writes its partial product to C + kz c_stride.
 template <typename TOut, bool SYM = false>
 __global__ __launch_bounds__(THREADS, 2) void gemm_bf16x3_nt_kernel(
     int M, int N, int K, float alpha, const uint16_t* __restrict__ Ahi,
     const uint16_t* __restrict__ Alo, long long lda, const uint16_t* __restrict__ Bhi,
     const uint16_t* __restrict__ Blo, long long ldb, TOut* __restrict__ C, long long ldc,
-    int tiles_m, int tiles_n, afq::sym_dst sd = afq::sym_dst{0, 0, 0}) {
+    int tiles_m, int tiles_n, afq::sym_dst sd = afq::sym_dst{0, 0, 0}, int kc = 0, long long c_stride = 0) {
+    if (gridDim.y > 1) {
+        const int k_begin = blockIdx.y * kc;
+        K = min(K - k_begin, kc);
+        Ahi += k_begin;
+        Alo += k_begin;
+        Bhi += k_begin;
+        Blo += k_begin;
+        C += blockIdx.y * c_stride;
+    }
     __shared__ __attribute__((aligned(16))) uint16_t S[2][4][BM * PITCH];
 
     const int total = tiles_m * tiles_n;
@@ -344,6 +355,22 @@ __global__ __launch_bounds__(THREADS2, 1) void gemm_bf16x3_nt_256_kernel(
         }
 }
 
+// C = alpha * sum_kz part[kz] (kz in order), the split-K partials of M x N
+__global__ void splitk_reduce_kernel(int M, int N, int splits, const float* __restrict__ part, long long stride,
+                                     float alpha, void* __restrict__ C, long long ldc, int c_fp64) {
+    const long long total = (long long)M * N;
+    for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < total;
+         e += (long long)gridDim.x * blockDim.x) {
+        double acc = 0.0;
+        for (int z = 0; z < splits; ++z) acc += (double)part[z * stride + e];
+        const long long r = e / N, c = e - r * N;
+        if (c_fp64)
+            ((double*)C)[r * ldc + c] = (double)alpha * acc;
+        else
+            ((float*)C)[r * ldc + c] = (float)((double)alpha * acc);
+    }
+}
+
 __global__ void split_bf16_kernel(int rows, int cols, const double* __restrict__ src,
                                   long long ld_src, uint16_t* __restrict__ hi,
                                   uint16_t* __restrict__ lo, long long ld_dst) {
@@ -414,6 +441,33 @@ extern "C" int afq_gemm_bf16x3_nt(int M, int N, int K, float alpha, const unsign
                            M, N, K, alpha, A_hi, A_lo, lda, B_hi, B_lo, ldb, (float*)C, ldc,
                            tiles_m, tiles_n);
     return afq::check_launch("afq_gemm_bf16x3_nt");
+}
+
+extern "C" int afq_gemm_bf16x3_nt_splitk(int M, int N, int K, float alpha, const unsigned short* A_hi,
+                                         const unsigned short* A_lo, long long lda, const unsigned short* B_hi,
+                                         const unsigned short* B_lo, long long ldb, void* C, long long ldc,
+                                         int c_fp64, int splits, float* workspace, void* stream) {
+    AFQ_REQUIRE(M >= 0 && N >= 0 && K >= 0, "afq_gemm_bf16x3_nt_splitk: negative size");
+    if (M == 0 || N == 0) return 0;
+    AFQ_REQUIRE(splits >= 1 && splits <= 16, "afq_gemm_bf16x3_nt_splitk: splits=%d not in 1..16", splits);
+    AFQ_REQUIRE(A_hi && A_lo && B_hi && B_lo && C && workspace, "afq_gemm_bf16x3_nt_splitk: null pointer");
+    AFQ_REQUIRE(K % 8 == 0 && lda % 8 == 0 && ldb % 8 == 0,
+                "afq_gemm_bf16x3_nt_splitk: K=%d, lda and ldb must be multiples of 8", K);
+    AFQ_REQUIRE(lda >= K && ldb >= K && ldc >= N, "afq_gemm_bf16x3_nt_splitk: leading dimension too small");
+    const int tiles_m = afq::ceil_div(M, BM);
+    const int tiles_n = afq::ceil_div(N, BN);
+    AFQ_REQUIRE((long long)tiles_m * tiles_n < (1LL << 31), "afq_gemm_bf16x3_nt_splitk: problem too large");
+    // K chunks of whole 32-wide k-steps; no empty chunk
+    const int kc = afq::round_up(afq::ceil_div(K, splits), 32);
+    const int s = K == 0 ? 1 : afq::ceil_div(K, kc);
+    const long long stride = (long long)M * N;
+    hipStream_t st = (hipStream_t)stream;
+    hipLaunchKernelGGL((gemm_bf16x3_nt_kernel<float, false>), dim3((unsigned)(tiles_m * tiles_n), (unsigned)s),
+                       dim3(THREADS), 0, st, M, N, K, 1.0f, A_hi, A_lo, lda, B_hi, B_lo, ldb, workspace, (long long)N,
+                       tiles_m, tiles_n, afq::sym_dst{0, 0, 0}, kc, stride);
+    hipLaunchKernelGGL(splitk_reduce_kernel, dim3(grid_for(stride)), dim3(256), 0, st, M, N, s, workspace, stride,
+                       alpha, C, ldc, c_fp64);
+    return afq::check_launch("afq_gemm_bf16x3_nt_splitk");
 }
 
 extern "C" int afq_vhs_sym(int rows, int n, int ldv, int k, int arith, const void* a_hi, const void* a_lo,

@@ -203,16 +203,37 @@ def convert_f32(x: torch.Tensor, ld: int | None = None):
     return out
 
 
-def gemm_bf16x3_nt(a_hi, a_lo, b_hi, b_lo, C, alpha=1.0, tag=None):
-    """C (M, N) = alpha * (Ahi Bhi^T + Ahi Blo^T + Alo Bhi^T); planes from split_bf16."""
+def splitk_for(M: int, N: int, K: int) -> int:
+    """K chunks for the bf16x3 GEMM: problems that the 256 x 256 tiles fill run
+    whole; otherwise enough chunks (at most 4, each >= 512 deep) to give the
+    128 x 128 tile grid about four workgroups per CU (256 CUs)."""
+    if ((M + 255) // 256) * ((N + 255) // 256) >= 512:
+        return 1
+    tiles = ((M + 127) // 128) * ((N + 127) // 128)
+    return max(1, min(4, 1024 // tiles, K // 512))
+
+
+def gemm_bf16x3_nt(a_hi, a_lo, b_hi, b_lo, C, alpha=1.0, tag=None, splits=None):
+    """C (M, N) = alpha * (Ahi Bhi^T + Ahi Blo^T + Alo Bhi^T); planes from split_bf16.
+    ``splits`` > 1 cuts K into that many chunks run side by side (fp32
+    partials in a workspace, summed in chunk order in fp64); None picks it
+    from the shape (splitk_for)."""
     M, K = a_hi.shape
     N = b_hi.shape[0]
+    if splits is None:
+        splits = splitk_for(M, N, K)
     for t, nm, sh in ((a_hi, "a_hi", (M, K)), (a_lo, "a_lo", (M, K)), (b_hi, "b_hi", (N, K)),
                       (b_lo, "b_lo", (N, K))):
         _check(t, torch.int16, sh, nm, a_hi.device)
     if C.dtype not in (f32, f64):
         raise TypeError("C must be float32 or float64")
     _check(C, C.dtype, (M, N), "C", a_hi.device)
+    if splits > 1:
+        ws = torch.empty((splits, M, N), dtype=f32, device=a_hi.device)
+        _call("afq_gemm_bf16x3_nt_splitk", a_hi, M, N, K, float(alpha), a_hi.data_ptr(), a_lo.data_ptr(), K,
+              b_hi.data_ptr(), b_lo.data_ptr(), K, C.data_ptr(), N, int(C.dtype == f64), int(splits),
+              ws.data_ptr(), _stream(a_hi), tag=tag)
+        return C
     _call("afq_gemm_bf16x3_nt", a_hi, M, N, K, float(alpha), a_hi.data_ptr(), a_lo.data_ptr(), K,
           b_hi.data_ptr(), b_lo.data_ptr(), K, C.data_ptr(), N, int(C.dtype == f64), _stream(a_hi),
           tag=tag)

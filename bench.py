@@ -399,6 +399,7 @@ def kernel_table(ktimes, n, o, g, nw, nspin, rhf, precision):
         "afq_force_bias": (fl["force_bias"] * nw, "fp64"),
         "afq_sgemm_nt:force_bias": (fl["force_bias"] * nw, "fp32"),
         "afq_gemm_bf16x3_nt:force_bias": (fl["force_bias"] * nw, "bf16x3"),
+        "afq_gemm_bf16x3_nt_splitk:force_bias": (fl["force_bias"] * nw, "bf16x3"),
         "afq_trotter": (taylor, "fp64"),
         "afq_trotter_f32": (taylor, "fp32"),
         "afq_trotter_bf16x3": (taylor, "bf16x3"),

@@ -63,6 +63,19 @@ int afq_gemm_bf16x3_nt(int M, int N, int K, float alpha, const unsigned short* A
                        int c_fp64, void* stream);
 
 /*
+ * afq_gemm_bf16x3_nt with the K range cut into `splits` chunks (whole 32-wide
+ * k-steps) that run side by side: for problems of few output tiles, which then
+ * fill the GPU.  Each chunk writes its fp32 partial product to workspace
+ * (splits x M x N floats, caller-owned), and a second kernel forms
+ * C = alpha * (sum of the partials in chunk order, in fp64): deterministic.
+ * Same operand rules as afq_gemm_bf16x3_nt; 1 <= splits <= 16.
+ */
+int afq_gemm_bf16x3_nt_splitk(int M, int N, int K, float alpha, const unsigned short* A_hi,
+                              const unsigned short* A_lo, long long lda, const unsigned short* B_hi,
+                              const unsigned short* B_lo, long long ldb, void* C, long long ldc, int c_fp64,
+                              int splits, float* workspace, void* stream);
+
+/*
  * x (rows x cols, float64, row pitch ld_src) -> hi = bf16(x), lo = bf16(x - hi)
  * (rows x ld_dst, zero in columns >= cols).  Operand preparation for
  * afq_gemm_bf16x3_nt; no counterpart in the reference (it computes in fp64).

@@ -330,3 +330,30 @@ def test_reconfigure_gathers_greens_planes(gpu_device):
     st2, idx = reconfigure(st, sys=p.g["sys"], trial_data=p.g["trial"], meas_ops_=m_ops, zeta=0.3)
     rh, rl = kernels.split_bf16(st2.greens.greens.reshape(64, -1))
     assert torch.equal(st2.greens.split[0], rh) and torch.equal(st2.greens.split[1], rl)
+
+
+@pytest.mark.parametrize("M,N,K", [(4096, 1000, 6400), (37, 300, 1000), (130, 257, 8), (256, 96, 2400)])
+def test_bf16x3_gemm_split_k(gpu_device, M, N, K):
+    """Split K: every chunk count gives the unsplit product to fp32 rounding of
+    the partial sums, repeated calls are bit-identical (fixed-order
+    reduction), and K shorter than one chunk per split runs as fewer chunks."""
+    import torch
+    from ad_afqmc_prototype_amd import kernels
+    print(f"split-K bf16x3 M={M} N={N} K={K}", flush=True)
+    g = torch.Generator(device="cpu").manual_seed(M + K)
+    A = torch.randn((M, K), dtype=torch.float64, generator=g).to(gpu_device)
+    B = torch.randn((N, K), dtype=torch.float64, generator=g).to(gpu_device)
+    ah, al = kernels.split_bf16(A)
+    bh, bl = kernels.split_bf16(B)
+    base = torch.empty((M, N), dtype=torch.float64, device=gpu_device)
+    kernels.gemm_bf16x3_nt(ah, al, bh, bl, base, alpha=0.5, splits=1)
+    ref = 0.5 * (A @ B.T)
+    scale = float(ref.abs().max())
+    for s in (2, 3, 4, 5):
+        c1 = torch.empty_like(base)
+        c2 = torch.empty_like(base)
+        kernels.gemm_bf16x3_nt(ah, al, bh, bl, c1, alpha=0.5, splits=s)
+        kernels.gemm_bf16x3_nt(ah, al, bh, bl, c2, alpha=0.5, splits=s)
+        assert torch.equal(c1, c2)
+        assert float((c1 - base).abs().max()) <= 2e-6 * scale * max(1.0, K / 1000) ** 0.5
+        assert float((c1 - ref).abs().max()) <= 2e-5 * scale
