@@ -105,6 +105,34 @@ __device__ __forceinline__ void load_row8(rsrc_t r, int ld, int n, int row, int 
     x[4] = c.x; x[5] = c.y; x[6] = c.z; x[7] = c.w;
 }
 
+__device__ __forceinline__ float load_f1(rsrc_t r, unsigned off) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, off, 0, 0));
+}
+
+// The A fragment V[16 bi + lr][32 kb + 8 q .. + 7] of a symmetric V stored as
+// its upper 16 x 16 blocks (block (r, c), r <= c, numbered row by row, entry
+// (pi, qi) at 16 pi + qi; afq_vhs_sym's column order): a row of an upper
+// block (two 16-byte loads), or, below the diagonal, a column of the stored
+// transposed block (eight 4-byte loads 64 bytes apart, each coalesced over the
+// 16 lanes of a column); zero past the blocks.  Nothing here waits on the data.
+__device__ __forceinline__ void load_frag_sym(rsrc_t r, int nb, int bi, int kb, int lr, int q, float (&x)[8]) {
+    const int bj = 2 * kb + (q >> 1);
+    const int cb = 8 * (q & 1);
+    if (bi > bj) {
+        const int blk = bj * nb - ((bj * (bj - 1)) >> 1) + (bi - bj);
+        const unsigned base = (bi < nb) ? (unsigned)(blk * 256 + cb * 16 + lr) * 4u : kOutside;
+#pragma unroll
+        for (int j = 0; j < 8; ++j) x[j] = load_f1(r, base + 64u * j);
+    } else {
+        const int blk = bi * nb - ((bi * (bi - 1)) >> 1) + (bj - bi);
+        const unsigned base = (bj < nb) ? (unsigned)(blk * 256 + lr * 16 + cb) * 4u : kOutside;
+        const float4 a = load_f4(r, base);
+        const float4 c = load_f4(r, base + 16u);
+        x[0] = a.x; x[1] = a.y; x[2] = a.z; x[3] = a.w;
+        x[4] = c.x; x[5] = c.y; x[6] = c.z; x[7] = c.w;
+    }
+}
+
 // x through an empty asm: the compiler recomputes what depends on it instead
 // of keeping (and spilling) values from before a long loop
 __device__ __forceinline__ int opaque(int x) {
@@ -121,7 +149,9 @@ __device__ __forceinline__ split8 split_minus_identity(const float (&x)[8], int 
     return split_f32x8(y);
 }
 
-template <int NL>
+// SYMV: vhs holds each walker's V as its upper 16 x 16 blocks (afq_vhs_sym's
+// column order, rows of nb (nb + 1) / 2 * 256 floats) instead of (n, ldv) rows.
+template <int NL, bool SYMV = false>
 __global__ __launch_bounds__(512, 1) void trotter_bf16x3_kernel(
     int nw, int n, int o, int nspin, int nct, int ldv, int n_terms, double sqrt_dt,
     const float* __restrict__ E, const float* __restrict__ vhs, long long vplane,
@@ -152,8 +182,10 @@ __global__ __launch_bounds__(512, 1) void trotter_bf16x3_kernel(
     const rsrc_t Wr = make_rsrc((s == 0 ? win_up : win_dn) + woff, (unsigned)n * o * 16u);
     const unsigned mat_bytes = (unsigned)n * ldv * 4u;
     const rsrc_t Er = make_rsrc(E, mat_bytes);
-    const rsrc_t Vr = make_rsrc(vhs + (long long)w * n * ldv, mat_bytes);
-    const rsrc_t Vi = make_rsrc(vhs + vplane + (long long)w * n * ldv, mat_bytes);
+    const int nbv = NP / 16;
+    const long long vrow = SYMV ? (long long)nbv * (nbv + 1) / 2 * 256 : (long long)n * ldv;
+    const rsrc_t Vr = make_rsrc(vhs + (long long)w * vrow, (unsigned)vrow * 4u);
+    const rsrc_t Vi = make_rsrc(vhs + vplane + (long long)w * vrow, (unsigned)vrow * 4u);
 
     // blockDim.x = 64 * nk: one wave per pair of 16-row tiles
     const int tid = threadIdx.x;
@@ -272,8 +304,13 @@ __global__ __launch_bounds__(512, 1) void trotter_bf16x3_kernel(
 #pragma unroll
             for (int l = 0; l < NL; ++l) {
                 const int k0 = (KR + l) * 32 + kq;
-                load_row8(Vr, ldv, n, row0 + 16 * t, k0, xl[t][l][0]);
-                load_row8(Vi, ldv, n, row0 + 16 * t, k0, xl[t][l][1]);
+                if constexpr (SYMV) {
+                    load_frag_sym(Vr, nbv, wave * RT + t, KR + l, lr, q, xl[t][l][0]);
+                    load_frag_sym(Vi, nbv, wave * RT + t, KR + l, lr, q, xl[t][l][1]);
+                } else {
+                    load_row8(Vr, ldv, n, row0 + 16 * t, k0, xl[t][l][0]);
+                    load_row8(Vi, ldv, n, row0 + 16 * t, k0, xl[t][l][1]);
+                }
             }
 #pragma unroll
         for (int t = 0; t < RT; ++t)
@@ -296,8 +333,13 @@ __global__ __launch_bounds__(512, 1) void trotter_bf16x3_kernel(
 #pragma unroll
             for (int kb = 0; kb < KR; ++kb) {
                 const int k0 = kb * 32 + kq;
-                load_row8(Vr, ldv, n, row0 + 16 * t, k0, xv[t][kb][0]);
-                load_row8(Vi, ldv, n, row0 + 16 * t, k0, xv[t][kb][1]);
+                if constexpr (SYMV) {
+                    load_frag_sym(Vr, nbv, wave * RT + t, kb, lr, q, xv[t][kb][0]);
+                    load_frag_sym(Vi, nbv, wave * RT + t, kb, lr, q, xv[t][kb][1]);
+                } else {
+                    load_row8(Vr, ldv, n, row0 + 16 * t, k0, xv[t][kb][0]);
+                    load_row8(Vi, ldv, n, row0 + 16 * t, k0, xv[t][kb][1]);
+                }
             }
 #pragma unroll
         for (int t = 0; t < RT; ++t)
@@ -427,7 +469,7 @@ size_t bf16x3_lds(int n, int nl) {
     return (size_t)4 * 16 * (nk * 32 + 8) * 2 + (size_t)nwave * RT * nl * 4 * 64 * 16;
 }
 
-template <int NL>
+template <int NL, bool SYMV>
 int launch_bf16x3(int nw, int n, int o, int nspin, int ldv, int n_terms, double sqrt_dt,
                   const float* E, const float* vhs, const double2* wu, const double2* wd, double2* ou,
                   double2* od, hipStream_t st) {
@@ -436,22 +478,32 @@ int launch_bf16x3(int nw, int n, int o, int nspin, int ldv, int n_terms, double 
     const int nwave = afq::ceil_div(NP / 16, RT);
     const size_t lds = bf16x3_lds(n, NL);
     const int group_map = (nw % 8 == 0) ? 1 : 0;
-    if (lds > 64 * 1024) afq::allow_full_lds((const void*)trotter_bf16x3_kernel<NL>);
-    hipLaunchKernelGGL((trotter_bf16x3_kernel<NL>), dim3((unsigned)(nw * nspin * nct)), dim3(64 * nwave),
-                       lds, st, nw, n, o, nspin, nct, ldv, n_terms, sqrt_dt, E, vhs,
-                       (long long)nw * n * ldv, wu, wd, ou, od, group_map);
-    return afq::check_launch("afq_trotter_bf16x3");
+    const int nb = NP / 16;
+    const long long vplane = SYMV ? (long long)nw * nb * (nb + 1) / 2 * 256 : (long long)nw * n * ldv;
+    if (lds > 64 * 1024) afq::allow_full_lds((const void*)trotter_bf16x3_kernel<NL, SYMV>);
+    hipLaunchKernelGGL((trotter_bf16x3_kernel<NL, SYMV>), dim3((unsigned)(nw * nspin * nct)), dim3(64 * nwave),
+                       lds, st, nw, n, o, nspin, nct, ldv, n_terms, sqrt_dt, E, vhs, vplane, wu, wd, ou, od,
+                       group_map);
+    return afq::check_launch(SYMV ? "afq_trotter_bf16x3_sym" : "afq_trotter_bf16x3");
 }
 
-}  // namespace
+template <bool SYMV>
+int launch_bf16x3_nl(int nl, int nw, int n, int o, int nspin, int ldv, int n_terms, double sqrt_dt,
+                     const float* E, const float* vhs, const double* win_up, const double* win_dn,
+                     double* wout_up, double* wout_dn, hipStream_t st) {
+    const double2* wu = (const double2*)win_up;
+    const double2* wd = (const double2*)win_dn;
+    double2* ou = (double2*)wout_up;
+    double2* od = (double2*)wout_dn;
+    if (nl == 0) return launch_bf16x3<0, SYMV>(nw, n, o, nspin, ldv, n_terms, sqrt_dt, E, vhs, wu, wd, ou, od, st);
+    if (nl == 1) return launch_bf16x3<1, SYMV>(nw, n, o, nspin, ldv, n_terms, sqrt_dt, E, vhs, wu, wd, ou, od, st);
+    return launch_bf16x3<2, SYMV>(nw, n, o, nspin, ldv, n_terms, sqrt_dt, E, vhs, wu, wd, ou, od, st);
+}
 
-// Declared in afqmc_hip.h.  Sizes beyond the on-chip capacity of the split
-// kernel (n > 224) go to the fp32 step, which is more accurate.
-extern "C" int afq_trotter_bf16x3(int nw, int n, int o, int nspin, int ldv, int n_terms,
-                                  double sqrt_dt, const float* exp_h1_half, const float* vhs,
-                                  const double* win_up, const double* win_dn, double* wout_up,
-                                  double* wout_dn, void* stream) {
-    const char* name = "afq_trotter_bf16x3";
+// the checks both entry points share; the k-blocks of V kept in LDS in *nl
+int check_bf16x3(const char* name, int nw, int n, int o, int nspin, int ldv, int n_terms,
+                 const float* exp_h1_half, const float* vhs, const double* win_up, const double* win_dn,
+                 double* wout_up, double* wout_dn, int* nl) {
     AFQ_REQUIRE(nw > 0 && n > 0 && o > 0 && o <= n, "%s: bad sizes nw=%d n=%d o=%d", name, nw, n, o);
     AFQ_REQUIRE(nspin == 1 || nspin == 2, "%s: nspin must be 1 or 2", name);
     AFQ_REQUIRE(ldv >= n && ldv % 4 == 0, "%s: ldv=%d must be >= n and a multiple of 4", name, ldv);
@@ -462,18 +514,40 @@ extern "C" int afq_trotter_bf16x3(int nw, int n, int o, int nspin, int ldv, int 
     AFQ_REQUIRE((long long)nw * nspin * afq::ceil_div(o, 16) < (1LL << 31), "%s: too many walkers", name);
     const int NP = afq::round_up(n, 16);
     const int nk = afq::ceil_div(NP, 32);
-    const int nl = nk > KR ? nk - KR : 0;
+    *nl = nk > KR ? nk - KR : 0;
+    return 0;
+}
+
+}  // namespace
+
+// Declared in afqmc_hip.h.  Sizes beyond the on-chip capacity of the split
+// kernel (n > 224) go to the fp32 step, which is more accurate.
+extern "C" int afq_trotter_bf16x3(int nw, int n, int o, int nspin, int ldv, int n_terms,
+                                  double sqrt_dt, const float* exp_h1_half, const float* vhs,
+                                  const double* win_up, const double* win_dn, double* wout_up,
+                                  double* wout_dn, void* stream) {
+    int nl = 0;
+    const int rc = check_bf16x3("afq_trotter_bf16x3", nw, n, o, nspin, ldv, n_terms, exp_h1_half, vhs, win_up,
+                                win_dn, wout_up, wout_dn, &nl);
+    if (rc != 0) return rc;
     if (nl > NL_MAX || bf16x3_lds(n, nl) > 160 * 1024)
         return afq_trotter_f32(nw, n, o, nspin, ldv, n_terms, sqrt_dt, exp_h1_half, vhs, win_up, win_dn,
                                wout_up, wout_dn, stream);
-    const double2* wu = (const double2*)win_up;
-    const double2* wd = (const double2*)win_dn;
-    double2* ou = (double2*)wout_up;
-    double2* od = (double2*)wout_dn;
-    hipStream_t st = (hipStream_t)stream;
-    if (nl == 0)
-        return launch_bf16x3<0>(nw, n, o, nspin, ldv, n_terms, sqrt_dt, exp_h1_half, vhs, wu, wd, ou, od, st);
-    if (nl == 1)
-        return launch_bf16x3<1>(nw, n, o, nspin, ldv, n_terms, sqrt_dt, exp_h1_half, vhs, wu, wd, ou, od, st);
-    return launch_bf16x3<2>(nw, n, o, nspin, ldv, n_terms, sqrt_dt, exp_h1_half, vhs, wu, wd, ou, od, st);
+    return launch_bf16x3_nl<false>(nl, nw, n, o, nspin, ldv, n_terms, sqrt_dt, exp_h1_half, vhs, win_up, win_dn,
+                                   wout_up, wout_dn, (hipStream_t)stream);
+}
+
+// Declared in afqmc_hip.h: V as upper 16 x 16 blocks; n <= 224 only (there is
+// no fp32 kernel for this layout).
+extern "C" int afq_trotter_bf16x3_sym(int nw, int n, int o, int nspin, int ldv, int n_terms, double sqrt_dt,
+                                      const float* exp_h1_half, const float* vhs_sym, const double* win_up,
+                                      const double* win_dn, double* wout_up, double* wout_dn, void* stream) {
+    int nl = 0;
+    const int rc = check_bf16x3("afq_trotter_bf16x3_sym", nw, n, o, nspin, ldv, n_terms, exp_h1_half, vhs_sym,
+                                win_up, win_dn, wout_up, wout_dn, &nl);
+    if (rc != 0) return rc;
+    AFQ_REQUIRE(nl <= NL_MAX && bf16x3_lds(n, nl) <= 160 * 1024,
+                "afq_trotter_bf16x3_sym: n=%d exceeds the on-chip maximum of 224", n);
+    return launch_bf16x3_nl<true>(nl, nw, n, o, nspin, ldv, n_terms, sqrt_dt, exp_h1_half, vhs_sym, win_up, win_dn,
+                                  wout_up, wout_dn, (hipStream_t)stream);
 }

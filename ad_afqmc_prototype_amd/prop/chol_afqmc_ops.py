@@ -138,11 +138,21 @@ def make_chol_afqmc_ops(ham_data: ham_chol, walker_kind: str,
     def n_fields_() -> int:
         return nf
 
+    # bf16x3 (n <= 224): the Trotter reads V as the upper blocks themselves,
+    # so the contraction's output is used as it comes (0.58 of the bytes)
+    v_blocks = precision == "bf16x3" and n <= kernels.BF16X3_SYM_MAX_N
+
     def make_vhs(xbar: torch.Tensor) -> torch.Tensor:
-        """VHS planes (2, nw, n, ldv) in the step's precision, from the
-        symmetric contraction (upper 16 x 16 blocks, stored expanded)."""
+        """VHS planes in the step's precision, from the symmetric contraction
+        (upper 16 x 16 blocks): (2, nw, sym_blocks(n) * 256) upper blocks for the
+        bf16x3 Trotter, else (2, nw, n, ldv) rows (each entry stored twice)."""
         nwc = xbar.shape[1]
         flat = xbar.reshape(2 * nwc, nf)
+        if v_blocks:
+            out = torch.empty((2 * nwc, chol_hi.shape[0]), dtype=torch.float32, device=xbar.device)
+            xh, xl = kernels.split_bf16(flat)
+            kernels.gemm_bf16x3_nt(xh, xl, chol_hi, chol_lo, out, tag="vhs")
+            return out.view(2, nwc, -1)
         if precision == "fp64":
             out = kernels.vhs_sym(flat, chol_s, n)
         elif precision == "fp32":

@@ -403,6 +403,8 @@ def kernel_table(ktimes, n, o, g, nw, nspin, rhf, precision):
         "afq_trotter": (taylor, "fp64"),
         "afq_trotter_f32": (taylor, "fp32"),
         "afq_trotter_bf16x3": (taylor, "bf16x3"),
+        "afq_trotter_bf16x3_sym": (taylor, "bf16x3"),
+        "afq_gemm_bf16x3_nt:vhs": (fl["vhs"] * nw, "bf16x3"),
         "afq_overlap_greens": (fl["greens"] * nw, "fp64"),
         "afq_energy": (4.0 * g * o * o * n * nspin * nw, "fp64"),
         "afq_energy_f32": (4.0 * g * o * o * n * nspin * nw, "fp32"),

@@ -357,3 +357,38 @@ def test_bf16x3_gemm_split_k(gpu_device, M, N, K):
         assert torch.equal(c1, c2)
         assert float((c1 - base).abs().max()) <= 2e-6 * scale * max(1.0, K / 1000) ** 0.5
         assert float((c1 - ref).abs().max()) <= 2e-5 * scale
+
+
+@pytest.mark.parametrize("nw,n,o,nspin", [(8, 24, 6, 1), (5, 45, 11, 2), (3, 17, 5, 1), (16, 200, 32, 1),
+                                          (4, 224, 16, 2)])
+def test_trotter_bf16x3_upper_block_vhs_equals_full_rows(gpu_device, nw, n, o, nspin):
+    """The bf16x3 Trotter on V stored as upper 16 x 16 blocks equals the one on
+    full (n, ldv) rows bit for bit: the same contraction fills both layouts,
+    and the lower-block fragments read from the stored transposes are the
+    same floats the full rows hold."""
+    import torch
+    from ad_afqmc_prototype_amd import kernels
+    from ad_afqmc_prototype_amd.prop.chol_afqmc_ops import chol_sym_rows
+    print(f"upper-block Trotter nw={nw} n={n} o={o} nspin={nspin}", flush=True)
+    rng = np.random.default_rng(n + nw)
+    g = 40
+    a = rng.standard_normal((g, n, n)) * 0.05
+    chol = torch.tensor(0.5 * (a + a.transpose(0, 2, 1)), device=gpu_device)
+    x = torch.tensor(rng.standard_normal((2 * nw, g)), device=gpu_device)
+    xs = kernels.split_bf16(x)
+    cs = kernels.split_bf16(chol_sym_rows(chol))
+    v_full = kernels.vhs_sym(xs, cs, n).view(2, nw, n, kernels.ldv_for(n))
+    v_blk = torch.empty((2 * nw, cs[0].shape[0]), dtype=torch.float32, device=gpu_device)
+    kernels.gemm_bf16x3_nt(*xs, *cs, v_blk)
+    v_blk = v_blk.view(2, nw, -1)
+    ldv = kernels.ldv_for(n)
+    h = rng.standard_normal((n, n)) * 0.01
+    e = torch.zeros((n, ldv), dtype=torch.float32, device=gpu_device)
+    e[:, :n] = torch.tensor(np.eye(n) + 0.5 * (h + h.T), dtype=torch.float32, device=gpu_device)
+    ws = [torch.tensor(rng.standard_normal((nw, n, o)) + 1j * rng.standard_normal((nw, n, o)),
+                       device=gpu_device) for _ in range(nspin)]
+    walk = ws[0] if nspin == 1 else tuple(ws)
+    r_full = kernels.trotter(walk, v_full, e, 0.07, 6, arith="bf16x3")
+    r_blk = kernels.trotter(walk, v_blk, e, 0.07, 6, arith="bf16x3")
+    for a_, b_ in zip(r_full if nspin == 2 else (r_full,), r_blk if nspin == 2 else (r_blk,)):
+        assert torch.equal(a_, b_)
