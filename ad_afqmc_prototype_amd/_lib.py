@@ -33,8 +33,8 @@ _SIGNATURES = {
                         c_vp, c_vp, c_vp],
     "afq_trotter_bf16x3": [c_int, c_int, c_int, c_int, c_int, c_int, c_double, c_vp, c_vp, c_vp,
                            c_vp, c_vp, c_vp, c_vp],
-    "afq_trotter_bf16x3_sym": [c_int, c_int, c_int, c_int, c_int, c_int, c_double, c_vp, c_vp, c_vp,
-                           c_vp, c_vp, c_vp, c_vp],
+    "afq_trotter_bf16x3_sym": [c_int, c_int, c_int, c_int, c_int, c_double, c_vp, c_vp, c_vp, c_vp, c_vp,
+                               c_vp, c_vp],
     "afq_overlap_greens": [c_int, c_int, c_int, c_int, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp],
     "afq_overlap_greens_split": [c_int, c_int, c_int, c_int, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp,
                                  c_ll, c_vp],

@@ -149,8 +149,9 @@ __device__ __forceinline__ split8 split_minus_identity(const float (&x)[8], int 
     return split_f32x8(y);
 }
 
-// SYMV: vhs holds each walker's V as its upper 16 x 16 blocks (afq_vhs_sym's
-// column order, rows of nb (nb + 1) / 2 * 256 floats) instead of (n, ldv) rows.
+// SYMV: vhs holds each walker's V, and E holds exp(-dt h1 / 2), as upper 16 x 16
+// blocks (afq_vhs_sym's column order, nb (nb + 1) / 2 * 256 floats per matrix)
+// instead of (n, ldv) rows; both are symmetric.
 template <int NL, bool SYMV = false>
 __global__ __launch_bounds__(512, 1) void trotter_bf16x3_kernel(
     int nw, int n, int o, int nspin, int nct, int ldv, int n_terms, double sqrt_dt,
@@ -181,9 +182,9 @@ __global__ __launch_bounds__(512, 1) void trotter_bf16x3_kernel(
     double2* Wout = (s == 0 ? wout_up : wout_dn) + woff;
     const rsrc_t Wr = make_rsrc((s == 0 ? win_up : win_dn) + woff, (unsigned)n * o * 16u);
     const unsigned mat_bytes = (unsigned)n * ldv * 4u;
-    const rsrc_t Er = make_rsrc(E, mat_bytes);
     const int nbv = NP / 16;
     const long long vrow = SYMV ? (long long)nbv * (nbv + 1) / 2 * 256 : (long long)n * ldv;
+    const rsrc_t Er = make_rsrc(E, SYMV ? (unsigned)vrow * 4u : mat_bytes);
     const rsrc_t Vr = make_rsrc(vhs + (long long)w * vrow, (unsigned)vrow * 4u);
     const rsrc_t Vi = make_rsrc(vhs + vplane + (long long)w * vrow, (unsigned)vrow * 4u);
 
@@ -205,12 +206,17 @@ __global__ __launch_bounds__(512, 1) void trotter_bf16x3_kernel(
         return load_d2(Wr, j < o ? (unsigned)(p * o + j) * 16u : kOutside);
     };
     // all E fragments of this wave's rows, as raw fp32 (one round trip)
+    // (SYMV: E is given as its upper blocks too, and read like V)
     auto load_e = [&](float (&er)[RT][KMAX][8], int r0, int k8) {
 #pragma unroll
         for (int t = 0; t < RT; ++t)
 #pragma unroll
-            for (int kb = 0; kb < KMAX; ++kb)
-                load_row8(Er, ldv, n, r0 + 16 * t, kb * 32 + k8, er[t][kb]);
+            for (int kb = 0; kb < KMAX; ++kb) {
+                if constexpr (SYMV)
+                    load_frag_sym(Er, nbv, (r0 >> 4) + t, kb, r0 & 15, k8 >> 3, er[t][kb]);
+                else
+                    load_row8(Er, ldv, n, r0 + 16 * t, kb * 32 + k8, er[t][kb]);
+            }
     };
     // acc += (E - I) . T
     auto e_product = [&](f32x4 (&acc)[RT][2], const float (&er)[RT][KMAX][8], int r0, int k8) {
@@ -537,17 +543,18 @@ extern "C" int afq_trotter_bf16x3(int nw, int n, int o, int nspin, int ldv, int 
                                    wout_up, wout_dn, (hipStream_t)stream);
 }
 
-// Declared in afqmc_hip.h: V as upper 16 x 16 blocks; n <= 224 only (there is
-// no fp32 kernel for this layout).
-extern "C" int afq_trotter_bf16x3_sym(int nw, int n, int o, int nspin, int ldv, int n_terms, double sqrt_dt,
-                                      const float* exp_h1_half, const float* vhs_sym, const double* win_up,
+// Declared in afqmc_hip.h: V and E as upper 16 x 16 blocks; n <= 224 only
+// (there is no fp32 kernel for this layout).
+extern "C" int afq_trotter_bf16x3_sym(int nw, int n, int o, int nspin, int n_terms, double sqrt_dt,
+                                      const float* exp_h1_half_sym, const float* vhs_sym, const double* win_up,
                                       const double* win_dn, double* wout_up, double* wout_dn, void* stream) {
     int nl = 0;
-    const int rc = check_bf16x3("afq_trotter_bf16x3_sym", nw, n, o, nspin, ldv, n_terms, exp_h1_half, vhs_sym,
-                                win_up, win_dn, wout_up, wout_dn, &nl);
+    const int ldv = afq::round_up(n, 16);  // unused by this layout
+    const int rc = check_bf16x3("afq_trotter_bf16x3_sym", nw, n, o, nspin, ldv, n_terms, exp_h1_half_sym,
+                                vhs_sym, win_up, win_dn, wout_up, wout_dn, &nl);
     if (rc != 0) return rc;
     AFQ_REQUIRE(nl <= NL_MAX && bf16x3_lds(n, nl) <= 160 * 1024,
                 "afq_trotter_bf16x3_sym: n=%d exceeds the on-chip maximum of 224", n);
-    return launch_bf16x3_nl<true>(nl, nw, n, o, nspin, ldv, n_terms, sqrt_dt, exp_h1_half, vhs_sym, win_up, win_dn,
-                                  wout_up, wout_dn, (hipStream_t)stream);
+    return launch_bf16x3_nl<true>(nl, nw, n, o, nspin, ldv, n_terms, sqrt_dt, exp_h1_half_sym, vhs_sym, win_up,
+                                  win_dn, wout_up, wout_dn, (hipStream_t)stream);
 }

@@ -328,8 +328,9 @@ def trotter(walkers, vhs_planes, exp_h1_half_pad, sqrt_dt, n_terms, out=None, ar
     """Fused Trotter step; fp64 when vhs / exp_h1_half are float64.  With
     float32 operands ``arith`` picks the products: "fp32" (default) or
     "bf16x3" (split-bf16 MFMAs, fp32 sums).  Walkers are complex128 either way.
-    bf16x3 also takes V as upper 16 x 16 blocks: vhs_planes (2, nw,
-    sym_blocks(n) * 256) float32 (n <= 224)."""
+    bf16x3 also takes V and E as upper 16 x 16 blocks: vhs_planes (2, nw,
+    sym_blocks(n) * 256) and exp_h1_half_pad (sym_blocks(n) * 256,) float32
+    (n <= 224)."""
     wu, wd = (walkers if isinstance(walkers, tuple) else (walkers, None))
     nspin = 1 if wd is None else 2
     nw, n, o = wu.shape
@@ -345,11 +346,11 @@ def trotter(walkers, vhs_planes, exp_h1_half_pad, sqrt_dt, n_terms, out=None, ar
         if n > BF16X3_SYM_MAX_N:
             raise ValueError(f"upper-block VHS planes need n <= {BF16X3_SYM_MAX_N}, got n={n}")
         _check(vhs_planes, f32, (2, nw, sym_blocks(n) * 256), "vhs", dev)
-        _check(exp_h1_half_pad, f32, (n, ldv), "exp_h1_half", dev)
+        _check(exp_h1_half_pad, f32, (sym_blocks(n) * 256,), "exp_h1_half (upper blocks)", dev)
         if out is None:
             out = (torch.empty_like(wu), torch.empty_like(wd)) if nspin == 2 else torch.empty_like(wu)
         ou, od = (out if isinstance(out, tuple) else (out, None))
-        _call("afq_trotter_bf16x3_sym", wu, nw, n, o, nspin, ldv, int(n_terms), float(sqrt_dt),
+        _call("afq_trotter_bf16x3_sym", wu, nw, n, o, nspin, int(n_terms), float(sqrt_dt),
               exp_h1_half_pad.data_ptr(), vhs_planes.data_ptr(), wu.data_ptr(), _ptr(wd), ou.data_ptr(),
               _ptr(od), _stream(wu))
         return out

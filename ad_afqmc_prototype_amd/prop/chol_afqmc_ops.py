@@ -30,6 +30,7 @@ class chol_afqmc_ctx:
     h0_prop: complex
     exp_h1_half_pad: torch.Tensor   # (n, ldv) device layout of exp_h1_half
     exp_h1_half_pad32: torch.Tensor = None  # float32 copy for the mixed step
+    exp_h1_half_sym32: torch.Tensor = None  # float32 upper 16 x 16 blocks (bf16x3 Trotter)
 
 
 class chol_afqmc_ops(NamedTuple):
@@ -64,12 +65,17 @@ def _build_prop_ctx(ham_data: ham_chol, trial_rdm1, dt: float) -> chol_afqmc_ctx
     v1m = torch.einsum("g,gik->ik", mf_r, chol)
     h1_eff = ham_data.h1 - v0m - v1m
     exp_h1_half = torch.linalg.matrix_exp(-0.5 * dt * h1_eff)
+    # the exponential of a symmetric matrix, made exactly symmetric (the
+    # rounding of the matrix products leaves ~1e-16 asymmetry), so that every
+    # layout that stores one triangle holds the same matrix
+    exp_h1_half = 0.5 * (exp_h1_half + exp_h1_half.T)
     ldv = kernels.ldv_for(n)
     pad = torch.zeros((n, ldv), dtype=torch.float64, device=chol.device)
     pad[:, :n] = exp_h1_half
     return chol_afqmc_ctx(dt=float(dt), sqrt_dt=float(dt) ** 0.5, exp_h1_half=exp_h1_half,
                           mf_shifts=mf.contiguous(), h0_prop=h0_prop, exp_h1_half_pad=pad,
-                          exp_h1_half_pad32=pad.to(torch.float32))
+                          exp_h1_half_pad32=pad.to(torch.float32),
+                          exp_h1_half_sym32=upper_blocks(exp_h1_half).to(torch.float32))
 
 
 def chol_transposed(chol: torch.Tensor) -> torch.Tensor:
@@ -96,6 +102,12 @@ def chol_sym_rows(chol: torch.Tensor) -> torch.Tensor:
     ok = (p < n) & (q < n)
     out[ok] = chol[:, p[ok], q[ok]].T
     return out
+
+
+def upper_blocks(mat: torch.Tensor) -> torch.Tensor:
+    """The upper 16 x 16 blocks of a symmetric n x n matrix, flat
+    (sym_blocks(n) * 256,) in the order of chol_sym_rows, zero past n."""
+    return chol_sym_rows(mat[None]).reshape(-1)
 
 
 def _chunks(nw: int, n_chunks: int):
@@ -162,6 +174,8 @@ def make_chol_afqmc_ops(ham_data: ham_chol, walker_kind: str,
         return out.view(2, nwc, n, ldv)
 
     def e_half(ctx: chol_afqmc_ctx) -> torch.Tensor:
+        if v_blocks:
+            return ctx.exp_h1_half_sym32
         return ctx.exp_h1_half_pad if precision == "fp64" else ctx.exp_h1_half_pad32
 
     def build_prop_ctx(trial_rdm1, dt: float) -> chol_afqmc_ctx:

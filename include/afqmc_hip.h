@@ -199,16 +199,17 @@ int afq_trotter_bf16x3(int nw, int n, int o, int nspin, int ldv, int n_terms, do
                        double* wout_up, double* wout_dn, void* stream);
 
 /*
- * afq_trotter_bf16x3 with V given as its upper 16 x 16 blocks: vhs_sym holds
- * two planes (real, imaginary) of nw rows of nb (nb + 1) / 2 * 256 floats,
+ * afq_trotter_bf16x3 with V and E = exp(-dt h1 / 2) (both symmetric) given as
+ * their upper 16 x 16 blocks: nb (nb + 1) / 2 * 256 floats per matrix,
  * nb = ceil(n / 16), in afq_vhs_sym's column order (block (r, c), r <= c,
  * numbered row by row, entry (pi, qi) at 256 blk + 16 pi + qi), entries past n
- * zero: the product of afq_gemm_bf16x3_nt with the upper-block Cholesky rows,
- * 0.58 of the bytes of full rows at n = 200.  exp_h1_half stays (n, ldv).
- * n <= 224 (the on-chip capacity; larger n is an error).
+ * zero.  vhs_sym holds two planes (real, imaginary) of nw such rows: the
+ * product of afq_gemm_bf16x3_nt with the upper-block Cholesky rows, 0.58 of
+ * the bytes of full rows at n = 200.  n <= 224 (the on-chip capacity; larger
+ * n is an error).
  */
-int afq_trotter_bf16x3_sym(int nw, int n, int o, int nspin, int ldv, int n_terms, double sqrt_dt,
-                           const float* exp_h1_half, const float* vhs_sym, const double* win_up,
+int afq_trotter_bf16x3_sym(int nw, int n, int o, int nspin, int n_terms, double sqrt_dt,
+                           const float* exp_h1_half_sym, const float* vhs_sym, const double* win_up,
                            const double* win_dn, double* wout_up, double* wout_dn, void* stream);
 
 /*

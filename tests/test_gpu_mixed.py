@@ -362,13 +362,13 @@ def test_bf16x3_gemm_split_k(gpu_device, M, N, K):
 @pytest.mark.parametrize("nw,n,o,nspin", [(8, 24, 6, 1), (5, 45, 11, 2), (3, 17, 5, 1), (16, 200, 32, 1),
                                           (4, 224, 16, 2)])
 def test_trotter_bf16x3_upper_block_vhs_equals_full_rows(gpu_device, nw, n, o, nspin):
-    """The bf16x3 Trotter on V stored as upper 16 x 16 blocks equals the one on
-    full (n, ldv) rows bit for bit: the same contraction fills both layouts,
-    and the lower-block fragments read from the stored transposes are the
-    same floats the full rows hold."""
+    """The bf16x3 Trotter on V and E stored as upper 16 x 16 blocks equals the
+    one on full (n, ldv) rows bit for bit: the same contraction fills both V
+    layouts, E is exactly symmetric, and the lower-block fragments read from
+    the stored transposes are the same floats the full rows hold."""
     import torch
     from ad_afqmc_prototype_amd import kernels
-    from ad_afqmc_prototype_amd.prop.chol_afqmc_ops import chol_sym_rows
+    from ad_afqmc_prototype_amd.prop.chol_afqmc_ops import chol_sym_rows, upper_blocks
     print(f"upper-block Trotter nw={nw} n={n} o={o} nspin={nspin}", flush=True)
     rng = np.random.default_rng(n + nw)
     g = 40
@@ -389,6 +389,7 @@ def test_trotter_bf16x3_upper_block_vhs_equals_full_rows(gpu_device, nw, n, o, n
                        device=gpu_device) for _ in range(nspin)]
     walk = ws[0] if nspin == 1 else tuple(ws)
     r_full = kernels.trotter(walk, v_full, e, 0.07, 6, arith="bf16x3")
-    r_blk = kernels.trotter(walk, v_blk, e, 0.07, 6, arith="bf16x3")
+    e_blk = upper_blocks(e[:, :n].double()).float()
+    r_blk = kernels.trotter(walk, v_blk, e_blk, 0.07, 6, arith="bf16x3")
     for a_, b_ in zip(r_full if nspin == 2 else (r_full,), r_blk if nspin == 2 else (r_blk,)):
         assert torch.equal(a_, b_)
