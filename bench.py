@@ -109,27 +109,34 @@ def run_steps(ctx, state, fields, t0, k, cadence=True, energies=None, comb=None)
     return state
 
 
-def cpu_baseline(workload: str, budget_s: float = 20.0):
+def cpu_baseline(workload: str, budget_s: float = 20.0, single_thread: bool = False, nw: int = 32):
     """The CPU checker (numpy fp64, op for op the reference algorithm) on a
-    subsample of the same workload's walkers, timed on this host."""
+    subsample of the same workload's walkers, timed on this host with all of
+    its BLAS threads, or with one (the reference pins its own CPU runs to one
+    thread, config.py:49-54)."""
+    import contextlib
+
+    from threadpoolctl import threadpool_info, threadpool_limits
+
+    ctx = threadpool_limits(limits=1, user_api="blas") if single_thread else contextlib.nullcontext()
+    with ctx:
+        threads = max([i.get("num_threads", 1) for i in threadpool_info()
+                       if i.get("user_api") == "blas"] or [1])
+        return _cpu_baseline_run(workload, budget_s, threads, nw)
+
+
+def _cpu_baseline_run(workload: str, budget_s: float, threads: int, nw: int):
     import numpy as np
 
     from ad_afqmc_prototype_amd import synthetic as syn
     from oracle import afqmc_oracle as orc
 
-    try:
-        from threadpoolctl import threadpool_info
-        threads = max([i.get("num_threads", 1) for i in threadpool_info()
-                       if i.get("user_api") == "blas"] or [1])
-    except Exception:  # pragma: no cover
-        threads = os.cpu_count() or 1
     cfg = syn.WORKLOADS[workload]
     s = syn.make_system(cfg["seed"], cfg["norb"], cfg["nocc"], cfg["naux"], uhf=cfg["uhf"],
                         chol_scale=cfg["chol_scale"])
     kind = cfg["walker_kind"]
     sd = orc.make_system_data(s.h0, s.h1, s.chol, (s.mo_up, s.mo_dn), kind, cfg["dt"])
     prm = orc.params(dt=cfg["dt"])
-    nw = 32
     rdm = orc.get_rdm1(sd.trial)
     w = orc.init_walkers(rdm, s.nocc, nw, kind)
     st = orc.state(w, np.ones(nw), orc.overlap(w, sd.trial, kind, "log"), 0.0, 0.0)
@@ -585,6 +592,7 @@ def main():
         out["adjoint"] = adjoint_run_bench(device)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(args.workload)
+        out["cpu_baseline_single_thread"] = cpu_baseline(args.workload, budget_s=12.0, single_thread=True, nw=8)
     print(json.dumps(out), flush=True)
 
 
