@@ -142,9 +142,9 @@ __device__ __forceinline__ void sym_block(int blk, int nb, int& bi, int& bj) {
 
 // One lane's values v[j] = entry (bi*16 + pi0 + j, bj*16 + qi), j < NPI, of one
 // output row (row points at that row's n x ldv matrix).  The mirror entries
-// (bj*16 + qi, bi*16 + pi0 + j) are consecutive and go out in 16-byte
-// stores: NPI * sizeof(T) is a multiple of 16, pi0 a multiple of 16 / sizeof(T),
-// and ldv and the row base are 16-byte aligned.  Entries past n in the stored
+// (bj*16 + qi, bi*16 + pi0 + j) are consecutive and go out in vector stores of
+// W = min(NPI, 16 / sizeof(T)) elements: pi0 a multiple of W, ldv a multiple of
+// 4 and the row base 16-byte aligned.  Entries past n in the stored
 // blocks are zero (zero operand rows), so the padding columns n..ldv-1 get
 // zeros from the direct stores; mirror rows past n are skipped.
 template <int NPI, typename T>
@@ -159,7 +159,7 @@ __device__ __forceinline__ void sym_store(T* __restrict__ row, const sym_dst& d,
         }
     }
     if (bi != bj && q < d.n) {
-        constexpr int W = 16 / sizeof(T);
+        constexpr int W = (NPI * (int)sizeof(T) >= 16) ? 16 / (int)sizeof(T) : NPI;
         typedef T vec_t __attribute__((ext_vector_type(W)));
         vec_t* dst = reinterpret_cast<vec_t*>(row + (long long)q * d.ldv + bi * 16 + pi0);
 #pragma unroll

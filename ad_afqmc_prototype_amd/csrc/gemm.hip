@@ -226,7 +226,10 @@ __global__ __launch_bounds__(THREADS, 2) void sgemm_grouped_kernel(int count, in
     const int n0 = (local / tiles_m) * CF::BN;
     typename CF::acc_t acc[TM][TN];
     gemm_nt_tile<float, TM, TN>(M, N, p.K, p.A, p.lda, p.B, p.ldb, m0, n0, As, Bs, acc);
-    if (p.c_fp64)
+    if (p.sym_n > 0)
+        gemm_store_sym<float, TM, TN>(M, p.alpha, (float*)p.C,
+                                      afq::sym_dst{p.sym_n, p.sym_ldv, afq::ceil_div(p.sym_n, 16)}, m0, n0, acc);
+    else if (p.c_fp64)
         gemm_store<float, double, TM, TN>(M, N, p.alpha, 0.0f, (double*)p.C, p.ldc, m0, n0, acc);
     else
         gemm_store<float, float, TM, TN>(M, N, p.alpha, 0.0f, (float*)p.C, p.ldc, m0, n0, acc);

@@ -40,7 +40,7 @@ from ..ham.chol import ham_chol
 from ..meas.rhf import make_rhf_meas_ops
 from ..trial.rhf import make_rhf_trial_ops, rhf_trial
 from .afqmc import init_prop_state
-from .chol_afqmc_ops import chol_transposed, make_chol_afqmc_ops
+from .chol_afqmc_ops import chol_sym_rows, make_chol_afqmc_ops
 
 c128 = torch.complex128
 f64 = torch.float64
@@ -70,11 +70,12 @@ class _gemm_problem(ctypes.Structure):
 
     _fields_ = [("M", ctypes.c_int), ("N", ctypes.c_int), ("K", ctypes.c_int), ("tile_start", ctypes.c_int),
                 ("lda", ctypes.c_longlong), ("ldb", ctypes.c_longlong), ("ldc", ctypes.c_longlong),
-                ("A", _vp), ("B", _vp), ("C", _vp), ("alpha", ctypes.c_float), ("c_fp64", ctypes.c_int)]
+                ("A", _vp), ("B", _vp), ("C", _vp), ("alpha", ctypes.c_float), ("c_fp64", ctypes.c_int),
+                ("sym_n", ctypes.c_int), ("sym_ldv", ctypes.c_int)]
 
 
 assert ctypes.sizeof(_system_step) == 8 * 4 + 8 * 8 + 24 * 8
-assert ctypes.sizeof(_gemm_problem) == 72
+assert ctypes.sizeof(_gemm_problem) == 80
 
 GEMM_SHAPES = (0, 1, 2, 3)
 
@@ -181,7 +182,7 @@ class system_batch:
                              trial_ops_=t_ops, trial_data=s.trial, meas_ops=m_ops)
         rot_pad, rot_h1 = m_ctx.energy_operands(1)
         return dict(n=n, o=o, g=g, nw=s.n_walkers, ldv=kernels.ldv_for(n), h0=float(s.ham.h0),
-                    trial=s.trial.mo_up.contiguous(), chol_t32=kernels.convert_f32(chol_transposed(s.ham.chol)),
+                    trial=s.trial.mo_up.contiguous(), chol_s32=kernels.convert_f32(chol_sym_rows(s.ham.chol)),
                     rot32=m_ctx.fb_operand(1), rot_pad=rot_pad, rot_h1=rot_h1,
                     e_half=p_ctx.exp_h1_half_pad32.contiguous(), mf=p_ctx.mf_shifts.contiguous(),
                     h0_prop=float(complex(p_ctx.h0_prop).real), e_est=float(st.e_estimate), state0=st)
@@ -263,11 +264,13 @@ class system_batch:
                 p.A, p.B, p.C = self.greens32.ptr(i), m["rot32"].data_ptr(), self.fb.ptr(i)
                 p.alpha, p.c_fp64 = 2.0, 1
             else:
-                nc = m["n"] * m["ldv"]
-                p.M, p.N, p.K = 2 * m["nw"], nc, m["g"]
-                p.lda, p.ldb, p.ldc = m["g"], m["g"], nc
-                p.A, p.B, p.C = self.xbar32.ptr(i), m["chol_t32"].data_ptr(), self.vhs.ptr(i)
+                # symmetric VHS: the upper 16 x 16 blocks, each entry stored twice
+                # into the (n, ldv) rows the Trotter reads
+                p.M, p.N, p.K = 2 * m["nw"], m["chol_s32"].shape[0], m["g"]
+                p.lda, p.ldb, p.ldc = m["g"], m["g"], 0
+                p.A, p.B, p.C = self.xbar32.ptr(i), m["chol_s32"].data_ptr(), self.vhs.ptr(i)
                 p.alpha, p.c_fp64 = 1.0, 0
+                p.sym_n, p.sym_ldv = m["n"], m["ldv"]
             p.tile_start = start
             start += -(-p.M // bm) * -(-p.N // bn)
         return arr, start
@@ -402,12 +405,13 @@ class system_batch:
 
     def flops_per_step(self) -> float:
         """Algorithmic flops of one step of every system (real flops; complex x
-        real MAC = 4, complex x complex MAC = 8; force bias, VHS, Taylor, the two
-        one-body halves, C^T w, LU and the Green's solve)."""
+        real MAC = 4, complex x complex MAC = 8; force bias, VHS (its n (n + 1) / 2
+        distinct entries), Taylor, the two one-body halves, C^T w, LU and the
+        Green's solve)."""
         tot = 0.0
         for m in self.members:
             n, o, g, nw = m["n"], m["o"], m["g"], m["nw"]
-            tot += nw * (4 * g * n * n + 4 * g * o * n + self.n_exp_terms * 8 * n * n * o
+            tot += nw * (2 * g * n * (n + 1) + 4 * g * o * n + self.n_exp_terms * 8 * n * n * o
                          + 2 * 4 * n * n * o + 4 * o * o * n + 8 / 3 * o ** 3 + 8 * o * o * n)
         return tot
 

@@ -392,6 +392,12 @@ typedef struct afq_gemm_problem {
     void* C;                     /* float32, or float64 when c_fp64 != 0 */
     float alpha;
     int c_fp64;
+    /* sym_n > 0: the symmetric VHS form of afq_vhs_sym: the N columns are the
+     * upper 16 x 16 block entries of an sym_n x sym_n matrix (N = nb (nb + 1) / 2
+     * * 256, nb = ceil(sym_n / 16)) and every result goes to both of its places
+     * in C's rows of sym_n x sym_ldv floats (sym_ldv a multiple of 4, C float32
+     * and 16-byte aligned; ldc unused) */
+    int sym_n, sym_ldv;
 } afq_gemm_problem;
 
 /*
