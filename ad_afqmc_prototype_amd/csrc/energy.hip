@@ -412,27 +412,28 @@ __global__ __launch_bounds__(THREADS_B, 1) void energy_bf16x3_kernel(
                 for (int ti = 0; ti < NT; ++ti)
 #pragma unroll
                     for (int tj = 0; tj < NT; ++tj) F[b][ti][tj][0] = F[b][ti][tj][1] = MF::zero();
-            // A fragments of R_g (rows ti*16 + lr, k = kb*32 + 8q..), one k-block ahead
-            bf16x8 ring[2][NT][2];
+            // A fragments of R_g (rows ti*16 + lr, k = kb*32 + 8q..), two k-blocks ahead
+            bf16x8 ring[3][NT][2];
             auto issue = [&](int kb) {
 #pragma unroll
                 for (int ti = 0; ti < NT; ++ti) {
                     const long long off = (long long)(ti * 16 + lr) * ldr + (kb < nk ? kb : 0) * 32 + 8 * q;
-                    ring[kb % 2][ti][0] = *reinterpret_cast<const bf16x8*>(Ah + off);
-                    ring[kb % 2][ti][1] = *reinterpret_cast<const bf16x8*>(Al + off);
+                    ring[kb % 3][ti][0] = *reinterpret_cast<const bf16x8*>(Ah + off);
+                    ring[kb % 3][ti][1] = *reinterpret_cast<const bf16x8*>(Al + off);
                 }
             };
             issue(0);
+            if (nk > 1) issue(1);
             // fully unrolled to KMAX_E k-blocks so that every ring index is a constant
 #pragma unroll
             for (int kb = 0; kb < KMAX_E; ++kb) {
                 if (kb >= nk) break;
-                if (kb + 1 < nk) issue(kb + 1);
+                if (kb + 2 < nk) issue(kb + 2);
                 bf16x8 ah[NT], al[NT];
 #pragma unroll
                 for (int ti = 0; ti < NT; ++ti) {
-                    ah[ti] = ring[kb % 2][ti][0];
-                    al[ti] = ring[kb % 2][ti][1];
+                    ah[ti] = ring[kb % 3][ti][0];
+                    al[ti] = ring[kb % 3][ti][1];
                 }
 #pragma unroll
                 for (int b = 0; b < WPB; ++b)
