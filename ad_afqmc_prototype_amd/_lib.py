@@ -40,6 +40,8 @@ _SIGNATURES = {
                                  c_ll, c_vp],
     "afq_force_bias": [c_int, c_int, c_int, c_double, c_vp, c_vp, c_vp, c_vp],
     "afq_field_shifts": [c_int, c_int, c_double, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp],
+    "afq_field_shifts_split": [c_int, c_int, c_double, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_ll,
+                               c_vp],
     "afq_vhs": [c_int, c_int, c_int, c_int, c_vp, c_vp, c_vp, c_vp],
     "afq_vhs_sym": [c_int, c_int, c_int, c_int, c_int, c_vp, c_vp, c_ll, c_vp, c_vp, c_ll, c_vp, c_vp],
     "afq_trotter": [c_int, c_int, c_int, c_int, c_int, c_int, c_double, c_vp, c_vp, c_vp, c_vp,

@@ -30,10 +30,14 @@ __device__ __forceinline__ void block_sum(double (&v)[4], double* sh) {
 }
 
 // One walker: row w of the (nw, naux) arrays (planes nw * naux apart).
+// SPLIT: also the split-bf16 planes of the shifted fields, rows w (real) and
+// nw + w (imaginary) of ldk, columns naux..ldk-1 zero (afq::split_bf16).
+template <bool SPLIT = false>
 __device__ __forceinline__ void field_shift_body(
     int nw, int naux, int w, double sqrt_dt, const double* __restrict__ x, const double* __restrict__ fb,
     const double2* __restrict__ mf, double* __restrict__ xbar, double2* __restrict__ shift_term,
-    double2* __restrict__ fb_term) {
+    double2* __restrict__ fb_term, uint16_t* __restrict__ xhi = nullptr, uint16_t* __restrict__ xlo = nullptr,
+    long long ldk = 0) {
     __shared__ double sh[16];
     const long long row = (long long)w * naux;
     const long long plane = (long long)nw * naux;
@@ -51,12 +55,25 @@ __device__ __forceinline__ void field_shift_body(
         const double xi = -fsi;
         xbar[row + g] = xr;
         xbar[plane + row + g] = xi;
+        if constexpr (SPLIT) {
+            const long long er = (long long)w * ldk + g, ei = ((long long)nw + w) * ldk + g;
+            afq::split_bf16(xr, xhi[er], xlo[er]);
+            afq::split_bf16(xi, xhi[ei], xlo[ei]);
+        }
         // shift_term += xbar * mf
         acc[0] += xr * m.x - xi * m.y;
         acc[1] += xr * m.y + xi * m.x;
         // fb_term += x fs - fs^2 / 2
         acc[2] += xv * fsr - 0.5 * (fsr * fsr - fsi * fsi);
         acc[3] += xv * fsi - 0.5 * (2.0 * fsr * fsi);
+    }
+    if constexpr (SPLIT) {
+        for (long long g = naux + threadIdx.x; g < ldk; g += 256) {
+            xhi[(long long)w * ldk + g] = 0;
+            xlo[(long long)w * ldk + g] = 0;
+            xhi[((long long)nw + w) * ldk + g] = 0;
+            xlo[((long long)nw + w) * ldk + g] = 0;
+        }
     }
     block_sum<256>(acc, sh);
     if (threadIdx.x == 0) {
@@ -71,6 +88,13 @@ __global__ __launch_bounds__(256) void field_shift_kernel(
     const double2* __restrict__ mf, double* __restrict__ xbar, double2* __restrict__ shift_term,
     double2* __restrict__ fb_term) {
     field_shift_body(nw, naux, blockIdx.x, sqrt_dt, x, fb, mf, xbar, shift_term, fb_term);
+}
+
+__global__ __launch_bounds__(256) void field_shift_split_kernel(
+    int nw, int naux, double sqrt_dt, const double* __restrict__ x, const double* __restrict__ fb,
+    const double2* __restrict__ mf, double* __restrict__ xbar, double2* __restrict__ shift_term,
+    double2* __restrict__ fb_term, uint16_t* __restrict__ xhi, uint16_t* __restrict__ xlo, long long ldk) {
+    field_shift_body<true>(nw, naux, blockIdx.x, sqrt_dt, x, fb, mf, xbar, shift_term, fb_term, xhi, xlo, ldk);
 }
 
 // One workgroup per walker of every system.
@@ -181,6 +205,19 @@ extern "C" int afq_field_shifts(int nw, int naux, double sqrt_dt, const double* 
                        naux, sqrt_dt, fields, fb, (const double2*)mf, xbar, (double2*)shift_term,
                        (double2*)fb_term);
     return afq::check_launch("afq_field_shifts");
+}
+
+extern "C" int afq_field_shifts_split(int nw, int naux, double sqrt_dt, const double* fields, const double* fb,
+                                      const double* mf, double* xbar, double* shift_term, double* fb_term,
+                                      unsigned short* x_hi, unsigned short* x_lo, long long ldk, void* stream) {
+    AFQ_REQUIRE(nw > 0 && naux > 0, "afq_field_shifts_split: bad sizes nw=%d naux=%d", nw, naux);
+    AFQ_REQUIRE(ldk >= naux, "afq_field_shifts_split: ldk=%lld < naux=%d", ldk, naux);
+    AFQ_REQUIRE(fields && fb && mf && xbar && shift_term && fb_term && x_hi && x_lo,
+                "afq_field_shifts_split: null pointer");
+    hipLaunchKernelGGL(field_shift_split_kernel, dim3((unsigned)nw), dim3(256), 0, (hipStream_t)stream, nw, naux,
+                       sqrt_dt, fields, fb, (const double2*)mf, xbar, (double2*)shift_term, (double2*)fb_term,
+                       (uint16_t*)x_hi, (uint16_t*)x_lo, ldk);
+    return afq::check_launch("afq_field_shifts_split");
 }
 
 extern "C" int afq_phaseless_update(int nw, int nspin, const double* logdet_new,

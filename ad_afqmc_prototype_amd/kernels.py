@@ -289,7 +289,10 @@ def force_bias(greens, rot, alpha, fb=None):
     return fb
 
 
-def field_shifts(fields, fb, mf, sqrt_dt, xbar=None, shift_term=None, fb_term=None):
+def field_shifts(fields, fb, mf, sqrt_dt, xbar=None, shift_term=None, fb_term=None, split=False):
+    """(xbar planes (2, nw, naux), shift_term, fb_term); with ``split`` also the
+    (hi, lo) bf16 planes of the xbar rows (2 nw, pad8(naux)) that split_bf16
+    would make, from the same pass."""
     nw, naux = fields.shape
     dev = fields.device
     _check(fields, f64, (nw, naux), "fields")
@@ -301,6 +304,14 @@ def field_shifts(fields, fb, mf, sqrt_dt, xbar=None, shift_term=None, fb_term=No
         shift_term = torch.empty((nw,), dtype=c128, device=dev)
     if fb_term is None:
         fb_term = torch.empty((nw,), dtype=c128, device=dev)
+    if split:
+        ldk = pad8(naux)
+        hi = torch.empty((2 * nw, ldk), dtype=torch.int16, device=dev)
+        lo = torch.empty((2 * nw, ldk), dtype=torch.int16, device=dev)
+        _call("afq_field_shifts_split", fields, nw, naux, float(sqrt_dt), fields.data_ptr(), fb.data_ptr(),
+              mf.data_ptr(), xbar.data_ptr(), shift_term.data_ptr(), fb_term.data_ptr(), hi.data_ptr(),
+              lo.data_ptr(), ldk, _stream(fields))
+        return xbar, shift_term, fb_term, (hi, lo)
     _call("afq_field_shifts", fields, nw, naux, float(sqrt_dt), fields.data_ptr(), fb.data_ptr(),
               mf.data_ptr(), xbar.data_ptr(), shift_term.data_ptr(), fb_term.data_ptr(),
               _stream(fields))

@@ -104,9 +104,16 @@ def afqmc_step(state: prop_state, *, params: afqmc_params, ham_data: ham_chol, t
         fields = fields.to(torch.float64).contiguous()
 
     fb = _force_bias_planes(state, ham_data, trial_data, meas_ops, meas_ctx, params.n_chunks)
-    xbar, shift_term, fb_term = kernels.field_shifts(fields, fb, prop_ctx.mf_shifts, prop_ctx.sqrt_dt)
-    walkers_new = prop_ops.apply_trotter_planes(state.walkers, xbar, prop_ctx, params.n_exp_terms,
-                                                params.n_chunks)
+    if getattr(prop_ops, "xbar_split", False) and params.n_chunks == 1:
+        # the bf16x3 VHS takes xbar as bf16 planes, made in the same pass
+        xbar, shift_term, fb_term, xs = kernels.field_shifts(fields, fb, prop_ctx.mf_shifts, prop_ctx.sqrt_dt,
+                                                             split=True)
+        walkers_new = prop_ops.apply_trotter_planes(state.walkers, xbar, prop_ctx, params.n_exp_terms,
+                                                    params.n_chunks, xbar_split=xs)
+    else:
+        xbar, shift_term, fb_term = kernels.field_shifts(fields, fb, prop_ctx.mf_shifts, prop_ctx.sqrt_dt)
+        walkers_new = prop_ops.apply_trotter_planes(state.walkers, xbar, prop_ctx, params.n_exp_terms,
+                                                    params.n_chunks)
     if meas_ops.greens is not None:
         greens_new = meas_ops.greens(walkers_new, trial_data)
         logdet = greens_new.logdet

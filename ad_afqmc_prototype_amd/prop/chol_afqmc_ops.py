@@ -39,6 +39,9 @@ class chol_afqmc_ops(NamedTuple):
     apply_trotter: Callable[..., Any]       # (walkers, fields (nw, G), ctx, n_terms) -> walkers
     apply_trotter_planes: Callable[..., Any]  # (walkers, xbar planes (2, nw, G), ctx, n_terms)
     precision: str = "fp64"
+    # the VHS contraction takes xbar as bf16 planes: the step asks the field
+    # shift kernel for them (apply_trotter_planes(..., xbar_split=(hi, lo)))
+    xbar_split: bool = False
 
 
 def _as_total_rdm1_restricted(dm: torch.Tensor) -> torch.Tensor:
@@ -154,7 +157,7 @@ def make_chol_afqmc_ops(ham_data: ham_chol, walker_kind: str,
     # so the contraction's output is used as it comes (0.58 of the bytes)
     v_blocks = precision == "bf16x3" and n <= kernels.BF16X3_SYM_MAX_N
 
-    def make_vhs(xbar: torch.Tensor) -> torch.Tensor:
+    def make_vhs(xbar: torch.Tensor, xbar_split=None) -> torch.Tensor:
         """VHS planes in the step's precision, from the symmetric contraction
         (upper 16 x 16 blocks): (2, nw, sym_blocks(n) * 256) upper blocks for the
         bf16x3 Trotter, else (2, nw, n, ldv) rows (each entry stored twice)."""
@@ -162,7 +165,7 @@ def make_chol_afqmc_ops(ham_data: ham_chol, walker_kind: str,
         flat = xbar.reshape(2 * nwc, nf)
         if v_blocks:
             out = torch.empty((2 * nwc, chol_hi.shape[0]), dtype=torch.float32, device=xbar.device)
-            xh, xl = kernels.split_bf16(flat)
+            xh, xl = xbar_split if xbar_split is not None else kernels.split_bf16(flat)
             kernels.gemm_bf16x3_nt(xh, xl, chol_hi, chol_lo, out, tag="vhs")
             return out.view(2, nwc, -1)
         if precision == "fp64":
@@ -182,21 +185,25 @@ def make_chol_afqmc_ops(ham_data: ham_chol, walker_kind: str,
         return _build_prop_ctx(ham_data, trial_rdm1, dt)
 
     def apply_trotter_planes(w, xbar: torch.Tensor, ctx: chol_afqmc_ctx, n_terms: int,
-                             n_chunks: int = 1):
+                             n_chunks: int = 1, xbar_split=None):
+        """``xbar_split``: optional (hi, lo) bf16 planes of xbar's rows (from
+        kernels.field_shifts(split=True)), used by the bf16x3 VHS when n_chunks == 1."""
         if generalized:
             # _apply_trotter_g_from_restricted (prop/chol_afqmc_ops.py:200-212): the
             # spin blocks of each (2n, o') walker with the walker's one VHS
             top, bot = apply_spin_blocks((w[:, :n, :].contiguous(), w[:, n:, :].contiguous()), xbar,
-                                         ctx, n_terms, n_chunks)
+                                         ctx, n_terms, n_chunks, xbar_split)
             return torch.cat([top, bot], dim=1)
-        return apply_spin_blocks(w, xbar, ctx, n_terms, n_chunks)
+        return apply_spin_blocks(w, xbar, ctx, n_terms, n_chunks, xbar_split)
 
-    def apply_spin_blocks(w, xbar: torch.Tensor, ctx: chol_afqmc_ctx, n_terms: int, n_chunks: int):
+    def apply_spin_blocks(w, xbar: torch.Tensor, ctx: chol_afqmc_ctx, n_terms: int, n_chunks: int,
+                          xbar_split=None):
         unr = isinstance(w, tuple)
         w0 = w[0] if unr else w
         nw = w0.shape[0]
         if n_chunks == 1:
-            return kernels.trotter(w, make_vhs(xbar), e_half(ctx), ctx.sqrt_dt, n_terms, arith=precision)
+            return kernels.trotter(w, make_vhs(xbar, xbar_split), e_half(ctx), ctx.sqrt_dt, n_terms,
+                                   arith=precision)
         out = (torch.empty_like(w[0]), torch.empty_like(w[1])) if unr else torch.empty_like(w0)
         for a, b in _chunks(nw, n_chunks):
             v = make_vhs(xbar[:, a:b].contiguous())
@@ -214,4 +221,4 @@ def make_chol_afqmc_ops(ham_data: ham_chol, walker_kind: str,
             xbar = torch.stack([f, torch.zeros_like(f)]).to(torch.float64).contiguous()
         return apply_trotter_planes(w, xbar, ctx, n_terms, n_chunks)
 
-    return chol_afqmc_ops(n_fields_, build_prop_ctx, apply_trotter, apply_trotter_planes, precision)
+    return chol_afqmc_ops(n_fields_, build_prop_ctx, apply_trotter, apply_trotter_planes, precision, v_blocks)

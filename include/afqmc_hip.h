@@ -140,6 +140,16 @@ int afq_field_shifts(int nw, int naux, double sqrt_dt, const double* fields,
                      double* shift_term, double* fb_term, void* stream);
 
 /*
+ * afq_field_shifts that also writes the shifted fields as split-bf16 planes
+ * x_hi / x_lo (2 nw rows of ldk: real rows, then imaginary rows, the values
+ * afq_split_bf16 makes of xbar; columns naux .. ldk - 1 zeroed): the A operand
+ * of the bf16x3 VHS contraction, without a second pass over xbar.
+ */
+int afq_field_shifts_split(int nw, int naux, double sqrt_dt, const double* fields, const double* fb,
+                           const double* mf, double* xbar, double* shift_term, double* fb_term,
+                           unsigned short* x_hi, unsigned short* x_lo, long long ldk, void* stream);
+
+/*
  * VHS = sum_g xbar_g L_g, planes (2, nw, n, ldv) (ldv = n rounded up to 4).
  * chol_t (n*ldv, naux) is L transposed with zero rows at q >= n.
  * Replaces _make_vhs_split_flat (prop/chol_afqmc_ops.py:70-74).

@@ -55,6 +55,8 @@ def test_library_loads_and_reports_abi():
     ("afq_overlap_greens_split", (4, 8, 2, 1, 1, 0, 1, 0, 1, 1, 0, 1, 16, 0), "null"),
     ("afq_vhs", (4, 10, 10, 5, 1, 1, 1, 0), "multiple of 4"),
     ("afq_vhs_sym", (4, 10, 10, 5, 0, 16, 0, 5, 16, 0, 5, 16, 0), "multiple of 4"),
+    ("afq_field_shifts_split", (4, 10, 0.1, 1, 1, 1, 1, 1, 1, 1, 1, 8, 0), "ldk"),
+    ("afq_field_shifts_split", (4, 10, 0.1, 1, 1, 1, 1, 1, 1, 0, 1, 16, 0), "null"),
     ("afq_trotter_bf16x3_sym", (1, 240, 8, 1, 6, 0.1, 1, 1, 2, 0, 3, 0, 0), "224"),
     ("afq_trotter_bf16x3_sym", (1, 8, 2, 1, 6, 0.1, 1, 1, 2, 0, 2, 0, 0), "aliases"),
     ("afq_gemm_bf16x3_nt_splitk", (4, 4, 8, 1.0, 16, 16, 8, 16, 16, 8, 16, 4, 1, 0, 16, 0), "splits"),

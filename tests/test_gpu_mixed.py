@@ -393,3 +393,23 @@ def test_trotter_bf16x3_upper_block_vhs_equals_full_rows(gpu_device, nw, n, o, n
     r_blk = kernels.trotter(walk, v_blk, e_blk, 0.07, 6, arith="bf16x3")
     for a_, b_ in zip(r_full if nspin == 2 else (r_full,), r_blk if nspin == 2 else (r_blk,)):
         assert torch.equal(a_, b_)
+
+
+@pytest.mark.parametrize("nw,naux", [(8, 96), (5, 130), (33, 1000)])
+def test_field_shifts_split_planes_equal_split_of_xbar(gpu_device, nw, naux):
+    """afq_field_shifts_split: the same xbar, shift and force-bias terms as
+    afq_field_shifts, and bf16 planes equal to split_bf16(xbar rows) bit for
+    bit (zero padding columns included)."""
+    import torch
+    from ad_afqmc_prototype_amd import kernels
+    print(f"field shifts split nw={nw} naux={naux}", flush=True)
+    g = torch.Generator(device="cpu").manual_seed(naux)
+    x = torch.randn((nw, naux), dtype=torch.float64, generator=g).to(gpu_device)
+    fb = torch.randn((2, nw, naux), dtype=torch.float64, generator=g).to(gpu_device)
+    mf = torch.randn((naux,), dtype=torch.complex128, generator=g).to(gpu_device)
+    a = kernels.field_shifts(x, fb, mf, 0.07)
+    b = kernels.field_shifts(x, fb, mf, 0.07, split=True)
+    for u, v in zip(a, b[:3]):
+        assert torch.equal(u, v)
+    rh, rl = kernels.split_bf16(a[0].reshape(2 * nw, naux))
+    assert torch.equal(b[3][0], rh) and torch.equal(b[3][1], rl)
