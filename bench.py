@@ -465,9 +465,26 @@ def timed_run(args, precision, device):
         elapsed_inst = time.perf_counter() - t1
     ktimes = tm.summary()
     e_block = [float(e.item()) for e in energies]
+    # the driver's path: whole blocks (n_prop_steps steps with QR and the
+    # measurement + comb) replayed as one HIP graph each, fields and comb
+    # uniforms drawn on the device from the state's key inside the graph
+    elapsed_graph = None
+    if not args.no_cadence and args.steps % ctx["params"].n_prop_steps == 0:
+        from ad_afqmc_prototype_amd.prop.blocks import captured_block
+        cb = captured_block(state, sys=ctx["sys"], params=ctx["params"], ham_data=ctx["ham"],
+                            trial_data=ctx["trial"], meas_ops=ctx["m_ops"], prop_ops=ctx["p_ops"],
+                            prop_ctx=ctx["p_ctx"], meas_ctx=ctx["m_ctx"])
+        cb.replay()
+        torch.cuda.synchronize()
+        t2 = time.perf_counter()
+        for _ in range(args.steps // ctx["params"].n_prop_steps):
+            cb.replay()
+        torch.cuda.synchronize()
+        elapsed_graph = time.perf_counter() - t2
+        del cb
     del ctx, state, fields
     torch.cuda.empty_cache()
-    return elapsed, ktimes, e_block, elapsed_inst
+    return elapsed, ktimes, e_block, elapsed_inst, elapsed_graph
 
 
 def main():
@@ -522,7 +539,7 @@ def main():
     n, o, g, nw = cfg["norb"], cfg["nocc"], cfg["naux"], cfg["n_walkers"]
     nspin = 2 if cfg["walker_kind"] == "unrestricted" else 1
 
-    elapsed, ktimes, e_block, elapsed_inst = timed_run(args, args.precision, device)
+    elapsed, ktimes, e_block, elapsed_inst, elapsed_graph = timed_run(args, args.precision, device)
     table = kernel_table(ktimes, n, o, g, nw, nspin, not cfg["uhf"], args.precision)
     dominant = max((k for k in table if "tflops" in table[k]), key=lambda k: table[k]["total_ms"])
     dom = table[dominant]
@@ -542,6 +559,9 @@ def main():
         # the same K steps again with HIP events around every library call (the
         # kernel table and roofline come from that pass)
         "ms_per_step_with_kernel_events": 1e3 * elapsed_inst / args.steps,
+        # the same cadence as captured blocks (one HIP graph per block, fields
+        # and comb uniforms from the device key inside the graph)
+        "ms_per_step_graph": None if elapsed_graph is None else 1e3 * elapsed_graph / args.steps,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -560,9 +580,10 @@ def main():
         "block_energies": e_block,
     }
     if args.precision != "fp64" and not args.no_fp64_compare:
-        el64, kt64, e64, _ = timed_run(args, "fp64", device)
+        el64, kt64, e64, _, el64_graph = timed_run(args, "fp64", device)
         out["fp64_value"] = nw * args.steps / el64
         out["fp64_ms_per_step"] = 1e3 * el64 / args.steps
+        out["fp64_ms_per_step_graph"] = None if el64_graph is None else 1e3 * el64_graph / args.steps
         out["speedup_vs_fp64"] = out["value"] / out["fp64_value"]
         out["fp64_kernels"] = kernel_table(kt64, n, o, g, nw, nspin, not cfg["uhf"], "fp64")
         out["block_energy_abs_diff_mHa"] = [1e3 * abs(a - b) for a, b in zip(e_block, e64)]
