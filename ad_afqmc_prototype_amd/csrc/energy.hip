@@ -97,9 +97,10 @@ __device__ __forceinline__ void energy_body(
                     }
             if (active) {
                 const R* A = Rs + (long long)g * TC * ldr;
-                for (int kb = 0; kb < nkb; ++kb) {
+                // A fragments of k-block kb: rows ti * 16 + lr, k = kb * 16 + lk ..
+                // + 3 (zero past n); loaded two k-blocks ahead of their use
+                auto load_a = [&](int kb, R (&a)[NT][4]) {
                     const int k0 = kb * 16 + lk;
-                    R a[NT][4];
 #pragma unroll
                     for (int ti = 0; ti < NT; ++ti) {
                         if (k0 < n) {
@@ -109,6 +110,9 @@ __device__ __forceinline__ void energy_body(
                             a[ti][0] = a[ti][1] = a[ti][2] = a[ti][3] = R(0);
                         }
                     }
+                };
+                auto product = [&](int kb, const R (&a)[NT][4]) {
+                    const int k0 = kb * 16 + lk;
 #pragma unroll
                     for (int b = 0; b < WPB; ++b) {
                         const R* Xr = X + (size_t)(b * 2) * NP * XP;
@@ -130,6 +134,20 @@ __device__ __forceinline__ void energy_body(
                                 }
                         }
                     }
+                };
+                // three fragment sets in rotation, the loop unrolled by three so
+                // that no in-flight set is ever copied
+                R a0[NT][4], a1[NT][4], a2[NT][4];
+                load_a(0, a0);
+                if (nkb > 1) load_a(1, a1);
+                auto step = [&](int kb, const R (&cur)[NT][4], R (&ahead)[NT][4]) {
+                    if (kb + 2 < nkb) load_a(kb + 2, ahead);
+                    product(kb, cur);
+                };
+                for (int kb = 0; kb < nkb; kb += 3) {
+                    step(kb, a0, a2);
+                    if (kb + 1 < nkb) step(kb + 1, a1, a0);
+                    if (kb + 2 < nkb) step(kb + 2, a2, a1);
                 }
             }
 #pragma unroll
