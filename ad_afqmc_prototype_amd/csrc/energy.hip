@@ -14,10 +14,10 @@
 
 namespace {
 
-constexpr int THREADS = 256;
 
-// Walkers w0 .. w0 + WPB - 1 of one batch (nw walkers).
-template <typename R, int NT, int WPB>
+// Walkers w0 .. w0 + WPB - 1 of one batch (nw walkers); NWV waves, each on its
+// own g.
+template <typename R, int NT, int WPB, int NWV>
 __device__ __forceinline__ void energy_body(
     int nw, int n, int o, int nspin, int naux, double h0, const double* __restrict__ greens,
     const R* __restrict__ rotp, int ldr, const double* __restrict__ rot_h1,
@@ -28,11 +28,12 @@ __device__ __forceinline__ void energy_body(
     constexpr int TC = NT * 16;
     constexpr int XP = TC + 1;
     constexpr int SP = TC + 1;
-    __shared__ double red[4][6 * WPB];
+    constexpr int THREADS = 64 * NWV;
+    __shared__ double red[NWV][6 * WPB];
     const int NP = afq::round_up(n, 16);
     R* X = reinterpret_cast<R*>(lds_raw);               // [WPB][2][NP][XP]
-    R* S = X + (size_t)WPB * 2 * NP * XP;               // [4 waves][TC][SP]
-    size_t cs_off = ((size_t)WPB * 2 * NP * XP + 4 * TC * SP) * sizeof(R);
+    R* S = X + (size_t)WPB * 2 * NP * XP;               // [NWV][TC][SP]
+    size_t cs_off = ((size_t)WPB * 2 * NP * XP + NWV * TC * SP) * sizeof(R);
     cs_off = (cs_off + 15) & ~(size_t)15;
     double2* cs = reinterpret_cast<double2*>(lds_raw + cs_off);  // [WPB][naux] (nspin == 2)
 
@@ -82,7 +83,7 @@ __device__ __forceinline__ void energy_body(
 
         const R* Rs = rotp + (long long)s * naux * TC * ldr;
         const int nkb = afq::ceil_div(n, 16);
-        for (int g0 = 0; g0 < naux; g0 += 4) {
+        for (int g0 = 0; g0 < naux; g0 += NWV) {
             const int g = g0 + wave;
             const bool active = g < naux;
             ACC F[WPB][NT][NT][2];
@@ -179,7 +180,10 @@ __device__ __forceinline__ void energy_body(
                         c2i[b] += 2.0 * tr * tim;
                     }
                 }
-                // exchange: sum_ij f_ij f_ji, real plane then imaginary plane
+                // exchange: sum_ij f_ij f_ji, real plane then imaginary plane,
+                // through this wave's own LDS tile; a wave's LDS accesses execute
+                // in program order, the asm keeps the compiler from moving them
+                // across one another (no workgroup barrier in the g loop)
 #pragma unroll
                 for (int pl = 0; pl < 2; ++pl) {
 #pragma unroll
@@ -192,7 +196,7 @@ __device__ __forceinline__ void energy_body(
                                 const int col = tj * 16 + lr;
                                 Sw[row * SP + col] = F[b][ti][tj][pl][r];
                             }
-                    __syncthreads();
+                    asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
 #pragma unroll
                     for (int ti = 0; ti < NT; ++ti)
 #pragma unroll
@@ -210,7 +214,7 @@ __device__ __forceinline__ void energy_body(
                                     exi[b] += (double)F[b][ti][tj][0][r] * t;
                                 }
                             }
-                    __syncthreads();
+                    asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
                 }
             }
         }
@@ -227,7 +231,7 @@ __device__ __forceinline__ void energy_body(
     if (tid < nwb) {
         const int b = tid;
         double v[6] = {0, 0, 0, 0, 0, 0};
-        for (int q = 0; q < 4; ++q)
+        for (int q = 0; q < NWV; ++q)
             for (int k = 0; k < 6; ++k) v[k] += red[q][6 * b + k];
         double er, ei;
         if (nspin == 1) {
@@ -241,54 +245,56 @@ __device__ __forceinline__ void energy_body(
     }
 }
 
-template <typename R, int NT, int WPB>
-__global__ __launch_bounds__(THREADS, 1) void energy_kernel(
+template <typename R, int NT, int WPB, int NWV>
+__global__ __launch_bounds__(64 * NWV, 1) void energy_kernel(
     int nw, int n, int o, int nspin, int naux, double h0, const double* __restrict__ greens,
     const R* __restrict__ rotp, int ldr, const double* __restrict__ rot_h1,
     double2* __restrict__ eout) {
     extern __shared__ __attribute__((aligned(16))) unsigned char lds_raw[];
-    energy_body<R, NT, WPB>(nw, n, o, nspin, naux, h0, greens, rotp, ldr, rot_h1, eout,
+    energy_body<R, NT, WPB, NWV>(nw, n, o, nspin, naux, h0, greens, rotp, ldr, rot_h1, eout,
                             blockIdx.x * WPB, lds_raw);
 }
 
 // One workgroup per walker of every system (restricted walkers, fp32
-// contractions); every system has ceil(o / 16) == NT.
+// contractions); every system has ceil(o / 16) == NT.  Eight waves, as the
+// single-system fp32 kernel always has (same g partition, so the same sums).
+constexpr int F32_WAVES = 8;
 template <int NT>
-__global__ __launch_bounds__(THREADS, 1) void energy_f32_grouped_kernel(
+__global__ __launch_bounds__(64 * F32_WAVES, 1) void energy_f32_grouped_kernel(
     int count, const afq_system_step* __restrict__ S) {
     extern __shared__ __attribute__((aligned(16))) unsigned char lds_raw[];
     const int b = blockIdx.x;
     const int si = afq::find_group(count, b, [&](int i) { return S[i].walker0; });
     const afq_system_step& d = S[si];
-    energy_body<float, NT, 1>(d.nw, d.n, d.o, 1, d.naux, d.h0, d.greens, d.rot_chol, d.ldr, d.rot_h1,
+    energy_body<float, NT, 1, F32_WAVES>(d.nw, d.n, d.o, 1, d.naux, d.h0, d.greens, d.rot_chol, d.ldr, d.rot_h1,
                               (double2*)d.energy, b - d.walker0, lds_raw);
 }
 
-template <typename R, int NT, int WPB>
+template <typename R, int NT, int WPB, int NWV>
 size_t energy_lds(int n, int nspin, int naux) {
     constexpr int TC = NT * 16;
     const int NP = afq::round_up(n, 16);
-    size_t b = sizeof(R) * ((size_t)WPB * 2 * NP * (TC + 1) + (size_t)4 * TC * (TC + 1));
+    size_t b = sizeof(R) * ((size_t)WPB * 2 * NP * (TC + 1) + (size_t)NWV * TC * (TC + 1));
     b = (b + 15) & ~(size_t)15;
     if (nspin == 2) b += sizeof(double2) * (size_t)WPB * naux;
     return b;
 }
 
-template <typename R, int NT, int WPB>
+template <typename R, int NT, int WPB, int NWV>
 bool energy_fits(int n, int nspin, int naux) {
-    return energy_lds<R, NT, WPB>(n, nspin, naux) + 4 * 6 * WPB * sizeof(double) <= 160 * 1024;
+    return energy_lds<R, NT, WPB, NWV>(n, nspin, naux) + NWV * 6 * WPB * sizeof(double) <= 160 * 1024;
 }
 
-template <typename R, int NT, int WPB>
+template <typename R, int NT, int WPB, int NWV>
 int launch_energy(const char* name, int nw, int n, int o, int nspin, int naux, double h0,
                   const double* greens, const R* rotp, int ldr, const double* rot_h1,
                   double* energy, hipStream_t st) {
-    const size_t lds = energy_lds<R, NT, WPB>(n, nspin, naux);
-    AFQ_REQUIRE((energy_fits<R, NT, WPB>(n, nspin, naux)),
+    const size_t lds = energy_lds<R, NT, WPB, NWV>(n, nspin, naux);
+    AFQ_REQUIRE((energy_fits<R, NT, WPB, NWV>(n, nspin, naux)),
                 "%s: n=%d o=%d naux=%d needs more than 160 KB of LDS", name, n, o, naux);
-    if (lds > 64 * 1024) afq::allow_full_lds((const void*)energy_kernel<R, NT, WPB>);
-    hipLaunchKernelGGL((energy_kernel<R, NT, WPB>), dim3((unsigned)afq::ceil_div(nw, WPB)),
-                       dim3(THREADS), lds, st, nw, n, o, nspin, naux, h0, greens, rotp, ldr,
+    if (lds > 64 * 1024) afq::allow_full_lds((const void*)energy_kernel<R, NT, WPB, NWV>);
+    hipLaunchKernelGGL((energy_kernel<R, NT, WPB, NWV>), dim3((unsigned)afq::ceil_div(nw, WPB)),
+                       dim3(64 * NWV), lds, st, nw, n, o, nspin, naux, h0, greens, rotp, ldr,
                        rot_h1, (double2*)energy);
     return afq::check_launch(name);
 }
@@ -302,21 +308,28 @@ int energy_entry(const char* name, int nw, int n, int o, int nspin, int naux, do
     AFQ_REQUIRE(ldr >= n && ldr % 4 == 0, "%s: ldr=%d must be >= n and a multiple of 4", name, ldr);
     AFQ_REQUIRE(greens && rotp && rot_h1 && energy, "%s: null pointer", name);
     AFQ_REQUIRE(o <= 32, "%s: o=%d exceeds the supported maximum of 32", name, o);
-    // fp32: two walkers per workgroup when they fit in LDS (halves the
-    // rotated-Cholesky traffic), else one
+    // fp32: eight waves; two walkers per workgroup when they fit in LDS
+    // (halves the rotated-Cholesky traffic), else one (the walker count
+    // changes no sum).  fp64: one walker, eight waves when they fit, else four.
     if constexpr (sizeof(R) == sizeof(float)) {
+        constexpr int W = F32_WAVES;
         if (o <= 16) {
-            if (energy_fits<R, 1, 2>(n, nspin, naux))
-                return launch_energy<R, 1, 2>(name, nw, n, o, nspin, naux, h0, greens, rotp, ldr, rot_h1, energy, st);
-            return launch_energy<R, 1, 1>(name, nw, n, o, nspin, naux, h0, greens, rotp, ldr, rot_h1, energy, st);
+            if (energy_fits<R, 1, 2, W>(n, nspin, naux))
+                return launch_energy<R, 1, 2, W>(name, nw, n, o, nspin, naux, h0, greens, rotp, ldr, rot_h1, energy, st);
+            return launch_energy<R, 1, 1, W>(name, nw, n, o, nspin, naux, h0, greens, rotp, ldr, rot_h1, energy, st);
         }
-        if (energy_fits<R, 2, 2>(n, nspin, naux))
-            return launch_energy<R, 2, 2>(name, nw, n, o, nspin, naux, h0, greens, rotp, ldr, rot_h1, energy, st);
-        return launch_energy<R, 2, 1>(name, nw, n, o, nspin, naux, h0, greens, rotp, ldr, rot_h1, energy, st);
+        if (energy_fits<R, 2, 2, W>(n, nspin, naux))
+            return launch_energy<R, 2, 2, W>(name, nw, n, o, nspin, naux, h0, greens, rotp, ldr, rot_h1, energy, st);
+        return launch_energy<R, 2, 1, W>(name, nw, n, o, nspin, naux, h0, greens, rotp, ldr, rot_h1, energy, st);
     } else {
-        if (o <= 16)
-            return launch_energy<R, 1, 1>(name, nw, n, o, nspin, naux, h0, greens, rotp, ldr, rot_h1, energy, st);
-        return launch_energy<R, 2, 1>(name, nw, n, o, nspin, naux, h0, greens, rotp, ldr, rot_h1, energy, st);
+        if (o <= 16) {
+            if (energy_fits<R, 1, 1, 8>(n, nspin, naux))
+                return launch_energy<R, 1, 1, 8>(name, nw, n, o, nspin, naux, h0, greens, rotp, ldr, rot_h1, energy, st);
+            return launch_energy<R, 1, 1, 4>(name, nw, n, o, nspin, naux, h0, greens, rotp, ldr, rot_h1, energy, st);
+        }
+        if (energy_fits<R, 2, 1, 8>(n, nspin, naux))
+            return launch_energy<R, 2, 1, 8>(name, nw, n, o, nspin, naux, h0, greens, rotp, ldr, rot_h1, energy, st);
+        return launch_energy<R, 2, 1, 4>(name, nw, n, o, nspin, naux, h0, greens, rotp, ldr, rot_h1, energy, st);
     }
 }
 
@@ -625,16 +638,18 @@ extern "C" int afq_energy_f32_grouped(int count, const afq_system_step* sys, int
     AFQ_REQUIRE(max_n > 0 && max_n <= 256, "afq_energy_f32_grouped: max_n=%d", max_n);
     hipStream_t st = (hipStream_t)stream;
     if (nt == 1) {
-        const size_t lds = energy_lds<float, 1, 1>(max_n, 1, 0);
-        AFQ_REQUIRE((energy_fits<float, 1, 1>(max_n, 1, 0)), "afq_energy_f32_grouped: max_n=%d too large", max_n);
+        const size_t lds = energy_lds<float, 1, 1, F32_WAVES>(max_n, 1, 0);
+        AFQ_REQUIRE((energy_fits<float, 1, 1, F32_WAVES>(max_n, 1, 0)), "afq_energy_f32_grouped: max_n=%d too large",
+                    max_n);
         if (lds > 64 * 1024) afq::allow_full_lds((const void*)energy_f32_grouped_kernel<1>);
-        hipLaunchKernelGGL(energy_f32_grouped_kernel<1>, dim3((unsigned)total_walkers), dim3(THREADS), lds, st,
+        hipLaunchKernelGGL(energy_f32_grouped_kernel<1>, dim3((unsigned)total_walkers), dim3(64 * F32_WAVES), lds, st,
                            count, sys);
     } else {
-        const size_t lds = energy_lds<float, 2, 1>(max_n, 1, 0);
-        AFQ_REQUIRE((energy_fits<float, 2, 1>(max_n, 1, 0)), "afq_energy_f32_grouped: max_n=%d too large", max_n);
+        const size_t lds = energy_lds<float, 2, 1, F32_WAVES>(max_n, 1, 0);
+        AFQ_REQUIRE((energy_fits<float, 2, 1, F32_WAVES>(max_n, 1, 0)), "afq_energy_f32_grouped: max_n=%d too large",
+                    max_n);
         if (lds > 64 * 1024) afq::allow_full_lds((const void*)energy_f32_grouped_kernel<2>);
-        hipLaunchKernelGGL(energy_f32_grouped_kernel<2>, dim3((unsigned)total_walkers), dim3(THREADS), lds, st,
+        hipLaunchKernelGGL(energy_f32_grouped_kernel<2>, dim3((unsigned)total_walkers), dim3(64 * F32_WAVES), lds, st,
                            count, sys);
     }
     return afq::check_launch("afq_energy_f32_grouped");
