@@ -17,6 +17,8 @@
 // remapped so that each XCD owns a contiguous range of tiles, and tiles are
 // ordered in groups of 8 M-tiles so that concurrently running workgroups share
 // A and B panels in L2.
+#include <algorithm>
+
 #include "afqmc_hip.h"
 #include "common.h"
 
@@ -179,6 +181,20 @@ __device__ __forceinline__ void gemm_store_sym(int M, T alpha, T* __restrict__ C
     }
 }
 
+// Origin (m0, n0) of tile blockIdx.x of a tiles_m x tiles_n grid: XCD-contiguous
+// remap, then groups of GROUP_M tile rows walked column by column.
+__device__ __forceinline__ void tile_origin(int tiles_m, int tiles_n, int bm, int bn, int& m0, int& n0) {
+    const int total = tiles_m * tiles_n;
+    const int t = afq::xcd_contiguous(blockIdx.x, total);
+    const int per_group = GROUP_M * tiles_n;
+    const int group = t / per_group;
+    const int first_m = group * GROUP_M;
+    const int gsize = min(tiles_m - first_m, GROUP_M);
+    const int in_group = t - group * per_group;
+    m0 = (first_m + in_group % gsize) * bm;
+    n0 = (in_group / gsize) * bn;
+}
+
 template <typename T, typename TOut, bool SYM = false>
 __global__ __launch_bounds__(THREADS, 2) void gemm_nt_kernel(
     int M, int N, int K, T alpha, const T* __restrict__ A, long long lda,
@@ -188,15 +204,8 @@ __global__ __launch_bounds__(THREADS, 2) void gemm_nt_kernel(
     __shared__ T As[2 * CF::BM * PITCH];
     __shared__ T Bs[2 * CF::BN * PITCH];
 
-    const int total = tiles_m * tiles_n;
-    const int t = afq::xcd_contiguous(blockIdx.x, total);
-    const int per_group = GROUP_M * tiles_n;
-    const int group = t / per_group;
-    const int first_m = group * GROUP_M;
-    const int gsize = min(tiles_m - first_m, GROUP_M);
-    const int in_group = t - group * per_group;
-    const int m0 = (first_m + in_group % gsize) * CF::BM;
-    const int n0 = (in_group / gsize) * CF::BN;
+    int m0, n0;
+    tile_origin(tiles_m, tiles_n, CF::BM, CF::BN, m0, n0);
 
     typename CF::acc_t acc[4][4];
     gemm_nt_tile<T, 4, 4>(M, N, K, A, lda, B, ldb, m0, n0, As, Bs, acc);
@@ -204,6 +213,36 @@ __global__ __launch_bounds__(THREADS, 2) void gemm_nt_kernel(
         gemm_store_sym<T, 4, 4>(M, alpha, (T*)C, sd, m0, n0, acc);
     else
         gemm_store<T, TOut, 4, 4>(M, N, alpha, beta, C, ldc, m0, n0, acc);
+}
+
+// One K chunk of a split-K fp64 GEMM: chunk blockIdx.y covers k in
+// [kz kc, min(K, (kz + 1) kc)) and leaves its partial product, unscaled, at
+// part + kz M N (row-major, leading dimension N).
+__global__ __launch_bounds__(THREADS, 2) void dgemm_nt_splitk_kernel(
+    int M, int N, int K, int kc, const double* __restrict__ A, long long lda, const double* __restrict__ B,
+    long long ldb, double* __restrict__ part, int tiles_m, int tiles_n) {
+    using CF = tile_cfg<double, 4, 4>;
+    __shared__ double As[2 * CF::BM * PITCH];
+    __shared__ double Bs[2 * CF::BN * PITCH];
+    int m0, n0;
+    tile_origin(tiles_m, tiles_n, CF::BM, CF::BN, m0, n0);
+    const int k0 = blockIdx.y * kc;
+    typename CF::acc_t acc[4][4];
+    gemm_nt_tile<double, 4, 4>(M, N, min(kc, K - k0), A + k0, lda, B + k0, ldb, m0, n0, As, Bs, acc);
+    gemm_store<double, double, 4, 4>(M, N, 1.0, 0.0, part + (long long)blockIdx.y * M * N, N, m0, n0, acc);
+}
+
+// C = alpha * (part[0] + part[1] + ...), the chunks summed in order
+__global__ void dsplitk_reduce_kernel(int M, int N, int splits, const double* __restrict__ part, double alpha,
+                                      double* __restrict__ C, long long ldc) {
+    const long long total = (long long)M * N;
+    for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < total;
+         e += (long long)gridDim.x * blockDim.x) {
+        double acc = part[e];
+        for (int z = 1; z < splits; ++z) acc += part[z * total + e];
+        const long long r = e / N, c = e - r * N;
+        C[r * ldc + c] = alpha * acc;
+    }
 }
 
 // Grouped fp32 GEMM: workgroup -> (problem, tile) by the problems' tile_start
@@ -299,6 +338,31 @@ extern "C" int afq_dgemm_nt(int M, int N, int K, double alpha, const double* A, 
                             const double* B, long long ldb, double beta, double* C, long long ldc,
                             void* stream) {
     return afq::dgemm_nt(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, (hipStream_t)stream);
+}
+
+extern "C" int afq_dgemm_nt_splitk(int M, int N, int K, double alpha, const double* A, long long lda,
+                                   const double* B, long long ldb, double* C, long long ldc, int splits,
+                                   double* workspace, void* stream) {
+    AFQ_REQUIRE(M >= 0 && N >= 0 && K >= 0, "afq_dgemm_nt_splitk: negative size (M=%d N=%d K=%d)", M, N, K);
+    if (M == 0 || N == 0) return 0;
+    AFQ_REQUIRE(splits >= 1 && splits <= 16, "afq_dgemm_nt_splitk: splits=%d not in 1..16", splits);
+    AFQ_REQUIRE(A && B && C && workspace, "afq_dgemm_nt_splitk: null pointer");
+    AFQ_REQUIRE(lda >= K && ldb >= K && ldc >= N, "afq_dgemm_nt_splitk: leading dimension too small");
+    using CF = tile_cfg<double, 4, 4>;
+    const int tiles_m = afq::ceil_div(M, CF::BM);
+    const int tiles_n = afq::ceil_div(N, CF::BN);
+    AFQ_REQUIRE((long long)tiles_m * tiles_n < (1LL << 31), "afq_dgemm_nt_splitk: problem too large");
+    // chunks of whole 16-deep k-blocks, none empty
+    const int kc = K == 0 ? BK : afq::round_up(afq::ceil_div(K, splits), BK);
+    const int s = K == 0 ? 1 : afq::ceil_div(K, kc);
+    const long long total = (long long)M * N;
+    hipStream_t st = (hipStream_t)stream;
+    hipLaunchKernelGGL(dgemm_nt_splitk_kernel, dim3((unsigned)(tiles_m * tiles_n), (unsigned)s), dim3(THREADS), 0, st,
+                       M, N, K, kc, A, lda, B, ldb, workspace, tiles_m, tiles_n);
+    const long long blocks = std::min<long long>((total + 255) / 256, 4096);
+    hipLaunchKernelGGL(dsplitk_reduce_kernel, dim3((unsigned)blocks), dim3(256), 0, st, M, N, s, workspace, alpha, C,
+                       ldc);
+    return afq::check_launch("afq_dgemm_nt_splitk");
 }
 
 extern "C" int afq_force_bias(int nw, int naux, int k, double alpha, const double* greens,

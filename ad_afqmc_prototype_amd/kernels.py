@@ -275,8 +275,20 @@ def overlap_greens(walkers, trial_up, trial_dn=None, greens=None, logdet=None, s
     return greens, logdet
 
 
-def force_bias(greens, rot, alpha, fb=None):
-    """fb planes (2, nw, naux) = alpha * greens_rows . rot^T."""
+def splitk_for_f64(M: int, N: int, K: int) -> int:
+    """K chunks for the fp64 GEMM (128 x 128 tiles, two workgroups per CU):
+    whole when the tile grid covers 256 CUs twice; otherwise enough chunks (at
+    most 8, each >= 512 deep) for about four workgroups per CU."""
+    tiles = ((M + 127) // 128) * ((N + 127) // 128)
+    if tiles >= 512:
+        return 1
+    return max(1, min(8, 1024 // tiles, K // 512))
+
+
+def force_bias(greens, rot, alpha, fb=None, splits=None):
+    """fb planes (2, nw, naux) = alpha * greens_rows . rot^T.  ``splits`` > 1
+    cuts the contraction (K = nspin*o*n) into that many chunks run side by side
+    and summed in order; None picks it from the shape (splitk_for_f64)."""
     _, nw, k = greens.shape
     naux = rot.shape[0]
     _check(greens, f64, (2, nw, k), "greens")
@@ -284,6 +296,13 @@ def force_bias(greens, rot, alpha, fb=None):
     if fb is None:
         fb = torch.empty((2, nw, naux), dtype=f64, device=greens.device)
     _check(fb, f64, (2, nw, naux), "fb", greens.device)
+    if splits is None:
+        splits = splitk_for_f64(2 * nw, naux, k)
+    if splits > 1:
+        ws = torch.empty((splits, 2 * nw, naux), dtype=f64, device=greens.device)
+        _call("afq_dgemm_nt_splitk", greens, 2 * nw, naux, k, float(alpha), greens.data_ptr(), k, rot.data_ptr(), k,
+              fb.data_ptr(), naux, int(splits), ws.data_ptr(), _stream(greens), tag="force_bias")
+        return fb
     _call("afq_force_bias", greens, nw, naux, k, float(alpha), greens.data_ptr(), rot.data_ptr(),
               fb.data_ptr(), _stream(greens))
     return fb

@@ -417,6 +417,7 @@ def kernel_table(ktimes, n, o, g, nw, nspin, rhf, precision):
     spec = {
         "afq_vhs_sym:vhs": (fl["vhs"] * nw, precision),
         "afq_force_bias": (fl["force_bias"] * nw, "fp64"),
+        "afq_dgemm_nt_splitk:force_bias": (fl["force_bias"] * nw, "fp64"),
         "afq_sgemm_nt:force_bias": (fl["force_bias"] * nw, "fp32"),
         "afq_gemm_bf16x3_nt:force_bias": (fl["force_bias"] * nw, "bf16x3"),
         "afq_gemm_bf16x3_nt_splitk:force_bias": (fl["force_bias"] * nw, "bf16x3"),

@@ -40,6 +40,18 @@ int afq_dgemm_nt(int M, int N, int K, double alpha, const double* A, long long l
                  void* stream);
 
 /*
+ * fp64 GEMM  C = alpha * A * B^T  with K cut into `splits` chunks (1..16, each
+ * a multiple of 16 deep) run side by side: each chunk's partial product goes
+ * to workspace (splits * M * N doubles, caller-owned), then the chunks are
+ * summed in order (deterministic).  For problems with few output tiles and a
+ * long K, the force bias of the fp64 step (meas/rhf.py:27-33, 60-72: A = the
+ * Green's-function rows, B = the rotated Cholesky rows, K = nspin*o*n).
+ */
+int afq_dgemm_nt_splitk(int M, int N, int K, double alpha, const double* A, long long lda,
+                        const double* B, long long ldb, double* C, long long ldc, int splits,
+                        double* workspace, void* stream);
+
+/*
  * fp32 GEMM  C = alpha * A * B^T + beta * C  on v_mfma_f32_16x16x4_f32, same
  * layout as afq_dgemm_nt; C is float32, or float64 when c_fp64 != 0.
  * Force-bias / VHS contractions of the mixed step in fp32 mode (meas/rhf.py:33,

@@ -47,6 +47,11 @@ def test_library_loads_and_reports_abi():
 @pytest.mark.parametrize("name,args,msg", [
     ("afq_dgemm_nt", (-1, 2, 2, 1.0, 1, 2, 1, 2, 0.0, 1, 2, 0), "negative size"),
     ("afq_dgemm_nt", (4, 4, 4, 1.0, 1, 2, 1, 4, 0.0, 1, 4, 0), "leading dimension"),
+    ("afq_dgemm_nt_splitk", (-1, 4, 4, 1.0, 16, 4, 16, 4, 16, 4, 2, 16, 0), "negative size"),
+    ("afq_dgemm_nt_splitk", (4, 4, 64, 1.0, 16, 64, 16, 64, 16, 4, 0, 16, 0), "splits"),
+    ("afq_dgemm_nt_splitk", (4, 4, 64, 1.0, 16, 64, 16, 64, 16, 4, 17, 16, 0), "splits"),
+    ("afq_dgemm_nt_splitk", (4, 4, 64, 1.0, 16, 64, 16, 64, 16, 4, 2, 0, 0), "null"),
+    ("afq_dgemm_nt_splitk", (4, 4, 64, 1.0, 16, 32, 16, 64, 16, 4, 2, 16, 0), "leading dimension"),
     ("afq_overlap_greens", (4, 8, 9, 1, 1, 0, 1, 0, 1, 1, 0), "bad sizes"),
     ("afq_overlap_greens", (4, 100, 70, 1, 1, 0, 1, 0, 1, 1, 0), "maximum of 64"),
     ("afq_overlap_greens", (4, 8, 2, 2, 1, 0, 1, 0, 1, 1, 0), "spin-down"),

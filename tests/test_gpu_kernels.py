@@ -40,6 +40,31 @@ def test_dgemm_matches_torch_fp64(gpu_device):
         assert err < 1e-13, f"M={M} N={N} K={K} rel err {err:.3e}"
 
 
+@pytest.mark.parametrize("nw,naux,k,splits", [
+    (3, 5, 7, 2),          # K shorter than one chunk of 16: a single chunk runs
+    (40, 150, 630, 3),     # last chunk short (224-deep chunks: 224, 224, 182)
+    (64, 96, 1000, 8),
+    (100, 600, 4800, None),  # cfg2-like: the shape picks 8 chunks
+])
+def test_force_bias_split_k(gpu_device, nw, naux, k, splits):
+    """The split-K fp64 force bias against a CPU product (each chunk sums its
+    k range, the chunks are added in order: fp64 rounding only), and identical
+    across two calls (fixed order)."""
+    import torch
+    from ad_afqmc_prototype_amd import kernels
+    g = torch.Generator(device="cpu").manual_seed(nw + k)
+    G = torch.randn((2, nw, k), dtype=torch.float64, generator=g)
+    R = torch.randn((naux, k), dtype=torch.float64, generator=g)
+    ref = 2.0 * torch.einsum("pwk,gk->pwg", G, R)
+    Gd, Rd = G.to(gpu_device), R.to(gpu_device)
+    fb1 = kernels.force_bias(Gd, Rd, 2.0, splits=splits)
+    fb2 = kernels.force_bias(Gd, Rd, 2.0, splits=splits)
+    whole = kernels.force_bias(Gd, Rd, 2.0, splits=1)
+    assert torch.equal(fb1, fb2)
+    assert rel(fb1.cpu().numpy(), ref.numpy()) < 1e-13
+    assert rel(whole.cpu().numpy(), ref.numpy()) < 1e-13
+
+
 @pytest.mark.parametrize("seed,n,o,naux,uhf,kind", SHAPES)
 def test_greens_and_log_overlap(gpu_device, seed, n, o, naux, uhf, kind):
     from ad_afqmc_prototype_amd.trial import rhf as trial_rhf
