@@ -529,6 +529,10 @@ def main():
                     help="skip the adjoint field (gradient of the 96-orbital run vs central differences)")
     ap.add_argument("--no-cadence", action="store_true",
                     help="time afqmc_step alone (no QR / measurement / reconfiguration)")
+    ap.add_argument("--fp64-production", default="n120_o20_g600_w1024_uhf", metavar="WORKLOAD",
+                    help="workload of the fp64_production field (the plain fp64 method at production "
+                         "shape: 120 orbitals, 20 occupied per spin, UHF trial, 600 Cholesky vectors, "
+                         "1024 walkers); '' skips it")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -603,6 +607,21 @@ def main():
         out["speedup_vs_fp64"] = out["value"] / out["fp64_value"]
         out["fp64_kernels"] = kernel_table(kt64, n, o, g, nw, nspin, not cfg["uhf"], "fp64")
         out["block_energy_abs_diff_mHa"] = [1e3 * abs(a - b) for a, b in zip(e_block, e64)]
+    if args.fp64_production:
+        import copy
+        pargs = copy.copy(args)
+        pargs.workload = args.fp64_production
+        pc = syn.WORKLOADS[pargs.workload]
+        pn, po, pg, pnw = pc["norb"], pc["nocc"], pc["naux"], pc["n_walkers"]
+        pspin = 2 if pc["walker_kind"] == "unrestricted" else 1
+        elp, ktp, ep, _, elp_graph = timed_run(pargs, "fp64", device)
+        out["fp64_production"] = {
+            "workload": pargs.workload, "norb": pn, "nocc": po, "naux": pg, "n_walkers": pnw,
+            "walker_kind": pc["walker_kind"], "dt": pc["dt"],
+            "walker_steps_per_s": pnw * args.steps / elp, "ms_per_step": 1e3 * elp / args.steps,
+            "ms_per_step_graph": None if elp_graph is None else 1e3 * elp_graph / args.steps,
+            "block_energies": ep,
+            "kernels": kernel_table(ktp, pn, po, pg, pnw, pspin, not pc["uhf"], "fp64")}
     if args.precision != "fp64" and args.accuracy_scale != 0:
         scale = cfg["chol_scale"] if args.accuracy_scale is None else args.accuracy_scale
         out["energy_check"] = accuracy_check(args.workload, args.precision, device, scale,
