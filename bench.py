@@ -532,7 +532,7 @@ def main():
     ap.add_argument("--fp64-production", default="n120_o20_g600_w1024_uhf", metavar="WORKLOAD",
                     help="workload of the fp64_production field (the plain fp64 method at production "
                          "shape: 120 orbitals, 20 occupied per spin, UHF trial, 600 Cholesky vectors, "
-                         "1024 walkers); '' skips it")
+                         "1024 walkers); 'none' skips it")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -607,7 +607,7 @@ def main():
         out["speedup_vs_fp64"] = out["value"] / out["fp64_value"]
         out["fp64_kernels"] = kernel_table(kt64, n, o, g, nw, nspin, not cfg["uhf"], "fp64")
         out["block_energy_abs_diff_mHa"] = [1e3 * abs(a - b) for a, b in zip(e_block, e64)]
-    if args.fp64_production:
+    if args.fp64_production not in ("", "none"):
         import copy
         pargs = copy.copy(args)
         pargs.workload = args.fp64_production
