@@ -293,6 +293,14 @@ class system_batch:
             self.trotter_buckets.append(dict(
                 tables=[_upload(t[0]).to(self.device) for t in tabs], count=len(idx),
                 blocks=tabs[0][2], max_n=max(self.members[i]["n"] for i in idx)))
+        # Green's-function launches per bucket of the kernel's occupied-orbital
+        # capacity (8, 16 or 32: its per-wave LDS and registers scale with it)
+        self.greens_buckets = []
+        for idx in buckets(lambda m: 8 if m["o"] <= 8 else 16 if m["o"] <= 16 else 32):
+            tabs = [self._desc(p, idx) for p in (0, 1)]
+            self.greens_buckets.append(dict(
+                tables=[_upload(t[0]).to(self.device) for t in tabs], count=len(idx), walkers=tabs[0][1],
+                max_o=max(self.members[i]["o"] for i in idx)))
         self.energy_buckets = []
         for idx in buckets(lambda m: (m["o"] + 15) // 16):
             arr, nw, _ = self._desc(0, idx)
@@ -319,6 +327,13 @@ class system_batch:
         kernels._call("afq_convert_f32", src.buf, 1, n, src.buf.data_ptr(), n, dst.buf.data_ptr(), n,
                       kernels._stream(src.buf))
 
+    def _greens(self, p: int, st: int):
+        """Green's functions and log-overlaps of every system's new walkers
+        (buffer set p), one launch per occupied-orbital bucket."""
+        for bk in self.greens_buckets:
+            t = bk["tables"][p]
+            kernels._call("afq_overlap_greens_grouped", t, bk["count"], t.data_ptr(), bk["walkers"], bk["max_o"], st)
+
     def step(self):
         """One step of every system (fields drawn from each system's key)."""
         p = self.steps_done % 2
@@ -334,8 +349,7 @@ class system_batch:
             t = bk["tables"][p]
             kernels._call("afq_trotter_f32_grouped", t, bk["count"], t.data_ptr(), bk["blocks"],
                           bk["max_n"], self.n_exp_terms, st)
-        kernels._call("afq_overlap_greens_grouped", d, self.count, d.data_ptr(), self.total_walkers,
-                      self.max_o, st)
+        self._greens(p, st)
         kernels._call("afq_phaseless_update_grouped", d, self.count, d.data_ptr(), st)
         self.steps_done += 1
 
@@ -348,8 +362,7 @@ class system_batch:
         st = kernels._stream(d)
         kernels._call("afq_orthonormalize_grouped", d, self.count, d.data_ptr(), self.total_walkers,
                       self.max_n, self.max_o, st)
-        kernels._call("afq_overlap_greens_grouped", d, self.count, d.data_ptr(), self.total_walkers,
-                      self.max_o, st)
+        self._greens(0, st)
         self.walkers[0].buf.copy_(self.walkers[1].buf)
         torch.mul(self.logdet.buf, 2.0, out=self.lo[0].buf)
 
