@@ -16,9 +16,9 @@ namespace {
 
 
 // Walkers w0 .. w0 + WPB - 1 of one batch (nw walkers); NWV waves, each on its
-// own g.
+// own g: f_g on an NT x NT grid of 16 x 16 tiles.
 template <typename R, int NT, int WPB, int NWV>
-__device__ __forceinline__ void energy_body(
+__device__ __forceinline__ void energy_body_g(
     int nw, int n, int o, int nspin, int naux, double h0, const double* __restrict__ greens,
     const R* __restrict__ rotp, int ldr, const double* __restrict__ rot_h1,
     double2* __restrict__ eout, int w0, unsigned char* lds_raw) {
@@ -245,19 +245,321 @@ __device__ __forceinline__ void energy_body(
     }
 }
 
-template <typename R, int NT, int WPB, int NWV>
+// Walkers w0 .. w0 + WPB - 1 of one batch (nw walkers); NWV waves.  The
+// contraction f_g = R_g G^T runs on stacked rows: a chunk holds q = 16 T / o
+// whole g's (o rows each) in T row tiles of 16, so with o not a multiple of 16
+// far fewer MFMA rows are padding than with one 16-row tile grid per g; each
+// wave takes chunks wave, wave + NWV, ...
+template <typename R, int NT, int WPB, int NWV, int T>
+__device__ __forceinline__ void energy_body_stacked(
+    int nw, int n, int o, int nspin, int naux, double h0, const double* __restrict__ greens,
+    const R* __restrict__ rotp, int ldr, const double* __restrict__ rot_h1,
+    double2* __restrict__ eout, int w0, unsigned char* lds_raw) {
+    using MF = afq::mfma16<R>;
+    using ACC = typename MF::acc;
+    using V4 = typename afq::vec4<R>::type;
+    constexpr int TC = NT * 16;
+    constexpr int XP = TC + 1;
+    constexpr int SP = TC + 1;
+    constexpr int THREADS = 64 * NWV;
+    __shared__ double red[NWV][6 * WPB];
+    const int NP = afq::round_up(n, 16);
+    R* X = reinterpret_cast<R*>(lds_raw);               // [WPB][2][NP][XP]
+    R* S = X + (size_t)WPB * 2 * NP * XP;               // [NWV][16 T][SP]
+    size_t cs_off = ((size_t)WPB * 2 * NP * XP + NWV * 16 * T * SP) * sizeof(R);
+    cs_off = (cs_off + 15) & ~(size_t)15;
+    double2* cs = reinterpret_cast<double2*>(lds_raw + cs_off);  // [WPB][naux] (nspin == 2)
+
+    const int nwb = min(WPB, nw - w0);
+    const long long ldg = (long long)nspin * o * n;
+    const int tid = threadIdx.x;
+    const int lane = tid & 63;
+    const int wave = tid >> 6;
+    const int lr = lane & 15;
+    const int lk = (lane >> 4) * 4;
+    R* Sw = S + wave * 16 * T * SP;
+
+    double e1r[WPB], e1i[WPB], exr[WPB], exi[WPB], c2r[WPB], c2i[WPB];
+#pragma unroll
+    for (int b = 0; b < WPB; ++b) e1r[b] = e1i[b] = exr[b] = exi[b] = c2r[b] = c2i[b] = 0.0;
+
+    for (int s = 0; s < nspin; ++s) {
+        __syncthreads();
+#pragma unroll
+        for (int b = 0; b < WPB; ++b) {
+            R* Xr = X + (size_t)(b * 2) * NP * XP;
+            R* Xi = Xr + (size_t)NP * XP;
+            const bool have = b < nwb;
+            const double* gre = greens + (long long)(w0 + (have ? b : 0)) * ldg + (long long)s * o * n;
+            const double* gim = gre + (long long)nw * ldg;
+            for (int e = tid; e < NP * TC; e += THREADS) {
+                const int j = e / NP;
+                const int p = e - j * NP;
+                double vr = 0.0, vi = 0.0;
+                if (have && p < n && j < o) {
+                    vr = gre[j * n + p];
+                    vi = gim[j * n + p];
+                }
+                Xr[p * XP + j] = (R)vr;
+                Xi[p * XP + j] = (R)vi;
+            }
+            if (have) {
+                const double* rh = rot_h1 + (long long)s * o * n;
+                for (int e = tid; e < o * n; e += THREADS) {
+                    const double h = rh[e];
+                    e1r[b] += gre[e] * h;
+                    e1i[b] += gim[e] * h;
+                }
+            }
+        }
+        __syncthreads();
+
+        const R* Rs = rotp + (long long)s * naux * TC * ldr;
+        const int nkb = afq::ceil_div(n, 16);
+        // chunks of q whole g's stacked into T row tiles of 16: stacked row
+        // r = gl * o + i is row i of f_{g0 + gl}; rows past q * o are zero
+        const int q = min(16 * T / o, 16);
+        const int nchunk = afq::ceil_div(naux, q);
+        // row / o for rows < 64 as (row + 1/2) (1/o) rounded down: the fraction
+        // stays at least 1/(2o) away from an integer, far above the rounding
+        const float inv_o = 1.0f / (float)o;
+        for (int c = wave; c < nchunk; c += NWV) {
+            const int g0 = c * q;
+            const int qc = min(q, naux - g0);
+            ACC F[WPB][T][NT][2];
+#pragma unroll
+            for (int b = 0; b < WPB; ++b)
+#pragma unroll
+                for (int t = 0; t < T; ++t)
+#pragma unroll
+                    for (int tj = 0; tj < NT; ++tj) {
+                        F[b][t][tj][0] = MF::zero();
+                        F[b][t][tj][1] = MF::zero();
+                    }
+            // this lane's A rows (t * 16 + lr of the stack) in the padded
+            // (g, TC, ldr) layout of the rotated Cholesky vectors
+            const R* arow[T];
+            bool aok[T];
+#pragma unroll
+            for (int t = 0; t < T; ++t) {
+                const int r = t * 16 + lr;
+                const int gl = (int)(((float)r + 0.5f) * inv_o);
+                aok[t] = gl < qc;
+                arow[t] = Rs + ((long long)(g0 + (aok[t] ? gl : 0)) * TC + (aok[t] ? r - gl * o : 0)) * ldr;
+            }
+            // A fragments of k-block kb: k = kb * 16 + lk .. + 3 (zero past n);
+            // loaded two k-blocks ahead of their use
+            auto load_a = [&](int kb, R (&a)[T][4]) {
+                const int k0 = kb * 16 + lk;
+#pragma unroll
+                for (int t = 0; t < T; ++t) {
+                    if (aok[t] && k0 < n) {
+                        const V4 v = *reinterpret_cast<const V4*>(arow[t] + k0);
+                        a[t][0] = v.x; a[t][1] = v.y; a[t][2] = v.z; a[t][3] = v.w;
+                    } else {
+                        a[t][0] = a[t][1] = a[t][2] = a[t][3] = R(0);
+                    }
+                }
+            };
+            auto product = [&](int kb, const R (&a)[T][4]) {
+                const int k0 = kb * 16 + lk;
+#pragma unroll
+                for (int b = 0; b < WPB; ++b) {
+                    const R* Xr = X + (size_t)(b * 2) * NP * XP;
+                    const R* Xi = Xr + (size_t)NP * XP;
+#pragma unroll
+                    for (int kk = 0; kk < 4; ++kk) {
+                        R br[NT], bi[NT];
+#pragma unroll
+                        for (int tj = 0; tj < NT; ++tj) {
+                            br[tj] = Xr[(k0 + kk) * XP + tj * 16 + lr];
+                            bi[tj] = Xi[(k0 + kk) * XP + tj * 16 + lr];
+                        }
+#pragma unroll
+                        for (int t = 0; t < T; ++t)
+#pragma unroll
+                            for (int tj = 0; tj < NT; ++tj) {
+                                F[b][t][tj][0] = MF::fma(a[t][kk], br[tj], F[b][t][tj][0]);
+                                F[b][t][tj][1] = MF::fma(a[t][kk], bi[tj], F[b][t][tj][1]);
+                            }
+                    }
+                }
+            };
+            // fragment sets in rotation (three, two k-blocks ahead; two, one
+            // ahead, for four row tiles, whose accumulators need the
+            // registers), the loop unrolled by the set count so that no
+            // in-flight set is ever copied
+            if constexpr (T == 4) {
+                R a0[T][4], a1[T][4];
+                load_a(0, a0);
+                for (int kb = 0; kb < nkb; kb += 2) {
+                    if (kb + 1 < nkb) load_a(kb + 1, a1);
+                    product(kb, a0);
+                    if (kb + 1 < nkb) {
+                        if (kb + 2 < nkb) load_a(kb + 2, a0);
+                        product(kb + 1, a1);
+                    }
+                }
+            } else {
+                R a0[T][4], a1[T][4], a2[T][4];
+                load_a(0, a0);
+                if (nkb > 1) load_a(1, a1);
+                auto step = [&](int kb, const R (&cur)[T][4], R (&ahead)[T][4]) {
+                    if (kb + 2 < nkb) load_a(kb + 2, ahead);
+                    product(kb, cur);
+                };
+                for (int kb = 0; kb < nkb; kb += 3) {
+                    step(kb, a0, a2);
+                    if (kb + 1 < nkb) step(kb + 1, a1, a0);
+                    if (kb + 2 < nkb) step(kb + 2, a2, a1);
+                }
+            }
+#pragma unroll
+            for (int b = 0; b < WPB; ++b) {
+                // Each plane (real, then imaginary) goes through this wave's own
+                // LDS tile (T * 16 rows): lane gl < qc sums the diagonal of
+                // f_{g0 + gl} (Coulomb), and every lane adds f_ij f_ji of its
+                // entries (exchange).  A wave's LDS accesses execute in program
+                // order; the asm keeps the compiler from moving them across one
+                // another (no workgroup barrier in the chunk loop).
+                double cre = 0.0, cim = 0.0;
+                // the lane id through an empty asm: the entry and partner
+                // indices below are recomputed per chunk instead of being
+                // kept in registers across the chunk loop
+                int lx = lane;
+                asm volatile("" : "+v"(lx));
+                const int lrx = lx & 15;
+#pragma unroll
+                for (int pl = 0; pl < 2; ++pl) {
+#pragma unroll
+                    for (int t = 0; t < T; ++t)
+#pragma unroll
+                        for (int tj = 0; tj < NT; ++tj)
+#pragma unroll
+                            for (int r = 0; r < 4; ++r) {
+                                const int row = t * 16 + MF::row(lx, r);
+                                const int col = tj * 16 + lrx;
+                                Sw[row * SP + col] = F[b][t][tj][pl][r];
+                            }
+                    asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
+                    {
+                        // lane l holds the diagonal entry of stacked row l; the
+                        // trace of f_{g0 + gl} is the sum over its o rows
+                        const int glx = (int)(((float)lx + 0.5f) * inv_o);
+                        const double d = lx < qc * o ? (double)Sw[lx * SP + (lx - glx * o)] : 0.0;
+                        for (int gl = 0; gl < qc; ++gl) {
+                            const double sg = afq::wave_sum(glx == gl ? d : 0.0);
+                            if (lx == gl) {
+                                if (pl == 0)
+                                    cre = sg;
+                                else
+                                    cim = sg;
+                            }
+                        }
+                    }
+#pragma unroll
+                    for (int t = 0; t < T; ++t)
+#pragma unroll
+                        for (int r = 0; r < 4; ++r) {
+                            const int row = t * 16 + MF::row(lx, r);
+                            const int gl = (int)(((float)row + 0.5f) * inv_o);
+                            const int i = row - gl * o;
+#pragma unroll
+                            for (int tj = 0; tj < NT; ++tj) {
+                                const int col = tj * 16 + lrx;
+                                if (gl < qc && col < o) {
+                                    const double x = (double)Sw[(gl * o + col) * SP + i];
+                                    // converted at each use (an empty asm per plane keeps
+                                    // the compiler from holding both planes' conversions)
+                                    R fr = F[b][t][tj][0][r], fi = F[b][t][tj][1][r];
+                                    asm volatile("" : "+v"(fr), "+v"(fi));
+                                    if (pl == 0) {
+                                        exr[b] += (double)fr * x;
+                                        exi[b] += (double)fi * x;
+                                    } else {
+                                        exr[b] -= (double)fi * x;
+                                        exi[b] += (double)fr * x;
+                                    }
+                                }
+                            }
+                        }
+                    asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
+                }
+                // Coulomb (c_g = tr f_g; per lane, summed over lanes at the end)
+                if (lane < qc && b < nwb) {
+                    const int g = g0 + lane;
+                    if (nspin == 1) {
+                        c2r[b] += cre * cre - cim * cim;
+                        c2i[b] += 2.0 * cre * cim;
+                    } else if (s == 0) {
+                        cs[b * naux + g] = make_double2(cre, cim);
+                    } else {
+                        const double2 cu = cs[b * naux + g];
+                        const double tr = cu.x + cre;
+                        const double tim = cu.y + cim;
+                        c2r[b] += tr * tr - tim * tim;
+                        c2i[b] += 2.0 * tr * tim;
+                    }
+                }
+            }
+        }
+    }
+
+#pragma unroll
+    for (int b = 0; b < WPB; ++b) {
+        const double v[6] = {afq::wave_sum(e1r[b]), afq::wave_sum(e1i[b]), afq::wave_sum(exr[b]),
+                             afq::wave_sum(exi[b]), afq::wave_sum(c2r[b]), afq::wave_sum(c2i[b])};
+        if (lane == 0)
+            for (int k = 0; k < 6; ++k) red[wave][6 * b + k] = v[k];
+    }
+    __syncthreads();
+    if (tid < nwb) {
+        const int b = tid;
+        double v[6] = {0, 0, 0, 0, 0, 0};
+        for (int q = 0; q < NWV; ++q)
+            for (int k = 0; k < 6; ++k) v[k] += red[q][6 * b + k];
+        double er, ei;
+        if (nspin == 1) {
+            er = h0 + 2.0 * v[0] + 2.0 * v[4] - v[2];
+            ei = 2.0 * v[1] + 2.0 * v[5] - v[3];
+        } else {
+            er = h0 + v[0] + 0.5 * (v[4] - v[2]);
+            ei = v[1] + 0.5 * (v[5] - v[3]);
+        }
+        eout[w0 + b] = make_double2(er, ei);
+    }
+}
+
+// Kernel shape of the f_g contraction for o occupied orbitals: 4 (stacked
+// chunks of four row tiles) when 64 stacked rows hold more than twice the
+// whole g's of 32 (o = 7, 9, 11, 12, 17..21: the per-g tile grid would be
+// mostly padding), else 0 (one NT x NT tile grid per g, which measures faster
+// when stacking gains little).  A function of o alone, so a system gets the
+// same partition of g's (and the same sums) in the single-system and the
+// grouped kernels.
+__host__ __device__ inline int stack_tiles(int o) {
+    const int q2 = min(32 / o, 16), q4 = min(64 / o, 16);
+    return q4 > 2 * q2 ? 4 : 0;
+}
+
+template <typename R, int NT, int WPB, int NWV, int T>
 __global__ __launch_bounds__(64 * NWV, 1) void energy_kernel(
     int nw, int n, int o, int nspin, int naux, double h0, const double* __restrict__ greens,
     const R* __restrict__ rotp, int ldr, const double* __restrict__ rot_h1,
     double2* __restrict__ eout) {
     extern __shared__ __attribute__((aligned(16))) unsigned char lds_raw[];
-    energy_body<R, NT, WPB, NWV>(nw, n, o, nspin, naux, h0, greens, rotp, ldr, rot_h1, eout,
-                            blockIdx.x * WPB, lds_raw);
+    if constexpr (T == 0)
+        energy_body_g<R, NT, WPB, NWV>(nw, n, o, nspin, naux, h0, greens, rotp, ldr, rot_h1, eout, blockIdx.x * WPB,
+                                       lds_raw);
+    else
+        energy_body_stacked<R, NT, WPB, NWV, T>(nw, n, o, nspin, naux, h0, greens, rotp, ldr, rot_h1, eout,
+                                                blockIdx.x * WPB, lds_raw);
 }
 
 // One workgroup per walker of every system (restricted walkers, fp32
-// contractions); every system has ceil(o / 16) == NT.  Eight waves, as the
-// single-system fp32 kernel always has (same g partition, so the same sums).
+// contractions); every system has ceil(o / 16) == NT.  Eight waves and the
+// chunk shape stack_tiles(o), as the single-system fp32 kernel has (same g
+// partition, so the same sums); the LDS is sized for four row tiles.
 constexpr int F32_WAVES = 8;
 template <int NT>
 __global__ __launch_bounds__(64 * F32_WAVES, 1) void energy_f32_grouped_kernel(
@@ -266,37 +568,69 @@ __global__ __launch_bounds__(64 * F32_WAVES, 1) void energy_f32_grouped_kernel(
     const int b = blockIdx.x;
     const int si = afq::find_group(count, b, [&](int i) { return S[i].walker0; });
     const afq_system_step& d = S[si];
-    energy_body<float, NT, 1, F32_WAVES>(d.nw, d.n, d.o, 1, d.naux, d.h0, d.greens, d.rot_chol, d.ldr, d.rot_h1,
-                              (double2*)d.energy, b - d.walker0, lds_raw);
+    if (stack_tiles(d.o) == 4)
+        energy_body_stacked<float, NT, 1, F32_WAVES, 4>(d.nw, d.n, d.o, 1, d.naux, d.h0, d.greens, d.rot_chol,
+                                                        d.ldr, d.rot_h1, (double2*)d.energy, b - d.walker0, lds_raw);
+    else
+        energy_body_g<float, NT, 1, F32_WAVES>(d.nw, d.n, d.o, 1, d.naux, d.h0, d.greens, d.rot_chol, d.ldr,
+                                               d.rot_h1, (double2*)d.energy, b - d.walker0, lds_raw);
 }
 
-template <typename R, int NT, int WPB, int NWV>
+template <typename R, int NT, int WPB, int NWV, int T>
 size_t energy_lds(int n, int nspin, int naux) {
     constexpr int TC = NT * 16;
     const int NP = afq::round_up(n, 16);
-    size_t b = sizeof(R) * ((size_t)WPB * 2 * NP * (TC + 1) + (size_t)NWV * TC * (TC + 1));
+    const int srows = T == 0 ? TC : 16 * T;  // rows of a wave's exchange tile
+    size_t b = sizeof(R) * ((size_t)WPB * 2 * NP * (TC + 1) + (size_t)NWV * srows * (TC + 1));
     b = (b + 15) & ~(size_t)15;
     if (nspin == 2) b += sizeof(double2) * (size_t)WPB * naux;
     return b;
 }
 
-template <typename R, int NT, int WPB, int NWV>
+template <typename R, int NT, int WPB, int NWV, int T>
 bool energy_fits(int n, int nspin, int naux) {
-    return energy_lds<R, NT, WPB, NWV>(n, nspin, naux) + NWV * 6 * WPB * sizeof(double) <= 160 * 1024;
+    return energy_lds<R, NT, WPB, NWV, T>(n, nspin, naux) + NWV * 6 * WPB * sizeof(double) <= 160 * 1024;
 }
 
-template <typename R, int NT, int WPB, int NWV>
+template <typename R, int NT, int WPB, int NWV, int T>
 int launch_energy(const char* name, int nw, int n, int o, int nspin, int naux, double h0,
                   const double* greens, const R* rotp, int ldr, const double* rot_h1,
                   double* energy, hipStream_t st) {
-    const size_t lds = energy_lds<R, NT, WPB, NWV>(n, nspin, naux);
-    AFQ_REQUIRE((energy_fits<R, NT, WPB, NWV>(n, nspin, naux)),
+    const size_t lds = energy_lds<R, NT, WPB, NWV, T>(n, nspin, naux);
+    AFQ_REQUIRE((energy_fits<R, NT, WPB, NWV, T>(n, nspin, naux)),
                 "%s: n=%d o=%d naux=%d needs more than 160 KB of LDS", name, n, o, naux);
-    if (lds > 64 * 1024) afq::allow_full_lds((const void*)energy_kernel<R, NT, WPB, NWV>);
-    hipLaunchKernelGGL((energy_kernel<R, NT, WPB, NWV>), dim3((unsigned)afq::ceil_div(nw, WPB)),
+    if (lds > 64 * 1024) afq::allow_full_lds((const void*)energy_kernel<R, NT, WPB, NWV, T>);
+    hipLaunchKernelGGL((energy_kernel<R, NT, WPB, NWV, T>), dim3((unsigned)afq::ceil_div(nw, WPB)),
                        dim3(64 * NWV), lds, st, nw, n, o, nspin, naux, h0, greens, rotp, ldr,
                        rot_h1, (double2*)energy);
     return afq::check_launch(name);
+}
+
+// T = 0: per-g tile grid, T = 4: stacked chunks.  fp32: eight waves; two
+// walkers per workgroup when they fit in LDS (halves the rotated-Cholesky
+// traffic) and registers (not stacked with two column tiles), else one (the
+// walker count changes no sum).  fp64: one walker, eight waves when they fit
+// (per-g grid only), else four.
+template <typename R, int NT, int T>
+int energy_dispatch(const char* name, int nw, int n, int o, int nspin, int naux, double h0,
+                    const double* greens, const R* rotp, int ldr, const double* rot_h1, double* energy,
+                    hipStream_t st) {
+    if constexpr (sizeof(R) == sizeof(float)) {
+        constexpr int W = F32_WAVES;
+        if constexpr (T == 0 || NT == 1) {
+            if (energy_fits<R, NT, 2, W, T>(n, nspin, naux))
+                return launch_energy<R, NT, 2, W, T>(name, nw, n, o, nspin, naux, h0, greens, rotp, ldr, rot_h1, energy,
+                                                     st);
+        }
+        return launch_energy<R, NT, 1, W, T>(name, nw, n, o, nspin, naux, h0, greens, rotp, ldr, rot_h1, energy, st);
+    } else {
+        if constexpr (T == 0) {
+            if (energy_fits<R, NT, 1, 8, T>(n, nspin, naux))
+                return launch_energy<R, NT, 1, 8, T>(name, nw, n, o, nspin, naux, h0, greens, rotp, ldr, rot_h1, energy,
+                                                     st);
+        }
+        return launch_energy<R, NT, 1, 4, T>(name, nw, n, o, nspin, naux, h0, greens, rotp, ldr, rot_h1, energy, st);
+    }
 }
 
 template <typename R>
@@ -308,29 +642,19 @@ int energy_entry(const char* name, int nw, int n, int o, int nspin, int naux, do
     AFQ_REQUIRE(ldr >= n && ldr % 4 == 0, "%s: ldr=%d must be >= n and a multiple of 4", name, ldr);
     AFQ_REQUIRE(greens && rotp && rot_h1 && energy, "%s: null pointer", name);
     AFQ_REQUIRE(o <= 32, "%s: o=%d exceeds the supported maximum of 32", name, o);
-    // fp32: eight waves; two walkers per workgroup when they fit in LDS
-    // (halves the rotated-Cholesky traffic), else one (the walker count
-    // changes no sum).  fp64: one walker, eight waves when they fit, else four.
-    if constexpr (sizeof(R) == sizeof(float)) {
-        constexpr int W = F32_WAVES;
-        if (o <= 16) {
-            if (energy_fits<R, 1, 2, W>(n, nspin, naux))
-                return launch_energy<R, 1, 2, W>(name, nw, n, o, nspin, naux, h0, greens, rotp, ldr, rot_h1, energy, st);
-            return launch_energy<R, 1, 1, W>(name, nw, n, o, nspin, naux, h0, greens, rotp, ldr, rot_h1, energy, st);
-        }
-        if (energy_fits<R, 2, 2, W>(n, nspin, naux))
-            return launch_energy<R, 2, 2, W>(name, nw, n, o, nspin, naux, h0, greens, rotp, ldr, rot_h1, energy, st);
-        return launch_energy<R, 2, 1, W>(name, nw, n, o, nspin, naux, h0, greens, rotp, ldr, rot_h1, energy, st);
-    } else {
-        if (o <= 16) {
-            if (energy_fits<R, 1, 1, 8>(n, nspin, naux))
-                return launch_energy<R, 1, 1, 8>(name, nw, n, o, nspin, naux, h0, greens, rotp, ldr, rot_h1, energy, st);
-            return launch_energy<R, 1, 1, 4>(name, nw, n, o, nspin, naux, h0, greens, rotp, ldr, rot_h1, energy, st);
-        }
-        if (energy_fits<R, 2, 1, 8>(n, nspin, naux))
-            return launch_energy<R, 2, 1, 8>(name, nw, n, o, nspin, naux, h0, greens, rotp, ldr, rot_h1, energy, st);
-        return launch_energy<R, 2, 1, 4>(name, nw, n, o, nspin, naux, h0, greens, rotp, ldr, rot_h1, energy, st);
-    }
+    // stacked chunks where they pack (fp32 always, fp64 with two column tiles:
+    // with one, its four-wave workgroups measure slower than the per-g grid)
+    // and their LDS tile fits; the per-g tile grid otherwise
+    constexpr bool f32 = sizeof(R) == sizeof(float);
+    bool stacked = stack_tiles(o) == 4 && (f32 || o > 16);
+    if (stacked)
+        stacked = o <= 16 ? energy_fits<R, 1, 1, f32 ? F32_WAVES : 4, 4>(n, nspin, naux)
+                          : energy_fits<R, 2, 1, f32 ? F32_WAVES : 4, 4>(n, nspin, naux);
+    if (o <= 16)
+        return stacked ? energy_dispatch<R, 1, 4>(name, nw, n, o, nspin, naux, h0, greens, rotp, ldr, rot_h1, energy, st)
+                       : energy_dispatch<R, 1, 0>(name, nw, n, o, nspin, naux, h0, greens, rotp, ldr, rot_h1, energy, st);
+    return stacked ? energy_dispatch<R, 2, 4>(name, nw, n, o, nspin, naux, h0, greens, rotp, ldr, rot_h1, energy, st)
+                   : energy_dispatch<R, 2, 0>(name, nw, n, o, nspin, naux, h0, greens, rotp, ldr, rot_h1, energy, st);
 }
 
 // ---------------------------------------------------------------------------
@@ -638,15 +962,15 @@ extern "C" int afq_energy_f32_grouped(int count, const afq_system_step* sys, int
     AFQ_REQUIRE(max_n > 0 && max_n <= 256, "afq_energy_f32_grouped: max_n=%d", max_n);
     hipStream_t st = (hipStream_t)stream;
     if (nt == 1) {
-        const size_t lds = energy_lds<float, 1, 1, F32_WAVES>(max_n, 1, 0);
-        AFQ_REQUIRE((energy_fits<float, 1, 1, F32_WAVES>(max_n, 1, 0)), "afq_energy_f32_grouped: max_n=%d too large",
+        const size_t lds = energy_lds<float, 1, 1, F32_WAVES, 4>(max_n, 1, 0);
+        AFQ_REQUIRE((energy_fits<float, 1, 1, F32_WAVES, 4>(max_n, 1, 0)), "afq_energy_f32_grouped: max_n=%d too large",
                     max_n);
         if (lds > 64 * 1024) afq::allow_full_lds((const void*)energy_f32_grouped_kernel<1>);
         hipLaunchKernelGGL(energy_f32_grouped_kernel<1>, dim3((unsigned)total_walkers), dim3(64 * F32_WAVES), lds, st,
                            count, sys);
     } else {
-        const size_t lds = energy_lds<float, 2, 1, F32_WAVES>(max_n, 1, 0);
-        AFQ_REQUIRE((energy_fits<float, 2, 1, F32_WAVES>(max_n, 1, 0)), "afq_energy_f32_grouped: max_n=%d too large",
+        const size_t lds = energy_lds<float, 2, 1, F32_WAVES, 4>(max_n, 1, 0);
+        AFQ_REQUIRE((energy_fits<float, 2, 1, F32_WAVES, 4>(max_n, 1, 0)), "afq_energy_f32_grouped: max_n=%d too large",
                     max_n);
         if (lds > 64 * 1024) afq::allow_full_lds((const void*)energy_f32_grouped_kernel<2>);
         hipLaunchKernelGGL(energy_f32_grouped_kernel<2>, dim3((unsigned)total_walkers), dim3(64 * F32_WAVES), lds, st,
