@@ -176,6 +176,19 @@ def test_energy(gpu_device, seed, n, o, naux, uhf, kind):
     assert_rel(e.cpu().numpy(), e_ref, TOL)
 
 
+@pytest.mark.parametrize("seed,n,o,naux,uhf,kind", [
+    # stacked-row energy (o packs poorly into 16-row tiles): q = 64 // o g's per
+    # chunk, naux not a multiple of q, so the last chunk is ragged
+    (3, 60, 20, 97, True, "unrestricted"),    # q = 3, last chunk 1 g
+    (4, 50, 17, 50, False, "restricted"),     # q = 3, last chunk 2 g's
+    (6, 64, 21, 40, False, "unrestricted"),   # q = 3, last chunk 1 g
+])
+def test_energy_stacked_rows(gpu_device, seed, n, o, naux, uhf, kind):
+    """The fp64 energy on the shapes that take stacked g chunks, against the
+    checker at 1e-10."""
+    test_energy(gpu_device, seed, n, o, naux, uhf, kind)
+
+
 @pytest.mark.parametrize("n,o", [(24, 6), (120, 20), (200, 32), (64, 64)])
 def test_orthonormalize(gpu_device, n, o):
     from ad_afqmc_prototype_amd import kernels

@@ -162,6 +162,11 @@ def test_mixed_block_energy_close_to_fp64(gpu_device, precision):
     dict(seed=2, n=200, o=32, g=1000, uhf=False, kind="restricted", nw=5),
     dict(seed=1, n=120, o=20, g=600, uhf=True, kind="unrestricted", nw=7),
     dict(seed=0, n=24, o=6, g=96, uhf=False, kind="restricted", nw=9),
+    # stacked g chunks (o = 7: 9 g's per chunk, o = 12: 5, o = 19: 3), the last
+    # chunk ragged
+    dict(seed=5, n=40, o=7, g=101, uhf=False, kind="restricted", nw=5),
+    dict(seed=6, n=48, o=12, g=61, uhf=True, kind="unrestricted", nw=6),
+    dict(seed=7, n=70, o=19, g=130, uhf=False, kind="restricted", nw=3),
 ])
 def test_energy_fp32_contractions(gpu_device, case):
     """fp32 f_g contractions with fp64 sums: per-walker energies within 2e-5 of
