@@ -646,15 +646,17 @@ int energy_entry(const char* name, int nw, int n, int o, int nspin, int naux, do
     // with one, its four-wave workgroups measure slower than the per-g grid)
     // and their LDS tile fits; the per-g tile grid otherwise
     constexpr bool f32 = sizeof(R) == sizeof(float);
-    bool stacked = stack_tiles(o) == 4 && (f32 || o > 16);
-    if (stacked)
-        stacked = o <= 16 ? energy_fits<R, 1, 1, f32 ? F32_WAVES : 4, 4>(n, nspin, naux)
-                          : energy_fits<R, 2, 1, f32 ? F32_WAVES : 4, 4>(n, nspin, naux);
-    if (o <= 16)
-        return stacked ? energy_dispatch<R, 1, 4>(name, nw, n, o, nspin, naux, h0, greens, rotp, ldr, rot_h1, energy, st)
-                       : energy_dispatch<R, 1, 0>(name, nw, n, o, nspin, naux, h0, greens, rotp, ldr, rot_h1, energy, st);
-    return stacked ? energy_dispatch<R, 2, 4>(name, nw, n, o, nspin, naux, h0, greens, rotp, ldr, rot_h1, energy, st)
-                   : energy_dispatch<R, 2, 0>(name, nw, n, o, nspin, naux, h0, greens, rotp, ldr, rot_h1, energy, st);
+    const bool stacked = stack_tiles(o) == 4;
+    if (o <= 16) {
+        if constexpr (f32) {
+            if (stacked && energy_fits<R, 1, 1, F32_WAVES, 4>(n, nspin, naux))
+                return energy_dispatch<R, 1, 4>(name, nw, n, o, nspin, naux, h0, greens, rotp, ldr, rot_h1, energy, st);
+        }
+        return energy_dispatch<R, 1, 0>(name, nw, n, o, nspin, naux, h0, greens, rotp, ldr, rot_h1, energy, st);
+    }
+    if (stacked && energy_fits<R, 2, 1, f32 ? F32_WAVES : 4, 4>(n, nspin, naux))
+        return energy_dispatch<R, 2, 4>(name, nw, n, o, nspin, naux, h0, greens, rotp, ldr, rot_h1, energy, st);
+    return energy_dispatch<R, 2, 0>(name, nw, n, o, nspin, naux, h0, greens, rotp, ldr, rot_h1, energy, st);
 }
 
 // ---------------------------------------------------------------------------
@@ -664,7 +666,7 @@ int energy_entry(const char* name, int nw, int n, int o, int nspin, int naux, do
 // (re hi/lo, im hi/lo) in its stored order (occupied index j, then orbital p),
 // which is the B-fragment order (8 consecutive p of one column j per lane).
 // R_g (hi/lo, rows padded to TC, pitch ldr a multiple of 32) is read from
-// L2 as A fragments, one k-block ahead.  Eight waves, each on its own g; the
+// L2 as A fragments, two k-blocks ahead.  Eight waves, each on its own g; the
 // exchange transpose goes through a per-wave LDS tile, ordered within the wave
 // (no workgroup barrier inside the g loop).
 // ---------------------------------------------------------------------------

@@ -7,7 +7,7 @@
 // 106 KB at n=200, o=32 in fp64); the running sum lives in MFMA accumulators.  Every
 // product is  P = A . T  with A either the real one-body propagator E or the
 // walker's complex VHS, streamed from global memory (L2/MALL) straight into
-// MFMA A-operand registers, one k-block ahead of use.  Wave wv owns the row
+// MFMA A-operand registers, one (k-block, row tile) step ahead of use.  Wave wv owns the row
 // tiles wv, wv+4, ...; each lane feeds four consecutive k of one row to four
 // v_mfma_{f64,f32}_16x16x4 (the B operand from LDS uses the same k order).
 //
