@@ -23,6 +23,12 @@ __device__ __forceinline__ double2 cmul(double2 a, double2 b) {
     return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
 }
 
+// y -= a b as four fused multiply-adds (the product is not rounded on its own)
+__device__ __forceinline__ void csub_mul(double2& y, double2 a, double2 b) {
+    y.x = fma(-a.x, b.x, fma(a.y, b.y, y.x));
+    y.y = fma(-a.x, b.y, fma(-a.y, b.x, y.y));
+}
+
 __device__ __forceinline__ double2 crecip(double2 a) {
     // 1/a by the scaled formula (Smith's method) to avoid overflow.
     if (fabs(a.x) >= fabs(a.y)) {
@@ -131,9 +137,9 @@ __device__ __forceinline__ void greens_body(int n, int o, const double2* __restr
         for (int e = tid; e < rem * rem; e += THREADS) {
             const int r = k + 1 + e / rem;
             const int c = k + 1 + e % rem;
-            const double2 t = cmul(m[r][k], m[k][c]);
-            m[r][c].x -= t.x;
-            m[r][c].y -= t.y;
+            double2 y = m[r][c];
+            csub_mul(y, m[r][k], m[k][c]);
+            m[r][c] = y;
         }
         __syncthreads();
     }
@@ -165,13 +171,8 @@ __device__ __forceinline__ void greens_body(int n, int o, const double2* __restr
             if (j < o) {
                 x[j] = cmul(x[j], dinv[j]);
 #pragma unroll
-                for (int c = j + 1; c < MAXO; ++c) {
-                    if (c < o) {
-                        const double2 t = cmul(x[j], m[j][c]);
-                        x[c].x -= t.x;
-                        x[c].y -= t.y;
-                    }
-                }
+                for (int c = j + 1; c < MAXO; ++c)
+                    if (c < o) csub_mul(x[c], x[j], m[j][c]);
             }
         }
         // v L = u
@@ -179,11 +180,7 @@ __device__ __forceinline__ void greens_body(int n, int o, const double2* __restr
         for (int j = MAXO - 1; j >= 0; --j) {
             if (j < o) {
 #pragma unroll
-                for (int c = 0; c < j; ++c) {
-                    const double2 t = cmul(x[j], m[j][c]);
-                    x[c].x -= t.x;
-                    x[c].y -= t.y;
-                }
+                for (int c = 0; c < j; ++c) csub_mul(x[c], x[j], m[j][c]);
             }
         }
 #pragma unroll
@@ -331,9 +328,9 @@ __device__ __forceinline__ void greens_wave_body(int n, int o, const double2* __
         for (int e = lane; e < rem * rem; e += 64) {
             const int r = k + 1 + e / rem;
             const int c = k + 1 + e % rem;
-            const double2 t = cmul(S.m[r][k], S.m[k][c]);
-            S.m[r][c].x -= t.x;
-            S.m[r][c].y -= t.y;
+            double2 y = S.m[r][c];
+            csub_mul(y, S.m[r][k], S.m[k][c]);
+            S.m[r][c] = y;
         }
         wave_lds_sync();
     }
@@ -366,24 +363,15 @@ __device__ __forceinline__ void greens_wave_body(int n, int o, const double2* __
             if (j < o) {
                 xr[j] = cmul(xr[j], S.dinv[j]);
 #pragma unroll
-                for (int c = j + 1; c < MAXO; ++c) {
-                    if (c < o) {
-                        const double2 t = cmul(xr[j], S.m[j][c]);
-                        xr[c].x -= t.x;
-                        xr[c].y -= t.y;
-                    }
-                }
+                for (int c = j + 1; c < MAXO; ++c)
+                    if (c < o) csub_mul(xr[c], xr[j], S.m[j][c]);
             }
         }
 #pragma unroll
         for (int j = MAXO - 1; j >= 0; --j) {
             if (j < o) {
 #pragma unroll
-                for (int c = 0; c < j; ++c) {
-                    const double2 t = cmul(xr[j], S.m[j][c]);
-                    xr[c].x -= t.x;
-                    xr[c].y -= t.y;
-                }
+                for (int c = 0; c < j; ++c) csub_mul(xr[c], xr[j], S.m[j][c]);
             }
         }
 #pragma unroll
@@ -468,6 +456,15 @@ __global__ __launch_bounds__(THREADS) void greens_grouped_kernel(int count, int 
                            gre + (long long)d.nw * o * n, (double2*)d.logdet + w, L[wave]);
 }
 
+// the wave kernel for capacity MAXO
+template <int MAXO, bool SPLIT>
+void launch_greens_wave(dim3 grid, hipStream_t st, int nw, int n, int o, int nspin, const double2* wu,
+                        const double2* wd, const double* cu, const double* cd, double* greens, double2* ld,
+                        uint16_t* hi, uint16_t* lo, long long ldk) {
+    hipLaunchKernelGGL((greens_wave_kernel<MAXO, SPLIT>), grid, dim3(THREADS), 0, st, nw, n, o, nspin, wu, wd, cu, cd,
+                       greens, ld, hi, lo, ldk);
+}
+
 }  // namespace
 
 extern "C" int afq_overlap_greens(int nw, int n, int o, int nspin, const double* walkers_up,
@@ -487,14 +484,11 @@ extern "C" int afq_overlap_greens(int nw, int n, int o, int nspin, const double*
     double2* ld = (double2*)logdet;
     const dim3 wgrid((unsigned)afq::ceil_div(nw * nspin, THREADS / 64));
     if (o <= 8)
-        hipLaunchKernelGGL((greens_wave_kernel<8, false>), wgrid, dim3(THREADS), 0, st, nw, n, o, nspin, wu, wd, trial_up, cd, greens, ld,
-                           nullptr, nullptr, 0LL);
+        launch_greens_wave<8, false>(wgrid, st, nw, n, o, nspin, wu, wd, trial_up, cd, greens, ld, nullptr, nullptr, 0LL);
     else if (o <= 16)
-        hipLaunchKernelGGL((greens_wave_kernel<16, false>), wgrid, dim3(THREADS), 0, st, nw, n, o, nspin, wu, wd, trial_up, cd, greens, ld,
-                           nullptr, nullptr, 0LL);
+        launch_greens_wave<16, false>(wgrid, st, nw, n, o, nspin, wu, wd, trial_up, cd, greens, ld, nullptr, nullptr, 0LL);
     else if (o <= 32)
-        hipLaunchKernelGGL((greens_wave_kernel<32, false>), wgrid, dim3(THREADS), 0, st, nw, n, o, nspin, wu, wd, trial_up, cd, greens, ld,
-                           nullptr, nullptr, 0LL);
+        launch_greens_wave<32, false>(wgrid, st, nw, n, o, nspin, wu, wd, trial_up, cd, greens, ld, nullptr, nullptr, 0LL);
     else
         hipLaunchKernelGGL(greens_kernel<64>, grid, dim3(THREADS), 0, st, nw, n, o, nspin, wu, wd, trial_up, cd, greens, ld);
     return afq::check_launch("afq_overlap_greens");
@@ -547,13 +541,10 @@ extern "C" int afq_overlap_greens_split(int nw, int n, int o, int nspin, const d
     uint16_t* lo = (uint16_t*)g_lo;
     const dim3 wgrid((unsigned)afq::ceil_div(nw * nspin, THREADS / 64));
     if (o <= 8)
-        hipLaunchKernelGGL((greens_wave_kernel<8, true>), wgrid, dim3(THREADS), 0, st, nw, n, o, nspin, wu, wd, trial_up,
-                           cd, greens, ld, hi, lo, ldk);
+        launch_greens_wave<8, true>(wgrid, st, nw, n, o, nspin, wu, wd, trial_up, cd, greens, ld, hi, lo, ldk);
     else if (o <= 16)
-        hipLaunchKernelGGL((greens_wave_kernel<16, true>), wgrid, dim3(THREADS), 0, st, nw, n, o, nspin, wu, wd,
-                           trial_up, cd, greens, ld, hi, lo, ldk);
+        launch_greens_wave<16, true>(wgrid, st, nw, n, o, nspin, wu, wd, trial_up, cd, greens, ld, hi, lo, ldk);
     else
-        hipLaunchKernelGGL((greens_wave_kernel<32, true>), wgrid, dim3(THREADS), 0, st, nw, n, o, nspin, wu, wd,
-                           trial_up, cd, greens, ld, hi, lo, ldk);
+        launch_greens_wave<32, true>(wgrid, st, nw, n, o, nspin, wu, wd, trial_up, cd, greens, ld, hi, lo, ldk);
     return afq::check_launch("afq_overlap_greens_split");
 }
