@@ -197,9 +197,11 @@ __device__ __forceinline__ void cholqr_pass(int n, int o, const double2* W, doub
             const int i = k + 1 + e / rem;
             const int j = k + 1 + e % rem;
             if (j >= i) {
-                const double2 a = S.g[k][i], b = S.g[k][j];  // conj(a) b
-                S.g[i][j].x -= a.x * b.x + a.y * b.y;
-                S.g[i][j].y -= a.x * b.y - a.y * b.x;
+                const double2 a = S.g[k][i], b = S.g[k][j];  // g_ij -= conj(a) b, as four FMAs
+                double2 g = S.g[i][j];
+                g.x = fma(-a.x, b.x, fma(-a.y, b.y, g.x));
+                g.y = fma(-a.x, b.y, fma(a.y, b.x, g.y));
+                S.g[i][j] = g;
             }
         }
         wave_lds_sync();
@@ -219,9 +221,10 @@ __device__ __forceinline__ void cholqr_pass(int n, int o, const double2* W, doub
 #pragma unroll
                 for (int j = i + 1; j < MAXO; ++j) {
                     if (j < o) {
+                        // x_j -= x_i r_ij, as four FMAs (no separately rounded product)
                         const double2 r = S.g[i][j];
-                        xr[j].x -= xr[i].x * r.x - xr[i].y * r.y;
-                        xr[j].y -= xr[i].x * r.y + xr[i].y * r.x;
+                        xr[j].x = fma(-xr[i].x, r.x, fma(xr[i].y, r.y, xr[j].x));
+                        xr[j].y = fma(-xr[i].x, r.y, fma(-xr[i].y, r.x, xr[j].y));
                     }
                 }
             }
