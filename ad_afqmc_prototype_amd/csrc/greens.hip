@@ -324,13 +324,18 @@ __device__ __forceinline__ void greens_wave_body(int n, int o, const double2* __
         const double2 inv = crecip(S.m[k][k]);
         for (int r = k + 1 + lane; r < o; r += 64) S.m[r][k] = cmul(S.m[r][k], inv);
         wave_lds_sync();
-        const int rem = o - k - 1;
-        for (int e = lane; e < rem * rem; e += 64) {
-            const int r = k + 1 + e / rem;
-            const int c = k + 1 + e % rem;
-            double2 y = S.m[r][c];
-            csub_mul(y, S.m[r][k], S.m[k][c]);
-            S.m[r][c] = y;
+        // trailing update: lane & 31 is the column, lane >> 5 the row parity
+        // (o <= 32: one row of columns per half-wave, no index division)
+        {
+            const int c = k + 1 + (lane & 31);
+            if (c < o) {
+                const double2 mkc = S.m[k][c];
+                for (int r = k + 1 + (lane >> 5); r < o; r += 2) {
+                    double2 y = S.m[r][c];
+                    csub_mul(y, S.m[r][k], mkc);
+                    S.m[r][c] = y;
+                }
+            }
         }
         wave_lds_sync();
     }

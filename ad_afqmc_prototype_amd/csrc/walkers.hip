@@ -192,16 +192,19 @@ __device__ __forceinline__ void cholqr_pass(int n, int o, const double2* W, doub
         }
         if (lane == 0) S.rinv[k] = dinv;
         wave_lds_sync();
-        const int rem = o - k - 1;
-        for (int e = lane; e < rem * rem; e += 64) {
-            const int i = k + 1 + e / rem;
-            const int j = k + 1 + e % rem;
-            if (j >= i) {
-                const double2 a = S.g[k][i], b = S.g[k][j];  // g_ij -= conj(a) b, as four FMAs
-                double2 g = S.g[i][j];
-                g.x = fma(-a.x, b.x, fma(-a.y, b.y, g.x));
-                g.y = fma(-a.x, b.y, fma(a.y, b.x, g.y));
-                S.g[i][j] = g;
+        // trailing update of the upper triangle: lane & 31 is the column j,
+        // lane >> 5 the row parity (o <= 32: no index division)
+        {
+            const int j = k + 1 + (lane & 31);
+            if (j < o) {
+                const double2 b = S.g[k][j];
+                for (int i = k + 1 + (lane >> 5); i <= j; i += 2) {
+                    const double2 a = S.g[k][i];  // g_ij -= conj(a) b, as four FMAs
+                    double2 g = S.g[i][j];
+                    g.x = fma(-a.x, b.x, fma(-a.y, b.y, g.x));
+                    g.y = fma(-a.x, b.y, fma(a.y, b.x, g.y));
+                    S.g[i][j] = g;
+                }
             }
         }
         wave_lds_sync();
