@@ -101,13 +101,9 @@ __device__ __forceinline__ double wave_sum(double v) {
     return v;
 }
 
-// bf16 bits of x, round to nearest even (NaN -> quiet NaN)
-__device__ __forceinline__ uint16_t f32_to_bf16_rne(float f) {
-    uint32_t u = __float_as_uint(f);
-    if ((u & 0x7fffffffu) > 0x7f800000u) return 0x7fc0;
-    u += 0x7fffu + ((u >> 16) & 1u);
-    return (uint16_t)(u >> 16);
-}
+// bf16 bits of x, round to nearest even (NaN -> quiet NaN): one hardware
+// conversion (v_cvt_pk_bf16_f32 on gfx950)
+__device__ __forceinline__ uint16_t f32_to_bf16_rne(float f) { return __builtin_bit_cast(uint16_t, (__bf16)f); }
 
 __device__ __forceinline__ double bf16_to_f64(uint16_t h) { return (double)__uint_as_float(((uint32_t)h) << 16); }
 
