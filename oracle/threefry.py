@@ -29,12 +29,16 @@ algorithm, restated:
   max(lo, f * (hi - lo) + lo).
 * normal: sqrt(2) * erfinv(uniform(lo = nextafter(-1, 0), hi = 1)).
 
-Pinning: the Threefry-2x32-20 block function against the published
-known-answer vectors (tests/test_rng.py).  The layers above it (split, bit
-assembly, uniform, normal) restate jax's published code paths and are
-**parity unpinned**: checking them needs jax itself.  erfinv here is
-scipy.special.erfinv (not XLA's polynomial), so normals agree with jax's to a
-few ulp at best even where the uniforms agree bit for bit.
+Pinning (tests/test_rng.py): the Threefry-2x32-20 block function against the
+published known-answer vectors; split against the keys jax's documentation
+prints for split(PRNGKey(0)) (bit exact); the count layout, odd-count padding,
+mantissa recipe, [nextafter(-1, 0), 1) interval and sqrt(2) erfinv transform
+through the float32 path (x64 off) against the normals jax's documentation
+prints for normal(PRNGKey(0), (10,)) and normal(PRNGKey(42)).  What those
+leave **parity unpinned** is only the x64 specialisation: the two 32-bit words
+per value (value i from outputs i and n + i) and the 52-bit mantissa shift;
+checking them needs jax itself.  erfinv here is scipy.special.erfinv (not XLA's
+polynomial), so normals agree with jax's to a few ulp, not bit for bit.
 """
 from __future__ import annotations
 
@@ -95,16 +99,30 @@ def random_bits64(key, n: int) -> np.ndarray:
     return (hi << np.uint64(32)) | lo
 
 
-def uniform(key, shape=(), lo=0.0, hi=1.0) -> np.ndarray:
+def random_bits32(key, n: int) -> np.ndarray:
+    """32-bit random bits (x64 off): threefry_2x32(key, iota(n)), value i = output i."""
+    return threefry_counts(key, np.arange(n, dtype=np.uint32))
+
+
+def uniform(key, shape=(), lo=0.0, hi=1.0, dtype=np.float64) -> np.ndarray:
+    """jax.random.uniform; float64 (the reference's x64 mode) or float32."""
     n = int(np.prod(shape)) if shape else 1
-    bits = random_bits64(key, n)
-    f = ((bits >> np.uint64(12)) | np.uint64(0x3FF0000000000000)).view(np.float64) - 1.0
-    lo = np.float64(lo)
-    hi = np.float64(hi)
+    if np.dtype(dtype) == np.float32:
+        bits = random_bits32(key, n)
+        f = ((bits >> np.uint32(9)) | np.uint32(0x3F800000)).view(np.float32) - np.float32(1.0)
+        lo = np.float32(lo)
+        hi = np.float32(hi)
+    else:
+        bits = random_bits64(key, n)
+        f = ((bits >> np.uint64(12)) | np.uint64(0x3FF0000000000000)).view(np.float64) - 1.0
+        lo = np.float64(lo)
+        hi = np.float64(hi)
     u = np.maximum(lo, f * (hi - lo) + lo)
     return u.reshape(shape) if shape else u[0]
 
 
-def normal(key, shape) -> np.ndarray:
-    u = uniform(key, shape, np.nextafter(-1.0, 0.0), 1.0)
-    return np.sqrt(2.0) * erfinv(u)
+def normal(key, shape, dtype=np.float64) -> np.ndarray:
+    """jax.random.normal; float64 or float32 (erfinv evaluated in float64)."""
+    dt = np.dtype(dtype)
+    u = uniform(key, shape, np.nextafter(dt.type(-1.0), dt.type(0.0)), 1.0, dt)
+    return (np.sqrt(2.0) * erfinv(np.asarray(u, np.float64))).astype(dt)

@@ -34,6 +34,38 @@ def test_key_split_and_bit_layout():
     assert bits.tolist() == [(int(x) << 32) | int(y) for x, y in zip(a, b)]
 
 
+# Outputs printed in jax's own documentation (original Threefry scheme, x64 off):
+# the "Pseudo random numbers" tutorial splits PRNGKey(0) and draws
+# normal(PRNGKey(42)); the quickstart draws normal(PRNGKey(0), (10,)).
+JAX_SPLIT_KEY0 = ([4146024105, 967050713], [2718843009, 1272950319])
+JAX_NORMAL_KEY0_10 = [-0.3721109, 0.26423115, -0.18252768, -0.7368197, -0.44030377,
+                      -0.1521442, -0.67135346, -0.5908641, 0.73168886, 0.5673026]
+JAX_NORMAL_KEY42 = -0.18471177
+
+
+def test_split_matches_published_jax_keys():
+    nk, sk = tf.split(tf.prng_key(0))
+    assert (nk.tolist(), sk.tolist()) == JAX_SPLIT_KEY0
+
+
+def test_float32_normals_match_published_jax_values():
+    # the values are printed to float32 precision and erfinv here is not XLA's
+    # polynomial: agreement to a few float32 ulp (2^-24 relative each)
+    x = tf.normal(tf.prng_key(0), (10,), np.float32)
+    assert x.dtype == np.float32
+    np.testing.assert_allclose(x, JAX_NORMAL_KEY0_10, rtol=5e-7, atol=0)
+    y = tf.normal(tf.prng_key(42), (), np.float32)
+    np.testing.assert_allclose(y, JAX_NORMAL_KEY42, rtol=5e-7, atol=0)
+    # a planted layout mistake (pairs (2i, 2i + 1) instead of (i, n/2 + i))
+    # moves every value off the published ones
+    c = np.arange(10, dtype=np.uint32)
+    a, b = tf.threefry2x32(0, 0, c[0::2], c[1::2])
+    wrong = np.stack([a, b], 1).ravel()
+    f = ((wrong >> np.uint32(9)) | np.uint32(0x3F800000)).view(np.float32) - np.float32(1.0)
+    assert not np.allclose(f * 2 - 1, tf.uniform(tf.prng_key(0), (10,), -1.0, 1.0, np.float32),
+                           rtol=1e-3)
+
+
 def test_uniform_and_normal_ranges():
     key = tf.split(tf.prng_key(11))[1]
     u = tf.uniform(key, (200000,))
