@@ -66,6 +66,8 @@ _SIGNATURES = {
     # grouped launches over many systems (descriptor tables in device memory)
     "afq_gemm_tile_shape": [c_int, ctypes.POINTER(c_int), ctypes.POINTER(c_int)],
     "afq_sgemm_nt_grouped": [c_int, c_vp, c_int, c_int, c_vp],
+    "afq_gemm_bf16x3_nt_grouped": [c_int, c_vp, c_int, c_vp],
+    "afq_split_bf16_grouped": [c_int, c_vp, c_ll, c_vp],
     "afq_rng_fields_grouped": [c_int, c_vp, c_int, c_vp],
     "afq_field_shifts_grouped": [c_int, c_vp, c_int, c_vp],
     "afq_trotter_f32_grouped": [c_int, c_vp, c_int, c_int, c_int, c_vp],
@@ -80,7 +82,7 @@ _SIGNATURES = {
     "afq_unpack_pairs": [c_int, c_int, c_vp, c_ll, c_vp, c_vp],
 }
 
-ABI_VERSION = 6
+ABI_VERSION = 7
 
 
 class HipLibraryError(RuntimeError):

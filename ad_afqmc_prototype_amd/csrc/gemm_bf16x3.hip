@@ -32,33 +32,14 @@ constexpr int GROUP_M = 8;
 // expanded (afq::sym_store); C is then float, ldc unused.
 // Split K: blockIdx.y = kz takes k in [kz kc, (kz + 1) kc) (kc a multiple of
 // 32) and writes its partial product to C + kz c_stride.
-template <typename TOut, bool SYM = false>
-__global__ __launch_bounds__(THREADS, 2) void gemm_bf16x3_nt_kernel(
-    int M, int N, int K, float alpha, const uint16_t* __restrict__ Ahi,
-    const uint16_t* __restrict__ Alo, long long lda, const uint16_t* __restrict__ Bhi,
-    const uint16_t* __restrict__ Blo, long long ldb, TOut* __restrict__ C, long long ldc,
-    int tiles_m, int tiles_n, afq::sym_dst sd = afq::sym_dst{0, 0, 0}, int kc = 0, long long c_stride = 0) {
-    if (gridDim.y > 1) {
-        const int k_begin = blockIdx.y * kc;
-        K = min(K - k_begin, kc);
-        Ahi += k_begin;
-        Alo += k_begin;
-        Bhi += k_begin;
-        Blo += k_begin;
-        C += blockIdx.y * c_stride;
-    }
-    __shared__ __attribute__((aligned(16))) uint16_t S[2][4][BM * PITCH];
-
-    const int total = tiles_m * tiles_n;
-    const int t = afq::xcd_contiguous(blockIdx.x, total);
-    const int per_group = GROUP_M * tiles_n;
-    const int group = t / per_group;
-    const int first_m = group * GROUP_M;
-    const int gsize = min(tiles_m - first_m, GROUP_M);
-    const int in_group = t - group * per_group;
-    const int m0 = (first_m + in_group % gsize) * BM;
-    const int n0 = (in_group / gsize) * BN;
-
+// One 128 x 128 tile of C (rows m0.., columns n0..) over the whole K, staged
+// through the workgroup's double-buffered LDS planes S.
+template <typename TOut, bool SYM>
+__device__ __forceinline__ void bf16x3_tile(int M, int N, int K, float alpha, const uint16_t* __restrict__ Ahi,
+                                            const uint16_t* __restrict__ Alo, long long lda,
+                                            const uint16_t* __restrict__ Bhi, const uint16_t* __restrict__ Blo,
+                                            long long ldb, TOut* __restrict__ C, long long ldc, int m0, int n0,
+                                            afq::sym_dst sd, uint16_t (&S)[2][4][BM * PITCH]) {
     const int tid = threadIdx.x;
     const int lane = tid & 63;
     const int wave = tid >> 6;
@@ -171,6 +152,61 @@ __global__ __launch_bounds__(THREADS, 2) void gemm_bf16x3_nt_kernel(
                 if (row < M && col < N) C[(long long)row * ldc + col] = (TOut)(alpha * acc[i][j][r]);
             }
         }
+}
+
+template <typename TOut, bool SYM = false>
+__global__ __launch_bounds__(THREADS, 2) void gemm_bf16x3_nt_kernel(
+    int M, int N, int K, float alpha, const uint16_t* __restrict__ Ahi,
+    const uint16_t* __restrict__ Alo, long long lda, const uint16_t* __restrict__ Bhi,
+    const uint16_t* __restrict__ Blo, long long ldb, TOut* __restrict__ C, long long ldc,
+    int tiles_m, int tiles_n, afq::sym_dst sd = afq::sym_dst{0, 0, 0}, int kc = 0, long long c_stride = 0) {
+    if (gridDim.y > 1) {
+        const int k_begin = blockIdx.y * kc;
+        K = min(K - k_begin, kc);
+        Ahi += k_begin;
+        Alo += k_begin;
+        Bhi += k_begin;
+        Blo += k_begin;
+        C += blockIdx.y * c_stride;
+    }
+    __shared__ __attribute__((aligned(16))) uint16_t S[2][4][BM * PITCH];
+
+    const int total = tiles_m * tiles_n;
+    const int t = afq::xcd_contiguous(blockIdx.x, total);
+    const int per_group = GROUP_M * tiles_n;
+    const int group = t / per_group;
+    const int first_m = group * GROUP_M;
+    const int gsize = min(tiles_m - first_m, GROUP_M);
+    const int in_group = t - group * per_group;
+    const int m0 = (first_m + in_group % gsize) * BM;
+    const int n0 = (in_group / gsize) * BN;
+
+    bf16x3_tile<TOut, SYM>(M, N, K, alpha, Ahi, Alo, lda, Bhi, Blo, ldb, C, ldc, m0, n0, sd, S);
+}
+
+// Grouped bf16x3 GEMM: workgroup -> (problem, 128 x 128 tile) by the
+// problems' tile_start prefix sums, the linear workgroup id first made
+// XCD-contiguous (each XCD runs a contiguous range of tiles and shares their
+// panels in its L2).
+__global__ __launch_bounds__(THREADS, 2) void gemm_bf16x3_grouped_kernel(int count, int total,
+                                                                         const afq_gemm_problem_bf16x3* __restrict__ P) {
+    __shared__ __attribute__((aligned(16))) uint16_t S[2][4][BM * PITCH];
+    const int t = afq::xcd_contiguous(blockIdx.x, total);
+    const int pi = afq::find_group(count, t, [&](int i) { return P[i].tile_start; });
+    const afq_gemm_problem_bf16x3& p = P[pi];
+    const int tiles_m = afq::ceil_div(p.M, BM);
+    const int local = t - p.tile_start;
+    const int m0 = (local % tiles_m) * BM;
+    const int n0 = (local / tiles_m) * BN;
+    if (p.sym_n > 0)
+        bf16x3_tile<float, true>(p.M, p.N, p.K, p.alpha, p.A_hi, p.A_lo, p.lda, p.B_hi, p.B_lo, p.ldb, (float*)p.C,
+                                 0LL, m0, n0, afq::sym_dst{p.sym_n, p.sym_ldv, afq::ceil_div(p.sym_n, 16)}, S);
+    else if (p.c_fp64)
+        bf16x3_tile<double, false>(p.M, p.N, p.K, p.alpha, p.A_hi, p.A_lo, p.lda, p.B_hi, p.B_lo, p.ldb,
+                                   (double*)p.C, p.ldc, m0, n0, afq::sym_dst{0, 0, 0}, S);
+    else
+        bf16x3_tile<float, false>(p.M, p.N, p.K, p.alpha, p.A_hi, p.A_lo, p.lda, p.B_hi, p.B_lo, p.ldb,
+                                  (float*)p.C, p.ldc, m0, n0, afq::sym_dst{0, 0, 0}, S);
 }
 
 // Large-problem variant: 256 x 256 x 32 per 512-thread workgroup (4 x 2
@@ -384,6 +420,20 @@ __global__ void split_bf16_kernel(int rows, int cols, const double* __restrict__
     }
 }
 
+// Per problem (blockIdx.y): rows x cols of src (leading dimension ld_src) ->
+// hi / lo planes of rows x ld_dst, columns cols..ld_dst-1 zero.
+__global__ void split_bf16_grouped_kernel(const afq_split_problem* __restrict__ P) {
+    const afq_split_problem& p = P[blockIdx.y];
+    const long long total = (long long)p.rows * p.ld_dst;
+    for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < total;
+         e += (long long)gridDim.x * blockDim.x) {
+        const long long r = e / p.ld_dst;
+        const long long c = e - r * p.ld_dst;
+        const double x = c < p.cols ? p.src[r * p.ld_src + c] : 0.0;
+        afq::split_bf16(x, p.hi[e], p.lo[e]);
+    }
+}
+
 __global__ void convert_f32_kernel(int rows, int cols, const double* __restrict__ src,
                                    long long ld_src, float* __restrict__ dst, long long ld_dst) {
     const long long total = (long long)rows * ld_dst;
@@ -531,4 +581,26 @@ extern "C" int afq_convert_f32(int rows, int cols, const double* src, long long 
     hipLaunchKernelGGL(convert_f32_kernel, dim3(grid_for((long long)rows * ld_dst)), dim3(256), 0,
                        (hipStream_t)stream, rows, cols, src, ld_src, dst, ld_dst);
     return afq::check_launch("afq_convert_f32");
+}
+
+extern "C" int afq_gemm_bf16x3_nt_grouped(int count, const afq_gemm_problem_bf16x3* problems, int total_tiles,
+                                          void* stream) {
+    AFQ_REQUIRE(count > 0 && total_tiles > 0, "afq_gemm_bf16x3_nt_grouped: count=%d total_tiles=%d", count,
+                total_tiles);
+    AFQ_REQUIRE(problems, "afq_gemm_bf16x3_nt_grouped: null pointer");
+    hipLaunchKernelGGL(gemm_bf16x3_grouped_kernel, dim3((unsigned)total_tiles), dim3(THREADS), 0, (hipStream_t)stream,
+                       count, total_tiles, problems);
+    return afq::check_launch("afq_gemm_bf16x3_nt_grouped");
+}
+
+extern "C" int afq_split_bf16_grouped(int count, const afq_split_problem* problems, long long max_elems,
+                                      void* stream) {
+    AFQ_REQUIRE(count > 0 && count <= 65535, "afq_split_bf16_grouped: count=%d not in 1..65535", count);
+    AFQ_REQUIRE(max_elems > 0, "afq_split_bf16_grouped: max_elems=%lld", max_elems);
+    AFQ_REQUIRE(problems, "afq_split_bf16_grouped: null pointer");
+    long long b = (max_elems + 255) / 256;
+    if (b > 2048) b = 2048;
+    hipLaunchKernelGGL(split_bf16_grouped_kernel, dim3((unsigned)b, (unsigned)count), dim3(256), 0,
+                       (hipStream_t)stream, problems);
+    return afq::check_launch("afq_split_bf16_grouped");
 }

@@ -74,8 +74,28 @@ class _gemm_problem(ctypes.Structure):
                 ("sym_n", ctypes.c_int), ("sym_ldv", ctypes.c_int)]
 
 
+class _gemm_problem_bf16x3(ctypes.Structure):
+    """Host mirror of afq_gemm_problem_bf16x3 (include/afqmc_hip.h)."""
+
+    _fields_ = [("M", ctypes.c_int), ("N", ctypes.c_int), ("K", ctypes.c_int), ("tile_start", ctypes.c_int),
+                ("lda", ctypes.c_longlong), ("ldb", ctypes.c_longlong), ("ldc", ctypes.c_longlong),
+                ("A_hi", _vp), ("A_lo", _vp), ("B_hi", _vp), ("B_lo", _vp), ("C", _vp),
+                ("alpha", ctypes.c_float), ("c_fp64", ctypes.c_int), ("sym_n", ctypes.c_int),
+                ("sym_ldv", ctypes.c_int)]
+
+
+class _split_problem(ctypes.Structure):
+    """Host mirror of afq_split_problem (include/afqmc_hip.h)."""
+
+    _fields_ = [("src", _vp), ("hi", _vp), ("lo", _vp), ("rows", ctypes.c_int), ("cols", ctypes.c_int),
+                ("ld_src", ctypes.c_int), ("ld_dst", ctypes.c_int)]
+
+
 assert ctypes.sizeof(_system_step) == 8 * 4 + 8 * 8 + 24 * 8
 assert ctypes.sizeof(_gemm_problem) == 80
+assert ctypes.sizeof(_gemm_problem_bf16x3) == 96
+assert ctypes.sizeof(_split_problem) == 40
+BF16X3_TILE = 128  # rows and columns of C per workgroup of the grouped bf16x3 GEMM
 
 GEMM_SHAPES = (0, 1, 2, 3)
 
@@ -131,14 +151,22 @@ class system_batch:
     buffer sets (A, B); steps alternate A -> B and B -> A, so after an even
     number of steps the state is in A again.  QR (every ``qr_every`` steps),
     measurement and comb (every ``sr_every`` steps, prop/blocks.py:53-92) run
-    on the A set; both cadences must be even."""
+    on the A set; both cadences must be even.
+
+    ``gemm`` is the arithmetic of the force-bias and VHS contractions:
+    "bf16x3" (split-bf16 operands on the bf16 matrix cores, fp32 sums; one
+    128 x 128 tile shape) or "fp32" (fp32 matrix cores, tile shape picked by
+    tune()).  Trotter, Green's functions and energies are fp32 / fp64 in both."""
 
     def __init__(self, systems: Sequence[batch_system], *, dt: float, qr_every: int = 10,
                  sr_every: int = 20, n_exp_terms: int = 6, weight_floor: float = 1.0e-3,
                  weight_cap: float = 100.0, damping: float = 0.1, shift_ema: float = 0.1,
-                 gemm_shapes: tuple[int, int] | None = None):
+                 gemm_shapes: tuple[int, int] | None = None, gemm: str = "bf16x3"):
         if not systems:
             raise ValueError("empty batch")
+        if gemm not in ("bf16x3", "fp32"):
+            raise ValueError(f"gemm must be 'bf16x3' or 'fp32', not {gemm!r}")
+        self.gemm = gemm
         if qr_every % 2 or sr_every % 2 or qr_every <= 0 or sr_every <= 0:
             raise ValueError("qr_every and sr_every must be positive and even")
         dev = systems[0].ham.h1.device
@@ -181,9 +209,13 @@ class system_batch:
         st = init_prop_state(sys=sys_, n_walkers=s.n_walkers, seed=s.seed, ham_data=s.ham,
                              trial_ops_=t_ops, trial_data=s.trial, meas_ops=m_ops)
         rot_pad, rot_h1 = m_ctx.energy_operands(1)
+        if self.gemm == "bf16x3":
+            ops = dict(chol_hl=kernels.split_bf16(chol_sym_rows(s.ham.chol)),
+                       rot_hl=kernels.split_bf16(m_ctx.rot_chol_flat))
+        else:
+            ops = dict(chol_s32=kernels.convert_f32(chol_sym_rows(s.ham.chol)), rot32=m_ctx.fb_operand(1))
         return dict(n=n, o=o, g=g, nw=s.n_walkers, ldv=kernels.ldv_for(n), h0=float(s.ham.h0),
-                    trial=s.trial.mo_up.contiguous(), chol_s32=kernels.convert_f32(chol_sym_rows(s.ham.chol)),
-                    rot32=m_ctx.fb_operand(1), rot_pad=rot_pad, rot_h1=rot_h1,
+                    trial=s.trial.mo_up.contiguous(), **ops, rot_pad=rot_pad, rot_h1=rot_h1,
                     e_half=p_ctx.exp_h1_half_pad32.contiguous(), mf=p_ctx.mf_shifts.contiguous(),
                     h0_prop=float(complex(p_ctx.h0_prop).real), e_est=float(st.e_estimate), state0=st)
 
@@ -192,12 +224,19 @@ class system_batch:
         P = lambda sizes, dt: _packer(sizes, dt, dev)  # noqa: E731
         self.walkers = [P([m["nw"] * m["n"] * m["o"] for m in ms], c128) for _ in range(2)]
         self.greens = P([2 * m["nw"] * m["o"] * m["n"] for m in ms], f64)
-        self.greens32 = P(self.greens.sizes, f32)
+        pad8 = kernels.pad8
+        if self.gemm == "bf16x3":
+            i16 = torch.int16
+            self.greens_hl = [P([2 * m["nw"] * pad8(m["o"] * m["n"]) for m in ms], i16) for _ in range(2)]
+            self.xbar_hl = [P([2 * m["nw"] * pad8(m["g"]) for m in ms], i16) for _ in range(2)]
+        else:
+            self.greens32 = P(self.greens.sizes, f32)
         self.logdet = P([m["nw"] for m in ms], c128)
         self.fields = P([m["nw"] * m["g"] for m in ms], f64)
         self.fb = P([2 * m["nw"] * m["g"] for m in ms], f64)
         self.xbar = P([2 * m["nw"] * m["g"] for m in ms], f64)
-        self.xbar32 = P(self.xbar.sizes, f32)
+        if self.gemm == "fp32":
+            self.xbar32 = P(self.xbar.sizes, f32)
         self.shift = P([m["nw"] for m in ms], c128)
         self.fbt = P([m["nw"] for m in ms], c128)
         self.vhs = P([2 * m["nw"] * m["n"] * m["ldv"] for m in ms], f32)
@@ -275,6 +314,44 @@ class system_batch:
             start += -(-p.M // bm) * -(-p.N // bn)
         return arr, start
 
+    def _problems_bf16x3(self, which: str):
+        pad8 = kernels.pad8
+        arr = (_gemm_problem_bf16x3 * self.count)()
+        start = 0
+        for i, m in enumerate(self.members):
+            p = arr[i]
+            if which == "fb":
+                k = pad8(m["o"] * m["n"])
+                (bh, bl), (ah, al) = m["rot_hl"], self.greens_hl
+                p.M, p.N, p.K = 2 * m["nw"], m["g"], k
+                p.ldc, p.C, p.alpha, p.c_fp64 = m["g"], self.fb.ptr(i), 2.0, 1
+            else:
+                k = pad8(m["g"])
+                (bh, bl), (ah, al) = m["chol_hl"], self.xbar_hl
+                p.M, p.N, p.K = 2 * m["nw"], bh.shape[0], k
+                p.ldc, p.C, p.alpha, p.c_fp64 = 0, self.vhs.ptr(i), 1.0, 0
+                p.sym_n, p.sym_ldv = m["n"], m["ldv"]
+            assert bh.shape[1] == k and bl.shape[1] == k
+            p.lda = p.ldb = k
+            p.A_hi, p.A_lo, p.B_hi, p.B_lo = ah.ptr(i), al.ptr(i), bh.data_ptr(), bl.data_ptr()
+            p.tile_start = start
+            start += -(-p.M // BF16X3_TILE) * -(-p.N // BF16X3_TILE)
+        return arr, start
+
+    def _split_table(self, which: str):
+        """afq_split_bf16_grouped problems: the Green's-function rows (force
+        bias) or the shifted-field rows (VHS) of every system, to padded planes."""
+        arr = (_split_problem * self.count)()
+        biggest = 0
+        for i, m in enumerate(self.members):
+            q = arr[i]
+            src, (hi, lo), cols = ((self.greens, self.greens_hl, m["o"] * m["n"]) if which == "fb"
+                                   else (self.xbar, self.xbar_hl, m["g"]))
+            q.src, q.hi, q.lo = src.ptr(i), hi.ptr(i), lo.ptr(i)
+            q.rows, q.cols, q.ld_src, q.ld_dst = 2 * m["nw"], cols, cols, kernels.pad8(cols)
+            biggest = max(biggest, q.rows * q.ld_dst)
+        return _upload(arr).to(self.device), biggest
+
     def _tables(self):
         full = [self._desc(0), self._desc(1)]
         self.desc_host = [f[0] for f in full]
@@ -308,7 +385,13 @@ class system_batch:
                                             walkers=nw, max_n=max(self.members[i]["n"] for i in idx),
                                             nt=(self.members[idx[0]]["o"] + 15) // 16))
         self.gemm_tables = {}
+        self.split_tables = {}
         for which in ("fb", "vhs"):
+            if self.gemm == "bf16x3":
+                arr, tiles = self._problems_bf16x3(which)
+                self.gemm_tables[which] = (_upload(arr).to(self.device), tiles)
+                self.split_tables[which] = self._split_table(which)
+                continue
             for shape in GEMM_SHAPES:
                 arr, tiles = self._problems(which, shape)
                 self.gemm_tables[(which, shape)] = (_upload(arr).to(self.device), tiles)
@@ -317,6 +400,15 @@ class system_batch:
 
     # ------------------------------------------------------------------ stages
     def _grouped_gemm(self, which: str):
+        if self.gemm == "bf16x3":
+            # the A operand's bf16 planes, then the product
+            stab, biggest = self.split_tables[which]
+            kernels._call("afq_split_bf16_grouped", stab, self.count, stab.data_ptr(), biggest,
+                          kernels._stream(stab), tag=which)
+            tab, tiles = self.gemm_tables[which]
+            kernels._call("afq_gemm_bf16x3_nt_grouped", tab, self.count, tab.data_ptr(), tiles,
+                          kernels._stream(tab), tag=which)
+            return
         shape = self.gemm_shapes[0 if which == "fb" else 1]
         tab, tiles = self.gemm_tables[(which, shape)]
         kernels._call("afq_sgemm_nt_grouped", tab, self.count, tab.data_ptr(), tiles, shape,
@@ -340,10 +432,12 @@ class system_batch:
         d = self.desc[p]
         st = kernels._stream(d)
         kernels._call("afq_rng_fields_grouped", d, self.count, d.data_ptr(), self.total_walkers, st)
-        self._convert(self.greens, self.greens32)
+        if self.gemm == "fp32":
+            self._convert(self.greens, self.greens32)
         self._grouped_gemm("fb")
         kernels._call("afq_field_shifts_grouped", d, self.count, d.data_ptr(), self.total_walkers, st)
-        self._convert(self.xbar, self.xbar32)
+        if self.gemm == "fp32":
+            self._convert(self.xbar, self.xbar32)
         self._grouped_gemm("vhs")
         for bk in self.trotter_buckets:
             t = bk["tables"][p]
@@ -437,6 +531,9 @@ class system_batch:
         contraction by timing each shape on this batch (HIP events, ``reps``
         launches after one warm-up).  With ``cache_path`` the choice is read
         from / written to a JSON file keyed by the batch's problem shapes."""
+        if self.gemm != "fp32":
+            # one tile shape: nothing to choose
+            return {"shapes": list(self.gemm_shapes), "ms": {}, "gemm": self.gemm}
         key = self.signature()
         cache = {}
         if cache_path and os.path.exists(cache_path):

@@ -438,6 +438,48 @@ int afq_gemm_tile_shape(int shape, int* bm, int* bn);
 int afq_sgemm_nt_grouped(int count, const afq_gemm_problem* problems, int total_tiles, int shape,
                          void* stream);
 
+/* One problem of a grouped bf16x3 GEMM: C (M x N) = alpha A (M x K) B^T (N x K)
+ * with A and B as bf16 hi / lo planes (afq_split_bf16); K, lda and ldb
+ * multiples of 8, planes 16-byte aligned. */
+typedef struct afq_gemm_problem_bf16x3 {
+    int M, N, K;
+    int tile_start;              /* first workgroup: prefix sum of the problems' 128 x 128 tiles */
+    long long lda, ldb, ldc;
+    const unsigned short* A_hi;
+    const unsigned short* A_lo;
+    const unsigned short* B_hi;
+    const unsigned short* B_lo;
+    void* C;                     /* float32, or float64 when c_fp64 != 0 */
+    float alpha;
+    int c_fp64;
+    int sym_n, sym_ldv;          /* sym_n > 0: the symmetric VHS store of afq_gemm_problem */
+} afq_gemm_problem_bf16x3;
+
+/*
+ * Every problem's bf16x3 GEMM (afq_gemm_bf16x3_nt's 128 x 128 tile) in one
+ * launch of total_tiles workgroups.  The force-bias and VHS contractions of
+ * every system of a grouped step in split-bf16 precision (meas/rhf.py:33,
+ * prop/chol_afqmc_ops.py:72-73 per system).
+ */
+int afq_gemm_bf16x3_nt_grouped(int count, const afq_gemm_problem_bf16x3* problems, int total_tiles,
+                               void* stream);
+
+/* One problem of afq_split_bf16_grouped: afq_split_bf16's arguments. */
+typedef struct afq_split_problem {
+    const double* src;
+    unsigned short* hi;
+    unsigned short* lo;
+    int rows, cols;
+    int ld_src, ld_dst;
+} afq_split_problem;
+
+/*
+ * afq_split_bf16 of every problem in one launch (count <= 65535; max_elems =
+ * the largest rows * ld_dst).  Operand preparation of the grouped bf16x3
+ * GEMM; no counterpart in the reference.
+ */
+int afq_split_bf16_grouped(int count, const afq_split_problem* problems, long long max_elems, void* stream);
+
 /*
  * Per system: key_out, sub = split(key_in); fields = normal(sub, (nw, naux))
  * (prop/afqmc.py:86-88), one launch for all systems; nw * naux < 2^31 each.

@@ -6,7 +6,10 @@
   system run alone with afqmc_step / orthonormalize_state / measure /
   reconfigure in fp32 precision: walkers, weights, log-overlaps, population
   shifts, keys and block energies, bit for bit (the same kernel bodies);
-* a block captured as a HIP graph and replayed equals the eager batch.
+* a block captured as a HIP graph and replayed equals the eager batch;
+* the grouped bf16x3 GEMM and operand split equal their single-problem
+  entry points bit for bit, and a batch with bf16x3 contractions stays within
+  the split-bf16 error of the fp32 batch.
 """
 import numpy as np
 import pytest
@@ -123,7 +126,7 @@ def test_batch_matches_systems_run_alone(gpu_device, shapes):
     from ad_afqmc_prototype_amd.prop.batched import system_batch
     print(f"uneven batch of {len(SYSTEMS)} systems, GEMM shapes {shapes}", flush=True)
     members = _members(gpu_device)
-    batch = system_batch(members, dt=DT, qr_every=QR, sr_every=SR, gemm_shapes=shapes)
+    batch = system_batch(members, dt=DT, qr_every=QR, sr_every=SR, gemm_shapes=shapes, gemm="fp32")
     n_steps = 2 * SR
     per_block = []
     for _ in range(n_steps // SR):
@@ -138,12 +141,13 @@ def test_batch_matches_systems_run_alone(gpu_device, shapes):
     assert float(w.min()) > 0.0 and np.isfinite(per_block[-1]).all()
 
 
-def test_batch_graph_replay_matches_eager(gpu_device):
+@pytest.mark.parametrize("gemm", ["bf16x3", "fp32"])
+def test_batch_graph_replay_matches_eager(gpu_device, gemm):
     import torch
     from ad_afqmc_prototype_amd.prop.batched import captured_batch_block, system_batch
-    print("captured batch block vs eager", flush=True)
-    eager = system_batch(_members(gpu_device), dt=DT, qr_every=QR, sr_every=SR)
-    graph_batch = system_batch(_members(gpu_device), dt=DT, qr_every=QR, sr_every=SR)
+    print(f"captured batch block vs eager ({gemm})", flush=True)
+    eager = system_batch(_members(gpu_device), dt=DT, qr_every=QR, sr_every=SR, gemm=gemm)
+    graph_batch = system_batch(_members(gpu_device), dt=DT, qr_every=QR, sr_every=SR, gemm=gemm)
     cap = captured_batch_block(graph_batch, warmup_blocks=1)   # advances one block eagerly
     eager.advance(SR)
     for rep in range(3):
@@ -161,10 +165,87 @@ def test_batch_graph_replay_matches_eager(gpu_device):
 
 def test_tune_chooses_shapes_and_caches(gpu_device, tmp_path):
     from ad_afqmc_prototype_amd.prop.batched import GEMM_SHAPES, system_batch
-    batch = system_batch(_members(gpu_device), dt=DT, qr_every=QR, sr_every=SR)
+    batch = system_batch(_members(gpu_device), dt=DT, qr_every=QR, sr_every=SR, gemm="fp32")
     path = str(tmp_path / "tiles.json")
     entry = batch.tune(reps=2, cache_path=path)
     assert tuple(entry["shapes"]) == batch.gemm_shapes
     assert all(s in GEMM_SHAPES for s in batch.gemm_shapes)
-    again = system_batch(_members(gpu_device), dt=DT, qr_every=QR, sr_every=SR)
+    again = system_batch(_members(gpu_device), dt=DT, qr_every=QR, sr_every=SR, gemm="fp32")
     assert again.tune(cache_path=path)["shapes"] == entry["shapes"]
+
+
+def test_grouped_bf16x3_gemm_and_split_match_single(gpu_device):
+    import torch
+    from ad_afqmc_prototype_amd import _lib, kernels
+    from ad_afqmc_prototype_amd.prop import batched as B
+    g = torch.Generator(device="cpu").manual_seed(5)
+    dims = [(1, 1, 1), (37, 70, 19), (130, 129, 260), (64, 700, 96), (300, 17, 1001)]
+    st = torch.cuda.current_stream().cuda_stream
+    ops, splits = [], (B._split_problem * len(dims))()
+    sources = []  # the split reads them after the loop: keep them allocated
+    biggest = 0
+    for i, (M, N, K) in enumerate(dims):
+        a = torch.randn((M, K), generator=g, dtype=torch.float64).to(gpu_device)
+        b = torch.randn((N, K), generator=g, dtype=torch.float64).to(gpu_device)
+        sources.append(a)
+        ah, al = kernels.split_bf16(a)
+        bh, bl = kernels.split_bf16(b)
+        kp = ah.shape[1]
+        # the grouped split of a into fresh planes
+        gh, gl = torch.full_like(ah, -1), torch.full_like(al, -1)
+        q = splits[i]
+        q.src, q.hi, q.lo = a.data_ptr(), gh.data_ptr(), gl.data_ptr()
+        q.rows, q.cols, q.ld_src, q.ld_dst = M, K, K, kp
+        biggest = max(biggest, M * kp)
+        c64 = i % 2 == 0
+        ref = torch.empty((M, N), dtype=torch.float64 if c64 else torch.float32, device=gpu_device)
+        _lib.call("afq_gemm_bf16x3_nt", M, N, kp, 0.75, ah.data_ptr(), al.data_ptr(), kp, bh.data_ptr(),
+                  bl.data_ptr(), kp, ref.data_ptr(), N, int(c64), st)
+        ops.append((ah, al, bh, bl, gh, gl, ref, c64))
+    stab = B._upload(splits).to(gpu_device)
+    _lib.call("afq_split_bf16_grouped", len(dims), stab.data_ptr(), biggest, st)
+    arr = (B._gemm_problem_bf16x3 * len(ops))()
+    outs, start = [], 0
+    for p, (ah, al, bh, bl, gh, gl, ref, c64) in zip(arr, ops):
+        out = torch.full_like(ref, float("nan"))
+        outs.append(out)
+        M, kp = ah.shape
+        N = bh.shape[0]
+        p.M, p.N, p.K, p.tile_start = M, N, kp, start
+        p.lda, p.ldb, p.ldc = kp, kp, N
+        p.A_hi, p.A_lo, p.B_hi, p.B_lo = gh.data_ptr(), gl.data_ptr(), bh.data_ptr(), bl.data_ptr()
+        p.C, p.alpha, p.c_fp64 = out.data_ptr(), 0.75, int(c64)
+        start += -(-M // B.BF16X3_TILE) * -(-N // B.BF16X3_TILE)
+    tab = B._upload(arr).to(gpu_device)
+    _lib.call("afq_gemm_bf16x3_nt_grouped", len(ops), tab.data_ptr(), start, st)
+    torch.cuda.synchronize()
+    for i, (out, (ah, al, _, _, gh, gl, ref, _)) in enumerate(zip(outs, ops)):
+        assert torch.equal(gh, ah) and torch.equal(gl, al), f"problem {i}: grouped split differs"
+        assert torch.equal(out, ref), f"problem {i}: grouped bf16x3 GEMM differs"
+
+
+def test_batch_bf16x3_close_to_fp32(gpu_device):
+    """Same fields, same start: after two steps (one QR) the walkers and
+    weights of the bf16x3 batch are within the split-bf16 error (~2^-16
+    relative per contraction) of the fp32 batch's, and so is the first block
+    energy (measured before any comb)."""
+    import torch
+    from ad_afqmc_prototype_amd.prop.batched import system_batch
+    print("bf16x3 batch vs fp32 batch", flush=True)
+    a = system_batch(_members(gpu_device), dt=DT, qr_every=QR, sr_every=SR, gemm="bf16x3")
+    b = system_batch(_members(gpu_device), dt=DT, qr_every=QR, sr_every=SR, gemm="fp32")
+    a.advance(2)
+    b.advance(2)
+    torch.cuda.synchronize()
+    for i in range(len(SYSTEMS)):
+        x, y = a.system_state(i), b.system_state(i)
+        scale = float(y["walkers"].abs().max())
+        dw = float((x["walkers"] - y["walkers"]).abs().max()) / scale
+        assert dw < 1e-4, f"system {i}: walkers differ by {dw:.2e} (relative to max |w|)"
+        assert torch.allclose(x["weights"], y["weights"], rtol=1e-4, atol=0), f"system {i}: weights"
+        assert torch.equal(x["key"], y["key"]), f"system {i}: key"
+    a.advance(SR - 2)
+    b.advance(SR - 2)
+    ea, eb = a.block_stats.cpu().numpy(), b.block_stats.cpu().numpy()
+    assert np.isfinite(ea).all()
+    np.testing.assert_allclose(ea[:, 0], eb[:, 0], rtol=1e-5, atol=0)
