@@ -71,6 +71,7 @@ _SIGNATURES = {
     "afq_rng_fields_grouped": [c_int, c_vp, c_int, c_vp],
     "afq_field_shifts_grouped": [c_int, c_vp, c_int, c_vp],
     "afq_trotter_f32_grouped": [c_int, c_vp, c_int, c_int, c_int, c_vp],
+    "afq_trotter_bf16x3_grouped": [c_int, c_vp, c_int, c_int, c_int, c_vp],
     "afq_overlap_greens_grouped": [c_int, c_vp, c_int, c_int, c_vp],
     "afq_phaseless_update_grouped": [c_int, c_vp, c_vp],
     "afq_orthonormalize_grouped": [c_int, c_vp, c_int, c_int, c_int, c_vp],
@@ -82,7 +83,7 @@ _SIGNATURES = {
     "afq_unpack_pairs": [c_int, c_int, c_vp, c_ll, c_vp, c_vp],
 }
 
-ABI_VERSION = 7
+ABI_VERSION = 8
 
 
 class HipLibraryError(RuntimeError):

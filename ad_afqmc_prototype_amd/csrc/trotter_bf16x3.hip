@@ -152,14 +152,39 @@ __device__ __forceinline__ split8 split_minus_identity(const float (&x)[8], int 
 // SYMV: vhs holds each walker's V, and E holds exp(-dt h1 / 2), as upper 16 x 16
 // blocks (afq_vhs_sym's column order, nb (nb + 1) / 2 * 256 floats per matrix)
 // instead of (n, ldv) rows; both are symmetric.
-template <int NL, bool SYMV = false>
+// GROUPED: the workgroup's system is found in the descriptor table S by the
+// prefix sums tblock0 (count systems, restricted walkers, (n, ldv) layouts);
+// its sizes and pointers replace the arguments, and blockIdx.x counts from
+// the system's first workgroup.
+template <int NL, bool SYMV = false, bool GROUPED = false>
 __global__ __launch_bounds__(512, 1) void trotter_bf16x3_kernel(
     int nw, int n, int o, int nspin, int nct, int ldv, int n_terms, double sqrt_dt,
     const float* __restrict__ E, const float* __restrict__ vhs, long long vplane,
     const double2* __restrict__ win_up, const double2* __restrict__ win_dn,
-    double2* __restrict__ wout_up, double2* __restrict__ wout_dn, int group_map) {
+    double2* __restrict__ wout_up, double2* __restrict__ wout_dn, int group_map, int count = 0,
+    const afq_system_step* __restrict__ S = nullptr) {
     constexpr int KMAX = KR + NL;
     extern __shared__ __attribute__((aligned(16))) unsigned char lds_raw[];
+    int b = blockIdx.x;
+    if constexpr (GROUPED) {
+        const int t = afq::xcd_contiguous(blockIdx.x, gridDim.x);
+        const int si = afq::find_group(count, t, [&](int i) { return S[i].tblock0; });
+        const afq_system_step& d = S[si];
+        n = d.n;
+        o = d.o;
+        ldv = d.ldv;
+        nw = d.nw;
+        nspin = 1;
+        nct = afq::ceil_div(o, 16);
+        sqrt_dt = d.sqrt_dt;
+        E = d.exp_h1_half;
+        vhs = d.vhs;
+        vplane = (long long)nw * n * ldv;
+        win_up = win_dn = (const double2*)d.walkers_in;
+        wout_up = wout_dn = (double2*)d.walkers_out;
+        group_map = 0;
+        b = t - d.tblock0;
+    }
     const int NP = afq::round_up(n, 16);
     const int nk = afq::ceil_div(NP, 32);  // == KMAX when NL > 0, <= KR otherwise
     const int TP = nk * 32 + 8;            // bf16 pitch of one column of T
@@ -167,7 +192,6 @@ __global__ __launch_bounds__(512, 1) void trotter_bf16x3_kernel(
     uint4* VL = reinterpret_cast<uint4*>(lds_raw + (size_t)4 * 16 * TP * sizeof(__bf16));
 
     const int G = nspin * nct;
-    const int b = blockIdx.x;
     int w, sub;
     if (group_map) {
         w = (b / (8 * G)) * 8 + (b % 8);
@@ -557,4 +581,36 @@ extern "C" int afq_trotter_bf16x3_sym(int nw, int n, int o, int nspin, int n_ter
                 "afq_trotter_bf16x3_sym: n=%d exceeds the on-chip maximum of 224", n);
     return launch_bf16x3_nl<true>(nl, nw, n, o, nspin, ldv, n_terms, sqrt_dt, exp_h1_half_sym, vhs_sym, win_up,
                                   win_dn, wout_up, wout_dn, (hipStream_t)stream);
+}
+
+extern "C" int afq_trotter_bf16x3_grouped(int count, const afq_system_step* sys, int total_blocks, int max_n,
+                                          int n_terms, void* stream) {
+    AFQ_REQUIRE(count > 0 && total_blocks > 0, "afq_trotter_bf16x3_grouped: count=%d total_blocks=%d", count,
+                total_blocks);
+    AFQ_REQUIRE(sys, "afq_trotter_bf16x3_grouped: null pointer");
+    AFQ_REQUIRE(max_n > 0 && max_n <= 16 * 2 * (KR + NL_MAX), "afq_trotter_bf16x3_grouped: max_n=%d not in 1..%d",
+                max_n, 16 * 2 * (KR + NL_MAX));
+    AFQ_REQUIRE(n_terms >= 0, "afq_trotter_bf16x3_grouped: n_terms < 0");
+    const int NP = afq::round_up(max_n, 16);
+    const int nk = afq::ceil_div(NP, 32);
+    const int nl = nk > KR ? nk - KR : 0;
+    const size_t lds = bf16x3_lds(max_n, nl);
+    AFQ_REQUIRE(lds <= 160 * 1024, "afq_trotter_bf16x3_grouped: max_n=%d needs %zu bytes of LDS", max_n, lds);
+    const dim3 block(64 * afq::ceil_div(NP / 16, RT));
+    hipStream_t st = (hipStream_t)stream;
+    const dim3 grid((unsigned)total_blocks);
+#define AFQ_TROTTER_GROUPED(NLV)                                                                                   \
+    do {                                                                                                           \
+        if (lds > 64 * 1024) afq::allow_full_lds((const void*)trotter_bf16x3_kernel<NLV, false, true>);            \
+        hipLaunchKernelGGL((trotter_bf16x3_kernel<NLV, false, true>), grid, block, lds, st, 0, 0, 0, 1, 0, 0,         \
+                           n_terms, 0.0, nullptr, nullptr, 0LL, nullptr, nullptr, nullptr, nullptr, 0, count, sys); \
+    } while (0)
+    if (nl == 0)
+        AFQ_TROTTER_GROUPED(0);
+    else if (nl == 1)
+        AFQ_TROTTER_GROUPED(1);
+    else
+        AFQ_TROTTER_GROUPED(2);
+#undef AFQ_TROTTER_GROUPED
+    return afq::check_launch("afq_trotter_bf16x3_grouped");
 }

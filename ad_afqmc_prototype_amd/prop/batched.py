@@ -153,20 +153,21 @@ class system_batch:
     measurement and comb (every ``sr_every`` steps, prop/blocks.py:53-92) run
     on the A set; both cadences must be even.
 
-    ``gemm`` is the arithmetic of the force-bias and VHS contractions:
-    "bf16x3" (split-bf16 operands on the bf16 matrix cores, fp32 sums; one
-    128 x 128 tile shape) or "fp32" (fp32 matrix cores, tile shape picked by
-    tune()).  Trotter, Green's functions and energies are fp32 / fp64 in both."""
+    ``precision`` is the arithmetic of the force-bias and VHS contractions and
+    of the Trotter step: "bf16x3" (split-bf16 operands on the bf16 matrix
+    cores, fp32 sums; one 128 x 128 GEMM tile shape) or "fp32" (fp32 matrix
+    cores, GEMM tile shape picked by tune()).  Green's functions, QR, weights
+    and the energy (fp32 products, fp64 sums) are the same in both."""
 
     def __init__(self, systems: Sequence[batch_system], *, dt: float, qr_every: int = 10,
                  sr_every: int = 20, n_exp_terms: int = 6, weight_floor: float = 1.0e-3,
                  weight_cap: float = 100.0, damping: float = 0.1, shift_ema: float = 0.1,
-                 gemm_shapes: tuple[int, int] | None = None, gemm: str = "bf16x3"):
+                 gemm_shapes: tuple[int, int] | None = None, precision: str = "bf16x3"):
         if not systems:
             raise ValueError("empty batch")
-        if gemm not in ("bf16x3", "fp32"):
-            raise ValueError(f"gemm must be 'bf16x3' or 'fp32', not {gemm!r}")
-        self.gemm = gemm
+        if precision not in ("bf16x3", "fp32"):
+            raise ValueError(f"precision must be 'bf16x3' or 'fp32', not {precision!r}")
+        self.precision = precision
         if qr_every % 2 or sr_every % 2 or qr_every <= 0 or sr_every <= 0:
             raise ValueError("qr_every and sr_every must be positive and even")
         dev = systems[0].ham.h1.device
@@ -209,7 +210,7 @@ class system_batch:
         st = init_prop_state(sys=sys_, n_walkers=s.n_walkers, seed=s.seed, ham_data=s.ham,
                              trial_ops_=t_ops, trial_data=s.trial, meas_ops=m_ops)
         rot_pad, rot_h1 = m_ctx.energy_operands(1)
-        if self.gemm == "bf16x3":
+        if self.precision == "bf16x3":
             ops = dict(chol_hl=kernels.split_bf16(chol_sym_rows(s.ham.chol)),
                        rot_hl=kernels.split_bf16(m_ctx.rot_chol_flat))
         else:
@@ -225,7 +226,7 @@ class system_batch:
         self.walkers = [P([m["nw"] * m["n"] * m["o"] for m in ms], c128) for _ in range(2)]
         self.greens = P([2 * m["nw"] * m["o"] * m["n"] for m in ms], f64)
         pad8 = kernels.pad8
-        if self.gemm == "bf16x3":
+        if self.precision == "bf16x3":
             i16 = torch.int16
             self.greens_hl = [P([2 * m["nw"] * pad8(m["o"] * m["n"]) for m in ms], i16) for _ in range(2)]
             self.xbar_hl = [P([2 * m["nw"] * pad8(m["g"]) for m in ms], i16) for _ in range(2)]
@@ -235,7 +236,7 @@ class system_batch:
         self.fields = P([m["nw"] * m["g"] for m in ms], f64)
         self.fb = P([2 * m["nw"] * m["g"] for m in ms], f64)
         self.xbar = P([2 * m["nw"] * m["g"] for m in ms], f64)
-        if self.gemm == "fp32":
+        if self.precision == "fp32":
             self.xbar32 = P(self.xbar.sizes, f32)
         self.shift = P([m["nw"] for m in ms], c128)
         self.fbt = P([m["nw"] for m in ms], c128)
@@ -364,8 +365,11 @@ class system_batch:
             for i, m in enumerate(self.members):
                 out.setdefault(key(m), []).append(i)
             return [out[k] for k in sorted(out)]
+        # (bf16x3: one bucket per pair of row tiles, the bf16x3 kernel's wave count)
         self.trotter_buckets = []
-        for idx in buckets(lambda m: (m["n"] + 15) // 16):
+        tiles = ((lambda m: ((m["n"] + 15) // 16 + 1) // 2) if self.precision == "bf16x3"
+                 else (lambda m: (m["n"] + 15) // 16))
+        for idx in buckets(tiles):
             tabs = [self._desc(p, idx) for p in (0, 1)]
             self.trotter_buckets.append(dict(
                 tables=[_upload(t[0]).to(self.device) for t in tabs], count=len(idx),
@@ -387,7 +391,7 @@ class system_batch:
         self.gemm_tables = {}
         self.split_tables = {}
         for which in ("fb", "vhs"):
-            if self.gemm == "bf16x3":
+            if self.precision == "bf16x3":
                 arr, tiles = self._problems_bf16x3(which)
                 self.gemm_tables[which] = (_upload(arr).to(self.device), tiles)
                 self.split_tables[which] = self._split_table(which)
@@ -400,7 +404,7 @@ class system_batch:
 
     # ------------------------------------------------------------------ stages
     def _grouped_gemm(self, which: str):
-        if self.gemm == "bf16x3":
+        if self.precision == "bf16x3":
             # the A operand's bf16 planes, then the product
             stab, biggest = self.split_tables[which]
             kernels._call("afq_split_bf16_grouped", stab, self.count, stab.data_ptr(), biggest,
@@ -432,17 +436,17 @@ class system_batch:
         d = self.desc[p]
         st = kernels._stream(d)
         kernels._call("afq_rng_fields_grouped", d, self.count, d.data_ptr(), self.total_walkers, st)
-        if self.gemm == "fp32":
+        if self.precision == "fp32":
             self._convert(self.greens, self.greens32)
         self._grouped_gemm("fb")
         kernels._call("afq_field_shifts_grouped", d, self.count, d.data_ptr(), self.total_walkers, st)
-        if self.gemm == "fp32":
+        if self.precision == "fp32":
             self._convert(self.xbar, self.xbar32)
         self._grouped_gemm("vhs")
         for bk in self.trotter_buckets:
             t = bk["tables"][p]
-            kernels._call("afq_trotter_f32_grouped", t, bk["count"], t.data_ptr(), bk["blocks"],
-                          bk["max_n"], self.n_exp_terms, st)
+            kernels._call("afq_trotter_bf16x3_grouped" if self.precision == "bf16x3" else "afq_trotter_f32_grouped",
+                          t, bk["count"], t.data_ptr(), bk["blocks"], bk["max_n"], self.n_exp_terms, st)
         self._greens(p, st)
         kernels._call("afq_phaseless_update_grouped", d, self.count, d.data_ptr(), st)
         self.steps_done += 1
@@ -531,9 +535,9 @@ class system_batch:
         contraction by timing each shape on this batch (HIP events, ``reps``
         launches after one warm-up).  With ``cache_path`` the choice is read
         from / written to a JSON file keyed by the batch's problem shapes."""
-        if self.gemm != "fp32":
+        if self.precision != "fp32":
             # one tile shape: nothing to choose
-            return {"shapes": list(self.gemm_shapes), "ms": {}, "gemm": self.gemm}
+            return {"shapes": list(self.gemm_shapes), "ms": {}, "precision": self.precision}
         key = self.signature()
         cache = {}
         if cache_path and os.path.exists(cache_path):

@@ -272,7 +272,7 @@ def batched_run(device, n_steps: int, n_systems: int = 64, chol_scale: float = 0
     out = {"workload": f"{n_systems} uneven systems (seeds {specs[0]['seed']}..{specs[-1]['seed']}), "
                        f"{nw_tot} walkers, chol_scale {chol_scale}",
            "n_systems": n_systems, "total_walkers": nw_tot, "steps": n_steps, "dt": dt,
-           "qr_every": qr, "measure_sr_every": sr, "gemm": batch.gemm, "gemm_shapes": list(batch.gemm_shapes),
+           "qr_every": qr, "measure_sr_every": sr, "precision": batch.precision, "gemm_shapes": list(batch.gemm_shapes),
            "tune_ms": tuned["ms"], "gflop_per_step": round(batch.flops_per_step() / 1e9, 2)}
     # eager grouped launches (one block of warm-up first)
     batch.advance(sr)
@@ -297,8 +297,8 @@ def batched_run(device, n_steps: int, n_systems: int = 64, chol_scale: float = 0
     out["block_energy_finite"] = bool(np.isfinite(stats).all())
     del cap, batch
     torch.cuda.empty_cache()
-    # the same batch with fp32 force-bias / VHS contractions (tuned tile shapes)
-    b32 = system_batch(members, dt=dt, qr_every=qr, sr_every=sr, gemm="fp32")
+    # the same batch in fp32 (fp32 contractions with tuned tile shapes, fp32 Trotter)
+    b32 = system_batch(members, dt=dt, qr_every=qr, sr_every=sr, precision="fp32")
     t32 = b32.tune(reps=5)
     b32.advance(sr)
     torch.cuda.synchronize()
@@ -306,7 +306,7 @@ def batched_run(device, n_steps: int, n_systems: int = 64, chol_scale: float = 0
     b32.advance(n_steps)
     torch.cuda.synchronize()
     el = time.perf_counter() - t0
-    out["grouped_eager_fp32_gemm"] = {"walker_steps_per_s": nw_tot * n_steps / el, "ms_per_step": 1e3 * el / n_steps,
+    out["grouped_eager_fp32"] = {"walker_steps_per_s": nw_tot * n_steps / el, "ms_per_step": 1e3 * el / n_steps,
                                       "gemm_shapes": list(b32.gemm_shapes), "tune_ms": t32["ms"]}
     del b32
     torch.cuda.empty_cache()

@@ -499,6 +499,16 @@ int afq_trotter_f32_grouped(int count, const afq_system_step* sys, int total_blo
                             int n_terms, void* stream);
 
 /*
+ * afq_trotter_bf16x3 for every system of one bucket of equal ceil(round_up(n,
+ * 16) / 32) (the workgroup size): walkers_in -> walkers_out with the system's
+ * fp32 exp_h1_half and VHS in (n, ldv) rows (prop/chol_afqmc_ops.py:173-196,
+ * prop/utils.py:20-31).  total_blocks = sum of nw * ceil(o / 16); max_n the
+ * largest n of the bucket (<= 224).
+ */
+int afq_trotter_bf16x3_grouped(int count, const afq_system_step* sys, int total_blocks, int max_n,
+                               int n_terms, void* stream);
+
+/*
  * afq_overlap_greens for every system, of walkers_out into greens / logdet
  * (meas/rhf.py:15-24, trial/rhf.py:25-37).  max_o the largest o.
  */

@@ -8,8 +8,8 @@
   shifts, keys and block energies, bit for bit (the same kernel bodies);
 * a block captured as a HIP graph and replayed equals the eager batch;
 * the grouped bf16x3 GEMM and operand split equal their single-problem
-  entry points bit for bit, and a batch with bf16x3 contractions stays within
-  the split-bf16 error of the fp32 batch.
+  entry points bit for bit, and a batch in bf16x3 precision (contractions and
+  Trotter) stays within the split-bf16 error of the fp32 batch.
 """
 import numpy as np
 import pytest
@@ -126,7 +126,7 @@ def test_batch_matches_systems_run_alone(gpu_device, shapes):
     from ad_afqmc_prototype_amd.prop.batched import system_batch
     print(f"uneven batch of {len(SYSTEMS)} systems, GEMM shapes {shapes}", flush=True)
     members = _members(gpu_device)
-    batch = system_batch(members, dt=DT, qr_every=QR, sr_every=SR, gemm_shapes=shapes, gemm="fp32")
+    batch = system_batch(members, dt=DT, qr_every=QR, sr_every=SR, gemm_shapes=shapes, precision="fp32")
     n_steps = 2 * SR
     per_block = []
     for _ in range(n_steps // SR):
@@ -141,13 +141,13 @@ def test_batch_matches_systems_run_alone(gpu_device, shapes):
     assert float(w.min()) > 0.0 and np.isfinite(per_block[-1]).all()
 
 
-@pytest.mark.parametrize("gemm", ["bf16x3", "fp32"])
-def test_batch_graph_replay_matches_eager(gpu_device, gemm):
+@pytest.mark.parametrize("precision", ["bf16x3", "fp32"])
+def test_batch_graph_replay_matches_eager(gpu_device, precision):
     import torch
     from ad_afqmc_prototype_amd.prop.batched import captured_batch_block, system_batch
-    print(f"captured batch block vs eager ({gemm})", flush=True)
-    eager = system_batch(_members(gpu_device), dt=DT, qr_every=QR, sr_every=SR, gemm=gemm)
-    graph_batch = system_batch(_members(gpu_device), dt=DT, qr_every=QR, sr_every=SR, gemm=gemm)
+    print(f"captured batch block vs eager ({precision})", flush=True)
+    eager = system_batch(_members(gpu_device), dt=DT, qr_every=QR, sr_every=SR, precision=precision)
+    graph_batch = system_batch(_members(gpu_device), dt=DT, qr_every=QR, sr_every=SR, precision=precision)
     cap = captured_batch_block(graph_batch, warmup_blocks=1)   # advances one block eagerly
     eager.advance(SR)
     for rep in range(3):
@@ -165,12 +165,12 @@ def test_batch_graph_replay_matches_eager(gpu_device, gemm):
 
 def test_tune_chooses_shapes_and_caches(gpu_device, tmp_path):
     from ad_afqmc_prototype_amd.prop.batched import GEMM_SHAPES, system_batch
-    batch = system_batch(_members(gpu_device), dt=DT, qr_every=QR, sr_every=SR, gemm="fp32")
+    batch = system_batch(_members(gpu_device), dt=DT, qr_every=QR, sr_every=SR, precision="fp32")
     path = str(tmp_path / "tiles.json")
     entry = batch.tune(reps=2, cache_path=path)
     assert tuple(entry["shapes"]) == batch.gemm_shapes
     assert all(s in GEMM_SHAPES for s in batch.gemm_shapes)
-    again = system_batch(_members(gpu_device), dt=DT, qr_every=QR, sr_every=SR, gemm="fp32")
+    again = system_batch(_members(gpu_device), dt=DT, qr_every=QR, sr_every=SR, precision="fp32")
     assert again.tune(cache_path=path)["shapes"] == entry["shapes"]
 
 
@@ -232,8 +232,8 @@ def test_batch_bf16x3_close_to_fp32(gpu_device):
     import torch
     from ad_afqmc_prototype_amd.prop.batched import system_batch
     print("bf16x3 batch vs fp32 batch", flush=True)
-    a = system_batch(_members(gpu_device), dt=DT, qr_every=QR, sr_every=SR, gemm="bf16x3")
-    b = system_batch(_members(gpu_device), dt=DT, qr_every=QR, sr_every=SR, gemm="fp32")
+    a = system_batch(_members(gpu_device), dt=DT, qr_every=QR, sr_every=SR, precision="bf16x3")
+    b = system_batch(_members(gpu_device), dt=DT, qr_every=QR, sr_every=SR, precision="fp32")
     a.advance(2)
     b.advance(2)
     torch.cuda.synchronize()
