@@ -281,7 +281,11 @@ void launch_cholqr_grouped(int count, int total, const afq_system_step* sys, hip
 // ---------------------------------------------------------------------------
 // Comb indices.  The cumulative sum is one sequential left-to-right fp64 pass
 // (numpy.cumsum order) and z_k = (tot * (k + zeta)) / nw is evaluated without
-// contraction, so the indices are reproducible bit for bit.
+// contraction, so the indices are reproducible bit for bit.  The search uses
+// searchsorted's order, in which NaN sorts above every number: with a NaN
+// weight the total and every tooth are NaN, and each tooth lands on the first
+// NaN of the cumulative sum (the walker whose weight is NaN).  An index past
+// the end is clamped to nw - 1, as an out-of-range gather index is.
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(1024) void comb_kernel(int nw, const double* __restrict__ weights,
                                                      double zeta_v, const double* __restrict__ zeta_p,
@@ -310,7 +314,8 @@ __global__ __launch_bounds__(1024) void comb_kernel(int nw, const double* __rest
         int lo = 0, hi = nw;
         while (lo < hi) {
             const int mid = (lo + hi) >> 1;
-            if (cw[mid] < z)
+            const double c = cw[mid];
+            if (c < z || (z != z && c == c))
                 lo = mid + 1;
             else
                 hi = mid;

@@ -50,7 +50,9 @@ def run_afqmc_energy(*, sys: system, params: afqmc_params, ham_data: ham_chol, t
                      block_fn: Callable = afqmc_block, prop_ops=None, verbose: bool = True,
                      use_graph: bool = False):
     """Returns (mean_energy, stderr, block_energies, block_weights) like
-    driver.py:39-176; block arrays include the initial estimate.  use_graph
+    driver.py:39-176: the block arrays hold the initial estimate, the
+    equilibration blocks and the sampling blocks left after outlier rejection
+    (driver.py:161-176).  use_graph
     replays one captured afqmc_block per block (same results as the eager
     block loop)."""
     if use_graph and block_fn is not afqmc_block:
@@ -127,5 +129,5 @@ def run_afqmc_energy(*, sys: system, params: afqmc_params, ham_data: ham_chol, t
     log(f"\nRejected {e_s.shape[0] - clean.shape[0]} outlier blocks.")
     log("\nFinal blocking analysis:")
     st = blocking_analysis_ratio(clean[:, 0], clean[:, 1], print_q=verbose)
-    return (st["mu"], st["se_star"], np.concatenate([np.asarray(e_eq), e_s]),
-            np.concatenate([np.asarray(w_eq), w_s]))
+    return (st["mu"], st["se_star"], np.concatenate([np.asarray(e_eq), clean[:, 0]]),
+            np.concatenate([np.asarray(w_eq), clean[:, 1]]))

@@ -355,25 +355,46 @@ def field_shift_terms(fields: np.ndarray, fb: np.ndarray, pctx: prop_ctx):
     return shifted, shift_term, fb_term
 
 
-def phaseless_update(ov_new, ov_old, shift_term, fb_term, weights, pop_shift, e_estimate,
-                     pctx: prop_ctx, prm: params, overlap_mode: str = "raw"):
-    """prop/afqmc.py:116-138.  Returns (weights_new, pop_shift_new).
+def importance(ov_new, ov_old, shift_term, fb_term, pop_shift, pctx: prop_ctx,
+               overlap_mode: str = "raw"):
+    """prop/afqmc.py:116-124: the complex importance function
+    imp_fun = exp(exponent) * ratio with
+    exponent = -sqrt_dt shift_term + fb_term + dt (pop_shift + h0_prop),
+    ratio = ov_new / ov_old, and the phase theta = angle(exp(-sqrt_dt shift_term) * ratio).
 
-    overlap_mode="raw" is the reference formula verbatim.  In "log" mode the
-    importance magnitude is exp(Re(exponent + dlog)) and the phase
-    Im(-sqrt_dt shift_term + dlog): the same numbers wherever the raw
-    exp(exponent) * ratio is finite, without its overflow."""
+    Returns (log_imp, mag, theta): log_imp the complex log of imp_fun (its
+    imaginary part is defined modulo 2 pi), mag = |imp_fun|.  overlap_mode="raw"
+    is the reference formula verbatim; "log" takes ov_* as log-overlaps and
+    forms the same numbers as exp(Re(exponent + dlog)) and
+    Im(-sqrt_dt shift_term + dlog), without the overflow of the raw product."""
     with np.errstate(all="ignore"):
         exponent = (-pctx.sqrt_dt * shift_term + fb_term
                     + pctx.dt * (pop_shift + pctx.h0_prop))
         if overlap_mode == "log":
             dlog = ov_new - ov_old
-            mag = np.exp(np.real(exponent + dlog))
+            log_imp = exponent + dlog
+            mag = np.exp(np.real(log_imp))
             theta = np.imag(-pctx.sqrt_dt * shift_term + dlog)
         else:
             ratio = ov_new / ov_old
-            mag = np.abs(np.exp(exponent) * ratio)
+            imp = np.exp(exponent) * ratio
+            log_imp = np.log(imp)
+            mag = np.abs(imp)
             theta = np.angle(np.exp(-pctx.sqrt_dt * shift_term) * ratio)
+    return log_imp, mag, theta
+
+
+def phaseless_update(ov_new, ov_old, shift_term, fb_term, weights, pop_shift, e_estimate,
+                     pctx: prop_ctx, prm: params, overlap_mode: str = "raw"):
+    """prop/afqmc.py:116-138.  Returns (weights_new, pop_shift_new).
+
+    The importance magnitude and phase come from importance(); then the
+    phaseless projection |imp| cos(theta), the floor / cap cuts (NaN passes
+    both, as NaN comparisons are false), the weight cap and the population
+    shift from the clipped mean weight."""
+    _, mag, theta = importance(ov_new, ov_old, shift_term, fb_term, pop_shift, pctx,
+                               overlap_mode)
+    with np.errstate(all="ignore"):
         imp_ph = mag * np.cos(theta)
         imp_ph = np.where(imp_ph < prm.weight_floor, 0.0, imp_ph)
         imp_ph = np.where(imp_ph > prm.weight_cap, 0.0, imp_ph)
@@ -422,11 +443,18 @@ def orthonormalize(w, kind: str):
 
 
 def sr_indices(weights: np.ndarray, zeta: float, nw: int) -> np.ndarray:
-    """_sr_indices (walkers.py:233-240): sequential cumsum, comb, searchsorted left."""
+    """_sr_indices (walkers.py:233-240): sequential cumsum, comb, searchsorted left.
+
+    searchsorted orders NaN after every number (numpy and jax.numpy alike), so
+    with a NaN weight every tooth (z = NaN) lands on the first NaN of the
+    cumulative sum.  An index past the end (a tooth above the last cumulative
+    sum by rounding) is clamped to nw - 1, as jax's gather clamps the
+    out-of-range index of w[idx] (walkers.py:258-263)."""
     cw = np.cumsum(np.abs(weights))
     tot = cw[-1]
     z = tot * (np.arange(nw) + zeta) / nw
-    return np.searchsorted(cw, z, side="left").astype(np.int32)
+    idx = np.searchsorted(cw, z, side="left")
+    return np.minimum(idx, nw - 1).astype(np.int32)
 
 
 def stochastic_reconfiguration(w, weights: np.ndarray, zeta: float, kind: str):
@@ -441,23 +469,35 @@ def stochastic_reconfiguration(w, weights: np.ndarray, zeta: float, kind: str):
     return w[idx], weights_new, idx
 
 
+def block_energy(e_samples: np.ndarray, weights: np.ndarray, e_ref: float, pop_shift: float,
+                 dt: float, shift_ema: float):
+    """prop/blocks.py:59-79: a sample farther than sqrt(2/dt) from e_ref is
+    replaced by e_ref (NaN samples stay: the comparison is false); the block
+    energy is the weighted mean, or e_ref when the weights sum to zero; the
+    population shift moves by the EMA (1 - a) pop + a e_block.
+    Returns (e_block, w_sum, e_clipped, pop_new)."""
+    e = np.real(np.asarray(e_samples))
+    thresh = np.sqrt(2.0 / dt)
+    with np.errstate(invalid="ignore"):
+        e = np.where(np.abs(e - e_ref) > thresh, e_ref, e)
+    w_sum = np.sum(weights)
+    w_sum_safe = 1.0 if w_sum == 0 else w_sum
+    e_block = np.sum(weights * e) / w_sum_safe
+    if w_sum == 0:
+        e_block = e_ref
+    pop = (1.0 - shift_ema) * pop_shift + shift_ema * e_block
+    return float(e_block), float(w_sum), e, float(np.real(pop))
+
+
 def measure_block(st: state, sd: system_data, prm: params, overlap_mode: str = "raw"):
     """prop/blocks.py:53-79: QR, overlaps, clipped energies, block energy, pop EMA."""
     walkers = orthonormalize(st.walkers, sd.kind)
     ov = overlap(walkers, sd.trial, sd.kind, overlap_mode)
     e = np.real(energy(walkers, sd.h0, sd.trial, sd.mctx, sd.kind))
-    thresh = np.sqrt(2.0 / prm.dt)
-    e_ref = st.e_estimate
-    e = np.where(np.abs(e - e_ref) > thresh, e_ref, e)
-    w_sum = np.sum(st.weights)
-    w_sum_safe = 1.0 if w_sum == 0 else w_sum
-    e_block = np.sum(st.weights * e) / w_sum_safe
-    if w_sum == 0:
-        e_block = e_ref
-    alpha = prm.shift_ema
-    pop = (1.0 - alpha) * st.pop_control_ene_shift + alpha * e_block
-    st = replace(st, walkers=walkers, overlaps=ov, pop_control_ene_shift=float(pop))
-    return st, float(e_block), float(w_sum), e
+    e_block, w_sum, e, pop = block_energy(e, st.weights, st.e_estimate,
+                                          st.pop_control_ene_shift, prm.dt, prm.shift_ema)
+    st = replace(st, walkers=walkers, overlaps=ov, pop_control_ene_shift=pop)
+    return st, e_block, w_sum, e
 
 
 def afqmc_block(st: state, fields_seq, zeta: float, sd: system_data, prm: params,

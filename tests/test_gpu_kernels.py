@@ -217,6 +217,42 @@ def test_comb_indices_bit_exact(gpu_device):
             np.testing.assert_array_equal(w_new.cpu().numpy(), np.full(nw, np.cumsum(np.abs(w))[-1] / nw))
 
 
+def test_comb_indices_nan_weights(gpu_device):
+    """A NaN weight: every tooth lands on the first NaN of the cumulative sum
+    (searchsorted's NaN order), all new weights NaN -- as the checker."""
+    import torch
+    from ad_afqmc_prototype_amd import kernels
+    rng = np.random.default_rng(17)
+    for nw, pos in ((8, 0), (8, 3), (8, 7), (2048, 1500)):
+        w = rng.random(nw)
+        w[pos] = np.nan
+        if nw > 8:
+            w[pos + 10] = np.nan
+        idx, w_new = kernels.comb_indices(torch.tensor(w, device=gpu_device), 0.41)
+        np.testing.assert_array_equal(idx.cpu().numpy(), orc.sr_indices(w, 0.41, nw))
+        np.testing.assert_array_equal(idx.cpu().numpy(), np.full(nw, pos))
+        assert np.all(np.isnan(w_new.cpu().numpy()))
+
+
+def test_block_energy_hand_cases(gpu_device):
+    """Clip window sqrt(2/dt) (0.99 t kept, 1.01 t replaced), all-zero weights
+    (block energy = e_ref) and the shift EMA, against known answers."""
+    import math
+    import torch
+    from ad_afqmc_prototype_amd import kernels
+    from .test_oracle_pinning import _block_cases
+    for e, w, e_ref, pop, dt, ema, (eb, ws, pn) in _block_cases():
+        out = kernels.block_energy(torch.tensor(np.array(e, dtype=complex), device=gpu_device),
+                                   torch.tensor(np.array(w), device=gpu_device),
+                                   torch.tensor(pop, dtype=torch.float64, device=gpu_device),
+                                   e_ref=e_ref, thresh=math.sqrt(2.0 / dt), shift_ema=ema).cpu().numpy()
+        assert out[0] == pytest.approx(eb, rel=1e-14, abs=1e-14)
+        assert out[1] == ws
+        assert out[2] == pytest.approx(pn, rel=1e-14, abs=1e-14)
+        ref = orc.block_energy(np.array(e), np.array(w), e_ref, pop, dt, ema)
+        assert out[0] == pytest.approx(ref[0], rel=1e-15, abs=1e-15)
+
+
 def test_gather_rows(gpu_device):
     import torch
     from ad_afqmc_prototype_amd import kernels

@@ -1,0 +1,428 @@
+"""Pins the checker's field shifts, importance function, block end and comb
+against what the mathematics fixes, independently of the formulas restated in
+oracle/afqmc_oracle.py (prop/afqmc.py:97-138, prop/blocks.py:59-92,
+walkers.py:233-265 of the reference).
+
+* Gaussian-shift identity.  For any shift f (complex), phi(x) e^{x f - f^2/2}
+  = phi(x - f), so the integral over x ~ N(0,1) of the complex importance
+  function exp(exponent) <T|B(x - f)|w>/<T|w> must not depend on the force bias
+  that sets f.  Gauss-Hermite quadrature makes the integral exact to rounding
+  (the integrand is a polynomial times an exponential of a linear form).
+* Many-body brute force.  With one field and a one-body part that commutes
+  with L, the Hubbard-Stratonovich integral is exact, so the same integral
+  equals e^{dt pop} <T|exp(-dt H)|w>/<T|w>, evaluated by expanding the
+  determinants over every occupation of L's eigen-orbitals (Cauchy-Binet).
+  This fixes h0_prop, the mean-field shifts, h1_eff and the exponent's sign
+  conventions, which the shift identity alone leaves open.
+* Force-bias flatness.  With the checker's own force bias, d log(imp)/dx at
+  x = 0 vanishes to leading order: what is left comes from the one-body halves
+  and the O(sqrt dt) shifted field acting on the mixed estimate, O(dt^{3/2}),
+  so quartering dt divides it by 8.  A flipped shift leaves O(sqrt dt)
+  (quartering dt halves it).
+* The phase depends on the shifted fields only.
+* Block-end and comb hand cases with known answers.
+
+Each check is also run on a planted mistake (a mutated copy of the checker's
+function, on the CPU) and must reject it.
+"""
+import itertools
+import math
+
+import numpy as np
+import pytest
+from numpy.polynomial.hermite_e import hermegauss
+
+from oracle import afqmc_oracle as orc
+
+
+def _rand_orthonormal(rng, n, k, cplx=True):
+    a = rng.standard_normal((n, k))
+    if cplx:
+        a = a + 1j * rng.standard_normal((n, k))
+    return np.linalg.qr(a)[0]
+
+
+def _general_system(seed, n, o, g, kind, uhf, dt):
+    rng = np.random.default_rng(seed)
+    a = rng.standard_normal((n, n))
+    h1 = 0.5 * (a + a.T)
+    b = rng.standard_normal((g, n, n))
+    chol = 0.35 * (b + b.transpose(0, 2, 1))
+    cu = _rand_orthonormal(rng, n, o, cplx=False)
+    cd = _rand_orthonormal(rng, n, o, cplx=False) if uhf else cu
+    sd = orc.make_system_data(0.4, h1, chol, (cu, cd), kind, dt)
+    if kind == "restricted":
+        w = cu + 0.4 * (rng.standard_normal((n, o)) + 1j * rng.standard_normal((n, o)))
+    else:
+        w = (cu + 0.4 * (rng.standard_normal((n, o)) + 1j * rng.standard_normal((n, o))),
+             cd + 0.4 * (rng.standard_normal((n, o)) + 1j * rng.standard_normal((n, o))))
+    return sd, w, rng
+
+
+def _bcast(w, k):
+    if isinstance(w, tuple):
+        return tuple(np.broadcast_to(x, (k,) + x.shape).copy() for x in w)
+    return np.broadcast_to(w, (k,) + w.shape).copy()
+
+
+def _quadrature(g, k):
+    x, wt = hermegauss(k)
+    wt = wt / math.sqrt(2.0 * math.pi)
+    nodes = np.array(list(itertools.product(x, repeat=g)))
+    weights = np.prod(np.array(list(itertools.product(wt, repeat=g))), axis=1)
+    return nodes, weights
+
+
+def _integral(sd, w, fb, *, pop, n_terms=6, k=40, mode="log", fields_at=None,
+              shift_fn=None, imp_fn=None):
+    """sum_k omega_k imp(x_k) over Gauss-Hermite nodes x_k (one walker, fb fixed)."""
+    shift_fn = shift_fn or orc.field_shift_terms
+    imp_fn = imp_fn or orc.importance
+    g = sd.chol.shape[0]
+    if fields_at is None:
+        nodes, weights = _quadrature(g, k)
+    else:
+        nodes, weights = fields_at, None
+    nq = nodes.shape[0]
+    wb = _bcast(w, nq)
+    fbq = np.broadcast_to(np.asarray(fb, dtype=complex), (nq, g))
+    shifted, st, ft = shift_fn(nodes, fbq, sd.pctx)
+    wn = orc.apply_trotter(wb, shifted, sd.pctx, sd.chol, n_terms, sd.kind)
+    ov_new = orc.overlap(wn, sd.trial, sd.kind, mode)
+    ov_old = orc.overlap(wb, sd.trial, sd.kind, mode)
+    log_imp, mag, theta = imp_fn(ov_new, ov_old, st, ft, pop, sd.pctx, mode)
+    if weights is None:
+        return log_imp, mag, theta
+    return np.sum(weights * np.exp(log_imp))
+
+
+# --------------------------------------------------------------------------
+# planted mistakes (mutated copies of the checker's functions)
+# --------------------------------------------------------------------------
+
+def _shift_plus_half(fields, fb, pctx):
+    mf = pctx.mf_shifts
+    fs = -pctx.sqrt_dt * (1.0j * fb - mf)
+    shifted = fields - fs
+    return shifted, np.sum(shifted * mf, axis=1), np.sum(fields * fs + 0.5 * fs * fs, axis=1)
+
+
+def _shift_xbar_in_fb_term(fields, fb, pctx):
+    mf = pctx.mf_shifts
+    fs = -pctx.sqrt_dt * (1.0j * fb - mf)
+    shifted = fields - fs
+    return shifted, np.sum(shifted * mf, axis=1), np.sum(shifted * fs - 0.5 * fs * fs, axis=1)
+
+
+def _shift_x_in_shift_term(fields, fb, pctx):
+    mf = pctx.mf_shifts
+    fs = -pctx.sqrt_dt * (1.0j * fb - mf)
+    shifted = fields - fs
+    return shifted, np.sum(fields * mf, axis=1), np.sum(fields * fs - 0.5 * fs * fs, axis=1)
+
+
+def _shift_flipped(fields, fb, pctx):
+    mf = pctx.mf_shifts
+    fs = pctx.sqrt_dt * (1.0j * fb - mf)
+    shifted = fields - fs
+    return shifted, np.sum(shifted * mf, axis=1), np.sum(fields * fs - 0.5 * fs * fs, axis=1)
+
+
+def _imp_shift_sign(ov_new, ov_old, st, ft, pop, pctx, mode):
+    exponent = pctx.sqrt_dt * st + ft + pctx.dt * (pop + pctx.h0_prop)
+    dlog = ov_new - ov_old
+    return exponent + dlog, np.exp(np.real(exponent + dlog)), np.imag(pctx.sqrt_dt * st + dlog)
+
+
+def _imp_h0_sign(ov_new, ov_old, st, ft, pop, pctx, mode):
+    exponent = -pctx.sqrt_dt * st + ft + pctx.dt * (pop - pctx.h0_prop)
+    dlog = ov_new - ov_old
+    return exponent + dlog, np.exp(np.real(exponent + dlog)), np.imag(-pctx.sqrt_dt * st + dlog)
+
+
+def _imp_theta_with_fb(ov_new, ov_old, st, ft, pop, pctx, mode):
+    log_imp, mag, _ = orc.importance(ov_new, ov_old, st, ft, pop, pctx, mode)
+    return log_imp, mag, np.imag(log_imp)
+
+
+# --------------------------------------------------------------------------
+# Gaussian-shift identity
+# --------------------------------------------------------------------------
+
+def _shift_identity_error(kind, uhf, mode, shift_fn=None, imp_fn=None):
+    sd, w, rng = _general_system(21, 5, 2, 2, kind, uhf, dt=0.02)
+    base = _integral(sd, w, np.zeros(2), pop=0.3, mode=mode, shift_fn=shift_fn, imp_fn=imp_fn)
+    errs = []
+    for _ in range(2):
+        # |f| = sqrt(dt) |i fb - mf| ~ 0.3
+        fb = 1.5 * (rng.standard_normal(2) + 1j * rng.standard_normal(2))
+        val = _integral(sd, w, fb, pop=0.3, mode=mode, shift_fn=shift_fn, imp_fn=imp_fn)
+        errs.append(abs(val - base) / abs(base))
+    return max(errs), sd
+
+
+@pytest.mark.parametrize("kind,uhf", [("restricted", False), ("unrestricted", True)])
+@pytest.mark.parametrize("mode", ["raw", "log"])
+def test_gaussian_shift_identity(kind, uhf, mode):
+    err, sd = _shift_identity_error(kind, uhf, mode)
+    assert err < 1e-12, err
+    fs = sd.pctx.sqrt_dt * 1.5 * 2.0  # the shifts tried are far from zero
+    assert fs > 0.2
+
+
+@pytest.mark.parametrize("mutant", [_shift_plus_half, _shift_xbar_in_fb_term, _shift_x_in_shift_term])
+def test_gaussian_shift_identity_rejects_planted_mistakes(mutant):
+    err, _ = _shift_identity_error("restricted", False, "log", shift_fn=mutant)
+    assert err > 1e-4, err
+
+
+# --------------------------------------------------------------------------
+# exact many-body propagator (one field, one-body part commuting with L)
+# --------------------------------------------------------------------------
+
+def _commuting_system(kind, uhf, dt, seed=5, n=5, o=2):
+    rng = np.random.default_rng(seed)
+    b = rng.standard_normal((n, n))
+    lmat = 0.45 * (b + b.T)
+    h1 = 0.7 * lmat - 0.2 * lmat @ lmat + 0.1 * np.eye(n)
+    chol = lmat[None]
+    cu = _rand_orthonormal(rng, n, o, cplx=False)
+    cd = _rand_orthonormal(rng, n, o, cplx=False) if uhf else cu
+    h0 = -0.8
+    sd = orc.make_system_data(h0, h1, chol, (cu, cd), kind, dt)
+    if kind == "restricted":
+        w = rng.standard_normal((n, o)) + 1j * rng.standard_normal((n, o))
+    else:
+        w = (rng.standard_normal((n, o)) + 1j * rng.standard_normal((n, o)),
+             rng.standard_normal((n, o)) + 1j * rng.standard_normal((n, o)))
+    return sd, w, lmat, h1, h0
+
+
+def _brute_force(sd, w, lmat, h1, h0, dt, op):
+    """<T| op(H) |w> / <T|w> by expanding both determinants (each spin) over
+    the occupations of L's eigen-orbitals; H = h0 + h1 + 1/2 sum :L L: is
+    diagonal there because h1 commutes with L."""
+    n, o = sd.trial[0].shape
+    lam, u = np.linalg.eigh(lmat)
+    hp = np.einsum("pi,pq,qi->i", u, h1, u)
+    occs = list(itertools.combinations(range(n), o))
+    wu, wd = (w, w) if sd.kind == "restricted" else w
+    cu, cd = sd.trial
+
+    def amp(c, idx):
+        return np.linalg.det(u[:, list(idx)].T @ c)
+
+    num = 0.0
+    den = 0.0
+    for i_up in occs:
+        for i_dn in occs:
+            occ_l = lam[list(i_up)].sum() + lam[list(i_dn)].sum()
+            e = (h0 + hp[list(i_up)].sum() + hp[list(i_dn)].sum() + 0.5 * occ_l ** 2
+                 - 0.5 * ((lam[list(i_up)] ** 2).sum() + (lam[list(i_dn)] ** 2).sum()))
+            a = np.conj(amp(cu, i_up) * amp(cd, i_dn)) * amp(wu, i_up) * amp(wd, i_dn)
+            num += a * op(e)
+            den += a
+    return num / den, den
+
+
+@pytest.mark.parametrize("kind,uhf", [("restricted", False), ("unrestricted", False),
+                                      ("unrestricted", True)])
+@pytest.mark.parametrize("fb_kind", ["zero", "force_bias", "random"])
+def test_importance_integral_is_exact_propagator(kind, uhf, fb_kind):
+    dt, pop = 0.02, 0.37
+    sd, w, lmat, h1, h0 = _commuting_system(kind, uhf, dt)
+    rng = np.random.default_rng(3)
+    if fb_kind == "zero":
+        fb = np.zeros(1)
+    elif fb_kind == "random":
+        fb = 2.0 * (rng.standard_normal(1) + 1j * rng.standard_normal(1))
+    else:
+        wb = _bcast(w, 1)
+        fb = orc.force_bias(wb, sd.trial, sd.mctx, sd.kind)[0]
+    val = _integral(sd, w, fb, pop=pop, n_terms=40, k=120)
+    ref, den = _brute_force(sd, w, lmat, h1, h0, dt, lambda e: math.exp(-dt * e))
+    ref = ref * math.exp(dt * pop)
+    assert abs(val - ref) / abs(ref) < 1e-11, (val, ref)
+    # the expansion's overlap is the checker's overlap (Cauchy-Binet)
+    ov = orc.overlap(_bcast(w, 1), sd.trial, sd.kind)[0]
+    assert abs(den - ov) / abs(ov) < 1e-12
+
+
+@pytest.mark.parametrize("kind,uhf", [("restricted", False), ("unrestricted", True)])
+def test_energy_is_brute_force_mixed_estimator(kind, uhf):
+    """<T|H|w>/<T|w> from the occupation expansion equals the checker's
+    Green's-function energy (meas/rhf.py:36-50,75-105)."""
+    sd, w, lmat, h1, h0 = _commuting_system(kind, uhf, 0.01, seed=8, n=6, o=2)
+    ref, _ = _brute_force(sd, w, lmat, h1, h0, 0.01, lambda e: e)
+    e = orc.energy(_bcast(w, 1), h0, sd.trial, sd.mctx, sd.kind)[0]
+    assert abs(e - ref) / abs(ref) < 1e-11
+
+
+@pytest.mark.parametrize("mutant", [_imp_shift_sign, _imp_h0_sign])
+def test_exact_propagator_rejects_planted_mistakes(mutant):
+    dt, pop = 0.02, 0.37
+    sd, w, lmat, h1, h0 = _commuting_system("restricted", False, dt)
+    fb = orc.force_bias(_bcast(w, 1), sd.trial, sd.mctx, sd.kind)[0]
+    val = _integral(sd, w, fb, pop=pop, n_terms=40, k=120, imp_fn=mutant)
+    ref, _ = _brute_force(sd, w, lmat, h1, h0, dt, lambda e: math.exp(-dt * e))
+    ref = ref * math.exp(dt * pop)
+    assert abs(val - ref) / abs(ref) > 1e-4
+
+
+# --------------------------------------------------------------------------
+# force-bias flatness
+# --------------------------------------------------------------------------
+
+def _slope(kind, uhf, dt, shift_fn=None):
+    sd, w, _ = _general_system(33, 6, 2, 3, kind, uhf, dt=dt)
+    fb = orc.force_bias(_bcast(w, 1), sd.trial, sd.mctx, sd.kind)[0]
+    g = sd.chol.shape[0]
+    h = 1e-4
+    pts = np.concatenate([np.eye(g) * h, -np.eye(g) * h]).astype(complex)
+    log_imp, _, _ = _integral(sd, w, fb, pop=0.0, fields_at=pts, shift_fn=shift_fn)
+    slope = (log_imp[:g] - log_imp[g:]) / (2 * h)
+    return np.linalg.norm(slope), np.linalg.norm(fb)
+
+
+@pytest.mark.parametrize("kind,uhf", [("restricted", False), ("unrestricted", True)])
+def test_force_bias_makes_importance_flat(kind, uhf):
+    s1, fbn = _slope(kind, uhf, 0.01)
+    s4, _ = _slope(kind, uhf, 0.0025)
+    assert 6.5 < s1 / s4 < 9.5, (s1, s4)
+    assert s1 < 0.25 * math.sqrt(0.01) * fbn
+
+
+def test_force_bias_flatness_rejects_flipped_shift():
+    s1, fbn = _slope("restricted", False, 0.01, shift_fn=_shift_flipped)
+    s4, _ = _slope("restricted", False, 0.0025, shift_fn=_shift_flipped)
+    assert s1 / s4 < 2.5
+    assert s1 > 0.5 * math.sqrt(0.01) * fbn
+
+
+# --------------------------------------------------------------------------
+# the phase depends on the shifted fields only
+# --------------------------------------------------------------------------
+
+def _phase_spread(imp_fn=None):
+    sd, w, rng = _general_system(44, 5, 2, 2, "restricted", False, dt=0.02)
+    xbar = rng.standard_normal((1, 2)) + 0.2j * rng.standard_normal((1, 2))
+    thetas, mags = [], []
+    for fb in (np.zeros(2), 1.3 * (rng.standard_normal(2) + 1j * rng.standard_normal(2))):
+        fs = -sd.pctx.sqrt_dt * (1.0j * fb - sd.pctx.mf_shifts)
+        x = xbar + fs  # fields whose shifted fields are xbar
+        _, mag, theta = _integral(sd, w, fb, pop=0.0, fields_at=x, imp_fn=imp_fn)
+        thetas.append(theta[0])
+        mags.append(mag[0])
+    return abs(thetas[0] - thetas[1]), abs(mags[0] - mags[1]) / mags[0]
+
+
+def test_phase_depends_on_shifted_fields_only():
+    dth, dmag = _phase_spread()
+    assert dth < 1e-12
+    assert dmag > 1e-3  # the magnitude does carry the Gaussian-shift factor
+
+
+def test_phase_check_rejects_planted_mistake():
+    dth, _ = _phase_spread(imp_fn=_imp_theta_with_fb)
+    assert dth > 1e-4
+
+
+# --------------------------------------------------------------------------
+# block end (prop/blocks.py:59-79): hand cases with known answers
+# --------------------------------------------------------------------------
+
+def _block_cases():
+    """(energies, weights, e_ref, pop, dt, ema) -> (e_block, w_sum, pop_new), by hand."""
+    dt = 0.01
+    t = math.sqrt(2.0 / dt)
+    e_ref = -10.0
+    return [
+        # one sample inside the window (0.99 t), two outside (+-1.01 t), one near
+        ((e_ref + 0.99 * t, e_ref + 1.01 * t, e_ref - 1.01 * t, e_ref - 0.5), (1.0, 2.0, 3.0, 4.0),
+         e_ref, 3.0, dt, 0.1,
+         (e_ref + (0.99 * t - 2.0) / 10.0, 10.0, 0.9 * 3.0 + 0.1 * (e_ref + (0.99 * t - 2.0) / 10.0))),
+        # all weights zero: the block energy is e_ref
+        ((e_ref + 1.0, e_ref - 2.0, 7.0), (0.0, 0.0, 0.0), e_ref, 3.0, dt, 0.1,
+         (e_ref, 0.0, 1.7)),
+        # plain weighted mean and EMA
+        ((1.0, 3.0), (1.0, 1.0), 2.0, 5.0, dt, 0.1, (2.0, 2.0, 4.7)),
+    ]
+
+
+@pytest.mark.parametrize("case", range(3))
+def test_block_energy_hand_cases(case):
+    e, w, e_ref, pop, dt, ema, (eb, ws, pn) = _block_cases()[case]
+    got = orc.block_energy(np.array(e), np.array(w), e_ref, pop, dt, ema)
+    assert got[0] == pytest.approx(eb, rel=1e-14, abs=1e-14)
+    assert got[1] == pytest.approx(ws, rel=1e-14, abs=0)
+    assert got[3] == pytest.approx(pn, rel=1e-14, abs=1e-14)
+
+
+def test_block_energy_rejects_planted_window():
+    """A 2/dt window (instead of sqrt(2/dt)) keeps the 1.01 t samples."""
+    e, w, e_ref, pop, dt, ema, (eb, _, _) = _block_cases()[0]
+    e = np.array(e)
+    keep = np.abs(e - e_ref) <= 2.0 / dt
+    wrong = np.sum(np.array(w) * np.where(keep, e, e_ref)) / np.sum(w)
+    assert abs(wrong - eb) > 1.0
+
+
+def test_block_energy_nan_sample_propagates():
+    got = orc.block_energy(np.array([1.0, np.nan]), np.array([1.0, 1.0]), 1.0, 0.0, 0.01, 0.1)
+    assert np.isnan(got[0]) and np.isnan(got[3])
+
+
+def test_measure_block_uses_block_energy():
+    """measure_block = QR, energy, then the block reduction above."""
+    from ad_afqmc_prototype_amd import synthetic as syn
+    s = syn.make_system(0, 8, 2, 16)
+    sd = orc.make_system_data(s.h0, s.h1, s.chol, (s.mo_up, s.mo_dn), "restricted", 0.01)
+    st = orc.init_state(sd, 6)
+    st = orc.state(st.walkers, np.array([0.0, 1.0, 2.0, 0.5, 0.0, 1.5]), st.overlaps, 1.25,
+                   st.e_estimate)
+    out, e_block, w_sum, e = orc.measure_block(st, sd, orc.params(dt=0.01))
+    e_raw = np.real(orc.energy(orc.orthonormalize(st.walkers, "restricted"), sd.h0, sd.trial,
+                               sd.mctx, "restricted"))
+    assert w_sum == 5.0
+    assert e_block == pytest.approx(np.sum(st.weights * e_raw) / 5.0, rel=1e-13)
+    assert out.pop_control_ene_shift == pytest.approx(0.9 * 1.25 + 0.1 * e_block, rel=1e-13)
+
+
+# --------------------------------------------------------------------------
+# comb with NaN weights (walkers.py:233-265 under searchsorted's NaN order)
+# --------------------------------------------------------------------------
+
+def _comb_total_order(weights, zeta):
+    """Brute-force comb; NaN compares above every number (sort order)."""
+    nw = len(weights)
+    cw, s = [], 0.0
+    for x in weights:
+        s = s + abs(x)
+        cw.append(s)
+    tot = cw[-1]
+
+    def less(a, b):
+        return a < b or (math.isnan(b) and not math.isnan(a))
+
+    out = []
+    for k in range(nw):
+        z = tot * (k + zeta) / nw
+        j = 0
+        while j < nw and less(cw[j], z):
+            j += 1
+        out.append(min(j, nw - 1))
+    return np.array(out)
+
+
+@pytest.mark.parametrize("pos", [0, 3, 7])
+def test_comb_with_nan_weight(pos):
+    rng = np.random.default_rng(pos)
+    w = rng.random(8)
+    w[pos] = np.nan
+    idx = orc.sr_indices(w, 0.41, 8)
+    np.testing.assert_array_equal(idx, _comb_total_order(w, 0.41))
+    np.testing.assert_array_equal(idx, np.full(8, pos))
+    walkers = rng.standard_normal((8, 3, 2)) + 0j
+    w_sr, weights_sr, _ = orc.stochastic_reconfiguration(walkers, w, 0.41, "restricted")
+    np.testing.assert_array_equal(w_sr, np.broadcast_to(walkers[pos], w_sr.shape))
+    assert np.all(np.isnan(weights_sr))
