@@ -58,6 +58,7 @@ _SIGNATURES = {
     "afq_block_energy": [c_int, c_vp, c_vp, c_double, c_double, c_double, c_vp, c_vp, c_vp],
     "afq_comb_indices": [c_int, c_vp, c_double, c_vp, c_vp, c_vp, c_vp],
     "afq_comb_indices_dev": [c_int, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp],
+    "afq_comb_indices_noalloc": [c_int, c_vp, c_vp, c_vp, c_vp, c_vp],
     "afq_rng_split": [c_vp, c_vp, c_vp, c_vp],
     "afq_rng_normal": [c_vp, c_ll, c_vp, c_vp],
     "afq_rng_uniform": [c_vp, c_vp, c_vp],
@@ -83,7 +84,7 @@ _SIGNATURES = {
     "afq_unpack_pairs": [c_int, c_int, c_vp, c_ll, c_vp, c_vp],
 }
 
-ABI_VERSION = 8
+ABI_VERSION = 9
 
 
 class HipLibraryError(RuntimeError):

@@ -2,6 +2,8 @@
 // indices and the row gather.
 #include <cmath>
 
+#include <mutex>
+
 #include "afqmc_hip.h"
 #include "common.h"
 
@@ -382,6 +384,73 @@ extern "C" int afq_comb_indices_dev(int nw, const double* weights, const double*
     hipLaunchKernelGGL(comb_kernel, dim3(1), dim3(1024), 0, (hipStream_t)stream, nw, weights, 0.0, zeta,
                        cumsum, idx, weights_new);
     return afq::check_launch("afq_comb_indices_dev");
+}
+
+// ---------------------------------------------------------------------------
+// Comb indices without caller scratch: the cumulative sum lives in a
+// file-scope device array (one copy per GPU, held until the process ends).
+// Its address is looked up once per device with that device current; the
+// kernel receives that address, never the array's host-side name.  Every call
+// on one GPU shares the array: calls that overlap on one GPU (different
+// streams, or graphs replayed concurrently) race on it.  Nothing in it has to
+// start at zero: the kernel writes cw[0..nw) before any lane reads it.
+// ---------------------------------------------------------------------------
+namespace {
+constexpr int kCombScratchWalkers = 1 << 18;  // 2 MiB of fp64
+constexpr int kMaxDevices = 64;
+__device__ __align__(256) double scratch[kCombScratchWalkers];
+std::mutex g_scratch_mu;
+double* g_scratch_ptr[kMaxDevices] = {};
+
+int comb_scratch(hipStream_t st, double** out) {
+    int cur = 0;
+    if (hipGetDevice(&cur) != hipSuccess) {
+        afq::set_error("afq_comb_indices_noalloc: hipGetDevice failed");
+        return -2;
+    }
+    int dev = cur;
+    if (st != nullptr && hipStreamGetDevice(st, &dev) != hipSuccess) {
+        afq::set_error("afq_comb_indices_noalloc: hipStreamGetDevice failed");
+        return -2;
+    }
+    if (dev < 0 || dev >= kMaxDevices) {
+        afq::set_error("afq_comb_indices_noalloc: device %d beyond %d", dev, kMaxDevices);
+        return -1;
+    }
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (g_scratch_ptr[dev] == nullptr) {
+        if (dev != cur && hipSetDevice(dev) != hipSuccess) {
+            afq::set_error("afq_comb_indices_noalloc: hipSetDevice(%d) failed", dev);
+            return -2;
+        }
+        void* ptr = nullptr;
+        const hipError_t err = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (dev != cur) (void)hipSetDevice(cur);
+        if (err != hipSuccess || ptr == nullptr) {
+            afq::set_error("afq_comb_indices_noalloc: hipGetSymbolAddress failed: %s",
+                           hipGetErrorString(err));
+            return -2;
+        }
+        g_scratch_ptr[dev] = (double*)ptr;
+    }
+    *out = g_scratch_ptr[dev];
+    return 0;
+}
+}  // namespace
+
+extern "C" int afq_comb_indices_noalloc(int nw, const double* weights, const double* zeta, int* idx,
+                                        double* weights_new, void* stream) {
+    AFQ_REQUIRE(nw > 0, "afq_comb_indices_noalloc: nw=%d", nw);
+    AFQ_REQUIRE(nw <= kCombScratchWalkers,
+                "afq_comb_indices_noalloc: nw=%d exceeds the library's scratch of %d walkers", nw,
+                kCombScratchWalkers);
+    AFQ_REQUIRE(weights && zeta && idx && weights_new, "afq_comb_indices_noalloc: null pointer");
+    double* cw = nullptr;
+    const int rc = comb_scratch((hipStream_t)stream, &cw);
+    if (rc != 0) return rc;
+    hipLaunchKernelGGL(comb_kernel, dim3(1), dim3(1024), 0, (hipStream_t)stream, nw, weights, 0.0, zeta,
+                       cw, idx, weights_new);
+    return afq::check_launch("afq_comb_indices_noalloc");
 }
 
 extern "C" int afq_gather_rows(int nrows, long long row_len, int nplanes, long long src_plane_stride,

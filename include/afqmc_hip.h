@@ -321,6 +321,19 @@ int afq_comb_indices_dev(int nw, const double* weights, const double* zeta, doub
                          int* idx, double* weights_new, void* stream);
 
 /*
+ * afq_comb_indices_dev without the cumsum scratch argument: it allocates
+ * nothing and takes no workspace.  No counterpart in the reference: it is for
+ * callers that cannot allocate (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc).  The cumulative sum goes to a
+ * static device array of the library (2^18 fp64 entries, 2 MiB per GPU, held
+ * until the process ends); nw above 2^18 is rejected (-1).  Every call on one
+ * GPU shares that array: calls that overlap on one GPU race.  Same results as
+ * afq_comb_indices_dev.
+ */
+int afq_comb_indices_noalloc(int nw, const double* weights, const double* zeta, int* idx,
+                             double* weights_new, void* stream);
+
+/*
  * The step's random stream on the device (prop/afqmc.py:32,86-88,
  * prop/blocks.py:81-82: key, subkey = split(key); normal(subkey, shape) /
  * uniform(subkey)), in jax's original Threefry-2x32 scheme (config.py:27).

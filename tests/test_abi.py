@@ -77,6 +77,9 @@ def test_library_loads_and_reports_abi():
     ("afq_phaseless_update", (4, 1, 1, 1, 1, 1, 1, 1, 0.0, 0.0, 0.0, -1.0, 0.1, 1e-3, 100.0, 0.1,
                               1, 1, 1, 0), "dt must be positive"),
     ("afq_comb_indices", (0, 1, 0.5, 1, 1, 1, 0), "nw=0"),
+    ("afq_comb_indices_noalloc", (0, 1, 1, 1, 1, 0), "nw=0"),
+    ("afq_comb_indices_noalloc", ((1 << 18) + 1, 1, 1, 1, 1, 0), "exceeds"),
+    ("afq_comb_indices_noalloc", (4, 1, 0, 1, 1, 0), "null"),
     ("afq_gather_rows", (4, 0, 1, 0, 0, 1, 1, 2, 0), "bad sizes"),
 ])
 def test_invalid_arguments_rejected_on_host(name, args, msg):
