@@ -235,7 +235,9 @@ def make_vhs(chol: np.ndarray, x: np.ndarray) -> np.ndarray:
     """_make_vhs_split_flat (prop/chol_afqmc_ops.py:70-74), batched: (nw, n, n)."""
     g, n = chol.shape[0], chol.shape[1]
     flat = chol.reshape(g, n * n)
-    v = (x.real @ flat) + 1j * (x.imag @ flat)
+    # contiguous copies of the real / imaginary planes: a strided operand
+    # sends numpy's matmul down its slow non-BLAS loop
+    v = (np.ascontiguousarray(x.real) @ flat) + 1j * (np.ascontiguousarray(x.imag) @ flat)
     return v.reshape(x.shape[0], n, n)
 
 
