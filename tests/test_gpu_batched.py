@@ -40,7 +40,7 @@ def _members(device, systems=SYSTEMS, scale=0.25):
     return out
 
 
-def _alone(member, n_steps):
+def _alone(member, n_steps, qr=QR, sr=SR):
     """The system run by itself through the single-system step and block functions."""
     from ad_afqmc_prototype_amd.core.system import system
     from ad_afqmc_prototype_amd.meas.rhf import make_rhf_meas_ops
@@ -64,13 +64,13 @@ def _alone(member, n_steps):
     for t in range(1, n_steps + 1):
         st = afqmc_step(st, params=params, ham_data=ham, trial_data=trial, meas_ops=m_ops,
                         prop_ops=p_ops, prop_ctx=p_ctx, meas_ctx=m_ctx)
-        if t % SR == 0:
+        if t % sr == 0:
             st = orthonormalize_state(st, sys=sys_, trial_data=trial, meas_ops_=m_ops)
             st, sc, _ = measure(st, params=params, ham_data=ham, trial_data=trial, meas_ops=m_ops,
                                 meas_ctx=m_ctx)
             blocks.append((float(sc["energy"]), float(sc["weight"])))
             st, _ = reconfigure(st, sys=sys_, trial_data=trial, meas_ops_=m_ops)
-        elif t % QR == 0:
+        elif t % qr == 0:
             st = orthonormalize_state(st, sys=sys_, trial_data=trial, meas_ops_=m_ops)
     return st, blocks
 
@@ -249,3 +249,114 @@ def test_batch_bf16x3_close_to_fp32(gpu_device):
     ea, eb = a.block_stats.cpu().numpy(), b.block_stats.cpu().numpy()
     assert np.isfinite(ea).all()
     np.testing.assert_allclose(ea[:, 0], eb[:, 0], rtol=1e-5, atol=0)
+
+
+# --------------------------------------------------------------------------
+# the 64-system workload: seeds 100..163, N ~ U[40,160], nocc ~ U[6,24],
+# naux = round(N U[4,6]), walkers ~ U[64,512]; dt 0.005, QR every 10,
+# measurement + comb every 20, Cholesky scale 0.05
+# --------------------------------------------------------------------------
+
+def _cfg4_members(device):
+    specs = syn.uneven_batch_specs(64)
+    return specs, _members(device, [(sp["seed"], sp["norb"], sp["nocc"], sp["naux"], sp["n_walkers"])
+                                    for sp in specs], scale=0.05)
+
+
+def _checker_block(sp, scale=0.05, dt=0.005, qr=10, sr=20):
+    """One block of system ``sp`` on the CPU checker, with the fields and the
+    comb uniform of the system's own Threefry stream (split per step, normals;
+    split, uniform at the block end)."""
+    from oracle import afqmc_oracle as orc
+    from oracle import threefry as tf
+    s = syn.make_system(sp["seed"], sp["norb"], sp["nocc"], sp["naux"], chol_scale=scale)
+    sd = orc.make_system_data(s.h0, s.h1, s.chol, (s.mo_up, s.mo_dn), "restricted", dt)
+    key = tf.prng_key(sp["seed"])
+    fields = []
+    for _ in range(sr):
+        key, sub = tf.split(key)
+        fields.append(tf.normal(sub, (sp["n_walkers"], sp["naux"])))
+    key, sub = tf.split(key)
+    zeta = float(tf.uniform(sub))
+    st0 = orc.init_state(sd, sp["n_walkers"], overlap_mode="log")
+    st, blocks = orc.run_cadenced(st0, sd, orc.params(dt=dt), n_steps=sr, qr_every=qr, sr_every=sr,
+                                  fields_fn=lambda t: fields[t], zeta_fn=lambda k: zeta,
+                                  overlap_mode="log")
+    return st, blocks[0]
+
+
+def test_cfg4_batch_block(gpu_device):
+    """One block of the 64-system workload in fp32: every system bit for bit
+    equal to its single-system fp32 run; the smallest, the largest-n and the
+    largest-nocc system against the fp64 checker (block energy within 0.1 mHa,
+    weight sum to 1e-5, walker projectors to 1e-4); the bf16x3 batch's block
+    energies within 1e-5 relative of the fp32 batch's."""
+    import torch
+    from ad_afqmc_prototype_amd.prop.batched import system_batch
+    specs, members = _cfg4_members(gpu_device)
+    print(f"64-system fp32 batch, {sum(sp['n_walkers'] for sp in specs)} walkers", flush=True)
+    batch = system_batch(members, dt=0.005, qr_every=10, sr_every=20, precision="fp32")
+    batch.advance(20)
+    stats = batch.block_stats.cpu().numpy().copy()
+    assert np.isfinite(stats).all()
+    for i, m in enumerate(members):
+        st, blocks = _alone(m, 20, qr=10, sr=20)
+        _compare_system(batch, i, st, blocks, [(float(stats[i, 0]), float(stats[i, 1]))])
+    print("64 systems equal their single-system runs", flush=True)
+    picks = {min(range(64), key=lambda i: specs[i]["norb"] * specs[i]["n_walkers"]),
+             max(range(64), key=lambda i: specs[i]["norb"]),
+             max(range(64), key=lambda i: specs[i]["nocc"])}
+    for i in sorted(picks):
+        st_ref, blk = _checker_block(specs[i])
+        assert abs(stats[i, 0] - blk["energy"]) < 1e-4, (i, stats[i, 0], blk["energy"])
+        assert abs(stats[i, 1] - blk["weight"]) / blk["weight"] < 1e-5, i
+        w = batch.system_state(i)["walkers"].cpu().numpy()
+        proj = w @ np.conj(np.swapaxes(w, 1, 2))
+        proj_ref = st_ref.walkers @ np.conj(np.swapaxes(st_ref.walkers, 1, 2))
+        err = np.linalg.norm(proj - proj_ref) / np.linalg.norm(proj_ref)
+        assert err < 1e-4, (i, err)
+        print(f"system {i} (seed {specs[i]['seed']}, n={specs[i]['norb']}, o={specs[i]['nocc']}): "
+              f"block energy {stats[i, 0]:.10f} vs checker {blk['energy']:.10f}", flush=True)
+    del batch
+    torch.cuda.empty_cache()
+    b16 = system_batch(members, dt=0.005, qr_every=10, sr_every=20, precision="bf16x3")
+    b16.advance(20)
+    s16 = b16.block_stats.cpu().numpy()
+    np.testing.assert_allclose(s16[:, 0], stats[:, 0], rtol=1e-5, atol=0)
+
+
+def test_grouped_bf16x3_trotter_buckets_match_single(gpu_device):
+    """One grouped bf16x3 Trotter launch per bucket (n = 40, 160, 176, 208:
+    no, one and two V k-blocks held in LDS) equals afq_trotter_bf16x3 on each
+    system alone, bit for bit, with random complex-symmetric V rows."""
+    import torch
+    from ad_afqmc_prototype_amd import kernels
+    from ad_afqmc_prototype_amd.prop.batched import system_batch
+    systems = [(111, 40, 6, 170, 24), (112, 160, 24, 700, 20), (113, 176, 17, 800, 12),
+               (114, 208, 32, 900, 10)]
+    members = _members(gpu_device, systems, scale=0.05)
+    batch = system_batch(members, dt=DT, qr_every=QR, sr_every=SR, precision="bf16x3")
+    g = torch.Generator(device="cpu").manual_seed(9)
+    for i, m in enumerate(batch.members):
+        nw, n, ldv = m["nw"], m["n"], m["ldv"]
+        a = torch.randn((2, nw, n, n), generator=g, dtype=torch.float64)
+        a = 0.5 * (a + a.transpose(2, 3))
+        a *= 0.2 / (DT ** 0.5 * n ** 0.5)
+        v = torch.zeros((2, nw, n, ldv), dtype=torch.float32)
+        v[..., :n] = a.float()
+        batch.vhs.part(i, (2, nw, n, ldv)).copy_(v.to(gpu_device))
+    st = kernels._stream(batch.desc[0])
+    assert len(batch.trotter_buckets) == 4
+    for bk in batch.trotter_buckets:
+        t = bk["tables"][0]
+        kernels._call("afq_trotter_bf16x3_grouped", t, bk["count"], t.data_ptr(), bk["blocks"], bk["max_n"],
+                      6, st)
+    torch.cuda.synchronize()
+    for i, m in enumerate(batch.members):
+        nw, n, o, ldv = m["nw"], m["n"], m["o"], m["ldv"]
+        win = batch.walkers[0].part(i, (nw, n, o))
+        ref = kernels.trotter(win, batch.vhs.part(i, (2, nw, n, ldv)), batch.e_half.part(i, (n, ldv)),
+                              batch.sqrt_dt, 6, arith="bf16x3")
+        got = batch.walkers[1].part(i, (nw, n, o))
+        assert torch.equal(got, ref), f"system {i} (n={n}): grouped bf16x3 Trotter differs"
+        assert float((ref - win).abs().max()) > 1e-3  # the step moved the walkers
