@@ -122,7 +122,7 @@ def warm_block_end(ctx, state):
     del warm
 
 
-def cpu_baseline(workload: str, budget_s: float = 20.0, single_thread: bool = False, nw: int = 32):
+def cpu_baseline(workload: str, single_thread: bool = False, nw: int = 128):
     """The CPU checker (numpy fp64, op for op the reference algorithm) on a
     subsample of the same workload's walkers, timed on this host with all of
     its BLAS threads, or with one (the reference pins its own CPU runs to one
@@ -135,10 +135,15 @@ def cpu_baseline(workload: str, budget_s: float = 20.0, single_thread: bool = Fa
     with ctx:
         threads = max([i.get("num_threads", 1) for i in threadpool_info()
                        if i.get("user_api") == "blas"] or [1])
-        return _cpu_baseline_run(workload, budget_s, threads, nw)
+        return _cpu_baseline_run(workload, threads, nw)
 
 
-def _cpu_baseline_run(workload: str, budget_s: float, threads: int, nw: int):
+def _cpu_baseline_run(workload: str, threads: int, nw: int):
+    """One block of the workload's cadence (sr_every steps, a QR every
+    qr_every, then measurement and comb) on ``nw`` walkers: the work the GPU's
+    timed region does per walker-step, at the same amortisation."""
+    import os
+
     import numpy as np
 
     from ad_afqmc_prototype_amd import synthetic as syn
@@ -150,21 +155,20 @@ def _cpu_baseline_run(workload: str, budget_s: float, threads: int, nw: int):
     kind = cfg["walker_kind"]
     sd = orc.make_system_data(s.h0, s.h1, s.chol, (s.mo_up, s.mo_dn), kind, cfg["dt"])
     prm = orc.params(dt=cfg["dt"])
-    rdm = orc.get_rdm1(sd.trial)
-    w = orc.init_walkers(rdm, s.nocc, nw, kind)
-    st = orc.state(w, np.ones(nw), orc.overlap(w, sd.trial, kind, "log"), 0.0, 0.0)
-    st = orc.afqmc_step(st, syn.fields_for_step(cfg["seed"], 0, nw, cfg["naux"]), sd, prm, "log")
-    steps = 0
+    st = orc.init_state(sd, nw, overlap_mode="log")
+    # one untimed step (first-touch of the BLAS buffers)
+    orc.afqmc_step(st, syn.fields_for_step(cfg["seed"], 0, nw, cfg["naux"]), sd, prm, "log")
+    n_steps = cfg["sr_every"]
     t0 = time.perf_counter()
-    while time.perf_counter() - t0 < budget_s and steps < 40:
-        st = orc.afqmc_step(st, syn.fields_for_step(cfg["seed"], steps + 1, nw, cfg["naux"]), sd,
-                            prm, "log")
-        steps += 1
+    orc.run_cadenced(st, sd, prm, n_steps=n_steps, qr_every=cfg["qr_every"], sr_every=cfg["sr_every"],
+                     fields_fn=lambda t: syn.fields_for_step(cfg["seed"], t + 1, nw, cfg["naux"]),
+                     zeta_fn=lambda k: syn.zeta_for_event(cfg["seed"], k), overlap_mode="log")
     el = time.perf_counter() - t0
-    return {"value": nw * steps / el, "unit": "walker-steps/s", "cores": int(threads),
-            "kind": "port",
-            "sample": f"{workload} shapes, {nw} walkers x {steps} afqmc_step (numpy fp64 checker, "
-                      f"log-overlaps, {el:.1f} s, {threads} BLAS threads)"}
+    return {"value": nw * n_steps / el, "unit": "walker-steps/s", "cores": int(threads),
+            "host_cpus": os.cpu_count(), "kind": "port",
+            "sample": f"{workload} shapes, {nw} walkers x {n_steps} afqmc_step with a QR every "
+                      f"{cfg['qr_every']} and measurement + comb after {cfg['sr_every']} (numpy fp64 "
+                      f"checker, log-overlaps, {el:.1f} s, {threads} BLAS threads)"}
 
 
 def accuracy_check(workload: str, precision: str, device, chol_scale: float, n_blocks: int):
@@ -526,6 +530,8 @@ def main():
     ap.add_argument("--no-fp64-compare", action="store_true",
                     help="skip the fp64 run that the mixed throughput is compared with")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-fp32", action="store_true",
+                    help="skip the fp32 run (fp32 contractions and Trotter) of the bf16x3 bench")
     ap.add_argument("--accuracy-scale", type=float, default=None,
                     help="Cholesky scale of the common-random-number energy check (default: the "
                          "workload's own; 0 skips it)")
@@ -620,6 +626,20 @@ def main():
         out["speedup_vs_fp64"] = out["value"] / out["fp64_value"]
         out["fp64_kernels"] = kernel_table(kt64, n, o, g, nw, nspin, not cfg["uhf"], "fp64")
         out["block_energy_abs_diff_mHa"] = [1e3 * abs(a - b) for a, b in zip(e_block, e64)]
+    if args.precision == "bf16x3" and not args.no_fp32:
+        # the workload's other mixed precision: fp32 contractions and Trotter
+        el32, kt32, e32, _, el32_graph = timed_run(args, "fp32", device)
+        out["fp32_value"] = nw * args.steps / el32
+        out["fp32_ms_per_step"] = 1e3 * el32 / args.steps
+        out["fp32_ms_per_step_graph"] = None if el32_graph is None else 1e3 * el32_graph / args.steps
+        out["fp32_speedup_vs_fp64"] = (out["fp32_value"] / out["fp64_value"]) if "fp64_value" in out else None
+        out["fp32_kernels"] = kernel_table(kt32, n, o, g, nw, nspin, not cfg["uhf"], "fp32")
+        if "fp64_value" in out:
+            out["fp32_block_energy_abs_diff_mHa"] = [1e3 * abs(a - b) for a, b in zip(e32, e64)]
+        if args.accuracy_scale != 0:
+            scale = cfg["chol_scale"] if args.accuracy_scale is None else args.accuracy_scale
+            out["fp32_energy_check"] = accuracy_check(args.workload, "fp32", device, scale,
+                                                      args.accuracy_blocks)
     if args.fp64_production not in ("", "none"):
         import copy
         pargs = copy.copy(args)
@@ -661,7 +681,7 @@ def main():
         out["adjoint"] = adjoint_run_bench(device)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(args.workload)
-        out["cpu_baseline_single_thread"] = cpu_baseline(args.workload, budget_s=12.0, single_thread=True, nw=8)
+        out["cpu_baseline_single_thread"] = cpu_baseline(args.workload, single_thread=True, nw=48)
     print(json.dumps(out), flush=True)
 
 
