@@ -49,6 +49,10 @@ class meas_ops:
     # (2, nw, naux): the same values as the force_bias kernel without the
     # complex tensor in between (the step consumes planes)
     force_bias_planes: Optional[Callable[..., Any]] = None
+    # (walkers, trial_data) -> greens_cache of the step's new walkers when the
+    # step needs less than ``greens`` (the bf16x3 step: the force bias's bf16
+    # planes and log det, no fp64 rows); None: the step uses ``greens``
+    step_greens: Optional[Callable[[Any, Any], Any]] = None
 
     def has_kernel(self, name: str) -> bool:
         return name in self.kernels

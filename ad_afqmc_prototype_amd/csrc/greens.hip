@@ -429,6 +429,383 @@ __global__ __launch_bounds__(THREADS) void greens_wave_kernel(
                                   gre + (long long)nw * ldg, logdet + (long long)w * nspin + s, L[wave], sp);
 }
 
+// ---------------------------------------------------------------------------
+// Split-bf16 Green's function of the bf16x3 step, o <= 16 NT: the rows of G
+// only ever feed the split-bf16 force-bias GEMM there, so they are formed as a
+// product on the bf16 matrix cores and written straight as its bf16 planes.
+//   1. m = C^T w on v_mfma_f64_16x16x4 (fp64, as greens_wave_body);
+//   2. Gauss-Jordan inversion with partial pivoting (|re| + |im|, first index
+//      on ties: the LU's pivots) in the m accumulator registers, lane
+//      (lr, q) holding a[s][t] = m[4 s + q][16 t + lr]; the pivots give
+//      log det m in fp64 exactly as the LU does;
+//   3. A = (P m)^-1 split hi / lo into LDS as A^T, then G[perm[j]][p] =
+//      sum_k A[k][j] w[p][k] on v_mfma_f32_16x16x32_bf16 (al.bh + ah.bl +
+//      ah.bh per real product), each 16-row tile of w split on load.
+// One wave per (walker, spin), four per workgroup; broadcasts between lanes
+// go through the wave's own LDS region (no workgroup barriers).
+// ---------------------------------------------------------------------------
+typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
+
+template <int NT>
+struct gj_lds {
+    static constexpr int O = 16 * NT;
+    static constexpr int AP = O + 8;  // bf16 pitch of one row of A^T (16-byte multiple)
+    double2 col[O];
+    double2 rowk[O];
+    double2 rowp[O];
+    int perm[O];
+    uint16_t at[4][O][AP];  // A^T planes: re hi, re lo, im hi, im lo
+};
+
+__device__ __forceinline__ bf16x8_t neg_bf16x8(bf16x8_t x) {
+    uint4 u = __builtin_bit_cast(uint4, x);
+    u.x ^= 0x80008000u;
+    u.y ^= 0x80008000u;
+    u.z ^= 0x80008000u;
+    u.w ^= 0x80008000u;
+    return __builtin_bit_cast(bf16x8_t, u);
+}
+
+// lane src's value of x (src uniform)
+__device__ __forceinline__ double readlane_f64(double x, int src) {
+    const unsigned long long u = __builtin_bit_cast(unsigned long long, x);
+    const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)u, src);
+    const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(u >> 32), src);
+    return __builtin_bit_cast(double, ((unsigned long long)hi << 32) | lo);
+}
+
+__device__ __forceinline__ f32x4 mfma_bf16x32(bf16x8_t a, bf16x8_t b, f32x4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
+}
+
+template <int NT>
+__device__ __forceinline__ void greens_gj_split_body(int n, int o, const double2* __restrict__ W,
+                                                     const double* __restrict__ Cm, double2* __restrict__ ld,
+                                                     gj_lds<NT>& S, split_rows sp) {
+    constexpr int O = 16 * NT;
+    constexpr int NS = 4 * NT;  // row slots per lane
+    const int lane = threadIdx.x & 63;
+    const int lr = lane & 15;
+    const int q = lane >> 4;
+    const int lk = q * 4;
+
+    // ---- 1. m = C^T w (fp64 MFMA, next k-block's operands loaded ahead) ----
+    f64x4 acc[NT][NT][2];
+    {
+        const int nkb = (n + 15) >> 4;
+#pragma unroll
+        for (int ti = 0; ti < NT; ++ti)
+#pragma unroll
+            for (int tj = 0; tj < NT; ++tj) acc[ti][tj][0] = acc[ti][tj][1] = f64x4{0.0, 0.0, 0.0, 0.0};
+        double a[NT][4], an[NT][4];
+        double2 x[NT][4], xn[NT][4];
+        auto load = [&](int kb, double (&av)[NT][4], double2 (&xv)[NT][4]) {
+#pragma unroll
+            for (int kk = 0; kk < 4; ++kk) {
+                const int p = kb * 16 + lk + kk;
+                const bool okp = p < n;
+#pragma unroll
+                for (int t = 0; t < NT; ++t) {
+                    const int c = t * 16 + lr;
+                    av[t][kk] = (okp && c < o) ? Cm[p * o + c] : 0.0;
+                    xv[t][kk] = (okp && c < o) ? W[p * o + c] : make_double2(0.0, 0.0);
+                }
+            }
+        };
+        load(0, a, x);
+        for (int kb = 0; kb < nkb; ++kb) {
+            load(kb + 1, an, xn);
+#pragma unroll
+            for (int kk = 0; kk < 4; ++kk)
+#pragma unroll
+                for (int ti = 0; ti < NT; ++ti)
+#pragma unroll
+                    for (int tj = 0; tj < NT; ++tj) {
+                        acc[ti][tj][0] = afq::mfma_f64(a[ti][kk], x[tj][kk].x, acc[ti][tj][0]);
+                        acc[ti][tj][1] = afq::mfma_f64(a[ti][kk], x[tj][kk].y, acc[ti][tj][1]);
+                    }
+#pragma unroll
+            for (int t = 0; t < NT; ++t)
+#pragma unroll
+                for (int kk = 0; kk < 4; ++kk) {
+                    a[t][kk] = an[t][kk];
+                    x[t][kk] = xn[t][kk];
+                }
+        }
+    }
+    // accumulator element r of tile (ti, tj): m[16 ti + 4 r + q][16 tj + lr]
+    double2 m[NS][NT];
+#pragma unroll
+    for (int ti = 0; ti < NT; ++ti)
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+#pragma unroll
+            for (int tj = 0; tj < NT; ++tj) m[4 * ti + r][tj] = make_double2(acc[ti][tj][0][r], acc[ti][tj][1][r]);
+    if (lane < O) S.perm[lane] = lane;
+
+    // ---- 2. Gauss-Jordan with partial pivoting ----
+    double lre = 0.0, lim = 0.0;
+    int nswap = 0;
+    // k = 4 ks + kq: the row slot ks (and the column tile kt) are compile-time
+    // constants of the unrolled outer loop, so every register index is static
+#pragma unroll
+    for (int ks = 0; ks < NS; ++ks) {
+        const int kt = ks >> 2;
+#pragma unroll 1
+        for (int kq = 0; kq < 4; ++kq) {
+            const int k = 4 * ks + kq;
+            if (k >= o) break;
+            const int kl = k & 15;
+            // pivot: rows k..o-1 of column k (lanes lr == kl hold it)
+            double best = -1.0;
+            int bi = k;
+#pragma unroll
+            for (int s2 = 0; s2 < NS; ++s2) {
+                const int r = 4 * s2 + q;
+                const double v = fabs(m[s2][kt].x) + fabs(m[s2][kt].y);
+                if (r >= k && r < o && v > best) {
+                    best = v;
+                    bi = r;
+                }
+            }
+#pragma unroll
+            for (int off = 16; off < 64; off <<= 1) {
+                const double ob = __shfl_xor(best, off, 64);
+                const int oi = __shfl_xor(bi, off, 64);
+                if (ob > best || (ob == best && oi < bi)) {
+                    best = ob;
+                    bi = oi;
+                }
+            }
+            const int piv = __builtin_amdgcn_readlane(bi, kl);
+            if (piv != k) {
+                const int pq = piv & 3, ps = piv >> 2;
+                // rows k and piv through LDS
+                if (q == kq) {
+#pragma unroll
+                    for (int tj = 0; tj < NT; ++tj) S.rowk[tj * 16 + lr] = m[ks][tj];
+                }
+                if (q == pq) {
+#pragma unroll
+                    for (int tj = 0; tj < NT; ++tj) {
+                        double2 v = m[0][tj];
+#pragma unroll
+                        for (int s2 = 1; s2 < NS; ++s2)
+                            if (s2 == ps) v = m[s2][tj];
+                        S.rowp[tj * 16 + lr] = v;
+                    }
+                }
+                if (lane == 0) {
+                    const int t = S.perm[k];
+                    S.perm[k] = S.perm[piv];
+                    S.perm[piv] = t;
+                }
+                wave_lds_sync();
+                if (q == kq) {
+#pragma unroll
+                    for (int tj = 0; tj < NT; ++tj) m[ks][tj] = S.rowp[tj * 16 + lr];
+                }
+                if (q == pq) {
+#pragma unroll
+                    for (int tj = 0; tj < NT; ++tj) {
+                        const double2 v = S.rowk[tj * 16 + lr];
+#pragma unroll
+                        for (int s2 = 0; s2 < NS; ++s2)
+                            if (s2 == ps) m[s2][tj] = v;
+                    }
+                }
+                ++nswap;
+                wave_lds_sync();
+            }
+            // pivot value (uniform): lane (kl, kq), slot (ks, kt)
+            const int src = kl + 16 * kq;
+            const double2 pv = make_double2(readlane_f64(m[ks][kt].x, src), readlane_f64(m[ks][kt].y, src));
+            lre += log(hypot(pv.x, pv.y));
+            lim += atan2(pv.y, pv.x);
+            const double2 inv = crecip(pv);
+            // scale row k (A[k][k] <- 1 / p) and publish it and column k
+            if (q == kq) {
+#pragma unroll
+                for (int tj = 0; tj < NT; ++tj) {
+                    const double2 v = (tj * 16 + lr == k) ? inv : cmul(m[ks][tj], inv);
+                    m[ks][tj] = v;
+                    S.rowk[tj * 16 + lr] = v;
+                }
+            }
+            if (lr == kl) {
+#pragma unroll
+                for (int s2 = 0; s2 < NS; ++s2) S.col[4 * s2 + q] = m[s2][kt];
+            }
+            wave_lds_sync();
+            double2 rk[NT];
+#pragma unroll
+            for (int tj = 0; tj < NT; ++tj) rk[tj] = S.rowk[tj * 16 + lr];
+#pragma unroll
+            for (int s2 = 0; s2 < NS; ++s2) {
+                const int r = 4 * s2 + q;
+                if (r != k) {
+                    const double2 f = S.col[r];
+#pragma unroll
+                    for (int tj = 0; tj < NT; ++tj) {
+                        double2 y = (tj * 16 + lr == k) ? make_double2(0.0, 0.0) : m[s2][tj];
+                        csub_mul(y, f, rk[tj]);
+                        m[s2][tj] = y;
+                    }
+                }
+            }
+            wave_lds_sync();
+        }
+    }
+    if (nswap & 1) lim += M_PI;
+    if (lane == 0) *ld = make_double2(lre, lim);
+
+    // ---- 3. A^T (split) into LDS, then G rows = A^T w^T on bf16 MFMA ----
+#pragma unroll
+    for (int s2 = 0; s2 < NS; ++s2)
+#pragma unroll
+        for (int tj = 0; tj < NT; ++tj) {
+            const int j = tj * 16 + lr, kk = 4 * s2 + q;
+            uint16_t h, l;
+            afq::split_bf16(m[s2][tj].x, h, l);
+            S.at[0][j][kk] = h;
+            S.at[1][j][kk] = l;
+            afq::split_bf16(m[s2][tj].y, h, l);
+            S.at[2][j][kk] = h;
+            S.at[3][j][kk] = l;
+        }
+    wave_lds_sync();
+    // A fragments of output tile tjo, k-block kb (32 wide): row j = 16 tjo + lr,
+    // k = 32 kb + 8 q .. + 7
+    constexpr int NKB = (O + 31) / 32;
+    bf16x8_t Arh[NT][NKB], Arl[NT][NKB], Aih[NT][NKB], Ail[NT][NKB];
+#pragma unroll
+    for (int t = 0; t < NT; ++t)
+#pragma unroll
+        for (int kb = 0; kb < NKB; ++kb) {
+            const int kk = kb * 32 + 8 * q;
+            if (kk < O) {
+                Arh[t][kb] = *reinterpret_cast<const bf16x8_t*>(&S.at[0][t * 16 + lr][kk]);
+                Arl[t][kb] = *reinterpret_cast<const bf16x8_t*>(&S.at[1][t * 16 + lr][kk]);
+                Aih[t][kb] = *reinterpret_cast<const bf16x8_t*>(&S.at[2][t * 16 + lr][kk]);
+                Ail[t][kb] = *reinterpret_cast<const bf16x8_t*>(&S.at[3][t * 16 + lr][kk]);
+            } else {
+                Arh[t][kb] = Arl[t][kb] = Aih[t][kb] = Ail[t][kb] = bf16x8_t{};
+            }
+        }
+    int prow[NT][4];
+#pragma unroll
+    for (int t = 0; t < NT; ++t)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int j = t * 16 + 4 * q + r;
+            prow[t][r] = j < o ? S.perm[j] : -1;
+        }
+    // w rows p = 16 pt + lr, columns 8 q .. 8 q + 7 of each k-block, as fp32
+    auto load_w = [&](int pt, float (&re)[NKB][8], float (&im)[NKB][8]) {
+        const int p = pt * 16 + lr;
+#pragma unroll
+        for (int kb = 0; kb < NKB; ++kb)
+#pragma unroll
+            for (int e = 0; e < 8; ++e) {
+                const int kk = kb * 32 + 8 * q + e;
+                const double2 v = (p < n && kk < o) ? W[p * o + kk] : make_double2(0.0, 0.0);
+                re[kb][e] = (float)v.x;
+                im[kb][e] = (float)v.y;
+            }
+    };
+    auto split8 = [&](const float (&x)[8], bf16x8_t& h, bf16x8_t& l) {
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+            const __bf16 hh = (__bf16)x[e];
+            h[e] = hh;
+            l[e] = (__bf16)(x[e] - (float)hh);
+        }
+    };
+    const int npt = (n + 15) >> 4;
+    float wre[NKB][8], wim[NKB][8], nre[NKB][8], nim[NKB][8];
+    load_w(0, wre, wim);
+    for (int pt = 0; pt < npt; ++pt) {
+        load_w(pt + 1, nre, nim);  // past the end: all zeros, nothing read
+        bf16x8_t brh[NKB], brl[NKB], bih[NKB], bil[NKB];
+#pragma unroll
+        for (int kb = 0; kb < NKB; ++kb) {
+            split8(wre[kb], brh[kb], brl[kb]);
+            split8(wim[kb], bih[kb], bil[kb]);
+        }
+        const int p = pt * 16 + lr;
+#pragma unroll
+        for (int t = 0; t < NT; ++t) {
+            f32x4 cr = {0.f, 0.f, 0.f, 0.f}, ci = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+            for (int kb = 0; kb < NKB; ++kb) {
+                // re = Ar Wr - Ai Wi, im = Ar Wi + Ai Wr
+                cr = mfma_bf16x32(Arl[t][kb], brh[kb], cr);
+                cr = mfma_bf16x32(Arh[t][kb], brl[kb], cr);
+                cr = mfma_bf16x32(Arh[t][kb], brh[kb], cr);
+                ci = mfma_bf16x32(Arl[t][kb], bih[kb], ci);
+                ci = mfma_bf16x32(Arh[t][kb], bil[kb], ci);
+                ci = mfma_bf16x32(Arh[t][kb], bih[kb], ci);
+                const bf16x8_t nih = neg_bf16x8(Aih[t][kb]), nil = neg_bf16x8(Ail[t][kb]);
+                cr = mfma_bf16x32(nil, bih[kb], cr);
+                cr = mfma_bf16x32(nih, bil[kb], cr);
+                cr = mfma_bf16x32(nih, bih[kb], cr);
+                ci = mfma_bf16x32(Ail[t][kb], brh[kb], ci);
+                ci = mfma_bf16x32(Aih[t][kb], brl[kb], ci);
+                ci = mfma_bf16x32(Aih[t][kb], brh[kb], ci);
+            }
+            if (p < n) {
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const int i = prow[t][r];
+                    if (i >= 0) {
+                        const long long e = (long long)i * n + p;
+                        const __bf16 hr = (__bf16)cr[r];
+                        const __bf16 hi_ = (__bf16)ci[r];
+                        sp.hre[e] = __builtin_bit_cast(uint16_t, hr);
+                        sp.lre[e] = __builtin_bit_cast(uint16_t, (__bf16)(cr[r] - (float)hr));
+                        sp.him[e] = __builtin_bit_cast(uint16_t, hi_);
+                        sp.lim[e] = __builtin_bit_cast(uint16_t, (__bf16)(ci[r] - (float)hi_));
+                    }
+                }
+            }
+        }
+#pragma unroll
+        for (int kb = 0; kb < NKB; ++kb)
+#pragma unroll
+            for (int e = 0; e < 8; ++e) {
+                wre[kb][e] = nre[kb][e];
+                wim[kb][e] = nim[kb][e];
+            }
+    }
+}
+
+template <int NT>
+__global__ __launch_bounds__(THREADS) void greens_gj_split_kernel(
+    int nw, int n, int o, int nspin, const double2* __restrict__ wup, const double2* __restrict__ wdn,
+    const double* __restrict__ cup, const double* __restrict__ cdn, double2* __restrict__ logdet,
+    uint16_t* __restrict__ ghi, uint16_t* __restrict__ glo, long long ldk) {
+    __shared__ gj_lds<NT> L[THREADS / 64];
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int item = blockIdx.x * (THREADS / 64) + wave;
+    if (item >= nw * nspin) return;
+    const int w = item / nspin;
+    const int s = item - w * nspin;
+    const long long ldg = (long long)nspin * o * n;
+    const long long rre = (long long)w * ldk + (long long)s * o * n;
+    const long long rim = rre + (long long)nw * ldk;
+    const int lane = threadIdx.x & 63;
+    if (s == nspin - 1 && lane < ldk - ldg) {
+        const long long c = (long long)w * ldk + ldg + lane;
+        ghi[c] = 0;
+        glo[c] = 0;
+        ghi[c + (long long)nw * ldk] = 0;
+        glo[c + (long long)nw * ldk] = 0;
+    }
+    greens_gj_split_body<NT>(n, o, (s == 0 ? wup : wdn) + (long long)w * n * o, s == 0 ? cup : cdn,
+                             logdet + (long long)w * nspin + s, L[wave],
+                             split_rows{ghi + rre, glo + rre, ghi + rim, glo + rim});
+}
+
 template <int MAXO>
 __global__ __launch_bounds__(THREADS) void greens_kernel(
     int nw, int n, int o, int nspin, const double2* __restrict__ wup,
@@ -552,4 +929,32 @@ extern "C" int afq_overlap_greens_split(int nw, int n, int o, int nspin, const d
     else
         launch_greens_wave<32, true>(wgrid, st, nw, n, o, nspin, wu, wd, trial_up, cd, greens, ld, hi, lo, ldk);
     return afq::check_launch("afq_overlap_greens_split");
+}
+
+extern "C" int afq_overlap_greens_bf16x3(int nw, int n, int o, int nspin, const double* walkers_up,
+                                         const double* walkers_dn, const double* trial_up, const double* trial_dn,
+                                         double* logdet, unsigned short* g_hi, unsigned short* g_lo, long long ldk,
+                                         void* stream) {
+    AFQ_REQUIRE(nw > 0 && n > 0 && o > 0 && o <= n, "afq_overlap_greens_bf16x3: bad sizes nw=%d n=%d o=%d", nw, n,
+                o);
+    AFQ_REQUIRE(o <= 32, "afq_overlap_greens_bf16x3: o=%d exceeds the maximum of 32", o);
+    AFQ_REQUIRE(nspin == 1 || nspin == 2, "afq_overlap_greens_bf16x3: nspin must be 1 or 2");
+    AFQ_REQUIRE(walkers_up && trial_up && logdet && g_hi && g_lo, "afq_overlap_greens_bf16x3: null pointer");
+    AFQ_REQUIRE(nspin == 1 || (walkers_dn && trial_dn), "afq_overlap_greens_bf16x3: null spin-down pointer");
+    const long long ldg = (long long)nspin * o * n;
+    AFQ_REQUIRE(ldk >= ldg && ldk - ldg < 64, "afq_overlap_greens_bf16x3: ldk=%lld not in [%lld, %lld)", ldk, ldg,
+                ldg + 64);
+    AFQ_REQUIRE((long long)n * o < (1LL << 31), "afq_overlap_greens_bf16x3: n * o too large");
+    hipStream_t st = (hipStream_t)stream;
+    const double2* wu = (const double2*)walkers_up;
+    const double2* wd = (const double2*)(nspin == 2 ? walkers_dn : walkers_up);
+    const double* cd = nspin == 2 ? trial_dn : trial_up;
+    const dim3 grid((unsigned)afq::ceil_div(nw * nspin, THREADS / 64));
+    if (o <= 16)
+        hipLaunchKernelGGL(greens_gj_split_kernel<1>, grid, dim3(THREADS), 0, st, nw, n, o, nspin, wu, wd, trial_up,
+                           cd, (double2*)logdet, (uint16_t*)g_hi, (uint16_t*)g_lo, ldk);
+    else
+        hipLaunchKernelGGL(greens_gj_split_kernel<2>, grid, dim3(THREADS), 0, st, nw, n, o, nspin, wu, wd, trial_up,
+                           cd, (double2*)logdet, (uint16_t*)g_hi, (uint16_t*)g_lo, ldk);
+    return afq::check_launch("afq_overlap_greens_bf16x3");
 }

@@ -275,6 +275,32 @@ def overlap_greens(walkers, trial_up, trial_dn=None, greens=None, logdet=None, s
     return greens, logdet
 
 
+def overlap_greens_bf16x3(walkers, trial_up, trial_dn=None):
+    """log det (nw, nspin) complex128 and the (hi, lo) bf16 planes (2 nw,
+    pad8(nspin*o*n)) of the Green's function rows, without the fp64 rows: the
+    rows are a split-bf16 product of w and the fp64 inverse of C^T w (o <= 32).
+    The operands of the bf16x3 force bias."""
+    wu, wd = (walkers if isinstance(walkers, tuple) else (walkers, None))
+    nspin = 1 if wd is None else 2
+    nw, n, o = wu.shape
+    _check(wu, c128, (nw, n, o), "walkers")
+    dev = wu.device
+    if wd is not None:
+        _check(wd, c128, (nw, n, o), "walkers_dn", dev)
+    _check(trial_up, f64, (n, o), "trial_up", dev)
+    if nspin == 2:
+        trial_dn = trial_up if trial_dn is None else trial_dn
+        _check(trial_dn, f64, (n, o), "trial_dn", dev)
+    logdet = torch.empty((nw, nspin), dtype=c128, device=dev)
+    ldk = pad8(nspin * o * n)
+    hi = torch.empty((2 * nw, ldk), dtype=torch.int16, device=dev)
+    lo = torch.empty((2 * nw, ldk), dtype=torch.int16, device=dev)
+    _call("afq_overlap_greens_bf16x3", wu, nw, n, o, nspin, wu.data_ptr(), _ptr(wd), trial_up.data_ptr(),
+          _ptr(trial_dn if nspin == 2 else None), logdet.data_ptr(), hi.data_ptr(), lo.data_ptr(), ldk,
+          _stream(wu))
+    return logdet, (hi, lo)
+
+
 def splitk_for_f64(M: int, N: int, K: int) -> int:
     """K chunks for the fp64 GEMM (128 x 128 tiles, two workgroups per CU):
     whole when the tile grid covers 256 CUs twice; otherwise enough chunks (at

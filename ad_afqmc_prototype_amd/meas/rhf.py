@@ -102,7 +102,28 @@ def force_bias_planes(walkers, meas_ctx: rhf_meas_ctx, trial_data, greens=None) 
     nspin = _nspin(walkers)
     if greens is None:
         greens = trial_rhf.greens(walkers, trial_data, split=meas_ctx.precision == "bf16x3")
+    if greens.greens is None:
+        # only the bf16 planes: the bf16x3 force bias reads nothing else
+        if meas_ctx.precision != "bf16x3" or greens.split is None:
+            greens = trial_rhf.greens(walkers, trial_data, split=meas_ctx.precision == "bf16x3")
+        else:
+            w0 = walkers[0] if nspin == 2 else walkers
+            nw, n, o = w0.shape
+            return _force_bias_split(greens.split, nw, nspin * o * n, meas_ctx, nspin,
+                                     2.0 if nspin == 1 else 1.0)
     return _force_bias_rows(greens.greens, meas_ctx, nspin, 2.0 if nspin == 1 else 1.0, greens.split)
+
+
+def _force_bias_split(split: tuple, nw: int, k: int, meas_ctx: rhf_meas_ctx, nspin: int,
+                      alpha: float) -> torch.Tensor:
+    """bf16x3 force bias (2, nw, naux) from the bf16 planes (2 nw, pad8(k))
+    of the Green's function rows."""
+    op = meas_ctx.fb_operand(nspin)
+    naux = meas_ctx.rot_chol.shape[1]
+    gh, gl = split
+    fb = torch.empty((2, nw, naux), dtype=torch.float64, device=gh.device)
+    kernels.gemm_bf16x3_nt(gh, gl, op[0], op[1], fb.view(2 * nw, naux), alpha=alpha, tag="force_bias")
+    return fb
 
 
 def force_bias_planes_g(walkers, meas_ctx: rhf_meas_ctx, trial_data, greens=None) -> torch.Tensor:
@@ -161,7 +182,7 @@ def _energy(walkers, ham_data: ham_chol, meas_ctx: rhf_meas_ctx, trial_data, gre
     nspin = _nspin(walkers)
     w0 = walkers[0] if nspin == 2 else walkers
     nw, n, o = w0.shape
-    if greens is None:
+    if greens is None or greens.greens is None:
         greens = trial_rhf.greens(walkers, trial_data)
     rot_pad, rot_h1 = meas_ctx.energy_operands(nspin)
     return kernels.energy(greens.greens, nw, n, o, nspin, rot_pad, rot_h1, float(ham_data.h0))
@@ -189,16 +210,18 @@ def make_rhf_meas_ops(sys: system, precision: str = "fp64") -> meas_ops:
     # the bf16x3 force bias takes the Green's function rows as bf16 planes,
     # which the Green's function kernel writes in the same pass
     greens_p = partial(trial_rhf.greens, split=precision == "bf16x3")
+    # the step's new walkers need only what the next force bias reads
+    step_greens = trial_rhf.greens_split if precision == "bf16x3" else None
     if wk == "restricted":
         return meas_ops(overlap=trial_rhf.overlap_r, build_meas_ctx=build_meas_ctx_p,
                         kernels={k_force_bias: force_bias_kernel_r, k_energy: energy_kernel_r},
                         log_overlap=trial_rhf.log_overlap, greens=greens_p,
-                        force_bias_planes=force_bias_planes)
+                        force_bias_planes=force_bias_planes, step_greens=step_greens)
     if wk == "unrestricted":
         return meas_ops(overlap=trial_rhf.overlap_u, build_meas_ctx=build_meas_ctx_p,
                         kernels={k_force_bias: force_bias_kernel_u, k_energy: energy_kernel_u},
                         log_overlap=trial_rhf.log_overlap, greens=greens_p,
-                        force_bias_planes=force_bias_planes)
+                        force_bias_planes=force_bias_planes, step_greens=step_greens)
     if wk == "generalized":
         # meas/rhf.py:169-174: overlap and force bias, no energy kernel
         return meas_ops(overlap=trial_rhf.overlap_g, build_meas_ctx=build_meas_ctx_p,

@@ -115,7 +115,8 @@ def afqmc_step(state: prop_state, *, params: afqmc_params, ham_data: ham_chol, t
         walkers_new = prop_ops.apply_trotter_planes(state.walkers, xbar, prop_ctx, params.n_exp_terms,
                                                     params.n_chunks)
     if meas_ops.greens is not None:
-        greens_new = meas_ops.greens(walkers_new, trial_data)
+        make_greens = meas_ops.step_greens or meas_ops.greens
+        greens_new = make_greens(walkers_new, trial_data)
         logdet = greens_new.logdet
     else:
         greens_new = None

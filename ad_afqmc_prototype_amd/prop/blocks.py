@@ -78,8 +78,8 @@ def reconfigure(state: prop_state, *, sys: system, trial_data, meas_ops_: meas_o
             nw_old = greens.logdet.shape[0]
             split = tuple(kernels.gather_rows(x.view(2, nw_old, -1).view(torch.float64), idx, nplanes=2)
                           .view(torch.int16).reshape(2 * idx.shape[0], -1) for x in split)
-        greens = type(greens)(kernels.gather_rows(greens.greens, idx, nplanes=2),
-                              kernels.gather_rows(greens.logdet, idx), split)
+        rows = None if greens.greens is None else kernels.gather_rows(greens.greens, idx, nplanes=2)
+        greens = type(greens)(rows, kernels.gather_rows(greens.logdet, idx), split)
     return state._replace(walkers=w_sr, weights=w_new, log_overlaps=lo, rng_key=key,
                           greens=greens), idx
 
@@ -153,8 +153,13 @@ class captured_block:
                  prop_ctx: chol_afqmc_ctx, meas_ctx: Any):
         self._kw = dict(sys=sys, params=params, ham_data=ham_data, trial_data=trial_data,
                         meas_ops=meas_ops, prop_ops=prop_ops, prop_ctx=prop_ctx, meas_ctx=meas_ctx)
-        if state.greens is None and meas_ops.greens is not None:
-            state = state._replace(greens=meas_ops.greens(state.walkers, trial_data))
+        if meas_ops.greens is not None and (state.greens is None or state.greens.greens is None):
+            # the block ends with the full cache (QR refresh, comb gather): the
+            # static state must hold the same layout.  A cache with only the
+            # bf16 planes keeps them (the first step reads them) and gains rows.
+            full = meas_ops.greens(state.walkers, trial_data)
+            state = state._replace(greens=full if state.greens is None
+                                   else state.greens._replace(greens=full.greens))
         self.state = _clone_state(state)
         # one eager block on a copy first (lazy operand layouts, kernel
         # attributes), on a side stream as graph capture requires

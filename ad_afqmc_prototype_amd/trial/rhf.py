@@ -76,9 +76,11 @@ class uhf_trial:
 class greens_cache(NamedTuple):
     """Half-rotated Green's functions (split planes (2, nw, nspin*o*n)) and
     log det(C_s^T w_s) (nw, nspin) of one walker batch; optionally the
-    (hi, lo) bf16 planes of the greens rows for the bf16x3 force bias."""
+    (hi, lo) bf16 planes of the greens rows for the bf16x3 force bias.
+    ``greens`` is None when only the bf16 planes were made (the bf16x3 step's
+    cache): consumers that need the fp64 rows recompute them."""
 
-    greens: torch.Tensor
+    greens: torch.Tensor | None
     logdet: torch.Tensor
     split: tuple | None = None
 
@@ -104,6 +106,21 @@ def greens(walkers, trial_data, split: bool = False) -> greens_cache:
             raise ValueError("restricted walkers need an RHF trial")
         out = kernels.overlap_greens(walkers, trial_data.mo_up, split=split)
     return greens_cache(*out)
+
+
+def greens_split(walkers, trial_data) -> greens_cache:
+    """The bf16x3 step's Green's function: log det and the bf16 planes of the
+    rows only (o <= 32; larger o takes greens(..., split=True))."""
+    w0 = walkers[0] if isinstance(walkers, tuple) else walkers
+    if w0.shape[2] > 32:
+        return greens(walkers, trial_data, split=True)
+    if isinstance(walkers, tuple):
+        ld, sp = kernels.overlap_greens_bf16x3(walkers, trial_data.mo_up, trial_data.mo_dn)
+    else:
+        if not trial_data.restricted:
+            raise ValueError("restricted walkers need an RHF trial")
+        ld, sp = kernels.overlap_greens_bf16x3(walkers, trial_data.mo_up)
+    return greens_cache(None, ld, sp)
 
 
 def log_overlap(walkers, trial_data) -> torch.Tensor:

@@ -120,6 +120,22 @@ int afq_overlap_greens(int nw, int n, int o, int nspin,
                        double* greens, double* logdet, void* stream);
 
 /*
+ * The bf16x3 step's Green's function (meas/rhf.py:15-33, trial/rhf.py:25-37):
+ * log det(C_s^T w_s) in fp64 as afq_overlap_greens (partial pivoting on
+ * |re| + |im|, the LU's pivots, here in a Gauss-Jordan inversion), and the
+ * rows of G = (w m^-1)^T written only as their split-bf16 planes g_hi / g_lo
+ * (afq_overlap_greens_split's layout: 2 nw rows of ldk, ldk in
+ * [nspin*o*n, nspin*o*n + 64), padding zeroed), formed on the bf16 matrix
+ * cores as (split m^-1)^T (split w)^T: accurate to the split-bf16 product
+ * (~2^-16 relative), the precision the bf16x3 force bias reads them at.  No
+ * fp64 rows.  o <= 32.
+ */
+int afq_overlap_greens_bf16x3(int nw, int n, int o, int nspin, const double* walkers_up,
+                              const double* walkers_dn, const double* trial_up, const double* trial_dn,
+                              double* logdet, unsigned short* g_hi, unsigned short* g_lo, long long ldk,
+                              void* stream);
+
+/*
  * afq_overlap_greens that also writes the Green's function rows as split-bf16
  * planes g_hi / g_lo (2 nw rows of ldk: real rows, then imaginary rows, the
  * values afq_split_bf16 makes of greens; columns nspin o n .. ldk - 1 zeroed),

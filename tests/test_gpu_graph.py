@@ -78,6 +78,28 @@ def test_captured_block_replays_match_eager(gpu_device, precision):
         assert np.isfinite(x)
 
 
+def test_captured_block_from_a_stepped_bf16x3_state(gpu_device):
+    """The bf16x3 step leaves only the force bias's bf16 planes in the Green's
+    cache; a block captured from such a state (the cache refilled at capture)
+    replays like the eager blocks from the same state."""
+    import torch
+    from ad_afqmc_prototype_amd.prop.afqmc import afqmc_step
+    from ad_afqmc_prototype_amd.prop.blocks import _clone_state, afqmc_block, captured_block
+    st, kw, _ = _setup(gpu_device, precision="bf16x3")
+    st = afqmc_step(st, **{k: v for k, v in kw.items() if k != "sys"})
+    assert st.greens.greens is None and st.greens.split is not None
+    ref, st_e = [], _clone_state(st)
+    for _ in range(2):
+        st_e, obs = afqmc_block(st_e, **kw)
+        ref.append((obs.scalars["energy"].item(), st_e.walkers.clone()))
+    cb = captured_block(st, **kw)
+    for i in range(2):
+        e, _ = cb.replay()
+        torch.cuda.synchronize()
+        assert e.item() == ref[i][0], (i, e.item(), ref[i][0])
+        assert torch.equal(cb.state.walkers, ref[i][1])
+
+
 def test_driver_graph_equals_eager(gpu_device):
     from ad_afqmc_prototype_amd.driver import run_afqmc_energy
     out = []

@@ -418,3 +418,46 @@ def test_field_shifts_split_planes_equal_split_of_xbar(gpu_device, nw, naux):
         assert torch.equal(u, v)
     rh, rl = kernels.split_bf16(a[0].reshape(2 * nw, naux))
     assert torch.equal(b[3][0], rh) and torch.equal(b[3][1], rl)
+
+
+@pytest.mark.parametrize("case", [
+    dict(seed=2, n=200, o=32, uhf=False, kind="restricted", nw=67),
+    dict(seed=1, n=120, o=20, uhf=True, kind="unrestricted", nw=33),
+    dict(seed=0, n=24, o=6, uhf=False, kind="restricted", nw=40),
+    dict(seed=5, n=45, o=16, uhf=False, kind="restricted", nw=9),
+    dict(seed=6, n=61, o=17, uhf=True, kind="unrestricted", nw=5),
+])
+def test_greens_bf16x3_split_only(gpu_device, case):
+    """afq_overlap_greens_bf16x3 (Gauss-Jordan inverse, rows as a split-bf16
+    product) against the fp64 LU kernel: log det to 1e-12, the bf16 planes'
+    value hi + lo within 1e-4 of the fp64 rows (relative, norm-wise per walker),
+    padding columns zero; walkers near the trial and far from it."""
+    import torch
+    from ad_afqmc_prototype_amd import kernels
+    from ._pairs import random_walkers
+    s = syn.make_system(case["seed"], case["n"], case["o"], 10, uhf=case["uhf"])
+    rng = np.random.default_rng(case["seed"])
+    for spread in (0.3, 3.0):
+        w = random_walkers(rng, case["kind"], case["nw"], case["n"], case["o"], s.mo_up, s.mo_dn, spread=spread)
+        wd = to_dev(w, gpu_device)
+        cu = torch.tensor(s.mo_up, device=gpu_device)
+        cd = torch.tensor(s.mo_dn, device=gpu_device)
+        g64, ld64, (h64, l64) = kernels.overlap_greens(wd, cu, cd, split=True)
+        ld, (hi, lo) = kernels.overlap_greens_bf16x3(wd, cu, cd)
+        ref = ld64.cpu().numpy()
+        got = ld.cpu().numpy()
+        np.testing.assert_allclose(got.real, ref.real, rtol=1e-12, atol=1e-12)
+        np.testing.assert_allclose(np.exp(1j * got.imag), np.exp(1j * ref.imag), atol=1e-12)
+
+        def value(h, l):
+            hv = (h.cpu().to(torch.int32) & 0xFFFF) << 16
+            lv = (l.cpu().to(torch.int32) & 0xFFFF) << 16
+            return hv.view(torch.float32).double() + lv.view(torch.float32).double()
+
+        nw = case["nw"]
+        k = g64.shape[2]
+        v = value(hi, lo).numpy()
+        assert np.all(v[:, k:] == 0.0)
+        g = g64.cpu().numpy().reshape(2 * nw, k)
+        err = np.linalg.norm(v[:, :k] - g, axis=1) / np.linalg.norm(g, axis=1)
+        assert err.max() < 1e-4, (spread, err.max())
