@@ -114,6 +114,20 @@ __device__ __forceinline__ void split_bf16(double x, uint16_t& hi, uint16_t& lo)
     lo = f32_to_bf16_rne((float)(x - bf16_to_f64(hi)));
 }
 
+// An fp32 value stored as its split-bf16 pair in 4 bytes (the size of the
+// float it replaces): hi = bf16(v) in the low half, lo = bf16(v - hi) in the
+// high half, both rounded to nearest even.  The split a bf16x3 consumer would
+// otherwise compute on load, done once by the producer.
+struct bf16_pair {
+    uint32_t bits;
+    bf16_pair() = default;
+    __device__ explicit bf16_pair(float v) {
+        const __bf16 h = (__bf16)v;
+        const __bf16 l = (__bf16)(v - (float)h);
+        bits = (uint32_t)__builtin_bit_cast(uint16_t, h) | ((uint32_t)__builtin_bit_cast(uint16_t, l) << 16);
+    }
+};
+
 // ---------------------------------------------------------------------------
 // Symmetric VHS epilogue.  The VHS GEMM can run on the upper 16 x 16 blocks of
 // the n x n plane only (V = sum_g x_g L_g is symmetric): its column

@@ -400,7 +400,9 @@ __global__ void splitk_reduce_kernel(int M, int N, int splits, const float* __re
         double acc = 0.0;
         for (int z = 0; z < splits; ++z) acc += (double)part[z * stride + e];
         const long long r = e / N, c = e - r * N;
-        if (c_fp64)
+        if (c_fp64 == 2)
+            ((afq::bf16_pair*)C)[r * ldc + c] = afq::bf16_pair((float)((double)alpha * acc));
+        else if (c_fp64)
             ((double*)C)[r * ldc + c] = (double)alpha * acc;
         else
             ((float*)C)[r * ldc + c] = (float)((double)alpha * acc);
@@ -467,8 +469,13 @@ extern "C" int afq_gemm_bf16x3_nt(int M, int N, int K, float alpha, const unsign
     const int tiles_m2 = afq::ceil_div(M, BM2);
     const int tiles_n2 = afq::ceil_div(N, BN2);
     const long long total2 = (long long)tiles_m2 * tiles_n2;
+    AFQ_REQUIRE(c_fp64 >= 0 && c_fp64 <= 2, "afq_gemm_bf16x3_nt: c_fp64=%d not in 0..2", c_fp64);
     if (total2 >= 512 && total2 < (1LL << 31)) {
-        if (c_fp64)
+        if (c_fp64 == 2)
+            hipLaunchKernelGGL(gemm_bf16x3_nt_256_kernel<afq::bf16_pair>, dim3((unsigned)total2), dim3(THREADS2), 0,
+                               st, M, N, K, alpha, A_hi, A_lo, lda, B_hi, B_lo, ldb, (afq::bf16_pair*)C, ldc,
+                               tiles_m2, tiles_n2);
+        else if (c_fp64)
             hipLaunchKernelGGL(gemm_bf16x3_nt_256_kernel<double>, dim3((unsigned)total2), dim3(THREADS2), 0, st,
                                M, N, K, alpha, A_hi, A_lo, lda, B_hi, B_lo, ldb, (double*)C, ldc, tiles_m2,
                                tiles_n2);
@@ -482,7 +489,11 @@ extern "C" int afq_gemm_bf16x3_nt(int M, int N, int K, float alpha, const unsign
     const int tiles_n = afq::ceil_div(N, BN);
     const long long total = (long long)tiles_m * tiles_n;
     AFQ_REQUIRE(total < (1LL << 31), "afq_gemm_bf16x3_nt: problem too large");
-    if (c_fp64)
+    if (c_fp64 == 2)
+        hipLaunchKernelGGL(gemm_bf16x3_nt_kernel<afq::bf16_pair>, dim3((unsigned)total), dim3(THREADS), 0, st,
+                           M, N, K, alpha, A_hi, A_lo, lda, B_hi, B_lo, ldb, (afq::bf16_pair*)C, ldc,
+                           tiles_m, tiles_n);
+    else if (c_fp64)
         hipLaunchKernelGGL(gemm_bf16x3_nt_kernel<double>, dim3((unsigned)total), dim3(THREADS), 0, st,
                            M, N, K, alpha, A_hi, A_lo, lda, B_hi, B_lo, ldb, (double*)C, ldc,
                            tiles_m, tiles_n);
@@ -500,6 +511,7 @@ extern "C" int afq_gemm_bf16x3_nt_splitk(int M, int N, int K, float alpha, const
     AFQ_REQUIRE(M >= 0 && N >= 0 && K >= 0, "afq_gemm_bf16x3_nt_splitk: negative size");
     if (M == 0 || N == 0) return 0;
     AFQ_REQUIRE(splits >= 1 && splits <= 16, "afq_gemm_bf16x3_nt_splitk: splits=%d not in 1..16", splits);
+    AFQ_REQUIRE(c_fp64 >= 0 && c_fp64 <= 2, "afq_gemm_bf16x3_nt_splitk: c_fp64=%d not in 0..2", c_fp64);
     AFQ_REQUIRE(A_hi && A_lo && B_hi && B_lo && C && workspace, "afq_gemm_bf16x3_nt_splitk: null pointer");
     AFQ_REQUIRE(K % 8 == 0 && lda % 8 == 0 && ldb % 8 == 0,
                 "afq_gemm_bf16x3_nt_splitk: K=%d, lda and ldb must be multiples of 8", K);

@@ -66,6 +66,24 @@ __device__ __forceinline__ split8 split_f32x8(const float (&x)[8]) {
     return s;
 }
 
+// Eight bf16_pair words (hi in the low half, lo in the high half), loaded as
+// raw 32-bit values, regrouped into the hi and lo operand vectors: one byte
+// permute per pair of elements and half.
+__device__ __forceinline__ split8 unpack_pairs(const float (&x)[8]) {
+    uint32_t h[4], l[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        const uint32_t a = __builtin_bit_cast(uint32_t, x[2 * j]);
+        const uint32_t b = __builtin_bit_cast(uint32_t, x[2 * j + 1]);
+        h[j] = __builtin_amdgcn_perm(b, a, 0x05040100u);
+        l[j] = __builtin_amdgcn_perm(b, a, 0x07060302u);
+    }
+    split8 s;
+    s.hi = __builtin_bit_cast(bf16x8, make_uint4(h[0], h[1], h[2], h[3]));
+    s.lo = __builtin_bit_cast(bf16x8, make_uint4(l[0], l[1], l[2], l[3]));
+    return s;
+}
+
 // Raw buffer loads: a 32-bit byte offset from a per-array base in SGPRs, and
 // zeros for offsets at or past the array's size (gfx9 descriptor word 3).
 typedef __amdgpu_buffer_rsrc_t rsrc_t;
@@ -149,9 +167,11 @@ __device__ __forceinline__ split8 split_minus_identity(const float (&x)[8], int 
     return split_f32x8(y);
 }
 
-// SYMV: vhs holds each walker's V, and E holds exp(-dt h1 / 2), as upper 16 x 16
-// blocks (afq_vhs_sym's column order, nb (nb + 1) / 2 * 256 floats per matrix)
-// instead of (n, ldv) rows; both are symmetric.
+// SYMV: vhs holds each walker's V, and E holds exp(-dt h1 / 2) - I, as upper
+// 16 x 16 blocks (afq_vhs_sym's column order, nb (nb + 1) / 2 * 256 entries per
+// matrix) instead of (n, ldv) rows; both are symmetric, and every entry is
+// already split: a bf16_pair word (the split the kernel otherwise does on
+// load, from the same fp32 value), so the fragments are regrouped, not converted.
 // GROUPED: the workgroup's system is found in the descriptor table S by the
 // prefix sums tblock0 (count systems, restricted walkers, (n, ldv) layouts);
 // its sizes and pointers replace the arguments, and blockIdx.x counts from
@@ -249,7 +269,12 @@ __global__ __launch_bounds__(512, 1) void trotter_bf16x3_kernel(
             if (kb < nk) {
                 split8 ef[RT];
 #pragma unroll
-                for (int t = 0; t < RT; ++t) ef[t] = split_minus_identity(er[t][kb], r0 + 16 * t, n, kb * 32 + k8);
+                for (int t = 0; t < RT; ++t) {
+                    if constexpr (SYMV)
+                        ef[t] = unpack_pairs(er[t][kb]);
+                    else
+                        ef[t] = split_minus_identity(er[t][kb], r0 + 16 * t, n, kb * 32 + k8);
+                }
                 const bf16x8 brh = read_b(kb, 0), brl = read_b(kb, 1);
                 const bf16x8 bih = read_b(kb, 2), bil = read_b(kb, 3);
 #pragma unroll
@@ -347,8 +372,8 @@ __global__ __launch_bounds__(512, 1) void trotter_bf16x3_kernel(
 #pragma unroll
             for (int l = 0; l < NL; ++l) {
                 uint4* dst = VL + ((size_t)((wave * RT + t) * NL + l) * 4) * 64 + lane;
-                const split8 a = split_f32x8(xl[t][l][0]);
-                const split8 c = split_f32x8(xl[t][l][1]);
+                const split8 a = SYMV ? unpack_pairs(xl[t][l][0]) : split_f32x8(xl[t][l][0]);
+                const split8 c = SYMV ? unpack_pairs(xl[t][l][1]) : split_f32x8(xl[t][l][1]);
                 dst[0 * 64] = __builtin_bit_cast(uint4, a.hi);
                 dst[1 * 64] = __builtin_bit_cast(uint4, a.lo);
                 dst[2 * 64] = __builtin_bit_cast(uint4, c.hi);
@@ -375,8 +400,13 @@ __global__ __launch_bounds__(512, 1) void trotter_bf16x3_kernel(
         for (int t = 0; t < RT; ++t)
 #pragma unroll
             for (int kb = 0; kb < KR; ++kb) {
-                vr[t][kb] = split_f32x8(xv[t][kb][0]);
-                vi[t][kb] = split_f32x8(xv[t][kb][1]);
+                if constexpr (SYMV) {
+                    vr[t][kb] = unpack_pairs(xv[t][kb][0]);
+                    vi[t][kb] = unpack_pairs(xv[t][kb][1]);
+                } else {
+                    vr[t][kb] = split_f32x8(xv[t][kb][0]);
+                    vi[t][kb] = split_f32x8(xv[t][kb][1]);
+                }
             }
     }
     __syncthreads();
@@ -570,17 +600,19 @@ extern "C" int afq_trotter_bf16x3(int nw, int n, int o, int nspin, int ldv, int 
 // Declared in afqmc_hip.h: V and E as upper 16 x 16 blocks; n <= 224 only
 // (there is no fp32 kernel for this layout).
 extern "C" int afq_trotter_bf16x3_sym(int nw, int n, int o, int nspin, int n_terms, double sqrt_dt,
-                                      const float* exp_h1_half_sym, const float* vhs_sym, const double* win_up,
-                                      const double* win_dn, double* wout_up, double* wout_dn, void* stream) {
+                                      const unsigned* exp_h1_half_sym, const unsigned* vhs_sym,
+                                      const double* win_up, const double* win_dn, double* wout_up,
+                                      double* wout_dn, void* stream) {
     int nl = 0;
     const int ldv = afq::round_up(n, 16);  // unused by this layout
-    const int rc = check_bf16x3("afq_trotter_bf16x3_sym", nw, n, o, nspin, ldv, n_terms, exp_h1_half_sym,
-                                vhs_sym, win_up, win_dn, wout_up, wout_dn, &nl);
+    const int rc = check_bf16x3("afq_trotter_bf16x3_sym", nw, n, o, nspin, ldv, n_terms,
+                                (const float*)exp_h1_half_sym, (const float*)vhs_sym, win_up, win_dn, wout_up,
+                                wout_dn, &nl);
     if (rc != 0) return rc;
     AFQ_REQUIRE(nl <= NL_MAX && bf16x3_lds(n, nl) <= 160 * 1024,
                 "afq_trotter_bf16x3_sym: n=%d exceeds the on-chip maximum of 224", n);
-    return launch_bf16x3_nl<true>(nl, nw, n, o, nspin, ldv, n_terms, sqrt_dt, exp_h1_half_sym, vhs_sym, win_up,
-                                  win_dn, wout_up, wout_dn, (hipStream_t)stream);
+    return launch_bf16x3_nl<true>(nl, nw, n, o, nspin, ldv, n_terms, sqrt_dt, (const float*)exp_h1_half_sym,
+                                  (const float*)vhs_sym, win_up, win_dn, wout_up, wout_dn, (hipStream_t)stream);
 }
 
 extern "C" int afq_trotter_bf16x3_grouped(int count, const afq_system_step* sys, int total_blocks, int max_n,

@@ -225,17 +225,18 @@ def gemm_bf16x3_nt(a_hi, a_lo, b_hi, b_lo, C, alpha=1.0, tag=None, splits=None):
     for t, nm, sh in ((a_hi, "a_hi", (M, K)), (a_lo, "a_lo", (M, K)), (b_hi, "b_hi", (N, K)),
                       (b_lo, "b_lo", (N, K))):
         _check(t, torch.int16, sh, nm, a_hi.device)
-    if C.dtype not in (f32, f64):
-        raise TypeError("C must be float32 or float64")
+    if C.dtype not in (f32, f64, torch.int32):
+        raise TypeError("C must be float32, float64 or int32 (split-bf16 pair words)")
     _check(C, C.dtype, (M, N), "C", a_hi.device)
+    mode = {f32: 0, f64: 1, torch.int32: 2}[C.dtype]
     if splits > 1:
         ws = torch.empty((splits, M, N), dtype=f32, device=a_hi.device)
         _call("afq_gemm_bf16x3_nt_splitk", a_hi, M, N, K, float(alpha), a_hi.data_ptr(), a_lo.data_ptr(), K,
-              b_hi.data_ptr(), b_lo.data_ptr(), K, C.data_ptr(), N, int(C.dtype == f64), int(splits),
+              b_hi.data_ptr(), b_lo.data_ptr(), K, C.data_ptr(), N, mode, int(splits),
               ws.data_ptr(), _stream(a_hi), tag=tag)
         return C
     _call("afq_gemm_bf16x3_nt", a_hi, M, N, K, float(alpha), a_hi.data_ptr(), a_lo.data_ptr(), K,
-          b_hi.data_ptr(), b_lo.data_ptr(), K, C.data_ptr(), N, int(C.dtype == f64), _stream(a_hi),
+          b_hi.data_ptr(), b_lo.data_ptr(), K, C.data_ptr(), N, mode, _stream(a_hi),
           tag=tag)
     return C
 
@@ -384,9 +385,9 @@ def trotter(walkers, vhs_planes, exp_h1_half_pad, sqrt_dt, n_terms, out=None, ar
     """Fused Trotter step; fp64 when vhs / exp_h1_half are float64.  With
     float32 operands ``arith`` picks the products: "fp32" (default) or
     "bf16x3" (split-bf16 MFMAs, fp32 sums).  Walkers are complex128 either way.
-    bf16x3 also takes V and E as upper 16 x 16 blocks: vhs_planes (2, nw,
-    sym_blocks(n) * 256) and exp_h1_half_pad (sym_blocks(n) * 256,) float32
-    (n <= 224)."""
+    bf16x3 also takes V and E - I as upper 16 x 16 blocks of split-bf16 pair
+    words: vhs_planes (2, nw, sym_blocks(n) * 256) and exp_h1_half_pad
+    (sym_blocks(n) * 256,) int32 (n <= 224)."""
     wu, wd = (walkers if isinstance(walkers, tuple) else (walkers, None))
     nspin = 1 if wd is None else 2
     nw, n, o = wu.shape
@@ -397,12 +398,12 @@ def trotter(walkers, vhs_planes, exp_h1_half_pad, sqrt_dt, n_terms, out=None, ar
         _check(wd, c128, (nw, n, o), "walkers_dn", dev)
     rt = vhs_planes.dtype
     if vhs_planes.dim() == 3:
-        if arith != "bf16x3" or rt != f32:
-            raise ValueError("upper-block VHS planes are a bf16x3 operand (float32)")
+        if arith != "bf16x3" or rt != torch.int32:
+            raise ValueError("upper-block VHS planes are a bf16x3 operand (int32 split-bf16 pairs)")
         if n > BF16X3_SYM_MAX_N:
             raise ValueError(f"upper-block VHS planes need n <= {BF16X3_SYM_MAX_N}, got n={n}")
-        _check(vhs_planes, f32, (2, nw, sym_blocks(n) * 256), "vhs", dev)
-        _check(exp_h1_half_pad, f32, (sym_blocks(n) * 256,), "exp_h1_half (upper blocks)", dev)
+        _check(vhs_planes, torch.int32, (2, nw, sym_blocks(n) * 256), "vhs", dev)
+        _check(exp_h1_half_pad, torch.int32, (sym_blocks(n) * 256,), "exp_h1_half - I (upper-block pairs)", dev)
         if out is None:
             out = (torch.empty_like(wu), torch.empty_like(wd)) if nspin == 2 else torch.empty_like(wu)
         ou, od = (out if isinstance(out, tuple) else (out, None))

@@ -30,7 +30,9 @@ class chol_afqmc_ctx:
     h0_prop: complex
     exp_h1_half_pad: torch.Tensor   # (n, ldv) device layout of exp_h1_half
     exp_h1_half_pad32: torch.Tensor = None  # float32 copy for the mixed step
-    exp_h1_half_sym32: torch.Tensor = None  # float32 upper 16 x 16 blocks (bf16x3 Trotter)
+    # exp(-dt h1_eff / 2) - I as upper 16 x 16 blocks of split-bf16 pair words
+    # (int32, split_pairs of the float32 entries): the bf16x3 Trotter's E operand
+    exp_h1_half_sym_pairs: torch.Tensor = None
 
 
 class chol_afqmc_ops(NamedTuple):
@@ -78,7 +80,7 @@ def _build_prop_ctx(ham_data: ham_chol, trial_rdm1, dt: float) -> chol_afqmc_ctx
     return chol_afqmc_ctx(dt=float(dt), sqrt_dt=float(dt) ** 0.5, exp_h1_half=exp_h1_half,
                           mf_shifts=mf.contiguous(), h0_prop=h0_prop, exp_h1_half_pad=pad,
                           exp_h1_half_pad32=pad.to(torch.float32),
-                          exp_h1_half_sym32=upper_blocks(exp_h1_half).to(torch.float32))
+                          exp_h1_half_sym_pairs=minus_identity_pairs(upper_blocks(exp_h1_half).to(torch.float32), n))
 
 
 def chol_transposed(chol: torch.Tensor) -> torch.Tensor:
@@ -111,6 +113,23 @@ def upper_blocks(mat: torch.Tensor) -> torch.Tensor:
     """The upper 16 x 16 blocks of a symmetric n x n matrix, flat
     (sym_blocks(n) * 256,) in the order of chol_sym_rows, zero past n."""
     return chol_sym_rows(mat[None]).reshape(-1)
+
+
+def split_pairs(x32: torch.Tensor) -> torch.Tensor:
+    """float32 values as split-bf16 pair words (int32): bf16(x) in the low 16
+    bits, bf16(x - bf16(x)) in the high 16 bits, both rounded to nearest even
+    (the split the bf16x3 kernels make of an fp32 operand)."""
+    hi = x32.to(torch.bfloat16)
+    lo = (x32 - hi.to(torch.float32)).to(torch.bfloat16)
+    return ((hi.view(torch.int16).to(torch.int32) & 0xFFFF) | (lo.view(torch.int16).to(torch.int32) << 16)).contiguous()
+
+
+def minus_identity_pairs(e_blocks32: torch.Tensor, n: int) -> torch.Tensor:
+    """Split pairs of E - I from E's float32 upper blocks: the identity is
+    subtracted in float32 on the diagonal entries of the first n rows, as the
+    full-row bf16x3 Trotter does on load."""
+    eye = upper_blocks(torch.eye(n, dtype=torch.float64, device=e_blocks32.device)).to(torch.float32)
+    return split_pairs(e_blocks32 - eye)
 
 
 def _chunks(nw: int, n_chunks: int):
@@ -164,7 +183,8 @@ def make_chol_afqmc_ops(ham_data: ham_chol, walker_kind: str,
         nwc = xbar.shape[1]
         flat = xbar.reshape(2 * nwc, nf)
         if v_blocks:
-            out = torch.empty((2 * nwc, chol_hi.shape[0]), dtype=torch.float32, device=xbar.device)
+            # V leaves the contraction as split-bf16 pair words
+            out = torch.empty((2 * nwc, chol_hi.shape[0]), dtype=torch.int32, device=xbar.device)
             xh, xl = xbar_split if xbar_split is not None else kernels.split_bf16(flat)
             kernels.gemm_bf16x3_nt(xh, xl, chol_hi, chol_lo, out, tag="vhs")
             return out.view(2, nwc, -1)
@@ -178,7 +198,7 @@ def make_chol_afqmc_ops(ham_data: ham_chol, walker_kind: str,
 
     def e_half(ctx: chol_afqmc_ctx) -> torch.Tensor:
         if v_blocks:
-            return ctx.exp_h1_half_sym32
+            return ctx.exp_h1_half_sym_pairs
         return ctx.exp_h1_half_pad if precision == "fp64" else ctx.exp_h1_half_pad32
 
     def build_prop_ctx(trial_rdm1, dt: float) -> chol_afqmc_ctx:

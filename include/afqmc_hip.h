@@ -66,7 +66,8 @@ int afq_sgemm_nt(int M, int N, int K, float alpha, const float* A, long long lda
  * accumulation (3 x v_mfma_f32_16x16x32_bf16 per step).  A (M x K) and B
  * (N x K) are given as bf16 hi/lo planes (uint16 bit patterns) from
  * afq_split_bf16; K, lda, ldb multiples of 8.  C is float32, or float64 when
- * c_fp64 != 0.  Force-bias / VHS contractions of the mixed step in bf16x3
+ * c_fp64 == 1, or split-bf16 pair words (bf16(v) | bf16(v - bf16(v)) << 16, v
+ * the fp32 result) when c_fp64 == 2.  Force-bias / VHS contractions of the mixed step in bf16x3
  * mode (meas/rhf.py:33, prop/chol_afqmc_ops.py:72-73).
  */
 int afq_gemm_bf16x3_nt(int M, int N, int K, float alpha, const unsigned short* A_hi,
@@ -237,17 +238,19 @@ int afq_trotter_bf16x3(int nw, int n, int o, int nspin, int ldv, int n_terms, do
                        double* wout_up, double* wout_dn, void* stream);
 
 /*
- * afq_trotter_bf16x3 with V and E = exp(-dt h1 / 2) (both symmetric) given as
- * their upper 16 x 16 blocks: nb (nb + 1) / 2 * 256 floats per matrix,
- * nb = ceil(n / 16), in afq_vhs_sym's column order (block (r, c), r <= c,
- * numbered row by row, entry (pi, qi) at 256 blk + 16 pi + qi), entries past n
- * zero.  vhs_sym holds two planes (real, imaginary) of nw such rows: the
- * product of afq_gemm_bf16x3_nt with the upper-block Cholesky rows, 0.58 of
- * the bytes of full rows at n = 200.  n <= 224 (the on-chip capacity; larger
- * n is an error).
+ * afq_trotter_bf16x3 with V and E - I (E = exp(-dt h1 / 2); both symmetric)
+ * given as their upper 16 x 16 blocks: nb (nb + 1) / 2 * 256 entries per
+ * matrix, nb = ceil(n / 16), in afq_vhs_sym's column order (block (r, c),
+ * r <= c, numbered row by row, entry (pi, qi) at 256 blk + 16 pi + qi),
+ * entries past n zero.  Every entry is a split-bf16 pair word: bf16(v) in the
+ * low 16 bits, bf16(v - bf16(v)) in the high 16 bits, v the fp32 value
+ * (afq_gemm_bf16x3_nt with c_fp64 = 2 writes V so).  vhs_sym holds two planes
+ * (real, imaginary) of nw such rows: the product of afq_gemm_bf16x3_nt with
+ * the upper-block Cholesky rows, 0.58 of the bytes of full rows at n = 200.
+ * n <= 224 (the on-chip capacity; larger n is an error).
  */
 int afq_trotter_bf16x3_sym(int nw, int n, int o, int nspin, int n_terms, double sqrt_dt,
-                           const float* exp_h1_half_sym, const float* vhs_sym, const double* win_up,
+                           const unsigned* exp_h1_half_sym, const unsigned* vhs_sym, const double* win_up,
                            const double* win_dn, double* wout_up, double* wout_dn, void* stream);
 
 /*

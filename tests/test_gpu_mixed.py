@@ -373,7 +373,7 @@ def test_trotter_bf16x3_upper_block_vhs_equals_full_rows(gpu_device, nw, n, o, n
     the stored transposes are the same floats the full rows hold."""
     import torch
     from ad_afqmc_prototype_amd import kernels
-    from ad_afqmc_prototype_amd.prop.chol_afqmc_ops import chol_sym_rows, upper_blocks
+    from ad_afqmc_prototype_amd.prop.chol_afqmc_ops import chol_sym_rows, minus_identity_pairs, upper_blocks
     print(f"upper-block Trotter nw={nw} n={n} o={o} nspin={nspin}", flush=True)
     rng = np.random.default_rng(n + nw)
     g = 40
@@ -383,9 +383,16 @@ def test_trotter_bf16x3_upper_block_vhs_equals_full_rows(gpu_device, nw, n, o, n
     xs = kernels.split_bf16(x)
     cs = kernels.split_bf16(chol_sym_rows(chol))
     v_full = kernels.vhs_sym(xs, cs, n).view(2, nw, n, kernels.ldv_for(n))
-    v_blk = torch.empty((2 * nw, cs[0].shape[0]), dtype=torch.float32, device=gpu_device)
+    v_blk = torch.empty((2 * nw, cs[0].shape[0]), dtype=torch.int32, device=gpu_device)
     kernels.gemm_bf16x3_nt(*xs, *cs, v_blk)
     v_blk = v_blk.view(2, nw, -1)
+    # the pair words hold the float32 products split: hi + lo is within the
+    # split of the float32 output of the same contraction
+    v32 = torch.empty((2 * nw, cs[0].shape[0]), dtype=torch.float32, device=gpu_device)
+    kernels.gemm_bf16x3_nt(*xs, *cs, v32)
+    hv = ((v_blk.reshape(-1) & 0xFFFF) << 16).view(torch.float32).double()
+    lv = (v_blk.reshape(-1) & ~0xFFFF).view(torch.float32).double()
+    assert float(((hv + lv) - v32.reshape(-1).double()).abs().max()) <= 2.0 ** -16 * float(v32.abs().max())
     ldv = kernels.ldv_for(n)
     h = rng.standard_normal((n, n)) * 0.01
     e = torch.zeros((n, ldv), dtype=torch.float32, device=gpu_device)
@@ -394,7 +401,7 @@ def test_trotter_bf16x3_upper_block_vhs_equals_full_rows(gpu_device, nw, n, o, n
                        device=gpu_device) for _ in range(nspin)]
     walk = ws[0] if nspin == 1 else tuple(ws)
     r_full = kernels.trotter(walk, v_full, e, 0.07, 6, arith="bf16x3")
-    e_blk = upper_blocks(e[:, :n].double()).float()
+    e_blk = minus_identity_pairs(upper_blocks(e[:, :n].double()).float(), n)
     r_blk = kernels.trotter(walk, v_blk, e_blk, 0.07, 6, arith="bf16x3")
     for a_, b_ in zip(r_full if nspin == 2 else (r_full,), r_blk if nspin == 2 else (r_blk,)):
         assert torch.equal(a_, b_)
