@@ -120,7 +120,7 @@ __global__ __launch_bounds__(256 * NC, 1) void trotter_kernel(
     int nw, int n, int o, int nspin, int ldv, int n_terms, double sqrt_dt,
     const R* __restrict__ E, const R* __restrict__ vhs, long long vplane,
     const double2* __restrict__ win_up, const double2* __restrict__ win_dn,
-    double2* __restrict__ wout_up, double2* __restrict__ wout_dn, int pair_map) {
+    double2* __restrict__ wout_up, double2* __restrict__ wout_dn, int pair_map, int ngrp) {
     using ACC = typename afq::mfma16<R>::acc;
     extern __shared__ __attribute__((aligned(16))) unsigned char lds_raw[];
     R* lds = reinterpret_cast<R*>(lds_raw);
@@ -131,10 +131,15 @@ __global__ __launch_bounds__(256 * NC, 1) void trotter_kernel(
     R* Tr = lds;
     R* Ti = lds + NP * TC;
 
+    // o > TC: the workgroups of one (walker, spin) take consecutive groups of
+    // TC columns (the Taylor recursion never mixes columns) and run back to
+    // back, reading the walker's VHS from L2 after the first.
+    const int grp = blockIdx.x % ngrp;
+    const int c0 = grp * TC;
     // Both spins of a walker go to the same XCD, close together in time, so
     // the second reads the walker's VHS from L2.
     int w, s;
-    const int b = blockIdx.x;
+    const int b = blockIdx.x / ngrp;
     if (nspin == 1) {
         w = b;
         s = 0;
@@ -159,7 +164,7 @@ __global__ __launch_bounds__(256 * NC, 1) void trotter_kernel(
 
     for (int e = tid; e < NP * TC; e += THREADS) {
         const int p = e / TC;
-        const int j = e - p * TC;
+        const int j = c0 + (e - p * TC);
         const double2 x = (p < n && j < o) ? Win[p * o + j] : make_double2(0.0, 0.0);
         Tr[e] = (R)x.x;
         Ti[e] = (R)x.y;
@@ -203,7 +208,7 @@ __global__ __launch_bounds__(256 * NC, 1) void trotter_kernel(
     store_T<R, NC, RPW>(Tr, Ti, rg, ct, lane, ntile, out);
     __syncthreads();
     tile_product<R, NC, RPW, false>(E, nullptr, ldv, n, ntile, Tr, Ti, rg, ct, lane, P);
-    const int col = ct * 16 + (lane & 15);
+    const int col = c0 + ct * 16 + (lane & 15);
 #pragma unroll
     for (int i = 0; i < RPW; ++i) {
         const int rt = rg + 4 * i;
@@ -225,9 +230,10 @@ int launch_trotter(int nw, int n, int o, int nspin, int ldv, int n_terms, double
     const size_t lds = (size_t)2 * NP * NC * 16 * sizeof(R);
     const int pair_map = (nspin == 2 && nw % 8 == 0) ? 1 : 0;
     if (lds > 64 * 1024) afq::allow_full_lds((const void*)trotter_kernel<R, NC, RPW>);
-    hipLaunchKernelGGL((trotter_kernel<R, NC, RPW>), dim3((unsigned)(nw * nspin)), dim3(256 * NC), lds,
+    const int ngrp = afq::ceil_div(o, NC * 16);
+    hipLaunchKernelGGL((trotter_kernel<R, NC, RPW>), dim3((unsigned)(nw * nspin * ngrp)), dim3(256 * NC), lds,
                        st, nw, n, o, nspin, ldv, n_terms, sqrt_dt, E, vhs,
-                       (long long)nw * n * ldv, wu, wd, ou, od, pair_map);
+                       (long long)nw * n * ldv, wu, wd, ou, od, pair_map, ngrp);
     return afq::check_launch("afq_trotter");
 }
 
@@ -502,11 +508,13 @@ int trotter_entry(const char* name, int nw, int n, int o, int nspin, int ldv, in
     AFQ_REQUIRE(nspin == 1 || (win_dn && wout_dn), "%s: null spin-down pointer", name);
     AFQ_REQUIRE(win_up != wout_up && (nspin == 1 || win_dn != wout_dn), "%s: output aliases input", name);
     const int NP = afq::round_up(n, 16);
-    const int nc = afq::ceil_div(o, 16);
+    // column tiles per workgroup: two (groups of 32 columns) once o > 16
+    const int nc = o > 16 ? 2 : 1;
     const int ntile = NP / 16;
     const int rpw = afq::ceil_div(ntile, 4);
-    AFQ_REQUIRE(nc <= 2, "%s: o=%d exceeds the supported maximum of 32", name, o);
+    AFQ_REQUIRE(o <= 64, "%s: o=%d exceeds the supported maximum of 64", name, o);
     AFQ_REQUIRE(rpw <= 5, "%s: n=%d exceeds the supported maximum of 320", name, n);
+    AFQ_REQUIRE((long long)nw * nspin * afq::ceil_div(o, 32) < (1LL << 31), "%s: too many walkers", name);
     AFQ_REQUIRE((size_t)2 * NP * nc * 16 * sizeof(R) <= 160 * 1024,
                 "%s: n=%d o=%d needs more than 160 KB of LDS", name, n, o);
     const double2* wu = (const double2*)win_up;

@@ -23,8 +23,9 @@ def as_device_f64(x, device=None) -> torch.Tensor:
 
 @dataclass(frozen=True)
 class ham_chol:
-    """h0 scalar, h1 (n, n), chol (n_fields, n, n); basis "restricted" only on
-    the HIP path (generalized-basis Hamiltonians raise at build time)."""
+    """h0 scalar, h1 (n, n), chol (n_fields, n, n) in the restricted basis, or
+    h1 (nso, nso), chol (n_fields, nso, nso) in a generalized (spin-orbital)
+    basis, nso = 2 norb (ham/chol.py:12-61)."""
 
     h0: float
     h1: torch.Tensor

@@ -178,12 +178,55 @@ def force_bias_kernel_u(walkers, _ham, meas_ctx: rhf_meas_ctx, trial_data, green
     return torch.complex(fb[0], fb[1])
 
 
+# occupied orbitals per spin up to which the energy runs in the fused kernels
+ENERGY_KERNEL_MAX_O = 32
+
+
+def _energy_library(g: torch.Tensor, nw: int, n: int, o: int, nspin: int, meas_ctx: rhf_meas_ctx,
+                    h0: float, budget_bytes: int = 1 << 29) -> torch.Tensor:
+    """meas/rhf.py:36-50 (restricted) and 75-105 (unrestricted) on the device's
+    library fp64 GEMMs, for shapes the fused energy kernels do not take
+    (o > 32 per spin): f_g = rot_g G^T of a chunk of walkers at a time,
+    (chunk, naux, o, o) complex, then the Coulomb traces c_g = tr f_g and the
+    exchange sum_g tr(f_g f_g).  Every precision measures here in fp64."""
+    gc = torch.complex(g[0], g[1]).view(nw, nspin, o, n)
+    naux = meas_ctx.rot_chol.shape[1]
+    out = torch.empty((nw,), dtype=torch.complex128, device=g.device)
+    chunk = max(1, budget_bytes // (2 * naux * o * o * 16))
+    for a in range(0, nw, chunk):
+        b = min(nw, a + chunk)
+        cs, exs, e1 = [], [], 0.0
+        for s in range(nspin):
+            rot = meas_ctx.rot_chol[s].reshape(naux * o, n)
+            gs = gc[a:b, s]                                    # (m, o, n)
+            gt = gs.transpose(1, 2).contiguous()               # (m, n, o)
+            if rot.is_complex():
+                f = torch.matmul(rot, gt)
+            else:
+                f = torch.complex(torch.matmul(rot, gt.real.contiguous()), torch.matmul(rot, gt.imag.contiguous()))
+            f = f.view(b - a, naux, o, o)
+            cs.append(torch.diagonal(f, dim1=-2, dim2=-1).sum(-1))              # (m, naux)
+            exs.append((f * f.transpose(-1, -2)).sum(dim=(-1, -2)).sum(-1))    # (m,)
+            e1 = e1 + (gs * meas_ctx.rot_h1[s]).sum(dim=(-1, -2))
+            del f
+        if nspin == 1:
+            e2 = 2.0 * (cs[0] * cs[0]).sum(-1) - exs[0]
+            e1 = 2.0 * e1
+        else:
+            e2 = ((cs[0] * cs[0]).sum(-1) + (cs[1] * cs[1]).sum(-1) + 2.0 * (cs[0] * cs[1]).sum(-1)
+                  - exs[0] - exs[1]) / 2.0
+        out[a:b] = h0 + e1 + e2
+    return out
+
+
 def _energy(walkers, ham_data: ham_chol, meas_ctx: rhf_meas_ctx, trial_data, greens=None):
     nspin = _nspin(walkers)
     w0 = walkers[0] if nspin == 2 else walkers
     nw, n, o = w0.shape
     if greens is None or greens.greens is None:
         greens = trial_rhf.greens(walkers, trial_data)
+    if o > ENERGY_KERNEL_MAX_O or meas_ctx.rot_chol.is_complex():
+        return _energy_library(greens.greens, nw, n, o, nspin, meas_ctx, float(ham_data.h0))
     rot_pad, rot_h1 = meas_ctx.energy_operands(nspin)
     return kernels.energy(greens.greens, nw, n, o, nspin, rot_pad, rot_h1, float(ham_data.h0))
 

@@ -52,23 +52,44 @@ def _as_total_rdm1_restricted(dm: torch.Tensor) -> torch.Tensor:
     return dm
 
 
+def spin_orbital_rdm1(trial_rdm1: torch.Tensor) -> torch.Tensor:
+    """A generalized basis takes the trial's one-body density matrix in the
+    spin-orbital basis (nso, nso); a per-spin stack (2, n, n) (the RHF / UHF
+    trials' get_rdm1) becomes its block diagonal diag(dm_up, dm_dn)."""
+    if trial_rdm1.ndim == 3 and trial_rdm1.shape[0] == 2:
+        n = trial_rdm1.shape[1]
+        out = torch.zeros((2 * n, 2 * n), dtype=trial_rdm1.dtype, device=trial_rdm1.device)
+        out[:n, :n] = trial_rdm1[0]
+        out[n:, n:] = trial_rdm1[1]
+        return out
+    return trial_rdm1
+
+
 def _mf_shifts(ham_data: ham_chol, trial_rdm1: torch.Tensor) -> torch.Tensor:
-    """prop/chol_afqmc_ops.py:59-63: 1j * tr(L_g dm)."""
-    dm = _as_total_rdm1_restricted(torch.as_tensor(trial_rdm1, device=ham_data.device,
-                                                   dtype=torch.float64))
+    """prop/chol_afqmc_ops.py:59-63: 1j * tr(L_g dm), dm the total density
+    (restricted basis) or the spin-orbital one (generalized basis)."""
+    dm = torch.as_tensor(trial_rdm1, device=ham_data.device, dtype=torch.float64)
+    if ham_data.basis == "restricted":
+        dm = _as_total_rdm1_restricted(dm)
+    else:
+        dm = spin_orbital_rdm1(dm)
     return 1.0j * torch.einsum("gij,ji->g", ham_data.chol, dm).to(torch.complex128)
 
 
 def _build_prop_ctx(ham_data: ham_chol, trial_rdm1, dt: float) -> chol_afqmc_ctx:
-    """prop/chol_afqmc_ops.py:77-103."""
+    """prop/chol_afqmc_ops.py:77-103: in the restricted basis
+    h1_eff = h1 - 1/2 sum_g L_g L_g - sum_g Re(i mf_g) L_g; a generalized
+    basis keeps h1_eff = h1 (prop/chol_afqmc_ops.py:86-94)."""
     mf = _mf_shifts(ham_data, trial_rdm1)
     h0_prop = complex(-ham_data.h0 - 0.5 * torch.sum(mf ** 2).item())
     chol = ham_data.chol
     g, n, _ = chol.shape
-    v0m = 0.5 * torch.einsum("gik,gkj->ij", chol, chol)
-    mf_r = (1.0j * mf).real
-    v1m = torch.einsum("g,gik->ik", mf_r, chol)
-    h1_eff = ham_data.h1 - v0m - v1m
+    h1_eff = ham_data.h1
+    if ham_data.basis == "restricted":
+        v0m = 0.5 * torch.einsum("gik,gkj->ij", chol, chol)
+        mf_r = (1.0j * mf).real
+        v1m = torch.einsum("g,gik->ik", mf_r, chol)
+        h1_eff = ham_data.h1 - v0m - v1m
     exp_h1_half = torch.linalg.matrix_exp(-0.5 * dt * h1_eff)
     # the exponential of a symmetric matrix, made exactly symmetric (the
     # rounding of the matrix products leaves ~1e-16 asymmetry), so that every
@@ -144,8 +165,9 @@ PRECISIONS = ("fp64", "fp32", "bf16x3")
 
 def make_chol_afqmc_ops(ham_data: ham_chol, walker_kind: str,
                         precision: str = "fp64") -> chol_afqmc_ops:
-    """prop/chol_afqmc_ops.py:215-262 (restricted basis; restricted, unrestricted or
-    generalized walkers)."""
+    """prop/chol_afqmc_ops.py:215-262: a restricted basis with restricted,
+    unrestricted or generalized walkers, or a generalized (spin-orbital) basis
+    with generalized walkers."""
     if precision not in PRECISIONS:
         raise ValueError(f"unknown precision {precision!r}; expected one of {PRECISIONS}")
     walker_kind = walker_kind.lower()
@@ -153,11 +175,13 @@ def make_chol_afqmc_ops(ham_data: ham_chol, walker_kind: str,
     n = int(ham_data.chol.shape[1])
     if ham_data.basis == "generalized" and walker_kind != "generalized":
         raise ValueError("ham_data.basis='generalized' requires walker_kind='generalized'")
-    if ham_data.basis == "generalized":
-        raise NotImplementedError("generalized-basis Hamiltonians are not implemented on the HIP path")
     if walker_kind not in ("restricted", "unrestricted", "generalized"):
         raise ValueError(f"unknown walker_kind: {walker_kind}")
-    generalized = walker_kind == "generalized"
+    # generalized walkers in the restricted basis: both spin blocks of the
+    # (2n, o') walker with the n x n VHS; in a generalized basis the (nso, o')
+    # walker is propagated whole with the nso x nso VHS, as a restricted walker
+    # of nso orbitals (prop/chol_afqmc_ops.py:258-262)
+    generalized = walker_kind == "generalized" and ham_data.basis == "restricted"
 
     # the VHS contraction against the upper blocks only, each entry stored twice
     chol_s = chol_sym_rows(ham_data.chol)

@@ -195,9 +195,14 @@ class prop_ctx:
     h0_prop: complex
 
 
-def mf_shifts(chol: np.ndarray, rdm1: np.ndarray) -> np.ndarray:
-    """_mf_shifts (prop/chol_afqmc_ops.py:53-63): 1j * tr(L_g dm_total)."""
-    dm = rdm1[0] + rdm1[1] if rdm1.ndim == 3 and rdm1.shape[0] == 2 else rdm1
+def mf_shifts(chol: np.ndarray, rdm1: np.ndarray, basis: str = "restricted") -> np.ndarray:
+    """_mf_shifts (prop/chol_afqmc_ops.py:53-63): 1j * tr(L_g dm) with dm the
+    total density (restricted basis) or the given spin-orbital one
+    (generalized basis)."""
+    if basis == "restricted":
+        dm = rdm1[0] + rdm1[1] if rdm1.ndim == 3 and rdm1.shape[0] == 2 else rdm1
+    else:
+        dm = rdm1
     return 1.0j * np.einsum("gij,ji->g", chol, dm, optimize=True)
 
 
@@ -208,14 +213,17 @@ def get_rdm1(trial) -> np.ndarray:
 
 
 def build_prop_ctx(h0: float, h1: np.ndarray, chol: np.ndarray, rdm1: np.ndarray,
-                   dt: float) -> prop_ctx:
-    """_build_prop_ctx (prop/chol_afqmc_ops.py:77-103), restricted basis."""
-    mf = mf_shifts(chol, rdm1)
+                   dt: float, basis: str = "restricted") -> prop_ctx:
+    """_build_prop_ctx (prop/chol_afqmc_ops.py:77-103); a generalized basis
+    skips the v0 / v1 corrections (:86-94)."""
+    mf = mf_shifts(chol, rdm1, basis)
     h0_prop = -h0 - 0.5 * np.sum(mf ** 2)
-    v0m = 0.5 * np.einsum("gik,gkj->ij", chol, chol, optimize=True)
-    mf_r = (1.0j * mf).real
-    v1m = np.einsum("g,gik->ik", mf_r, chol, optimize=True)
-    h1_eff = h1 - v0m - v1m
+    h1_eff = h1
+    if basis == "restricted":
+        v0m = 0.5 * np.einsum("gik,gkj->ij", chol, chol, optimize=True)
+        mf_r = (1.0j * mf).real
+        v1m = np.einsum("g,gik->ik", mf_r, chol, optimize=True)
+        h1_eff = h1 - v0m - v1m
     exp_h1_half = scipy.linalg.expm(-0.5 * dt * h1_eff)
     return prop_ctx(dt=float(dt), sqrt_dt=float(np.sqrt(dt)), exp_h1_half=exp_h1_half,
                     mf_shifts=mf, h0_prop=complex(h0_prop))

@@ -69,3 +69,63 @@ def test_init_walkers_and_no_energy():
     np.testing.assert_allclose(abs(orc.overlap(w, trial, "generalized")), 1.0, rtol=1e-12)
     with pytest.raises(KeyError):
         orc.energy(w, h0, trial, orc.build_meas_ctx(h1, chol, trial), "generalized")
+
+
+# --------------------------------------------------------------------------
+# generalized (spin-orbital) basis: prop/chol_afqmc_ops.py:77-103,215-262
+# --------------------------------------------------------------------------
+
+def _spin_orbital(h1, chol):
+    n = h1.shape[0]
+    h1s = np.zeros((2 * n, 2 * n))
+    h1s[:n, :n] = h1s[n:, n:] = h1
+    ls = np.zeros((chol.shape[0], 2 * n, 2 * n))
+    ls[:, :n, :n] = ls[:, n:, n:] = chol
+    return h1s, ls
+
+
+def test_generalized_basis_prop_ctx_closed_forms():
+    """mf_g = i tr(L_g dm) with the spin-orbital dm as given, h0_prop =
+    -h0 - 1/2 sum mf^2, and exp_h1_half = exp(-dt/2 h1) with no v0 / v1
+    correction (prop/chol_afqmc_ops.py:86-94): by eigendecomposition."""
+    rng = np.random.default_rng(2)
+    nso, g, dt = 8, 5, 0.03
+    a = rng.standard_normal((nso, nso))
+    h1 = 0.5 * (a + a.T)
+    b = rng.standard_normal((g, nso, nso))
+    chol = 0.5 * (b + b.transpose(0, 2, 1))
+    c = np.linalg.qr(rng.standard_normal((nso, 3)))[0]
+    dm = c @ c.T
+    p = orc.build_prop_ctx(0.6, h1, chol, dm, dt, basis="generalized")
+    mf = 1j * np.array([np.trace(chol[i] @ dm) for i in range(g)])
+    np.testing.assert_allclose(p.mf_shifts, mf, rtol=1e-12)
+    np.testing.assert_allclose(p.h0_prop, -0.6 + 0.5 * np.sum(mf.imag ** 2), rtol=1e-12)
+    ev, vec = np.linalg.eigh(h1)
+    np.testing.assert_allclose(p.exp_h1_half, vec @ np.diag(np.exp(-0.5 * dt * ev)) @ vec.T, rtol=1e-10,
+                               atol=1e-13)
+
+
+def test_generalized_basis_embedding_equals_restricted_generalized_walkers():
+    """A restricted Hamiltonian written in the spin-orbital basis (block
+    diagonal L and h1, with h1 the restricted h1_eff since a generalized basis
+    applies no correction) propagates a generalized walker exactly as the
+    restricted basis propagates it (both spin blocks with the n x n VHS)."""
+    h0, h1, chol, trial = _setup()
+    n, o = trial[0].shape
+    dt = 0.01
+    rdm = orc.get_rdm1(trial)
+    pr = orc.build_prop_ctx(h0, h1, chol, rdm, dt)
+    v0m = 0.5 * np.einsum("gik,gkj->ij", chol, chol)
+    v1m = np.einsum("g,gik->ik", (1j * pr.mf_shifts).real, chol)
+    h1s, ls = _spin_orbital(h1 - v0m - v1m, chol)
+    dms = np.zeros((2 * n, 2 * n))
+    dms[:n, :n], dms[n:, n:] = rdm[0], rdm[1]
+    ps = orc.build_prop_ctx(h0, h1s, ls, dms, dt, basis="generalized")
+    np.testing.assert_allclose(ps.mf_shifts, pr.mf_shifts, rtol=1e-12)
+    assert abs(ps.h0_prop - pr.h0_prop) < 1e-12 * abs(pr.h0_prop)
+    rng = np.random.default_rng(6)
+    w = _rand(rng, 3, 2 * n, 2 * o)
+    x = _rand(rng, 3, chol.shape[0])
+    a = orc.apply_trotter(w, x, ps, ls, 6, "restricted")
+    b = orc.apply_trotter(w, x, pr, chol, 6, "generalized")
+    np.testing.assert_allclose(a, b, rtol=1e-11, atol=1e-13)

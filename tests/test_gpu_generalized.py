@@ -109,3 +109,34 @@ def test_mixed_precision_trotter(gpu_device, precision, tol):
     w1 = p_ops.apply_trotter(c["wdev"], torch.tensor(x, device=gpu_device), p_ctx, 6).cpu().numpy()
     w1_ref = orc.apply_trotter(c["w0"], x, c["sd"].pctx, c["s"].chol, 6, "generalized")
     assert _rel(w1, w1_ref) < tol
+
+
+def test_generalized_basis_trotter(gpu_device):
+    """A generalized-basis Hamiltonian (spin-orbital h1, L of size nso = 2n):
+    prop ctx (no v0 / v1 correction) and the Trotter step of generalized
+    walkers against the checker (1e-10), and the reference's factory checks."""
+    import torch
+    from ad_afqmc_prototype_amd.ham.chol import ham_chol
+    from ad_afqmc_prototype_amd.prop.chol_afqmc_ops import make_chol_afqmc_ops
+    rng = np.random.default_rng(8)
+    nso, g, o2, nw, dt = 2 * N, 30, 2 * O, 12, 0.01
+    a = rng.standard_normal((nso, nso))
+    h1 = 0.3 * (a + a.T)
+    b = rng.standard_normal((g, nso, nso))
+    chol = 0.1 * (b + b.transpose(0, 2, 1))
+    ham = ham_chol.from_arrays(0.4, h1, chol, basis="generalized", device=gpu_device)
+    with pytest.raises(ValueError):
+        make_chol_afqmc_ops(ham, "restricted")
+    c = np.linalg.qr(rng.standard_normal((nso, o2)))[0]
+    dm = c @ c.T
+    w = c[None] + 0.2 * (rng.standard_normal((nw, nso, o2)) + 1j * rng.standard_normal((nw, nso, o2)))
+    x = rng.standard_normal((nw, g))
+    ref_ctx = orc.build_prop_ctx(0.4, h1, chol, dm, dt, basis="generalized")
+    ref = orc.apply_trotter(w, x, ref_ctx, chol, 6, "restricted")
+    for prec, tol in (("fp64", 1e-10), ("fp32", 2e-5), ("bf16x3", 5e-4)):
+        ops = make_chol_afqmc_ops(ham, "generalized", precision=prec)
+        ctx = ops.build_prop_ctx(torch.tensor(dm, device=gpu_device), dt)
+        assert _rel(ctx.mf_shifts.cpu().numpy(), ref_ctx.mf_shifts) < 1e-12
+        assert abs(ctx.h0_prop - ref_ctx.h0_prop) < 1e-12 * abs(ref_ctx.h0_prop)
+        out = ops.apply_trotter(torch.tensor(w, device=gpu_device), torch.tensor(x, device=gpu_device), ctx, 6)
+        assert _rel(out.cpu().numpy(), ref) < tol, prec

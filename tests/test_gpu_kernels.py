@@ -16,6 +16,10 @@ SHAPES = [
     (1, 40, 7, 150, True, "unrestricted"),
     (5, 45, 11, 130, False, "unrestricted"),
     (2, 200, 32, 1000, False, "restricted"),
+    # more than 32 occupied orbitals per spin: column groups in the Trotter,
+    # the block Green's function, the library-GEMM energy
+    (8, 60, 48, 90, True, "unrestricted"),
+    (9, 70, 64, 80, False, "restricted"),
 ]
 
 

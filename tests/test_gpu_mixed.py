@@ -219,6 +219,7 @@ def _planes(e, v, ldv):
     (16, 200, 32, 1),  # two k-blocks in LDS (the 200-orbital shape)
     (3, 224, 17, 2),   # largest on-chip size
     (2, 250, 8, 1),    # beyond on-chip capacity: the fp32 step runs
+    (6, 80, 48, 2),    # more than 32 occupied orbitals (three column tiles)
 ])
 def test_trotter_bf16x3_vs_fp64(gpu_device, nw, n, o, nspin):
     """Split-bf16 Trotter step against the fp64 step (pinned to the checker)

@@ -61,6 +61,7 @@ def _cmp_state(gs, st, wtol=TOL):
     dict(seed=1, n=120, o=20, g=600, nw=48, uhf=True, kind="unrestricted", dt=0.005, steps=2),
     dict(seed=4, n=60, o=10, g=240, nw=40, uhf=False, kind="unrestricted", dt=0.01, steps=3),
     dict(seed=2, n=200, o=32, g=1000, nw=24, uhf=False, kind="restricted", dt=0.005, steps=1),
+    dict(seed=9, n=60, o=48, g=90, nw=12, uhf=True, kind="unrestricted", dt=0.01, steps=2),
 ])
 def test_step_trajectory_matches_checker(gpu_device, case):
     p = make_pair(case["seed"], case["n"], case["o"], case["g"], uhf=case["uhf"], kind=case["kind"],
