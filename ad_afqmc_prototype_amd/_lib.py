@@ -39,6 +39,7 @@ _SIGNATURES = {
     "afq_overlap_greens": [c_int, c_int, c_int, c_int, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp],
     "afq_overlap_greens_split": [c_int, c_int, c_int, c_int, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp,
                                  c_ll, c_vp],
+    "afq_overlap_greens_ctrial": [c_int, c_int, c_int, c_int, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp],
     "afq_overlap_greens_bf16x3": [c_int, c_int, c_int, c_int, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_ll, c_vp],
     "afq_force_bias": [c_int, c_int, c_int, c_double, c_vp, c_vp, c_vp, c_vp],
     "afq_field_shifts": [c_int, c_int, c_double, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp],
@@ -85,7 +86,7 @@ _SIGNATURES = {
     "afq_unpack_pairs": [c_int, c_int, c_vp, c_ll, c_vp, c_vp],
 }
 
-ABI_VERSION = 11
+ABI_VERSION = 12
 
 
 class HipLibraryError(RuntimeError):

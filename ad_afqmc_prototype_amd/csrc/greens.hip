@@ -43,7 +43,9 @@ __device__ __forceinline__ double2 crecip(double2 a) {
 
 // One (walker, spin): W (n x o complex), Cm (n x o real) -> G^T rows into
 // gre / gim (o x n each), log det into *ld.
-template <int MAXO>
+// CT: complex trial orbitals, Cm points at interleaved (re, im) pairs and
+// m = C^H w = sum_p (Cr - i Ci)(wr + i wi).
+template <int MAXO, bool CT = false>
 __device__ __forceinline__ void greens_body(int n, int o, const double2* __restrict__ W,
                                             const double* __restrict__ Cm, double* __restrict__ gre,
                                             double* __restrict__ gim, double2* __restrict__ ld) {
@@ -74,10 +76,19 @@ __device__ __forceinline__ void greens_body(int n, int o, const double2* __restr
                 for (int kk = 0; kk < 4; ++kk) {
                     const int p = kb * 16 + lk + kk;
                     const bool okp = p < n;
-                    const double a = (okp && i < o) ? Cm[p * o + i] : 0.0;
                     const double2 x = (okp && j < o) ? W[p * o + j] : make_double2(0.0, 0.0);
-                    ar = afq::mfma_f64(a, x.x, ar);
-                    ai = afq::mfma_f64(a, x.y, ai);
+                    if constexpr (CT) {
+                        const double2 a = (okp && i < o) ? reinterpret_cast<const double2*>(Cm)[p * o + i]
+                                                         : make_double2(0.0, 0.0);
+                        ar = afq::mfma_f64(a.x, x.x, ar);
+                        ar = afq::mfma_f64(a.y, x.y, ar);
+                        ai = afq::mfma_f64(a.x, x.y, ai);
+                        ai = afq::mfma_f64(-a.y, x.x, ai);
+                    } else {
+                        const double a = (okp && i < o) ? Cm[p * o + i] : 0.0;
+                        ar = afq::mfma_f64(a, x.x, ar);
+                        ai = afq::mfma_f64(a, x.y, ai);
+                    }
                 }
             }
 #pragma unroll
@@ -217,7 +228,7 @@ struct split_rows {
     uint16_t *hre, *lre, *him, *lim;
 };
 
-template <int MAXO, bool SPLIT = false>
+template <int MAXO, bool SPLIT = false, bool CT = false>
 __device__ __forceinline__ void greens_wave_body(int n, int o, const double2* __restrict__ W,
                                                  const double* __restrict__ Cm, double* __restrict__ gre,
                                                  double* __restrict__ gim, double2* __restrict__ ld,
@@ -236,9 +247,12 @@ __device__ __forceinline__ void greens_wave_body(int n, int o, const double2* __
         for (int ti = 0; ti < NT; ++ti)
 #pragma unroll
             for (int tj = 0; tj < NT; ++tj) acc[ti][tj][0] = acc[ti][tj][1] = f64x4{0.0, 0.0, 0.0, 0.0};
-        double a[NT][4], an[NT][4];
+        // complex trial (CT): Cm holds interleaved (re, im) pairs and
+        // m = C^H w = sum_p (Cr - i Ci)(wr + i wi)
+        constexpr int NA = CT ? 2 : 1;
+        double a[NT][4][NA], an[NT][4][NA];
         double2 x[NT][4], xn[NT][4];
-        auto load = [&](int kb, double (&av)[NT][4], double2 (&xv)[NT][4]) {
+        auto load = [&](int kb, double (&av)[NT][4][NA], double2 (&xv)[NT][4]) {
 #pragma unroll
             for (int kk = 0; kk < 4; ++kk) {
                 const int p = kb * 16 + lk + kk;
@@ -246,7 +260,14 @@ __device__ __forceinline__ void greens_wave_body(int n, int o, const double2* __
 #pragma unroll
                 for (int t = 0; t < NT; ++t) {
                     const int c = t * 16 + lr;
-                    av[t][kk] = (okp && c < o) ? Cm[p * o + c] : 0.0;
+                    if constexpr (CT) {
+                        const double2 cv = (okp && c < o) ? reinterpret_cast<const double2*>(Cm)[p * o + c]
+                                                          : make_double2(0.0, 0.0);
+                        av[t][kk][0] = cv.x;
+                        av[t][kk][NA - 1] = cv.y;
+                    } else {
+                        av[t][kk][0] = (okp && c < o) ? Cm[p * o + c] : 0.0;
+                    }
                     xv[t][kk] = (okp && c < o) ? W[p * o + c] : make_double2(0.0, 0.0);
                 }
             }
@@ -260,14 +281,19 @@ __device__ __forceinline__ void greens_wave_body(int n, int o, const double2* __
                 for (int ti = 0; ti < NT; ++ti)
 #pragma unroll
                     for (int tj = 0; tj < NT; ++tj) {
-                        acc[ti][tj][0] = afq::mfma_f64(a[ti][kk], x[tj][kk].x, acc[ti][tj][0]);
-                        acc[ti][tj][1] = afq::mfma_f64(a[ti][kk], x[tj][kk].y, acc[ti][tj][1]);
+                        acc[ti][tj][0] = afq::mfma_f64(a[ti][kk][0], x[tj][kk].x, acc[ti][tj][0]);
+                        acc[ti][tj][1] = afq::mfma_f64(a[ti][kk][0], x[tj][kk].y, acc[ti][tj][1]);
+                        if constexpr (CT) {
+                            acc[ti][tj][0] = afq::mfma_f64(a[ti][kk][NA - 1], x[tj][kk].y, acc[ti][tj][0]);
+                            acc[ti][tj][1] = afq::mfma_f64(-a[ti][kk][NA - 1], x[tj][kk].x, acc[ti][tj][1]);
+                        }
                     }
 #pragma unroll
             for (int t = 0; t < NT; ++t)
 #pragma unroll
                 for (int kk = 0; kk < 4; ++kk) {
-                    a[t][kk] = an[t][kk];
+#pragma unroll
+                    for (int e = 0; e < NA; ++e) a[t][kk][e] = an[t][kk][e];
                     x[t][kk] = xn[t][kk];
                 }
         }
@@ -806,6 +832,38 @@ __global__ __launch_bounds__(THREADS) void greens_gj_split_kernel(
                              split_rows{ghi + rre, glo + rre, ghi + rim, glo + rim});
 }
 
+// Complex trial orbitals (interleaved complex128 C), fp64 rows only.
+template <int MAXO>
+__global__ __launch_bounds__(THREADS) void greens_wave_ctrial_kernel(
+    int nw, int n, int o, int nspin, const double2* __restrict__ wup, const double2* __restrict__ wdn,
+    const double* __restrict__ cup, const double* __restrict__ cdn, double* __restrict__ greens,
+    double2* __restrict__ logdet) {
+    __shared__ greens_wave_lds<MAXO> L[THREADS / 64];
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int item = blockIdx.x * (THREADS / 64) + wave;
+    if (item >= nw * nspin) return;
+    const int w = item / nspin;
+    const int s = item - w * nspin;
+    const long long ldg = (long long)nspin * o * n;
+    double* gre = greens + (long long)w * ldg + (long long)s * o * n;
+    greens_wave_body<MAXO, false, true>(n, o, (s == 0 ? wup : wdn) + (long long)w * n * o, s == 0 ? cup : cdn, gre,
+                                        gre + (long long)nw * ldg, logdet + (long long)w * nspin + s, L[wave]);
+}
+
+template <int MAXO>
+__global__ __launch_bounds__(THREADS) void greens_ctrial_kernel(
+    int nw, int n, int o, int nspin, const double2* __restrict__ wup,
+    const double2* __restrict__ wdn, const double* __restrict__ cup, const double* __restrict__ cdn,
+    double* __restrict__ greens, double2* __restrict__ logdet) {
+    const int b = blockIdx.x;
+    const int w = b / nspin;
+    const int s = b - w * nspin;
+    const long long ldg = (long long)nspin * o * n;
+    double* gre = greens + (long long)w * ldg + (long long)s * o * n;
+    greens_body<MAXO, true>(n, o, (s == 0 ? wup : wdn) + (long long)w * n * o, s == 0 ? cup : cdn, gre,
+                            gre + (long long)nw * ldg, logdet + (long long)w * nspin + s);
+}
+
 template <int MAXO>
 __global__ __launch_bounds__(THREADS) void greens_kernel(
     int nw, int n, int o, int nspin, const double2* __restrict__ wup,
@@ -957,4 +1015,31 @@ extern "C" int afq_overlap_greens_bf16x3(int nw, int n, int o, int nspin, const 
         hipLaunchKernelGGL(greens_gj_split_kernel<2>, grid, dim3(THREADS), 0, st, nw, n, o, nspin, wu, wd, trial_up,
                            cd, (double2*)logdet, (uint16_t*)g_hi, (uint16_t*)g_lo, ldk);
     return afq::check_launch("afq_overlap_greens_bf16x3");
+}
+
+extern "C" int afq_overlap_greens_ctrial(int nw, int n, int o, int nspin, const double* walkers_up,
+                                         const double* walkers_dn, const double* trial_up,
+                                         const double* trial_dn, double* greens, double* logdet, void* stream) {
+    AFQ_REQUIRE(nw > 0 && n > 0 && o > 0 && o <= n, "afq_overlap_greens_ctrial: bad sizes nw=%d n=%d o=%d", nw, n,
+                o);
+    AFQ_REQUIRE(nspin == 1 || nspin == 2, "afq_overlap_greens_ctrial: nspin must be 1 or 2");
+    AFQ_REQUIRE(walkers_up && trial_up && greens && logdet, "afq_overlap_greens_ctrial: null pointer");
+    AFQ_REQUIRE(nspin == 1 || (walkers_dn && trial_dn), "afq_overlap_greens_ctrial: null spin-down pointer");
+    AFQ_REQUIRE(o <= 64, "afq_overlap_greens_ctrial: o=%d exceeds the supported maximum of 64", o);
+    hipStream_t st = (hipStream_t)stream;
+    const double2* wu = (const double2*)walkers_up;
+    const double2* wd = (const double2*)(nspin == 2 ? walkers_dn : walkers_up);
+    const double* cd = nspin == 2 ? trial_dn : trial_up;
+    double2* ld = (double2*)logdet;
+    const dim3 wgrid((unsigned)afq::ceil_div(nw * nspin, THREADS / 64));
+    if (o <= 16)
+        hipLaunchKernelGGL(greens_wave_ctrial_kernel<16>, wgrid, dim3(THREADS), 0, st, nw, n, o, nspin, wu, wd,
+                           trial_up, cd, greens, ld);
+    else if (o <= 32)
+        hipLaunchKernelGGL(greens_wave_ctrial_kernel<32>, wgrid, dim3(THREADS), 0, st, nw, n, o, nspin, wu, wd,
+                           trial_up, cd, greens, ld);
+    else
+        hipLaunchKernelGGL(greens_ctrial_kernel<64>, dim3((unsigned)(nw * nspin)), dim3(THREADS), 0, st, nw, n, o,
+                           nspin, wu, wd, trial_up, cd, greens, ld);
+    return afq::check_launch("afq_overlap_greens_ctrial");
 }

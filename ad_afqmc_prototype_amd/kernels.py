@@ -252,10 +252,27 @@ def overlap_greens(walkers, trial_up, trial_dn=None, greens=None, logdet=None, s
     dev = wu.device
     if wd is not None:
         _check(wd, c128, (nw, n, o), "walkers_dn", dev)
-    _check(trial_up, f64, (n, o), "trial_up", dev)
+    ctrial = trial_up.is_complex() or (trial_dn is not None and trial_dn.is_complex())
+    tdt = c128 if ctrial else f64
+    if ctrial:
+        trial_up = trial_up.to(c128)
+        trial_dn = None if trial_dn is None else trial_dn.to(c128)
+    _check(trial_up, tdt, (n, o), "trial_up", dev)
     if nspin == 2:
         trial_dn = trial_up if trial_dn is None else trial_dn
-        _check(trial_dn, f64, (n, o), "trial_dn", dev)
+        _check(trial_dn, tdt, (n, o), "trial_dn", dev)
+    if ctrial:
+        # complex trial orbitals: m = C^H w in the ctrial kernel, the bf16
+        # planes (if asked) split from its fp64 rows
+        if greens is None:
+            greens = torch.empty((2, nw, nspin * o * n), dtype=f64, device=dev)
+        if logdet is None:
+            logdet = torch.empty((nw, nspin), dtype=c128, device=dev)
+        _call("afq_overlap_greens_ctrial", wu, nw, n, o, nspin, wu.data_ptr(), _ptr(wd), trial_up.data_ptr(),
+              _ptr(trial_dn if nspin == 2 else None), greens.data_ptr(), logdet.data_ptr(), _stream(wu))
+        if split:
+            return greens, logdet, split_bf16(greens.view(2 * nw, nspin * o * n))
+        return greens, logdet
     if greens is None:
         greens = torch.empty((2, nw, nspin * o * n), dtype=f64, device=dev)
     if logdet is None:

@@ -36,7 +36,8 @@ class rhf_meas_ctx:
 
     def fb_operand(self, nspin: int):
         """B operand of the force-bias GEMM: float64 (naux, k); float32 copy in
-        fp32 mode; (hi, lo) bf16 planes (naux, k padded to 8) in bf16x3 mode."""
+        fp32 mode; (hi, lo) bf16 planes (naux, k padded to 8) in bf16x3 mode.
+        For complex trial orbitals a pair (real part, imaginary part) of them."""
         key = ("fb", nspin)
         if key not in self._cache:
             if nspin == 1:
@@ -45,12 +46,20 @@ class rhf_meas_ctx:
                 g = self.rot_chol.shape[1]
                 op = torch.cat([self.rot_chol[0].reshape(g, -1),
                                 self.rot_chol[1].reshape(g, -1)], dim=1).contiguous()
-            if self.precision == "fp32":
-                op = kernels.convert_f32(op)
-            elif self.precision == "bf16x3":
-                op = kernels.split_bf16(op)
-            self._cache[key] = op
+
+            def fmt(x):
+                x = x.contiguous()
+                if self.precision == "fp32":
+                    return kernels.convert_f32(x)
+                if self.precision == "bf16x3":
+                    return kernels.split_bf16(x)
+                return x
+            self._cache[key] = (fmt(op.real), fmt(op.imag)) if op.is_complex() else fmt(op)
         return self._cache[key]
+
+    @property
+    def is_complex(self) -> bool:
+        return self.rot_chol.is_complex()
 
     def energy_operands(self, nspin: int):
         """(rot_chol_pad, rot_h1 (nspin, o, n)) for the energy kernels:
@@ -81,8 +90,9 @@ def build_meas_ctx(ham_data: ham_chol, trial_data, precision: str = "fp64") -> r
     if ham_data.basis != "restricted":
         raise ValueError("RHF meas_ops currently assumes ham_chol.basis == 'restricted'.")
     cs = [trial_data.mo_up] if trial_data.restricted else [trial_data.mo_up, trial_data.mo_dn]
-    rot_h1 = [c.T @ ham_data.h1 for c in cs]
-    rot_chol = [torch.einsum("pi,gpj->gij", c, ham_data.chol).contiguous() for c in cs]
+    # C^H h1 and C^H L_g (complex for complex trial orbitals)
+    rot_h1 = [c.conj().T @ ham_data.h1.to(c.dtype) for c in cs]
+    rot_chol = [torch.einsum("pi,gpj->gij", c.conj(), ham_data.chol.to(c.dtype)).contiguous() for c in cs]
     if trial_data.restricted:
         rh = torch.stack([rot_h1[0], rot_h1[0]])
         rc = rot_chol[0].unsqueeze(0).expand(2, *rot_chol[0].shape)
@@ -121,9 +131,25 @@ def _force_bias_split(split: tuple, nw: int, k: int, meas_ctx: rhf_meas_ctx, nsp
     op = meas_ctx.fb_operand(nspin)
     naux = meas_ctx.rot_chol.shape[1]
     gh, gl = split
+    if meas_ctx.is_complex:
+        return _combine_complex_fb([_gemm_planes_bf16x3(gh, gl, o_, nw, naux, alpha) for o_ in op])
     fb = torch.empty((2, nw, naux), dtype=torch.float64, device=gh.device)
     kernels.gemm_bf16x3_nt(gh, gl, op[0], op[1], fb.view(2 * nw, naux), alpha=alpha, tag="force_bias")
     return fb
+
+
+def _gemm_planes_bf16x3(gh, gl, op, nw, naux, alpha):
+    out = torch.empty((2, nw, naux), dtype=torch.float64, device=gh.device)
+    kernels.gemm_bf16x3_nt(gh, gl, op[0], op[1], out.view(2 * nw, naux), alpha=alpha, tag="force_bias")
+    return out
+
+
+def _combine_complex_fb(parts):
+    """Force-bias planes with complex half-rotated Cholesky R = Rr + i Ri from
+    the products of the Green's planes (Gr, Gi) with Rr and with Ri:
+    fb = (Gr Rr - Gi Ri) + i (Gr Ri + Gi Rr)."""
+    pr, pi = parts
+    return torch.stack([pr[0] - pi[1], pi[0] + pr[1]]).contiguous()
 
 
 def force_bias_planes_g(walkers, meas_ctx: rhf_meas_ctx, trial_data, greens=None) -> torch.Tensor:
@@ -144,6 +170,14 @@ def _force_bias_rows(g: torch.Tensor, meas_ctx: rhf_meas_ctx, nspin: int, alpha:
     (bf16x3: ``split``, the bf16 planes of g's rows when the Green's function
     kernel made them, else split here)."""
     op = meas_ctx.fb_operand(nspin)
+    if meas_ctx.is_complex:
+        # two real products (real and imaginary part of C^H L) combined
+        return _combine_complex_fb([_force_bias_real(g, o_, meas_ctx, alpha, split) for o_ in op])
+    return _force_bias_real(g, op, meas_ctx, alpha, split)
+
+
+def _force_bias_real(g: torch.Tensor, op, meas_ctx: rhf_meas_ctx, alpha: float,
+                     split: tuple | None = None) -> torch.Tensor:
     if meas_ctx.precision == "fp64":
         return kernels.force_bias(g, op, alpha)
     _, nw, k = g.shape
@@ -225,7 +259,7 @@ def _energy(walkers, ham_data: ham_chol, meas_ctx: rhf_meas_ctx, trial_data, gre
     nw, n, o = w0.shape
     if greens is None or greens.greens is None:
         greens = trial_rhf.greens(walkers, trial_data)
-    if o > ENERGY_KERNEL_MAX_O or meas_ctx.rot_chol.is_complex():
+    if o > ENERGY_KERNEL_MAX_O or meas_ctx.is_complex:
         return _energy_library(greens.greens, nw, n, o, nspin, meas_ctx, float(ham_data.h0))
     rot_pad, rot_h1 = meas_ctx.energy_operands(nspin)
     return kernels.energy(greens.greens, nw, n, o, nspin, rot_pad, rot_h1, float(ham_data.h0))

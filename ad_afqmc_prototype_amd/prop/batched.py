@@ -185,6 +185,8 @@ class system_batch:
             n, o, g = s.ham.h1.shape[0], s.trial.nocc, s.ham.chol.shape[0]
             if s.ham.basis != "restricted" or not s.trial.restricted:
                 raise ValueError("batched systems need a restricted basis and an RHF trial")
+            if s.trial.mo_coeff.is_complex():
+                raise ValueError("batched systems need real trial orbitals")
             if o > 32 or n > 208 or o > n:
                 raise ValueError(f"batched system n={n} o={o} outside n <= 208, o <= 32")
             if s.n_walkers * g >= 2 ** 31:

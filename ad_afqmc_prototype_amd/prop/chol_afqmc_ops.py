@@ -68,12 +68,13 @@ def spin_orbital_rdm1(trial_rdm1: torch.Tensor) -> torch.Tensor:
 def _mf_shifts(ham_data: ham_chol, trial_rdm1: torch.Tensor) -> torch.Tensor:
     """prop/chol_afqmc_ops.py:59-63: 1j * tr(L_g dm), dm the total density
     (restricted basis) or the spin-orbital one (generalized basis)."""
-    dm = torch.as_tensor(trial_rdm1, device=ham_data.device, dtype=torch.float64)
+    dm = torch.as_tensor(trial_rdm1, device=ham_data.device)
+    dm = dm.to(torch.complex128 if dm.is_complex() else torch.float64)
     if ham_data.basis == "restricted":
         dm = _as_total_rdm1_restricted(dm)
     else:
         dm = spin_orbital_rdm1(dm)
-    return 1.0j * torch.einsum("gij,ji->g", ham_data.chol, dm).to(torch.complex128)
+    return 1.0j * torch.einsum("gij,ji->g", ham_data.chol.to(dm.dtype), dm).to(torch.complex128)
 
 
 def _build_prop_ctx(ham_data: ham_chol, trial_rdm1, dt: float) -> chol_afqmc_ctx:

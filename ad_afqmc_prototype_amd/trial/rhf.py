@@ -18,12 +18,23 @@ from ..core.system import system
 from ..ham.chol import as_device_f64
 
 
+def as_device_orbitals(x, device=None) -> torch.Tensor:
+    """Trial orbitals on the device: complex128 when any imaginary part is
+    nonzero (the reference takes C^H, trial/rhf.py:25-44), else float64."""
+    t = torch.as_tensor(x)
+    if t.is_complex() and bool(torch.any(t.imag != 0)):
+        if device is None:
+            device = t.device if t.device.type == "cuda" else torch.device("cuda", 0)
+        return t.to(device=device, dtype=torch.complex128).contiguous()
+    return as_device_f64(t.real if t.is_complex() else t, device)
+
+
 @dataclass(frozen=True)
 class rhf_trial:
-    mo_coeff: torch.Tensor  # (norb, nocc), real
+    mo_coeff: torch.Tensor  # (norb, nocc), real or complex
 
     def __post_init__(self):
-        object.__setattr__(self, "mo_coeff", as_device_f64(self.mo_coeff))
+        object.__setattr__(self, "mo_coeff", as_device_orbitals(self.mo_coeff))
 
     @property
     def norb(self) -> int:
@@ -45,6 +56,10 @@ class rhf_trial:
     def restricted(self) -> bool:
         return True
 
+    @property
+    def is_complex(self) -> bool:
+        return self.mo_coeff.is_complex()
+
 
 @dataclass(frozen=True)
 class uhf_trial:
@@ -55,8 +70,12 @@ class uhf_trial:
     mo_dn: torch.Tensor
 
     def __post_init__(self):
-        object.__setattr__(self, "mo_up", as_device_f64(self.mo_up))
-        object.__setattr__(self, "mo_dn", as_device_f64(self.mo_dn, self.mo_up.device))
+        up = as_device_orbitals(self.mo_up)
+        dn = as_device_orbitals(self.mo_dn, up.device)
+        if up.is_complex() or dn.is_complex():
+            up, dn = up.to(torch.complex128), dn.to(torch.complex128)
+        object.__setattr__(self, "mo_up", up)
+        object.__setattr__(self, "mo_dn", dn)
         if self.mo_up.shape != self.mo_dn.shape:
             raise ValueError("uhf_trial: both spins need the same (norb, nocc)")
 
@@ -71,6 +90,10 @@ class uhf_trial:
     @property
     def restricted(self) -> bool:
         return False
+
+    @property
+    def is_complex(self) -> bool:
+        return self.mo_up.is_complex()
 
 
 class greens_cache(NamedTuple):
@@ -94,7 +117,7 @@ class greens_cache(NamedTuple):
 def get_rdm1(trial_data) -> torch.Tensor:
     """trial/rhf.py:29-32: stack of C C^T per spin, (2, n, n)."""
     cu, cd = trial_data.mo_up, trial_data.mo_dn
-    return torch.stack([cu @ cu.T, cd @ cd.T], dim=0)
+    return torch.stack([cu @ cu.conj().T, cd @ cd.conj().T], dim=0)
 
 
 def greens(walkers, trial_data, split: bool = False) -> greens_cache:
@@ -112,7 +135,7 @@ def greens_split(walkers, trial_data) -> greens_cache:
     """The bf16x3 step's Green's function: log det and the bf16 planes of the
     rows only (o <= 32; larger o takes greens(..., split=True))."""
     w0 = walkers[0] if isinstance(walkers, tuple) else walkers
-    if w0.shape[2] > 32:
+    if w0.shape[2] > 32 or trial_data.mo_up.is_complex():
         return greens(walkers, trial_data, split=True)
     if isinstance(walkers, tuple):
         ld, sp = kernels.overlap_greens_bf16x3(walkers, trial_data.mo_up, trial_data.mo_dn)

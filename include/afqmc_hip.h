@@ -121,6 +121,15 @@ int afq_overlap_greens(int nw, int n, int o, int nspin,
                        double* greens, double* logdet, void* stream);
 
 /*
+ * afq_overlap_greens with complex trial orbitals (trial/rhf.py:25-44 takes
+ * C^H): trial_up / trial_dn are complex128 (n, o) interleaved (re, im), and
+ * m = C^H w.  fp64 rows and log det as afq_overlap_greens; o <= 64.
+ */
+int afq_overlap_greens_ctrial(int nw, int n, int o, int nspin, const double* walkers_up,
+                              const double* walkers_dn, const double* trial_up, const double* trial_dn,
+                              double* greens, double* logdet, void* stream);
+
+/*
  * The bf16x3 step's Green's function (meas/rhf.py:15-33, trial/rhf.py:25-37):
  * log det(C_s^T w_s) in fp64 as afq_overlap_greens (partial pivoting on
  * |re| + |im|, the LU's pivots, here in a Gauss-Jordan inversion), and the

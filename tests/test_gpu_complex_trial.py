@@ -1,0 +1,150 @@
+"""Complex trial orbitals (the reference takes C^H, trial/rhf.py:25-44,
+meas/rhf.py:15-50,141-150): overlap, Green's function, force bias, energy and
+a few steps plus a block on the HIP path against the CPU checker (fp64,
+1e-10 relative), RHF and UHF trials; the mixed precisions' force bias within
+their tolerances; and the gauge invariance C -> C U (U unitary) of the
+Green's function, force bias and energy."""
+import numpy as np
+import pytest
+
+from ad_afqmc_prototype_amd import synthetic as syn
+from oracle import afqmc_oracle as orc
+
+from ._pairs import assert_rel, rel, to_dev, to_np
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-10
+
+
+def _complex_orbitals(rng, c):
+    n, o = c.shape
+    a = c + 0.3 * (rng.standard_normal((n, o)) + 1j * rng.standard_normal((n, o)))
+    return np.linalg.qr(a)[0]
+
+
+def _build(device, s, cu, cd, kind, dt, precision="fp64"):
+    from ad_afqmc_prototype_amd.core.system import system
+    from ad_afqmc_prototype_amd.ham.chol import ham_chol
+    from ad_afqmc_prototype_amd.meas.rhf import make_rhf_meas_ops
+    from ad_afqmc_prototype_amd.prop.chol_afqmc_ops import make_chol_afqmc_ops
+    from ad_afqmc_prototype_amd.prop.types import afqmc_params
+    from ad_afqmc_prototype_amd.trial import rhf as trial_rhf
+    n, o = cu.shape
+    sys_ = system(n, (o, o), kind)
+    ham = ham_chol.from_arrays(s.h0, s.h1, s.chol, device=device)
+    if cd is None:
+        trial = trial_rhf.rhf_trial(cu)
+        t_ops = trial_rhf.make_rhf_trial_ops(sys_)
+    else:
+        trial = trial_rhf.uhf_trial(cu, cd)
+        t_ops = trial_rhf.make_uhf_trial_ops(sys_)
+    m_ops = make_rhf_meas_ops(sys_, precision=precision)
+    p_ops = make_chol_afqmc_ops(ham, kind, precision=precision)
+    return dict(sys=sys_, ham=ham, trial=trial, t_ops=t_ops, m_ops=m_ops, p_ops=p_ops,
+                p_ctx=p_ops.build_prop_ctx(t_ops.get_rdm1(trial), dt),
+                m_ctx=m_ops.build_meas_ctx(ham, trial), params=afqmc_params(dt=dt))
+
+
+CASES = [dict(seed=3, n=30, o=5, g=80, kind="restricted", uhf=False),
+         dict(seed=4, n=40, o=9, g=110, kind="unrestricted", uhf=True),
+         dict(seed=5, n=44, o=36, g=60, kind="unrestricted", uhf=True)]
+
+
+@pytest.mark.parametrize("case", CASES)
+def test_complex_trial_measurements(gpu_device, case):
+    from ad_afqmc_prototype_amd.core.ops import k_energy, k_force_bias
+    from ._pairs import random_walkers
+    rng = np.random.default_rng(case["seed"])
+    s = syn.make_system(case["seed"], case["n"], case["o"], case["g"], uhf=case["uhf"], chol_scale=0.3)
+    cu = _complex_orbitals(rng, s.mo_up)
+    cd = _complex_orbitals(rng, s.mo_dn) if case["uhf"] else None
+    trial = (cu, cd if cd is not None else cu)
+    c = _build(gpu_device, s, cu, cd, case["kind"], 0.01)
+    sd = orc.make_system_data(s.h0, s.h1, s.chol, trial, case["kind"], 0.01)
+    w = random_walkers(rng, case["kind"], 12, case["n"], case["o"], trial[0], trial[1])
+    wd = to_dev(w, gpu_device)
+    lo = c["m_ops"].log_overlap(wd, c["trial"]).cpu().numpy()
+    lo_ref = orc.overlap(w, sd.trial, case["kind"], "log")
+    np.testing.assert_allclose(lo.real, lo_ref.real, rtol=1e-11, atol=1e-11)
+    np.testing.assert_allclose(np.exp(1j * lo.imag), np.exp(1j * lo_ref.imag), atol=1e-10)
+    fb = c["m_ops"].require_kernel(k_force_bias)(wd, c["ham"], c["m_ctx"], c["trial"]).cpu().numpy()
+    assert_rel(fb, orc.force_bias(w, sd.trial, sd.mctx, case["kind"]), TOL, "force bias")
+    e = c["m_ops"].require_kernel(k_energy)(wd, c["ham"], c["m_ctx"], c["trial"]).cpu().numpy()
+    assert_rel(e, orc.energy(w, sd.h0, sd.trial, sd.mctx, case["kind"]), TOL, "energy")
+    # prop context: purely imaginary mean-field shifts from the complex density
+    np.testing.assert_allclose(c["p_ctx"].mf_shifts.cpu().numpy(), sd.pctx.mf_shifts, rtol=1e-12, atol=1e-13)
+    for prec, tol in (("fp32", 1e-5), ("bf16x3", 3e-4)):
+        cm = _build(gpu_device, s, cu, cd, case["kind"], 0.01, precision=prec)
+        fbm = cm["m_ops"].require_kernel(k_force_bias)(wd, cm["ham"], cm["m_ctx"], cm["trial"]).cpu().numpy()
+        assert rel(fbm, fb) < tol, prec
+
+
+def test_complex_gauge_of_a_real_trial(gpu_device):
+    """C U with U unitary and complex spans the same space: the half-rotated
+    Green's function becomes U^T G (m = U^H C^T w), the force bias and energy
+    equal those of the real trial C, and the overlap changes by the constant
+    factor det(U^H)^2."""
+    import torch
+    from ad_afqmc_prototype_amd.core.ops import k_energy, k_force_bias
+    from ._pairs import random_walkers
+    rng = np.random.default_rng(11)
+    s = syn.make_system(11, 36, 7, 90, chol_scale=0.3)
+    u = np.linalg.qr(rng.standard_normal((7, 7)) + 1j * rng.standard_normal((7, 7)))[0]
+    a = _build(gpu_device, s, s.mo_up, None, "restricted", 0.01)
+    b = _build(gpu_device, s, s.mo_up @ u, None, "restricted", 0.01)
+    assert b["trial"].mo_coeff.is_complex() and not a["trial"].mo_coeff.is_complex()
+    w = to_dev(random_walkers(rng, "restricted", 10, 36, 7, s.mo_up, s.mo_up), gpu_device)
+    ga, gb = a["m_ops"].greens(w, a["trial"]), b["m_ops"].greens(w, b["trial"])
+    g_a, g_b = ga.greens.cpu().numpy(), gb.greens.cpu().numpy()
+    g_a = (g_a[0] + 1j * g_a[1]).reshape(10, 7, 36)
+    g_b = (g_b[0] + 1j * g_b[1]).reshape(10, 7, 36)
+    assert_rel(g_b, u.T @ g_a, 1e-11, "greens")
+    ratio = torch.exp(gb.log_overlap - ga.log_overlap).cpu().numpy()
+    np.testing.assert_allclose(ratio, np.conj(np.linalg.det(u)) ** 2 * np.ones(10), rtol=1e-11)
+    for k in (k_force_bias, k_energy):
+        va = a["m_ops"].require_kernel(k)(w, a["ham"], a["m_ctx"], a["trial"]).cpu().numpy()
+        vb = b["m_ops"].require_kernel(k)(w, b["ham"], b["m_ctx"], b["trial"]).cpu().numpy()
+        assert_rel(vb, va, 1e-10, k)
+
+
+@pytest.mark.parametrize("case", CASES[:2])
+def test_complex_trial_steps_and_block(gpu_device, case):
+    """A block of 6 steps (QR at 3) and the measurement + comb, common fields
+    and comb uniform, against the checker."""
+    import dataclasses
+    import torch
+    from ad_afqmc_prototype_amd.prop.afqmc import make_key
+    from ad_afqmc_prototype_amd.prop.blocks import afqmc_block
+    from ad_afqmc_prototype_amd.prop.types import prop_state
+    rng = np.random.default_rng(case["seed"] + 50)
+    s = syn.make_system(case["seed"], case["n"], case["o"], case["g"], uhf=case["uhf"], chol_scale=0.3)
+    cu = _complex_orbitals(rng, s.mo_up)
+    cd = _complex_orbitals(rng, s.mo_dn) if case["uhf"] else None
+    trial = (cu, cd if cd is not None else cu)
+    dt, nw, steps = 0.01, 16, 6
+    c = _build(gpu_device, s, cu, cd, case["kind"], dt)
+    sd = orc.make_system_data(s.h0, s.h1, s.chol, trial, case["kind"], dt)
+    st0 = orc.init_state(sd, nw, overlap_mode="log")
+    fields = [rng.standard_normal((nw, case["g"])) for _ in range(steps)]
+    ref, blocks = orc.run_cadenced(st0, sd, orc.params(dt=dt), n_steps=steps, qr_every=3, sr_every=steps,
+                                   fields_fn=lambda t: fields[t], zeta_fn=lambda k: 0.37, overlap_mode="log")
+    w = to_dev(st0.walkers, gpu_device)
+    g = c["m_ops"].greens(w, c["trial"])
+    gs = prop_state(walkers=w, weights=torch.ones(nw, dtype=torch.float64, device=gpu_device),
+                    log_overlaps=torch.tensor(st0.overlaps, device=gpu_device), rng_key=make_key(0),
+                    pop_control_ene_shift=torch.tensor(st0.e_estimate, dtype=torch.float64, device=gpu_device),
+                    e_estimate=st0.e_estimate, greens=g)
+    params = dataclasses.replace(c["params"], n_prop_steps=steps, n_ortho=3)
+    gs, obs = afqmc_block(gs, sys=c["sys"], params=params, ham_data=c["ham"], trial_data=c["trial"],
+                          meas_ops=c["m_ops"], prop_ops=c["p_ops"], prop_ctx=c["p_ctx"], meas_ctx=c["m_ctx"],
+                          fields=[torch.tensor(f, device=gpu_device) for f in fields], zeta=0.37)
+    np.testing.assert_allclose(obs.scalars["energy"].item(), blocks[0]["energy"], rtol=TOL)
+    np.testing.assert_allclose(obs.scalars["weight"].item(), blocks[0]["weight"], rtol=1e-9)
+    assert np.ptp(ref.weights) == 0.0  # after the comb every weight is the mean
+    np.testing.assert_allclose(gs.weights.cpu().numpy(), ref.weights, rtol=1e-9)
+    wg = to_np(gs.walkers)
+    if case["kind"] == "restricted":
+        assert rel(wg, ref.walkers) < 1e-9
+    else:
+        assert max(rel(wg[0], ref.walkers[0]), rel(wg[1], ref.walkers[1])) < 1e-9
