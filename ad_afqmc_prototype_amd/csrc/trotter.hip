@@ -496,6 +496,251 @@ int launch_trotter_resident(int nw, int n, int o, int nspin, int ldv, int n_term
     return afq::check_launch("afq_trotter_f32");
 }
 
+// ---------------------------------------------------------------------------
+// fp64 Trotter step with the walker's VHS resident on chip, for n <= 128.
+// One workgroup per (walker, column chunk) where the chunk runs over the
+// walker's columns of both spins laid side by side (spin-up columns 0..o-1,
+// then spin-down): the spins share V, so V crosses HBM once per walker and
+// the MFMA column tiles are filled across the spin boundary (o = 20 per spin:
+// 40 columns in three 16-column tiles instead of two tiles per spin).  One
+// wave per 16-row tile of V; k-blocks 0..KRB-1 of its rows stay in
+// registers as MFMA A fragments, the rest in LDS in fragment order.  T
+// (ceil16(n) x 16*NCT, split planes) is the B operand in LDS; each wave
+// computes its row tile for every column tile, so no partial sums cross waves.
+// ---------------------------------------------------------------------------
+constexpr int F64_RES_MAX_NP = 128;
+constexpr int F64_RES_MAX_COLS = 48;
+
+// register-resident k-blocks of V: three column tiles leave room for six
+__host__ __device__ constexpr int f64_res_krb(int nct) { return nct >= 3 ? 6 : 7; }
+
+size_t f64_resident_lds(int n, int nct) {
+    const int NP = afq::round_up(n, 16);
+    const int nkb = NP / 16;
+    const int krb = f64_res_krb(nct);
+    const int nkl = nkb > krb ? nkb - krb : 0;
+    return sizeof(double) * ((size_t)2 * NP * 16 * nct + (size_t)(NP / 16) * nkl * 8 * 64);
+}
+
+template <int NCT>
+__global__ __launch_bounds__(512, 1) void trotter_f64_resident_kernel(
+    int nw, int n, int o, int nspin, int chunk, int ngrp, int ldv, int n_terms, double sqrt_dt,
+    const double* __restrict__ E, const double* __restrict__ vhs, long long vplane,
+    const double2* __restrict__ win_up, const double2* __restrict__ win_dn,
+    double2* __restrict__ wout_up, double2* __restrict__ wout_dn) {
+    constexpr int KRB = f64_res_krb(NCT);
+    constexpr int TC = 16 * NCT;
+    using MF = afq::mfma16<double>;
+    using ACC = MF::acc;
+    extern __shared__ __attribute__((aligned(16))) unsigned char lds_raw[];
+    const int NP = afq::round_up(n, 16);
+    const int nkb = NP / 16;
+    const int nkl = nkb > KRB ? nkb - KRB : 0;
+    double* Tr = reinterpret_cast<double*>(lds_raw);
+    double* Ti = Tr + NP * TC;
+    double* VL = Ti + NP * TC;  // [wave][nkl][4 kk][2 planes][64 lanes]
+
+    // the ngrp chunks of one walker are consecutive blocks
+    const int w = blockIdx.x / ngrp;
+    const int j0 = (blockIdx.x - w * ngrp) * chunk;  // first merged column
+    const int ncols = nspin * o;
+    const long long woff = (long long)w * n * o;
+    const double* Vr = vhs + (long long)w * n * ldv;
+    const double* Vi = Vr + vplane;
+
+    const int tid = threadIdx.x;
+    const int nthreads = blockDim.x;
+    const int lane = tid & 63;
+    const int rt = tid >> 6;  // row tile of this wave
+    const int lr = lane & 15;
+    const int lk = (lane >> 4) * 4;
+    const int row = rt * 16 + lr;
+
+    auto load4 = [&](const double* A, int kb, double (&x)[4]) {
+        const int k0 = kb * 16 + lk;
+        const bool ok = row < n && k0 < n;
+        const double4 v = *reinterpret_cast<const double4*>(A + (long long)(ok ? row : 0) * ldv + (ok ? k0 : 0));
+        x[0] = ok ? v.x : 0.0; x[1] = ok ? v.y : 0.0; x[2] = ok ? v.z : 0.0; x[3] = ok ? v.w : 0.0;
+    };
+
+    // merged column t of this chunk -> (spin, walker column); -1 past the end
+    auto col_of = [&](int t, int& s, int& c) {
+        const int j = j0 + t;
+        if (t >= chunk || j >= ncols) return false;
+        s = j >= o ? 1 : 0;
+        c = j - s * o;
+        return true;
+    };
+
+    for (int e = tid; e < NP * TC; e += nthreads) {
+        const int p = e / TC;
+        const int t = e - p * TC;
+        int s, c;
+        double2 x = make_double2(0.0, 0.0);
+        if (p < n && col_of(t, s, c)) x = (s == 0 ? win_up : win_dn)[woff + (long long)p * o + c];
+        Tr[e] = x.x;
+        Ti[e] = x.y;
+    }
+    __syncthreads();
+
+    ACC P[NCT][2], out[NCT][2];
+
+    // P = E . T, E real from L2; this wave's E fragments are loaded first
+    auto e_product = [&]() {
+        double a[F64_RES_MAX_NP / 16][4];
+#pragma unroll
+        for (int kb = 0; kb < F64_RES_MAX_NP / 16; ++kb)
+            if (kb < nkb) load4(E, kb, a[kb]);
+#pragma unroll
+        for (int c = 0; c < NCT; ++c) P[c][0] = P[c][1] = MF::zero();
+#pragma unroll
+        for (int kb = 0; kb < F64_RES_MAX_NP / 16; ++kb) {
+            if (kb < nkb) {
+#pragma unroll
+                for (int kk = 0; kk < 4; ++kk) {
+                    const int k = kb * 16 + lk + kk;
+#pragma unroll
+                    for (int c = 0; c < NCT; ++c) {
+                        P[c][0] = MF::fma(a[kb][kk], Tr[k * TC + c * 16 + lr], P[c][0]);
+                        P[c][1] = MF::fma(a[kb][kk], Ti[k * TC + c * 16 + lr], P[c][1]);
+                    }
+                }
+            }
+        }
+    };
+    auto store_T = [&](const ACC (&X)[NCT][2]) {
+#pragma unroll
+        for (int c = 0; c < NCT; ++c)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int rr = rt * 16 + MF::row(lane, r);
+                Tr[rr * TC + c * 16 + lr] = X[c][0][r];
+                Ti[rr * TC + c * 16 + lr] = X[c][1][r];
+            }
+    };
+    auto vt_step = [&](int k, double ar, double ai) {
+#pragma unroll
+        for (int c = 0; c < NCT; ++c) {
+            const double br = Tr[k * TC + c * 16 + lr];
+            const double bi = Ti[k * TC + c * 16 + lr];
+            P[c][0] = MF::fma(ar, br, P[c][0]);
+            P[c][1] = MF::fma(ar, bi, P[c][1]);
+            P[c][0] = MF::fma(ai, -bi, P[c][0]);
+            P[c][1] = MF::fma(ai, br, P[c][1]);
+        }
+    };
+
+    // T = E w, before V occupies the registers
+    e_product();
+
+    double vr[KRB][4], vi[KRB][4];
+#pragma unroll
+    for (int kb = 0; kb < KRB; ++kb) {
+        load4(Vr, kb < nkb ? kb : 0, vr[kb]);
+        load4(Vi, kb < nkb ? kb : 0, vi[kb]);
+    }
+    for (int kb = KRB; kb < nkb; ++kb) {
+        double xr[4], xi[4];
+        load4(Vr, kb, xr);
+        load4(Vi, kb, xi);
+        double* dst = VL + ((size_t)(rt * nkl + (kb - KRB)) * 8) * 64 + lane;
+#pragma unroll
+        for (int kk = 0; kk < 4; ++kk) {
+            dst[(kk * 2 + 0) * 64] = xr[kk];
+            dst[(kk * 2 + 1) * 64] = xi[kk];
+        }
+    }
+
+    __syncthreads();
+    store_T(P);
+#pragma unroll
+    for (int c = 0; c < NCT; ++c) {
+        out[c][0] = P[c][0];
+        out[c][1] = P[c][1];
+    }
+    __syncthreads();
+
+#pragma unroll 1
+    for (int k = 1; k <= n_terms; ++k) {
+#pragma unroll
+        for (int c = 0; c < NCT; ++c) P[c][0] = P[c][1] = MF::zero();
+#pragma unroll
+        for (int kb = 0; kb < KRB; ++kb) {
+            if (kb < nkb) {
+#pragma unroll
+                for (int kk = 0; kk < 4; ++kk) vt_step(kb * 16 + lk + kk, vr[kb][kk], vi[kb][kk]);
+            }
+        }
+        for (int kb = KRB; kb < nkb; ++kb) {
+            const double* src = VL + ((size_t)(rt * nkl + (kb - KRB)) * 8) * 64 + lane;
+#pragma unroll
+            for (int kk = 0; kk < 4; ++kk)
+                vt_step(kb * 16 + lk + kk, src[(kk * 2 + 0) * 64], src[(kk * 2 + 1) * 64]);
+        }
+        const double ck = sqrt_dt / (double)k;
+#pragma unroll
+        for (int c = 0; c < NCT; ++c) {
+            const ACC re = -ck * P[c][1];
+            const ACC im = ck * P[c][0];
+            P[c][0] = re;
+            P[c][1] = im;
+            out[c][0] += re;
+            out[c][1] += im;
+        }
+        __syncthreads();
+        store_T(P);
+        __syncthreads();
+    }
+
+    store_T(out);
+    __syncthreads();
+    e_product();
+#pragma unroll
+    for (int c = 0; c < NCT; ++c) {
+        int s, cc;
+        if (col_of(c * 16 + lr, s, cc)) {
+            double2* Wout = (s == 0 ? wout_up : wout_dn) + woff;
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int rr = rt * 16 + MF::row(lane, r);
+                if (rr < n) Wout[(long long)rr * o + cc] = make_double2(P[c][0][r], P[c][1][r]);
+            }
+        }
+    }
+}
+
+// merged-spin column chunking: chunks of at most F64_RES_MAX_COLS columns
+void f64_resident_shape(int o, int nspin, int& ngrp, int& chunk, int& nct) {
+    const int cols = nspin * o;
+    ngrp = afq::ceil_div(cols, F64_RES_MAX_COLS);
+    chunk = afq::ceil_div(cols, ngrp);
+    nct = afq::ceil_div(chunk, 16);
+}
+
+int launch_trotter_f64_resident(int nw, int n, int o, int nspin, int ldv, int n_terms, double sqrt_dt,
+                                const double* E, const double* vhs, const double2* wu, const double2* wd,
+                                double2* ou, double2* od, hipStream_t st) {
+    const int NP = afq::round_up(n, 16);
+    int ngrp, chunk, nct;
+    f64_resident_shape(o, nspin, ngrp, chunk, nct);
+    const size_t lds = f64_resident_lds(n, nct);
+    const dim3 grid((unsigned)(nw * ngrp)), block(64 * (NP / 16));
+    const long long vplane = (long long)nw * n * ldv;
+#define AFQ_F64_RES(NCT_)                                                                              \
+    if (nct == NCT_) {                                                                                 \
+        if (lds > 64 * 1024) afq::allow_full_lds((const void*)trotter_f64_resident_kernel<NCT_>);     \
+        hipLaunchKernelGGL((trotter_f64_resident_kernel<NCT_>), grid, block, lds, st, nw, n, o, nspin, \
+                           chunk, ngrp, ldv, n_terms, sqrt_dt, E, vhs, vplane, wu, wd, ou, od);        \
+        return afq::check_launch("afq_trotter");                                                       \
+    }
+    AFQ_F64_RES(1)
+    AFQ_F64_RES(2)
+    AFQ_F64_RES(3)
+#undef AFQ_F64_RES
+    afq::set_error("afq_trotter: no resident kernel for o=%d nspin=%d", o, nspin);
+    return -1;
+}
+
 template <typename R>
 int trotter_entry(const char* name, int nw, int n, int o, int nspin, int ldv, int n_terms,
                   double sqrt_dt, const R* exp_h1_half, const R* vhs, const double* win_up,
@@ -521,6 +766,16 @@ int trotter_entry(const char* name, int nw, int n, int o, int nspin, int ldv, in
     const double2* wd = (const double2*)(nspin == 2 ? win_dn : win_up);
     double2* ou = (double2*)wout_up;
     double2* od = (double2*)(nspin == 2 ? wout_dn : wout_up);
+    if constexpr (sizeof(R) == sizeof(double)) {
+        // VHS resident in registers + LDS up to 128 rows, both spins per workgroup
+        if (NP <= F64_RES_MAX_NP) {
+            int ngrp, chunk, nct;
+            f64_resident_shape(o, nspin, ngrp, chunk, nct);
+            if (f64_resident_lds(n, nct) <= 160 * 1024 && (long long)nw * ngrp < (1LL << 31))
+                return launch_trotter_f64_resident(nw, n, o, nspin, ldv, n_terms, sqrt_dt, exp_h1_half,
+                                                   vhs, wu, wd, ou, od, st);
+        }
+    }
     if constexpr (sizeof(R) == sizeof(float)) {
         // VHS resident on chip when one wave per row tile fits (<= 16 waves)
         // and the LDS share of V plus T fits in 160 KB

@@ -20,6 +20,11 @@ SHAPES = [
     # the block Green's function, the library-GEMM energy
     (8, 60, 48, 90, True, "unrestricted"),
     (9, 70, 64, 80, False, "restricted"),
+    # the 120-orbital UHF production shape (fp64 Trotter: both spins' 40
+    # columns in one workgroup, two k-blocks of V in LDS) and 128 rows with
+    # one LDS k-block
+    (1, 120, 20, 600, True, "unrestricted"),
+    (10, 128, 16, 200, False, "restricted"),
 ]
 
 
