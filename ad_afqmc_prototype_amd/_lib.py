@@ -69,12 +69,14 @@ _SIGNATURES = {
     # grouped launches over many systems (descriptor tables in device memory)
     "afq_gemm_tile_shape": [c_int, ctypes.POINTER(c_int), ctypes.POINTER(c_int)],
     "afq_sgemm_nt_grouped": [c_int, c_vp, c_int, c_int, c_vp],
-    "afq_gemm_bf16x3_nt_grouped": [c_int, c_vp, c_int, c_vp],
+    "afq_gemm_bf16x3_nt_grouped": [c_int, c_vp, c_int, c_int, c_vp],
+    "afq_gemm_bf16x3_tile_shape": [c_int, ctypes.POINTER(c_int), ctypes.POINTER(c_int)],
     "afq_split_bf16_grouped": [c_int, c_vp, c_ll, c_vp],
     "afq_rng_fields_grouped": [c_int, c_vp, c_int, c_vp],
     "afq_field_shifts_grouped": [c_int, c_vp, c_int, c_vp],
     "afq_trotter_f32_grouped": [c_int, c_vp, c_int, c_int, c_int, c_vp],
     "afq_trotter_bf16x3_grouped": [c_int, c_vp, c_int, c_int, c_int, c_vp],
+    "afq_trotter_bf16x3_sym_grouped": [c_int, c_vp, c_int, c_int, c_int, c_vp],
     "afq_overlap_greens_grouped": [c_int, c_vp, c_int, c_int, c_vp],
     "afq_phaseless_update_grouped": [c_int, c_vp, c_vp],
     "afq_orthonormalize_grouped": [c_int, c_vp, c_int, c_int, c_int, c_vp],
@@ -86,7 +88,7 @@ _SIGNATURES = {
     "afq_unpack_pairs": [c_int, c_int, c_vp, c_ll, c_vp, c_vp],
 }
 
-ABI_VERSION = 12
+ABI_VERSION = 13
 
 
 class HipLibraryError(RuntimeError):

@@ -201,6 +201,9 @@ __global__ __launch_bounds__(THREADS, 2) void gemm_bf16x3_grouped_kernel(int cou
     if (p.sym_n > 0)
         bf16x3_tile<float, true>(p.M, p.N, p.K, p.alpha, p.A_hi, p.A_lo, p.lda, p.B_hi, p.B_lo, p.ldb, (float*)p.C,
                                  0LL, m0, n0, afq::sym_dst{p.sym_n, p.sym_ldv, afq::ceil_div(p.sym_n, 16)}, S);
+    else if (p.c_fp64 == 2)
+        bf16x3_tile<afq::bf16_pair, false>(p.M, p.N, p.K, p.alpha, p.A_hi, p.A_lo, p.lda, p.B_hi, p.B_lo, p.ldb,
+                                           (afq::bf16_pair*)p.C, p.ldc, m0, n0, afq::sym_dst{0, 0, 0}, S);
     else if (p.c_fp64)
         bf16x3_tile<double, false>(p.M, p.N, p.K, p.alpha, p.A_hi, p.A_lo, p.lda, p.B_hi, p.B_lo, p.ldb,
                                    (double*)p.C, p.ldc, m0, n0, afq::sym_dst{0, 0, 0}, S);
@@ -232,59 +235,65 @@ __device__ __forceinline__ rsrc_t make_rsrc(const void* p, unsigned long long by
                                             (int)(bytes > 0x7fffffffull ? 0x7fffffffull : bytes), 0x00020000);
 }
 
-template <typename TOut, bool SYM = false>
-__global__ __launch_bounds__(THREADS2, 1) void gemm_bf16x3_nt_256_kernel(
-    int M, int N, int K, float alpha, const uint16_t* __restrict__ Ahi,
-    const uint16_t* __restrict__ Alo, long long lda, const uint16_t* __restrict__ Bhi,
-    const uint16_t* __restrict__ Blo, long long ldb, TOut* __restrict__ C, long long ldc,
-    int tiles_m, int tiles_n, afq::sym_dst sd = afq::sym_dst{0, 0, 0}) {
-    // [stage][plane: A hi, A lo, B hi, B lo][256 rows x 64 bytes]
-    __shared__ __attribute__((aligned(1024))) uint4 S[2][4][BM2 * 4];
+// Tile geometry of the LDS-DMA main loop: BM = 64 WM rows x 256 columns of C
+// per workgroup of 2 WM waves (WM x 2 waves of 64 x 128 each).  A k-step
+// moves the four operand planes (A hi, A lo: BM rows; B hi, B lo: 256 rows;
+// 64 bytes per row) in 1 KB blocks of 16 rows, BPW blocks per wave.
+template <int WM>
+struct dma_shape {
+    static constexpr int BM = 64 * WM;
+    static constexpr int THREADS = 128 * WM;
+    static constexpr int ABLK = BM / 16;            // blocks per A plane
+    static constexpr int NBLK = 2 * ABLK + 2 * 16;  // blocks per k-step
+    static constexpr int BPW = NBLK / (2 * WM);     // blocks per wave
+    static constexpr int STAGE = NBLK * 64;         // uint4 per k-step
+    __host__ __device__ static constexpr int plane_off(int p) {
+        return p < 2 ? p * ABLK * 64 : 2 * ABLK * 64 + (p - 2) * 16 * 64;
+    }
+};
 
-    const int total = tiles_m * tiles_n;
-    const int t = afq::xcd_contiguous(blockIdx.x, total);
-    const int per_group = GROUP_M2 * tiles_n;
-    const int group = t / per_group;
-    const int first_m = group * GROUP_M2;
-    const int gsize = min(tiles_m - first_m, GROUP_M2);
-    const int in_group = t - group * per_group;
-    const int m0 = (first_m + in_group % gsize) * BM2;
-    const int n0 = (in_group / gsize) * BN2;
-
+// acc[i][j] (rows wm*64 + 16 i, columns wn*128 + 16 j of the tile at m0, n0)
+// = A B^T over the whole K, with S the double-buffered stages (2 STAGE uint4).
+template <int WM>
+__device__ __forceinline__ void dma_mainloop(int M, int N, int K, const uint16_t* __restrict__ Ahi,
+                                             const uint16_t* __restrict__ Alo, long long lda,
+                                             const uint16_t* __restrict__ Bhi, const uint16_t* __restrict__ Blo,
+                                             long long ldb, int m0, int n0, uint4* S, f32x4 (&acc)[4][8]) {
+    using D = dma_shape<WM>;
     const int tid = threadIdx.x;
     const int lane = tid & 63;
     // wave-uniform (scalar): the descriptors and LDS bases below must be
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
-    const int wm = wave & 3;   // 64-row slice of the tile
-    const int wn = wave >> 2;  // 128-column slice
+    const int wm = wave % WM;  // 64-row slice of the tile
+    const int wn = wave / WM;  // 128-column slice
 
-    // copy assignment: wave w moves plane w / 2 (A hi, A lo, B hi, B lo), 1 KB
-    // blocks 8 (w % 2) + i, i < 8 (rows 16 blk ..); lane L fills slot L of a
-    // block: row L / 4, position L % 4, i.e. chunk (L % 4) ^ ((L / 8) & 3)
-    const int plane = wave >> 1;
-    const uint16_t* src = plane == 0 ? Ahi + (long long)m0 * lda
-                        : plane == 1 ? Alo + (long long)m0 * lda
-                        : plane == 2 ? Bhi + (long long)n0 * ldb
-                                     : Blo + (long long)n0 * ldb;
-    const long long ld = plane < 2 ? lda : ldb;
-    const int rows_left = max(plane < 2 ? M - m0 : N - n0, 0);
-    const rsrc_t R = make_rsrc(src, (unsigned long long)rows_left * ld * 2);
+    // copy assignment: block gb = wave * BPW + i of the k-step, in plane order
+    // (A hi, A lo, B hi, B lo); lane L fills slot L of a block: row L / 4,
+    // position L % 4, i.e. chunk (L % 4) ^ ((L / 8) & 3)
+    const rsrc_t R0 = make_rsrc(Ahi + (long long)m0 * lda, (unsigned long long)max(M - m0, 0) * lda * 2);
+    const rsrc_t R1 = make_rsrc(Alo + (long long)m0 * lda, (unsigned long long)max(M - m0, 0) * lda * 2);
+    const rsrc_t R2 = make_rsrc(Bhi + (long long)n0 * ldb, (unsigned long long)max(N - n0, 0) * ldb * 2);
+    const rsrc_t R3 = make_rsrc(Blo + (long long)n0 * ldb, (unsigned long long)max(N - n0, 0) * ldb * 2);
     const int r_loc = lane >> 2;
     const int chunk = (lane & 3) ^ ((lane >> 3) & 3);
     auto issue_stage = [&](int kt, int buf) {
         const int k = kt * 32 + chunk * 8;
+        // k >= K sets the top bit: the offset lands outside the descriptor
+        const unsigned past = (unsigned)((K - 1 - k) >> 31) & kOutside;
 #pragma unroll
-        for (int i = 0; i < 8; ++i) {
-            const int blk = (wave & 1) * 8 + i;
+        for (int i = 0; i < D::BPW; ++i) {
+            const int gb = wave * D::BPW + i;
+            const int plane = gb < D::ABLK ? 0 : gb < 2 * D::ABLK ? 1 : gb < 2 * D::ABLK + 16 ? 2 : 3;
+            const int blk = gb - (plane < 2 ? plane * D::ABLK : 2 * D::ABLK + (plane - 2) * 16);
+            const long long ld = plane < 2 ? lda : ldb;
+            const rsrc_t R = plane == 0 ? R0 : plane == 1 ? R1 : plane == 2 ? R2 : R3;
             const int row = blk * 16 + r_loc;
-            // k >= K sets the top bit: the offset lands outside the descriptor
-            const unsigned past = (unsigned)((K - 1 - k) >> 31) & kOutside;
             const unsigned off = (unsigned)(((long long)row * ld + k) * 2) | past;
-            __builtin_amdgcn_raw_ptr_buffer_load_lds(R, (lds_ptr_t)&S[buf][plane][blk * 64], 16, off, 0, 0, 0);
+            __builtin_amdgcn_raw_ptr_buffer_load_lds(
+                R, (lds_ptr_t)&S[buf * D::STAGE + D::plane_off(plane) + blk * 64], 16, off, 0, 0, 0);
         }
     };
 
-    f32x4 acc[4][8];
 #pragma unroll
     for (int i = 0; i < 4; ++i)
 #pragma unroll
@@ -302,12 +311,13 @@ __global__ __launch_bounds__(THREADS2, 1) void gemm_bf16x3_nt_256_kernel(
     for (int kt = 0; kt < nk; ++kt) {
         const int buf = kt & 1;
         if (kt + 1 < nk) issue_stage(kt + 1, buf ^ 1);
+        const uint4* Sb = S + buf * D::STAGE;
         bf16x8 ah[4], al[4];
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
             const int slot = (wm * 64 + i * 16) * 4 + lane_slot;
-            ah[i] = __builtin_bit_cast(bf16x8, S[buf][0][slot]);
-            al[i] = __builtin_bit_cast(bf16x8, S[buf][1][slot]);
+            ah[i] = __builtin_bit_cast(bf16x8, Sb[D::plane_off(0) + slot]);
+            al[i] = __builtin_bit_cast(bf16x8, Sb[D::plane_off(1) + slot]);
         }
 #pragma unroll
         for (int jh = 0; jh < 2; ++jh) {
@@ -315,8 +325,8 @@ __global__ __launch_bounds__(THREADS2, 1) void gemm_bf16x3_nt_256_kernel(
 #pragma unroll
             for (int j = 0; j < 4; ++j) {
                 const int slot = (wn * 128 + (jh * 4 + j) * 16) * 4 + lane_slot;
-                bh[j] = __builtin_bit_cast(bf16x8, S[buf][2][slot]);
-                bl[j] = __builtin_bit_cast(bf16x8, S[buf][3][slot]);
+                bh[j] = __builtin_bit_cast(bf16x8, Sb[D::plane_off(2) + slot]);
+                bl[j] = __builtin_bit_cast(bf16x8, Sb[D::plane_off(3) + slot]);
             }
 #pragma unroll
             for (int i = 0; i < 4; ++i)
@@ -332,6 +342,59 @@ __global__ __launch_bounds__(THREADS2, 1) void gemm_bf16x3_nt_256_kernel(
         asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
         __syncthreads();
     }
+}
+
+// Plain store of a DMA tile: element (row, col) = alpha acc, inside M x N.
+template <int WM, typename TOut>
+__device__ __forceinline__ void dma_store(int M, int N, float alpha, TOut* __restrict__ C, long long ldc, int m0,
+                                          int n0, const f32x4 (&acc)[4][8]) {
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    const int wm = wave % WM, wn = wave / WM;
+    const int lr = lane & 15, q = lane >> 4;
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            const int col = n0 + wn * 128 + j * 16 + lr;
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int row = m0 + wm * 64 + i * 16 + q * 4 + r;
+                if (row < M && col < N) C[(long long)row * ldc + col] = (TOut)(alpha * acc[i][j][r]);
+            }
+        }
+}
+
+template <typename TOut, bool SYM = false>
+__global__ __launch_bounds__(THREADS2, 1) void gemm_bf16x3_nt_256_kernel(
+    int M, int N, int K, float alpha, const uint16_t* __restrict__ Ahi,
+    const uint16_t* __restrict__ Alo, long long lda, const uint16_t* __restrict__ Bhi,
+    const uint16_t* __restrict__ Blo, long long ldb, TOut* __restrict__ C, long long ldc,
+    int tiles_m, int tiles_n, afq::sym_dst sd = afq::sym_dst{0, 0, 0}) {
+    // [stage][plane: A hi, A lo, B hi, B lo][256 rows x 64 bytes]
+    __shared__ __attribute__((aligned(1024))) uint4 S[2][4][BM2 * 4];
+    static_assert(sizeof(S) == 2 * dma_shape<4>::STAGE * sizeof(uint4), "stage layout");
+
+    const int total = tiles_m * tiles_n;
+    const int t = afq::xcd_contiguous(blockIdx.x, total);
+    const int per_group = GROUP_M2 * tiles_n;
+    const int group = t / per_group;
+    const int first_m = group * GROUP_M2;
+    const int gsize = min(tiles_m - first_m, GROUP_M2);
+    const int in_group = t - group * per_group;
+    const int m0 = (first_m + in_group % gsize) * BM2;
+    const int n0 = (in_group / gsize) * BN2;
+
+    f32x4 acc[4][8];
+    dma_mainloop<4>(M, N, K, Ahi, Alo, lda, Bhi, Blo, ldb, m0, n0, &S[0][0][0], acc);
+
+    const int tid = threadIdx.x;
+    const int lane = tid & 63;
+    const int wave = tid >> 6;
+    const int wm = wave & 3;
+    const int wn = wave >> 2;
+    const int lr = lane & 15;
+    const int q = lane >> 4;
 
     if constexpr (SYM) {
         // The 256-column tile is one block (bi, bj).  It leaves through LDS in
@@ -378,17 +441,53 @@ __global__ __launch_bounds__(THREADS2, 1) void gemm_bf16x3_nt_256_kernel(
         }
         return;
     }
+    dma_store<4>(M, N, alpha, C, ldc, m0, n0, acc);
+}
+
+// Grouped LDS-DMA GEMM (tile shapes 1 and 2: 128 x 256 and 256 x 256):
+// workgroup -> (problem, tile) by the problems' tile_start prefix sums after
+// the XCD-contiguous remap; the symmetric VHS store as in the 128 x 128
+// grouped kernel (the 256-column tile is one block of V).
+template <int WM>
+__global__ __launch_bounds__(128 * WM, 1) void gemm_bf16x3_grouped_dma_kernel(
+    int count, int total, const afq_gemm_problem_bf16x3* __restrict__ P) {
+    using D = dma_shape<WM>;
+    __shared__ __attribute__((aligned(1024))) uint4 S[2 * D::STAGE];
+    const int t = afq::xcd_contiguous(blockIdx.x, total);
+    const int pi = afq::find_group(count, t, [&](int i) { return P[i].tile_start; });
+    const afq_gemm_problem_bf16x3& p = P[pi];
+    const int tiles_m = afq::ceil_div(p.M, D::BM);
+    const int local = t - p.tile_start;
+    const int m0 = (local % tiles_m) * D::BM;
+    const int n0 = (local / tiles_m) * 256;
+    f32x4 acc[4][8];
+    dma_mainloop<WM>(p.M, p.N, p.K, p.A_hi, p.A_lo, p.lda, p.B_hi, p.B_lo, p.ldb, m0, n0, S, acc);
+    if (p.sym_n > 0) {
+        const afq::sym_dst sd{p.sym_n, p.sym_ldv, afq::ceil_div(p.sym_n, 16)};
+        int bi, bj;
+        afq::sym_block(n0 >> 8, sd.nb, bi, bj);
+        const int lane = threadIdx.x & 63;
+        const int wave = threadIdx.x >> 6;
+        const int wm = wave % WM, wn = wave / WM;
+        const long long ldf = (long long)sd.n * sd.ldv;
 #pragma unroll
-    for (int i = 0; i < 4; ++i)
-#pragma unroll
-        for (int j = 0; j < 8; ++j) {
-            const int col = n0 + wn * 128 + j * 16 + lr;
+        for (int i = 0; i < 4; ++i)
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
-                const int row = m0 + wm * 64 + i * 16 + q * 4 + r;
-                if (row < M && col < N) C[(long long)row * ldc + col] = (TOut)(alpha * acc[i][j][r]);
+                const int row = m0 + wm * 64 + i * 16 + (lane >> 4) * 4 + r;
+                if (row >= p.M) continue;
+                float v[8];
+#pragma unroll
+                for (int j = 0; j < 8; ++j) v[j] = p.alpha * acc[i][j][r];
+                afq::sym_store<8>((float*)p.C + row * ldf, sd, bi, bj, wn * 8, lane & 15, v);
             }
-        }
+    } else if (p.c_fp64 == 2) {
+        dma_store<WM>(p.M, p.N, p.alpha, (afq::bf16_pair*)p.C, p.ldc, m0, n0, acc);
+    } else if (p.c_fp64) {
+        dma_store<WM>(p.M, p.N, p.alpha, (double*)p.C, p.ldc, m0, n0, acc);
+    } else {
+        dma_store<WM>(p.M, p.N, p.alpha, (float*)p.C, p.ldc, m0, n0, acc);
+    }
 }
 
 // C = alpha * sum_kz part[kz] (kz in order), the split-K partials of M x N
@@ -595,13 +694,30 @@ extern "C" int afq_convert_f32(int rows, int cols, const double* src, long long 
     return afq::check_launch("afq_convert_f32");
 }
 
+extern "C" int afq_gemm_bf16x3_tile_shape(int shape, int* bm, int* bn) {
+    AFQ_REQUIRE(shape >= 0 && shape <= 2, "afq_gemm_bf16x3_tile_shape: shape=%d not in 0..2", shape);
+    AFQ_REQUIRE(bm && bn, "afq_gemm_bf16x3_tile_shape: null pointer");
+    *bm = shape == 0 ? BM : shape == 1 ? dma_shape<2>::BM : dma_shape<4>::BM;
+    *bn = shape == 0 ? BN : 256;
+    return 0;
+}
+
 extern "C" int afq_gemm_bf16x3_nt_grouped(int count, const afq_gemm_problem_bf16x3* problems, int total_tiles,
-                                          void* stream) {
+                                          int shape, void* stream) {
     AFQ_REQUIRE(count > 0 && total_tiles > 0, "afq_gemm_bf16x3_nt_grouped: count=%d total_tiles=%d", count,
                 total_tiles);
+    AFQ_REQUIRE(shape >= 0 && shape <= 2, "afq_gemm_bf16x3_nt_grouped: shape=%d not in 0..2", shape);
     AFQ_REQUIRE(problems, "afq_gemm_bf16x3_nt_grouped: null pointer");
-    hipLaunchKernelGGL(gemm_bf16x3_grouped_kernel, dim3((unsigned)total_tiles), dim3(THREADS), 0, (hipStream_t)stream,
-                       count, total_tiles, problems);
+    hipStream_t st = (hipStream_t)stream;
+    if (shape == 0)
+        hipLaunchKernelGGL(gemm_bf16x3_grouped_kernel, dim3((unsigned)total_tiles), dim3(THREADS), 0, st, count,
+                           total_tiles, problems);
+    else if (shape == 1)
+        hipLaunchKernelGGL(gemm_bf16x3_grouped_dma_kernel<2>, dim3((unsigned)total_tiles),
+                           dim3(dma_shape<2>::THREADS), 0, st, count, total_tiles, problems);
+    else
+        hipLaunchKernelGGL(gemm_bf16x3_grouped_dma_kernel<4>, dim3((unsigned)total_tiles),
+                           dim3(dma_shape<4>::THREADS), 0, st, count, total_tiles, problems);
     return afq::check_launch("afq_gemm_bf16x3_nt_grouped");
 }
 

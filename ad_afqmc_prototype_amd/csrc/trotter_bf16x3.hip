@@ -199,7 +199,12 @@ __global__ __launch_bounds__(512, 1) void trotter_bf16x3_kernel(
         sqrt_dt = d.sqrt_dt;
         E = d.exp_h1_half;
         vhs = d.vhs;
-        vplane = (long long)nw * n * ldv;
+        if constexpr (SYMV) {
+            const int nbg = afq::ceil_div(n, 16);
+            vplane = (long long)nw * nbg * (nbg + 1) / 2 * 256;
+        } else {
+            vplane = (long long)nw * n * ldv;
+        }
         win_up = win_dn = (const double2*)d.walkers_in;
         wout_up = wout_dn = (double2*)d.walkers_out;
         group_map = 0;
@@ -615,26 +620,27 @@ extern "C" int afq_trotter_bf16x3_sym(int nw, int n, int o, int nspin, int n_ter
                                   (const float*)vhs_sym, win_up, win_dn, wout_up, wout_dn, (hipStream_t)stream);
 }
 
-extern "C" int afq_trotter_bf16x3_grouped(int count, const afq_system_step* sys, int total_blocks, int max_n,
-                                          int n_terms, void* stream) {
-    AFQ_REQUIRE(count > 0 && total_blocks > 0, "afq_trotter_bf16x3_grouped: count=%d total_blocks=%d", count,
-                total_blocks);
-    AFQ_REQUIRE(sys, "afq_trotter_bf16x3_grouped: null pointer");
-    AFQ_REQUIRE(max_n > 0 && max_n <= 16 * 2 * (KR + NL_MAX), "afq_trotter_bf16x3_grouped: max_n=%d not in 1..%d",
-                max_n, 16 * 2 * (KR + NL_MAX));
-    AFQ_REQUIRE(n_terms >= 0, "afq_trotter_bf16x3_grouped: n_terms < 0");
+namespace {
+
+template <bool SYMV>
+int trotter_bf16x3_grouped(const char* name, int count, const afq_system_step* sys, int total_blocks, int max_n,
+                           int n_terms, hipStream_t st) {
+    AFQ_REQUIRE(count > 0 && total_blocks > 0, "%s: count=%d total_blocks=%d", name, count, total_blocks);
+    AFQ_REQUIRE(sys, "%s: null pointer", name);
+    AFQ_REQUIRE(max_n > 0 && max_n <= 16 * 2 * (KR + NL_MAX), "%s: max_n=%d not in 1..%d", name, max_n,
+                16 * 2 * (KR + NL_MAX));
+    AFQ_REQUIRE(n_terms >= 0, "%s: n_terms < 0", name);
     const int NP = afq::round_up(max_n, 16);
     const int nk = afq::ceil_div(NP, 32);
     const int nl = nk > KR ? nk - KR : 0;
     const size_t lds = bf16x3_lds(max_n, nl);
-    AFQ_REQUIRE(lds <= 160 * 1024, "afq_trotter_bf16x3_grouped: max_n=%d needs %zu bytes of LDS", max_n, lds);
+    AFQ_REQUIRE(lds <= 160 * 1024, "%s: max_n=%d needs %zu bytes of LDS", name, max_n, lds);
     const dim3 block(64 * afq::ceil_div(NP / 16, RT));
-    hipStream_t st = (hipStream_t)stream;
     const dim3 grid((unsigned)total_blocks);
-#define AFQ_TROTTER_GROUPED(NLV)                                                                                   \
-    do {                                                                                                           \
-        if (lds > 64 * 1024) afq::allow_full_lds((const void*)trotter_bf16x3_kernel<NLV, false, true>);            \
-        hipLaunchKernelGGL((trotter_bf16x3_kernel<NLV, false, true>), grid, block, lds, st, 0, 0, 0, 1, 0, 0,         \
+#define AFQ_TROTTER_GROUPED(NLV)                                                                                  \
+    do {                                                                                                          \
+        if (lds > 64 * 1024) afq::allow_full_lds((const void*)trotter_bf16x3_kernel<NLV, SYMV, true>);            \
+        hipLaunchKernelGGL((trotter_bf16x3_kernel<NLV, SYMV, true>), grid, block, lds, st, 0, 0, 0, 1, 0, 0,         \
                            n_terms, 0.0, nullptr, nullptr, 0LL, nullptr, nullptr, nullptr, nullptr, 0, count, sys); \
     } while (0)
     if (nl == 0)
@@ -644,5 +650,19 @@ extern "C" int afq_trotter_bf16x3_grouped(int count, const afq_system_step* sys,
     else
         AFQ_TROTTER_GROUPED(2);
 #undef AFQ_TROTTER_GROUPED
-    return afq::check_launch("afq_trotter_bf16x3_grouped");
+    return afq::check_launch(name);
+}
+
+}  // namespace
+
+extern "C" int afq_trotter_bf16x3_grouped(int count, const afq_system_step* sys, int total_blocks, int max_n,
+                                          int n_terms, void* stream) {
+    return trotter_bf16x3_grouped<false>("afq_trotter_bf16x3_grouped", count, sys, total_blocks, max_n, n_terms,
+                                         (hipStream_t)stream);
+}
+
+extern "C" int afq_trotter_bf16x3_sym_grouped(int count, const afq_system_step* sys, int total_blocks, int max_n,
+                                              int n_terms, void* stream) {
+    return trotter_bf16x3_grouped<true>("afq_trotter_bf16x3_sym_grouped", count, sys, total_blocks, max_n, n_terms,
+                                        (hipStream_t)stream);
 }

@@ -17,11 +17,18 @@ them, the second with input and output buffers swapped, so consecutive steps
 alternate tables and nothing is rewritten between steps.  A block (steps, QR,
 measurement, comb) can be captured once as a HIP graph and replayed.
 
-GEMM tiles: the grouped GEMM has four tile shapes (128x128, 64x128, 128x64,
-64x64 rows x columns of C per workgroup); tune() times each on the batch's own
-problem set with HIP events and keeps the fastest for the force-bias and the
-VHS contraction separately (optionally cached in a JSON file keyed by the
+GEMM tiles: the grouped fp32 GEMM has four tile shapes (128x128, 64x128,
+128x64, 64x64 rows x columns of C per workgroup), the grouped bf16x3 GEMM
+three (128x128 staged through registers, 128x256 and 256x256 moved to LDS by
+buffer_load ... lds); tune() times each on the batch's own problem set with HIP
+events and keeps the fastest for the force-bias and the VHS contraction
+separately (optionally cached in a JSON file keyed by the precision and the
 problem shapes).
+
+In bf16x3 precision the VHS GEMM writes each system's V as split-bf16 pair
+words of its upper 16 x 16 blocks (half the bytes of full fp32 rows), and the
+Trotter step reads those and E - I in the same layout, as the single-system
+bf16x3 step does.
 """
 from __future__ import annotations
 
@@ -95,14 +102,15 @@ assert ctypes.sizeof(_system_step) == 8 * 4 + 8 * 8 + 24 * 8
 assert ctypes.sizeof(_gemm_problem) == 80
 assert ctypes.sizeof(_gemm_problem_bf16x3) == 96
 assert ctypes.sizeof(_split_problem) == 40
-BF16X3_TILE = 128  # rows and columns of C per workgroup of the grouped bf16x3 GEMM
+GEMM_SHAPES = (0, 1, 2, 3)        # grouped fp32 GEMM tile shapes
+BF16X3_SHAPES = (0, 1, 2)         # grouped bf16x3 GEMM tile shapes
 
-GEMM_SHAPES = (0, 1, 2, 3)
 
-
-def gemm_tile_shape(shape: int) -> tuple[int, int]:
+def gemm_tile_shape(shape: int, precision: str = "fp32") -> tuple[int, int]:
+    """(rows, columns) of C per workgroup of a grouped GEMM tile shape."""
     bm, bn = ctypes.c_int(), ctypes.c_int()
-    _lib.call("afq_gemm_tile_shape", shape, ctypes.byref(bm), ctypes.byref(bn))
+    entry = "afq_gemm_bf16x3_tile_shape" if precision == "bf16x3" else "afq_gemm_tile_shape"
+    _lib.call(entry, shape, ctypes.byref(bm), ctypes.byref(bn))
     return bm.value, bn.value
 
 
@@ -155,8 +163,8 @@ class system_batch:
 
     ``precision`` is the arithmetic of the force-bias and VHS contractions and
     of the Trotter step: "bf16x3" (split-bf16 operands on the bf16 matrix
-    cores, fp32 sums; one 128 x 128 GEMM tile shape) or "fp32" (fp32 matrix
-    cores, GEMM tile shape picked by tune()).  Green's functions, QR, weights
+    cores, fp32 sums) or "fp32" (fp32 matrix cores); the GEMM tile shapes are
+    ``gemm_shapes`` (force bias, VHS), picked by tune().  Green's functions, QR, weights
     and the energy (fp32 products, fp64 sums) are the same in both."""
 
     def __init__(self, systems: Sequence[batch_system], *, dt: float, qr_every: int = 10,
@@ -198,6 +206,9 @@ class system_batch:
         self._allocate()
         self._tables()
         self.gemm_shapes = tuple(gemm_shapes) if gemm_shapes is not None else (0, 0)
+        valid = BF16X3_SHAPES if precision == "bf16x3" else GEMM_SHAPES
+        if len(self.gemm_shapes) != 2 or any(x not in valid for x in self.gemm_shapes):
+            raise ValueError(f"gemm_shapes {self.gemm_shapes} not two of {valid} ({precision})")
         self.steps_done = 0
         self.block_stats = torch.zeros((self.count, 2), dtype=f64, device=dev)
 
@@ -214,10 +225,12 @@ class system_batch:
         rot_pad, rot_h1 = m_ctx.energy_operands(1)
         if self.precision == "bf16x3":
             ops = dict(chol_hl=kernels.split_bf16(chol_sym_rows(s.ham.chol)),
-                       rot_hl=kernels.split_bf16(m_ctx.rot_chol_flat))
+                       rot_hl=kernels.split_bf16(m_ctx.rot_chol_flat),
+                       e_pairs=p_ctx.exp_h1_half_sym_pairs.contiguous())
         else:
             ops = dict(chol_s32=kernels.convert_f32(chol_sym_rows(s.ham.chol)), rot32=m_ctx.fb_operand(1))
-        return dict(n=n, o=o, g=g, nw=s.n_walkers, ldv=kernels.ldv_for(n), h0=float(s.ham.h0),
+        return dict(n=n, o=o, g=g, nw=s.n_walkers, ldv=kernels.ldv_for(n), nsym=kernels.sym_blocks(n) * 256,
+                    h0=float(s.ham.h0),
                     trial=s.trial.mo_up.contiguous(), **ops, rot_pad=rot_pad, rot_h1=rot_h1,
                     e_half=p_ctx.exp_h1_half_pad32.contiguous(), mf=p_ctx.mf_shifts.contiguous(),
                     h0_prop=float(complex(p_ctx.h0_prop).real), e_est=float(st.e_estimate), state0=st)
@@ -242,13 +255,18 @@ class system_batch:
             self.xbar32 = P(self.xbar.sizes, f32)
         self.shift = P([m["nw"] for m in ms], c128)
         self.fbt = P([m["nw"] for m in ms], c128)
-        self.vhs = P([2 * m["nw"] * m["n"] * m["ldv"] for m in ms], f32)
+        if self.precision == "bf16x3":
+            # V as pair words of its upper blocks; E - I likewise
+            self.vhs = P([2 * m["nw"] * m["nsym"] for m in ms], torch.int32)
+            self.e_half = P([m["nsym"] for m in ms], torch.int32)
+        else:
+            self.vhs = P([2 * m["nw"] * m["n"] * m["ldv"] for m in ms], f32)
+            self.e_half = P([m["n"] * m["ldv"] for m in ms], f32)
         self.eloc = P([m["nw"] for m in ms], c128)
         self.lo = [P([m["nw"] for m in ms], c128) for _ in range(2)]
         self.weights = [P([m["nw"] for m in ms], f64) for _ in range(2)]
         self.pop = [torch.zeros((self.count,), dtype=f64, device=dev) for _ in range(2)]
         self.keys = [torch.zeros((self.count, 2), dtype=torch.int32, device=dev) for _ in range(2)]
-        self.e_half = P([m["n"] * m["ldv"] for m in ms], f32)
         self.mf = P([m["g"] for m in ms], c128)
         self.trials = P([m["n"] * m["o"] for m in ms], f64)
         for i, m in enumerate(ms):
@@ -259,7 +277,7 @@ class system_batch:
             self.weights[0].part(i).copy_(st.weights)
             self.pop[0][i] = st.pop_control_ene_shift
             self.keys[0][i] = st.rng_key
-            self.e_half.part(i).copy_(m["e_half"].reshape(-1))
+            self.e_half.part(i).copy_((m["e_pairs"] if self.precision == "bf16x3" else m["e_half"]).reshape(-1))
             self.mf.part(i).copy_(m["mf"])
             self.trials.part(i).copy_(m["trial"].reshape(-1))
 
@@ -317,8 +335,9 @@ class system_batch:
             start += -(-p.M // bm) * -(-p.N // bn)
         return arr, start
 
-    def _problems_bf16x3(self, which: str):
+    def _problems_bf16x3(self, which: str, shape: int):
         pad8 = kernels.pad8
+        bm, bn = gemm_tile_shape(shape, "bf16x3")
         arr = (_gemm_problem_bf16x3 * self.count)()
         start = 0
         for i, m in enumerate(self.members):
@@ -329,16 +348,19 @@ class system_batch:
                 p.M, p.N, p.K = 2 * m["nw"], m["g"], k
                 p.ldc, p.C, p.alpha, p.c_fp64 = m["g"], self.fb.ptr(i), 2.0, 1
             else:
+                # V's upper blocks as pair words, stored as they come
                 k = pad8(m["g"])
                 (bh, bl), (ah, al) = m["chol_hl"], self.xbar_hl
                 p.M, p.N, p.K = 2 * m["nw"], bh.shape[0], k
-                p.ldc, p.C, p.alpha, p.c_fp64 = 0, self.vhs.ptr(i), 1.0, 0
-                p.sym_n, p.sym_ldv = m["n"], m["ldv"]
+                p.ldc, p.C, p.alpha, p.c_fp64 = m["nsym"], self.vhs.ptr(i), 1.0, 2
+                assert bh.shape[0] == m["nsym"]
             assert bh.shape[1] == k and bl.shape[1] == k
+            # buffer descriptors of the LDS-DMA shapes address each plane in 32 bits
+            assert max(p.M, p.N) * k * 2 < 2 ** 31, "operand planes must stay below 2 GiB"
             p.lda = p.ldb = k
             p.A_hi, p.A_lo, p.B_hi, p.B_lo = ah.ptr(i), al.ptr(i), bh.data_ptr(), bl.data_ptr()
             p.tile_start = start
-            start += -(-p.M // BF16X3_TILE) * -(-p.N // BF16X3_TILE)
+            start += -(-p.M // bm) * -(-p.N // bn)
         return arr, start
 
     def _split_table(self, which: str):
@@ -394,8 +416,9 @@ class system_batch:
         self.split_tables = {}
         for which in ("fb", "vhs"):
             if self.precision == "bf16x3":
-                arr, tiles = self._problems_bf16x3(which)
-                self.gemm_tables[which] = (_upload(arr).to(self.device), tiles)
+                for shape in BF16X3_SHAPES:
+                    arr, tiles = self._problems_bf16x3(which, shape)
+                    self.gemm_tables[(which, shape)] = (_upload(arr).to(self.device), tiles)
                 self.split_tables[which] = self._split_table(which)
                 continue
             for shape in GEMM_SHAPES:
@@ -411,8 +434,9 @@ class system_batch:
             stab, biggest = self.split_tables[which]
             kernels._call("afq_split_bf16_grouped", stab, self.count, stab.data_ptr(), biggest,
                           kernels._stream(stab), tag=which)
-            tab, tiles = self.gemm_tables[which]
-            kernels._call("afq_gemm_bf16x3_nt_grouped", tab, self.count, tab.data_ptr(), tiles,
+            shape = self.gemm_shapes[0 if which == "fb" else 1]
+            tab, tiles = self.gemm_tables[(which, shape)]
+            kernels._call("afq_gemm_bf16x3_nt_grouped", tab, self.count, tab.data_ptr(), tiles, shape,
                           kernels._stream(tab), tag=which)
             return
         shape = self.gemm_shapes[0 if which == "fb" else 1]
@@ -447,7 +471,7 @@ class system_batch:
         self._grouped_gemm("vhs")
         for bk in self.trotter_buckets:
             t = bk["tables"][p]
-            kernels._call("afq_trotter_bf16x3_grouped" if self.precision == "bf16x3" else "afq_trotter_f32_grouped",
+            kernels._call("afq_trotter_bf16x3_sym_grouped" if self.precision == "bf16x3" else "afq_trotter_f32_grouped",
                           t, bk["count"], t.data_ptr(), bk["blocks"], bk["max_n"], self.n_exp_terms, st)
         self._greens(p, st)
         kernels._call("afq_phaseless_update_grouped", d, self.count, d.data_ptr(), st)
@@ -530,16 +554,22 @@ class system_batch:
 
     # ------------------------------------------------------------- autotuner
     def signature(self) -> str:
-        return ";".join(f"{m['n']},{m['o']},{m['g']},{m['nw']}" for m in self.members)
+        return self.precision + ":" + ";".join(f"{m['n']},{m['o']},{m['g']},{m['nw']}" for m in self.members)
+
+    def _launch_gemm(self, which: str, shape: int, stream) -> None:
+        tab, tiles = self.gemm_tables[(which, shape)]
+        if self.precision == "bf16x3":
+            _lib.call("afq_gemm_bf16x3_nt_grouped", self.count, tab.data_ptr(), tiles, shape, stream.cuda_stream)
+        else:
+            _lib.call("afq_sgemm_nt_grouped", self.count, tab.data_ptr(), tiles, shape, stream.cuda_stream)
 
     def tune(self, reps: int = 5, cache_path: str | None = None) -> dict:
         """Pick the grouped-GEMM tile shape of the force-bias and the VHS
-        contraction by timing each shape on this batch (HIP events, ``reps``
-        launches after one warm-up).  With ``cache_path`` the choice is read
-        from / written to a JSON file keyed by the batch's problem shapes."""
-        if self.precision != "fp32":
-            # one tile shape: nothing to choose
-            return {"shapes": list(self.gemm_shapes), "ms": {}, "precision": self.precision}
+        contraction by timing each shape of the batch's precision on this
+        batch (HIP events, ``reps`` launches after one warm-up; the operands
+        are whatever the buffers hold, the outputs are overwritten).  With
+        ``cache_path`` the choice is read from / written to a JSON file keyed
+        by the precision and the problem shapes."""
         key = self.signature()
         cache = {}
         if cache_path and os.path.exists(cache_path):
@@ -549,26 +579,26 @@ class system_batch:
                 self.gemm_shapes = tuple(cache[key]["shapes"])
                 return cache[key]
         stream = torch.cuda.current_stream(self.device)
+        shapes = BF16X3_SHAPES if self.precision == "bf16x3" else GEMM_SHAPES
         timings = {}
         best = []
         for which in ("fb", "vhs"):
             ms = {}
-            for shape in GEMM_SHAPES:
-                tab, tiles = self.gemm_tables[(which, shape)]
-                _lib.call("afq_sgemm_nt_grouped", self.count, tab.data_ptr(), tiles, shape, stream.cuda_stream)
+            for shape in shapes:
+                self._launch_gemm(which, shape, stream)
                 e0 = torch.cuda.Event(enable_timing=True)
                 e1 = torch.cuda.Event(enable_timing=True)
                 e0.record(stream)
                 for _ in range(reps):
-                    _lib.call("afq_sgemm_nt_grouped", self.count, tab.data_ptr(), tiles, shape,
-                              stream.cuda_stream)
+                    self._launch_gemm(which, shape, stream)
                 e1.record(stream)
                 e1.synchronize()
                 ms[shape] = e0.elapsed_time(e1) / reps
             timings[which] = ms
             best.append(min(ms, key=ms.get))
         self.gemm_shapes = tuple(best)
-        entry = {"shapes": list(best), "ms": {w: {str(k): v for k, v in t.items()} for w, t in timings.items()}}
+        entry = {"shapes": list(best), "precision": self.precision,
+                 "ms": {w: {str(k): v for k, v in t.items()} for w, t in timings.items()}}
         if cache_path:
             cache[key] = entry
             os.makedirs(os.path.dirname(os.path.abspath(cache_path)), exist_ok=True)

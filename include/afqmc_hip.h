@@ -484,26 +484,35 @@ int afq_sgemm_nt_grouped(int count, const afq_gemm_problem* problems, int total_
  * multiples of 8, planes 16-byte aligned. */
 typedef struct afq_gemm_problem_bf16x3 {
     int M, N, K;
-    int tile_start;              /* first workgroup: prefix sum of the problems' 128 x 128 tiles */
+    int tile_start;              /* first workgroup: prefix sum of the problems' tiles of the launch's shape */
     long long lda, ldb, ldc;
     const unsigned short* A_hi;
     const unsigned short* A_lo;
     const unsigned short* B_hi;
     const unsigned short* B_lo;
-    void* C;                     /* float32, or float64 when c_fp64 != 0 */
+    void* C;                     /* c_fp64 0: float32, 1: float64, 2: split-bf16 pair words (afq_gemm_bf16x3_nt) */
     float alpha;
     int c_fp64;
     int sym_n, sym_ldv;          /* sym_n > 0: the symmetric VHS store of afq_gemm_problem */
 } afq_gemm_problem_bf16x3;
 
 /*
- * Every problem's bf16x3 GEMM (afq_gemm_bf16x3_nt's 128 x 128 tile) in one
- * launch of total_tiles workgroups.  The force-bias and VHS contractions of
- * every system of a grouped step in split-bf16 precision (meas/rhf.py:33,
- * prop/chol_afqmc_ops.py:72-73 per system).
+ * Every problem's bf16x3 GEMM in one launch of total_tiles workgroups, all in
+ * tile shape `shape` (afq_gemm_bf16x3_tile_shape: 0 = 128 x 128 staged
+ * through registers, 1 = 128 x 256 and 2 = 256 x 256 moved to LDS by
+ * buffer_load ... lds; every problem's A and B planes below 2 GiB).  Each
+ * element's sum runs in the same order in every shape, so the shapes give the
+ * same bits.  The force-bias and VHS contractions of every system of a grouped
+ * step in split-bf16 precision (meas/rhf.py:33, prop/chol_afqmc_ops.py:72-73
+ * per system).
  */
-int afq_gemm_bf16x3_nt_grouped(int count, const afq_gemm_problem_bf16x3* problems, int total_tiles,
+int afq_gemm_bf16x3_nt_grouped(int count, const afq_gemm_problem_bf16x3* problems, int total_tiles, int shape,
                                void* stream);
+
+/* Rows (*bm) and columns (*bn) of C per workgroup of afq_gemm_bf16x3_nt_grouped
+ * tile shape `shape` (0..2); no counterpart in the reference (tile choice of
+ * the batched step's autotuner). */
+int afq_gemm_bf16x3_tile_shape(int shape, int* bm, int* bn);
 
 /* One problem of afq_split_bf16_grouped: afq_split_bf16's arguments. */
 typedef struct afq_split_problem {
@@ -548,6 +557,17 @@ int afq_trotter_f32_grouped(int count, const afq_system_step* sys, int total_blo
  */
 int afq_trotter_bf16x3_grouped(int count, const afq_system_step* sys, int total_blocks, int max_n,
                                int n_terms, void* stream);
+
+/*
+ * afq_trotter_bf16x3_sym for every system of one bucket, like
+ * afq_trotter_bf16x3_grouped: the descriptor's vhs holds the system's V as
+ * split-bf16 pair words of its upper 16 x 16 blocks (2 planes of nw *
+ * nb (nb + 1) / 2 * 256 words, nb = ceil(n / 16); the pair output of the VHS
+ * GEMM) and exp_h1_half holds E - I in the same layout (prop/chol_afqmc_ops.py:173-196,
+ * prop/utils.py:20-31).
+ */
+int afq_trotter_bf16x3_sym_grouped(int count, const afq_system_step* sys, int total_blocks, int max_n,
+                                   int n_terms, void* stream);
 
 /*
  * afq_overlap_greens for every system, of walkers_out into greens / logdet

@@ -163,27 +163,47 @@ def test_batch_graph_replay_matches_eager(gpu_device, precision):
         torch.zeros(1 << 20, device=gpu_device).sum().item()
 
 
-def test_tune_chooses_shapes_and_caches(gpu_device, tmp_path):
-    from ad_afqmc_prototype_amd.prop.batched import GEMM_SHAPES, system_batch
-    batch = system_batch(_members(gpu_device), dt=DT, qr_every=QR, sr_every=SR, precision="fp32")
+@pytest.mark.parametrize("precision", ["fp32", "bf16x3"])
+def test_tune_chooses_shapes_and_caches(gpu_device, tmp_path, precision):
+    from ad_afqmc_prototype_amd.prop.batched import BF16X3_SHAPES, GEMM_SHAPES, system_batch
+    shapes = BF16X3_SHAPES if precision == "bf16x3" else GEMM_SHAPES
+    batch = system_batch(_members(gpu_device), dt=DT, qr_every=QR, sr_every=SR, precision=precision)
     path = str(tmp_path / "tiles.json")
     entry = batch.tune(reps=2, cache_path=path)
     assert tuple(entry["shapes"]) == batch.gemm_shapes
-    assert all(s in GEMM_SHAPES for s in batch.gemm_shapes)
-    again = system_batch(_members(gpu_device), dt=DT, qr_every=QR, sr_every=SR, precision="fp32")
+    assert all(s in shapes for s in batch.gemm_shapes)
+    assert all(set(entry["ms"][w]) == {str(s) for s in shapes} for w in ("fb", "vhs"))
+    again = system_batch(_members(gpu_device), dt=DT, qr_every=QR, sr_every=SR, precision=precision)
     assert again.tune(cache_path=path)["shapes"] == entry["shapes"]
+    # every shape advances the batch identically (same sums, same order)
+    import torch
+    runs = []
+    for fb_shape, vhs_shape in [(shapes[0], shapes[0]), (shapes[-1], shapes[-1])]:
+        b = system_batch(_members(gpu_device), dt=DT, qr_every=QR, sr_every=SR, precision=precision,
+                         gemm_shapes=(fb_shape, vhs_shape))
+        b.advance(SR)
+        torch.cuda.synchronize()
+        runs.append([b.system_state(i) for i in range(len(SYSTEMS))] + [b.block_stats.clone()])
+    for x, y in zip(runs[0][:-1], runs[1][:-1]):
+        assert torch.equal(x["walkers"], y["walkers"]) and torch.equal(x["weights"], y["weights"])
+    assert torch.equal(runs[0][-1], runs[1][-1])
 
 
-def test_grouped_bf16x3_gemm_and_split_match_single(gpu_device):
+@pytest.mark.parametrize("shape", [0, 1, 2])
+def test_grouped_bf16x3_gemm_and_split_match_single(gpu_device, shape):
+    """Every grouped bf16x3 tile shape (128 x 128 through registers, 128 x 256
+    and 256 x 256 through buffer_load ... lds) equals afq_gemm_bf16x3_nt on
+    each problem, bit for bit, in all three output forms (fp32, fp64, pairs)."""
     import torch
     from ad_afqmc_prototype_amd import _lib, kernels
     from ad_afqmc_prototype_amd.prop import batched as B
     g = torch.Generator(device="cpu").manual_seed(5)
-    dims = [(1, 1, 1), (37, 70, 19), (130, 129, 260), (64, 700, 96), (300, 17, 1001)]
+    dims = [(1, 1, 1), (37, 70, 19), (130, 129, 260), (64, 700, 96), (300, 17, 1001), (513, 300, 64)]
     st = torch.cuda.current_stream().cuda_stream
     ops, splits = [], (B._split_problem * len(dims))()
     sources = []  # the split reads them after the loop: keep them allocated
     biggest = 0
+    out_dtype = {0: torch.float32, 1: torch.float64, 2: torch.int32}
     for i, (M, N, K) in enumerate(dims):
         a = torch.randn((M, K), generator=g, dtype=torch.float64).to(gpu_device)
         b = torch.randn((N, K), generator=g, dtype=torch.float64).to(gpu_device)
@@ -197,31 +217,33 @@ def test_grouped_bf16x3_gemm_and_split_match_single(gpu_device):
         q.src, q.hi, q.lo = a.data_ptr(), gh.data_ptr(), gl.data_ptr()
         q.rows, q.cols, q.ld_src, q.ld_dst = M, K, K, kp
         biggest = max(biggest, M * kp)
-        c64 = i % 2 == 0
-        ref = torch.empty((M, N), dtype=torch.float64 if c64 else torch.float32, device=gpu_device)
+        mode = i % 3
+        ref = torch.empty((M, N), dtype=out_dtype[mode], device=gpu_device)
         _lib.call("afq_gemm_bf16x3_nt", M, N, kp, 0.75, ah.data_ptr(), al.data_ptr(), kp, bh.data_ptr(),
-                  bl.data_ptr(), kp, ref.data_ptr(), N, int(c64), st)
-        ops.append((ah, al, bh, bl, gh, gl, ref, c64))
+                  bl.data_ptr(), kp, ref.data_ptr(), N, mode, st)
+        ops.append((ah, al, bh, bl, gh, gl, ref, mode))
     stab = B._upload(splits).to(gpu_device)
     _lib.call("afq_split_bf16_grouped", len(dims), stab.data_ptr(), biggest, st)
+    bm, bn = B.gemm_tile_shape(shape, "bf16x3")
+    assert (bm, bn) == [(128, 128), (128, 256), (256, 256)][shape]
     arr = (B._gemm_problem_bf16x3 * len(ops))()
     outs, start = [], 0
-    for p, (ah, al, bh, bl, gh, gl, ref, c64) in zip(arr, ops):
-        out = torch.full_like(ref, float("nan"))
+    for p, (ah, al, bh, bl, gh, gl, ref, mode) in zip(arr, ops):
+        out = torch.full_like(ref, -7)
         outs.append(out)
         M, kp = ah.shape
         N = bh.shape[0]
         p.M, p.N, p.K, p.tile_start = M, N, kp, start
         p.lda, p.ldb, p.ldc = kp, kp, N
         p.A_hi, p.A_lo, p.B_hi, p.B_lo = gh.data_ptr(), gl.data_ptr(), bh.data_ptr(), bl.data_ptr()
-        p.C, p.alpha, p.c_fp64 = out.data_ptr(), 0.75, int(c64)
-        start += -(-M // B.BF16X3_TILE) * -(-N // B.BF16X3_TILE)
+        p.C, p.alpha, p.c_fp64 = out.data_ptr(), 0.75, mode
+        start += -(-M // bm) * -(-N // bn)
     tab = B._upload(arr).to(gpu_device)
-    _lib.call("afq_gemm_bf16x3_nt_grouped", len(ops), tab.data_ptr(), start, st)
+    _lib.call("afq_gemm_bf16x3_nt_grouped", len(ops), tab.data_ptr(), start, shape, st)
     torch.cuda.synchronize()
     for i, (out, (ah, al, _, _, gh, gl, ref, _)) in enumerate(zip(outs, ops)):
         assert torch.equal(gh, ah) and torch.equal(gl, al), f"problem {i}: grouped split differs"
-        assert torch.equal(out, ref), f"problem {i}: grouped bf16x3 GEMM differs"
+        assert torch.equal(out, ref), f"problem {i}: grouped bf16x3 GEMM (shape {shape}) differs"
 
 
 def test_batch_bf16x3_close_to_fp32(gpu_device):
@@ -327,35 +349,37 @@ def test_cfg4_batch_block(gpu_device):
 
 def test_grouped_bf16x3_trotter_buckets_match_single(gpu_device):
     """One grouped bf16x3 Trotter launch per bucket (n = 40, 160, 176, 208:
-    no, one and two V k-blocks held in LDS) equals afq_trotter_bf16x3 on each
-    system alone, bit for bit, with random complex-symmetric V rows."""
+    no, one and two V k-blocks held in LDS) equals afq_trotter_bf16x3_sym on
+    each system alone, bit for bit, with random complex-symmetric V given as
+    pair words of its upper blocks."""
     import torch
     from ad_afqmc_prototype_amd import kernels
     from ad_afqmc_prototype_amd.prop.batched import system_batch
+    from ad_afqmc_prototype_amd.prop.chol_afqmc_ops import chol_sym_rows, split_pairs
     systems = [(111, 40, 6, 170, 24), (112, 160, 24, 700, 20), (113, 176, 17, 800, 12),
                (114, 208, 32, 900, 10)]
     members = _members(gpu_device, systems, scale=0.05)
     batch = system_batch(members, dt=DT, qr_every=QR, sr_every=SR, precision="bf16x3")
     g = torch.Generator(device="cpu").manual_seed(9)
     for i, m in enumerate(batch.members):
-        nw, n, ldv = m["nw"], m["n"], m["ldv"]
-        a = torch.randn((2, nw, n, n), generator=g, dtype=torch.float64)
-        a = 0.5 * (a + a.transpose(2, 3))
+        nw, n = m["nw"], m["n"]
+        a = torch.randn((2 * nw, n, n), generator=g, dtype=torch.float64)
+        a = 0.5 * (a + a.transpose(1, 2))
         a *= 0.2 / (DT ** 0.5 * n ** 0.5)
-        v = torch.zeros((2, nw, n, ldv), dtype=torch.float32)
-        v[..., :n] = a.float()
-        batch.vhs.part(i, (2, nw, n, ldv)).copy_(v.to(gpu_device))
+        # fp32 values, in the layout of the VHS GEMM's pair output: (2 nw, nsym)
+        blocks = chol_sym_rows(a.float().double().to(gpu_device)).T.float().contiguous()
+        batch.vhs.part(i, (2, nw, m["nsym"])).copy_(split_pairs(blocks).view(2, nw, m["nsym"]))
     st = kernels._stream(batch.desc[0])
     assert len(batch.trotter_buckets) == 4
     for bk in batch.trotter_buckets:
         t = bk["tables"][0]
-        kernels._call("afq_trotter_bf16x3_grouped", t, bk["count"], t.data_ptr(), bk["blocks"], bk["max_n"],
+        kernels._call("afq_trotter_bf16x3_sym_grouped", t, bk["count"], t.data_ptr(), bk["blocks"], bk["max_n"],
                       6, st)
     torch.cuda.synchronize()
     for i, m in enumerate(batch.members):
-        nw, n, o, ldv = m["nw"], m["n"], m["o"], m["ldv"]
+        nw, n, o = m["nw"], m["n"], m["o"]
         win = batch.walkers[0].part(i, (nw, n, o))
-        ref = kernels.trotter(win, batch.vhs.part(i, (2, nw, n, ldv)), batch.e_half.part(i, (n, ldv)),
+        ref = kernels.trotter(win, batch.vhs.part(i, (2, nw, m["nsym"])), batch.e_half.part(i),
                               batch.sqrt_dt, 6, arith="bf16x3")
         got = batch.walkers[1].part(i, (nw, n, o))
         assert torch.equal(got, ref), f"system {i} (n={n}): grouped bf16x3 Trotter differs"
