@@ -81,6 +81,8 @@ _SIGNATURES = {
     "afq_phaseless_update_grouped": [c_int, c_vp, c_vp],
     "afq_orthonormalize_grouped": [c_int, c_vp, c_int, c_int, c_int, c_vp],
     "afq_energy_f32_grouped": [c_int, c_vp, c_int, c_int, c_int, c_vp],
+    "afq_energy_bf16x3_grouped": [c_int, c_vp, c_int, c_int, c_int, c_int, c_vp],
+    "afq_energy_bf16x3_grouped_fits": [c_int, c_int, c_int],
     # Hamiltonian ingestion (modified Cholesky)
     "afq_mcd_workspace": [c_int, ctypes.POINTER(c_ll), ctypes.POINTER(c_ll)],
     "afq_mcd_start": [c_int, c_vp, c_ll, c_double, c_int, c_vp, c_vp, c_vp, c_vp, c_vp],
@@ -88,7 +90,7 @@ _SIGNATURES = {
     "afq_unpack_pairs": [c_int, c_int, c_vp, c_ll, c_vp, c_vp],
 }
 
-ABI_VERSION = 13
+ABI_VERSION = 14
 
 
 class HipLibraryError(RuntimeError):

@@ -688,15 +688,17 @@ size_t energy_bf16x3_lds(int n, int nspin, int naux) {
     return b;
 }
 
+// Walkers w0 .. w0 + WPB - 1 of one batch (nw walkers), in a workgroup of
+// THREADS_B threads whose dynamic LDS (lds_raw) holds energy_bf16x3_lds for
+// this ldr or a larger one.
 template <int NT, int WPB>
-__global__ __launch_bounds__(THREADS_B, 1) void energy_bf16x3_kernel(
+__device__ __forceinline__ void energy_bf16x3_body(
     int nw, int n, int o, int nspin, int naux, double h0, const double* __restrict__ greens,
     const uint16_t* __restrict__ rot_hi, const uint16_t* __restrict__ rot_lo, int ldr,
-    const double* __restrict__ rot_h1, double2* __restrict__ eout) {
+    const double* __restrict__ rot_h1, double2* __restrict__ eout, int w0, unsigned char* lds_raw) {
     using MF = afq::mfma16<float>;
     constexpr int TC = NT * 16;
     constexpr int SP = TC + 1;
-    extern __shared__ __attribute__((aligned(16))) unsigned char lds_raw[];
     __shared__ double red[8][6 * WPB];
     const int nk = ldr / 32;
     const int PP = ldr + 8;  // bf16 pitch of one column of G
@@ -706,7 +708,6 @@ __global__ __launch_bounds__(THREADS_B, 1) void energy_bf16x3_kernel(
     cs_off = (cs_off + 15) & ~(size_t)15;
     double2* cs = reinterpret_cast<double2*>(lds_raw + cs_off);   // [WPB][naux] (nspin == 2)
 
-    const int w0 = blockIdx.x * WPB;
     const int nwb = min(WPB, nw - w0);
     const long long ldg = (long long)nspin * o * n;
     const int tid = threadIdx.x;
@@ -900,6 +901,34 @@ __global__ __launch_bounds__(THREADS_B, 1) void energy_bf16x3_kernel(
 }
 
 template <int NT, int WPB>
+__global__ __launch_bounds__(THREADS_B, 1) void energy_bf16x3_kernel(
+    int nw, int n, int o, int nspin, int naux, double h0, const double* __restrict__ greens,
+    const uint16_t* __restrict__ rot_hi, const uint16_t* __restrict__ rot_lo, int ldr,
+    const double* __restrict__ rot_h1, double2* __restrict__ eout) {
+    extern __shared__ __attribute__((aligned(16))) unsigned char lds_raw[];
+    energy_bf16x3_body<NT, WPB>(nw, n, o, nspin, naux, h0, greens, rot_hi, rot_lo, ldr, rot_h1, eout,
+                                blockIdx.x * WPB, lds_raw);
+}
+
+// Every system of a bucket of equal ceil(o / 16) == NT (restricted walkers):
+// workgroup -> (system, WPB walkers) by the prefix sums tblock0 of
+// ceil(nw / WPB); the system's rot_chol points at its hi plane, followed by
+// the lo plane, (naux * 16 NT, ldr) each, ldr = n rounded up to 32.  The
+// same body and walker grouping as the single-system kernel with this WPB.
+template <int NT, int WPB>
+__global__ __launch_bounds__(THREADS_B, 1) void energy_bf16x3_grouped_kernel(int count,
+                                                                             const afq_system_step* __restrict__ S) {
+    extern __shared__ __attribute__((aligned(16))) unsigned char lds_raw[];
+    const int b = blockIdx.x;
+    const int si = afq::find_group(count, b, [&](int i) { return S[i].tblock0; });
+    const afq_system_step& d = S[si];
+    const uint16_t* rh = reinterpret_cast<const uint16_t*>(d.rot_chol);
+    const uint16_t* rl = rh + (long long)d.naux * (16 * NT) * d.ldr;
+    energy_bf16x3_body<NT, WPB>(d.nw, d.n, d.o, 1, d.naux, d.h0, d.greens, rh, rl, d.ldr, d.rot_h1,
+                                (double2*)d.energy, (b - d.tblock0) * WPB, lds_raw);
+}
+
+template <int NT, int WPB>
 int launch_energy_bf16x3(int nw, int n, int o, int nspin, int naux, double h0, const double* greens,
                          const uint16_t* rh, const uint16_t* rl, int ldr, const double* rot_h1,
                          double* energy, hipStream_t st) {
@@ -979,4 +1008,45 @@ extern "C" int afq_energy_f32_grouped(int count, const afq_system_step* sys, int
                            count, sys);
     }
     return afq::check_launch("afq_energy_f32_grouped");
+}
+
+extern "C" int afq_energy_bf16x3_grouped(int count, const afq_system_step* sys, int total_blocks, int max_n, int nt,
+                                         int wpb, void* stream) {
+    AFQ_REQUIRE(count > 0 && total_blocks > 0, "afq_energy_bf16x3_grouped: count=%d total_blocks=%d", count,
+                total_blocks);
+    AFQ_REQUIRE(sys, "afq_energy_bf16x3_grouped: null pointer");
+    AFQ_REQUIRE(nt == 1 || nt == 2, "afq_energy_bf16x3_grouped: nt=%d must be 1 or 2", nt);
+    AFQ_REQUIRE(wpb == 1 || wpb == 2, "afq_energy_bf16x3_grouped: wpb=%d must be 1 or 2", wpb);
+    AFQ_REQUIRE(max_n > 0 && max_n <= 32 * KMAX_E, "afq_energy_bf16x3_grouped: max_n=%d not in 1..%d", max_n,
+                32 * KMAX_E);
+    const int ldr = afq::round_up(max_n, 32);
+    hipStream_t st = (hipStream_t)stream;
+#define AFQ_ENERGY_GROUPED(NT_, WPB_)                                                                          \
+    if (nt == NT_ && wpb == WPB_) {                                                                            \
+        const size_t lds = energy_bf16x3_lds<NT_, WPB_>(ldr, 1, 0);                                            \
+        AFQ_REQUIRE(lds + 8 * 6 * WPB_ * sizeof(double) <= 160 * 1024,                                        \
+                    "afq_energy_bf16x3_grouped: max_n=%d nt=%d wpb=%d needs more than 160 KB of LDS", max_n, nt, \
+                    wpb);                                                                                      \
+        if (lds > 64 * 1024) afq::allow_full_lds((const void*)energy_bf16x3_grouped_kernel<NT_, WPB_>);        \
+        hipLaunchKernelGGL((energy_bf16x3_grouped_kernel<NT_, WPB_>), dim3((unsigned)total_blocks),            \
+                           dim3(THREADS_B), lds, st, count, sys);                                              \
+        return afq::check_launch("afq_energy_bf16x3_grouped");                                                 \
+    }
+    AFQ_ENERGY_GROUPED(1, 1)
+    AFQ_ENERGY_GROUPED(1, 2)
+    AFQ_ENERGY_GROUPED(2, 1)
+    AFQ_ENERGY_GROUPED(2, 2)
+#undef AFQ_ENERGY_GROUPED
+    afq::set_error("afq_energy_bf16x3_grouped: no kernel for nt=%d wpb=%d", nt, wpb);
+    return -1;
+}
+
+extern "C" int afq_energy_bf16x3_grouped_fits(int max_n, int nt, int wpb) {
+    const int ldr = afq::round_up(max_n, 32);
+    size_t lds = 0;
+    if (nt == 1)
+        lds = wpb == 1 ? energy_bf16x3_lds<1, 1>(ldr, 1, 0) : energy_bf16x3_lds<1, 2>(ldr, 1, 0);
+    else
+        lds = wpb == 1 ? energy_bf16x3_lds<2, 1>(ldr, 1, 0) : energy_bf16x3_lds<2, 2>(ldr, 1, 0);
+    return lds + 8 * 6 * wpb * sizeof(double) <= 160 * 1024 ? 1 : 0;
 }

@@ -222,7 +222,18 @@ class system_batch:
         m_ctx = m_ops.build_meas_ctx(s.ham, s.trial)
         st = init_prop_state(sys=sys_, n_walkers=s.n_walkers, seed=s.seed, ham_data=s.ham,
                              trial_ops_=t_ops, trial_data=s.trial, meas_ops=m_ops)
-        rot_pad, rot_h1 = m_ctx.energy_operands(1)
+        if self.precision == "bf16x3":
+            # split planes of the half-rotated Cholesky in the bf16x3 energy
+            # layout: (naux, round16(o), round32(n)) per plane, hi then lo
+            opad, ldr = (o + 15) // 16 * 16, (n + 31) // 32 * 32
+            pad = torch.zeros((g, opad, ldr), dtype=f64, device=s.ham.h1.device)
+            pad[:, :o, :n] = m_ctx.rot_chol[0]
+            rot_pad = torch.stack(kernels.split_bf16(pad.reshape(-1, ldr))).contiguous()
+            del pad
+            rot_h1 = m_ctx.rot_h1[:1].contiguous()
+        else:
+            ldr = kernels.ldv_for(n)
+            rot_pad, rot_h1 = m_ctx.energy_operands(1)
         if self.precision == "bf16x3":
             ops = dict(chol_hl=kernels.split_bf16(chol_sym_rows(s.ham.chol)),
                        rot_hl=kernels.split_bf16(m_ctx.rot_chol_flat),
@@ -231,7 +242,7 @@ class system_batch:
             ops = dict(chol_s32=kernels.convert_f32(chol_sym_rows(s.ham.chol)), rot32=m_ctx.fb_operand(1))
         return dict(n=n, o=o, g=g, nw=s.n_walkers, ldv=kernels.ldv_for(n), nsym=kernels.sym_blocks(n) * 256,
                     h0=float(s.ham.h0),
-                    trial=s.trial.mo_up.contiguous(), **ops, rot_pad=rot_pad, rot_h1=rot_h1,
+                    trial=s.trial.mo_up.contiguous(), **ops, rot_pad=rot_pad, rot_h1=rot_h1, ldr=ldr,
                     e_half=p_ctx.exp_h1_half_pad32.contiguous(), mf=p_ctx.mf_shifts.contiguous(),
                     h0_prop=float(complex(p_ctx.h0_prop).real), e_est=float(st.e_estimate), state0=st)
 
@@ -291,7 +302,7 @@ class system_batch:
         w0 = t0 = 0
         for d, i in zip(arr, idx):
             m = self.members[i]
-            d.n, d.o, d.naux, d.nw, d.ldv, d.ldr = m["n"], m["o"], m["g"], m["nw"], m["ldv"], m["ldv"]
+            d.n, d.o, d.naux, d.nw, d.ldv, d.ldr = m["n"], m["o"], m["g"], m["nw"], m["ldv"], m["ldr"]
             d.walker0, d.tblock0 = w0, t0
             w0 += m["nw"]
             t0 += m["nw"] * ((m["o"] + 15) // 16)
@@ -409,9 +420,20 @@ class system_batch:
         self.energy_buckets = []
         for idx in buckets(lambda m: (m["o"] + 15) // 16):
             arr, nw, _ = self._desc(0, idx)
+            max_n = max(self.members[i]["n"] for i in idx)
+            nt = (self.members[idx[0]]["o"] + 15) // 16
+            blocks, wpb = nw, 1
+            if self.precision == "bf16x3":
+                # two walkers per workgroup (half the rotated-Cholesky reads)
+                # when their Green's-function planes fit in LDS; tblock0 then
+                # counts the energy workgroups
+                wpb = 2 if _lib.load().afq_energy_bf16x3_grouped_fits(max_n, nt, 2) else 1
+                blocks = 0
+                for d, i in zip(arr, idx):
+                    d.tblock0 = blocks
+                    blocks += -(-self.members[i]["nw"] // wpb)
             self.energy_buckets.append(dict(table=_upload(arr).to(self.device), count=len(idx),
-                                            walkers=nw, max_n=max(self.members[i]["n"] for i in idx),
-                                            nt=(self.members[idx[0]]["o"] + 15) // 16))
+                                            walkers=nw, blocks=blocks, wpb=wpb, max_n=max_n, nt=nt))
         self.gemm_tables = {}
         self.split_tables = {}
         for which in ("fb", "vhs"):
@@ -497,8 +519,12 @@ class system_batch:
         thresh = (2.0 / self.dt) ** 0.5
         for bk in self.energy_buckets:
             t = bk["table"]
-            kernels._call("afq_energy_f32_grouped", t, bk["count"], t.data_ptr(), bk["walkers"],
-                          bk["max_n"], bk["nt"], kernels._stream(t))
+            if self.precision == "bf16x3":
+                kernels._call("afq_energy_bf16x3_grouped", t, bk["count"], t.data_ptr(), bk["blocks"],
+                              bk["max_n"], bk["nt"], bk["wpb"], kernels._stream(t))
+            else:
+                kernels._call("afq_energy_f32_grouped", t, bk["count"], t.data_ptr(), bk["walkers"],
+                              bk["max_n"], bk["nt"], kernels._stream(t))
         for i, m in enumerate(self.members):
             nw, n, o = m["nw"], m["n"], m["o"]
             g = self.greens.part(i, (2, nw, o * n))

@@ -245,7 +245,7 @@ def end_to_end(workload: str, precision: str, device, n_eql: int, n_blocks: int)
 
 
 def batched_run(device, n_steps: int, n_systems: int = 64, chol_scale: float = 0.05,
-                baseline_steps: int = 20):
+                baseline_steps: int = 20, baselines: bool = True):
     """Uneven-system batch (N ~ U[40,160], nocc ~ U[6,24], naux = round(N U[4,6]),
     walkers ~ U[64,512], seeds 100..), fp32 contractions, dt 0.005, QR every 10
     and measurement + comb every 20 steps: walker-steps/s of the grouped step
@@ -301,6 +301,8 @@ def batched_run(device, n_steps: int, n_systems: int = 64, chol_scale: float = 0
     out["block_energy_finite"] = bool(np.isfinite(stats).all())
     del cap, batch
     torch.cuda.empty_cache()
+    if not baselines:
+        return out
     # the same batch in fp32 (fp32 contractions with tuned tile shapes, fp32 Trotter)
     b32 = system_batch(members, dt=dt, qr_every=qr, sr_every=sr, precision="fp32")
     t32 = b32.tune(reps=5)
@@ -544,6 +546,8 @@ def main():
                          "0 0 skips it")
     ap.add_argument("--batched-steps", type=int, default=100,
                     help="steps of the 64-system uneven batch for the batched field; 0 skips it")
+    ap.add_argument("--batched-no-baselines", action="store_true",
+                    help="run only the bf16x3 batch (no fp32 batch, no one-system-at-a-time run)")
     ap.add_argument("--no-adjoint", action="store_true",
                     help="skip the adjoint field (gradient of the 96-orbital run vs central differences)")
     ap.add_argument("--no-cadence", action="store_true",
@@ -676,7 +680,7 @@ def main():
             e2e["max_abs_diff_mHa_common_trajectory"] = max(diffs[:same], default=None)
         out["end_to_end"] = e2e
     if args.batched_steps > 0:
-        out["batched"] = batched_run(device, args.batched_steps)
+        out["batched"] = batched_run(device, args.batched_steps, baselines=not args.batched_no_baselines)
     if not args.no_adjoint:
         out["adjoint"] = adjoint_run_bench(device)
     if not args.no_cpu_baseline:

@@ -415,8 +415,9 @@ typedef struct afq_system_step {
     int n, o, naux, nw;          /* orbitals, occupied orbitals, Cholesky vectors, walkers */
     int ldv;                     /* n rounded up to 4: row pitch of exp_h1_half and VHS */
     int walker0;                 /* first walker of this system in the batch (prefix sum of nw) */
-    int tblock0;                 /* first Trotter workgroup: prefix sum of nw * ceil(o / 16) */
-    int ldr;                     /* n rounded up to 4: row pitch of rot_chol */
+    int tblock0;                 /* first workgroup of the launch this table serves: prefix sum of
+                                    nw * ceil(o / 16) (Trotter), of ceil(nw / wpb) (bf16x3 energy) */
+    int ldr;                     /* row pitch of rot_chol: n rounded up to 4 (fp32), to 32 (bf16x3) */
     double sqrt_dt, dt, e_estimate, h0_prop;
     double weight_floor, weight_cap, damping, h0;
     const double* trial;         /* (n, o) float64 RHF orbitals */
@@ -430,8 +431,9 @@ typedef struct afq_system_step {
     double* xbar;                /* shifted-field planes (2, nw, naux) */
     double* shift_term;          /* complex128 (nw) */
     double* fb_term;             /* complex128 (nw) */
-    const float* exp_h1_half;    /* (n, ldv) float32 */
-    const float* vhs;            /* planes (2, nw, n, ldv) float32 */
+    const float* exp_h1_half;    /* (n, ldv) float32; bf16x3 pair-word Trotter: E - I as upper-block pairs */
+    const float* vhs;            /* planes (2, nw, n, ldv) float32; bf16x3 pair-word Trotter: (2, nw,
+                                    nb (nb + 1) / 2 * 256) pair words of V's upper 16 x 16 blocks */
     const double* log_overlaps_in;  /* complex128 (nw) */
     double* log_overlaps_out;       /* complex128 (nw) */
     const double* weights_in;    /* (nw) */
@@ -440,7 +442,8 @@ typedef struct afq_system_step {
     double* pop_out;             /* float64[1] */
     const unsigned* key_in;      /* uint32[2] Threefry key */
     unsigned* key_out;           /* uint32[2] */
-    const float* rot_chol;       /* (naux, round16(o), ldr) float32 half-rotated Cholesky, zero padded */
+    const float* rot_chol;       /* (naux, round16(o), ldr) float32 half-rotated Cholesky, zero padded;
+                                    bf16x3 energy: its hi bf16 plane, then the lo plane, same shape */
     const double* rot_h1;        /* (o, n) float64 */
     double* energy;              /* complex128 (nw) local energies of walkers_out */
 } afq_system_step;
@@ -586,6 +589,20 @@ int afq_phaseless_update_grouped(int count, const afq_system_step* sys, void* st
  */
 int afq_energy_f32_grouped(int count, const afq_system_step* sys, int total_walkers, int max_n,
                            int nt, void* stream);
+
+/*
+ * afq_energy_bf16x3 (restricted walkers) for every system of a table whose
+ * systems all have ceil(o / 16) == nt (1 or 2): wpb (1 or 2) walkers per
+ * workgroup, tblock0 the prefix sums of ceil(nw / wpb), total_blocks their
+ * sum; rot_chol the split planes of the half-rotated Cholesky (hi, then lo;
+ * ldr = n rounded up to 32); max_n the largest n (<= 256) (meas/rhf.py:36-50).
+ */
+int afq_energy_bf16x3_grouped(int count, const afq_system_step* sys, int total_blocks, int max_n, int nt,
+                              int wpb, void* stream);
+
+/* 1 when afq_energy_bf16x3_grouped's LDS for (max_n, nt, wpb) fits in a CU,
+ * else 0.  Host-only query, no device work; no counterpart in the reference. */
+int afq_energy_bf16x3_grouped_fits(int max_n, int nt, int wpb);
 
 /*
  * afq_orthonormalize for every system: walkers_in -> walkers_out

@@ -246,6 +246,32 @@ def test_grouped_bf16x3_gemm_and_split_match_single(gpu_device, shape):
         assert torch.equal(out, ref), f"problem {i}: grouped bf16x3 GEMM (shape {shape}) differs"
 
 
+def test_grouped_bf16x3_energy_matches_single(gpu_device):
+    """The grouped bf16x3 energy (one launch per occupied-tile bucket, two
+    walkers per workgroup where they fit) equals afq_energy_bf16x3 on each
+    system alone, bit for bit, on the batch's Green's functions after two steps."""
+    import torch
+    from ad_afqmc_prototype_amd import kernels
+    from ad_afqmc_prototype_amd.prop.batched import system_batch
+    batch = system_batch(_members(gpu_device), dt=DT, qr_every=QR, sr_every=SR, precision="bf16x3")
+    batch.advance(2)
+    batch._orthonormalize()
+    st = kernels._stream(batch.desc[0])
+    assert {bk["nt"] for bk in batch.energy_buckets} == {1, 2}
+    for bk in batch.energy_buckets:
+        t = bk["table"]
+        kernels._call("afq_energy_bf16x3_grouped", t, bk["count"], t.data_ptr(), bk["blocks"], bk["max_n"],
+                      bk["nt"], bk["wpb"], st)
+    torch.cuda.synchronize()
+    for i, m in enumerate(batch.members):
+        nw, n, o = m["nw"], m["n"], m["o"]
+        g = batch.greens.part(i, (2, nw, o * n))
+        ref = kernels.energy(g, nw, n, o, 1, (m["rot_pad"][0], m["rot_pad"][1]), m["rot_h1"], m["h0"])
+        got = batch.eloc.part(i)
+        assert torch.equal(got, ref), f"system {i}: grouped bf16x3 energy differs"
+        assert bool(torch.isfinite(ref.real).all())
+
+
 def test_batch_bf16x3_close_to_fp32(gpu_device):
     """Same fields, same start: after two steps (one QR) the walkers and
     weights of the bf16x3 batch are within the split-bf16 error (~2^-16
