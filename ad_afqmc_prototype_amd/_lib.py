@@ -78,6 +78,7 @@ _SIGNATURES = {
     "afq_trotter_bf16x3_grouped": [c_int, c_vp, c_int, c_int, c_int, c_vp],
     "afq_trotter_bf16x3_sym_grouped": [c_int, c_vp, c_int, c_int, c_int, c_vp],
     "afq_overlap_greens_grouped": [c_int, c_vp, c_int, c_int, c_vp],
+    "afq_overlap_greens_bf16x3_grouped": [c_int, c_vp, c_int, c_int, c_vp],
     "afq_phaseless_update_grouped": [c_int, c_vp, c_vp],
     "afq_orthonormalize_grouped": [c_int, c_vp, c_int, c_int, c_int, c_vp],
     "afq_energy_f32_grouped": [c_int, c_vp, c_int, c_int, c_int, c_vp],
@@ -90,7 +91,7 @@ _SIGNATURES = {
     "afq_unpack_pairs": [c_int, c_int, c_vp, c_ll, c_vp, c_vp],
 }
 
-ABI_VERSION = 14
+ABI_VERSION = 15
 
 
 class HipLibraryError(RuntimeError):

@@ -832,6 +832,36 @@ __global__ __launch_bounds__(THREADS) void greens_gj_split_kernel(
                              split_rows{ghi + rre, glo + rre, ghi + rim, glo + rim});
 }
 
+// Every system's walkers (restricted), one wave each: wave item -> (system,
+// walker) by the walker0 prefix sums; planes and pitch from the descriptor.
+template <int NT>
+__global__ __launch_bounds__(THREADS) void greens_gj_split_grouped_kernel(int count, int total,
+                                                                         const afq_system_step* __restrict__ S) {
+    __shared__ gj_lds<NT> L[THREADS / 64];
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int item = blockIdx.x * (THREADS / 64) + wave;
+    if (item >= total) return;
+    const int si = afq::find_group(count, item, [&](int i) { return S[i].walker0; });
+    const afq_system_step& d = S[si];
+    const int w = item - d.walker0;
+    const int n = d.n, o = d.o;
+    const long long ldk = d.ldk;
+    const long long ldg = (long long)o * n;
+    uint16_t* ghi = d.greens_hi;
+    uint16_t* glo = d.greens_lo;
+    const long long rre = (long long)w * ldk;
+    const long long rim = rre + (long long)d.nw * ldk;
+    const int lane = threadIdx.x & 63;
+    if (lane < ldk - ldg) {
+        ghi[rre + ldg + lane] = 0;
+        glo[rre + ldg + lane] = 0;
+        ghi[rim + ldg + lane] = 0;
+        glo[rim + ldg + lane] = 0;
+    }
+    greens_gj_split_body<NT>(n, o, (const double2*)d.walkers_out + (long long)w * n * o, d.trial,
+                             (double2*)d.logdet + w, L[wave], split_rows{ghi + rre, glo + rre, ghi + rim, glo + rim});
+}
+
 // Complex trial orbitals (interleaved complex128 C), fp64 rows only.
 template <int MAXO>
 __global__ __launch_bounds__(THREADS) void greens_wave_ctrial_kernel(
@@ -949,6 +979,21 @@ extern "C" int afq_overlap_greens_grouped(int count, const afq_system_step* sys,
     else
         hipLaunchKernelGGL(greens_grouped_kernel<32>, grid, dim3(THREADS), 0, st, count, total_walkers, sys);
     return afq::check_launch("afq_overlap_greens_grouped");
+}
+
+extern "C" int afq_overlap_greens_bf16x3_grouped(int count, const afq_system_step* sys, int total_walkers,
+                                                 int max_o, void* stream) {
+    AFQ_REQUIRE(count > 0 && total_walkers > 0, "afq_overlap_greens_bf16x3_grouped: count=%d total_walkers=%d",
+                count, total_walkers);
+    AFQ_REQUIRE(sys, "afq_overlap_greens_bf16x3_grouped: null pointer");
+    AFQ_REQUIRE(max_o > 0 && max_o <= 32, "afq_overlap_greens_bf16x3_grouped: max_o=%d not in 1..32", max_o);
+    const dim3 grid((unsigned)afq::ceil_div(total_walkers, THREADS / 64));
+    hipStream_t st = (hipStream_t)stream;
+    if (max_o <= 16)
+        hipLaunchKernelGGL(greens_gj_split_grouped_kernel<1>, grid, dim3(THREADS), 0, st, count, total_walkers, sys);
+    else
+        hipLaunchKernelGGL(greens_gj_split_grouped_kernel<2>, grid, dim3(THREADS), 0, st, count, total_walkers, sys);
+    return afq::check_launch("afq_overlap_greens_bf16x3_grouped");
 }
 
 extern "C" int afq_overlap_greens_split(int nw, int n, int o, int nspin, const double* walkers_up,

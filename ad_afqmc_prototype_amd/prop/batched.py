@@ -69,7 +69,8 @@ class _system_step(ctypes.Structure):
                     "trial", "walkers_in", "walkers_out", "greens", "logdet", "fields", "fb", "mf",
                     "xbar", "shift_term", "fb_term", "exp_h1_half", "vhs", "log_overlaps_in",
                     "log_overlaps_out", "weights_in", "weights_out", "pop_in", "pop_out", "key_in",
-                    "key_out", "rot_chol", "rot_h1", "energy")]
+                    "key_out", "rot_chol", "rot_h1", "energy", "greens_hi", "greens_lo")] + [
+                    ("ldk", ctypes.c_longlong)]
 
 
 class _gemm_problem(ctypes.Structure):
@@ -98,7 +99,7 @@ class _split_problem(ctypes.Structure):
                 ("ld_src", ctypes.c_int), ("ld_dst", ctypes.c_int)]
 
 
-assert ctypes.sizeof(_system_step) == 8 * 4 + 8 * 8 + 24 * 8
+assert ctypes.sizeof(_system_step) == 8 * 4 + 8 * 8 + 26 * 8 + 8
 assert ctypes.sizeof(_gemm_problem) == 80
 assert ctypes.sizeof(_gemm_problem_bf16x3) == 96
 assert ctypes.sizeof(_split_problem) == 40
@@ -211,6 +212,8 @@ class system_batch:
             raise ValueError(f"gemm_shapes {self.gemm_shapes} not two of {valid} ({precision})")
         self.steps_done = 0
         self.block_stats = torch.zeros((self.count, 2), dtype=f64, device=dev)
+        if self.precision == "bf16x3":
+            self._split("fb")  # planes of the initial walkers' Green's functions
 
     # ------------------------------------------------------------------ setup
     def _member(self, s: batch_system, n: int, o: int, g: int) -> dict:
@@ -320,6 +323,9 @@ class system_batch:
             d.key_in, d.key_out = self.keys[a][i].data_ptr(), self.keys[b][i].data_ptr()
             d.rot_chol, d.rot_h1, d.energy = (m["rot_pad"].data_ptr(), m["rot_h1"].data_ptr(),
                                               self.eloc.ptr(i))
+            if self.precision == "bf16x3":
+                d.greens_hi, d.greens_lo = self.greens_hl[0].ptr(i), self.greens_hl[1].ptr(i)
+                d.ldk = kernels.pad8(m["o"] * m["n"])
         return arr, w0, t0
 
     def _problems(self, which: str, shape: int):
@@ -417,6 +423,15 @@ class system_batch:
             self.greens_buckets.append(dict(
                 tables=[_upload(t[0]).to(self.device) for t in tabs], count=len(idx), walkers=tabs[0][1],
                 max_o=max(self.members[i]["o"] for i in idx)))
+        # bf16x3 steps: the Gauss-Jordan kernel writing the force bias's planes
+        # (capacity 16 or 32 occupied orbitals)
+        self.greens_split_buckets = []
+        if self.precision == "bf16x3":
+            for idx in buckets(lambda m: 16 if m["o"] <= 16 else 32):
+                tabs = [self._desc(p, idx) for p in (0, 1)]
+                self.greens_split_buckets.append(dict(
+                    tables=[_upload(t[0]).to(self.device) for t in tabs], count=len(idx), walkers=tabs[0][1],
+                    max_o=max(self.members[i]["o"] for i in idx)))
         self.energy_buckets = []
         for idx in buckets(lambda m: (m["o"] + 15) // 16):
             arr, nw, _ = self._desc(0, idx)
@@ -450,12 +465,19 @@ class system_batch:
             raise ValueError("batch too large for one conversion launch")
 
     # ------------------------------------------------------------------ stages
+    def _split(self, which: str):
+        """The bf16 planes of the force bias's (Green's-function rows) or the
+        VHS's (shifted fields) A operand, from their fp64 rows."""
+        stab, biggest = self.split_tables[which]
+        kernels._call("afq_split_bf16_grouped", stab, self.count, stab.data_ptr(), biggest,
+                      kernels._stream(stab), tag=which)
+
     def _grouped_gemm(self, which: str):
         if self.precision == "bf16x3":
-            # the A operand's bf16 planes, then the product
-            stab, biggest = self.split_tables[which]
-            kernels._call("afq_split_bf16_grouped", stab, self.count, stab.data_ptr(), biggest,
-                          kernels._stream(stab), tag=which)
+            # the A operand's bf16 planes (the force bias's come from the
+            # Green's stage), then the product
+            if which == "vhs":
+                self._split(which)
             shape = self.gemm_shapes[0 if which == "fb" else 1]
             tab, tiles = self.gemm_tables[(which, shape)]
             kernels._call("afq_gemm_bf16x3_nt_grouped", tab, self.count, tab.data_ptr(), tiles, shape,
@@ -471,12 +493,22 @@ class system_batch:
         kernels._call("afq_convert_f32", src.buf, 1, n, src.buf.data_ptr(), n, dst.buf.data_ptr(), n,
                       kernels._stream(src.buf))
 
-    def _greens(self, p: int, st: int):
+    def _greens(self, p: int, st: int, rows: bool = True):
         """Green's functions and log-overlaps of every system's new walkers
-        (buffer set p), one launch per occupied-orbital bucket."""
+        (buffer set p), one launch per occupied-orbital bucket: fp64 rows
+        (and, in bf16x3 precision, their planes split from them), or with
+        rows=False (bf16x3 steps) only the force bias's planes."""
+        if not rows:
+            for bk in self.greens_split_buckets:
+                t = bk["tables"][p]
+                kernels._call("afq_overlap_greens_bf16x3_grouped", t, bk["count"], t.data_ptr(), bk["walkers"],
+                              bk["max_o"], st)
+            return
         for bk in self.greens_buckets:
             t = bk["tables"][p]
             kernels._call("afq_overlap_greens_grouped", t, bk["count"], t.data_ptr(), bk["walkers"], bk["max_o"], st)
+        if self.precision == "bf16x3":
+            self._split("fb")
 
     def step(self):
         """One step of every system (fields drawn from each system's key)."""
@@ -495,7 +527,7 @@ class system_batch:
             t = bk["tables"][p]
             kernels._call("afq_trotter_bf16x3_sym_grouped" if self.precision == "bf16x3" else "afq_trotter_f32_grouped",
                           t, bk["count"], t.data_ptr(), bk["blocks"], bk["max_n"], self.n_exp_terms, st)
-        self._greens(p, st)
+        self._greens(p, st, rows=self.precision != "bf16x3")
         kernels._call("afq_phaseless_update_grouped", d, self.count, d.data_ptr(), st)
         self.steps_done += 1
 
@@ -543,6 +575,8 @@ class system_batch:
                                 out=self.walkers[0].part(i, (nw, n, o)))
             g.copy_(kernels.gather_rows(g, idx, nplanes=2))
             torch.mul(kernels.gather_rows(self.logdet.part(i), idx), 2.0, out=self.lo[0].part(i))
+        if self.precision == "bf16x3":
+            self._split("fb")  # the planes of the surviving walkers' rows
 
     def advance(self, n_steps: int):
         """n_steps steps of every system with the QR / measurement cadence

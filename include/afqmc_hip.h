@@ -446,6 +446,9 @@ typedef struct afq_system_step {
                                     bf16x3 energy: its hi bf16 plane, then the lo plane, same shape */
     const double* rot_h1;        /* (o, n) float64 */
     double* energy;              /* complex128 (nw) local energies of walkers_out */
+    unsigned short* greens_hi;   /* bf16x3 step: (2, nw, ldk) bf16 hi plane of the Green's-function rows */
+    unsigned short* greens_lo;   /* (2, nw, ldk) lo plane */
+    long long ldk;               /* row pitch of the planes: o*n <= ldk < o*n + 64 */
 } afq_system_step;
 
 /* One problem of a grouped GEMM: C (M x N) = alpha A (M x K) B^T (N x K). */
@@ -578,6 +581,16 @@ int afq_trotter_bf16x3_sym_grouped(int count, const afq_system_step* sys, int to
  */
 int afq_overlap_greens_grouped(int count, const afq_system_step* sys, int total_walkers, int max_o,
                                void* stream);
+
+/*
+ * afq_overlap_greens_bf16x3 (restricted walkers, o <= 32) for every system:
+ * log det(C^T walkers_out) into logdet and the Green's-function rows as the
+ * force bias's split-bf16 planes into greens_hi / greens_lo (padding columns
+ * zeroed); no fp64 rows (meas/rhf.py:15-24, trial/rhf.py:25-44 per system).
+ * max_o the largest o of the table (the kernel's capacity: 16 or 32).
+ */
+int afq_overlap_greens_bf16x3_grouped(int count, const afq_system_step* sys, int total_walkers, int max_o,
+                                      void* stream);
 
 /* afq_phaseless_update for every system, one workgroup each (prop/afqmc.py:116-138). */
 int afq_phaseless_update_grouped(int count, const afq_system_step* sys, void* stream);

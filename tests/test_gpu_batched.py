@@ -272,6 +272,27 @@ def test_grouped_bf16x3_energy_matches_single(gpu_device):
         assert bool(torch.isfinite(ref.real).all())
 
 
+def test_grouped_bf16x3_greens_matches_single(gpu_device):
+    """The grouped Gauss-Jordan Green's kernel of the bf16x3 step (force-bias
+    planes and log det, no fp64 rows) equals afq_overlap_greens_bf16x3 on each
+    system alone, bit for bit, after one step."""
+    import torch
+    from ad_afqmc_prototype_amd import kernels
+    from ad_afqmc_prototype_amd.prop.batched import system_batch
+    batch = system_batch(_members(gpu_device), dt=DT, qr_every=QR, sr_every=SR, precision="bf16x3")
+    batch.step()  # walkers -> set B, planes and log det of the new walkers
+    torch.cuda.synchronize()
+    assert {bk["max_o"] <= 16 for bk in batch.greens_split_buckets} == {True, False}
+    for i, m in enumerate(batch.members):
+        nw, n, o = m["nw"], m["n"], m["o"]
+        ldk = kernels.pad8(o * n)
+        ref_ld, (ref_hi, ref_lo) = kernels.overlap_greens_bf16x3(batch.walkers[1].part(i, (nw, n, o)),
+                                                                  batch.trials.part(i, (n, o)))
+        assert torch.equal(batch.logdet.part(i), ref_ld[:, 0]), f"system {i}: log det differs"
+        assert torch.equal(batch.greens_hl[0].part(i, (2 * nw, ldk)), ref_hi), f"system {i}: hi plane"
+        assert torch.equal(batch.greens_hl[1].part(i, (2 * nw, ldk)), ref_lo), f"system {i}: lo plane"
+
+
 def test_batch_bf16x3_close_to_fp32(gpu_device):
     """Same fields, same start: after two steps (one QR) the walkers and
     weights of the bf16x3 batch are within the split-bf16 error (~2^-16
