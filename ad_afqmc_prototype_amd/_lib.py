@@ -91,7 +91,7 @@ _SIGNATURES = {
     "afq_unpack_pairs": [c_int, c_int, c_vp, c_ll, c_vp, c_vp],
 }
 
-ABI_VERSION = 15
+ABI_VERSION = 16
 
 
 class HipLibraryError(RuntimeError):

@@ -103,8 +103,12 @@ __global__ __launch_bounds__(256) void field_shift_grouped_kernel(int count,
     const int b = blockIdx.x;
     const int si = afq::find_group(count, b, [&](int i) { return S[i].walker0; });
     const afq_system_step& d = S[si];
-    field_shift_body(d.nw, d.naux, b - d.walker0, d.sqrt_dt, d.fields, d.fb, (const double2*)d.mf,
-                     d.xbar, (double2*)d.shift_term, (double2*)d.fb_term);
+    if (d.xbar_hi)  // the VHS operand's split planes too (uniform per workgroup)
+        field_shift_body<true>(d.nw, d.naux, b - d.walker0, d.sqrt_dt, d.fields, d.fb, (const double2*)d.mf,
+                               d.xbar, (double2*)d.shift_term, (double2*)d.fb_term, d.xbar_hi, d.xbar_lo, d.ldx);
+    else
+        field_shift_body(d.nw, d.naux, b - d.walker0, d.sqrt_dt, d.fields, d.fb, (const double2*)d.mf,
+                         d.xbar, (double2*)d.shift_term, (double2*)d.fb_term);
 }
 
 __device__ __forceinline__ void phaseless_body(

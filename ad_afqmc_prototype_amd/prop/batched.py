@@ -70,7 +70,7 @@ class _system_step(ctypes.Structure):
                     "xbar", "shift_term", "fb_term", "exp_h1_half", "vhs", "log_overlaps_in",
                     "log_overlaps_out", "weights_in", "weights_out", "pop_in", "pop_out", "key_in",
                     "key_out", "rot_chol", "rot_h1", "energy", "greens_hi", "greens_lo")] + [
-                    ("ldk", ctypes.c_longlong)]
+                    ("ldk", ctypes.c_longlong), ("xbar_hi", _vp), ("xbar_lo", _vp), ("ldx", ctypes.c_longlong)]
 
 
 class _gemm_problem(ctypes.Structure):
@@ -99,7 +99,7 @@ class _split_problem(ctypes.Structure):
                 ("ld_src", ctypes.c_int), ("ld_dst", ctypes.c_int)]
 
 
-assert ctypes.sizeof(_system_step) == 8 * 4 + 8 * 8 + 26 * 8 + 8
+assert ctypes.sizeof(_system_step) == 8 * 4 + 8 * 8 + 26 * 8 + 8 + 2 * 8 + 8
 assert ctypes.sizeof(_gemm_problem) == 80
 assert ctypes.sizeof(_gemm_problem_bf16x3) == 96
 assert ctypes.sizeof(_split_problem) == 40
@@ -326,6 +326,8 @@ class system_batch:
             if self.precision == "bf16x3":
                 d.greens_hi, d.greens_lo = self.greens_hl[0].ptr(i), self.greens_hl[1].ptr(i)
                 d.ldk = kernels.pad8(m["o"] * m["n"])
+                d.xbar_hi, d.xbar_lo = self.xbar_hl[0].ptr(i), self.xbar_hl[1].ptr(i)
+                d.ldx = kernels.pad8(m["g"])
         return arr, w0, t0
 
     def _problems(self, which: str, shape: int):
@@ -456,7 +458,8 @@ class system_batch:
                 for shape in BF16X3_SHAPES:
                     arr, tiles = self._problems_bf16x3(which, shape)
                     self.gemm_tables[(which, shape)] = (_upload(arr).to(self.device), tiles)
-                self.split_tables[which] = self._split_table(which)
+                if which == "fb":  # (the field shifts write the VHS operand's planes)
+                    self.split_tables[which] = self._split_table(which)
                 continue
             for shape in GEMM_SHAPES:
                 arr, tiles = self._problems(which, shape)
@@ -474,10 +477,8 @@ class system_batch:
 
     def _grouped_gemm(self, which: str):
         if self.precision == "bf16x3":
-            # the A operand's bf16 planes (the force bias's come from the
-            # Green's stage), then the product
-            if which == "vhs":
-                self._split(which)
+            # the A operand's bf16 planes come from the Green's stage (force
+            # bias) and the field shifts (VHS)
             shape = self.gemm_shapes[0 if which == "fb" else 1]
             tab, tiles = self.gemm_tables[(which, shape)]
             kernels._call("afq_gemm_bf16x3_nt_grouped", tab, self.count, tab.data_ptr(), tiles, shape,

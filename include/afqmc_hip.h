@@ -449,6 +449,9 @@ typedef struct afq_system_step {
     unsigned short* greens_hi;   /* bf16x3 step: (2, nw, ldk) bf16 hi plane of the Green's-function rows */
     unsigned short* greens_lo;   /* (2, nw, ldk) lo plane */
     long long ldk;               /* row pitch of the planes: o*n <= ldk < o*n + 64 */
+    unsigned short* xbar_hi;     /* null, or (2, nw, ldx) bf16 hi plane of xbar written by the field shifts */
+    unsigned short* xbar_lo;     /* (2, nw, ldx) lo plane */
+    long long ldx;               /* row pitch of the xbar planes: >= naux */
 } afq_system_step;
 
 /* One problem of a grouped GEMM: C (M x N) = alpha A (M x K) B^T (N x K). */
@@ -542,7 +545,8 @@ int afq_split_bf16_grouped(int count, const afq_split_problem* problems, long lo
  */
 int afq_rng_fields_grouped(int count, const afq_system_step* sys, int total_walkers, void* stream);
 
-/* afq_field_shifts for every system (prop/afqmc.py:97-101). */
+/* afq_field_shifts for every system (prop/afqmc.py:97-101); a system whose
+ * xbar_hi is set also gets xbar's split-bf16 planes (afq_field_shifts_split). */
 int afq_field_shifts_grouped(int count, const afq_system_step* sys, int total_walkers, void* stream);
 
 /*
