@@ -49,6 +49,7 @@ _SIGNATURES = {
     "afq_vhs_sym": [c_int, c_int, c_int, c_int, c_int, c_vp, c_vp, c_ll, c_vp, c_vp, c_ll, c_vp, c_vp],
     "afq_trotter": [c_int, c_int, c_int, c_int, c_int, c_int, c_double, c_vp, c_vp, c_vp, c_vp,
                     c_vp, c_vp, c_vp],
+    "afq_trotter_adjoint": [c_int, c_int, c_int, c_int, c_int, c_double] + [c_vp] * 10,
     "afq_phaseless_update": [c_int, c_int, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_double, c_double,
                              c_double, c_double, c_double, c_double, c_double, c_double, c_vp,
                              c_vp, c_vp, c_vp],
@@ -91,7 +92,7 @@ _SIGNATURES = {
     "afq_unpack_pairs": [c_int, c_int, c_vp, c_ll, c_vp, c_vp],
 }
 
-ABI_VERSION = 16
+ABI_VERSION = 17
 
 
 class HipLibraryError(RuntimeError):

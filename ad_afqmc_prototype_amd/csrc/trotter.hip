@@ -802,6 +802,360 @@ int trotter_entry(const char* name, int nw, int n, int o, int nspin, int ldv, in
     return -1;
 }
 
+// ---------------------------------------------------------------------------
+// Adjoint of the fp64 Trotter step (restricted walkers, n <= 128, o <= 32),
+//   w' = E u,  u = sum_{k<=K} t_k,  t_0 = E w,  t_k = (a / k) V t_{k-1},
+//   a = i sqrt(dt), V and E symmetric:
+// given w'_bar, the reverse recursion
+//   u_bar = E^T w'_bar,  t_bar_K = u_bar,
+//   t_bar_{k-1} = u_bar + conj(a / k) V^H t_bar_k = u_bar + conj(V) tb_k,
+//   tb_k = conj(a / k) t_bar_k,  w_bar = E^T t_bar_0,
+//   V_bar = sum_k tb_k t_{k-1}^H
+// (prop/chol_afqmc_ops.py:173-196, prop/utils.py:20-31 differentiated).  One
+// workgroup per walker, one wave per 16-row tile, V resident on chip as in the
+// forward resident kernel.  The forward terms t_0..t_{K-1}, the scaled
+// adjoints tb_1..tb_K, u and t_bar_0 go to the caller's workspace ([2K + 2]
+// [nw][n][16 NCT] complex, padding columns zero); V_bar = sum_k tb_k
+// t_{k-1}^H and the walker's share of E's adjoint, Re(w'_bar u^H + t_bar_0
+// w^H), are products of those, each wave one 16-row slab.
+// ---------------------------------------------------------------------------
+template <int NCT>
+__global__ __launch_bounds__(512, 1) void trotter_adjoint_f64_kernel(
+    int nw, int n, int o, int ldv, int n_terms, double sqrt_dt, const double* __restrict__ E,
+    const double* __restrict__ Et, const double* __restrict__ vhs, long long vplane,
+    const double2* __restrict__ w_in, const double2* __restrict__ wbar_out, double2* __restrict__ wbar_in,
+    double* __restrict__ vbar, double* __restrict__ ebar, double2* __restrict__ ws) {
+    constexpr int KRB = f64_res_krb(NCT);
+    constexpr int TC = 16 * NCT;
+    using MF = afq::mfma16<double>;
+    using ACC = MF::acc;
+    extern __shared__ __attribute__((aligned(16))) unsigned char lds_raw[];
+    const int NP = afq::round_up(n, 16);
+    const int nkb = NP / 16;
+    const int nkl = nkb > KRB ? nkb - KRB : 0;
+    double* Tr = reinterpret_cast<double*>(lds_raw);
+    double* Ti = Tr + NP * TC;
+    double* VL = Ti + NP * TC;  // [wave][nkl][4 kk][2 planes][64 lanes]
+
+    const int w = blockIdx.x;
+    const long long woff = (long long)w * n * o;
+    const double* Vr = vhs + (long long)w * n * ldv;
+    const double* Vi = Vr + vplane;
+    // term k of this walker in a workspace array: (n, TC) complex
+    const long long tstride = (long long)nw * n * TC;
+    auto slot = [&](double2* base, int k) { return base + k * tstride + (long long)w * n * TC; };
+    double2* terms = ws;                                  // t_0 .. t_{K-1}
+    double2* tbars = ws + (long long)n_terms * tstride;   // tb_1 .. tb_K
+    double2* u_ws = slot(ws, 2 * n_terms);                // u
+    double2* y0_ws = slot(ws, 2 * n_terms + 1);           // t_bar_0
+
+    const int tid = threadIdx.x;
+    const int nthreads = blockDim.x;
+    const int lane = tid & 63;
+    const int rt = tid >> 6;
+    const int lr = lane & 15;
+    const int lk = (lane >> 4) * 4;
+    const int row = rt * 16 + lr;
+
+    auto load4 = [&](const double* A, int kb, double (&x)[4]) {
+        const int k0 = kb * 16 + lk;
+        const bool ok = row < n && k0 < n;
+        const double4 v = *reinterpret_cast<const double4*>(A + (long long)(ok ? row : 0) * ldv + (ok ? k0 : 0));
+        x[0] = ok ? v.x : 0.0; x[1] = ok ? v.y : 0.0; x[2] = ok ? v.z : 0.0; x[3] = ok ? v.w : 0.0;
+    };
+    // T <- walker-shaped (n, o) complex array, zero padded to NP x TC
+    auto load_T = [&](const double2* src) {
+        for (int e = tid; e < NP * TC; e += nthreads) {
+            const int p = e / TC;
+            const int c = e - p * TC;
+            const double2 x = (p < n && c < o) ? src[woff + (long long)p * o + c] : make_double2(0.0, 0.0);
+            Tr[e] = x.x;
+            Ti[e] = x.y;
+        }
+    };
+    ACC P[NCT][2], out[NCT][2], cur[NCT][2];
+    auto e_product = [&](const double* M) {
+        double a[F64_RES_MAX_NP / 16][4];
+#pragma unroll
+        for (int kb = 0; kb < F64_RES_MAX_NP / 16; ++kb)
+            if (kb < nkb) load4(M, kb, a[kb]);
+#pragma unroll
+        for (int c = 0; c < NCT; ++c) P[c][0] = P[c][1] = MF::zero();
+#pragma unroll
+        for (int kb = 0; kb < F64_RES_MAX_NP / 16; ++kb) {
+            if (kb < nkb) {
+#pragma unroll
+                for (int kk = 0; kk < 4; ++kk) {
+                    const int k = kb * 16 + lk + kk;
+#pragma unroll
+                    for (int c = 0; c < NCT; ++c) {
+                        P[c][0] = MF::fma(a[kb][kk], Tr[k * TC + c * 16 + lr], P[c][0]);
+                        P[c][1] = MF::fma(a[kb][kk], Ti[k * TC + c * 16 + lr], P[c][1]);
+                    }
+                }
+            }
+        }
+    };
+    auto store_T = [&](const ACC (&X)[NCT][2]) {
+#pragma unroll
+        for (int c = 0; c < NCT; ++c)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int rr = rt * 16 + MF::row(lane, r);
+                Tr[rr * TC + c * 16 + lr] = X[c][0][r];
+                Ti[rr * TC + c * 16 + lr] = X[c][1][r];
+            }
+    };
+    // accumulator rows < n of X (times s = sr + i si) to a (n, TC) workspace term
+    auto store_term = [&](double2* dst, const ACC (&X)[NCT][2], double sr, double si) {
+#pragma unroll
+        for (int c = 0; c < NCT; ++c)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int rr = rt * 16 + MF::row(lane, r);
+                const double xr = X[c][0][r], xi = X[c][1][r];
+                if (rr < n) dst[(long long)rr * TC + c * 16 + lr] = make_double2(sr * xr - si * xi, sr * xi + si * xr);
+            }
+    };
+    // accumulator rows < n, columns < o of X to a walker-shaped output
+    auto store_walker = [&](double2* dst, const ACC (&X)[NCT][2]) {
+#pragma unroll
+        for (int c = 0; c < NCT; ++c) {
+            const int col = c * 16 + lr;
+            if (col < o) {
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const int rr = rt * 16 + MF::row(lane, r);
+                    if (rr < n) dst[woff + (long long)rr * o + col] = make_double2(X[c][0][r], X[c][1][r]);
+                }
+            }
+        }
+    };
+    // P += V T (conj: += conj(V) T) for one k with A = (ar, ai)
+    auto vt_step = [&](int k, double ar, double ai) {
+#pragma unroll
+        for (int c = 0; c < NCT; ++c) {
+            const double br = Tr[k * TC + c * 16 + lr];
+            const double bi = Ti[k * TC + c * 16 + lr];
+            P[c][0] = MF::fma(ar, br, P[c][0]);
+            P[c][1] = MF::fma(ar, bi, P[c][1]);
+            P[c][0] = MF::fma(ai, -bi, P[c][0]);
+            P[c][1] = MF::fma(ai, br, P[c][1]);
+        }
+    };
+
+    // ---- forward: t_0 = E w, then V resident ----
+    load_T(w_in);
+    __syncthreads();
+    e_product(E);
+    store_term(slot(terms, 0), P, 1.0, 0.0);
+    double vr[KRB][4], vi[KRB][4];
+#pragma unroll
+    for (int kb = 0; kb < KRB; ++kb) {
+        load4(Vr, kb < nkb ? kb : 0, vr[kb]);
+        load4(Vi, kb < nkb ? kb : 0, vi[kb]);
+    }
+    for (int kb = KRB; kb < nkb; ++kb) {
+        double xr[4], xi[4];
+        load4(Vr, kb, xr);
+        load4(Vi, kb, xi);
+        double* dst = VL + ((size_t)(rt * nkl + (kb - KRB)) * 8) * 64 + lane;
+#pragma unroll
+        for (int kk = 0; kk < 4; ++kk) {
+            dst[(kk * 2 + 0) * 64] = xr[kk];
+            dst[(kk * 2 + 1) * 64] = xi[kk];
+        }
+    }
+    __syncthreads();
+    store_T(P);
+#pragma unroll
+    for (int c = 0; c < NCT; ++c) {
+        out[c][0] = P[c][0];
+        out[c][1] = P[c][1];
+    }
+    __syncthreads();
+
+    // V (sgn = 1) or conj(V) (sgn = -1) times T into P
+    auto v_product = [&](double sgn) {
+#pragma unroll
+        for (int c = 0; c < NCT; ++c) P[c][0] = P[c][1] = MF::zero();
+#pragma unroll
+        for (int kb = 0; kb < KRB; ++kb) {
+            if (kb < nkb) {
+#pragma unroll
+                for (int kk = 0; kk < 4; ++kk) vt_step(kb * 16 + lk + kk, vr[kb][kk], sgn * vi[kb][kk]);
+            }
+        }
+        for (int kb = KRB; kb < nkb; ++kb) {
+            const double* src = VL + ((size_t)(rt * nkl + (kb - KRB)) * 8) * 64 + lane;
+#pragma unroll
+            for (int kk = 0; kk < 4; ++kk)
+                vt_step(kb * 16 + lk + kk, src[(kk * 2 + 0) * 64], sgn * src[(kk * 2 + 1) * 64]);
+        }
+    };
+
+#pragma unroll 1
+    for (int k = 1; k <= n_terms; ++k) {
+        v_product(1.0);
+        const double ck = sqrt_dt / (double)k;
+#pragma unroll
+        for (int c = 0; c < NCT; ++c) {  // t_k = i ck V t_{k-1}
+            const ACC re = -ck * P[c][1];
+            const ACC im = ck * P[c][0];
+            P[c][0] = re;
+            P[c][1] = im;
+            out[c][0] += re;
+            out[c][1] += im;
+        }
+        if (k < n_terms) store_term(slot(terms, k), P, 1.0, 0.0);
+        __syncthreads();
+        store_T(P);
+        __syncthreads();
+    }
+    store_term(u_ws, out, 1.0, 0.0);
+
+    // ---- reverse: u_bar = E^T w'_bar ----
+    __syncthreads();
+    load_T(wbar_out);
+    __syncthreads();
+    e_product(Et);
+#pragma unroll
+    for (int c = 0; c < NCT; ++c) {
+        out[c][0] = P[c][0];  // u_bar
+        out[c][1] = P[c][1];
+        cur[c][0] = P[c][0];  // t_bar_K
+        cur[c][1] = P[c][1];
+    }
+    __syncthreads();
+    store_T(cur);
+    __syncthreads();
+#pragma unroll 1
+    for (int k = n_terms; k >= 1; --k) {
+        const double ck = sqrt_dt / (double)k;
+        // tb_k = conj(i ck) t_bar_k = -i ck t_bar_k
+        store_term(slot(tbars, k - 1), cur, 0.0, -ck);
+        v_product(-1.0);
+#pragma unroll
+        for (int c = 0; c < NCT; ++c) {  // t_bar_{k-1} = u_bar - i ck conj(V) t_bar_k
+            cur[c][0] = out[c][0] + ck * P[c][1];
+            cur[c][1] = out[c][1] - ck * P[c][0];
+        }
+        __syncthreads();
+        store_T(cur);
+        __syncthreads();
+    }
+    store_term(y0_ws, cur, 1.0, 0.0);
+    e_product(Et);
+    store_walker(wbar_in, P);
+
+    // ---- V_bar = sum_k tb_k t_{k-1}^H: this wave's 16 rows, all columns ----
+    __syncthreads();  // every wave's workspace terms are written
+    {
+        constexpr int NTM = F64_RES_MAX_NP / 16;
+        ACC A[NTM][2];
+#pragma unroll
+        for (int ct = 0; ct < NTM; ++ct) A[ct][0] = A[ct][1] = MF::zero();
+        const double2* tb0 = slot(tbars, 0);
+        const double2* t0 = slot(terms, 0);
+        const int nj = n_terms * TC / 16;  // 16-deep blocks of the (term, column) index
+        const int rowa = row < n ? row : 0;
+        for (int jb = 0; jb < nj; ++jb) {
+            const int term = (jb * 16) / TC;
+            const int c0 = jb * 16 - term * TC + lk;
+            double2 a[4];
+#pragma unroll
+            for (int kk = 0; kk < 4; ++kk) {
+                const double2 v = tb0[term * tstride + (long long)rowa * TC + c0 + kk];
+                a[kk] = row < n ? v : make_double2(0.0, 0.0);
+            }
+#pragma unroll
+            for (int ct = 0; ct < NTM; ++ct) {
+                if (ct < nkb) {
+                    const int q = ct * 16 + lr;
+                    const int qa = q < n ? q : 0;
+#pragma unroll
+                    for (int kk = 0; kk < 4; ++kk) {
+                        double2 b = t0[term * tstride + (long long)qa * TC + c0 + kk];
+                        if (q >= n) b = make_double2(0.0, 0.0);
+                        // A . conj(B^T): B' = (br, -bi)
+                        A[ct][0] = MF::fma(a[kk].x, b.x, A[ct][0]);
+                        A[ct][1] = MF::fma(a[kk].x, -b.y, A[ct][1]);
+                        A[ct][0] = MF::fma(-a[kk].y, -b.y, A[ct][0]);
+                        A[ct][1] = MF::fma(a[kk].y, b.x, A[ct][1]);
+                    }
+                }
+            }
+        }
+        double* vbr = vbar + (long long)w * n * n;
+        double* vbi = vbr + (long long)nw * n * n;
+#pragma unroll
+        for (int ct = 0; ct < NTM; ++ct) {
+            const int q = ct * 16 + lr;
+            if (ct < nkb && q < n) {
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const int p = rt * 16 + MF::row(lane, r);
+                    if (p < n) {
+                        vbr[(long long)p * n + q] = A[ct][0][r];
+                        vbi[(long long)p * n + q] = A[ct][1][r];
+                    }
+                }
+            }
+        }
+    }
+
+    // ---- E's adjoint, this walker's share: Re(w'_bar u^H + t_bar_0 w^H) ----
+    {
+        constexpr int NTM = F64_RES_MAX_NP / 16;
+        ACC A[NTM];
+#pragma unroll
+        for (int ct = 0; ct < NTM; ++ct) A[ct] = MF::zero();
+        const int rowa = row < n ? row : 0;
+        // walker-shaped (n, o) or workspace (n, TC) element (p, c), zero outside
+        auto wk = [&](const double2* src, int p, int c) {
+            const bool ok = p < n && c < o;
+            const double2 v = src[woff + (long long)(ok ? p : 0) * o + (ok ? c : 0)];
+            return ok ? v : make_double2(0.0, 0.0);
+        };
+        auto wsv = [&](const double2* src, int p, int c) {
+            const double2 v = src[(long long)(p < n ? p : 0) * TC + c];
+            return p < n ? v : make_double2(0.0, 0.0);
+        };
+        for (int cb = 0; cb < NCT; ++cb) {
+#pragma unroll
+            for (int kk = 0; kk < 4; ++kk) {
+                const int c = cb * 16 + lk + kk;
+                const double2 a1 = wk(wbar_out, rowa, c);
+                const double2 a2 = wsv(y0_ws, row, c);
+#pragma unroll
+                for (int ct = 0; ct < NTM; ++ct) {
+                    if (ct < nkb) {
+                        const int q = ct * 16 + lr;
+                        const double2 b1 = wsv(u_ws, q, c);
+                        const double2 b2 = wk(w_in, q, c);
+                        A[ct] = MF::fma(row < n ? a1.x : 0.0, b1.x, A[ct]);
+                        A[ct] = MF::fma(row < n ? a1.y : 0.0, b1.y, A[ct]);
+                        A[ct] = MF::fma(a2.x, b2.x, A[ct]);
+                        A[ct] = MF::fma(a2.y, b2.y, A[ct]);
+                    }
+                }
+            }
+        }
+        double* eb = ebar + (long long)w * n * n;
+#pragma unroll
+        for (int ct = 0; ct < NTM; ++ct) {
+            const int q = ct * 16 + lr;
+            if (ct < nkb && q < n) {
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const int p = rt * 16 + MF::row(lane, r);
+                    if (p < n) eb[(long long)p * n + q] = A[ct][r];
+                }
+            }
+        }
+    }
+}
+
 }  // namespace
 
 extern "C" int afq_trotter(int nw, int n, int o, int nspin, int ldv, int n_terms, double sqrt_dt,
@@ -835,4 +1189,38 @@ extern "C" int afq_trotter_f32_grouped(int count, const afq_system_step* sys, in
                        dim3(64 * (NP / 16)), lds, (hipStream_t)stream, count, total_blocks, NP, n_terms,
                        sys);
     return afq::check_launch("afq_trotter_f32_grouped");
+}
+
+extern "C" int afq_trotter_adjoint(int nw, int n, int o, int ldv, int n_terms, double sqrt_dt,
+                                   const double* exp_h1_half, const double* exp_h1_half_t, const double* vhs,
+                                   const double* w_in, const double* w_bar_out, double* w_bar_in, double* vhs_bar,
+                                   double* e_half_bar, double* workspace, void* stream) {
+    AFQ_REQUIRE(nw > 0 && n > 0 && o > 0 && o <= n, "afq_trotter_adjoint: bad sizes nw=%d n=%d o=%d", nw, n, o);
+    AFQ_REQUIRE(n <= F64_RES_MAX_NP && o <= 32, "afq_trotter_adjoint: n=%d o=%d beyond n <= %d, o <= 32", n, o,
+                F64_RES_MAX_NP);
+    AFQ_REQUIRE(ldv >= n && ldv % 4 == 0, "afq_trotter_adjoint: ldv=%d must be >= n and a multiple of 4", ldv);
+    AFQ_REQUIRE(n_terms >= 1, "afq_trotter_adjoint: n_terms=%d < 1", n_terms);
+    AFQ_REQUIRE(exp_h1_half && exp_h1_half_t && vhs && w_in && w_bar_out && w_bar_in && vhs_bar && e_half_bar &&
+                    workspace,
+                "afq_trotter_adjoint: null pointer");
+    const int nct = afq::ceil_div(o, 16);
+    const int NP = afq::round_up(n, 16);
+    const long long vplane = (long long)nw * n * ldv;
+    hipStream_t st = (hipStream_t)stream;
+    const dim3 grid((unsigned)nw), block(64 * (NP / 16));
+#define AFQ_TROTTER_ADJ(NCT_)                                                                                    \
+    if (nct == NCT_) {                                                                                           \
+        const size_t lds = f64_resident_lds(n, NCT_);                                                            \
+        AFQ_REQUIRE(lds <= 160 * 1024, "afq_trotter_adjoint: n=%d needs %zu bytes of LDS", n, lds);              \
+        if (lds > 64 * 1024) afq::allow_full_lds((const void*)trotter_adjoint_f64_kernel<NCT_>);                \
+        hipLaunchKernelGGL((trotter_adjoint_f64_kernel<NCT_>), grid, block, lds, st, nw, n, o, ldv, n_terms, sqrt_dt, \
+                           exp_h1_half, exp_h1_half_t, vhs, vplane, (const double2*)w_in, (const double2*)w_bar_out, \
+                           (double2*)w_bar_in, vhs_bar, e_half_bar, (double2*)workspace);                        \
+        return afq::check_launch("afq_trotter_adjoint");                                                        \
+    }
+    AFQ_TROTTER_ADJ(1)
+    AFQ_TROTTER_ADJ(2)
+#undef AFQ_TROTTER_ADJ
+    afq::set_error("afq_trotter_adjoint: no kernel for o=%d", o);
+    return -1;
 }

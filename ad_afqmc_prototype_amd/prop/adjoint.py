@@ -124,8 +124,16 @@ class _layouts:
         self.n, self.o, self.g, self.nw = n, o, g, run.n_walkers
         ldv = kernels.ldv_for(n)
         self.e_half = e_half.contiguous()
+        self.ldv = ldv
         self.e_pad = torch.zeros((n, ldv), dtype=f64, device=chol.device)
         self.e_pad[:, :n] = e_half
+        self.e_pad_t = torch.zeros((n, ldv), dtype=f64, device=chol.device)
+        self.e_pad_t[:, :n] = e_half.T
+        # the Trotter adjoint's workspace: forward terms and scaled adjoints
+        tc = (o + 15) // 16 * 16
+        self.trotter_ws = torch.empty((2 * (2 * run.n_exp_terms + 2) * run.n_walkers * n * tc,), dtype=f64,
+                                      device=chol.device)
+        self.e_half_bar = torch.empty((run.n_walkers, n, n), dtype=f64, device=chol.device)
         self.chol = chol.contiguous()
         self.chol_t = chol_transposed(chol)
         self.mf = (1.0j * m).to(c128).contiguous()
@@ -143,28 +151,24 @@ class _layouts:
 
 
 def _dgemm(A, B, C, alpha: float = 1.0, beta: float = 0.0):
-    """C = alpha A B^T + beta C on the library's fp64 MFMA GEMM, with K split
-    into independent slices (partial products summed afterwards) when the
-    128 x 128 tiles of C alone would leave the GPU mostly idle."""
+    """C = alpha A B^T + beta C on the library's fp64 MFMA GEMM; when the 128 x
+    128 tiles of C alone would leave the GPU mostly idle, K is cut into chunks
+    that run side by side in one launch (afq_dgemm_nt_splitk) and are summed in
+    order."""
     M, K = A.shape
     N = B.shape[0]
     tiles = -(-M // 128) * -(-N // 128)
     splits = 1
-    while tiles * splits < 512 and K // (2 * splits) >= 64:
+    while tiles * splits < 512 and K // (2 * splits) >= 64 and splits < 16:
         splits *= 2
     if splits == 1:
         return kernels.dgemm_nt(A, B, C, alpha=alpha, beta=beta)
-    part = torch.empty((splits, M, N), dtype=f64, device=A.device)
-    step = -(-K // splits)
-    st = kernels._stream(A)
-    for s in range(splits):
-        k0, k1 = s * step, min(K, (s + 1) * step)
-        kernels._call("afq_dgemm_nt", A, M, N, k1 - k0, 1.0, A.data_ptr() + 8 * k0, K,
-                      B.data_ptr() + 8 * k0, K, 0.0, part[s].data_ptr(), N, st)
-    if beta == 0.0:
-        torch.mul(part.sum(0), alpha, out=C)
-    else:
-        C.mul_(beta).add_(part.sum(0), alpha=alpha)
+    ws = torch.empty((splits * M * N,), dtype=f64, device=A.device)
+    out = C if beta == 0.0 else torch.empty_like(C)
+    kernels._call("afq_dgemm_nt_splitk", A, M, N, K, float(alpha), A.data_ptr(), K, B.data_ptr(), K,
+                  out.data_ptr(), N, splits, ws.data_ptr(), kernels._stream(A))
+    if beta != 0.0:
+        C.mul_(beta).add_(out)
     return C
 
 
@@ -297,27 +301,21 @@ def _energy_adj(g_c, e_bar, lay: _layouts, acc, chunk: int = 64):
     return out
 
 
-def _trotter_adj(lay: _layouts, run: adjoint_run, w_in, vhs_c, w_bar_out, acc):
-    """(w_bar_in, VHS_bar) of w' = E T(E w); accumulates E_bar."""
-    e = lay.e_half.to(c128)
-    et = e.T
-    a = 1.0j * run.dt ** 0.5
-    y0 = e @ w_in
-    terms = [y0]
-    for k in range(1, run.n_exp_terms + 1):
-        terms.append((a / k) * (vhs_c @ terms[-1]))
-    u = torch.stack(terms).sum(0)
-    u_bar = et @ w_bar_out
-    acc["e_half"] += torch.einsum("wpa,wqa->pq", w_bar_out, u.conj()).real
-    vhs_h = _ct(vhs_c)
-    t_bar = u_bar
-    vhs_bar = torch.zeros_like(vhs_c)
-    for k in range(run.n_exp_terms, 0, -1):
-        ck = (a / k).conjugate()
-        vhs_bar += ck * (t_bar @ _ct(terms[k - 1]))
-        t_bar = u_bar + ck * (vhs_h @ t_bar)
-    acc["e_half"] += torch.einsum("wpa,wqa->pq", t_bar, w_in.conj()).real
-    return et @ t_bar, vhs_bar
+def _trotter_adj(lay: _layouts, run: adjoint_run, w_in, vhs_planes, w_bar_out, acc):
+    """(w_bar_in, VHS_bar planes (2, nw, n, n)) of w' = E T(E w) on the fused
+    HIP adjoint (afq_trotter_adjoint: forward terms recomputed with VHS on chip,
+    reverse recursion, VHS_bar = sum_k t_bar_k t_{k-1}^H); accumulates
+    E_bar = Re(w'_bar u^H + t_bar_0 w^H), summed over walkers."""
+    nw, n, o = w_in.shape
+    w_bar_out = w_bar_out.contiguous()
+    w_bar_in = torch.empty_like(w_in)
+    vhs_bar = torch.empty((2, nw, n, n), dtype=f64, device=w_in.device)
+    kernels._call("afq_trotter_adjoint", w_in, nw, n, o, lay.ldv, run.n_exp_terms, run.dt ** 0.5,
+                  lay.e_pad.data_ptr(), lay.e_pad_t.data_ptr(), vhs_planes.data_ptr(), w_in.data_ptr(),
+                  w_bar_out.data_ptr(), w_bar_in.data_ptr(), vhs_bar.data_ptr(), lay.e_half_bar.data_ptr(),
+                  lay.trotter_ws.data_ptr(), kernels._stream(w_in))
+    acc["e_half"] += lay.e_half_bar.sum(0)
+    return w_bar_in, vhs_bar
 
 
 def _step_adj(lay: _layouts, run: adjoint_run, tp, adj, e_est: float, acc):
@@ -354,18 +352,17 @@ def _step_adj(lay: _layouts, run: adjoint_run, tp, adj, e_est: float, acc):
     ld_bar_new = 2.0 * (lo_bar + amix)
     w_bar = w_bar + _greens_adj(tp["w_new"], lay.trial, ld_bar=ld_bar_new)
     # Trotter
-    vhs_c = torch.complex(tp["vhs"][0, :, :, :n], tp["vhs"][1, :, :, :n])
-    w_bar_in, vhs_bar = _trotter_adj(lay, run, tp["w"], vhs_c, w_bar, acc)
-    # VHS = sum_g xbar_g L_g, on the library's fp64 MFMA GEMM (C = alpha A B^T + beta C):
-    # xbar_bar = VHS_bar . L (real and imaginary planes as rows), and
-    # L_bar += Re(xbar)^T Re(VHS_bar) + Im(xbar)^T Im(VHS_bar) as one GEMM over K = 2 nw
-    v2 = torch.stack([vhs_bar.real, vhs_bar.imag]).reshape(2 * nw, n * n)
+    w_bar_in, vhs_bar = _trotter_adj(lay, run, tp["w"], tp["vhs"], w_bar, acc)
+    # VHS = sum_g xbar_g L_g: xbar_bar = VHS_bar . L (real and imaginary planes
+    # as rows) on the fp64 MFMA GEMM, and L_bar += Re(xbar)^T Re(VHS_bar) +
+    # Im(xbar)^T Im(VHS_bar) as one library GEMM over K = 2 nw
+    v2 = vhs_bar.reshape(2 * nw, n * n)
     xbb = torch.empty((2 * nw, g), dtype=f64, device=v2.device)
     _dgemm(v2, lay.chol_flat, xbb)
     xbar_bar = torch.complex(xbb[:nw], xbb[nw:])
     xbar_c = torch.complex(tp["xbar"][0], tp["xbar"][1])
-    _dgemm(tp["xbar"].reshape(2 * nw, g).T.contiguous(), v2.T.contiguous(),
-           acc["chol"].view(g, n * n), alpha=1.0, beta=1.0)
+    cb = acc["chol"].view(g, n * n)
+    cb.addmm_(tp["xbar"].reshape(2 * nw, g).T, v2)
     # field shifts: fs = -sqrt_dt (i fb - mf), xbar = x - fs
     mf = lay.mf
     xbar_bar = xbar_bar + mf.conj()[None, :] * shift_bar[:, None]
@@ -380,8 +377,8 @@ def _step_adj(lay: _layouts, run: adjoint_run, tp, adj, e_est: float, acc):
     gb = torch.empty((2 * nw, o * n), dtype=f64, device=fb2.device)
     _dgemm(fb2, lay.rot_flat_t, gb, alpha=2.0)
     g_bar = torch.complex(gb[:nw], gb[nw:]).view(nw, o, n)
-    _dgemm(fb2.T.contiguous(), tp["g"].reshape(2 * nw, o * n).T.contiguous(),
-           acc["rot"].view(g, o * n), alpha=2.0, beta=1.0)
+    rb = acc["rot"].view(g, o * n)
+    rb.addmm_(fb2.T, tp["g"].reshape(2 * nw, o * n), alpha=2.0)
     w_bar_in = w_bar_in + _greens_adj(tp["w"], lay.trial, g_c=g_in, g_bar=g_bar)
     return (w_bar_in, wt_bar_in, lo_bar_in, pop_bar_in)
 
@@ -472,6 +469,9 @@ def _check_run(h1, chol, run: adjoint_run):
         raise ValueError(f"shapes h1 {tuple(h1.shape)} / chol {tuple(chol.shape)} do not match")
     if tuple(run.trial.shape[:1]) != (n,) or run.trial.dtype != f64:
         raise ValueError("trial must be float64 (n, o)")
+    if n > 128 or run.trial.shape[1] > 32:
+        raise ValueError(f"the Trotter adjoint kernel takes n <= 128 and o <= 32, got n={n} "
+                         f"o={run.trial.shape[1]}")
     steps = run.n_blocks * run.n_prop_steps
     if tuple(run.fields.shape) != (steps, run.n_walkers, g) or run.fields.dtype != f64:
         raise ValueError(f"fields must be float64 {(steps, run.n_walkers, g)}")

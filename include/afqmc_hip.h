@@ -226,6 +226,23 @@ int afq_trotter(int nw, int n, int o, int nspin, int ldv, int n_terms, double sq
                 double* wout_up, double* wout_dn, void* stream);
 
 /*
+ * Adjoint of afq_trotter (fp64, restricted walkers, n <= 128, o <= 32) for
+ * the reverse pass of a differentiated run (prop/chol_afqmc_ops.py:173-196,
+ * prop/utils.py:20-31 differentiated): w' = E u, u = sum_k t_k, t_0 = E w,
+ * t_k = (i sqrt_dt / k) vhs t_{k-1}.  Given w_bar_out (dJ/dw' as complex
+ * adjoints, (nw, n, o) complex128), writes w_bar_in = E^T t_bar_0, vhs_bar
+ * (planes (2, nw, n, n): sum_k conj(i sqrt_dt / k) t_bar_k t_{k-1}^H) and
+ * e_half_bar (nw, n, n): each walker's share Re(w_bar_out u^H + t_bar_0 w^H)
+ * of E's adjoint (the caller sums them).  exp_h1_half and exp_h1_half_t (its
+ * transpose) are (n, ldv); vhs the forward step's planes (2, nw, n, ldv)
+ * (complex symmetric); workspace 2 (2 n_terms + 2) nw n 16 ceil(o / 16)
+ * doubles.  No counterpart in the reference (jax differentiates its step).
+ */
+int afq_trotter_adjoint(int nw, int n, int o, int ldv, int n_terms, double sqrt_dt, const double* exp_h1_half,
+                        const double* exp_h1_half_t, const double* vhs, const double* w_in, const double* w_bar_out,
+                        double* w_bar_in, double* vhs_bar, double* e_half_bar, double* workspace, void* stream);
+
+/*
  * afq_trotter with E, VHS and the Taylor recursion in fp32 (walkers still read
  * and written as complex128): the Trotter step of the mixed step.
  * exp_h1_half (n, ldv) float32; vhs planes (2, nw, n, ldv) float32.

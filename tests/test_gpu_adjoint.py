@@ -51,6 +51,59 @@ def test_adjoint_matches_autograd_restatement(gpu_device, kw):
     assert eh < 1e-8 and el < 1e-8, (eh, el)
 
 
+@pytest.mark.parametrize("n,o", [(24, 6), (96, 16), (120, 20), (128, 32)])
+def test_trotter_adjoint_kernel_matches_autograd(gpu_device, n, o):
+    """afq_trotter_adjoint against torch.autograd through a complex128
+    restatement of the Trotter step (CPU), for a general real E (so E^T is
+    exercised) and a complex-symmetric V: w_bar, V_bar and E_bar to 1e-12
+    relative.  (n, o) = (120, 20), (128, 32): two column tiles and V k-blocks
+    kept in LDS."""
+    import torch
+    from ad_afqmc_prototype_amd import kernels
+    nw, K, dt = 5, 6, 0.01
+    g = torch.Generator().manual_seed(n + o)
+    cplx = lambda *sh: torch.complex(torch.randn(*sh, generator=g, dtype=torch.float64),  # noqa: E731
+                                     torch.randn(*sh, generator=g, dtype=torch.float64))
+    w = cplx(nw, n, o)
+    v = cplx(nw, n, n)
+    v = 0.5 * (v + v.transpose(1, 2)) * (0.3 / n ** 0.5 / dt ** 0.5)
+    e = torch.eye(n, dtype=torch.float64) + 0.01 * torch.randn(n, n, generator=g, dtype=torch.float64)
+    wb = cplx(nw, n, o)
+    # reference: J = Re sum conj(wb) w'
+    wl, vl, el = w.clone().requires_grad_(), v.clone().requires_grad_(), e.clone().requires_grad_()
+    a = 1.0j * dt ** 0.5
+    t = el.to(torch.complex128) @ wl
+    u = t
+    for k in range(1, K + 1):
+        t = (a / k) * (vl @ t)
+        u = u + t
+    wp = el.to(torch.complex128) @ u
+    torch.sum(wb.conj() * wp).real.backward()
+    ldv = kernels.ldv_for(n)
+    dev = gpu_device
+    e_pad = torch.zeros((n, ldv), dtype=torch.float64)
+    e_pad[:, :n] = e
+    e_pad_t = torch.zeros((n, ldv), dtype=torch.float64)
+    e_pad_t[:, :n] = e.T
+    vp = torch.zeros((2, nw, n, ldv), dtype=torch.float64)
+    vp[0, :, :, :n], vp[1, :, :, :n] = v.real, v.imag
+    tc = (o + 15) // 16 * 16
+    ws = torch.empty((2 * (2 * K + 2) * nw * n * tc,), dtype=torch.float64, device=dev)
+    wd, wbd = w.to(dev), wb.to(dev)
+    w_bar = torch.empty_like(wd)
+    v_bar = torch.empty((2, nw, n, n), dtype=torch.float64, device=dev)
+    e_bar = torch.empty((nw, n, n), dtype=torch.float64, device=dev)
+    e_pad, e_pad_t, vp = e_pad.to(dev), e_pad_t.to(dev), vp.to(dev)
+    kernels._call("afq_trotter_adjoint", wd, nw, n, o, ldv, K, dt ** 0.5, e_pad.data_ptr(), e_pad_t.data_ptr(),
+                  vp.data_ptr(), wd.data_ptr(), wbd.data_ptr(), w_bar.data_ptr(), v_bar.data_ptr(),
+                  e_bar.data_ptr(), ws.data_ptr(), kernels._stream(wd))
+    torch.cuda.synchronize()
+    rel = lambda x, y: float((x - y).abs().max() / y.abs().max())  # noqa: E731
+    assert rel(w_bar.cpu(), wl.grad) < 1e-12
+    assert rel(torch.complex(v_bar[0], v_bar[1]).cpu(), vl.grad) < 1e-12
+    assert rel(e_bar.sum(0).cpu(), el.grad) < 1e-12
+
+
 def _workload(device, n_blocks=5, n_prop=10):
     import torch
     from ad_afqmc_prototype_amd.prop import adjoint as adj
