@@ -50,6 +50,7 @@ _SIGNATURES = {
     "afq_trotter": [c_int, c_int, c_int, c_int, c_int, c_int, c_double, c_vp, c_vp, c_vp, c_vp,
                     c_vp, c_vp, c_vp],
     "afq_trotter_adjoint": [c_int, c_int, c_int, c_int, c_int, c_double] + [c_vp] * 10,
+    "afq_greens_adjoint": [c_int, c_int, c_int, c_vp, c_vp, c_vp, c_vp, c_vp, c_int, c_vp],
     "afq_phaseless_update": [c_int, c_int, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_double, c_double,
                              c_double, c_double, c_double, c_double, c_double, c_double, c_vp,
                              c_vp, c_vp, c_vp],
@@ -92,7 +93,7 @@ _SIGNATURES = {
     "afq_unpack_pairs": [c_int, c_int, c_vp, c_ll, c_vp, c_vp],
 }
 
-ABI_VERSION = 17
+ABI_VERSION = 18
 
 
 class HipLibraryError(RuntimeError):

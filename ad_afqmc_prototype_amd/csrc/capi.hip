@@ -43,4 +43,4 @@ void allow_full_lds(const void* fn) {
 
 extern "C" const char* afq_last_error(void) { return g_last_error; }
 
-extern "C" int afq_abi_version(void) { return 17; }
+extern "C" int afq_abi_version(void) { return 18; }

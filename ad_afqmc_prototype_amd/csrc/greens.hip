@@ -935,6 +935,146 @@ void launch_greens_wave(dim3 grid, hipStream_t st, int nw, int n, int o, int nsp
                        greens, ld, hi, lo, ldk);
 }
 
+// ---------------------------------------------------------------------------
+// Adjoint of the Green's function and log-overlap (meas/rhf.py:15-24,
+// trial/rhf.py:25-44 differentiated; restricted walkers, real C): for
+// G = (w m^-1)^T, ld = log det m, m = C^T w, and complex adjoints G_bar
+// (o x n, optional) and ld_bar (optional),
+//   w_bar = [G_bar^T - C (m^-H Z - ld_bar I)] m^-H,   Z = w^H G_bar^T.
+// One 256-thread workgroup per walker: m and Z by direct sums over the n
+// orbitals, m^-1 by Gauss-Jordan elimination with partial pivoting on the
+// augmented [m | I] in LDS, then the two o-deep products.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(THREADS) void greens_adjoint_kernel(int n, int o, const double2* __restrict__ W,
+                                                                 const double* __restrict__ Cm,
+                                                                 const double2* __restrict__ gbar,
+                                                                 const double2* __restrict__ ldbar,
+                                                                 double2* __restrict__ wbar, int accumulate) {
+    extern __shared__ __attribute__((aligned(16))) unsigned char lds_raw[];
+    const int o2 = 2 * o;
+    double2* aug = reinterpret_cast<double2*>(lds_raw);  // [o][2o]: [m | I] -> [I | m^-1]
+    double2* Z = aug + o * o2;                            // [o][o]: Z
+    double2* Q = Z + o * o;                               // [o][o]: m^-H Z - ld_bar I
+    double2* Y = Q + o * o;                               // [n][o]
+    __shared__ double2 colk[64];
+    __shared__ int piv;
+    const int w = blockIdx.x;
+    const int tid = threadIdx.x;
+    const double2* Ww = W + (long long)w * n * o;
+    const double2* Gb = gbar ? gbar + (long long)w * o * n : nullptr;
+    const double2 lb = ldbar ? ldbar[w] : make_double2(0.0, 0.0);
+
+    // m = C^T w, Z = w^H G_bar^T
+    for (int e = tid; e < o * o; e += THREADS) {
+        const int i = e / o, j = e - (e / o) * o;
+        double2 m = make_double2(0.0, 0.0), z = make_double2(0.0, 0.0);
+        for (int p = 0; p < n; ++p) {
+            const double2 wj = Ww[p * o + j];
+            const double c = Cm[p * o + i];
+            m.x = fma(c, wj.x, m.x);
+            m.y = fma(c, wj.y, m.y);
+            if (Gb) {
+                const double2 wi = Ww[p * o + i];
+                const double2 g = Gb[j * n + p];
+                // conj(wi) g
+                z.x = fma(wi.x, g.x, fma(wi.y, g.y, z.x));
+                z.y = fma(wi.x, g.y, fma(-wi.y, g.x, z.y));
+            }
+        }
+        aug[i * o2 + j] = m;
+        aug[i * o2 + o + j] = make_double2(i == j ? 1.0 : 0.0, 0.0);
+        Z[e] = z;
+    }
+    __syncthreads();
+    // Gauss-Jordan with partial pivoting
+    for (int k = 0; k < o; ++k) {
+        if (tid < 64) {
+            // pivot: the largest |aug[r][k]|, r >= k (lowest r on ties)
+            double best = -1.0;
+            int br = k;
+            for (int r = k + tid; r < o; r += 64) {
+                const double2 v = aug[r * o2 + k];
+                const double a = v.x * v.x + v.y * v.y;
+                if (a > best) {
+                    best = a;
+                    br = r;
+                }
+            }
+#pragma unroll
+            for (int off = 32; off > 0; off >>= 1) {
+                const double ob = __shfl_xor(best, off, 64);
+                const int orr = __shfl_xor(br, off, 64);
+                if (ob > best || (ob == best && orr < br)) {
+                    best = ob;
+                    br = orr;
+                }
+            }
+            if (tid == 0) piv = br;
+        }
+        __syncthreads();
+        const int r = piv;
+        if (r != k) {
+            for (int j = tid; j < o2; j += THREADS) {
+                const double2 t = aug[k * o2 + j];
+                aug[k * o2 + j] = aug[r * o2 + j];
+                aug[r * o2 + j] = t;
+            }
+        }
+        __syncthreads();
+        const double2 inv = crecip(aug[k * o2 + k]);
+        if (tid < o) colk[tid] = aug[tid * o2 + k];  // the multipliers, before the row is scaled
+        __syncthreads();
+        for (int j = tid; j < o2; j += THREADS) aug[k * o2 + j] = cmul(aug[k * o2 + j], inv);
+        __syncthreads();
+        for (int e = tid; e < o * o2; e += THREADS) {
+            const int i = e / o2, j = e - (e / o2) * o2;
+            if (i != k) csub_mul(aug[e], colk[i], aug[k * o2 + j]);
+        }
+        __syncthreads();
+    }
+    // A = m^-1 = aug[:, o:]; Q = A^H Z - ld_bar I
+    for (int e = tid; e < o * o; e += THREADS) {
+        const int i = e / o, j = e - (e / o) * o;
+        double2 v = make_double2(i == j ? -lb.x : 0.0, i == j ? -lb.y : 0.0);
+        if (Gb) {
+            for (int k = 0; k < o; ++k) {
+                const double2 a = aug[k * o2 + o + i];
+                const double2 z = Z[k * o + j];
+                v.x = fma(a.x, z.x, fma(a.y, z.y, v.x));
+                v.y = fma(a.x, z.y, fma(-a.y, z.x, v.y));
+            }
+        }
+        Q[e] = v;
+    }
+    __syncthreads();
+    // Y = G_bar^T - C Z
+    for (int e = tid; e < n * o; e += THREADS) {
+        const int p = e / o, j = e - (e / o) * o;
+        double2 y = Gb ? Gb[j * n + p] : make_double2(0.0, 0.0);
+        for (int i = 0; i < o; ++i) {
+            const double c = Cm[p * o + i];
+            const double2 z = Q[i * o + j];
+            y.x = fma(-c, z.x, y.x);
+            y.y = fma(-c, z.y, y.y);
+        }
+        Y[e] = y;
+    }
+    __syncthreads();
+    // w_bar = Y A^H:  w_bar[p][j] = sum_i Y[p][i] conj(A[j][i])
+    double2* out = wbar + (long long)w * n * o;
+    for (int e = tid; e < n * o; e += THREADS) {
+        const int p = e / o, j = e - (e / o) * o;
+        double2 v = accumulate ? out[e] : make_double2(0.0, 0.0);
+        for (int i = 0; i < o; ++i) {
+            const double2 y = Y[p * o + i];
+            const double2 a = aug[j * o2 + o + i];
+            v.x = fma(y.x, a.x, fma(y.y, a.y, v.x));
+            v.y = fma(y.y, a.x, fma(-y.x, a.y, v.y));
+        }
+        out[e] = v;
+    }
+}
+
 }  // namespace
 
 extern "C" int afq_overlap_greens(int nw, int n, int o, int nspin, const double* walkers_up,
@@ -1087,4 +1227,20 @@ extern "C" int afq_overlap_greens_ctrial(int nw, int n, int o, int nspin, const 
         hipLaunchKernelGGL(greens_ctrial_kernel<64>, dim3((unsigned)(nw * nspin)), dim3(THREADS), 0, st, nw, n, o,
                            nspin, wu, wd, trial_up, cd, greens, ld);
     return afq::check_launch("afq_overlap_greens_ctrial");
+}
+
+extern "C" int afq_greens_adjoint(int nw, int n, int o, const double* walkers, const double* trial,
+                                  const double* g_bar, const double* ld_bar, double* w_bar, int accumulate,
+                                  void* stream) {
+    AFQ_REQUIRE(nw > 0 && n > 0 && o > 0 && o <= n, "afq_greens_adjoint: bad sizes nw=%d n=%d o=%d", nw, n, o);
+    AFQ_REQUIRE(o <= 32, "afq_greens_adjoint: o=%d exceeds the supported maximum of 32", o);
+    AFQ_REQUIRE(walkers && trial && w_bar, "afq_greens_adjoint: null pointer");
+    AFQ_REQUIRE(g_bar || ld_bar, "afq_greens_adjoint: neither g_bar nor ld_bar given");
+    const size_t lds = sizeof(double2) * ((size_t)o * 2 * o + (size_t)2 * o * o + (size_t)n * o);
+    AFQ_REQUIRE(lds <= 150 * 1024, "afq_greens_adjoint: n=%d o=%d needs %zu bytes of LDS", n, o, lds);
+    if (lds > 64 * 1024) afq::allow_full_lds((const void*)greens_adjoint_kernel);
+    hipLaunchKernelGGL(greens_adjoint_kernel, dim3((unsigned)nw), dim3(THREADS), lds, (hipStream_t)stream, n, o,
+                       (const double2*)walkers, trial, (const double2*)g_bar, (const double2*)ld_bar,
+                       (double2*)w_bar, accumulate);
+    return afq::check_launch("afq_greens_adjoint");
 }

@@ -254,21 +254,20 @@ def _ct(x):
     return x.conj().transpose(-1, -2)
 
 
-def _greens_adj(w, trial, g_c=None, g_bar=None, ld_bar=None):
-    """w_bar of G = (w m^-1)^T (adjoint g_bar) and log det m (adjoint ld_bar),
-    m = C^T w:  w_bar = [G_bar^T - C (conj(G) G_bar^T - ld_bar I)] m^-H."""
-    c = trial.to(c128)
-    m = c.T @ w
-    if g_bar is None:
-        y = ld_bar[:, None, None] * c
-    else:
-        hb = g_bar.transpose(1, 2)
-        inner = g_c.conj() @ hb
-        if ld_bar is not None:
-            inner = inner - ld_bar[:, None, None] * torch.eye(m.shape[1], dtype=c128, device=w.device)
-        y = hb - c @ inner
-    # X = Y m^-H  <=>  m X^H = Y^H
-    return _ct(torch.linalg.solve(m, _ct(y)))
+def _greens_adj(w, trial, g_bar=None, ld_bar=None, out=None):
+    """w_bar of G = (w m^-1)^T (adjoint g_bar, (nw, o, n)) and log det m
+    (adjoint ld_bar), m = C^T w:  w_bar = [G_bar^T - C (conj(G) G_bar^T -
+    ld_bar I)] m^-H, on afq_greens_adjoint; added to ``out`` when given."""
+    nw, n, o = w.shape
+    acc = out is not None
+    if out is None:
+        out = torch.empty_like(w)
+    gb = g_bar.contiguous() if g_bar is not None else None
+    lb = ld_bar.contiguous() if ld_bar is not None else None
+    kernels._call("afq_greens_adjoint", w, nw, n, o, w.data_ptr(), trial.data_ptr(),
+                  gb.data_ptr() if gb is not None else None, lb.data_ptr() if lb is not None else None,
+                  out.data_ptr(), int(acc), kernels._stream(w))
+    return out
 
 
 def _qr_adj(a, q, q_bar):
@@ -350,7 +349,7 @@ def _step_adj(lay: _layouts, run: adjoint_run, tp, adj, e_est: float, acc):
     acc["e_est"] += pop_bar
     lo_bar_in = -amix
     ld_bar_new = 2.0 * (lo_bar + amix)
-    w_bar = w_bar + _greens_adj(tp["w_new"], lay.trial, ld_bar=ld_bar_new)
+    w_bar = _greens_adj(tp["w_new"], lay.trial, ld_bar=ld_bar_new, out=w_bar.contiguous().clone())
     # Trotter
     w_bar_in, vhs_bar = _trotter_adj(lay, run, tp["w"], tp["vhs"], w_bar, acc)
     # VHS = sum_g xbar_g L_g: xbar_bar = VHS_bar . L (real and imaginary planes
@@ -372,14 +371,13 @@ def _step_adj(lay: _layouts, run: adjoint_run, tp, adj, e_est: float, acc):
     mf_bar = mf_bar + sd * fs_bar.sum(0)
     acc["m"] += mf_bar.imag
     # force bias fb = 2 rot . vec(G): G_bar = 2 fb_bar . rot, rot_bar += 2 Re(fb_bar^H G)
-    g_in = _gc(tp["g"], nw, o, n)
     fb2 = torch.stack([fb_bar.real, fb_bar.imag]).reshape(2 * nw, g)
     gb = torch.empty((2 * nw, o * n), dtype=f64, device=fb2.device)
     _dgemm(fb2, lay.rot_flat_t, gb, alpha=2.0)
     g_bar = torch.complex(gb[:nw], gb[nw:]).view(nw, o, n)
     rb = acc["rot"].view(g, o * n)
     rb.addmm_(fb2.T, tp["g"].reshape(2 * nw, o * n), alpha=2.0)
-    w_bar_in = w_bar_in + _greens_adj(tp["w"], lay.trial, g_c=g_in, g_bar=g_bar)
+    w_bar_in = _greens_adj(tp["w"], lay.trial, g_bar=g_bar, out=w_bar_in)
     return (w_bar_in, wt_bar_in, lo_bar_in, pop_bar_in)
 
 
@@ -411,7 +409,7 @@ def _block_end_adj(lay: _layouts, run: adjoint_run, end, adj, e_b_bar, w_b_bar, 
     e_i_bar = torch.where(clipped, torch.zeros_like(e_i_bar), e_i_bar)
     gq_c = _gc(end["gq"], nw, o, n)
     gq_bar = _energy_adj(gq_c, e_i_bar, lay, acc)
-    q_bar = q_bar + _greens_adj(end["q"], lay.trial, g_c=gq_c, g_bar=gq_bar, ld_bar=2.0 * lo_bar_q)
+    q_bar = _greens_adj(end["q"], lay.trial, g_bar=gq_bar, ld_bar=2.0 * lo_bar_q, out=q_bar)
     w_bar = _qr_adj(end["w"], end["q"], q_bar)
     return (w_bar, wt_bar, torch.zeros((nw,), dtype=c128, device=idx.device), pop_bar)
 

@@ -226,6 +226,19 @@ int afq_trotter(int nw, int n, int o, int nspin, int ldv, int n_terms, double sq
                 double* wout_up, double* wout_dn, void* stream);
 
 /*
+ * Adjoint of the Green's function and log-overlap (meas/rhf.py:15-24,
+ * trial/rhf.py:25-44 differentiated; restricted walkers, real trial C (n, o),
+ * o <= 32, n <= 128 with o = 32): for G = (w m^-1)^T (o x n), ld = log det m, m = C^T w, and the
+ * complex adjoints g_bar ((nw, o, n) complex128, may be null) and ld_bar ((nw)
+ * complex128, may be null; not both), writes (accumulate = 0) or adds
+ * (accumulate = 1) w_bar = [g_bar^T - C (m^-H w^H g_bar^T - ld_bar I)] m^-H
+ * into w_bar ((nw, n, o) complex128).  m^-1 by Gauss-Jordan elimination with
+ * partial pivoting.  No counterpart in the reference (jax differentiates it).
+ */
+int afq_greens_adjoint(int nw, int n, int o, const double* walkers, const double* trial, const double* g_bar,
+                       const double* ld_bar, double* w_bar, int accumulate, void* stream);
+
+/*
  * Adjoint of afq_trotter (fp64, restricted walkers, n <= 128, o <= 32) for
  * the reverse pass of a differentiated run (prop/chol_afqmc_ops.py:173-196,
  * prop/utils.py:20-31 differentiated): w' = E u, u = sum_k t_k, t_0 = E w,

@@ -104,6 +104,52 @@ def test_trotter_adjoint_kernel_matches_autograd(gpu_device, n, o):
     assert rel(e_bar.sum(0).cpu(), el.grad) < 1e-12
 
 
+@pytest.mark.parametrize("n,o,parts", [(24, 6, "both"), (96, 16, "g"), (96, 16, "ld"), (120, 32, "both"),
+                                        (128, 27, "both")])
+def test_greens_adjoint_kernel_matches_autograd(gpu_device, n, o, parts):
+    """afq_greens_adjoint against torch.autograd (CPU, complex128) of
+    G = (w m^-1)^T and log det m, m = C^T w, for J = Re sum conj(G_bar) G +
+    Re sum conj(ld_bar) ld: w_bar to 1e-11 relative, written and accumulated."""
+    import torch
+    from ad_afqmc_prototype_amd import kernels
+    nw = 6
+    g = torch.Generator().manual_seed(7 * n + o)
+    cplx = lambda *sh: torch.complex(torch.randn(*sh, generator=g, dtype=torch.float64),  # noqa: E731
+                                     torch.randn(*sh, generator=g, dtype=torch.float64))
+    c = torch.linalg.qr(torch.randn(n, o, generator=g, dtype=torch.float64))[0].contiguous()
+    w = (c.to(torch.complex128)[None] + 0.3 * cplx(nw, n, o)).contiguous()
+    gb = cplx(nw, o, n) if parts in ("g", "both") else None
+    lb = cplx(nw) if parts in ("ld", "both") else None
+    wl = w.clone().requires_grad_()
+    m = c.T.to(torch.complex128) @ wl
+    j = 0.0
+    if gb is not None:
+        gmat = (wl @ torch.linalg.inv(m)).transpose(1, 2)
+        j = j + torch.sum(gb.conj() * gmat).real
+    if lb is not None:
+        j = j + torch.sum(lb.conj() * torch.linalg.slogdet(m)[1]).real
+        j = j + torch.sum((lb.conj() * torch.log(torch.linalg.slogdet(m)[0]))).real
+    j.backward()
+    dev = gpu_device
+    wd, cd = w.to(dev), c.to(dev).contiguous()
+    gbd = gb.to(dev) if gb is not None else None
+    lbd = lb.to(dev) if lb is not None else None
+    out = torch.empty_like(wd)
+    kernels._call("afq_greens_adjoint", wd, nw, n, o, wd.data_ptr(), cd.data_ptr(),
+                  gbd.data_ptr() if gbd is not None else None, lbd.data_ptr() if lbd is not None else None,
+                  out.data_ptr(), 0, kernels._stream(wd))
+    base = cplx(nw, n, o).to(dev)
+    acc = base.clone()
+    kernels._call("afq_greens_adjoint", wd, nw, n, o, wd.data_ptr(), cd.data_ptr(),
+                  gbd.data_ptr() if gbd is not None else None, lbd.data_ptr() if lbd is not None else None,
+                  acc.data_ptr(), 1, kernels._stream(wd))
+    torch.cuda.synchronize()
+    ref = wl.grad
+    err = float((out.cpu() - ref).abs().max() / ref.abs().max())
+    assert err < 1e-11, err
+    assert torch.allclose(acc - base, out, rtol=0, atol=1e-12 * float(out.abs().max()))
+
+
 def _workload(device, n_blocks=5, n_prop=10):
     import torch
     from ad_afqmc_prototype_amd.prop import adjoint as adj
