@@ -58,7 +58,7 @@ import torch
 from .. import kernels
 from .. import walkers as wk
 from ..core.system import system
-from .chol_afqmc_ops import chol_transposed
+from .chol_afqmc_ops import chol_sym_rows
 
 c128 = torch.complex128
 f64 = torch.float64
@@ -135,7 +135,7 @@ class _layouts:
                                       device=chol.device)
         self.e_half_bar = torch.empty((run.n_walkers, n, n), dtype=f64, device=chol.device)
         self.chol = chol.contiguous()
-        self.chol_t = chol_transposed(chol)
+        self.chol_sym = chol_sym_rows(chol)                  # B operand of the symmetric VHS
         self.mf = (1.0j * m).to(c128).contiguous()
         self.h0_prop = float(h0_prop)
         self.rot = rot.contiguous()
@@ -192,7 +192,7 @@ def _block(lay: _layouts, run: adjoint_run, st, b: int, e_est: float, record: bo
         x = run.fields[b * run.n_prop_steps + t]
         fb = kernels.force_bias(gpl, lay.rot_flat, 2.0)
         xbar, shift, fbt = kernels.field_shifts(x, fb, lay.mf, sd)
-        vhs = kernels.vhs(xbar, lay.chol_t, lay.n)
+        vhs = kernels.vhs_sym(xbar.view(2 * lay.nw, -1), lay.chol_sym, lay.n).view(2, lay.nw, lay.n, lay.ldv)
         w_new = kernels.trotter(w, vhs, lay.e_pad, sd, run.n_exp_terms)
         g_new, ld_new = kernels.overlap_greens(w_new, lay.trial)
         lo_new, wt_new, pop_new = kernels.phaseless_update(
@@ -280,23 +280,51 @@ def _qr_adj(a, q, q_bar):
     return _ct(torch.linalg.solve_triangular(r, _ct(x), upper=True))
 
 
-def _energy_adj(g_c, e_bar, lay: _layouts, acc, chunk: int = 64):
+def _energy_adj(g_c, e_bar, lay: _layouts, acc, max_bytes: int = 1 << 30):
     """G_bar of the local energies (meas/rhf.py:36-50) for real adjoints e_bar of
-    Re E; accumulates rot_bar and rot_h1_bar."""
-    rc = lay.rot.to(c128)
-    rh = lay.rot_h1.to(c128)
+    Re E; accumulates rot_bar and rot_h1_bar.  The half-rotated integrals R are
+    real, so every product runs on the real and imaginary planes of G, as plain
+    library GEMMs over all walkers of a chunk at once:
+      F[(g,a),(w,b)] = sum_p R[g,a,p] G[w,b,p]             (f_g of walker w)
+      G_bar_w = e_bar_w conj(2 rot_h1 + 4 sum_g c_g R_g - 2 sum_{g,b} F[(g,a),(w,b)] R[g,b])
+      rot_bar += Re sum_w e_bar_w (4 c_g G_w - 2 f_g^T G_w)
+    with c_g = tr f_g; F is regrouped once per use ((w,a),(g,b) and the
+    transposed blocks (g,a),(w,b)) so that each sum is one GEMM."""
+    nw, o, n = g_c.shape
+    g = lay.g
+    r = lay.rot.reshape(g * o, n)                      # rows (g, a)
     out = torch.empty_like(g_c)
-    for s in range(0, g_c.shape[0], chunk):
-        gch = g_c[s:s + chunk]
-        eb = e_bar[s:s + chunk].to(c128)
-        f = torch.einsum("gap,wbp->wgab", rc, gch)
-        cg = torch.diagonal(f, dim1=2, dim2=3).sum(-1)
-        de = (2.0 * rh + 4.0 * torch.einsum("wg,gap->wap", cg, rc)
-              - 2.0 * torch.einsum("wgab,gbp->wap", f, rc))
-        out[s:s + chunk] = eb[:, None, None] * de.conj()
-        acc["rot_h1"] += 2.0 * torch.einsum("w,wap->ap", eb, gch).real
-        acc["rot"] += (4.0 * torch.einsum("wg,wap->gap", cg * eb[:, None], gch)
-                       - 2.0 * torch.einsum("wgba,wbp->gap", f * eb[:, None, None, None], gch)).real
+    # walkers per chunk: F and its two regroupings, real and imaginary planes
+    chunk = max(1, min(nw, max_bytes // (6 * 8 * g * o * o)))
+    for s0 in range(0, nw, chunk):
+        gch = g_c[s0:s0 + chunk]
+        W = gch.shape[0]
+        eb = e_bar[s0:s0 + chunk]
+        gp = (gch.real.reshape(W * o, n), gch.imag.reshape(W * o, n))
+        de = []
+        rot_acc = torch.zeros((g * o, n), dtype=f64, device=g_c.device)
+        c_planes = []
+        for k, gk in enumerate(gp):
+            f = r @ gk.T                                   # (g*o, W*o): [g][a][w][b]
+            f4 = f.view(g, o, W, o)
+            c = torch.diagonal(f4, dim1=1, dim2=3).sum(-1)  # (g, W)
+            c_planes.append(c)
+            # de_w[a][p] = sum_{g,b} f[g][a][w][b] R[g][b][p]: rows (w, a), K (g, b)
+            fw = f4.permute(2, 1, 0, 3).reshape(W * o, g * o)
+            d = -2.0 * (fw @ r).view(W, o, n)
+            d += 4.0 * (c.T @ lay.rot.reshape(g, o * n)).view(W, o, n)
+            de.append(d)
+            # rot_bar[g][a][p] += sign * sum_{w,b} f[g][b][w][a] eb_w G_k[w][b][p]
+            sign = 1.0 if k == 0 else -1.0
+            fg = f4.permute(0, 3, 2, 1).reshape(g * o, W * o)   # [g][a][w][b] <- f[g][b][w][a]
+            ge = (gk.view(W, o, n) * eb[:, None, None]).reshape(W * o, n)
+            rot_acc.add_(fg @ ge, alpha=-2.0 * sign)
+            rot_acc.view(g, o * n).add_((c * eb[None, :]) @ gk.view(W, o * n), alpha=4.0 * sign)
+            del f, fw, fg
+        de_r = de[0] + 2.0 * lay.rot_h1
+        out[s0:s0 + chunk] = torch.complex(eb[:, None, None] * de_r, -eb[:, None, None] * de[1])
+        acc["rot_h1"] += 2.0 * torch.einsum("w,wap->ap", eb, gch.real)
+        acc["rot"] += rot_acc.view(g, o, n)
     return out
 
 
