@@ -270,23 +270,44 @@ __device__ __forceinline__ void dma_mainloop(int M, int N, int K, const uint16_t
     // copy assignment: block gb = wave * BPW + i of the k-step, in plane order
     // (A hi, A lo, B hi, B lo); lane L fills slot L of a block: row L / 4,
     // position L % 4, i.e. chunk (L % 4) ^ ((L / 8) & 3)
-    const rsrc_t R0 = make_rsrc(Ahi + (long long)m0 * lda, (unsigned long long)max(M - m0, 0) * lda * 2);
-    const rsrc_t R1 = make_rsrc(Alo + (long long)m0 * lda, (unsigned long long)max(M - m0, 0) * lda * 2);
-    const rsrc_t R2 = make_rsrc(Bhi + (long long)n0 * ldb, (unsigned long long)max(N - n0, 0) * ldb * 2);
-    const rsrc_t R3 = make_rsrc(Blo + (long long)n0 * ldb, (unsigned long long)max(N - n0, 0) * ldb * 2);
     const int r_loc = lane >> 2;
     const int chunk = (lane & 3) ^ ((lane >> 3) & 3);
+    // the descriptor of plane p (A planes from row m0, B planes from row n0)
+    auto plane_rsrc = [&](int p) {
+        const uint16_t* src = p == 0 ? Ahi + (long long)m0 * lda
+                            : p == 1 ? Alo + (long long)m0 * lda
+                            : p == 2 ? Bhi + (long long)n0 * ldb
+                                     : Blo + (long long)n0 * ldb;
+        const long long ld = p < 2 ? lda : ldb;
+        const int rows_left = max(p < 2 ? M - m0 : N - n0, 0);
+        return make_rsrc(src, (unsigned long long)rows_left * ld * 2);
+    };
+    // WM = 4: each wave's eight blocks lie in one plane (one descriptor per wave)
+    const int plane4 = wave >> 1;
+    const rsrc_t R4 = plane_rsrc(WM == 4 ? plane4 : 0);
+    const rsrc_t R0 = WM == 4 ? R4 : plane_rsrc(0);
+    const rsrc_t R1 = WM == 4 ? R4 : plane_rsrc(1);
+    const rsrc_t R2 = WM == 4 ? R4 : plane_rsrc(2);
+    const rsrc_t R3 = WM == 4 ? R4 : plane_rsrc(3);
     auto issue_stage = [&](int kt, int buf) {
         const int k = kt * 32 + chunk * 8;
         // k >= K sets the top bit: the offset lands outside the descriptor
         const unsigned past = (unsigned)((K - 1 - k) >> 31) & kOutside;
 #pragma unroll
         for (int i = 0; i < D::BPW; ++i) {
-            const int gb = wave * D::BPW + i;
-            const int plane = gb < D::ABLK ? 0 : gb < 2 * D::ABLK ? 1 : gb < 2 * D::ABLK + 16 ? 2 : 3;
-            const int blk = gb - (plane < 2 ? plane * D::ABLK : 2 * D::ABLK + (plane - 2) * 16);
+            int plane, blk;
+            rsrc_t R;
+            if constexpr (WM == 4) {
+                plane = plane4;
+                blk = (wave & 1) * 8 + i;
+                R = R4;
+            } else {
+                const int gb = wave * D::BPW + i;
+                plane = gb < D::ABLK ? 0 : gb < 2 * D::ABLK ? 1 : gb < 2 * D::ABLK + 16 ? 2 : 3;
+                blk = gb - (plane < 2 ? plane * D::ABLK : 2 * D::ABLK + (plane - 2) * 16);
+                R = plane == 0 ? R0 : plane == 1 ? R1 : plane == 2 ? R2 : R3;
+            }
             const long long ld = plane < 2 ? lda : ldb;
-            const rsrc_t R = plane == 0 ? R0 : plane == 1 ? R1 : plane == 2 ? R2 : R3;
             const int row = blk * 16 + r_loc;
             const unsigned off = (unsigned)(((long long)row * ld + k) * 2) | past;
             __builtin_amdgcn_raw_ptr_buffer_load_lds(
@@ -373,7 +394,6 @@ __global__ __launch_bounds__(THREADS2, 1) void gemm_bf16x3_nt_256_kernel(
     int tiles_m, int tiles_n, afq::sym_dst sd = afq::sym_dst{0, 0, 0}) {
     // [stage][plane: A hi, A lo, B hi, B lo][256 rows x 64 bytes]
     __shared__ __attribute__((aligned(1024))) uint4 S[2][4][BM2 * 4];
-    static_assert(sizeof(S) == 2 * dma_shape<4>::STAGE * sizeof(uint4), "stage layout");
 
     const int total = tiles_m * tiles_n;
     const int t = afq::xcd_contiguous(blockIdx.x, total);
@@ -385,16 +405,87 @@ __global__ __launch_bounds__(THREADS2, 1) void gemm_bf16x3_nt_256_kernel(
     const int m0 = (first_m + in_group % gsize) * BM2;
     const int n0 = (in_group / gsize) * BN2;
 
-    f32x4 acc[4][8];
-    dma_mainloop<4>(M, N, K, Ahi, Alo, lda, Bhi, Blo, ldb, m0, n0, &S[0][0][0], acc);
-
     const int tid = threadIdx.x;
     const int lane = tid & 63;
-    const int wave = tid >> 6;
-    const int wm = wave & 3;
-    const int wn = wave >> 2;
+    // wave-uniform (scalar): the descriptors and LDS bases below must be
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int wm = wave & 3;   // 64-row slice of the tile
+    const int wn = wave >> 2;  // 128-column slice
+
+    // copy assignment: wave w moves plane w / 2 (A hi, A lo, B hi, B lo), 1 KB
+    // blocks 8 (w % 2) + i, i < 8 (rows 16 blk ..); lane L fills slot L of a
+    // block: row L / 4, position L % 4, i.e. chunk (L % 4) ^ ((L / 8) & 3)
+    const int plane = wave >> 1;
+    const uint16_t* src = plane == 0 ? Ahi + (long long)m0 * lda
+                        : plane == 1 ? Alo + (long long)m0 * lda
+                        : plane == 2 ? Bhi + (long long)n0 * ldb
+                                     : Blo + (long long)n0 * ldb;
+    const long long ld = plane < 2 ? lda : ldb;
+    const int rows_left = max(plane < 2 ? M - m0 : N - n0, 0);
+    const rsrc_t R = make_rsrc(src, (unsigned long long)rows_left * ld * 2);
+    const int r_loc = lane >> 2;
+    const int chunk = (lane & 3) ^ ((lane >> 3) & 3);
+    auto issue_stage = [&](int kt, int buf) {
+        const int k = kt * 32 + chunk * 8;
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+            const int blk = (wave & 1) * 8 + i;
+            const int row = blk * 16 + r_loc;
+            // k >= K sets the top bit: the offset lands outside the descriptor
+            const unsigned past = (unsigned)((K - 1 - k) >> 31) & kOutside;
+            const unsigned off = (unsigned)(((long long)row * ld + k) * 2) | past;
+            __builtin_amdgcn_raw_ptr_buffer_load_lds(R, (lds_ptr_t)&S[buf][plane][blk * 64], 16, off, 0, 0, 0);
+        }
+    };
+
+    f32x4 acc[4][8];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 8; ++j) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
+
+    // fragment read: row lr of a 16-row tile, chunk q at its swizzled position
     const int lr = lane & 15;
     const int q = lane >> 4;
+    const int lane_slot = lr * 4 + (q ^ ((lr >> 1) & 3));
+
+    const int nk = afq::ceil_div(K, 32);
+    issue_stage(0, 0);
+    asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+    __syncthreads();
+    for (int kt = 0; kt < nk; ++kt) {
+        const int buf = kt & 1;
+        if (kt + 1 < nk) issue_stage(kt + 1, buf ^ 1);
+        bf16x8 ah[4], al[4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const int slot = (wm * 64 + i * 16) * 4 + lane_slot;
+            ah[i] = __builtin_bit_cast(bf16x8, S[buf][0][slot]);
+            al[i] = __builtin_bit_cast(bf16x8, S[buf][1][slot]);
+        }
+#pragma unroll
+        for (int jh = 0; jh < 2; ++jh) {
+            bf16x8 bh[4], bl[4];
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const int slot = (wn * 128 + (jh * 4 + j) * 16) * 4 + lane_slot;
+                bh[j] = __builtin_bit_cast(bf16x8, S[buf][2][slot]);
+                bl[j] = __builtin_bit_cast(bf16x8, S[buf][3][slot]);
+            }
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    f32x4& c = acc[i][jh * 4 + j];
+                    c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[i], bh[j], c, 0, 0, 0);
+                    c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bl[j], c, 0, 0, 0);
+                    c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bh[j], c, 0, 0, 0);
+                }
+        }
+        // the next stage has landed in LDS and every wave is done with this one
+        asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+        __syncthreads();
+    }
 
     if constexpr (SYM) {
         // The 256-column tile is one block (bi, bj).  It leaves through LDS in
@@ -441,7 +532,17 @@ __global__ __launch_bounds__(THREADS2, 1) void gemm_bf16x3_nt_256_kernel(
         }
         return;
     }
-    dma_store<4>(M, N, alpha, C, ldc, m0, n0, acc);
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            const int col = n0 + wn * 128 + j * 16 + lr;
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int row = m0 + wm * 64 + i * 16 + q * 4 + r;
+                if (row < M && col < N) C[(long long)row * ldc + col] = (TOut)(alpha * acc[i][j][r]);
+            }
+        }
 }
 
 // Grouped LDS-DMA GEMM (tile shapes 1 and 2: 128 x 256 and 256 x 256):
