@@ -39,6 +39,26 @@ void allow_full_lds(const void* fn) {
     done.insert(fn);
 }
 
+int cu_count() {
+    static std::mutex mu;
+    static int counts[64] = {0};
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) {
+        (void)hipGetLastError();
+        return 256;
+    }
+    std::lock_guard<std::mutex> lock(mu);
+    if (counts[dev] == 0) {
+        int c = 0;
+        if (hipDeviceGetAttribute(&c, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || c <= 0) {
+            (void)hipGetLastError();
+            c = 256;
+        }
+        counts[dev] = c;
+    }
+    return counts[dev];
+}
+
 }  // namespace afq
 
 extern "C" const char* afq_last_error(void) { return g_last_error; }

@@ -18,6 +18,8 @@ void set_error(const char* fmt, ...);
 void allow_full_lds(const void* fn);
 // Converts a pending launch error into a return code (0 ok, -2 launch failure).
 int check_launch(const char* what);
+// Compute units of the current device (cached per device; 256 if the query fails).
+int cu_count();
 
 constexpr int kWave = 64;
 constexpr int kXcds = 8;  // MI355X: 8 XCDs, workgroups dispatched round-robin

@@ -176,7 +176,14 @@ __device__ __forceinline__ split8 split_minus_identity(const float (&x)[8], int 
 // prefix sums tblock0 (count systems, restricted walkers, (n, ldv) layouts);
 // its sizes and pointers replace the arguments, and blockIdx.x counts from
 // the system's first workgroup.
-template <int NL, bool SYMV = false, bool GROUPED = false>
+// PAIR (walkers with two items: nspin * ceil(o / 16) == 2, at least as many
+// walkers as CUs): one workgroup per walker runs both items, so V crosses into
+// the CU once per walker and E - I is loaded twice per walker instead of four
+// times (200 orbitals, 32 occupied, 2048 walkers: 0.707 -> 0.612 ms).  Both items' E W come first
+// (the second parked in its output rows), then V and the two Taylor series
+// (the first sum parked in its output rows), then E for both final halves.
+// Parked values are fp32 sums stored as fp64, so reading them back is exact.
+template <int NL, bool SYMV = false, bool GROUPED = false, bool PAIR = false>
 __global__ __launch_bounds__(512, 1) void trotter_bf16x3_kernel(
     int nw, int n, int o, int nspin, int nct, int ldv, int n_terms, double sqrt_dt,
     const float* __restrict__ E, const float* __restrict__ vhs, long long vplane,
@@ -218,7 +225,10 @@ __global__ __launch_bounds__(512, 1) void trotter_bf16x3_kernel(
 
     const int G = nspin * nct;
     int w, sub;
-    if (group_map) {
+    if (PAIR) {
+        w = b;
+        sub = 0;
+    } else if (group_map) {
         w = (b / (8 * G)) * 8 + (b % 8);
         sub = (b / 8) % G;
     } else {
@@ -320,18 +330,61 @@ __global__ __launch_bounds__(512, 1) void trotter_bf16x3_kernel(
             }
     };
 
+    // the second item of a PAIR walker: its columns, output and input
+    const int s1 = 1 / nct;
+    const int c01 = (1 - s1 * nct) * 16;
+    double2* Wout1 = (s1 == 0 ? wout_up : wout_dn) + woff;
+    // out-of-range rows and columns of a parked tile hold zeros (the walker's
+    // padding stays zero through every product)
+    auto park = [&](double2* dst, int cbase, const f32x4 (&X)[RT][2], int wv, int ln) {
+        const int cl = cbase + (ln & 15);
+        if (cl < o) {
+#pragma unroll
+            for (int t = 0; t < RT; ++t)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const int row = (wv * RT + t) * 16 + 4 * (ln >> 4) + r;
+                    if (row < n) dst[row * o + cl] = make_double2((double)X[t][0][r], (double)X[t][1][r]);
+                }
+        }
+    };
+    auto unpark = [&](const double2* src, int cbase, f32x4 (&X)[RT][2], int wv, int ln) {
+        const int cl = cbase + (ln & 15);
+#pragma unroll
+        for (int t = 0; t < RT; ++t)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int row = (wv * RT + t) * 16 + 4 * (ln >> 4) + r;
+                double2 v = make_double2(0.0, 0.0);
+                if (cl < o && row < n) v = src[row * o + cl];
+                X[t][0][r] = (float)v.x;
+                X[t][1][r] = (float)v.y;
+            }
+    };
+
     // ---- prologue: walker columns, accumulator rows and E in flight together ----
     f32x4 acc[RT][2];
     {
         // accumulator element r of tile t: row (wave*RT + t)*16 + 4q + r, column
         // col; the waves' accumulators cover all nk * 32 rows of T
-        double2 wa[RT][4];
+        double2 wa[RT][4], wb[RT][4];
 #pragma unroll
         for (int t = 0; t < RT; ++t)
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
                 wa[t][r] = load_w((wave * RT + t) * 16 + 4 * q + r, col);
             }
+        if constexpr (PAIR) {
+            const rsrc_t Wr1 = make_rsrc((s1 == 0 ? win_up : win_dn) + woff, (unsigned)n * o * 16u);
+            const int col1 = c01 + lr;
+#pragma unroll
+            for (int t = 0; t < RT; ++t)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const int p = (wave * RT + t) * 16 + 4 * q + r;
+                    wb[t][r] = load_d2(Wr1, col1 < o ? (unsigned)(p * o + col1) * 16u : kOutside);
+                }
+        }
         float er[RT][KMAX][8];
         load_e(er, row0, kq);
 #pragma unroll
@@ -345,6 +398,22 @@ __global__ __launch_bounds__(512, 1) void trotter_bf16x3_kernel(
         __syncthreads();
         // ---- T0 = E W ----
         e_product(acc, er, row0, kq);
+        if constexpr (PAIR) {
+            // the second item's E W, parked in its output rows
+            f32x4 acc1[RT][2];
+#pragma unroll
+            for (int t = 0; t < RT; ++t)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    acc1[t][0][r] = (float)wb[t][r].x;
+                    acc1[t][1][r] = (float)wb[t][r].y;
+                }
+            __syncthreads();
+            store_t(acc1);
+            __syncthreads();
+            e_product(acc1, er, row0, kq);
+            park(Wout1, c01, acc1, wave, lane);
+        }
     }
     f32x4 out[RT][2];
 #pragma unroll
@@ -457,6 +526,7 @@ __global__ __launch_bounds__(512, 1) void trotter_bf16x3_kernel(
     };
 
     // ---- Taylor terms: T_k = (i sqrt(dt) / k) V T_{k-1}, out += T_k ----
+    auto taylor = [&]() {
 #pragma unroll 1
     for (int k = 1; k <= n_terms; ++k) {
         f32x4 P[RT][2];
@@ -501,6 +571,19 @@ __global__ __launch_bounds__(512, 1) void trotter_bf16x3_kernel(
         store_t(P);
         __syncthreads();
     }
+    };
+    taylor();
+    if constexpr (PAIR) {
+        // first sum parked; the second item's T0 back as its sum's start
+        const int lane1 = opaque(lane);
+        const int wave1 = opaque(wave);
+        park(Wout, c0, out, wave1, lane1);
+        unpark(Wout1, c01, out, wave1, lane1);
+        __syncthreads();
+        store_t(out);
+        __syncthreads();
+        taylor();
+    }
 
     // ---- W_out = E out = out + (E - I) out ----
     // (indices recomputed from an opaque lane id: nothing address-like stays
@@ -510,6 +593,14 @@ __global__ __launch_bounds__(512, 1) void trotter_bf16x3_kernel(
     {
         float er[RT][KMAX][8];
         load_e(er, wave2 * RT * 16 + (lane2 & 15), 8 * (lane2 >> 4));
+        if constexpr (PAIR) {
+            __syncthreads();
+            store_t(out);
+            __syncthreads();
+            e_product(out, er, wave2 * RT * 16 + (lane2 & 15), 8 * (lane2 >> 4));
+            park(Wout1, c01, out, wave2, lane2);
+            unpark(Wout, c0, out, wave2, lane2);
+        }
         __syncthreads();
         store_t(out);
         __syncthreads();
@@ -545,10 +636,18 @@ int launch_bf16x3(int nw, int n, int o, int nspin, int ldv, int n_terms, double 
     const int group_map = (nw % 8 == 0) ? 1 : 0;
     const int nb = NP / 16;
     const long long vplane = SYMV ? (long long)nw * nb * (nb + 1) / 2 * 256 : (long long)nw * n * ldv;
-    if (lds > 64 * 1024) afq::allow_full_lds((const void*)trotter_bf16x3_kernel<NL, SYMV>);
-    hipLaunchKernelGGL((trotter_bf16x3_kernel<NL, SYMV>), dim3((unsigned)(nw * nspin * nct)), dim3(64 * nwave),
-                       lds, st, nw, n, o, nspin, nct, ldv, n_terms, sqrt_dt, E, vhs, vplane, wu, wd, ou, od,
-                       group_map);
+    // PAIR needs a workgroup per walker to fill the CUs; below that the
+    // per-item launch keeps more of them busy
+    if (nspin * nct == 2 && nw >= afq::cu_count()) {
+        if (lds > 64 * 1024) afq::allow_full_lds((const void*)trotter_bf16x3_kernel<NL, SYMV, false, true>);
+        hipLaunchKernelGGL((trotter_bf16x3_kernel<NL, SYMV, false, true>), dim3((unsigned)nw), dim3(64 * nwave),
+                           lds, st, nw, n, o, nspin, nct, ldv, n_terms, sqrt_dt, E, vhs, vplane, wu, wd, ou, od, 0);
+    } else {
+        if (lds > 64 * 1024) afq::allow_full_lds((const void*)trotter_bf16x3_kernel<NL, SYMV>);
+        hipLaunchKernelGGL((trotter_bf16x3_kernel<NL, SYMV>), dim3((unsigned)(nw * nspin * nct)), dim3(64 * nwave),
+                           lds, st, nw, n, o, nspin, nct, ldv, n_terms, sqrt_dt, E, vhs, vplane, wu, wd, ou, od,
+                           group_map);
+    }
     return afq::check_launch(SYMV ? "afq_trotter_bf16x3_sym" : "afq_trotter_bf16x3");
 }
 

@@ -408,6 +408,50 @@ def test_trotter_bf16x3_upper_block_vhs_equals_full_rows(gpu_device, nw, n, o, n
         assert torch.equal(a_, b_)
 
 
+@pytest.mark.parametrize("n,o,nspin", [(200, 32, 1), (180, 16, 2), (120, 20, 1)])
+def test_trotter_bf16x3_walker_pairs_equal_per_item_launch(gpu_device, n, o, nspin):
+    """With two items per walker (nspin * ceil(o / 16) == 2) and at least as
+    many walkers as CUs, one workgroup runs both items of a walker (E W of the
+    second item and the first Taylor sum parked in the output rows as fp64).
+    Every walker's result equals the per-item launch's (the same walkers, fewer
+    of them) bit for bit, in both V layouts, and the fp64 step within 2e-5."""
+    import torch
+    from ad_afqmc_prototype_amd import kernels
+    from ad_afqmc_prototype_amd.prop.chol_afqmc_ops import chol_sym_rows, minus_identity_pairs, split_pairs, upper_blocks
+    cus = torch.cuda.get_device_properties(gpu_device).multi_processor_count
+    nw = cus + 37
+    print(f"walker-pair Trotter nw={nw} n={n} o={o} nspin={nspin}", flush=True)
+    rng = np.random.default_rng(n + o)
+    dt = 0.005
+    e, v, ws = _trotter_inputs(rng, nw, n, o, nspin, dt, 0.3)
+    ldv = kernels.ldv_for(n)
+    ep, vp = _planes(e, v, ldv)
+    ep32 = torch.tensor(ep, device=gpu_device).float()
+    vp32 = torch.tensor(vp, device=gpu_device).float()
+    e_blk = minus_identity_pairs(upper_blocks(torch.tensor(e, device=gpu_device)).float(), n)
+    v_blk = split_pairs(chol_sym_rows(vp32.reshape(2 * nw, n, ldv)[:, :, :n].double()).T.float()).view(2, nw, -1)
+    wd = [torch.tensor(x, device=gpu_device) for x in ws]
+    few = 16
+    for lay in ("rows", "blocks"):
+        vv, ee = (vp32, ep32) if lay == "rows" else (v_blk, e_blk)
+        vf = vv[:, :few].contiguous()
+        walk = wd[0] if nspin == 1 else tuple(wd)
+        walk_few = wd[0][:few].contiguous() if nspin == 1 else tuple(x[:few].contiguous() for x in wd)
+        got = kernels.trotter(walk, vv, ee, dt ** 0.5, 6, arith="bf16x3")
+        ref = kernels.trotter(walk_few, vf, ee, dt ** 0.5, 6, arith="bf16x3")
+        gots = got if nspin == 2 else (got,)
+        refs = ref if nspin == 2 else (ref,)
+        for g_, r_ in zip(gots, refs):
+            assert torch.equal(g_[:few], r_), lay
+    v64 = torch.tensor(vp, device=gpu_device)[:, :few].contiguous()
+    e64 = torch.tensor(ep, device=gpu_device)
+    walk_few = wd[0][:few].contiguous() if nspin == 1 else tuple(x[:few].contiguous() for x in wd)
+    r64 = kernels.trotter(walk_few, v64, e64, dt ** 0.5, 6)
+    for g_, r_ in zip(gots, r64 if nspin == 2 else (r64,)):
+        g_, r_ = to_np(g_[:few]), to_np(r_)
+        assert max(rel(g_[i], r_[i]) for i in range(few)) < 2e-5
+
+
 @pytest.mark.parametrize("nw,naux", [(8, 96), (5, 130), (33, 1000)])
 def test_field_shifts_split_planes_equal_split_of_xbar(gpu_device, nw, naux):
     """afq_field_shifts_split: the same xbar, shift and force-bias terms as
