@@ -33,6 +33,7 @@
 namespace {
 
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef unsigned v4u32 __attribute__((ext_vector_type(4)));
 
 constexpr int RT = 2;  // 16-row tiles per wave
 constexpr int KR = 5;  // 32-wide k-blocks of V held in registers
@@ -176,14 +177,7 @@ __device__ __forceinline__ split8 split_minus_identity(const float (&x)[8], int 
 // prefix sums tblock0 (count systems, restricted walkers, (n, ldv) layouts);
 // its sizes and pointers replace the arguments, and blockIdx.x counts from
 // the system's first workgroup.
-// PAIR (walkers with two items: nspin * ceil(o / 16) == 2, at least as many
-// walkers as CUs): one workgroup per walker runs both items, so V crosses into
-// the CU once per walker and E - I is loaded twice per walker instead of four
-// times (200 orbitals, 32 occupied, 2048 walkers: 0.707 -> 0.612 ms).  Both items' E W come first
-// (the second parked in its output rows), then V and the two Taylor series
-// (the first sum parked in its output rows), then E for both final halves.
-// Parked values are fp32 sums stored as fp64, so reading them back is exact.
-template <int NL, bool SYMV = false, bool GROUPED = false, bool PAIR = false>
+template <int NL, bool SYMV = false, bool GROUPED = false>
 __global__ __launch_bounds__(512, 1) void trotter_bf16x3_kernel(
     int nw, int n, int o, int nspin, int nct, int ldv, int n_terms, double sqrt_dt,
     const float* __restrict__ E, const float* __restrict__ vhs, long long vplane,
@@ -225,10 +219,7 @@ __global__ __launch_bounds__(512, 1) void trotter_bf16x3_kernel(
 
     const int G = nspin * nct;
     int w, sub;
-    if (PAIR) {
-        w = b;
-        sub = 0;
-    } else if (group_map) {
+    if (group_map) {
         w = (b / (8 * G)) * 8 + (b % 8);
         sub = (b / 8) % G;
     } else {
@@ -330,61 +321,18 @@ __global__ __launch_bounds__(512, 1) void trotter_bf16x3_kernel(
             }
     };
 
-    // the second item of a PAIR walker: its columns, output and input
-    const int s1 = 1 / nct;
-    const int c01 = (1 - s1 * nct) * 16;
-    double2* Wout1 = (s1 == 0 ? wout_up : wout_dn) + woff;
-    // out-of-range rows and columns of a parked tile hold zeros (the walker's
-    // padding stays zero through every product)
-    auto park = [&](double2* dst, int cbase, const f32x4 (&X)[RT][2], int wv, int ln) {
-        const int cl = cbase + (ln & 15);
-        if (cl < o) {
-#pragma unroll
-            for (int t = 0; t < RT; ++t)
-#pragma unroll
-                for (int r = 0; r < 4; ++r) {
-                    const int row = (wv * RT + t) * 16 + 4 * (ln >> 4) + r;
-                    if (row < n) dst[row * o + cl] = make_double2((double)X[t][0][r], (double)X[t][1][r]);
-                }
-        }
-    };
-    auto unpark = [&](const double2* src, int cbase, f32x4 (&X)[RT][2], int wv, int ln) {
-        const int cl = cbase + (ln & 15);
-#pragma unroll
-        for (int t = 0; t < RT; ++t)
-#pragma unroll
-            for (int r = 0; r < 4; ++r) {
-                const int row = (wv * RT + t) * 16 + 4 * (ln >> 4) + r;
-                double2 v = make_double2(0.0, 0.0);
-                if (cl < o && row < n) v = src[row * o + cl];
-                X[t][0][r] = (float)v.x;
-                X[t][1][r] = (float)v.y;
-            }
-    };
-
     // ---- prologue: walker columns, accumulator rows and E in flight together ----
     f32x4 acc[RT][2];
     {
         // accumulator element r of tile t: row (wave*RT + t)*16 + 4q + r, column
         // col; the waves' accumulators cover all nk * 32 rows of T
-        double2 wa[RT][4], wb[RT][4];
+        double2 wa[RT][4];
 #pragma unroll
         for (int t = 0; t < RT; ++t)
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
                 wa[t][r] = load_w((wave * RT + t) * 16 + 4 * q + r, col);
             }
-        if constexpr (PAIR) {
-            const rsrc_t Wr1 = make_rsrc((s1 == 0 ? win_up : win_dn) + woff, (unsigned)n * o * 16u);
-            const int col1 = c01 + lr;
-#pragma unroll
-            for (int t = 0; t < RT; ++t)
-#pragma unroll
-                for (int r = 0; r < 4; ++r) {
-                    const int p = (wave * RT + t) * 16 + 4 * q + r;
-                    wb[t][r] = load_d2(Wr1, col1 < o ? (unsigned)(p * o + col1) * 16u : kOutside);
-                }
-        }
         float er[RT][KMAX][8];
         load_e(er, row0, kq);
 #pragma unroll
@@ -398,22 +346,6 @@ __global__ __launch_bounds__(512, 1) void trotter_bf16x3_kernel(
         __syncthreads();
         // ---- T0 = E W ----
         e_product(acc, er, row0, kq);
-        if constexpr (PAIR) {
-            // the second item's E W, parked in its output rows
-            f32x4 acc1[RT][2];
-#pragma unroll
-            for (int t = 0; t < RT; ++t)
-#pragma unroll
-                for (int r = 0; r < 4; ++r) {
-                    acc1[t][0][r] = (float)wb[t][r].x;
-                    acc1[t][1][r] = (float)wb[t][r].y;
-                }
-            __syncthreads();
-            store_t(acc1);
-            __syncthreads();
-            e_product(acc1, er, row0, kq);
-            park(Wout1, c01, acc1, wave, lane);
-        }
     }
     f32x4 out[RT][2];
 #pragma unroll
@@ -526,7 +458,6 @@ __global__ __launch_bounds__(512, 1) void trotter_bf16x3_kernel(
     };
 
     // ---- Taylor terms: T_k = (i sqrt(dt) / k) V T_{k-1}, out += T_k ----
-    auto taylor = [&]() {
 #pragma unroll 1
     for (int k = 1; k <= n_terms; ++k) {
         f32x4 P[RT][2];
@@ -571,19 +502,6 @@ __global__ __launch_bounds__(512, 1) void trotter_bf16x3_kernel(
         store_t(P);
         __syncthreads();
     }
-    };
-    taylor();
-    if constexpr (PAIR) {
-        // first sum parked; the second item's T0 back as its sum's start
-        const int lane1 = opaque(lane);
-        const int wave1 = opaque(wave);
-        park(Wout, c0, out, wave1, lane1);
-        unpark(Wout1, c01, out, wave1, lane1);
-        __syncthreads();
-        store_t(out);
-        __syncthreads();
-        taylor();
-    }
 
     // ---- W_out = E out = out + (E - I) out ----
     // (indices recomputed from an opaque lane id: nothing address-like stays
@@ -593,14 +511,6 @@ __global__ __launch_bounds__(512, 1) void trotter_bf16x3_kernel(
     {
         float er[RT][KMAX][8];
         load_e(er, wave2 * RT * 16 + (lane2 & 15), 8 * (lane2 >> 4));
-        if constexpr (PAIR) {
-            __syncthreads();
-            store_t(out);
-            __syncthreads();
-            e_product(out, er, wave2 * RT * 16 + (lane2 & 15), 8 * (lane2 >> 4));
-            park(Wout1, c01, out, wave2, lane2);
-            unpark(Wout, c0, out, wave2, lane2);
-        }
         __syncthreads();
         store_t(out);
         __syncthreads();
@@ -615,6 +525,407 @@ __global__ __launch_bounds__(512, 1) void trotter_bf16x3_kernel(
                 const int row = (wave2 * RT + t) * 16 + 4 * (lane2 >> 4) + r;
                 if (row < n) Wout[row * o + col2] = make_double2((double)out[t][0][r], (double)out[t][1][r]);
             }
+    }
+}
+
+// V and E - I as upper blocks, walkers with two items (nspin * ceil(o / 16) ==
+// 2) and at least as many walkers as CUs: one persistent workgroup per CU loops over walkers
+// w = blockIdx.x, + gridDim.x, ...; per walker
+//   E W of both items (the second parked in its output rows) -> V into the
+//   CU once -> the two Taylor series (the first sum parked in its output rows)
+//   -> E - I -> both final one-body halves, while the next walker's columns
+//   load;
+// E - I stays in registers from one walker's final halves into the next
+// walker's first ones, so it is loaded once per walker (the per-item launch
+// loads it twice per item).  Parked values are fp32 sums stored as fp64, so
+// reading them back is exact, and every product runs in the per-item kernel's
+// order: the results equal trotter_bf16x3_kernel's bit for bit.
+template <int NL>
+__global__ __launch_bounds__(512, 1) void trotter_bf16x3_pair_kernel(
+    int nw, int n, int o, int nspin, int nct, int n_terms, double sqrt_dt, const float* __restrict__ E,
+    const float* __restrict__ vhs, long long vplane, const double2* __restrict__ win_up,
+    const double2* __restrict__ win_dn, double2* __restrict__ wout_up, double2* __restrict__ wout_dn) {
+    constexpr int KMAX = KR + NL;
+    constexpr bool SYMV = true;
+    const int ldv = 0;  // V and E - I are upper blocks
+    extern __shared__ __attribute__((aligned(16))) unsigned char lds_raw[];
+    const int NP = afq::round_up(n, 16);
+    const int nk = afq::ceil_div(NP, 32);
+    const int TP = nk * 32 + 8;
+    __bf16* Tt = reinterpret_cast<__bf16*>(lds_raw);  // [4 planes][16 cols][TP]
+    uint4* VL = reinterpret_cast<uint4*>(lds_raw + (size_t)4 * 16 * TP * sizeof(__bf16));
+
+    // the two items: spin s_i, first column c_i
+    const int s1 = 1 / nct;
+    const int c1 = (1 - s1 * nct) * 16;
+    const unsigned mat_bytes = (unsigned)n * ldv * 4u;
+    const int nbv = NP / 16;
+    const long long vrow = SYMV ? (long long)nbv * (nbv + 1) / 2 * 256 : (long long)n * ldv;
+    const rsrc_t Er = make_rsrc(E, SYMV ? (unsigned)vrow * 4u : mat_bytes);
+
+    const int tid = threadIdx.x;
+    const int lane = tid & 63;
+    const int wave = tid >> 6;
+    const int lr = lane & 15;
+    const int q = lane >> 4;
+    const int kq = 8 * q;
+    const int row0 = wave * RT * 16 + lr;
+
+    auto read_b = [&](int kb, int plane) {
+        return *reinterpret_cast<const bf16x8*>(Tt + (plane * 16 + lr) * TP + kb * 32 + kq);
+    };
+    auto load_e = [&](float (&er)[RT][KMAX][8], int r0, int k8) {
+#pragma unroll
+        for (int t = 0; t < RT; ++t)
+#pragma unroll
+            for (int kb = 0; kb < KMAX; ++kb) {
+                if constexpr (SYMV)
+                    load_frag_sym(Er, nbv, (r0 >> 4) + t, kb, r0 & 15, k8 >> 3, er[t][kb]);
+                else
+                    load_row8(Er, ldv, n, r0 + 16 * t, kb * 32 + k8, er[t][kb]);
+            }
+    };
+    // acc += (E - I) . T
+    auto e_product = [&](f32x4 (&acc)[RT][2], const float (&er)[RT][KMAX][8], int r0, int k8) {
+#pragma unroll
+        for (int kb = 0; kb < KMAX; ++kb) {
+            if (kb < nk) {
+                split8 ef[RT];
+#pragma unroll
+                for (int t = 0; t < RT; ++t) {
+                    if constexpr (SYMV)
+                        ef[t] = unpack_pairs(er[t][kb]);
+                    else
+                        ef[t] = split_minus_identity(er[t][kb], r0 + 16 * t, n, kb * 32 + k8);
+                }
+                const bf16x8 brh = read_b(kb, 0), brl = read_b(kb, 1);
+                const bf16x8 bih = read_b(kb, 2), bil = read_b(kb, 3);
+#pragma unroll
+                for (int t = 0; t < RT; ++t) {
+                    acc[t][0] = mfma_bf16(ef[t].lo, brh, acc[t][0]);
+                    acc[t][1] = mfma_bf16(ef[t].lo, bih, acc[t][1]);
+                }
+#pragma unroll
+                for (int t = 0; t < RT; ++t) {
+                    acc[t][0] = mfma_bf16(ef[t].hi, brl, acc[t][0]);
+                    acc[t][1] = mfma_bf16(ef[t].hi, bil, acc[t][1]);
+                }
+#pragma unroll
+                for (int t = 0; t < RT; ++t) {
+                    acc[t][0] = mfma_bf16(ef[t].hi, brh, acc[t][0]);
+                    acc[t][1] = mfma_bf16(ef[t].hi, bih, acc[t][1]);
+                }
+            }
+        }
+    };
+    // T <- split(X): lane writes rows 4q..4q+3 of its tiles in column lr
+    auto store_t = [&](const f32x4 (&X)[RT][2]) {
+#pragma unroll
+        for (int t = 0; t < RT; ++t)
+#pragma unroll
+            for (int c = 0; c < 2; ++c) {
+                bf16x8 hl;  // [0..3] hi, [4..7] lo
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const __bf16 h = (__bf16)X[t][c][r];
+                    hl[r] = h;
+                    hl[4 + r] = (__bf16)(X[t][c][r] - (float)h);
+                }
+                const uint4 u = __builtin_bit_cast(uint4, hl);
+                const int k = (wave * RT + t) * 16 + 4 * q;
+                *reinterpret_cast<uint2*>(Tt + ((2 * c + 0) * 16 + lr) * TP + k) = make_uint2(u.x, u.y);
+                *reinterpret_cast<uint2*>(Tt + ((2 * c + 1) * 16 + lr) * TP + k) = make_uint2(u.z, u.w);
+            }
+    };
+    // the walker's columns of item i (zero outside n x o), as accumulator rows
+    auto load_cols = [&](double2 (&wx)[RT][4], int wv, int i) {
+        const int ln = opaque(lane);
+        const int wvo = opaque(wave);
+        const int si = i == 0 ? 0 : s1;
+        const int cl = (i == 0 ? 0 : c1) + (ln & 15);
+        const bool ok = wv < nw;
+        const double2* base = (si == 0 ? win_up : win_dn) + (ok ? (long long)wv * n * o : 0LL);
+        const rsrc_t Wr = make_rsrc(base, ok ? (unsigned)n * o * 16u : 0u);
+#pragma unroll
+        for (int t = 0; t < RT; ++t)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int p = (wvo * RT + t) * 16 + 4 * (ln >> 4) + r;
+                wx[t][r] = load_d2(Wr, cl < o ? (unsigned)(p * o + cl) * 16u : kOutside);
+            }
+    };
+    auto to_acc = [&](f32x4 (&X)[RT][2], const double2 (&wx)[RT][4]) {
+#pragma unroll
+        for (int t = 0; t < RT; ++t)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                X[t][0][r] = (float)wx[t][r].x;
+                X[t][1][r] = (float)wx[t][r].y;
+            }
+    };
+    // output rows of item i of walker wv: written (rows < n, columns < o) and
+    // read back (zero elsewhere: the walker's padding stays zero throughout),
+    // through buffer offsets (out of range: a dropped store, a zero load), so
+    // no store waits on a branch or a reloaded address
+    auto out_rsrc = [&](int wv, int i) {
+        const double2* p = ((i == 0 ? 0 : s1) == 0 ? wout_up : wout_dn) + (long long)wv * n * o;
+        return make_rsrc(p, (unsigned)n * o * 16u);
+    };
+    auto out_off = [&](int i, int t, int r, int wv, int ln) {
+        const int cl = (i == 0 ? 0 : c1) + (ln & 15);
+        const int row = (wv * RT + t) * 16 + 4 * (ln >> 4) + r;
+        return (cl < o && row < n) ? (unsigned)(row * o + cl) * 16u : kOutside;
+    };
+    auto park = [&](const rsrc_t& dst, int i, const f32x4 (&X)[RT][2], int wv, int ln) {
+#pragma unroll
+        for (int t = 0; t < RT; ++t)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const double2 v = make_double2((double)X[t][0][r], (double)X[t][1][r]);
+                __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(v4u32, v), dst, out_off(i, t, r, wv, ln), 0, 0);
+            }
+    };
+    auto unpark = [&](const rsrc_t& src, int i, f32x4 (&X)[RT][2], int wv, int ln) {
+#pragma unroll
+        for (int t = 0; t < RT; ++t)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const double2 v = load_d2(src, out_off(i, t, r, wv, ln));
+                X[t][0][r] = (float)v.x;
+                X[t][1][r] = (float)v.y;
+            }
+    };
+    // P += V . T for one k-block, A fragments (ar, ai) per row tile
+    auto vt_block = [&](f32x4 (&P)[RT][2], int kb, const split8 (&ar)[RT], const split8 (&ai)[RT]) {
+        const bf16x8 brh = read_b(kb, 0), brl = read_b(kb, 1);
+        bf16x8 bih = read_b(kb, 2), bil = read_b(kb, 3);
+#pragma unroll
+        for (int t = 0; t < RT; ++t) {
+            P[t][0] = mfma_bf16(ar[t].lo, brh, P[t][0]);
+            P[t][1] = mfma_bf16(ar[t].lo, bih, P[t][1]);
+        }
+#pragma unroll
+        for (int t = 0; t < RT; ++t) {
+            P[t][0] = mfma_bf16(ar[t].hi, brl, P[t][0]);
+            P[t][1] = mfma_bf16(ar[t].hi, bil, P[t][1]);
+        }
+#pragma unroll
+        for (int t = 0; t < RT; ++t) {
+            P[t][0] = mfma_bf16(ar[t].hi, brh, P[t][0]);
+            P[t][1] = mfma_bf16(ar[t].hi, bih, P[t][1]);
+        }
+        bih = neg8(bih);
+        bil = neg8(bil);
+#pragma unroll
+        for (int t = 0; t < RT; ++t) {
+            P[t][1] = mfma_bf16(ai[t].lo, brh, P[t][1]);
+            P[t][0] = mfma_bf16(ai[t].lo, bih, P[t][0]);
+        }
+#pragma unroll
+        for (int t = 0; t < RT; ++t) {
+            P[t][1] = mfma_bf16(ai[t].hi, brl, P[t][1]);
+            P[t][0] = mfma_bf16(ai[t].hi, bil, P[t][0]);
+        }
+#pragma unroll
+        for (int t = 0; t < RT; ++t) {
+            P[t][1] = mfma_bf16(ai[t].hi, brh, P[t][1]);
+            P[t][0] = mfma_bf16(ai[t].hi, bih, P[t][0]);
+        }
+    };
+
+    float er[RT][KMAX][8];
+    double2 wa[RT][4], wb[RT][4];
+    int w = blockIdx.x;
+    load_cols(wa, w, 0);
+    load_cols(wb, w, 1);
+    load_e(er, row0, kq);
+#pragma unroll 1
+    for (; w < nw; w += gridDim.x) {
+        const int wn = w + gridDim.x;
+        const rsrc_t out0 = out_rsrc(w, 0);
+        const rsrc_t out1 = out_rsrc(w, 1);
+        f32x4 out[RT][2];
+        // ---- E W of both items ----
+        {
+            f32x4 acc[RT][2];
+            to_acc(acc, wa);
+            __syncthreads();  // the previous walker's reads of T are done
+            store_t(acc);
+            __syncthreads();
+            e_product(acc, er, row0, kq);
+            f32x4 acc1[RT][2];
+            to_acc(acc1, wb);
+            __syncthreads();
+            store_t(acc1);
+            __syncthreads();
+            e_product(acc1, er, row0, kq);
+            park(out1, 1, acc1, wave, lane);
+#pragma unroll
+            for (int t = 0; t < RT; ++t) {
+                out[t][0] = acc[t][0];
+                out[t][1] = acc[t][1];
+            }
+            __syncthreads();
+            store_t(acc);
+        }
+        // ---- V resident: the LDS k-blocks first, then the register k-blocks
+        // (lane-derived offsets recomputed per walker from an opaque lane id,
+        // so none is kept, and spilled, across the walker loop) ----
+        const int lnv = opaque(lane);
+        const int wvv = opaque(wave);
+        const int lrv = lnv & 15, qv = lnv >> 4, kqv = 8 * qv, row0v = wvv * RT * 16 + lrv;
+        const rsrc_t Vr = make_rsrc(vhs + (long long)w * vrow, (unsigned)vrow * 4u);
+        const rsrc_t Vi = make_rsrc(vhs + vplane + (long long)w * vrow, (unsigned)vrow * 4u);
+        if constexpr (NL > 0) {
+            float xl[RT][NL][2][8];
+#pragma unroll
+            for (int t = 0; t < RT; ++t)
+#pragma unroll
+                for (int l = 0; l < NL; ++l) {
+                    const int k0 = (KR + l) * 32 + kqv;
+                    if constexpr (SYMV) {
+                        load_frag_sym(Vr, nbv, wvv * RT + t, KR + l, lrv, qv, xl[t][l][0]);
+                        load_frag_sym(Vi, nbv, wvv * RT + t, KR + l, lrv, qv, xl[t][l][1]);
+                    } else {
+                        load_row8(Vr, ldv, n, row0v + 16 * t, k0, xl[t][l][0]);
+                        load_row8(Vi, ldv, n, row0v + 16 * t, k0, xl[t][l][1]);
+                    }
+                }
+#pragma unroll
+            for (int t = 0; t < RT; ++t)
+#pragma unroll
+                for (int l = 0; l < NL; ++l) {
+                    uint4* dst = VL + ((size_t)((wvv * RT + t) * NL + l) * 4) * 64 + lnv;
+                    const split8 a = SYMV ? unpack_pairs(xl[t][l][0]) : split_f32x8(xl[t][l][0]);
+                    const split8 c = SYMV ? unpack_pairs(xl[t][l][1]) : split_f32x8(xl[t][l][1]);
+                    dst[0 * 64] = __builtin_bit_cast(uint4, a.hi);
+                    dst[1 * 64] = __builtin_bit_cast(uint4, a.lo);
+                    dst[2 * 64] = __builtin_bit_cast(uint4, c.hi);
+                    dst[3 * 64] = __builtin_bit_cast(uint4, c.lo);
+                }
+        }
+        split8 vr[RT][KR], vi[RT][KR];
+        {
+            float xv[RT][KR][2][8];
+#pragma unroll
+            for (int t = 0; t < RT; ++t)
+#pragma unroll
+                for (int kb = 0; kb < KR; ++kb) {
+                    const int k0 = kb * 32 + kqv;
+                    if constexpr (SYMV) {
+                        load_frag_sym(Vr, nbv, wvv * RT + t, kb, lrv, qv, xv[t][kb][0]);
+                        load_frag_sym(Vi, nbv, wvv * RT + t, kb, lrv, qv, xv[t][kb][1]);
+                    } else {
+                        load_row8(Vr, ldv, n, row0v + 16 * t, k0, xv[t][kb][0]);
+                        load_row8(Vi, ldv, n, row0v + 16 * t, k0, xv[t][kb][1]);
+                    }
+                }
+#pragma unroll
+            for (int t = 0; t < RT; ++t)
+#pragma unroll
+                for (int kb = 0; kb < KR; ++kb) {
+                    if constexpr (SYMV) {
+                        vr[t][kb] = unpack_pairs(xv[t][kb][0]);
+                        vi[t][kb] = unpack_pairs(xv[t][kb][1]);
+                    } else {
+                        vr[t][kb] = split_f32x8(xv[t][kb][0]);
+                        vi[t][kb] = split_f32x8(xv[t][kb][1]);
+                    }
+                }
+        }
+        __syncthreads();
+
+        // ---- Taylor terms: T_k = (i sqrt(dt) / k) V T_{k-1}, out += T_k ----
+        auto taylor = [&]() {
+#pragma unroll 1
+            for (int k = 1; k <= n_terms; ++k) {
+                f32x4 P[RT][2];
+#pragma unroll
+                for (int t = 0; t < RT; ++t) P[t][0] = P[t][1] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+                for (int kb = 0; kb < KR; ++kb) {
+                    if (kb < nk) {
+                        split8 ar[RT], ai[RT];
+#pragma unroll
+                        for (int t = 0; t < RT; ++t) {
+                            ar[t] = vr[t][kb];
+                            ai[t] = vi[t][kb];
+                        }
+                        vt_block(P, kb, ar, ai);
+                    }
+                }
+#pragma unroll
+                for (int l = 0; l < NL; ++l) {
+                    split8 ar[RT], ai[RT];
+#pragma unroll
+                    for (int t = 0; t < RT; ++t) {
+                        const uint4* src = VL + ((size_t)((wave * RT + t) * NL + l) * 4) * 64 + lane;
+                        ar[t].hi = __builtin_bit_cast(bf16x8, src[0 * 64]);
+                        ar[t].lo = __builtin_bit_cast(bf16x8, src[1 * 64]);
+                        ai[t].hi = __builtin_bit_cast(bf16x8, src[2 * 64]);
+                        ai[t].lo = __builtin_bit_cast(bf16x8, src[3 * 64]);
+                    }
+                    vt_block(P, KR + l, ar, ai);
+                }
+                const float ck = (float)(sqrt_dt / (double)k);
+#pragma unroll
+                for (int t = 0; t < RT; ++t) {
+                    const f32x4 re = -ck * P[t][1];
+                    const f32x4 im = ck * P[t][0];
+                    P[t][0] = re;
+                    P[t][1] = im;
+                    out[t][0] += re;
+                    out[t][1] += im;
+                }
+                __syncthreads();
+                store_t(P);
+                __syncthreads();
+            }
+        };
+        taylor();
+        {
+            // first sum parked; the second item's E W back as its sum's start
+            const int lane1 = opaque(lane);
+            const int wave1 = opaque(wave);
+            park(out0, 0, out, wave1, lane1);
+            unpark(out1, 1, out, wave1, lane1);
+            __syncthreads();
+            store_t(out);
+            __syncthreads();
+        }
+        taylor();
+
+        // ---- W_out = E out = out + (E - I) out for both items; the next
+        // walker's columns in flight (indices recomputed from an opaque lane id:
+        // nothing address-like stays live across the Taylor loops) ----
+        const int lane2 = opaque(lane);
+        const int wave2 = opaque(wave);
+        const int r02 = wave2 * RT * 16 + (lane2 & 15);
+        const int k82 = 8 * (lane2 >> 4);
+        load_e(er, r02, k82);
+        load_cols(wa, wn, 0);
+        load_cols(wb, wn, 1);
+        __syncthreads();
+        store_t(out);
+        __syncthreads();
+        e_product(out, er, r02, k82);
+        {
+            // (the load before the store: a load issued after a store waits
+            // for it, the counters being in order)
+            f32x4 nx[RT][2];
+            unpark(out0, 0, nx, wave2, lane2);
+            park(out1, 1, out, wave2, lane2);
+#pragma unroll
+            for (int t = 0; t < RT; ++t) {
+                out[t][0] = nx[t][0];
+                out[t][1] = nx[t][1];
+            }
+        }
+        __syncthreads();
+        store_t(out);
+        __syncthreads();
+        e_product(out, er, r02, k82);
+        park(out0, 0, out, wave2, lane2);
     }
 }
 
@@ -636,13 +947,18 @@ int launch_bf16x3(int nw, int n, int o, int nspin, int ldv, int n_terms, double 
     const int group_map = (nw % 8 == 0) ? 1 : 0;
     const int nb = NP / 16;
     const long long vplane = SYMV ? (long long)nw * nb * (nb + 1) / 2 * 256 : (long long)nw * n * ldv;
-    // PAIR needs a workgroup per walker to fill the CUs; below that the
-    // per-item launch keeps more of them busy
-    if (nspin * nct == 2 && nw >= afq::cu_count()) {
-        if (lds > 64 * 1024) afq::allow_full_lds((const void*)trotter_bf16x3_kernel<NL, SYMV, false, true>);
-        hipLaunchKernelGGL((trotter_bf16x3_kernel<NL, SYMV, false, true>), dim3((unsigned)nw), dim3(64 * nwave),
-                           lds, st, nw, n, o, nspin, nct, ldv, n_terms, sqrt_dt, E, vhs, vplane, wu, wd, ou, od, 0);
-    } else {
+    // upper-block V, two items per walker and a walker for every CU: the
+    // persistent pair kernel
+    bool pair = false;
+    if constexpr (SYMV) {
+        if (nspin * nct == 2 && nw >= afq::cu_count()) {
+            pair = true;
+            if (lds > 64 * 1024) afq::allow_full_lds((const void*)trotter_bf16x3_pair_kernel<NL>);
+            hipLaunchKernelGGL((trotter_bf16x3_pair_kernel<NL>), dim3((unsigned)afq::cu_count()), dim3(64 * nwave),
+                               lds, st, nw, n, o, nspin, nct, n_terms, sqrt_dt, E, vhs, vplane, wu, wd, ou, od);
+        }
+    }
+    if (!pair) {
         if (lds > 64 * 1024) afq::allow_full_lds((const void*)trotter_bf16x3_kernel<NL, SYMV>);
         hipLaunchKernelGGL((trotter_bf16x3_kernel<NL, SYMV>), dim3((unsigned)(nw * nspin * nct)), dim3(64 * nwave),
                            lds, st, nw, n, o, nspin, nct, ldv, n_terms, sqrt_dt, E, vhs, vplane, wu, wd, ou, od,

@@ -410,11 +410,12 @@ def test_trotter_bf16x3_upper_block_vhs_equals_full_rows(gpu_device, nw, n, o, n
 
 @pytest.mark.parametrize("n,o,nspin", [(200, 32, 1), (180, 16, 2), (120, 20, 1)])
 def test_trotter_bf16x3_walker_pairs_equal_per_item_launch(gpu_device, n, o, nspin):
-    """With two items per walker (nspin * ceil(o / 16) == 2) and at least as
-    many walkers as CUs, one workgroup runs both items of a walker (E W of the
-    second item and the first Taylor sum parked in the output rows as fp64).
-    Every walker's result equals the per-item launch's (the same walkers, fewer
-    of them) bit for bit, in both V layouts, and the fp64 step within 2e-5."""
+    """With upper-block V, two items per walker (nspin * ceil(o / 16) == 2) and
+    at least as many walkers as CUs, persistent workgroups run both items of a
+    walker (E W of the second item and the first Taylor sum parked in the output
+    rows as fp64, E - I kept from one walker to the next).  Every walker's result
+    equals the per-item launch's (the same walkers, fewer of them) bit for bit,
+    in both V layouts, and the fp64 step within 2e-5."""
     import torch
     from ad_afqmc_prototype_amd import kernels
     from ad_afqmc_prototype_amd.prop.chol_afqmc_ops import chol_sym_rows, minus_identity_pairs, split_pairs, upper_blocks
