@@ -105,12 +105,18 @@ def setup_tensors(h1: torch.Tensor, chol: torch.Tensor, trial: torch.Tensor, h0:
     """(exp(-dt/2 h1_eff), m, h0_prop, rot, rot_h1) with mf = i m, as
     differentiable torch functions of h1 and chol (prop/chol_afqmc_ops.py:59-103,
     meas/rhf.py:141-150, RHF trial: the total rdm1 is 2 C C^T)."""
+    # (each contraction as one GEMM over the flattened Cholesky index, the same
+    # general formula as the einsums of the reference, so autograd's adjoints
+    # of chol are the reference's)
+    g, n, _ = chol.shape
     p = trial @ trial.T
-    m = torch.einsum("gij,ji->g", chol, 2.0 * p)
+    m = chol.reshape(g, n * n) @ (2.0 * p).T.reshape(n * n)        # sum_ij L_g[i,j] (2P)[j,i]
     h0_prop = -h0 + 0.5 * torch.sum(m * m)
-    h1_eff = h1 - 0.5 * torch.einsum("gik,gkj->ij", chol, chol) + torch.einsum("g,gij->ij", m, chol)
+    rows = chol.permute(1, 0, 2).reshape(n, g * n)                  # [i, (g, k)] = L_g[i, k]
+    ll = rows @ chol.reshape(g * n, n)                               # sum_g L_g L_g
+    h1_eff = h1 - 0.5 * ll + (m @ chol.reshape(g, n * n)).view(n, n)
     e_half = torch.linalg.matrix_exp(-0.5 * dt * h1_eff)
-    rot = torch.einsum("pi,gpj->gij", trial, chol)
+    rot = (trial.T @ rows).view(trial.shape[1], g, n).permute(1, 0, 2)  # C^T L_g
     rot_h1 = trial.T @ h1
     return e_half, m, h0_prop, rot, rot_h1
 
@@ -148,6 +154,15 @@ class _layouts:
         self.rot_h1 = rot_h1.contiguous()
         self.rot_h1_k = self.rot_h1.reshape(1, o, n)
         self.trial = run.trial.contiguous()
+        self._y = None
+
+    def energy_adj_y(self):
+        """Y[(b,q),(a,p)] = sum_g R[g,a,q] R[g,b,p] of the energy adjoint, built
+        on first use."""
+        if self._y is None:
+            r = self.rot.reshape(self.g, self.o * self.n)
+            self._y = _regroup(r.T @ r, self.o, self.n).contiguous()
+        return self._y
 
 
 def _dgemm(A, B, C, alpha: float = 1.0, beta: float = 0.0):
@@ -280,52 +295,44 @@ def _qr_adj(a, q, q_bar):
     return _ct(torch.linalg.solve_triangular(r, _ct(x), upper=True))
 
 
-def _energy_adj(g_c, e_bar, lay: _layouts, acc, max_bytes: int = 1 << 30):
+def _energy_adj(g_c, e_bar, lay: _layouts, acc):
     """G_bar of the local energies (meas/rhf.py:36-50) for real adjoints e_bar of
     Re E; accumulates rot_bar and rot_h1_bar.  The half-rotated integrals R are
-    real, so every product runs on the real and imaginary planes of G, as plain
-    library GEMMs over all walkers of a chunk at once:
-      F[(g,a),(w,b)] = sum_p R[g,a,p] G[w,b,p]             (f_g of walker w)
-      G_bar_w = e_bar_w conj(2 rot_h1 + 4 sum_g c_g R_g - 2 sum_{g,b} F[(g,a),(w,b)] R[g,b])
-      rot_bar += Re sum_w e_bar_w (4 c_g G_w - 2 f_g^T G_w)
-    with c_g = tr f_g; F is regrouped once per use ((w,a),(g,b) and the
-    transposed blocks (g,a),(w,b)) so that each sum is one GEMM."""
+    real, so every product runs on the real and imaginary planes of G.  With
+    f_g = R_g G_w^T (o x o) and c_g = tr f_g:
+      G_bar_w = e_bar_w conj(2 rot_h1 + 4 sum_g c_g R_g - 2 sum_g f_g R_g)
+      rot_bar_g += Re sum_w e_bar_w (4 c_g G_w - 2 f_g^T G_w)
+    The sums over g never form f_g: sum_g f_g R_g = G_w Y with the run's fixed
+    Y[(b,q),(a,p)] = sum_g R[g,a,q] R[g,b,p] (the rows of R^T R regrouped), and
+    sum_w e_w f_g^T G_w = R_g Z with Z[(b,q),(a,p)] = sum_w e_w G_w[a,q] G_w[b,p]
+    (the rows of (e G)^T G regrouped), so each term is one GEMM over all
+    walkers on (o n)-wide operands."""
     nw, o, n = g_c.shape
     g = lay.g
-    r = lay.rot.reshape(g * o, n)                      # rows (g, a)
-    out = torch.empty_like(g_c)
-    # walkers per chunk: F and its two regroupings, real and imaginary planes
-    chunk = max(1, min(nw, max_bytes // (6 * 8 * g * o * o)))
-    for s0 in range(0, nw, chunk):
-        gch = g_c[s0:s0 + chunk]
-        W = gch.shape[0]
-        eb = e_bar[s0:s0 + chunk]
-        gp = (gch.real.reshape(W * o, n), gch.imag.reshape(W * o, n))
-        de = []
-        rot_acc = torch.zeros((g * o, n), dtype=f64, device=g_c.device)
-        c_planes = []
-        for k, gk in enumerate(gp):
-            f = r @ gk.T                                   # (g*o, W*o): [g][a][w][b]
-            f4 = f.view(g, o, W, o)
-            c = torch.diagonal(f4, dim1=1, dim2=3).sum(-1)  # (g, W)
-            c_planes.append(c)
-            # de_w[a][p] = sum_{g,b} f[g][a][w][b] R[g][b][p]: rows (w, a), K (g, b)
-            fw = f4.permute(2, 1, 0, 3).reshape(W * o, g * o)
-            d = -2.0 * (fw @ r).view(W, o, n)
-            d += 4.0 * (c.T @ lay.rot.reshape(g, o * n)).view(W, o, n)
-            de.append(d)
-            # rot_bar[g][a][p] += sign * sum_{w,b} f[g][b][w][a] eb_w G_k[w][b][p]
-            sign = 1.0 if k == 0 else -1.0
-            fg = f4.permute(0, 3, 2, 1).reshape(g * o, W * o)   # [g][a][w][b] <- f[g][b][w][a]
-            ge = (gk.view(W, o, n) * eb[:, None, None]).reshape(W * o, n)
-            rot_acc.add_(fg @ ge, alpha=-2.0 * sign)
-            rot_acc.view(g, o * n).add_((c * eb[None, :]) @ gk.view(W, o * n), alpha=4.0 * sign)
-            del f, fw, fg
-        de_r = de[0] + 2.0 * lay.rot_h1
-        out[s0:s0 + chunk] = torch.complex(eb[:, None, None] * de_r, -eb[:, None, None] * de[1])
-        acc["rot_h1"] += 2.0 * torch.einsum("w,wap->ap", eb, gch.real)
-        acc["rot"] += rot_acc.view(g, o, n)
+    on = o * n
+    r = lay.rot.reshape(g, on)                          # rows g, columns (a, p)
+    y = lay.energy_adj_y()
+    de = []
+    rot_acc = torch.zeros((g, on), dtype=f64, device=g_c.device)
+    for k, gk in enumerate((g_c.real.reshape(nw, on), g_c.imag.reshape(nw, on))):
+        sign = 1.0 if k == 0 else -1.0
+        c = r @ gk.T                                    # (g, nw): c_g of each walker
+        d = (gk @ y).mul_(-2.0)
+        d.addmm_(c.T, r, alpha=4.0)
+        de.append(d.view(nw, o, n))
+        ge = gk * e_bar[:, None]
+        z = _regroup(ge.T @ gk, o, n)                   # Z[(b,q),(a,p)]
+        rot_acc.addmm_(r, z, alpha=-2.0 * sign)
+        rot_acc.addmm_(c * e_bar[None, :], gk, alpha=4.0 * sign)
+    out = torch.complex(e_bar[:, None, None] * (de[0] + 2.0 * lay.rot_h1), -e_bar[:, None, None] * de[1])
+    acc["rot_h1"] += 2.0 * (e_bar @ g_c.real.reshape(nw, on)).view(o, n)
+    acc["rot"] += rot_acc.view(g, o, n)
     return out
+
+
+def _regroup(x, o: int, n: int):
+    """X[(a,q),(b,p)] -> X[(b,q),(a,p)] for an (o n) x (o n) matrix."""
+    return x.view(o, n, o, n).permute(2, 1, 0, 3).reshape(o * n, o * n)
 
 
 def _trotter_adj(lay: _layouts, run: adjoint_run, w_in, vhs_planes, w_bar_out, acc):
