@@ -15,11 +15,12 @@ Forward: the run's blocks (prop/blocks.py:22-95) on the library's HIP kernels
 (force bias, field shifts, VHS, Trotter, Green's functions / log-overlaps,
 phaseless update, QR, local energy, block energy, comb), restricted walkers and
 an RHF trial, the initial energy estimate included (prop/afqmc.py:17-71).  The
-state at the start of every block is kept as a checkpoint; nothing else.
+state at the start of every block is kept as a checkpoint, and the last block's
+intermediates (the segment the backward pass starts with); nothing else.
 
 Backward: block by block in reverse, the block is recomputed from its checkpoint
-(one block = n_prop_steps steps is the checkpoint interval) and its adjoint
-applied step by step in reverse.  Discrete decisions (comb indices, the floor /
+(one block = n_prop_steps steps is the checkpoint interval; the last block is
+not recomputed) and its adjoint applied step by step in reverse.  Discrete decisions (comb indices, the floor /
 cap cuts of the importance function, the energy clip) are held at the values the
 forward run took, i.e. the derivative of the branch taken.  Complex adjoints
 follow z_bar = dJ/dRe z + i dJ/dIm z.  Per step, in reverse
@@ -39,7 +40,8 @@ follow z_bar = dJ/dRe z + i dJ/dIm z.  Per step, in reverse
 
 and per block end (prop/blocks.py:53-92): comb (scatter-add of the gathered
 walkers, d|w|), block energy and shift EMA, energy clip, local energy
-(meas/rhf.py:36-50: dE/dG = 2 rot_h1 + 4 sum_g c_g rot_g - 2 sum_g f_g rot_g),
+(meas/rhf.py:36-50: dE/dG = 2 rot_h1 + 4 sum_g c_g rot_g - 2 sum_g f_g rot_g,
+the last sum as G Y with the fixed Y[(b,q),(a,p)] = sum_g R[g,a,q] R[g,b,p]),
 and the QR with diag(R) real (walkers.py:169-211):
 A_bar = [(I - Q Q^H) Q_bar + Q N] R^-H, K = Q^H Q_bar,
 N = tril(K - K^H, -1) + i Im diag(K).
@@ -248,17 +250,22 @@ def _forward(lay: _layouts, run: adjoint_run, keep_checkpoints: bool):
     st, e_est = _initial_state(lay, run)
     e_est_f = float(e_est)
     chk, eb, wb, idx_all = [], [], [], []
+    last = None
     for b in range(run.n_blocks):
         if keep_checkpoints:
             chk.append(st)
-        st, e_b, w_b, idx, _, _ = _block(lay, run, st, b, e_est_f, record=False)
+        # the last segment's intermediates are kept, not recomputed
+        rec = keep_checkpoints and b == run.n_blocks - 1
+        st, e_b, w_b, idx, steps, end = _block(lay, run, st, b, e_est_f, record=rec)
+        if rec:
+            last = (steps, end)
         eb.append(e_b)
         wb.append(w_b)
         idx_all.append(idx)
     eb = torch.stack(eb)
     wb = torch.stack(wb)
     j = torch.sum(wb * eb) / torch.sum(wb)
-    return dict(j=j, eb=eb, wb=wb, idx=idx_all, chk=chk, e_est=e_est_f)
+    return dict(j=j, eb=eb, wb=wb, idx=idx_all, chk=chk, e_est=e_est_f, last=last)
 
 
 # --------------------------------------------------------------------------
@@ -384,7 +391,8 @@ def _step_adj(lay: _layouts, run: adjoint_run, tp, adj, e_est: float, acc):
     acc["e_est"] += pop_bar
     lo_bar_in = -amix
     ld_bar_new = 2.0 * (lo_bar + amix)
-    w_bar = _greens_adj(tp["w_new"], lay.trial, ld_bar=ld_bar_new, out=w_bar.contiguous().clone())
+    # (w_bar is this pass's own tensor: accumulated in place)
+    w_bar = _greens_adj(tp["w_new"], lay.trial, ld_bar=ld_bar_new, out=w_bar.resolve_conj().contiguous())
     # Trotter
     w_bar_in, vhs_bar = _trotter_adj(lay, run, tp["w"], tp["vhs"], w_bar, acc)
     # VHS = sum_g xbar_g L_g: xbar_bar = VHS_bar . L (real and imaginary planes
@@ -480,7 +488,11 @@ class _RunEnergy(torch.autograd.Function):
         adj = (torch.zeros((nw, n, o), dtype=c128, device=dev), torch.zeros((nw,), dtype=f64, device=dev),
                torch.zeros((nw,), dtype=c128, device=dev), torch.zeros((), dtype=f64, device=dev))
         for b in range(run.n_blocks - 1, -1, -1):
-            _, _, _, _, steps, end = _block(lay, run, fw["chk"][b], b, e_est, record=True)
+            if b == run.n_blocks - 1:
+                steps, end = fw["last"]
+                fw["last"] = None
+            else:
+                _, _, _, _, steps, end = _block(lay, run, fw["chk"][b], b, e_est, record=True)
             adj = _block_end_adj(lay, run, end, adj, eb_bar[b], wb_bar[b], e_est, acc)
             for tp in reversed(steps):
                 adj = _step_adj(lay, run, tp, adj, e_est, acc)
