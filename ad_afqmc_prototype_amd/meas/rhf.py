@@ -4,8 +4,10 @@ reference), batched over walkers on the HIP kernels.
 Force bias: fb = alpha * rot_chol_flat . vec(G) as one fp64 MFMA GEMM over the
 whole walker batch (alpha = 2 for restricted walkers, meas/rhf.py:33;
 spin sectors side by side along K for unrestricted walkers, meas/rhf.py:71-72).
-Energy: meas/rhf.py:36-50 / 75-105 with the (naux, o, o) intermediate reduced
-inside the kernel.
+Energy: meas/rhf.py:36-50 / 75-105.  Where n < naux the exchange
+sum_g tr(f_g f_g) is the quadratic form vec(G)^T Y vec(G) with the fixed
+Y = sum_g R_g (x) R_g regrouped (one GEMM over all walkers); otherwise the
+(naux, o, o) intermediate is reduced inside the fused kernel.
 """
 from __future__ import annotations
 
@@ -26,12 +28,15 @@ class rhf_meas_ctx:
     rot_chol (2, naux, o, n) (both sectors alias one another for an RHF trial),
     rot_chol_flat = rot_chol[0] viewed as (naux, o*n), plus device layouts
     built on first use (force-bias operand per walker kind, padded energy
-    operand)."""
+    operand, exchange quadratic form).  exchange: "auto" measures the exchange
+    as a quadratic form where that is cheaper (quadratic_exchange), "per_g" on
+    the fused per-g kernels always."""
 
     rot_h1: torch.Tensor
     rot_chol: torch.Tensor
     rot_chol_flat: torch.Tensor
     precision: str = "fp64"
+    exchange: str = "auto"
     _cache: dict = field(default_factory=dict, compare=False, repr=False)
 
     def fb_operand(self, nspin: int):
@@ -61,6 +66,23 @@ class rhf_meas_ctx:
     def is_complex(self) -> bool:
         return self.rot_chol.is_complex()
 
+    def exchange_operand(self, s: int):
+        """Y of spin sector s, (o n, o n) with Y[(a,q),(b,p)] = sum_g R[g,a,p] R[g,b,q]
+        (R = rot_chol[s], real): the exchange sum_g tr(f_g f_g) of a walker with
+        f_g = R_g G^T is the quadratic form vec(G)^T Y vec(G).  Y is symmetric.
+        float64; float32 in fp32 mode; (hi, lo) bf16 planes in bf16x3 mode."""
+        key = ("exchange", s)
+        if key not in self._cache:
+            _, g, o, n = self.rot_chol.shape
+            r = self.rot_chol[s].reshape(g, o * n)
+            y = (r.T @ r).view(o, n, o, n).permute(0, 3, 2, 1).reshape(o * n, o * n).contiguous()
+            if self.precision == "fp32":
+                y = kernels.convert_f32(y)
+            elif self.precision == "bf16x3":
+                y = kernels.split_bf16(y)
+            self._cache[key] = y
+        return self._cache[key]
+
     def energy_operands(self, nspin: int):
         """(rot_chol_pad, rot_h1 (nspin, o, n)) for the energy kernels:
         rot_chol_pad is (nspin, naux, opad, ldr) float64 / float32 in fp64 /
@@ -85,7 +107,8 @@ class rhf_meas_ctx:
         return self._cache[key]
 
 
-def build_meas_ctx(ham_data: ham_chol, trial_data, precision: str = "fp64") -> rhf_meas_ctx:
+def build_meas_ctx(ham_data: ham_chol, trial_data, precision: str = "fp64",
+                   exchange: str = "auto") -> rhf_meas_ctx:
     """meas/rhf.py:141-150 (per spin sector for a UHF trial)."""
     if ham_data.basis != "restricted":
         raise ValueError("RHF meas_ops currently assumes ham_chol.basis == 'restricted'.")
@@ -100,7 +123,10 @@ def build_meas_ctx(ham_data: ham_chol, trial_data, precision: str = "fp64") -> r
         rh = torch.stack(rot_h1)
         rc = torch.stack(rot_chol)
     flat = rc[0].reshape(rc.shape[1], -1)
-    return rhf_meas_ctx(rot_h1=rh.contiguous(), rot_chol=rc, rot_chol_flat=flat, precision=precision)
+    if exchange not in ("auto", "per_g"):
+        raise ValueError(f"unknown exchange {exchange!r}")
+    return rhf_meas_ctx(rot_h1=rh.contiguous(), rot_chol=rc, rot_chol_flat=flat, precision=precision,
+                        exchange=exchange)
 
 
 def _nspin(walkers) -> int:
@@ -253,12 +279,63 @@ def _energy_library(g: torch.Tensor, nw: int, n: int, o: int, nspin: int, meas_c
     return out
 
 
+def quadratic_exchange(n: int, o: int, naux: int) -> bool:
+    """The exchange as a quadratic form costs 4 (o n)^2 per walker and sector
+    against 4 naux o^2 n for the sum over g of tr(f_g f_g): cheaper whenever
+    n < naux (molecular Cholesky sets have naux of 4-10 n), and Y must fit
+    (4 GiB in fp64)."""
+    return n < naux and (o * n) ** 2 * 8 <= (4 << 30)
+
+
+def _energy_quadratic(greens, nw: int, n: int, o: int, nspin: int, meas_ctx: rhf_meas_ctx,
+                      h0: float) -> torch.Tensor:
+    """meas/rhf.py:36-50 / 75-105 with the same sums regrouped:
+      c_g (summed over spin sectors) = the force-bias contraction with alpha 1;
+      Coulomb: restricted 2 sum_g c_g^2, unrestricted (sum_g (c_up + c_dn)_g^2) / 2;
+      exchange of sector s: sum_g tr(f_g f_g) = vec(G_s)^T Y_s vec(G_s), one GEMM
+      Z = G_s Y_s over all walkers (the ctx's precision, fp64 output) and row
+      dot products (complex, no conjugation);
+      one-body: tr(rot_h1_s G_s).
+    Every sum outside the GEMMs runs in fp64."""
+    g = greens.greens
+    on = o * n
+    c = _force_bias_rows(g, meas_ctx, nspin, 1.0, greens.split)
+    cr, ci = c[0], c[1]
+    coul = (cr * cr - ci * ci).sum(-1) + 2.0j * (cr * ci).sum(-1)
+    e1 = torch.zeros((nw,), dtype=torch.complex128, device=g.device)
+    exc = torch.zeros((nw,), dtype=torch.complex128, device=g.device)
+    z = torch.empty((2 * nw, on), dtype=torch.float64, device=g.device)
+    for sec in range(nspin):
+        rows = g.reshape(2 * nw, nspin * on)[:, sec * on:(sec + 1) * on]
+        if nspin > 1:
+            rows = rows.contiguous()
+        y = meas_ctx.exchange_operand(sec)
+        if meas_ctx.precision == "fp64":
+            kernels.dgemm_nt(rows, y, z, tag="exchange")
+        elif meas_ctx.precision == "fp32":
+            kernels.sgemm_nt(kernels.convert_f32(rows), y, z, tag="exchange")
+        else:
+            gh, gl = greens.split if nspin == 1 and greens.split is not None else kernels.split_bf16(rows)
+            kernels.gemm_bf16x3_nt(gh, gl, y[0], y[1], z, tag="exchange")
+        gr, gi = rows[:nw], rows[nw:]
+        zr, zi = z[:nw], z[nw:]
+        exc += torch.complex((gr * zr).sum(-1) - (gi * zi).sum(-1), (gr * zi).sum(-1) + (gi * zr).sum(-1))
+        h = meas_ctx.rot_h1[sec].reshape(on)
+        e1 += torch.complex(gr @ h, gi @ h)
+    if nspin == 1:
+        return h0 + 2.0 * e1 + 2.0 * coul - exc
+    return h0 + e1 + (coul - exc) / 2.0
+
+
 def _energy(walkers, ham_data: ham_chol, meas_ctx: rhf_meas_ctx, trial_data, greens=None):
     nspin = _nspin(walkers)
     w0 = walkers[0] if nspin == 2 else walkers
     nw, n, o = w0.shape
     if greens is None or greens.greens is None:
         greens = trial_rhf.greens(walkers, trial_data)
+    if (meas_ctx.exchange == "auto" and not meas_ctx.is_complex
+            and quadratic_exchange(n, o, meas_ctx.rot_chol.shape[1])):
+        return _energy_quadratic(greens, nw, n, o, nspin, meas_ctx, float(ham_data.h0))
     if o > ENERGY_KERNEL_MAX_O or meas_ctx.is_complex:
         return _energy_library(greens.greens, nw, n, o, nspin, meas_ctx, float(ham_data.h0))
     rot_pad, rot_h1 = meas_ctx.energy_operands(nspin)
@@ -275,14 +352,17 @@ def energy_kernel_u(walkers, ham_data, meas_ctx, trial_data, greens=None):
     return _energy(walkers, ham_data, meas_ctx, trial_data, greens)
 
 
-def make_rhf_meas_ops(sys: system, precision: str = "fp64") -> meas_ops:
+def make_rhf_meas_ops(sys: system, precision: str = "fp64", exchange: str = "auto") -> meas_ops:
     """meas/rhf.py:153-176; also serves UHF trials (per-spin integrals).
     precision selects the force-bias and energy contractions: "fp64", "fp32"
-    or "bf16x3"."""
+    or "bf16x3"; exchange the energy's exchange form ("auto" or "per_g", see
+    rhf_meas_ctx)."""
     from functools import partial
     if precision not in ("fp64", "fp32", "bf16x3"):
         raise ValueError(f"unknown precision {precision!r}")
-    build_meas_ctx_p = partial(build_meas_ctx, precision=precision)
+    if exchange not in ("auto", "per_g"):
+        raise ValueError(f"unknown exchange {exchange!r}")
+    build_meas_ctx_p = partial(build_meas_ctx, precision=precision, exchange=exchange)
     wk = sys.walker_kind.lower()
     # the bf16x3 force bias takes the Green's function rows as bf16 planes,
     # which the Green's function kernel writes in the same pass

@@ -219,7 +219,8 @@ class system_batch:
     def _member(self, s: batch_system, n: int, o: int, g: int) -> dict:
         sys_ = system(n, (o, o), "restricted")
         t_ops = make_rhf_trial_ops(sys_)
-        m_ops = make_rhf_meas_ops(sys_, precision="fp32")
+        # (the exchange on the per-g kernels, as the batch's own block ends measure it)
+        m_ops = make_rhf_meas_ops(sys_, precision="fp32", exchange="per_g")
         p_ops = make_chol_afqmc_ops(s.ham, "restricted", precision="fp32")
         p_ctx = p_ops.build_prop_ctx(t_ops.get_rdm1(s.trial), self.dt)
         m_ctx = m_ops.build_meas_ctx(s.ham, s.trial)

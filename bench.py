@@ -449,6 +449,11 @@ def kernel_table(ktimes, n, o, g, nw, nspin, rhf, precision):
         "afq_energy": (4.0 * g * o * o * n * nspin * nw, "fp64"),
         "afq_energy_f32": (4.0 * g * o * o * n * nspin * nw, "fp32"),
         "afq_energy_bf16x3": (4.0 * g * o * o * n * nspin * nw, "bf16x3"),
+        # the exchange as a quadratic form: Z = G Y per spin sector, M = 2 nw
+        # (real and imaginary rows), N = K = o n
+        "afq_dgemm_nt:exchange": (4.0 * nw * (o * n) ** 2, "fp64"),
+        "afq_sgemm_nt:exchange": (4.0 * nw * (o * n) ** 2, "fp32"),
+        "afq_gemm_bf16x3_nt:exchange": (4.0 * nw * (o * n) ** 2, "bf16x3"),
     }
     out = {}
     for name, t in ktimes.items():

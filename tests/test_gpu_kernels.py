@@ -173,14 +173,21 @@ def test_trotter_zero_terms_and_identity(gpu_device):
     np.testing.assert_array_equal(out, w)
 
 
+@pytest.mark.parametrize("exchange", ["auto", "per_g"])
 @pytest.mark.parametrize("seed,n,o,naux,uhf,kind", SHAPES)
-def test_energy(gpu_device, seed, n, o, naux, uhf, kind):
+def test_energy(gpu_device, seed, n, o, naux, uhf, kind, exchange):
+    """Local energies against the checker at 1e-10, the exchange both as the
+    quadratic form vec(G)^T Y vec(G) ("auto" where n < naux) and on the fused
+    per-g kernels."""
     from ad_afqmc_prototype_amd.core.ops import k_energy
+    from ad_afqmc_prototype_amd.meas.rhf import make_rhf_meas_ops
     p = make_pair(seed, n, o, naux, uhf=uhf, kind=kind, device=gpu_device)
     rng = np.random.default_rng(seed + 14)
     nw = 4 if n >= 200 else 12
     w = random_walkers(rng, kind, nw, n, o, p.s.mo_up, p.s.mo_dn)
-    e = p.g["m_ops"].require_kernel(k_energy)(to_dev(w, gpu_device), p.g["ham"], p.g["m_ctx"], p.g["trial"])
+    m_ops = make_rhf_meas_ops(p.g["sys"], exchange=exchange)
+    ctx = m_ops.build_meas_ctx(p.g["ham"], p.g["trial"])
+    e = m_ops.require_kernel(k_energy)(to_dev(w, gpu_device), p.g["ham"], ctx, p.g["trial"])
     e_ref = orc.energy(w, p.sd.h0, p.sd.trial, p.sd.mctx, kind)
     assert_rel(e.cpu().numpy(), e_ref, TOL)
 
@@ -195,7 +202,7 @@ def test_energy(gpu_device, seed, n, o, naux, uhf, kind):
 def test_energy_stacked_rows(gpu_device, seed, n, o, naux, uhf, kind):
     """The fp64 energy on the shapes that take stacked g chunks, against the
     checker at 1e-10."""
-    test_energy(gpu_device, seed, n, o, naux, uhf, kind)
+    test_energy(gpu_device, seed, n, o, naux, uhf, kind, "per_g")
 
 
 @pytest.mark.parametrize("n,o", [(24, 6), (120, 20), (200, 32), (64, 64)])

@@ -168,8 +168,10 @@ def test_mixed_block_energy_close_to_fp64(gpu_device, precision):
     dict(seed=6, n=48, o=12, g=61, uhf=True, kind="unrestricted", nw=6),
     dict(seed=7, n=70, o=19, g=130, uhf=False, kind="restricted", nw=3),
 ])
-def test_energy_fp32_contractions(gpu_device, case):
-    """fp32 f_g contractions with fp64 sums: per-walker energies within 2e-5 of
+@pytest.mark.parametrize("exchange", ["auto", "per_g"])
+def test_energy_fp32_contractions(gpu_device, case, exchange):
+    """fp32 f_g contractions with fp64 sums (or the fp32 G Y product of the
+    exchange as a quadratic form): per-walker energies within 2e-5 of
     |E| of the fp64 checker (two walkers per workgroup, odd walker count).  The
     random walkers here are far from the trial (|E| up to 2e4 Ha, large
     imaginary parts), the hardest case for fp32."""
@@ -177,7 +179,7 @@ def test_energy_fp32_contractions(gpu_device, case):
     from ad_afqmc_prototype_amd.meas.rhf import make_rhf_meas_ops
     p = make_pair(case["seed"], case["n"], case["o"], case["g"], uhf=case["uhf"], kind=case["kind"],
                   device=gpu_device)
-    m_ops = make_rhf_meas_ops(p.g["sys"], precision="fp32")
+    m_ops = make_rhf_meas_ops(p.g["sys"], precision="fp32", exchange=exchange)
     ctx = m_ops.build_meas_ctx(p.g["ham"], p.g["trial"])
     rng = np.random.default_rng(3)
     from ._pairs import random_walkers
@@ -271,8 +273,10 @@ def test_trotter_bf16x3_exact_identity(gpu_device):
     (dict(seed=0, n=24, o=6, g=96, uhf=False, kind="restricted", nw=9, spread=0.05), 1e-5),
     (dict(seed=2, n=200, o=32, g=1000, uhf=False, kind="restricted", nw=4, spread=0.3), 1e-4),
 ])
-def test_energy_bf16x3_contractions(gpu_device, case, tol):
-    """Split-bf16 f_g contractions with fp64 sums against the fp64 checker.
+@pytest.mark.parametrize("exchange", ["auto", "per_g"])
+def test_energy_bf16x3_contractions(gpu_device, case, tol, exchange):
+    """Split-bf16 f_g contractions with fp64 sums (or the split-bf16 G Y
+    product of the exchange as a quadratic form) against the fp64 checker.
     The split keeps ~2^-17 of each operand, and that error enters the
     Coulomb square and the exchange coherently, so the bound is relative to
     |E|: 2e-6 for walkers near the trial at the working Cholesky scale, 1e-5
@@ -285,7 +289,7 @@ def test_energy_bf16x3_contractions(gpu_device, case, tol):
     from ._pairs import random_walkers
     p = make_pair(case["seed"], case["n"], case["o"], case["g"], uhf=case["uhf"], kind=case["kind"],
                   chol_scale=0.03 if case["n"] == 200 else 1.0, device=gpu_device)
-    m_ops = make_rhf_meas_ops(p.g["sys"], precision="bf16x3")
+    m_ops = make_rhf_meas_ops(p.g["sys"], precision="bf16x3", exchange=exchange)
     ctx = m_ops.build_meas_ctx(p.g["ham"], p.g["trial"])
     rng = np.random.default_rng(4)
     w = random_walkers(rng, case["kind"], case["nw"], case["n"], case["o"], p.s.mo_up, p.s.mo_dn,
