@@ -86,6 +86,8 @@ _SIGNATURES = {
     "afq_energy_f32_grouped": [c_int, c_vp, c_int, c_int, c_int, c_vp],
     "afq_energy_bf16x3_grouped": [c_int, c_vp, c_int, c_int, c_int, c_int, c_vp],
     "afq_energy_bf16x3_grouped_fits": [c_int, c_int, c_int],
+    "afq_energy_assemble": [c_int, c_int, c_int, c_int, c_double, c_double, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp],
+    "afq_energy_assemble_grouped": [c_int, c_vp, c_int, c_vp],
     # Hamiltonian ingestion (modified Cholesky)
     "afq_mcd_workspace": [c_int, ctypes.POINTER(c_ll), ctypes.POINTER(c_ll)],
     "afq_mcd_start": [c_int, c_vp, c_ll, c_double, c_int, c_vp, c_vp, c_vp, c_vp, c_vp],

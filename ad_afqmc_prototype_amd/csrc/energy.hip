@@ -1050,3 +1050,115 @@ extern "C" int afq_energy_bf16x3_grouped_fits(int max_n, int nt, int wpb) {
         lds = wpb == 1 ? energy_bf16x3_lds<2, 1>(ldr, 1, 0) : energy_bf16x3_lds<2, 2>(ldr, 1, 0);
     return lds + 8 * 6 * wpb * sizeof(double) <= 160 * 1024 ? 1 : 0;
 }
+
+// ---------------------------------------------------------------------------
+// Local energies from the exchange as a quadratic form (meas/rhf.py:36-50,
+// 75-105 with the sum over g regrouped): per walker
+//   e1   = sum_s tr(rot_h1_s G_s)                      (one-body)
+//   coul = sum_g c_g^2, c = c_scale x the force-bias contraction (c summed over
+//          spin sectors: sum_g (c_up + c_dn)_g^2 for unrestricted walkers)
+//   exc  = sum_s vec(G_s)^T Z_s, Z_s = G_s Y_s the caller's GEMM of the sector's
+//          rows with Y_s[(a,q),(b,p)] = sum_g R[g,a,p] R[g,b,q]   (no conjugation)
+//   E    = h0 + 2 e1 + 2 coul - exc (restricted), h0 + e1 + (coul - exc) / 2
+// One workgroup per walker, fp64 sums: each thread strides its share in a fixed
+// order, then a fixed tree over the workgroup (the result does not depend on
+// the launch).
+namespace {
+
+constexpr int THREADS_A = 256;
+
+__device__ void energy_assemble_walker(int w, int nw, int on, int nspin, int naux, double h0, double c_scale,
+                                       const double* __restrict__ G, const double* __restrict__ Z,
+                                       const double* __restrict__ c, const double* __restrict__ h1,
+                                       double2* __restrict__ E, double (*red)[THREADS_A]) {
+    const int tid = threadIdx.x;
+    double v[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};  // e1 re/im, exc re/im, coul re/im
+    const long long rowg = (long long)nspin * on;
+    for (int s = 0; s < nspin; ++s) {
+        const double* gr = G + (long long)w * rowg + (long long)s * on;
+        const double* gi = G + (long long)(nw + w) * rowg + (long long)s * on;
+        const double* zr = Z + (long long)s * 2 * nw * on + (long long)w * on;
+        const double* zi = Z + (long long)s * 2 * nw * on + (long long)(nw + w) * on;
+        const double* hs = h1 + (long long)s * on;
+        for (int j = tid; j < on; j += THREADS_A) {
+            const double a = gr[j], b = gi[j], x = zr[j], y = zi[j], h = hs[j];
+            v[0] = fma(h, a, v[0]);
+            v[1] = fma(h, b, v[1]);
+            v[2] = fma(a, x, fma(-b, y, v[2]));
+            v[3] = fma(a, y, fma(b, x, v[3]));
+        }
+    }
+    const double* cr = c + (long long)w * naux;
+    const double* ci = c + (long long)(nw + w) * naux;
+    for (int g = tid; g < naux; g += THREADS_A) {
+        const double a = c_scale * cr[g], b = c_scale * ci[g];
+        v[4] = fma(a, a, fma(-b, b, v[4]));
+        v[5] = fma(2.0 * a, b, v[5]);
+    }
+#pragma unroll
+    for (int k = 0; k < 6; ++k) red[k][tid] = v[k];
+    __syncthreads();
+    for (int h = THREADS_A / 2; h > 0; h >>= 1) {
+        if (tid < h) {
+#pragma unroll
+            for (int k = 0; k < 6; ++k) red[k][tid] += red[k][tid + h];
+        }
+        __syncthreads();
+    }
+    if (tid == 0) {
+        const double e1r = red[0][0], e1i = red[1][0], xr = red[2][0], xi = red[3][0];
+        const double qr = red[4][0], qi = red[5][0];
+        if (nspin == 1)
+            E[w] = make_double2(h0 + 2.0 * e1r + 2.0 * qr - xr, 2.0 * e1i + 2.0 * qi - xi);
+        else
+            E[w] = make_double2(h0 + e1r + 0.5 * (qr - xr), e1i + 0.5 * (qi - xi));
+    }
+}
+
+__global__ __launch_bounds__(THREADS_A) void energy_assemble_kernel(int nw, int on, int nspin, int naux, double h0,
+                                                                    double c_scale, const double* __restrict__ G,
+                                                                    const double* __restrict__ Z,
+                                                                    const double* __restrict__ c,
+                                                                    const double* __restrict__ h1,
+                                                                    double2* __restrict__ E) {
+    __shared__ double red[6][THREADS_A];
+    energy_assemble_walker(blockIdx.x, nw, on, nspin, naux, h0, c_scale, G, Z, c, h1, E, red);
+}
+
+// one workgroup per walker of every system (restricted walkers, one sector):
+// G = greens, Z = exch_z, c = fb / 2 (the step's force bias, alpha 2), rot_h1,
+// h0, energy from the system's descriptor
+__global__ __launch_bounds__(THREADS_A) void energy_assemble_grouped_kernel(int count,
+                                                                            const afq_system_step* __restrict__ S) {
+    __shared__ double red[6][THREADS_A];
+    const int b = blockIdx.x;
+    const int si = afq::find_group(count, b, [&](int i) { return S[i].walker0; });
+    const afq_system_step& d = S[si];
+    energy_assemble_walker(b - d.walker0, d.nw, d.o * d.n, 1, d.naux, d.h0, 0.5, d.greens, d.exch_z, d.fb,
+                           d.rot_h1, (double2*)d.energy, red);
+}
+
+}  // namespace
+
+// Declared in afqmc_hip.h.
+extern "C" int afq_energy_assemble(int nw, int on, int nspin, int naux, double h0, double c_scale,
+                                   const double* greens, const double* z, const double* c, const double* rot_h1,
+                                   double* energy, void* stream) {
+    AFQ_REQUIRE(nw >= 0 && on > 0 && naux > 0 && (nspin == 1 || nspin == 2),
+                "afq_energy_assemble: bad sizes nw=%d on=%d nspin=%d naux=%d", nw, on, nspin, naux);
+    if (nw == 0) return 0;
+    AFQ_REQUIRE(greens && z && c && rot_h1 && energy, "afq_energy_assemble: null pointer");
+    hipLaunchKernelGGL(energy_assemble_kernel, dim3((unsigned)nw), dim3(THREADS_A), 0, (hipStream_t)stream, nw, on,
+                       nspin, naux, h0, c_scale, greens, z, c, rot_h1, (double2*)energy);
+    return afq::check_launch("afq_energy_assemble");
+}
+
+extern "C" int afq_energy_assemble_grouped(int count, const afq_system_step* sys, int total_walkers,
+                                           void* stream) {
+    AFQ_REQUIRE(count > 0 && total_walkers > 0, "afq_energy_assemble_grouped: count=%d total_walkers=%d", count,
+                total_walkers);
+    AFQ_REQUIRE(sys, "afq_energy_assemble_grouped: null pointer");
+    hipLaunchKernelGGL(energy_assemble_grouped_kernel, dim3((unsigned)total_walkers), dim3(THREADS_A), 0,
+                       (hipStream_t)stream, count, sys);
+    return afq::check_launch("afq_energy_assemble_grouped");
+}

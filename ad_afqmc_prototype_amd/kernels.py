@@ -478,6 +478,27 @@ def orthonormalize(w, out=None):
     return out
 
 
+def energy_assemble(greens, z, c, rot_h1, nw, o, n, nspin, h0, c_scale=1.0, out=None):
+    """Local energies (nw,) complex128 with the exchange as a quadratic form:
+    greens (2, nw, nspin*o*n) rows, z (nspin, 2*nw, o*n) = each sector's rows
+    times Y_s, c (2, nw, naux) planes (c_scale times the force-bias contraction
+    summed over sectors), rot_h1 (nspin, o, n)."""
+    dev = greens.device
+    on = o * n
+    _check(greens, f64, (2, nw, nspin * on), "greens")
+    _check(z, f64, (nspin, 2 * nw, on), "z", dev)
+    if c.dim() != 3 or c.shape[:2] != (2, nw):
+        raise ValueError(f"c must be (2, {nw}, naux), got {tuple(c.shape)}")
+    _check(c, f64, tuple(c.shape), "c", dev)
+    _check(rot_h1, f64, (nspin, o, n), "rot_h1", dev)
+    if out is None:
+        out = torch.empty((nw,), dtype=c128, device=dev)
+    _check(out, c128, (nw,), "energy", dev)
+    _call("afq_energy_assemble", greens, nw, on, nspin, c.shape[2], float(h0), float(c_scale), greens.data_ptr(),
+          z.data_ptr(), c.data_ptr(), rot_h1.data_ptr(), out.data_ptr(), _stream(greens))
+    return out
+
+
 def energy(greens, nw, n, o, nspin, rot_chol_pad, rot_h1, h0, out=None):
     """Local energies (nw,) complex128.  rot_chol_pad: (nspin, naux, opad,
     ldr) float64 or float32 (fp64 / fp32 contractions), or a (hi, lo) pair of

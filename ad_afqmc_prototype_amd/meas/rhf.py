@@ -293,38 +293,26 @@ def _energy_quadratic(greens, nw: int, n: int, o: int, nspin: int, meas_ctx: rhf
       c_g (summed over spin sectors) = the force-bias contraction with alpha 1;
       Coulomb: restricted 2 sum_g c_g^2, unrestricted (sum_g (c_up + c_dn)_g^2) / 2;
       exchange of sector s: sum_g tr(f_g f_g) = vec(G_s)^T Y_s vec(G_s), one GEMM
-      Z = G_s Y_s over all walkers (the ctx's precision, fp64 output) and row
-      dot products (complex, no conjugation);
-      one-body: tr(rot_h1_s G_s).
-    Every sum outside the GEMMs runs in fp64."""
+      Z = G_s Y_s over all walkers (the ctx's precision, fp64 output);
+      one-body: tr(rot_h1_s G_s);
+    the dot products and sums on afq_energy_assemble (fp64)."""
     g = greens.greens
     on = o * n
     c = _force_bias_rows(g, meas_ctx, nspin, 1.0, greens.split)
-    cr, ci = c[0], c[1]
-    coul = (cr * cr - ci * ci).sum(-1) + 2.0j * (cr * ci).sum(-1)
-    e1 = torch.zeros((nw,), dtype=torch.complex128, device=g.device)
-    exc = torch.zeros((nw,), dtype=torch.complex128, device=g.device)
-    z = torch.empty((2 * nw, on), dtype=torch.float64, device=g.device)
+    z = torch.empty((nspin, 2 * nw, on), dtype=torch.float64, device=g.device)
     for sec in range(nspin):
         rows = g.reshape(2 * nw, nspin * on)[:, sec * on:(sec + 1) * on]
         if nspin > 1:
             rows = rows.contiguous()
         y = meas_ctx.exchange_operand(sec)
         if meas_ctx.precision == "fp64":
-            kernels.dgemm_nt(rows, y, z, tag="exchange")
+            kernels.dgemm_nt(rows, y, z[sec], tag="exchange")
         elif meas_ctx.precision == "fp32":
-            kernels.sgemm_nt(kernels.convert_f32(rows), y, z, tag="exchange")
+            kernels.sgemm_nt(kernels.convert_f32(rows), y, z[sec], tag="exchange")
         else:
             gh, gl = greens.split if nspin == 1 and greens.split is not None else kernels.split_bf16(rows)
-            kernels.gemm_bf16x3_nt(gh, gl, y[0], y[1], z, tag="exchange")
-        gr, gi = rows[:nw], rows[nw:]
-        zr, zi = z[:nw], z[nw:]
-        exc += torch.complex((gr * zr).sum(-1) - (gi * zi).sum(-1), (gr * zi).sum(-1) + (gi * zr).sum(-1))
-        h = meas_ctx.rot_h1[sec].reshape(on)
-        e1 += torch.complex(gr @ h, gi @ h)
-    if nspin == 1:
-        return h0 + 2.0 * e1 + 2.0 * coul - exc
-    return h0 + e1 + (coul - exc) / 2.0
+            kernels.gemm_bf16x3_nt(gh, gl, y[0], y[1], z[sec], tag="exchange")
+    return kernels.energy_assemble(g, z, c, meas_ctx.rot_h1[:nspin].contiguous(), nw, o, n, nspin, h0)
 
 
 def _energy(walkers, ham_data: ham_chol, meas_ctx: rhf_meas_ctx, trial_data, greens=None):

@@ -44,7 +44,7 @@ from .. import _lib
 from .. import kernels
 from ..core.system import system
 from ..ham.chol import ham_chol
-from ..meas.rhf import make_rhf_meas_ops
+from ..meas.rhf import make_rhf_meas_ops, quadratic_exchange
 from ..trial.rhf import make_rhf_trial_ops, rhf_trial
 from .afqmc import init_prop_state
 from .chol_afqmc_ops import chol_sym_rows, make_chol_afqmc_ops
@@ -70,7 +70,8 @@ class _system_step(ctypes.Structure):
                     "xbar", "shift_term", "fb_term", "exp_h1_half", "vhs", "log_overlaps_in",
                     "log_overlaps_out", "weights_in", "weights_out", "pop_in", "pop_out", "key_in",
                     "key_out", "rot_chol", "rot_h1", "energy", "greens_hi", "greens_lo")] + [
-                    ("ldk", ctypes.c_longlong), ("xbar_hi", _vp), ("xbar_lo", _vp), ("ldx", ctypes.c_longlong)]
+                    ("ldk", ctypes.c_longlong), ("xbar_hi", _vp), ("xbar_lo", _vp), ("ldx", ctypes.c_longlong),
+                ("exch_z", _vp)]
 
 
 class _gemm_problem(ctypes.Structure):
@@ -99,7 +100,7 @@ class _split_problem(ctypes.Structure):
                 ("ld_src", ctypes.c_int), ("ld_dst", ctypes.c_int)]
 
 
-assert ctypes.sizeof(_system_step) == 8 * 4 + 8 * 8 + 26 * 8 + 8 + 2 * 8 + 8
+assert ctypes.sizeof(_system_step) == 8 * 4 + 8 * 8 + 26 * 8 + 8 + 2 * 8 + 8 + 8
 assert ctypes.sizeof(_gemm_problem) == 80
 assert ctypes.sizeof(_gemm_problem_bf16x3) == 96
 assert ctypes.sizeof(_split_problem) == 40
@@ -219,8 +220,7 @@ class system_batch:
     def _member(self, s: batch_system, n: int, o: int, g: int) -> dict:
         sys_ = system(n, (o, o), "restricted")
         t_ops = make_rhf_trial_ops(sys_)
-        # (the exchange on the per-g kernels, as the batch's own block ends measure it)
-        m_ops = make_rhf_meas_ops(sys_, precision="fp32", exchange="per_g")
+        m_ops = make_rhf_meas_ops(sys_, precision="fp32")
         p_ops = make_chol_afqmc_ops(s.ham, "restricted", precision="fp32")
         p_ctx = p_ops.build_prop_ctx(t_ops.get_rdm1(s.trial), self.dt)
         m_ctx = m_ops.build_meas_ctx(s.ham, s.trial)
@@ -244,6 +244,15 @@ class system_batch:
                        e_pairs=p_ctx.exp_h1_half_sym_pairs.contiguous())
         else:
             ops = dict(chol_s32=kernels.convert_f32(chol_sym_rows(s.ham.chol)), rot32=m_ctx.fb_operand(1))
+        if quadratic_exchange(n, o, g):
+            # Y of the exchange as a quadratic form, in the batch's precision
+            r = m_ctx.rot_chol[0].reshape(g, o * n)
+            y = (r.T @ r).view(o, n, o, n).permute(0, 3, 2, 1).reshape(o * n, o * n).contiguous()
+            if self.precision == "bf16x3":
+                ops["y_hl"] = kernels.split_bf16(y)
+            else:
+                ops["y32"] = kernels.convert_f32(y)
+            del y
         return dict(n=n, o=o, g=g, nw=s.n_walkers, ldv=kernels.ldv_for(n), nsym=kernels.sym_blocks(n) * 256,
                     h0=float(s.ham.h0),
                     trial=s.trial.mo_up.contiguous(), **ops, rot_pad=rot_pad, rot_h1=rot_h1, ldr=ldr,
@@ -265,6 +274,11 @@ class system_batch:
         self.logdet = P([m["nw"] for m in ms], c128)
         self.fields = P([m["nw"] * m["g"] for m in ms], f64)
         self.fb = P([2 * m["nw"] * m["g"] for m in ms], f64)
+        # block ends with the exchange as a quadratic form (every system
+        # n < naux): Z = G Y rows
+        self.quadratic = all("y32" in m or "y_hl" in m for m in ms)
+        if self.quadratic:
+            self.exch_z = P([2 * m["nw"] * m["o"] * m["n"] for m in ms], f64)
         self.xbar = P([2 * m["nw"] * m["g"] for m in ms], f64)
         if self.precision == "fp32":
             self.xbar32 = P(self.xbar.sizes, f32)
@@ -324,6 +338,8 @@ class system_batch:
             d.key_in, d.key_out = self.keys[a][i].data_ptr(), self.keys[b][i].data_ptr()
             d.rot_chol, d.rot_h1, d.energy = (m["rot_pad"].data_ptr(), m["rot_h1"].data_ptr(),
                                               self.eloc.ptr(i))
+            if self.quadratic:
+                d.exch_z = self.exch_z.ptr(i)
             if self.precision == "bf16x3":
                 d.greens_hi, d.greens_lo = self.greens_hl[0].ptr(i), self.greens_hl[1].ptr(i)
                 d.ldk = kernels.pad8(m["o"] * m["n"])
@@ -343,6 +359,12 @@ class system_batch:
                 p.lda, p.ldb, p.ldc = k, m["rot32"].shape[1], m["g"]
                 p.A, p.B, p.C = self.greens32.ptr(i), m["rot32"].data_ptr(), self.fb.ptr(i)
                 p.alpha, p.c_fp64 = 2.0, 1
+            elif which == "exch":
+                k = m["o"] * m["n"]
+                p.M, p.N, p.K = 2 * m["nw"], k, k
+                p.lda, p.ldb, p.ldc = k, k, k
+                p.A, p.B, p.C = self.greens32.ptr(i), m["y32"].data_ptr(), self.exch_z.ptr(i)
+                p.alpha, p.c_fp64 = 1.0, 1
             else:
                 # symmetric VHS: the upper 16 x 16 blocks, each entry stored twice
                 # into the (n, ldv) rows the Trotter reads
@@ -367,6 +389,11 @@ class system_batch:
                 (bh, bl), (ah, al) = m["rot_hl"], self.greens_hl
                 p.M, p.N, p.K = 2 * m["nw"], m["g"], k
                 p.ldc, p.C, p.alpha, p.c_fp64 = m["g"], self.fb.ptr(i), 2.0, 1
+            elif which == "exch":
+                k = pad8(m["o"] * m["n"])
+                (bh, bl), (ah, al) = m["y_hl"], self.greens_hl
+                p.M, p.N, p.K = 2 * m["nw"], m["o"] * m["n"], k
+                p.ldc, p.C, p.alpha, p.c_fp64 = m["o"] * m["n"], self.exch_z.ptr(i), 1.0, 1
             else:
                 # V's upper blocks as pair words, stored as they come
                 k = pad8(m["g"])
@@ -454,7 +481,7 @@ class system_batch:
                                             walkers=nw, blocks=blocks, wpb=wpb, max_n=max_n, nt=nt))
         self.gemm_tables = {}
         self.split_tables = {}
-        for which in ("fb", "vhs"):
+        for which in ("fb", "vhs") + (("exch",) if self.quadratic else ()):
             if self.precision == "bf16x3":
                 for shape in BF16X3_SHAPES:
                     arr, tiles = self._problems_bf16x3(which, shape)
@@ -480,12 +507,13 @@ class system_batch:
         if self.precision == "bf16x3":
             # the A operand's bf16 planes come from the Green's stage (force
             # bias) and the field shifts (VHS)
-            shape = self.gemm_shapes[0 if which == "fb" else 1]
+            shape = self.gemm_shapes[0 if which in ("fb", "exch") else 1]
             tab, tiles = self.gemm_tables[(which, shape)]
             kernels._call("afq_gemm_bf16x3_nt_grouped", tab, self.count, tab.data_ptr(), tiles, shape,
                           kernels._stream(tab), tag=which)
             return
-        shape = self.gemm_shapes[0 if which == "fb" else 1]
+        # (the exchange's G Y has M = 2 nw like the force bias: the force bias's shape)
+        shape = self.gemm_shapes[0 if which in ("fb", "exch") else 1]
         tab, tiles = self.gemm_tables[(which, shape)]
         kernels._call("afq_sgemm_nt_grouped", tab, self.count, tab.data_ptr(), tiles, shape,
                       kernels._stream(tab), tag=which)
@@ -551,7 +579,17 @@ class system_batch:
         (prop/blocks.py:59-79), comb with a uniform from the system's key
         (prop/blocks.py:81-92).  Called right after _orthonormalize."""
         thresh = (2.0 / self.dt) ** 0.5
-        for bk in self.energy_buckets:
+        if self.quadratic:
+            # c (the force bias of the measured walkers), Z = G Y, then the
+            # energies (afq_energy_assemble_grouped, c = fb / 2)
+            d = self.desc[0]
+            if self.precision == "fp32":
+                self._convert(self.greens, self.greens32)
+            self._grouped_gemm("fb")
+            self._grouped_gemm("exch")
+            kernels._call("afq_energy_assemble_grouped", d, self.count, d.data_ptr(), self.total_walkers,
+                          kernels._stream(d))
+        for bk in ([] if self.quadratic else self.energy_buckets):
             t = bk["table"]
             if self.precision == "bf16x3":
                 kernels._call("afq_energy_bf16x3_grouped", t, bk["count"], t.data_ptr(), bk["blocks"],

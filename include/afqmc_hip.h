@@ -482,6 +482,8 @@ typedef struct afq_system_step {
     unsigned short* xbar_hi;     /* null, or (2, nw, ldx) bf16 hi plane of xbar written by the field shifts */
     unsigned short* xbar_lo;     /* (2, nw, ldx) lo plane */
     long long ldx;               /* row pitch of the xbar planes: >= naux */
+    double* exch_z;              /* planes (2, nw, o*n): Z = G Y of the exchange as a quadratic form
+                                    (afq_energy_assemble_grouped) */
 } afq_system_step;
 
 /* One problem of a grouped GEMM: C (M x N) = alpha A (M x K) B^T (N x K). */
@@ -650,6 +652,27 @@ int afq_energy_bf16x3_grouped(int count, const afq_system_step* sys, int total_b
 /* 1 when afq_energy_bf16x3_grouped's LDS for (max_n, nt, wpb) fits in a CU,
  * else 0.  Host-only query, no device work; no counterpart in the reference. */
 int afq_energy_bf16x3_grouped_fits(int max_n, int nt, int wpb);
+
+/*
+ * Local energies with the exchange as a quadratic form: replaces
+ * energy_kernel_r / energy_kernel_u (meas/rhf.py:36-50, 75-105), the sum over g
+ * of tr(f_g f_g) regrouped as vec(G_s)^T Y_s vec(G_s) with the fixed
+ * Y_s[(a,q),(b,p)] = sum_g R_s[g,a,p] R_s[g,b,q].  greens: planes (2, nw,
+ * nspin*on) (on = o*n); z: per sector s the planes (2, nw, on) of G_s Y_s at z
+ * + s*2*nw*on (the caller's GEMM); c: planes (2, nw, naux), c_scale times the
+ * force-bias contraction summed over sectors; rot_h1: (nspin, on).  energy:
+ * complex128 (nw) = h0 + 2 e1 + 2 sum c^2 - exc (nspin 1) or
+ * h0 + e1 + (sum c^2 - exc) / 2 (nspin 2), fp64 sums in a fixed order.
+ */
+int afq_energy_assemble(int nw, int on, int nspin, int naux, double h0, double c_scale, const double* greens,
+                        const double* z, const double* c, const double* rot_h1, double* energy, void* stream);
+
+/*
+ * afq_energy_assemble for every system of a batch (restricted walkers): one
+ * workgroup per walker (total_walkers = sum of nw); per system greens, exch_z,
+ * c = fb / 2, rot_h1, h0 and energy from its descriptor.
+ */
+int afq_energy_assemble_grouped(int count, const afq_system_step* sys, int total_walkers, void* stream);
 
 /*
  * afq_orthonormalize for every system: walkers_in -> walkers_out

@@ -53,8 +53,7 @@ def _alone(member, n_steps, qr=QR, sr=SR):
     n, o = trial.norb, trial.nocc
     sys_ = system(n, (o, o), "restricted")
     t_ops = make_rhf_trial_ops(sys_)
-    # the exchange on the per-g kernels, as the batch measures it
-    m_ops = make_rhf_meas_ops(sys_, precision="fp32", exchange="per_g")
+    m_ops = make_rhf_meas_ops(sys_, precision="fp32")
     p_ops = make_chol_afqmc_ops(ham, "restricted", precision="fp32")
     params = afqmc_params(dt=DT, n_walkers=member.n_walkers)
     p_ctx = p_ops.build_prop_ctx(t_ops.get_rdm1(trial), DT)
