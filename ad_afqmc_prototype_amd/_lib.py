@@ -36,6 +36,8 @@ _SIGNATURES = {
                            c_vp, c_vp, c_vp, c_vp],
     "afq_trotter_bf16x3_sym": [c_int, c_int, c_int, c_int, c_int, c_double, c_vp, c_vp, c_vp, c_vp, c_vp,
                                c_vp, c_vp],
+    "afq_trotter_bf16x3_sym_halves": [c_int, c_int, c_int, c_int, c_int, c_double, c_vp, c_vp, c_vp, c_vp,
+                                      c_vp, c_vp, c_vp, c_vp],
     "afq_overlap_greens": [c_int, c_int, c_int, c_int, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp],
     "afq_overlap_greens_split": [c_int, c_int, c_int, c_int, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp,
                                  c_ll, c_vp],
@@ -95,7 +97,7 @@ _SIGNATURES = {
     "afq_unpack_pairs": [c_int, c_int, c_vp, c_ll, c_vp, c_vp],
 }
 
-ABI_VERSION = 18
+ABI_VERSION = 19
 
 
 class HipLibraryError(RuntimeError):

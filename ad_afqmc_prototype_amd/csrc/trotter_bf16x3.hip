@@ -183,7 +183,8 @@ __global__ __launch_bounds__(512, 1) void trotter_bf16x3_kernel(
     const float* __restrict__ E, const float* __restrict__ vhs, long long vplane,
     const double2* __restrict__ win_up, const double2* __restrict__ win_dn,
     double2* __restrict__ wout_up, double2* __restrict__ wout_dn, int group_map, int count = 0,
-    const afq_system_step* __restrict__ S = nullptr) {
+    const afq_system_step* __restrict__ S = nullptr, const float* __restrict__ Elast = nullptr,
+    int final_half = 1) {
     constexpr int KMAX = KR + NL;
     extern __shared__ __attribute__((aligned(16))) unsigned char lds_raw[];
     int b = blockIdx.x;
@@ -235,6 +236,7 @@ __global__ __launch_bounds__(512, 1) void trotter_bf16x3_kernel(
     const int nbv = NP / 16;
     const long long vrow = SYMV ? (long long)nbv * (nbv + 1) / 2 * 256 : (long long)n * ldv;
     const rsrc_t Er = make_rsrc(E, SYMV ? (unsigned)vrow * 4u : mat_bytes);
+    const rsrc_t ErL = make_rsrc(Elast ? Elast : E, SYMV ? (unsigned)vrow * 4u : mat_bytes);
     const rsrc_t Vr = make_rsrc(vhs + (long long)w * vrow, (unsigned)vrow * 4u);
     const rsrc_t Vi = make_rsrc(vhs + vplane + (long long)w * vrow, (unsigned)vrow * 4u);
 
@@ -255,17 +257,17 @@ __global__ __launch_bounds__(512, 1) void trotter_bf16x3_kernel(
     auto load_w = [&](int p, int j) {
         return load_d2(Wr, j < o ? (unsigned)(p * o + j) * 16u : kOutside);
     };
-    // all E fragments of this wave's rows, as raw fp32 (one round trip)
-    // (SYMV: E is given as its upper blocks too, and read like V)
-    auto load_e = [&](float (&er)[RT][KMAX][8], int r0, int k8) {
+    // all fragments of this wave's rows of the one-body matrix in src, as raw
+    // fp32 (one round trip) (SYMV: given as its upper blocks too, read like V)
+    auto load_e = [&](const rsrc_t& src, float (&er)[RT][KMAX][8], int r0, int k8) {
 #pragma unroll
         for (int t = 0; t < RT; ++t)
 #pragma unroll
             for (int kb = 0; kb < KMAX; ++kb) {
                 if constexpr (SYMV)
-                    load_frag_sym(Er, nbv, (r0 >> 4) + t, kb, r0 & 15, k8 >> 3, er[t][kb]);
+                    load_frag_sym(src, nbv, (r0 >> 4) + t, kb, r0 & 15, k8 >> 3, er[t][kb]);
                 else
-                    load_row8(Er, ldv, n, r0 + 16 * t, kb * 32 + k8, er[t][kb]);
+                    load_row8(src, ldv, n, r0 + 16 * t, kb * 32 + k8, er[t][kb]);
             }
     };
     // acc += (E - I) . T
@@ -334,7 +336,7 @@ __global__ __launch_bounds__(512, 1) void trotter_bf16x3_kernel(
                 wa[t][r] = load_w((wave * RT + t) * 16 + 4 * q + r, col);
             }
         float er[RT][KMAX][8];
-        load_e(er, row0, kq);
+        load_e(Er, er, row0, kq);
 #pragma unroll
         for (int t = 0; t < RT; ++t)
 #pragma unroll
@@ -503,14 +505,16 @@ __global__ __launch_bounds__(512, 1) void trotter_bf16x3_kernel(
         __syncthreads();
     }
 
-    // ---- W_out = E out = out + (E - I) out ----
+    // ---- W_out = E' out = out + (E' - I) out (E' = Elast, or E), or, with
+    // final_half == 0, W_out = out: the step's last one-body half is left to
+    // the next step, which applies it together with its own first half ----
     // (indices recomputed from an opaque lane id: nothing address-like stays
     // live across the Taylor loop)
     const int lane2 = opaque(lane);
     const int wave2 = opaque(wave);
-    {
+    if (final_half) {
         float er[RT][KMAX][8];
-        load_e(er, wave2 * RT * 16 + (lane2 & 15), 8 * (lane2 >> 4));
+        load_e(ErL, er, wave2 * RT * 16 + (lane2 & 15), 8 * (lane2 >> 4));
         __syncthreads();
         store_t(out);
         __syncthreads();
@@ -540,10 +544,14 @@ __global__ __launch_bounds__(512, 1) void trotter_bf16x3_kernel(
 // loads it twice per item).  Parked values are fp32 sums stored as fp64, so
 // reading them back is exact, and every product runs in the per-item kernel's
 // order: the results equal trotter_bf16x3_kernel's bit for bit.
-template <int NL>
+// MODE 1: both halves on E - I (as above).  MODE 0: no final halves (the sums
+// are the output; the next step applies this step's last half together with
+// its own first).  MODE 2: the final halves on Elast - I, and E - I loaded
+// again at the top of every walker (nothing carried between walkers).
+template <int NL, int MODE>
 __global__ __launch_bounds__(512, 1) void trotter_bf16x3_pair_kernel(
     int nw, int n, int o, int nspin, int nct, int n_terms, double sqrt_dt, const float* __restrict__ E,
-    const float* __restrict__ vhs, long long vplane, const double2* __restrict__ win_up,
+    const float* __restrict__ Elast, const float* __restrict__ vhs, long long vplane, const double2* __restrict__ win_up,
     const double2* __restrict__ win_dn, double2* __restrict__ wout_up, double2* __restrict__ wout_dn) {
     constexpr int KMAX = KR + NL;
     constexpr bool SYMV = true;
@@ -562,6 +570,7 @@ __global__ __launch_bounds__(512, 1) void trotter_bf16x3_pair_kernel(
     const int nbv = NP / 16;
     const long long vrow = SYMV ? (long long)nbv * (nbv + 1) / 2 * 256 : (long long)n * ldv;
     const rsrc_t Er = make_rsrc(E, SYMV ? (unsigned)vrow * 4u : mat_bytes);
+    const rsrc_t ErL = make_rsrc(MODE == 2 ? Elast : E, SYMV ? (unsigned)vrow * 4u : mat_bytes);
 
     const int tid = threadIdx.x;
     const int lane = tid & 63;
@@ -574,15 +583,15 @@ __global__ __launch_bounds__(512, 1) void trotter_bf16x3_pair_kernel(
     auto read_b = [&](int kb, int plane) {
         return *reinterpret_cast<const bf16x8*>(Tt + (plane * 16 + lr) * TP + kb * 32 + kq);
     };
-    auto load_e = [&](float (&er)[RT][KMAX][8], int r0, int k8) {
+    auto load_e = [&](const rsrc_t& src, float (&er)[RT][KMAX][8], int r0, int k8) {
 #pragma unroll
         for (int t = 0; t < RT; ++t)
 #pragma unroll
             for (int kb = 0; kb < KMAX; ++kb) {
                 if constexpr (SYMV)
-                    load_frag_sym(Er, nbv, (r0 >> 4) + t, kb, r0 & 15, k8 >> 3, er[t][kb]);
+                    load_frag_sym(src, nbv, (r0 >> 4) + t, kb, r0 & 15, k8 >> 3, er[t][kb]);
                 else
-                    load_row8(Er, ldv, n, r0 + 16 * t, kb * 32 + k8, er[t][kb]);
+                    load_row8(src, ldv, n, r0 + 16 * t, kb * 32 + k8, er[t][kb]);
             }
     };
     // acc += (E - I) . T
@@ -738,12 +747,17 @@ __global__ __launch_bounds__(512, 1) void trotter_bf16x3_pair_kernel(
     int w = blockIdx.x;
     load_cols(wa, w, 0);
     load_cols(wb, w, 1);
-    load_e(er, row0, kq);
+    if constexpr (MODE != 2) load_e(Er, er, row0, kq);
 #pragma unroll 1
     for (; w < nw; w += gridDim.x) {
         const int wn = w + gridDim.x;
         const rsrc_t out0 = out_rsrc(w, 0);
         const rsrc_t out1 = out_rsrc(w, 1);
+        if constexpr (MODE == 2) {
+            const int ln = opaque(lane);
+            const int wvo = opaque(wave);
+            load_e(Er, er, wvo * RT * 16 + (ln & 15), 8 * (ln >> 4));
+        }
         f32x4 out[RT][2];
         // ---- E W of both items ----
         {
@@ -902,30 +916,40 @@ __global__ __launch_bounds__(512, 1) void trotter_bf16x3_pair_kernel(
         const int wave2 = opaque(wave);
         const int r02 = wave2 * RT * 16 + (lane2 & 15);
         const int k82 = 8 * (lane2 >> 4);
-        load_e(er, r02, k82);
-        load_cols(wa, wn, 0);
-        load_cols(wb, wn, 1);
-        __syncthreads();
-        store_t(out);
-        __syncthreads();
-        e_product(out, er, r02, k82);
-        {
-            // (the load before the store: a load issued after a store waits
-            // for it, the counters being in order)
-            f32x4 nx[RT][2];
-            unpark(out0, 0, nx, wave2, lane2);
-            park(out1, 1, out, wave2, lane2);
+        if constexpr (MODE != 0) {
+            load_e(ErL, er, r02, k82);
+            load_cols(wa, wn, 0);
+            load_cols(wb, wn, 1);
+            __syncthreads();
+            store_t(out);
+            __syncthreads();
+            e_product(out, er, r02, k82);
+            {
+                // (the load before the store: a load issued after a store waits
+                // for it, the counters being in order)
+                f32x4 nx[RT][2];
+                unpark(out0, 0, nx, wave2, lane2);
+                park(out1, 1, out, wave2, lane2);
 #pragma unroll
-            for (int t = 0; t < RT; ++t) {
-                out[t][0] = nx[t][0];
-                out[t][1] = nx[t][1];
+                for (int t = 0; t < RT; ++t) {
+                    out[t][0] = nx[t][0];
+                    out[t][1] = nx[t][1];
+                }
             }
+            __syncthreads();
+            store_t(out);
+            __syncthreads();
+            e_product(out, er, r02, k82);
+            park(out0, 0, out, wave2, lane2);
+        } else {
+            // the first item's sum already sits in its output rows (parked
+            // after the first series); the second's goes to its rows, after
+            // the next walker's loads are issued (so they do not wait for it)
+            load_e(Er, er, r02, k82);
+            load_cols(wa, wn, 0);
+            load_cols(wb, wn, 1);
+            park(out1, 1, out, wave2, lane2);
         }
-        __syncthreads();
-        store_t(out);
-        __syncthreads();
-        e_product(out, er, r02, k82);
-        park(out0, 0, out, wave2, lane2);
     }
 }
 
@@ -936,10 +960,11 @@ size_t bf16x3_lds(int n, int nl) {
     return (size_t)4 * 16 * (nk * 32 + 8) * 2 + (size_t)nwave * RT * nl * 4 * 64 * 16;
 }
 
+// E: the first half's matrix; Elast: the final half's (nullptr: no final half)
 template <int NL, bool SYMV>
 int launch_bf16x3(int nw, int n, int o, int nspin, int ldv, int n_terms, double sqrt_dt,
-                  const float* E, const float* vhs, const double2* wu, const double2* wd, double2* ou,
-                  double2* od, hipStream_t st) {
+                  const float* E, const float* Elast, const float* vhs, const double2* wu, const double2* wd,
+                  double2* ou, double2* od, hipStream_t st) {
     const int NP = afq::round_up(n, 16);
     const int nct = afq::ceil_div(o, 16);
     const int nwave = afq::ceil_div(NP / 16, RT);
@@ -953,31 +978,44 @@ int launch_bf16x3(int nw, int n, int o, int nspin, int ldv, int n_terms, double 
     if constexpr (SYMV) {
         if (nspin * nct == 2 && nw >= afq::cu_count()) {
             pair = true;
-            if (lds > 64 * 1024) afq::allow_full_lds((const void*)trotter_bf16x3_pair_kernel<NL>);
-            hipLaunchKernelGGL((trotter_bf16x3_pair_kernel<NL>), dim3((unsigned)afq::cu_count()), dim3(64 * nwave),
-                               lds, st, nw, n, o, nspin, nct, n_terms, sqrt_dt, E, vhs, vplane, wu, wd, ou, od);
+            const dim3 grid((unsigned)afq::cu_count()), block(64 * nwave);
+#define AFQ_PAIR(M)                                                                                             \
+    do {                                                                                                        \
+        if (lds > 64 * 1024) afq::allow_full_lds((const void*)trotter_bf16x3_pair_kernel<NL, M>);               \
+        hipLaunchKernelGGL((trotter_bf16x3_pair_kernel<NL, M>), grid, block, lds, st, nw, n, o, nspin, nct,     \
+                           n_terms, sqrt_dt, E, Elast, vhs, vplane, wu, wd, ou, od);                            \
+    } while (0)
+            if (!Elast)
+                AFQ_PAIR(0);
+            else if (Elast == E)
+                AFQ_PAIR(1);
+            else
+                AFQ_PAIR(2);
+#undef AFQ_PAIR
         }
     }
     if (!pair) {
         if (lds > 64 * 1024) afq::allow_full_lds((const void*)trotter_bf16x3_kernel<NL, SYMV>);
         hipLaunchKernelGGL((trotter_bf16x3_kernel<NL, SYMV>), dim3((unsigned)(nw * nspin * nct)), dim3(64 * nwave),
                            lds, st, nw, n, o, nspin, nct, ldv, n_terms, sqrt_dt, E, vhs, vplane, wu, wd, ou, od,
-                           group_map);
+                           group_map, 0, nullptr, Elast, Elast ? 1 : 0);
     }
     return afq::check_launch(SYMV ? "afq_trotter_bf16x3_sym" : "afq_trotter_bf16x3");
 }
 
 template <bool SYMV>
 int launch_bf16x3_nl(int nl, int nw, int n, int o, int nspin, int ldv, int n_terms, double sqrt_dt,
-                     const float* E, const float* vhs, const double* win_up, const double* win_dn,
-                     double* wout_up, double* wout_dn, hipStream_t st) {
+                     const float* E, const float* Elast, const float* vhs, const double* win_up,
+                     const double* win_dn, double* wout_up, double* wout_dn, hipStream_t st) {
     const double2* wu = (const double2*)win_up;
     const double2* wd = (const double2*)win_dn;
     double2* ou = (double2*)wout_up;
     double2* od = (double2*)wout_dn;
-    if (nl == 0) return launch_bf16x3<0, SYMV>(nw, n, o, nspin, ldv, n_terms, sqrt_dt, E, vhs, wu, wd, ou, od, st);
-    if (nl == 1) return launch_bf16x3<1, SYMV>(nw, n, o, nspin, ldv, n_terms, sqrt_dt, E, vhs, wu, wd, ou, od, st);
-    return launch_bf16x3<2, SYMV>(nw, n, o, nspin, ldv, n_terms, sqrt_dt, E, vhs, wu, wd, ou, od, st);
+    if (nl == 0)
+        return launch_bf16x3<0, SYMV>(nw, n, o, nspin, ldv, n_terms, sqrt_dt, E, Elast, vhs, wu, wd, ou, od, st);
+    if (nl == 1)
+        return launch_bf16x3<1, SYMV>(nw, n, o, nspin, ldv, n_terms, sqrt_dt, E, Elast, vhs, wu, wd, ou, od, st);
+    return launch_bf16x3<2, SYMV>(nw, n, o, nspin, ldv, n_terms, sqrt_dt, E, Elast, vhs, wu, wd, ou, od, st);
 }
 
 // the checks both entry points share; the k-blocks of V kept in LDS in *nl
@@ -1013,9 +1051,29 @@ extern "C" int afq_trotter_bf16x3(int nw, int n, int o, int nspin, int ldv, int 
     if (nl > NL_MAX || bf16x3_lds(n, nl) > 160 * 1024)
         return afq_trotter_f32(nw, n, o, nspin, ldv, n_terms, sqrt_dt, exp_h1_half, vhs, win_up, win_dn,
                                wout_up, wout_dn, stream);
-    return launch_bf16x3_nl<false>(nl, nw, n, o, nspin, ldv, n_terms, sqrt_dt, exp_h1_half, vhs, win_up, win_dn,
-                                   wout_up, wout_dn, (hipStream_t)stream);
+    return launch_bf16x3_nl<false>(nl, nw, n, o, nspin, ldv, n_terms, sqrt_dt, exp_h1_half, exp_h1_half, vhs,
+                                   win_up, win_dn, wout_up, wout_dn, (hipStream_t)stream);
 }
+
+namespace {
+
+int trotter_bf16x3_sym(const char* name, int nw, int n, int o, int nspin, int n_terms, double sqrt_dt,
+                       const unsigned* e_first, const unsigned* e_last, const unsigned* vhs_sym,
+                       const double* win_up, const double* win_dn, double* wout_up, double* wout_dn,
+                       void* stream) {
+    int nl = 0;
+    const int ldv = afq::round_up(n, 16);  // unused by this layout
+    const int rc = check_bf16x3(name, nw, n, o, nspin, ldv, n_terms, (const float*)e_first,
+                                (const float*)vhs_sym, win_up, win_dn, wout_up, wout_dn, &nl);
+    if (rc != 0) return rc;
+    AFQ_REQUIRE(nl <= NL_MAX && bf16x3_lds(n, nl) <= 160 * 1024, "%s: n=%d exceeds the on-chip maximum of 224",
+                name, n);
+    return launch_bf16x3_nl<true>(nl, nw, n, o, nspin, ldv, n_terms, sqrt_dt, (const float*)e_first,
+                                  (const float*)e_last, (const float*)vhs_sym, win_up, win_dn, wout_up, wout_dn,
+                                  (hipStream_t)stream);
+}
+
+}  // namespace
 
 // Declared in afqmc_hip.h: V and E as upper 16 x 16 blocks; n <= 224 only
 // (there is no fp32 kernel for this layout).
@@ -1023,16 +1081,18 @@ extern "C" int afq_trotter_bf16x3_sym(int nw, int n, int o, int nspin, int n_ter
                                       const unsigned* exp_h1_half_sym, const unsigned* vhs_sym,
                                       const double* win_up, const double* win_dn, double* wout_up,
                                       double* wout_dn, void* stream) {
-    int nl = 0;
-    const int ldv = afq::round_up(n, 16);  // unused by this layout
-    const int rc = check_bf16x3("afq_trotter_bf16x3_sym", nw, n, o, nspin, ldv, n_terms,
-                                (const float*)exp_h1_half_sym, (const float*)vhs_sym, win_up, win_dn, wout_up,
-                                wout_dn, &nl);
-    if (rc != 0) return rc;
-    AFQ_REQUIRE(nl <= NL_MAX && bf16x3_lds(n, nl) <= 160 * 1024,
-                "afq_trotter_bf16x3_sym: n=%d exceeds the on-chip maximum of 224", n);
-    return launch_bf16x3_nl<true>(nl, nw, n, o, nspin, ldv, n_terms, sqrt_dt, (const float*)exp_h1_half_sym,
-                                  (const float*)vhs_sym, win_up, win_dn, wout_up, wout_dn, (hipStream_t)stream);
+    return trotter_bf16x3_sym("afq_trotter_bf16x3_sym", nw, n, o, nspin, n_terms, sqrt_dt, exp_h1_half_sym,
+                              exp_h1_half_sym, vhs_sym, win_up, win_dn, wout_up, wout_dn, stream);
+}
+
+// Declared in afqmc_hip.h: the same step with its first half on e_first and
+// its last on e_last (NULL: no last half).
+extern "C" int afq_trotter_bf16x3_sym_halves(int nw, int n, int o, int nspin, int n_terms, double sqrt_dt,
+                                             const unsigned* e_first, const unsigned* e_last,
+                                             const unsigned* vhs_sym, const double* win_up, const double* win_dn,
+                                             double* wout_up, double* wout_dn, void* stream) {
+    return trotter_bf16x3_sym("afq_trotter_bf16x3_sym_halves", nw, n, o, nspin, n_terms, sqrt_dt, e_first, e_last,
+                              vhs_sym, win_up, win_dn, wout_up, wout_dn, stream);
 }
 
 namespace {

@@ -398,13 +398,20 @@ def vhs(xbar, chol_t, n, vhs_out=None):
 BF16X3_SYM_MAX_N = 224  # on-chip capacity of the bf16x3 Trotter
 
 
-def trotter(walkers, vhs_planes, exp_h1_half_pad, sqrt_dt, n_terms, out=None, arith=None):
+_SAME = object()
+
+
+def trotter(walkers, vhs_planes, exp_h1_half_pad, sqrt_dt, n_terms, out=None, arith=None, e_last=_SAME):
     """Fused Trotter step; fp64 when vhs / exp_h1_half are float64.  With
     float32 operands ``arith`` picks the products: "fp32" (default) or
     "bf16x3" (split-bf16 MFMAs, fp32 sums).  Walkers are complex128 either way.
     bf16x3 also takes V and E - I as upper 16 x 16 blocks of split-bf16 pair
     words: vhs_planes (2, nw, sym_blocks(n) * 256) and exp_h1_half_pad
-    (sym_blocks(n) * 256,) int32 (n <= 224)."""
+    (sym_blocks(n) * 256,) int32 (n <= 224).  In that layout ``e_last`` may
+    name another pair-word matrix F - I for the last one-body half (the first
+    uses exp_h1_half_pad), or None to leave the last half out
+    (afq_trotter_bf16x3_sym_halves); launches without a last half are timed
+    under the tag "deferred"."""
     wu, wd = (walkers if isinstance(walkers, tuple) else (walkers, None))
     nspin = 1 if wd is None else 2
     nw, n, o = wu.shape
@@ -424,10 +431,19 @@ def trotter(walkers, vhs_planes, exp_h1_half_pad, sqrt_dt, n_terms, out=None, ar
         if out is None:
             out = (torch.empty_like(wu), torch.empty_like(wd)) if nspin == 2 else torch.empty_like(wu)
         ou, od = (out if isinstance(out, tuple) else (out, None))
-        _call("afq_trotter_bf16x3_sym", wu, nw, n, o, nspin, int(n_terms), float(sqrt_dt),
-              exp_h1_half_pad.data_ptr(), vhs_planes.data_ptr(), wu.data_ptr(), _ptr(wd), ou.data_ptr(),
-              _ptr(od), _stream(wu))
+        if e_last is _SAME or e_last is exp_h1_half_pad:
+            _call("afq_trotter_bf16x3_sym", wu, nw, n, o, nspin, int(n_terms), float(sqrt_dt),
+                  exp_h1_half_pad.data_ptr(), vhs_planes.data_ptr(), wu.data_ptr(), _ptr(wd), ou.data_ptr(),
+                  _ptr(od), _stream(wu))
+            return out
+        if e_last is not None:
+            _check(e_last, torch.int32, (sym_blocks(n) * 256,), "e_last (upper-block pairs)", dev)
+        _call("afq_trotter_bf16x3_sym_halves", wu, nw, n, o, nspin, int(n_terms), float(sqrt_dt),
+              exp_h1_half_pad.data_ptr(), _ptr(e_last), vhs_planes.data_ptr(), wu.data_ptr(), _ptr(wd),
+              ou.data_ptr(), _ptr(od), _stream(wu), tag="deferred" if e_last is None else None)
         return out
+    if e_last is not _SAME:
+        raise ValueError("separate one-body halves need the upper-block pair-word (bf16x3) operands")
     if rt not in (f64, f32):
         raise TypeError("vhs must be float64 or float32")
     if arith is None:

@@ -133,6 +133,33 @@ def _nspin(walkers) -> int:
     return 2 if isinstance(walkers, tuple) else 1
 
 
+def deferred_view(meas_ctx: rhf_meas_ctx, trial_data, e_half: torch.Tensor):
+    """(trial, meas ctx) for walkers held as E^-1 W (a step that left out its
+    final one-body half E, symmetric; prop_state.deferred).  The overlap matrix
+    C^T W = (E C)^T (E^-1 W), so the trial becomes E C: its log det and the
+    rows G' = ((E^-1 W) m^-1)^T = G E^-1 are what the Green's function kernels
+    compute from the held walkers.  Every contraction of G with a half-rotated
+    operand X (o x n) is one with X E of G': sum_ij X_ij G_ij = sum_ij (X E)_ij
+    G'_ij, so the context's rot_h1 and rot_chol become rot_h1 E and
+    rot_chol_g E (force bias, energy).  Built once per (context, trial, E)."""
+    key = ("deferred", id(trial_data), id(e_half))
+    hit = meas_ctx._cache.get(key)
+    if hit is not None and hit[0] is trial_data and hit[1] is e_half:
+        return hit[2], hit[3]
+    e = e_half.to(meas_ctx.rot_chol.dtype)
+    if trial_data.restricted:
+        trial_e = trial_rhf.rhf_trial(e @ trial_data.mo_coeff)
+    else:
+        trial_e = trial_rhf.uhf_trial(e @ trial_data.mo_up, e @ trial_data.mo_dn)
+    rot_h1 = (meas_ctx.rot_h1 @ e).contiguous()
+    rot_chol = (meas_ctx.rot_chol @ e).contiguous()
+    flat = rot_chol[0].reshape(rot_chol.shape[1], -1)
+    ctx_e = rhf_meas_ctx(rot_h1=rot_h1, rot_chol=rot_chol, rot_chol_flat=flat,
+                         precision=meas_ctx.precision, exchange=meas_ctx.exchange)
+    meas_ctx._cache[key] = (trial_data, e_half, trial_e, ctx_e)
+    return trial_e, ctx_e
+
+
 def force_bias_planes(walkers, meas_ctx: rhf_meas_ctx, trial_data, greens=None) -> torch.Tensor:
     """Force bias as split planes (2, nw, naux)."""
     nspin = _nspin(walkers)

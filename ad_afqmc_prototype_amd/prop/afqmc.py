@@ -16,6 +16,7 @@ from .. import walkers as wk
 from ..core.ops import k_energy, k_force_bias, meas_ops, trial_ops
 from ..core.system import system
 from ..ham.chol import ham_chol
+from ..meas.rhf import deferred_view, rhf_meas_ctx
 from .chol_afqmc_ops import chol_afqmc_ctx, chol_afqmc_ops
 from .types import afqmc_params, prop_state
 
@@ -86,14 +87,39 @@ def _force_bias_planes(state: prop_state, ham_data, trial_data, meas_ops_: meas_
     return fb
 
 
+def can_merge_halves(prop_ops: chol_afqmc_ops, meas_ops_: meas_ops, meas_ctx: Any, trial_data: Any) -> bool:
+    """Whether afqmc_step can leave a step's final one-body half to the next
+    step: the propagator takes its halves separately (the bf16x3 upper-block
+    Trotter) and the measurement is the RHF/UHF one with real trial orbitals
+    and a step Green's function (restricted or unrestricted walkers)."""
+    return (bool(getattr(prop_ops, "merges_halves", False)) and isinstance(meas_ctx, rhf_meas_ctx)
+            and not meas_ctx.is_complex and meas_ops_.step_greens is not None
+            and not bool(getattr(trial_data, "is_complex", False)))
+
+
 def afqmc_step(state: prop_state, *, params: afqmc_params, ham_data: ham_chol, trial_data: Any,
                meas_ops: meas_ops, prop_ops: chol_afqmc_ops, prop_ctx: chol_afqmc_ctx,
-               meas_ctx: Any, fields: torch.Tensor | None = None) -> prop_state:
+               meas_ctx: Any, fields: torch.Tensor | None = None, defer_final: bool = False) -> prop_state:
     """prop/afqmc.py:74-147.  ``fields`` (nw, n_fields) float64 may be injected;
     otherwise they are drawn from ``state.rng_key``.  Pure: ``state`` is not
-    modified, every output is a new tensor."""
+    modified, every output is a new tensor.
+
+    ``defer_final``: for a step that another step follows, leave out the final
+    exp(-dt h1_eff / 2) where can_merge_halves holds; the returned state is
+    ``deferred`` (walkers E^-1 W, Green's functions for the trial E C, log
+    overlaps and weights those of W), and the next step applies exp(-dt h1_eff)
+    before its Taylor series: E (T E W) and E E W' in one product.  Elsewhere
+    the flag does nothing."""
     nw = wk.n_walkers(state.walkers)
     device = state.weights.device
+    merge = can_merge_halves(prop_ops, meas_ops, meas_ctx, trial_data)
+    deferred_in = bool(state.deferred)
+    if deferred_in and not merge:
+        raise ValueError("a deferred state (walkers without their last one-body half) needs the "
+                         "propagator that merged the halves")
+    defer_out = bool(defer_final) and merge
+    if deferred_in or defer_out:
+        trial_d, meas_ctx_d = deferred_view(meas_ctx, trial_data, prop_ctx.exp_h1_half)
     if fields is None:
         fields, key = draw_fields(state.rng_key, nw, prop_ops.n_fields(), device)
     else:
@@ -103,20 +129,24 @@ def afqmc_step(state: prop_state, *, params: afqmc_params, ham_data: ham_chol, t
             fields = fields.real
         fields = fields.to(torch.float64).contiguous()
 
-    fb = _force_bias_planes(state, ham_data, trial_data, meas_ops, meas_ctx, params.n_chunks)
+    fb = _force_bias_planes(state, ham_data, trial_d if deferred_in else trial_data, meas_ops,
+                            meas_ctx_d if deferred_in else meas_ctx, params.n_chunks)
+    halves = {"merged_first": True} if deferred_in else {}
+    if defer_out:
+        halves["last_half"] = False
     if getattr(prop_ops, "xbar_split", False) and params.n_chunks == 1:
         # the bf16x3 VHS takes xbar as bf16 planes, made in the same pass
         xbar, shift_term, fb_term, xs = kernels.field_shifts(fields, fb, prop_ctx.mf_shifts, prop_ctx.sqrt_dt,
                                                              split=True)
         walkers_new = prop_ops.apply_trotter_planes(state.walkers, xbar, prop_ctx, params.n_exp_terms,
-                                                    params.n_chunks, xbar_split=xs)
+                                                    params.n_chunks, xbar_split=xs, **halves)
     else:
         xbar, shift_term, fb_term = kernels.field_shifts(fields, fb, prop_ctx.mf_shifts, prop_ctx.sqrt_dt)
         walkers_new = prop_ops.apply_trotter_planes(state.walkers, xbar, prop_ctx, params.n_exp_terms,
-                                                    params.n_chunks)
+                                                    params.n_chunks, **halves)
     if meas_ops.greens is not None:
         make_greens = meas_ops.step_greens or meas_ops.greens
-        greens_new = make_greens(walkers_new, trial_data)
+        greens_new = make_greens(walkers_new, trial_d if defer_out else trial_data)
         logdet = greens_new.logdet
     else:
         greens_new = None
@@ -131,4 +161,4 @@ def afqmc_step(state: prop_state, *, params: afqmc_params, ham_data: ham_chol, t
         damping=float(getattr(params, "pop_control_damping", 0.1)))
     return prop_state(walkers=walkers_new, weights=w_new, log_overlaps=lo_new, rng_key=key,
                       pop_control_ene_shift=pop_new, e_estimate=state.e_estimate,
-                      greens=greens_new)
+                      greens=greens_new, deferred=defer_out)

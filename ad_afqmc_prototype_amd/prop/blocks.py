@@ -32,8 +32,15 @@ def draw_zeta(key: torch.Tensor):
     return kernels.rng_uniform(sub), key_next
 
 
+def _require_whole(state: prop_state, what: str) -> None:
+    if state.deferred:
+        raise ValueError(f"{what}: the state's walkers lack their last one-body half "
+                         "(afqmc_step(defer_final=True)); only another afqmc_step reads it")
+
+
 def orthonormalize_state(state: prop_state, *, sys: system, trial_data, meas_ops_: meas_ops) -> prop_state:
     """walkers.py:206-211 + overlap refresh (prop/blocks.py:53-57)."""
+    _require_whole(state, "orthonormalize_state")
     w = wk.orthonormalize(state.walkers, sys.walker_kind)
     lo, greens = _log_overlap_and_greens(w, meas_ops_, trial_data)
     return state._replace(walkers=w, log_overlaps=lo, greens=greens)
@@ -43,6 +50,7 @@ def measure(state: prop_state, *, params: afqmc_params, ham_data: ham_chol, tria
             meas_ops: meas_ops, meas_ctx: Any):
     """prop/blocks.py:59-79 on orthonormal walkers.  Returns (state with the
     population shift updated, {"energy", "weight"}, per-walker energies)."""
+    _require_whole(state, "measure")
     e_kernel = meas_ops.require_kernel(k_energy)
     e = e_kernel(state.walkers, ham_data, meas_ctx, trial_data, greens=state.greens)
     thresh = (2.0 / float(params.dt)) ** 0.5
@@ -60,6 +68,7 @@ def reconfigure(state: prop_state, *, sys: system, trial_data, meas_ops_: meas_o
     the overlaps of the gathered walkers, which gives the same values).
     zeta: a float, a 0-d float64 device tensor, or None (drawn on the device
     from the state's key)."""
+    _require_whole(state, "reconfigure")
     if zeta is None:
         zeta, key = draw_zeta(state.rng_key)
     else:
@@ -89,14 +98,20 @@ def afqmc_block(state: prop_state, *, sys: system, params: afqmc_params, ham_dat
                 prop_ctx: chol_afqmc_ctx, meas_ctx: Any, fields=None, zeta: float | None = None,
                 return_indices: bool = False):
     """prop/blocks.py:22-95.  ``fields`` optionally injects the per-step fields
-    (indexable by step), ``zeta`` the comb uniform."""
+    (indexable by step), ``zeta`` the comb uniform.  With
+    params.merge_one_body_halves, a step followed by another step leaves its
+    final one-body half to it (afqmc_step(defer_final=True)); the steps before
+    a QR and the block's last step apply theirs."""
     n_ortho = params.n_ortho
+    merge = bool(getattr(params, "merge_one_body_halves", False))
     for t in range(params.n_prop_steps):
         f = None if fields is None else fields[t]
+        qr_next = bool(n_ortho) and (t + 1) % n_ortho == 0
         state = afqmc_step(state, params=params, ham_data=ham_data, trial_data=trial_data,
                            meas_ops=meas_ops, prop_ops=prop_ops, prop_ctx=prop_ctx,
-                           meas_ctx=meas_ctx, fields=f)
-        if n_ortho and (t + 1) % n_ortho == 0 and t + 1 < params.n_prop_steps:
+                           meas_ctx=meas_ctx, fields=f,
+                           defer_final=merge and not qr_next and t + 1 < params.n_prop_steps)
+        if qr_next and t + 1 < params.n_prop_steps:
             state = orthonormalize_state(state, sys=sys, trial_data=trial_data, meas_ops_=meas_ops)
     state = orthonormalize_state(state, sys=sys, trial_data=trial_data, meas_ops_=meas_ops)
     state, scalars, _ = measure(state, params=params, ham_data=ham_data, trial_data=trial_data,

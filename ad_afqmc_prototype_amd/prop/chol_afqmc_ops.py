@@ -33,6 +33,9 @@ class chol_afqmc_ctx:
     # exp(-dt h1_eff / 2) - I as upper 16 x 16 blocks of split-bf16 pair words
     # (int32, split_pairs of the float32 entries): the bf16x3 Trotter's E operand
     exp_h1_half_sym_pairs: torch.Tensor = None
+    # E^2 - I (E^2 = exp(-dt h1_eff), the product of two halves in fp64) in the
+    # same layout: the first half of a step after one that left out its last
+    exp_h1_sym_pairs: torch.Tensor = None
 
 
 class chol_afqmc_ops(NamedTuple):
@@ -44,6 +47,9 @@ class chol_afqmc_ops(NamedTuple):
     # the VHS contraction takes xbar as bf16 planes: the step asks the field
     # shift kernel for them (apply_trotter_planes(..., xbar_split=(hi, lo)))
     xbar_split: bool = False
+    # apply_trotter_planes takes merged_first / last_half: a step may leave out
+    # its final one-body half and the next apply E^2 as its first
+    merges_halves: bool = False
 
 
 def _as_total_rdm1_restricted(dm: torch.Tensor) -> torch.Tensor:
@@ -99,10 +105,13 @@ def _build_prop_ctx(ham_data: ham_chol, trial_rdm1, dt: float) -> chol_afqmc_ctx
     ldv = kernels.ldv_for(n)
     pad = torch.zeros((n, ldv), dtype=torch.float64, device=chol.device)
     pad[:, :n] = exp_h1_half
+    exp_h1 = exp_h1_half @ exp_h1_half
+    exp_h1 = 0.5 * (exp_h1 + exp_h1.T)
     return chol_afqmc_ctx(dt=float(dt), sqrt_dt=float(dt) ** 0.5, exp_h1_half=exp_h1_half,
                           mf_shifts=mf.contiguous(), h0_prop=h0_prop, exp_h1_half_pad=pad,
                           exp_h1_half_pad32=pad.to(torch.float32),
-                          exp_h1_half_sym_pairs=minus_identity_pairs(upper_blocks(exp_h1_half).to(torch.float32), n))
+                          exp_h1_half_sym_pairs=minus_identity_pairs(upper_blocks(exp_h1_half).to(torch.float32), n),
+                          exp_h1_sym_pairs=minus_identity_pairs(upper_blocks(exp_h1).to(torch.float32), n))
 
 
 def chol_transposed(chol: torch.Tensor) -> torch.Tensor:
@@ -200,6 +209,8 @@ def make_chol_afqmc_ops(ham_data: ham_chol, walker_kind: str,
     # bf16x3 (n <= 224): the Trotter reads V as the upper blocks themselves,
     # so the contraction's output is used as it comes (0.58 of the bytes)
     v_blocks = precision == "bf16x3" and n <= kernels.BF16X3_SYM_MAX_N
+    # the upper-block Trotter takes its two one-body halves separately
+    merges = v_blocks and not generalized
 
     def make_vhs(xbar: torch.Tensor, xbar_split=None) -> torch.Tensor:
         """VHS planes in the step's precision, from the symmetric contraction
@@ -230,31 +241,42 @@ def make_chol_afqmc_ops(ham_data: ham_chol, walker_kind: str,
         return _build_prop_ctx(ham_data, trial_rdm1, dt)
 
     def apply_trotter_planes(w, xbar: torch.Tensor, ctx: chol_afqmc_ctx, n_terms: int,
-                             n_chunks: int = 1, xbar_split=None):
+                             n_chunks: int = 1, xbar_split=None, merged_first: bool = False,
+                             last_half: bool = True):
         """``xbar_split``: optional (hi, lo) bf16 planes of xbar's rows (from
-        kernels.field_shifts(split=True)), used by the bf16x3 VHS when n_chunks == 1."""
+        kernels.field_shifts(split=True)), used by the bf16x3 VHS when n_chunks == 1.
+        ``merged_first``: the first one-body factor is E^2 (the previous step
+        left out its last half); ``last_half=False`` leaves out this step's last
+        half (W_out = Taylor . E w).  Only where ``merges_halves``."""
+        if (merged_first or not last_half) and not merges:
+            raise ValueError("this propagator applies both one-body halves in every step")
+        halves = (ctx.exp_h1_sym_pairs if merged_first else None, last_half)
         if generalized:
             # _apply_trotter_g_from_restricted (prop/chol_afqmc_ops.py:200-212): the
             # spin blocks of each (2n, o') walker with the walker's one VHS
             top, bot = apply_spin_blocks((w[:, :n, :].contiguous(), w[:, n:, :].contiguous()), xbar,
                                          ctx, n_terms, n_chunks, xbar_split)
             return torch.cat([top, bot], dim=1)
-        return apply_spin_blocks(w, xbar, ctx, n_terms, n_chunks, xbar_split)
+        return apply_spin_blocks(w, xbar, ctx, n_terms, n_chunks, xbar_split, halves)
 
     def apply_spin_blocks(w, xbar: torch.Tensor, ctx: chol_afqmc_ctx, n_terms: int, n_chunks: int,
-                          xbar_split=None):
+                          xbar_split=None, halves=(None, True)):
         unr = isinstance(w, tuple)
         w0 = w[0] if unr else w
         nw = w0.shape[0]
+        e = e_half(ctx)
+        e_first = e if halves[0] is None else halves[0]
+        # the last half: E (the same operand: the plain launch), or none
+        kw = {} if (halves[0] is None and halves[1]) else {"e_last": e if halves[1] else None}
         if n_chunks == 1:
-            return kernels.trotter(w, make_vhs(xbar, xbar_split), e_half(ctx), ctx.sqrt_dt, n_terms,
-                                   arith=precision)
+            return kernels.trotter(w, make_vhs(xbar, xbar_split), e_first, ctx.sqrt_dt, n_terms,
+                                   arith=precision, **kw)
         out = (torch.empty_like(w[0]), torch.empty_like(w[1])) if unr else torch.empty_like(w0)
         for a, b in _chunks(nw, n_chunks):
             v = make_vhs(xbar[:, a:b].contiguous())
             wc = (w[0][a:b], w[1][a:b]) if unr else w0[a:b]
             oc = (out[0][a:b], out[1][a:b]) if unr else out[a:b]
-            kernels.trotter(wc, v, e_half(ctx), ctx.sqrt_dt, n_terms, out=oc, arith=precision)
+            kernels.trotter(wc, v, e_first, ctx.sqrt_dt, n_terms, out=oc, arith=precision, **kw)
         return out
 
     def apply_trotter(w, fields, ctx: chol_afqmc_ctx, n_terms: int, n_chunks: int = 1):
@@ -266,4 +288,5 @@ def make_chol_afqmc_ops(ham_data: ham_chol, walker_kind: str,
             xbar = torch.stack([f, torch.zeros_like(f)]).to(torch.float64).contiguous()
         return apply_trotter_planes(w, xbar, ctx, n_terms, n_chunks)
 
-    return chol_afqmc_ops(n_fields_, build_prop_ctx, apply_trotter, apply_trotter_planes, precision, v_blocks)
+    return chol_afqmc_ops(n_fields_, build_prop_ctx, apply_trotter, apply_trotter_planes, precision, v_blocks,
+                          merges)

@@ -16,7 +16,10 @@ class prop_state(NamedTuple):
     finite), ``pop_control_ene_shift`` is a 0-d device tensor (so a step never
     synchronizes with the host), ``e_estimate`` is a Python float, and
     ``greens`` caches the Green's functions / log-determinants of ``walkers``
-    (None when they must be recomputed)."""
+    (None when they must be recomputed).  ``deferred``: the last step left out
+    its final one-body half (afqmc_step(defer_final=True)), so ``walkers`` holds
+    E^-1 W (E = exp(-dt h1_eff / 2)) and ``greens`` the matching G E^-1; only
+    the next afqmc_step reads such a state (it applies E^2 first)."""
 
     walkers: Any
     weights: torch.Tensor
@@ -25,6 +28,7 @@ class prop_state(NamedTuple):
     pop_control_ene_shift: torch.Tensor
     e_estimate: float
     greens: Optional[Any] = None
+    deferred: bool = False
 
     @property
     def overlaps(self) -> torch.Tensor:
@@ -35,7 +39,11 @@ class prop_state(NamedTuple):
 class afqmc_params:
     """Same knobs and defaults as the reference, plus ``n_ortho``: orthonormalize
     every n_ortho steps inside a block (None: only at the end of the block, as
-    in prop/blocks.py:53)."""
+    in prop/blocks.py:53), and ``merge_one_body_halves``: inside a block, a step
+    followed by another step leaves its final exp(-dt h1_eff / 2) to the next
+    one, which applies exp(-dt h1_eff) once (where the propagator supports it:
+    the bf16x3 upper-block Trotter; the walkers after the block are the same
+    up to rounding)."""
 
     dt: float = 0.005
     n_chunks: int = 1
@@ -50,3 +58,4 @@ class afqmc_params:
     n_walkers: int = 200
     seed: int = 42
     n_ortho: Optional[int] = None
+    merge_one_body_halves: bool = True

@@ -78,7 +78,10 @@ def build(workload: str, device, precision: str = "fp64", chol_scale: float | No
 def run_steps(ctx, state, fields, t0, k, cadence=True, energies=None, comb=None):
     """k steps starting at global step index t0 with the QR / measurement +
     reconfiguration cadence of the workload (block energies appended to
-    ``energies`` as device scalars, comb indices to ``comb``)."""
+    ``energies`` as device scalars, comb indices to ``comb``).  As in
+    afqmc_block, a step followed by another step leaves its final one-body
+    half to it where the propagator merges halves (params.merge_one_body_halves);
+    the state returned has all its halves."""
     from ad_afqmc_prototype_amd import synthetic as syn
     from ad_afqmc_prototype_amd.prop.afqmc import afqmc_step
     from ad_afqmc_prototype_amd.prop.blocks import measure, orthonormalize_state, reconfigure
@@ -86,9 +89,12 @@ def run_steps(ctx, state, fields, t0, k, cadence=True, energies=None, comb=None)
     cfg = ctx["cfg"]
     kw = dict(ham_data=ctx["ham"], trial_data=ctx["trial"], meas_ops=ctx["m_ops"],
               prop_ops=ctx["p_ops"], prop_ctx=ctx["p_ctx"], meas_ctx=ctx["m_ctx"])
+    merge = bool(getattr(ctx["params"], "merge_one_body_halves", False))
     for i in range(k):
         t = t0 + i
-        state = afqmc_step(state, params=ctx["params"], fields=fields[t % len(fields)], **kw)
+        block_end = cadence and ((t + 1) % cfg["sr_every"] == 0 or (t + 1) % cfg["qr_every"] == 0)
+        state = afqmc_step(state, params=ctx["params"], fields=fields[t % len(fields)],
+                           defer_final=merge and i + 1 < k and not block_end, **kw)
         if not cadence:
             continue
         if (t + 1) % cfg["sr_every"] == 0:
@@ -444,6 +450,9 @@ def kernel_table(ktimes, n, o, g, nw, nspin, rhf, precision):
         "afq_trotter_f32": (taylor, "fp32"),
         "afq_trotter_bf16x3": (taylor, "bf16x3"),
         "afq_trotter_bf16x3_sym": (taylor, "bf16x3"),
+        "afq_trotter_bf16x3_sym_halves": (taylor, "bf16x3"),
+        # without the last one-body half: the Taylor series and one half
+        "afq_trotter_bf16x3_sym_halves:deferred": ((fl["taylor"] + fl["one_body"] / 2) * nw, "bf16x3"),
         "afq_gemm_bf16x3_nt:vhs": (fl["vhs"] * nw, "bf16x3"),
         "afq_overlap_greens": (fl["greens"] * nw, "fp64"),
         "afq_energy": (4.0 * g * o * o * n * nspin * nw, "fp64"),

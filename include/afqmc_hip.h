@@ -293,6 +293,24 @@ int afq_trotter_bf16x3_sym(int nw, int n, int o, int nspin, int n_terms, double 
                            const double* win_dn, double* wout_up, double* wout_dn, void* stream);
 
 /*
+ * afq_trotter_bf16x3_sym with the two one-body halves given separately:
+ *   W_out = F_last . Taylor_m(i sqrt_dt V) . F_first . W_in,
+ * e_first = F_first - I and e_last = F_last - I in the same upper-block
+ * pair-word layout; e_last NULL leaves out the last half (W_out = Taylor .
+ * F_first . W_in).  Consecutive steps of prop/chol_afqmc_ops.py:173-196 apply
+ * E . E between two Taylor series; with e_last NULL on one step and e_first =
+ * E^2 - I (E^2 = exp(-dt h1)) on the next, the pair becomes one product.  A
+ * walker left without its last half is E^-1 W: its overlap matrix is
+ * (E C)^T (E^-1 W) with the trial E C, and its Green's function G E^-1, so
+ * the force bias contracts it with C^T L_g E.  afq_trotter_bf16x3_sym(..., e,
+ * ...) is afq_trotter_bf16x3_sym_halves(..., e, e, ...).
+ */
+int afq_trotter_bf16x3_sym_halves(int nw, int n, int o, int nspin, int n_terms, double sqrt_dt,
+                                  const unsigned* e_first, const unsigned* e_last, const unsigned* vhs_sym,
+                                  const double* win_up, const double* win_dn, double* wout_up, double* wout_dn,
+                                  void* stream);
+
+/*
  * Phaseless importance, cos-phase cut, floor/cap and population-control shift
  * of prop/afqmc.py:116-138, on log-overlaps:
  *   log ovlp_new = (nspin == 1 ? 2 : 1) * sum_s logdet_new[w, s]

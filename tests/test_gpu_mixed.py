@@ -457,6 +457,136 @@ def test_trotter_bf16x3_walker_pairs_equal_per_item_launch(gpu_device, n, o, nsp
         assert max(rel(g_[i], r_[i]) for i in range(few)) < 2e-5
 
 
+@pytest.mark.parametrize("n,o,nspin", [(200, 32, 1), (180, 16, 2), (61, 17, 1)])
+def test_trotter_bf16x3_separate_halves(gpu_device, n, o, nspin):
+    """afq_trotter_bf16x3_sym_halves: (i) a last-half operand equal to (a copy
+    of) the first gives the plain launch's result bit for bit; (ii) with no
+    last half, with E^2 first, and with E^2 first and E last, the walker-pair
+    launch equals the per-item launch bit for bit; (iii) two steps with the
+    halves between them merged, E T2 E^2 (T1 E W), are within 4e-5 of the fp64
+    pair of steps E T2 E E T1 E W per walker, and E (T1 E W) within 2e-5 of the
+    fp64 first step."""
+    import torch
+    from ad_afqmc_prototype_amd import kernels
+    from ad_afqmc_prototype_amd.prop.chol_afqmc_ops import chol_sym_rows, minus_identity_pairs, split_pairs, upper_blocks
+    cus = torch.cuda.get_device_properties(gpu_device).multi_processor_count
+    nw = cus + 37
+    few = 16
+    print(f"separate one-body halves nw={nw} n={n} o={o} nspin={nspin}", flush=True)
+    rng = np.random.default_rng(7 * n + o)
+    dt = 0.005
+    sq = dt ** 0.5
+    e, v1, ws = _trotter_inputs(rng, nw, n, o, nspin, dt, 0.3)
+    _, v2, _ = _trotter_inputs(rng, nw, n, o, nspin, dt, 0.3)
+    e2 = e @ e
+    e2 = 0.5 * (e2 + e2.T)
+    ldv = kernels.ldv_for(n)
+
+    def blocks(v):
+        vp = torch.tensor(np.stack([v.real, v.imag]).reshape(2 * nw, n, n), device=gpu_device)
+        return split_pairs(chol_sym_rows(vp.float().double()).T.float()).view(2, nw, -1).contiguous()
+
+    v1b, v2b = blocks(v1), blocks(v2)
+    e_blk = minus_identity_pairs(upper_blocks(torch.tensor(e, device=gpu_device)).float(), n)
+    e2_blk = minus_identity_pairs(upper_blocks(torch.tensor(e2, device=gpu_device)).float(), n)
+    wd = [torch.tensor(x, device=gpu_device) for x in ws]
+    walk = wd[0] if nspin == 1 else tuple(wd)
+    walk_few = wd[0][:few].contiguous() if nspin == 1 else tuple(x[:few].contiguous() for x in wd)
+
+    def parts(x):
+        return x if isinstance(x, tuple) else (x,)
+
+    plain = kernels.trotter(walk, v1b, e_blk, sq, 6, arith="bf16x3")
+    sep = kernels.trotter(walk, v1b, e_blk, sq, 6, arith="bf16x3", e_last=e_blk.clone())
+    for a, b in zip(parts(plain), parts(sep)):
+        assert torch.equal(a, b)
+    for ef, el in ((e_blk, None), (e2_blk, None), (e2_blk, e_blk)):
+        got = kernels.trotter(walk, v1b, ef, sq, 6, arith="bf16x3", e_last=el)
+        ref = kernels.trotter(walk_few, v1b[:, :few].contiguous(), ef, sq, 6, arith="bf16x3", e_last=el)
+        for a, b in zip(parts(got), parts(ref)):
+            assert torch.equal(a[:few], b), (ef is e2_blk, el is None)
+
+    half = kernels.trotter(walk, v1b, e_blk, sq, 6, arith="bf16x3", e_last=None)
+    two = kernels.trotter(half, v2b, e2_blk, sq, 6, arith="bf16x3", e_last=e_blk)
+    _, vp1 = _planes(e, v1, ldv)
+    ep, vp2 = _planes(e, v2, ldv)
+    e64 = torch.tensor(ep, device=gpu_device)
+    one64 = kernels.trotter(walk, torch.tensor(vp1, device=gpu_device), e64, sq, 6)
+    two64 = kernels.trotter(one64, torch.tensor(vp2, device=gpu_device), e64, sq, 6)
+    et = torch.tensor(e, device=gpu_device).to(torch.complex128)
+    for h_, o_, t_, r_ in zip(parts(half), parts(one64), parts(two), parts(two64)):
+        h_ = to_np(et @ h_)
+        o_, t_, r_ = to_np(o_), to_np(t_), to_np(r_)
+        assert max(rel(h_[i], o_[i]) for i in range(nw)) < 2e-5
+        assert max(rel(t_[i], r_[i]) for i in range(nw)) < 4e-5
+
+
+@pytest.mark.parametrize("kind,n,o,g,scale", [("restricted", 200, 32, 1000, 0.03),
+                                              ("unrestricted", 120, 16, 600, 0.05)])
+def test_block_with_merged_one_body_halves(gpu_device, kind, n, o, g, scale):
+    """bf16x3 blocks (10 steps, QR after 5) whose steps leave their final
+    exp(-dt h1_eff / 2) to the next step, which applies exp(-dt h1_eff) once
+    (the walker-pair launch: more walkers than CUs), against the same blocks
+    with both halves in every step, on the device stream's fields and comb
+    uniforms: identical comb picks; walkers, weights and log-overlaps within
+    1e-5; block energies within 2e-5 Ha.  A deferred state is refused by QR,
+    measurement and comb, and the fp64 propagator never defers."""
+    import dataclasses
+    import torch
+    from ad_afqmc_prototype_amd.meas.rhf import make_rhf_meas_ops
+    from ad_afqmc_prototype_amd.prop.afqmc import afqmc_step, can_merge_halves, init_prop_state
+    from ad_afqmc_prototype_amd.prop.blocks import afqmc_block, measure, orthonormalize_state, reconfigure
+    from ad_afqmc_prototype_amd.prop.chol_afqmc_ops import make_chol_afqmc_ops
+    cus = torch.cuda.get_device_properties(gpu_device).multi_processor_count
+    nw, dt = cus + 24, 0.005
+    print(f"merged one-body halves {kind} n={n} o={o} nw={nw}", flush=True)
+    p = make_pair(3, n, o, g, uhf=kind == "unrestricted", kind=kind, dt=dt, chol_scale=scale, device=gpu_device)
+    m_ops = make_rhf_meas_ops(p.g["sys"], precision="bf16x3")
+    p_ops = make_chol_afqmc_ops(p.g["ham"], kind, precision="bf16x3")
+    p_ctx = p_ops.build_prop_ctx(p.g["t_ops"].get_rdm1(p.g["trial"]), dt)
+    m_ctx = m_ops.build_meas_ctx(p.g["ham"], p.g["trial"])
+    assert can_merge_halves(p_ops, m_ops, m_ctx, p.g["trial"])
+    assert not can_merge_halves(p.g["p_ops"], p.g["m_ops"], p.g["m_ctx"], p.g["trial"])
+    st0 = init_prop_state(sys=p.g["sys"], n_walkers=nw, seed=5, ham_data=p.g["ham"], trial_ops_=p.g["t_ops"],
+                          trial_data=p.g["trial"], meas_ops=m_ops)
+    base = dataclasses.replace(p.g["params"], n_prop_steps=10, n_ortho=5)
+    kw = dict(ham_data=p.g["ham"], trial_data=p.g["trial"], meas_ops=m_ops, prop_ops=p_ops, prop_ctx=p_ctx,
+              meas_ctx=m_ctx)
+    res = {}
+    for merge in (True, False):
+        params = dataclasses.replace(base, merge_one_body_halves=merge)
+        st, es, picks = st0, [], []
+        for _ in range(2):
+            st, obs, idx = afqmc_block(st, sys=p.g["sys"], params=params, return_indices=True, **kw)
+            es.append(obs.scalars["energy"].item())
+            picks.append(idx)
+        assert not st.deferred
+        res[merge] = (st, es, picks)
+    (a, ea, pa), (b, eb, pb) = res[True], res[False]
+    for x, y in zip(pa, pb):
+        assert torch.equal(x, y)
+    wa = a.walkers if isinstance(a.walkers, tuple) else (a.walkers,)
+    wb = b.walkers if isinstance(b.walkers, tuple) else (b.walkers,)
+    for x, y in zip(wa, wb):
+        x, y = to_np(x), to_np(y)
+        assert max(rel(x[i], y[i]) for i in range(nw)) < 1e-5
+    assert rel(to_np(a.weights), to_np(b.weights)) < 1e-5
+    assert float((a.log_overlaps - b.log_overlaps).abs().max()) < 1e-5
+    assert max(abs(x - y) for x, y in zip(ea, eb)) < 2e-5, (ea, eb)
+    assert all(np.isfinite(ea)) and abs(ea[1] - ea[0]) > 1e-6
+
+    d = afqmc_step(st0, params=base, defer_final=True, **kw)
+    assert d.deferred
+    with pytest.raises(ValueError):
+        orthonormalize_state(d, sys=p.g["sys"], trial_data=p.g["trial"], meas_ops_=m_ops)
+    with pytest.raises(ValueError):
+        measure(d, params=base, ham_data=p.g["ham"], trial_data=p.g["trial"], meas_ops=m_ops, meas_ctx=m_ctx)
+    with pytest.raises(ValueError):
+        reconfigure(d, sys=p.g["sys"], trial_data=p.g["trial"], meas_ops_=m_ops, zeta=0.5)
+    # the next step takes it and returns a whole state
+    assert not afqmc_step(d, params=base, **kw).deferred
+
+
 @pytest.mark.parametrize("nw,naux", [(8, 96), (5, 130), (33, 1000)])
 def test_field_shifts_split_planes_equal_split_of_xbar(gpu_device, nw, naux):
     """afq_field_shifts_split: the same xbar, shift and force-bias terms as
