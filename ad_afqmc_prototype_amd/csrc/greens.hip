@@ -479,6 +479,7 @@ struct gj_lds {
     double2 col[O];
     double2 rowk[O];
     double2 rowp[O];
+    double2 piv[O];  // the pivots, for the log det after the elimination
     int perm[O];
     uint16_t at[4][O][AP];  // A^T planes: re hi, re lo, im hi, im lo
 };
@@ -570,7 +571,6 @@ __device__ __forceinline__ void greens_gj_split_body(int n, int o, const double2
     if (lane < O) S.perm[lane] = lane;
 
     // ---- 2. Gauss-Jordan with partial pivoting ----
-    double lre = 0.0, lim = 0.0;
     int nswap = 0;
     // k = 4 ks + kq: the row slot ks (and the column tile kt) are compile-time
     // constants of the unrolled outer loop, so every register index is static
@@ -646,8 +646,7 @@ __device__ __forceinline__ void greens_gj_split_body(int n, int o, const double2
             // pivot value (uniform): lane (kl, kq), slot (ks, kt)
             const int src = kl + 16 * kq;
             const double2 pv = make_double2(readlane_f64(m[ks][kt].x, src), readlane_f64(m[ks][kt].y, src));
-            lre += log(hypot(pv.x, pv.y));
-            lim += atan2(pv.y, pv.x);
+            if (lane == 0) S.piv[k] = pv;
             const double2 inv = crecip(pv);
             // scale row k (A[k][k] <- 1 / p) and publish it and column k
             if (q == kq) {
@@ -680,6 +679,23 @@ __device__ __forceinline__ void greens_gj_split_body(int n, int o, const double2
                 }
             }
             wave_lds_sync();
+        }
+    }
+    // log det = sum_k log |p_k| + i arg p_k (+ i pi per swap): lane k evaluates
+    // pivot k's logarithms (off the elimination's latency chain), the sums run
+    // over k in order
+    double lre = 0.0, lim = 0.0;
+    {
+        wave_lds_sync();
+        double ll = 0.0, la = 0.0;
+        if (lane < o) {
+            const double2 pv = S.piv[lane];
+            ll = log(hypot(pv.x, pv.y));
+            la = atan2(pv.y, pv.x);
+        }
+        for (int k = 0; k < o; ++k) {
+            lre += readlane_f64(ll, k);
+            lim += readlane_f64(la, k);
         }
     }
     if (nswap & 1) lim += M_PI;
