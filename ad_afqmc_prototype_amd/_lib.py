@@ -81,6 +81,7 @@ _SIGNATURES = {
     "afq_trotter_f32_grouped": [c_int, c_vp, c_int, c_int, c_int, c_vp],
     "afq_trotter_bf16x3_grouped": [c_int, c_vp, c_int, c_int, c_int, c_vp],
     "afq_trotter_bf16x3_sym_grouped": [c_int, c_vp, c_int, c_int, c_int, c_vp],
+    "afq_trotter_bf16x3_sym_grouped_halves": [c_int, c_vp, c_int, c_int, c_int, c_int, c_int, c_vp],
     "afq_overlap_greens_grouped": [c_int, c_vp, c_int, c_int, c_vp],
     "afq_overlap_greens_bf16x3_grouped": [c_int, c_vp, c_int, c_int, c_vp],
     "afq_phaseless_update_grouped": [c_int, c_vp, c_vp],
@@ -97,7 +98,7 @@ _SIGNATURES = {
     "afq_unpack_pairs": [c_int, c_int, c_vp, c_ll, c_vp, c_vp],
 }
 
-ABI_VERSION = 19
+ABI_VERSION = 20
 
 
 class HipLibraryError(RuntimeError):

@@ -184,7 +184,7 @@ __global__ __launch_bounds__(512, 1) void trotter_bf16x3_kernel(
     const double2* __restrict__ win_up, const double2* __restrict__ win_dn,
     double2* __restrict__ wout_up, double2* __restrict__ wout_dn, int group_map, int count = 0,
     const afq_system_step* __restrict__ S = nullptr, const float* __restrict__ Elast = nullptr,
-    int final_half = 1) {
+    int final_half = 1, int first_full = 0) {
     constexpr int KMAX = KR + NL;
     extern __shared__ __attribute__((aligned(16))) unsigned char lds_raw[];
     int b = blockIdx.x;
@@ -199,7 +199,8 @@ __global__ __launch_bounds__(512, 1) void trotter_bf16x3_kernel(
         nspin = 1;
         nct = afq::ceil_div(o, 16);
         sqrt_dt = d.sqrt_dt;
-        E = d.exp_h1_half;
+        E = first_full ? d.exp_h1 : d.exp_h1_half;
+        Elast = d.exp_h1_half;
         vhs = d.vhs;
         if constexpr (SYMV) {
             const int nbg = afq::ceil_div(n, 16);
@@ -1099,7 +1100,7 @@ namespace {
 
 template <bool SYMV>
 int trotter_bf16x3_grouped(const char* name, int count, const afq_system_step* sys, int total_blocks, int max_n,
-                           int n_terms, hipStream_t st) {
+                           int n_terms, hipStream_t st, int first_full = 0, int last_half = 1) {
     AFQ_REQUIRE(count > 0 && total_blocks > 0, "%s: count=%d total_blocks=%d", name, count, total_blocks);
     AFQ_REQUIRE(sys, "%s: null pointer", name);
     AFQ_REQUIRE(max_n > 0 && max_n <= 16 * 2 * (KR + NL_MAX), "%s: max_n=%d not in 1..%d", name, max_n,
@@ -1116,7 +1117,8 @@ int trotter_bf16x3_grouped(const char* name, int count, const afq_system_step* s
     do {                                                                                                          \
         if (lds > 64 * 1024) afq::allow_full_lds((const void*)trotter_bf16x3_kernel<NLV, SYMV, true>);            \
         hipLaunchKernelGGL((trotter_bf16x3_kernel<NLV, SYMV, true>), grid, block, lds, st, 0, 0, 0, 1, 0, 0,         \
-                           n_terms, 0.0, nullptr, nullptr, 0LL, nullptr, nullptr, nullptr, nullptr, 0, count, sys); \
+                           n_terms, 0.0, nullptr, nullptr, 0LL, nullptr, nullptr, nullptr, nullptr, 0, count, sys,  \
+                           nullptr, last_half, first_full);                                                         \
     } while (0)
     if (nl == 0)
         AFQ_TROTTER_GROUPED(0);
@@ -1140,4 +1142,11 @@ extern "C" int afq_trotter_bf16x3_sym_grouped(int count, const afq_system_step* 
                                               int n_terms, void* stream) {
     return trotter_bf16x3_grouped<true>("afq_trotter_bf16x3_sym_grouped", count, sys, total_blocks, max_n, n_terms,
                                         (hipStream_t)stream);
+}
+
+extern "C" int afq_trotter_bf16x3_sym_grouped_halves(int count, const afq_system_step* sys, int total_blocks,
+                                                     int max_n, int n_terms, int first_full, int last_half,
+                                                     void* stream) {
+    return trotter_bf16x3_grouped<true>("afq_trotter_bf16x3_sym_grouped_halves", count, sys, total_blocks, max_n,
+                                        n_terms, (hipStream_t)stream, first_full, last_half);
 }

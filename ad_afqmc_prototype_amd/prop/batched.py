@@ -44,7 +44,7 @@ from .. import _lib
 from .. import kernels
 from ..core.system import system
 from ..ham.chol import ham_chol
-from ..meas.rhf import make_rhf_meas_ops, quadratic_exchange
+from ..meas.rhf import deferred_view, make_rhf_meas_ops, quadratic_exchange
 from ..trial.rhf import make_rhf_trial_ops, rhf_trial
 from .afqmc import init_prop_state
 from .chol_afqmc_ops import chol_sym_rows, make_chol_afqmc_ops
@@ -71,7 +71,7 @@ class _system_step(ctypes.Structure):
                     "log_overlaps_out", "weights_in", "weights_out", "pop_in", "pop_out", "key_in",
                     "key_out", "rot_chol", "rot_h1", "energy", "greens_hi", "greens_lo")] + [
                     ("ldk", ctypes.c_longlong), ("xbar_hi", _vp), ("xbar_lo", _vp), ("ldx", ctypes.c_longlong),
-                ("exch_z", _vp)]
+                ("exch_z", _vp), ("exp_h1", _vp)]
 
 
 class _gemm_problem(ctypes.Structure):
@@ -100,7 +100,7 @@ class _split_problem(ctypes.Structure):
                 ("ld_src", ctypes.c_int), ("ld_dst", ctypes.c_int)]
 
 
-assert ctypes.sizeof(_system_step) == 8 * 4 + 8 * 8 + 26 * 8 + 8 + 2 * 8 + 8 + 8
+assert ctypes.sizeof(_system_step) == 8 * 4 + 8 * 8 + 26 * 8 + 8 + 2 * 8 + 8 + 8 + 8
 assert ctypes.sizeof(_gemm_problem) == 80
 assert ctypes.sizeof(_gemm_problem_bf16x3) == 96
 assert ctypes.sizeof(_split_problem) == 40
@@ -172,12 +172,19 @@ class system_batch:
     def __init__(self, systems: Sequence[batch_system], *, dt: float, qr_every: int = 10,
                  sr_every: int = 20, n_exp_terms: int = 6, weight_floor: float = 1.0e-3,
                  weight_cap: float = 100.0, damping: float = 0.1, shift_ema: float = 0.1,
-                 gemm_shapes: tuple[int, int] | None = None, precision: str = "bf16x3"):
+                 gemm_shapes: tuple[int, int] | None = None, precision: str = "bf16x3",
+                 merge_one_body_halves: bool = True):
         if not systems:
             raise ValueError("empty batch")
         if precision not in ("bf16x3", "fp32"):
             raise ValueError(f"precision must be 'bf16x3' or 'fp32', not {precision!r}")
         self.precision = precision
+        # bf16x3: a step followed by another leaves its last one-body half to it
+        # (prop/afqmc.py afqmc_step(defer_final=True)); between such steps the
+        # walkers are E^-1 W, their Green's functions are taken with the trial
+        # E C and the force bias with C^T L E
+        self.merge = bool(merge_one_body_halves) and precision == "bf16x3"
+        self.deferred = False
         if qr_every % 2 or sr_every % 2 or qr_every <= 0 or sr_every <= 0:
             raise ValueError("qr_every and sr_every must be positive and even")
         dev = systems[0].ham.h1.device
@@ -242,6 +249,12 @@ class system_batch:
             ops = dict(chol_hl=kernels.split_bf16(chol_sym_rows(s.ham.chol)),
                        rot_hl=kernels.split_bf16(m_ctx.rot_chol_flat),
                        e_pairs=p_ctx.exp_h1_half_sym_pairs.contiguous())
+            if self.merge:
+                # the deferred walkers' operands: trial E C, C^T L E, E^2 - I
+                trial_e, ctx_e = deferred_view(m_ctx, s.trial, p_ctx.exp_h1_half)
+                ops.update(trial_e=trial_e.mo_coeff.contiguous(),
+                           rot_e_hl=kernels.split_bf16(ctx_e.rot_chol_flat.contiguous()),
+                           e2_pairs=p_ctx.exp_h1_sym_pairs.contiguous())
         else:
             ops = dict(chol_s32=kernels.convert_f32(chol_sym_rows(s.ham.chol)), rot32=m_ctx.fb_operand(1))
         if quadratic_exchange(n, o, g):
@@ -288,6 +301,9 @@ class system_batch:
             # V as pair words of its upper blocks; E - I likewise
             self.vhs = P([2 * m["nw"] * m["nsym"] for m in ms], torch.int32)
             self.e_half = P([m["nsym"] for m in ms], torch.int32)
+            if self.merge:
+                self.e_full = P([m["nsym"] for m in ms], torch.int32)
+                self.trials_e = P([m["n"] * m["o"] for m in ms], f64)
         else:
             self.vhs = P([2 * m["nw"] * m["n"] * m["ldv"] for m in ms], f32)
             self.e_half = P([m["n"] * m["ldv"] for m in ms], f32)
@@ -309,11 +325,15 @@ class system_batch:
             self.e_half.part(i).copy_((m["e_pairs"] if self.precision == "bf16x3" else m["e_half"]).reshape(-1))
             self.mf.part(i).copy_(m["mf"])
             self.trials.part(i).copy_(m["trial"].reshape(-1))
+            if self.merge:
+                self.e_full.part(i).copy_(m["e2_pairs"].reshape(-1))
+                self.trials_e.part(i).copy_(m["trial_e"].reshape(-1))
 
-    def _desc(self, parity: int, idx: Sequence[int] | None = None):
+    def _desc(self, parity: int, idx: Sequence[int] | None = None, trial_e: bool = False):
         """Descriptor table of the systems ``idx`` (all by default) for steps
         from buffer set ``parity``; walker0 / tblock0 are prefix sums over the
-        table.  Returns (array, walkers, Trotter workgroups)."""
+        table.  ``trial_e``: the trial is E C (walkers without their last
+        one-body half).  Returns (array, walkers, Trotter workgroups)."""
         a, b = parity, 1 - parity
         idx = list(range(self.count)) if idx is None else list(idx)
         arr = (_system_step * len(idx))()
@@ -326,12 +346,14 @@ class system_batch:
             t0 += m["nw"] * ((m["o"] + 15) // 16)
             d.sqrt_dt, d.dt, d.e_estimate, d.h0_prop = self.sqrt_dt, self.dt, m["e_est"], m["h0_prop"]
             d.weight_floor, d.weight_cap, d.damping, d.h0 = self.floor, self.cap, self.damping, m["h0"]
-            d.trial = self.trials.ptr(i)
+            d.trial = (self.trials_e if trial_e else self.trials).ptr(i)
             d.walkers_in, d.walkers_out = self.walkers[a].ptr(i), self.walkers[b].ptr(i)
             d.greens, d.logdet = self.greens.ptr(i), self.logdet.ptr(i)
             d.fields, d.fb, d.mf, d.xbar = self.fields.ptr(i), self.fb.ptr(i), self.mf.ptr(i), self.xbar.ptr(i)
             d.shift_term, d.fb_term = self.shift.ptr(i), self.fbt.ptr(i)
             d.exp_h1_half, d.vhs = self.e_half.ptr(i), self.vhs.ptr(i)
+            if self.merge:
+                d.exp_h1 = self.e_full.ptr(i)
             d.log_overlaps_in, d.log_overlaps_out = self.lo[a].ptr(i), self.lo[b].ptr(i)
             d.weights_in, d.weights_out = self.weights[a].ptr(i), self.weights[b].ptr(i)
             d.pop_in, d.pop_out = self.pop[a][i:i + 1].data_ptr(), self.pop[b][i:i + 1].data_ptr()
@@ -384,9 +406,9 @@ class system_batch:
         start = 0
         for i, m in enumerate(self.members):
             p = arr[i]
-            if which == "fb":
+            if which in ("fb", "fb_e"):
                 k = pad8(m["o"] * m["n"])
-                (bh, bl), (ah, al) = m["rot_hl"], self.greens_hl
+                (bh, bl), (ah, al) = m["rot_hl" if which == "fb" else "rot_e_hl"], self.greens_hl
                 p.M, p.N, p.K = 2 * m["nw"], m["g"], k
                 p.ldc, p.C, p.alpha, p.c_fp64 = m["g"], self.fb.ptr(i), 2.0, 1
             elif which == "exch":
@@ -459,9 +481,11 @@ class system_batch:
         if self.precision == "bf16x3":
             for idx in buckets(lambda m: 16 if m["o"] <= 16 else 32):
                 tabs = [self._desc(p, idx) for p in (0, 1)]
-                self.greens_split_buckets.append(dict(
-                    tables=[_upload(t[0]).to(self.device) for t in tabs], count=len(idx), walkers=tabs[0][1],
-                    max_o=max(self.members[i]["o"] for i in idx)))
+                bk = dict(tables=[_upload(t[0]).to(self.device) for t in tabs], count=len(idx),
+                          walkers=tabs[0][1], max_o=max(self.members[i]["o"] for i in idx))
+                if self.merge:
+                    bk["tables_e"] = [_upload(self._desc(p, idx, trial_e=True)[0]).to(self.device) for p in (0, 1)]
+                self.greens_split_buckets.append(bk)
         self.energy_buckets = []
         for idx in buckets(lambda m: (m["o"] + 15) // 16):
             arr, nw, _ = self._desc(0, idx)
@@ -481,7 +505,7 @@ class system_batch:
                                             walkers=nw, blocks=blocks, wpb=wpb, max_n=max_n, nt=nt))
         self.gemm_tables = {}
         self.split_tables = {}
-        for which in ("fb", "vhs") + (("exch",) if self.quadratic else ()):
+        for which in ("fb", "vhs") + (("exch",) if self.quadratic else ()) + (("fb_e",) if self.merge else ()):
             if self.precision == "bf16x3":
                 for shape in BF16X3_SHAPES:
                     arr, tiles = self._problems_bf16x3(which, shape)
@@ -507,10 +531,10 @@ class system_batch:
         if self.precision == "bf16x3":
             # the A operand's bf16 planes come from the Green's stage (force
             # bias) and the field shifts (VHS)
-            shape = self.gemm_shapes[0 if which in ("fb", "exch") else 1]
+            shape = self.gemm_shapes[0 if which in ("fb", "exch", "fb_e") else 1]
             tab, tiles = self.gemm_tables[(which, shape)]
             kernels._call("afq_gemm_bf16x3_nt_grouped", tab, self.count, tab.data_ptr(), tiles, shape,
-                          kernels._stream(tab), tag=which)
+                          kernels._stream(tab), tag="fb" if which == "fb_e" else which)
             return
         # (the exchange's G Y has M = 2 nw like the force bias: the force bias's shape)
         shape = self.gemm_shapes[0 if which in ("fb", "exch") else 1]
@@ -523,14 +547,15 @@ class system_batch:
         kernels._call("afq_convert_f32", src.buf, 1, n, src.buf.data_ptr(), n, dst.buf.data_ptr(), n,
                       kernels._stream(src.buf))
 
-    def _greens(self, p: int, st: int, rows: bool = True):
+    def _greens(self, p: int, st: int, rows: bool = True, trial_e: bool = False):
         """Green's functions and log-overlaps of every system's new walkers
         (buffer set p), one launch per occupied-orbital bucket: fp64 rows
         (and, in bf16x3 precision, their planes split from them), or with
-        rows=False (bf16x3 steps) only the force bias's planes."""
+        rows=False (bf16x3 steps) only the force bias's planes (trial_e: of
+        walkers without their last one-body half, with the trial E C)."""
         if not rows:
             for bk in self.greens_split_buckets:
-                t = bk["tables"][p]
+                t = bk["tables_e" if trial_e else "tables"][p]
                 kernels._call("afq_overlap_greens_bf16x3_grouped", t, bk["count"], t.data_ptr(), bk["walkers"],
                               bk["max_o"], st)
             return
@@ -540,30 +565,43 @@ class system_batch:
         if self.precision == "bf16x3":
             self._split("fb")
 
-    def step(self):
-        """One step of every system (fields drawn from each system's key)."""
+    def step(self, last_half: bool = True):
+        """One step of every system (fields drawn from each system's key).
+        last_half=False (merging batches) leaves the step's last one-body half
+        to the next step; the batch is then ``deferred`` until a step runs
+        with last_half=True."""
         p = self.steps_done % 2
         d = self.desc[p]
         st = kernels._stream(d)
+        first_full = self.deferred
+        last_half = bool(last_half) or not self.merge
         kernels._call("afq_rng_fields_grouped", d, self.count, d.data_ptr(), self.total_walkers, st)
         if self.precision == "fp32":
             self._convert(self.greens, self.greens32)
-        self._grouped_gemm("fb")
+        self._grouped_gemm("fb_e" if first_full else "fb")
         kernels._call("afq_field_shifts_grouped", d, self.count, d.data_ptr(), self.total_walkers, st)
         if self.precision == "fp32":
             self._convert(self.xbar, self.xbar32)
         self._grouped_gemm("vhs")
         for bk in self.trotter_buckets:
             t = bk["tables"][p]
+            if first_full or not last_half:
+                kernels._call("afq_trotter_bf16x3_sym_grouped_halves", t, bk["count"], t.data_ptr(), bk["blocks"],
+                              bk["max_n"], self.n_exp_terms, int(first_full), int(last_half), st,
+                              tag=None if last_half else "deferred")
+                continue
             kernels._call("afq_trotter_bf16x3_sym_grouped" if self.precision == "bf16x3" else "afq_trotter_f32_grouped",
                           t, bk["count"], t.data_ptr(), bk["blocks"], bk["max_n"], self.n_exp_terms, st)
-        self._greens(p, st, rows=self.precision != "bf16x3")
+        self._greens(p, st, rows=self.precision != "bf16x3", trial_e=not last_half)
         kernels._call("afq_phaseless_update_grouped", d, self.count, d.data_ptr(), st)
         self.steps_done += 1
+        self.deferred = not last_half
 
     def _orthonormalize(self):
         """QR of every system (walkers.py:169-211) and the overlap refresh
         (prop/blocks.py:53-57); the state stays in set A."""
+        if self.deferred:
+            raise RuntimeError("QR of walkers without their last one-body half")
         if self.steps_done % 2:
             raise RuntimeError("QR needs the state in buffer set A (an even step count)")
         d = self.desc[0]
@@ -621,8 +659,9 @@ class system_batch:
     def advance(self, n_steps: int):
         """n_steps steps of every system with the QR / measurement cadence
         (counted from the batch's first step)."""
-        for _ in range(n_steps):
-            self.step()
+        for k in range(n_steps):
+            t = self.steps_done + 1
+            self.step(last_half=(t % self.sr_every == 0 or t % self.qr_every == 0 or k == n_steps - 1))
             t = self.steps_done
             if t % self.sr_every == 0:
                 self._orthonormalize()
@@ -634,6 +673,8 @@ class system_batch:
     def system_state(self, i: int) -> dict:
         """Copies of system i's walkers, weights, log-overlaps, pop shift and key
         (state in set A: call after an even number of steps)."""
+        if self.deferred:
+            raise RuntimeError("the walkers lack their last one-body half: run a step with last_half=True")
         m = self.members[i]
         a = self.steps_done % 2
         return dict(walkers=self.walkers[a].part(i, (m["nw"], m["n"], m["o"])).clone(),

@@ -502,6 +502,9 @@ typedef struct afq_system_step {
     long long ldx;               /* row pitch of the xbar planes: >= naux */
     double* exch_z;              /* planes (2, nw, o*n): Z = G Y of the exchange as a quadratic form
                                     (afq_energy_assemble_grouped) */
+    const float* exp_h1;         /* bf16x3 pair-word Trotter: E^2 - I (E^2 = exp(-dt h1_eff)) as upper-block
+                                    pairs, the first half after a step without its last
+                                    (afq_trotter_bf16x3_sym_grouped_halves) */
 } afq_system_step;
 
 /* One problem of a grouped GEMM: C (M x N) = alpha A (M x K) B^T (N x K). */
@@ -628,6 +631,14 @@ int afq_trotter_bf16x3_grouped(int count, const afq_system_step* sys, int total_
  */
 int afq_trotter_bf16x3_sym_grouped(int count, const afq_system_step* sys, int total_blocks, int max_n,
                                    int n_terms, void* stream);
+
+/*
+ * afq_trotter_bf16x3_sym_grouped with the halves of afq_trotter_bf16x3_sym_halves
+ * for every system: first_full != 0 takes each system's exp_h1 (E^2 - I) for the
+ * first half instead of exp_h1_half, last_half == 0 leaves out the last half.
+ */
+int afq_trotter_bf16x3_sym_grouped_halves(int count, const afq_system_step* sys, int total_blocks, int max_n,
+                                          int n_terms, int first_full, int last_half, void* stream);
 
 /*
  * afq_overlap_greens for every system, of walkers_out into greens / logdet

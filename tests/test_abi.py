@@ -111,7 +111,7 @@ int main(void) {
   O(afq_system_step, tblock0); O(afq_system_step, sqrt_dt); O(afq_system_step, damping);
   O(afq_system_step, trial); O(afq_system_step, vhs); O(afq_system_step, pop_out);
   O(afq_system_step, key_out); O(afq_system_step, ldr); O(afq_system_step, h0);
-  O(afq_system_step, rot_chol); O(afq_system_step, energy); O(afq_system_step, exch_z);
+  O(afq_system_step, rot_chol); O(afq_system_step, energy); O(afq_system_step, exch_z); O(afq_system_step, exp_h1);
   O(afq_gemm_problem, tile_start); O(afq_gemm_problem, ldc);
   O(afq_gemm_problem, C); O(afq_gemm_problem, alpha); O(afq_gemm_problem, c_fp64);
   return 0;
@@ -128,7 +128,7 @@ int main(void) {
     assert int(out["size_step"]) == ctypes.sizeof(B._system_step)
     assert int(out["size_gemm"]) == ctypes.sizeof(B._gemm_problem)
     for f in ("tblock0", "sqrt_dt", "damping", "trial", "vhs", "pop_out", "key_out", "ldr", "h0",
-              "rot_chol", "energy", "exch_z"):
+              "rot_chol", "energy", "exch_z", "exp_h1"):
         assert int(out[f]) == getattr(B._system_step, f).offset, f
     for f in ("tile_start", "ldc", "C", "alpha", "c_fp64"):
         assert int(out[f]) == getattr(B._gemm_problem, f).offset, f

@@ -320,6 +320,42 @@ def test_batch_bf16x3_close_to_fp32(gpu_device):
     np.testing.assert_allclose(ea[:, 0], eb[:, 0], rtol=1e-5, atol=0)
 
 
+def test_batch_bf16x3_merged_halves_match_unmerged(gpu_device):
+    """The bf16x3 batch whose steps leave their last one-body half to the next
+    step (the default: E^2 applied once, E C trial and C^T L E force bias for
+    the walkers in between) against the same batch applying both halves in
+    every step: after one block (QR and measurement + comb) the block energies
+    agree to 2e-5 Ha, walkers to 1e-5 of max |w| and weights to 1e-5; a
+    deferred batch refuses QR and state reads until a step applies its last
+    half."""
+    import torch
+    from ad_afqmc_prototype_amd.prop.batched import system_batch
+    print("merged vs unmerged one-body halves, bf16x3 batch", flush=True)
+    a = system_batch(_members(gpu_device), dt=DT, qr_every=QR, sr_every=SR, precision="bf16x3")
+    b = system_batch(_members(gpu_device), dt=DT, qr_every=QR, sr_every=SR, precision="bf16x3",
+                     merge_one_body_halves=False)
+    assert a.merge and not b.merge
+    a.advance(SR - 2)
+    b.advance(SR - 2)
+    for i in range(len(SYSTEMS)):
+        x, y = a.system_state(i), b.system_state(i)
+        scale = float(y["walkers"].abs().max())
+        assert float((x["walkers"] - y["walkers"]).abs().max()) / scale < 1e-5, f"system {i}: walkers"
+        assert torch.allclose(x["weights"], y["weights"], rtol=1e-5, atol=0), f"system {i}: weights"
+    a.advance(2)
+    b.advance(2)
+    ea, eb = a.block_stats.cpu().numpy(), b.block_stats.cpu().numpy()
+    assert np.isfinite(ea).all()
+    assert np.abs(ea[:, 0] - eb[:, 0]).max() < 2e-5, (ea[:, 0], eb[:, 0])
+    a.step(last_half=False)
+    assert a.deferred
+    with pytest.raises(RuntimeError):
+        a.system_state(0)
+    a.step()
+    assert not a.deferred
+    a.system_state(0)
+
+
 # --------------------------------------------------------------------------
 # the 64-system workload: seeds 100..163, N ~ U[40,160], nocc ~ U[6,24],
 # naux = round(N U[4,6]), walkers ~ U[64,512]; dt 0.005, QR every 10,
@@ -394,11 +430,13 @@ def test_cfg4_batch_block(gpu_device):
     np.testing.assert_allclose(s16[:, 0], stats[:, 0], rtol=1e-5, atol=0)
 
 
-def test_grouped_bf16x3_trotter_buckets_match_single(gpu_device):
+@pytest.mark.parametrize("first_full,last_half", [(0, 1), (1, 0), (0, 0), (1, 1)])
+def test_grouped_bf16x3_trotter_buckets_match_single(gpu_device, first_full, last_half):
     """One grouped bf16x3 Trotter launch per bucket (n = 40, 160, 176, 208:
     no, one and two V k-blocks held in LDS) equals afq_trotter_bf16x3_sym on
     each system alone, bit for bit, with random complex-symmetric V given as
-    pair words of its upper blocks."""
+    pair words of its upper blocks; with the halves of merging steps (E^2 - I
+    first, no last half) against afq_trotter_bf16x3_sym_halves."""
     import torch
     from ad_afqmc_prototype_amd import kernels
     from ad_afqmc_prototype_amd.prop.batched import system_batch
@@ -418,16 +456,23 @@ def test_grouped_bf16x3_trotter_buckets_match_single(gpu_device):
         batch.vhs.part(i, (2, nw, m["nsym"])).copy_(split_pairs(blocks).view(2, nw, m["nsym"]))
     st = kernels._stream(batch.desc[0])
     assert len(batch.trotter_buckets) == 4
+    plain = (first_full, last_half) == (0, 1)
     for bk in batch.trotter_buckets:
         t = bk["tables"][0]
-        kernels._call("afq_trotter_bf16x3_sym_grouped", t, bk["count"], t.data_ptr(), bk["blocks"], bk["max_n"],
-                      6, st)
+        if plain:
+            kernels._call("afq_trotter_bf16x3_sym_grouped", t, bk["count"], t.data_ptr(), bk["blocks"],
+                          bk["max_n"], 6, st)
+        else:
+            kernels._call("afq_trotter_bf16x3_sym_grouped_halves", t, bk["count"], t.data_ptr(), bk["blocks"],
+                          bk["max_n"], 6, first_full, last_half, st)
     torch.cuda.synchronize()
     for i, m in enumerate(batch.members):
         nw, n, o = m["nw"], m["n"], m["o"]
         win = batch.walkers[0].part(i, (nw, n, o))
-        ref = kernels.trotter(win, batch.vhs.part(i, (2, nw, m["nsym"])), batch.e_half.part(i),
-                              batch.sqrt_dt, 6, arith="bf16x3")
+        e_first = batch.e_full.part(i) if first_full else batch.e_half.part(i)
+        kw = {} if plain else {"e_last": batch.e_half.part(i) if last_half else None}
+        ref = kernels.trotter(win, batch.vhs.part(i, (2, nw, m["nsym"])), e_first,
+                              batch.sqrt_dt, 6, arith="bf16x3", **kw)
         got = batch.walkers[1].part(i, (nw, n, o))
         assert torch.equal(got, ref), f"system {i} (n={n}): grouped bf16x3 Trotter differs"
         assert float((ref - win).abs().max()) > 1e-3  # the step moved the walkers
