@@ -17,7 +17,7 @@ splitting, so that the two spin sectors really differ).
 Auxiliary fields for step ``t`` are x ~ N(0,1) of shape (n_walkers, naux)
 drawn from ``default_rng([seed, 1, t])``; comb uniforms for reconfiguration
 event ``k`` come from ``default_rng([seed, 2, k])``.  The same arrays are fed to
-the HIP path and to the CPU checker.
+the HIP path and to the CPU oracle.
 """
 from __future__ import annotations
 

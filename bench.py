@@ -129,7 +129,7 @@ def warm_block_end(ctx, state):
 
 
 def cpu_baseline(workload: str, single_thread: bool = False, nw: int = 128):
-    """The CPU checker (numpy fp64, op for op the reference algorithm) on a
+    """The CPU oracle (numpy fp64, op for op the reference algorithm) on a
     subsample of the same workload's walkers, timed on this host with all of
     its BLAS threads, or with one (the reference pins its own CPU runs to one
     thread, config.py:49-54)."""
@@ -174,7 +174,7 @@ def _cpu_baseline_run(workload: str, threads: int, nw: int):
             "host_cpus": os.cpu_count(), "kind": "port",
             "sample": f"{workload} shapes, {nw} walkers x {n_steps} afqmc_step with a QR every "
                       f"{cfg['qr_every']} and measurement + comb after {cfg['sr_every']} (numpy fp64 "
-                      f"checker, log-overlaps, {el:.1f} s, {threads} BLAS threads)"}
+                      f"oracle, log-overlaps, {el:.1f} s, {threads} BLAS threads)"}
 
 
 def accuracy_check(workload: str, precision: str, device, chol_scale: float, n_blocks: int):

@@ -1,6 +1,6 @@
 """CPU FP64 restatement of the reference phaseless-AFQMC hot path.
 
-TEST INFRASTRUCTURE ONLY.  This module is the checker for the HIP path: only
+TEST INFRASTRUCTURE ONLY.  This module is the oracle for the HIP path: only
 ``tests/``, ``__graft_entry__.smoke()`` and the ``cpu_baseline`` leg of
 ``bench.py`` may import it.  The product package never imports it and never
 falls back to it.

@@ -1,4 +1,4 @@
-"""Builds the same synthetic system for the CPU checker and for the HIP path."""
+"""Builds the same synthetic system for the CPU oracle and for the HIP path."""
 from __future__ import annotations
 
 from dataclasses import dataclass

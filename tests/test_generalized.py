@@ -1,5 +1,5 @@
 """Generalized (spin-orbital) walkers in the restricted basis with an RHF trial
-on the CPU checker (meas/rhf.py:108-130, trial/rhf.py:47-53,
+on the CPU oracle (meas/rhf.py:108-130, trial/rhf.py:47-53,
 prop/chol_afqmc_ops.py:118-212, walkers.py:19-49):
 
 * force bias = d/dx log <T|(1 + x L_g)|w> at 0 with L_g acting on both spin

@@ -368,8 +368,8 @@ def _cfg4_members(device):
                                     for sp in specs], scale=0.05)
 
 
-def _checker_block(sp, scale=0.05, dt=0.005, qr=10, sr=20):
-    """One block of system ``sp`` on the CPU checker, with the fields and the
+def _oracle_block(sp, scale=0.05, dt=0.005, qr=10, sr=20):
+    """One block of system ``sp`` on the CPU oracle, with the fields and the
     comb uniform of the system's own Threefry stream (split per step, normals;
     split, uniform at the block end)."""
     from oracle import afqmc_oracle as orc
@@ -393,7 +393,7 @@ def _checker_block(sp, scale=0.05, dt=0.005, qr=10, sr=20):
 def test_cfg4_batch_block(gpu_device):
     """One block of the 64-system workload in fp32: every system bit for bit
     equal to its single-system fp32 run; the smallest, the largest-n and the
-    largest-nocc system against the fp64 checker (block energy within 0.1 mHa,
+    largest-nocc system against the fp64 oracle (block energy within 0.1 mHa,
     weight sum to 1e-5, walker projectors to 1e-4); the bf16x3 batch's block
     energies within 1e-5 relative of the fp32 batch's."""
     import torch
@@ -412,7 +412,7 @@ def test_cfg4_batch_block(gpu_device):
              max(range(64), key=lambda i: specs[i]["norb"]),
              max(range(64), key=lambda i: specs[i]["nocc"])}
     for i in sorted(picks):
-        st_ref, blk = _checker_block(specs[i])
+        st_ref, blk = _oracle_block(specs[i])
         assert abs(stats[i, 0] - blk["energy"]) < 1e-4, (i, stats[i, 0], blk["energy"])
         assert abs(stats[i, 1] - blk["weight"]) / blk["weight"] < 1e-5, i
         w = batch.system_state(i)["walkers"].cpu().numpy()
@@ -421,7 +421,7 @@ def test_cfg4_batch_block(gpu_device):
         err = np.linalg.norm(proj - proj_ref) / np.linalg.norm(proj_ref)
         assert err < 1e-4, (i, err)
         print(f"system {i} (seed {specs[i]['seed']}, n={specs[i]['norb']}, o={specs[i]['nocc']}): "
-              f"block energy {stats[i, 0]:.10f} vs checker {blk['energy']:.10f}", flush=True)
+              f"block energy {stats[i, 0]:.10f} vs oracle {blk['energy']:.10f}", flush=True)
     del batch
     torch.cuda.empty_cache()
     b16 = system_batch(members, dt=0.005, qr_every=10, sr_every=20, precision="bf16x3")

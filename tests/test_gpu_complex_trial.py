@@ -1,6 +1,6 @@
 """Complex trial orbitals (the reference takes C^H, trial/rhf.py:25-44,
 meas/rhf.py:15-50,141-150): overlap, Green's function, force bias, energy and
-a few steps plus a block on the HIP path against the CPU checker (fp64,
+a few steps plus a block on the HIP path against the CPU oracle (fp64,
 1e-10 relative), RHF and UHF trials; the mixed precisions' force bias within
 their tolerances; and the gauge invariance C -> C U (U unitary) of the
 Green's function, force bias and energy."""
@@ -111,7 +111,7 @@ def test_complex_gauge_of_a_real_trial(gpu_device):
 @pytest.mark.parametrize("case", CASES[:2])
 def test_complex_trial_steps_and_block(gpu_device, case):
     """A block of 6 steps (QR at 3) and the measurement + comb, common fields
-    and comb uniform, against the checker."""
+    and comb uniform, against the oracle."""
     import dataclasses
     import torch
     from ad_afqmc_prototype_amd.prop.afqmc import make_key

@@ -1,6 +1,6 @@
 """The driver (driver.py:39-176 of the reference) on the HIP block against the
-CPU checker: equilibration and sampling blocks on the device Threefry stream
-of the seed, against the checker's blocks fed the same keys
+CPU oracle: equilibration and sampling blocks on the device Threefry stream
+of the seed, against the oracle's blocks fed the same keys
 (oracle/threefry.py: split per step, normals for the fields, a uniform per
 comb), then outlier rejection and the blocking analysis."""
 import dataclasses
@@ -37,8 +37,8 @@ def _gpu_setup(device):
     return s, kw
 
 
-def _checker_blocks(s, n_blocks):
-    """The checker's blocks with the fields and comb uniforms of the seed's
+def _oracle_blocks(s, n_blocks):
+    """The oracle's blocks with the fields and comb uniforms of the seed's
     stream in the reference's order (prop/afqmc.py:86-88, prop/blocks.py:81-82)."""
     sd = orc.make_system_data(s.h0, s.h1, s.chol, (s.mo_up, s.mo_dn), "restricted", DT)
     prm = orc.params(dt=DT, n_prop_steps=STEPS)
@@ -59,11 +59,11 @@ def _checker_blocks(s, n_blocks):
 
 
 @pytest.mark.parametrize("use_graph", [False, True])
-def test_driver_matches_checker(gpu_device, use_graph):
+def test_driver_matches_oracle(gpu_device, use_graph):
     from ad_afqmc_prototype_amd.driver import run_afqmc_energy
     s, kw = _gpu_setup(gpu_device)
     mu, se, e_b, w_b = run_afqmc_energy(**kw, verbose=False, use_graph=use_graph)
-    e_ref, w_ref, _ = _checker_blocks(s, 15)
+    e_ref, w_ref, _ = _oracle_blocks(s, 15)
     # the walk moves: no block is the clipped or fallback estimate
     assert np.all(np.abs(e_ref[1:] - e_ref[0]) > 1e-6) and np.ptp(e_ref) > 1e-3
     assert e_b.shape == e_ref.shape  # no outlier in this run
@@ -93,7 +93,7 @@ def test_driver_returns_outlier_cleaned_blocks(gpu_device):
         return state, obs
 
     mu, se, e_b, w_b = run_afqmc_energy(**kw, block_fn=planted, verbose=False)
-    e_ref, w_ref, _ = _checker_blocks(s, 15)
+    e_ref, w_ref, _ = _oracle_blocks(s, 15)
     keep = np.ones(15 + 1, dtype=bool)
     keep[9] = False
     assert e_b.shape == (15,)

@@ -1,5 +1,5 @@
 """Generalized walkers (nw, 2n, 2o) in the restricted basis with an RHF trial on
-the HIP path against the CPU checker (fp64, 1e-10 relative): overlap, force
+the HIP path against the CPU oracle (fp64, 1e-10 relative): overlap, force
 bias, Trotter step, a chain of afqmc_step calls with injected fields, QR and
 comb reconfiguration (bit-exact indices).  The reference has no energy kernel
 for these walkers (meas/rhf.py:169-174), so the state starts from a given
@@ -114,7 +114,7 @@ def test_mixed_precision_trotter(gpu_device, precision, tol):
 def test_generalized_basis_trotter(gpu_device):
     """A generalized-basis Hamiltonian (spin-orbital h1, L of size nso = 2n):
     prop ctx (no v0 / v1 correction) and the Trotter step of generalized
-    walkers against the checker (1e-10), and the reference's factory checks."""
+    walkers against the oracle (1e-10), and the reference's factory checks."""
     import torch
     from ad_afqmc_prototype_amd.ham.chol import ham_chol
     from ad_afqmc_prototype_amd.prop.chol_afqmc_ops import make_chol_afqmc_ops

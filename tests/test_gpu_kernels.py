@@ -1,4 +1,4 @@
-"""HIP kernels vs the CPU checker, one entry point at a time (fp64: 1e-10
+"""HIP kernels vs the CPU oracle, one entry point at a time (fp64: 1e-10
 relative, norm-wise; integer outputs bit-exact)."""
 import numpy as np
 import pytest
@@ -176,7 +176,7 @@ def test_trotter_zero_terms_and_identity(gpu_device):
 @pytest.mark.parametrize("exchange", ["auto", "per_g"])
 @pytest.mark.parametrize("seed,n,o,naux,uhf,kind", SHAPES)
 def test_energy(gpu_device, seed, n, o, naux, uhf, kind, exchange):
-    """Local energies against the checker at 1e-10, the exchange both as the
+    """Local energies against the oracle at 1e-10, the exchange both as the
     quadratic form vec(G)^T Y vec(G) ("auto" where n < naux) and on the fused
     per-g kernels."""
     from ad_afqmc_prototype_amd.core.ops import k_energy
@@ -201,7 +201,7 @@ def test_energy(gpu_device, seed, n, o, naux, uhf, kind, exchange):
 ])
 def test_energy_stacked_rows(gpu_device, seed, n, o, naux, uhf, kind):
     """The fp64 energy on the shapes that take stacked g chunks, against the
-    checker at 1e-10."""
+    oracle at 1e-10."""
     test_energy(gpu_device, seed, n, o, naux, uhf, kind, "per_g")
 
 
@@ -235,7 +235,7 @@ def test_comb_indices_bit_exact(gpu_device):
 
 def test_comb_indices_nan_weights(gpu_device):
     """A NaN weight: every tooth lands on the first NaN of the cumulative sum
-    (searchsorted's NaN order), all new weights NaN -- as the checker."""
+    (searchsorted's NaN order), all new weights NaN -- as the oracle."""
     import torch
     from ad_afqmc_prototype_amd import kernels
     rng = np.random.default_rng(17)

@@ -115,7 +115,7 @@ def test_mixed_force_bias(gpu_device, precision, tol):
 
 @pytest.mark.parametrize("precision", ["fp32", "bf16x3"])
 def test_mixed_block_energy_close_to_fp64(gpu_device, precision):
-    """Common random numbers: the mixed run and the fp64 checker see the same
+    """Common random numbers: the mixed run and the fp64 oracle see the same
     fields and comb uniforms; block energies agree to well below 0.1 mHa on the
     24-orbital workload at Cholesky scale 0.25 (the phaseless walk's working
     regime: no local energy is clipped)."""
@@ -172,7 +172,7 @@ def test_mixed_block_energy_close_to_fp64(gpu_device, precision):
 def test_energy_fp32_contractions(gpu_device, case, exchange):
     """fp32 f_g contractions with fp64 sums (or the fp32 G Y product of the
     exchange as a quadratic form): per-walker energies within 2e-5 of
-    |E| of the fp64 checker (two walkers per workgroup, odd walker count).  The
+    |E| of the fp64 oracle (two walkers per workgroup, odd walker count).  The
     random walkers here are far from the trial (|E| up to 2e4 Ha, large
     imaginary parts), the hardest case for fp32."""
     from ad_afqmc_prototype_amd.core.ops import k_energy
@@ -224,7 +224,7 @@ def _planes(e, v, ldv):
     (6, 80, 48, 2),    # more than 32 occupied orbitals (three column tiles)
 ])
 def test_trotter_bf16x3_vs_fp64(gpu_device, nw, n, o, nspin):
-    """Split-bf16 Trotter step against the fp64 step (pinned to the checker)
+    """Split-bf16 Trotter step against the fp64 step (pinned to the oracle)
     on the same E, VHS and walkers, sqrt(dt) ||V||_2 = 0.3, dt = 0.005.  The
     split (~2^-17 relative) enters through sqrt(dt) V / k and E - I, so each
     walker is within 2e-5 (norm-wise relative); the fp32 step within 2e-6."""
@@ -276,7 +276,7 @@ def test_trotter_bf16x3_exact_identity(gpu_device):
 @pytest.mark.parametrize("exchange", ["auto", "per_g"])
 def test_energy_bf16x3_contractions(gpu_device, case, tol, exchange):
     """Split-bf16 f_g contractions with fp64 sums (or the split-bf16 G Y
-    product of the exchange as a quadratic form) against the fp64 checker.
+    product of the exchange as a quadratic form) against the fp64 oracle.
     The split keeps ~2^-17 of each operand, and that error enters the
     Coulomb square and the exchange coherently, so the bound is relative to
     |E|: 2e-6 for walkers near the trial at the working Cholesky scale, 1e-5

@@ -1,4 +1,4 @@
-"""Composed step / block on the HIP path vs the CPU checker, with injected
+"""Composed step / block on the HIP path vs the CPU oracle, with injected
 fields and comb uniforms (fp64: 1e-10 relative; reconfiguration indices
 bit-exact)."""
 import numpy as np
@@ -63,7 +63,7 @@ def _cmp_state(gs, st, wtol=TOL):
     dict(seed=2, n=200, o=32, g=1000, nw=24, uhf=False, kind="restricted", dt=0.005, steps=1),
     dict(seed=9, n=60, o=48, g=90, nw=12, uhf=True, kind="unrestricted", dt=0.01, steps=2),
 ])
-def test_step_trajectory_matches_checker(gpu_device, case):
+def test_step_trajectory_matches_oracle(gpu_device, case):
     p = make_pair(case["seed"], case["n"], case["o"], case["g"], uhf=case["uhf"], kind=case["kind"],
                   dt=case["dt"], device=gpu_device)
     st = orc.init_state(p.sd, case["nw"], overlap_mode="log")
@@ -165,7 +165,7 @@ def _gpu_block(p, gs, fields, zeta, n_prop, n_ortho):
     dict(seed=6, n=30, o=5, g=90, nw=64, uhf=True, kind="unrestricted", dt=0.01, steps=20, qr=5, sr=10,
          scale=0.25),
 ])
-def test_blocks_match_checker(gpu_device, case):
+def test_blocks_match_oracle(gpu_device, case):
     import torch
     p = make_pair(case["seed"], case["n"], case["o"], case["g"], uhf=case["uhf"], kind=case["kind"],
                   dt=case["dt"], chol_scale=case["scale"], device=gpu_device)

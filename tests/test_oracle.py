@@ -1,4 +1,4 @@
-"""Pins the CPU checker (oracle/afqmc_oracle.py) against closed forms,
+"""Pins the CPU oracle (oracle/afqmc_oracle.py) against closed forms,
 invariants, finite differences of the overlap and brute force.
 
 The reference's own tests hold no numeric fixtures (they are closed-form and

@@ -1,4 +1,4 @@
-"""Pins the checker's field shifts, importance function, block end and comb
+"""Pins the oracle's field shifts, importance function, block end and comb
 against what the mathematics fixes, independently of the formulas restated in
 oracle/afqmc_oracle.py (prop/afqmc.py:97-138, prop/blocks.py:59-92,
 walkers.py:233-265 of the reference).
@@ -14,7 +14,7 @@ walkers.py:233-265 of the reference).
   determinants over every occupation of L's eigen-orbitals (Cauchy-Binet).
   This fixes h0_prop, the mean-field shifts, h1_eff and the exponent's sign
   conventions, which the shift identity alone leaves open.
-* Force-bias flatness.  With the checker's own force bias, d log(imp)/dx at
+* Force-bias flatness.  With the oracle's own force bias, d log(imp)/dx at
   x = 0 vanishes to leading order: what is left comes from the one-body halves
   and the O(sqrt dt) shifted field acting on the mixed estimate, O(dt^{3/2}),
   so quartering dt divides it by 8.  A flipped shift leaves O(sqrt dt)
@@ -22,7 +22,7 @@ walkers.py:233-265 of the reference).
 * The phase depends on the shifted fields only.
 * Block-end and comb hand cases with known answers.
 
-Each check is also run on a planted mistake (a mutated copy of the checker's
+Each check is also run on a planted mistake (a mutated copy of the oracle's
 function, on the CPU) and must reject it.
 """
 import itertools
@@ -97,7 +97,7 @@ def _integral(sd, w, fb, *, pop, n_terms=6, k=40, mode="log", fields_at=None,
 
 
 # --------------------------------------------------------------------------
-# planted mistakes (mutated copies of the checker's functions)
+# planted mistakes (mutated copies of the oracle's functions)
 # --------------------------------------------------------------------------
 
 def _shift_plus_half(fields, fb, pctx):
@@ -243,14 +243,14 @@ def test_importance_integral_is_exact_propagator(kind, uhf, fb_kind):
     ref, den = _brute_force(sd, w, lmat, h1, h0, dt, lambda e: math.exp(-dt * e))
     ref = ref * math.exp(dt * pop)
     assert abs(val - ref) / abs(ref) < 1e-11, (val, ref)
-    # the expansion's overlap is the checker's overlap (Cauchy-Binet)
+    # the expansion's overlap is the oracle's overlap (Cauchy-Binet)
     ov = orc.overlap(_bcast(w, 1), sd.trial, sd.kind)[0]
     assert abs(den - ov) / abs(ov) < 1e-12
 
 
 @pytest.mark.parametrize("kind,uhf", [("restricted", False), ("unrestricted", True)])
 def test_energy_is_brute_force_mixed_estimator(kind, uhf):
-    """<T|H|w>/<T|w> from the occupation expansion equals the checker's
+    """<T|H|w>/<T|w> from the occupation expansion equals the oracle's
     Green's-function energy (meas/rhf.py:36-50,75-105)."""
     sd, w, lmat, h1, h0 = _commuting_system(kind, uhf, 0.01, seed=8, n=6, o=2)
     ref, _ = _brute_force(sd, w, lmat, h1, h0, 0.01, lambda e: e)
