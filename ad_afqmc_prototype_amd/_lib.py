@@ -85,6 +85,8 @@ _SIGNATURES = {
     "afq_overlap_greens_grouped": [c_int, c_vp, c_int, c_int, c_vp],
     "afq_overlap_greens_bf16x3_grouped": [c_int, c_vp, c_int, c_int, c_vp],
     "afq_phaseless_update_grouped": [c_int, c_vp, c_vp],
+    "afq_block_end_grouped": [c_int, c_vp, c_double, c_double, c_vp, c_vp, c_vp, c_vp],
+    "afq_reconfigure_gather_grouped": [c_int, c_vp, c_int, c_vp, c_vp, c_vp, c_vp],
     "afq_orthonormalize_grouped": [c_int, c_vp, c_int, c_int, c_int, c_vp],
     "afq_energy_f32_grouped": [c_int, c_vp, c_int, c_int, c_int, c_vp],
     "afq_energy_bf16x3_grouped": [c_int, c_vp, c_int, c_int, c_int, c_int, c_vp],
@@ -98,7 +100,7 @@ _SIGNATURES = {
     "afq_unpack_pairs": [c_int, c_int, c_vp, c_ll, c_vp, c_vp],
 }
 
-ABI_VERSION = 20
+ABI_VERSION = 21
 
 
 class HipLibraryError(RuntimeError):

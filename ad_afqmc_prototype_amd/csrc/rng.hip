@@ -19,46 +19,9 @@
 
 #include "afqmc_hip.h"
 #include "common.h"
+#include "device_util.h"
 
 namespace {
-
-__device__ __forceinline__ uint32_t rotl32(uint32_t x, int r) { return (x << r) | (x >> (32 - r)); }
-
-__device__ __forceinline__ void threefry2x32(uint32_t k0, uint32_t k1, uint32_t& x0, uint32_t& x1) {
-    const uint32_t k2 = k0 ^ k1 ^ 0x1BD11BDAu;
-    const uint32_t ks[3] = {k0, k1, k2};
-    constexpr int R[8] = {13, 15, 26, 6, 17, 29, 16, 24};
-    x0 += k0;
-    x1 += k1;
-#pragma unroll
-    for (int r = 0; r < 20; ++r) {
-        x0 += x1;
-        x1 = rotl32(x1, R[r & 7]);
-        x1 ^= x0;
-        if ((r & 3) == 3) {
-            const int s = (r + 1) >> 2;
-            x0 += ks[s % 3];
-            x1 += ks[(s + 1) % 3] + (uint32_t)s;
-        }
-    }
-}
-
-// (new key, subkey) = split(key)
-__device__ __forceinline__ void split_key(const uint32_t* key, uint32_t (&nk)[2], uint32_t (&sk)[2]) {
-    uint32_t a0 = 0, b0 = 2, a1 = 1, b1 = 3;
-    threefry2x32(key[0], key[1], a0, b0);
-    threefry2x32(key[0], key[1], a1, b1);
-    nk[0] = a0;
-    nk[1] = a1;
-    sk[0] = b0;
-    sk[1] = b1;
-}
-
-// [0, 1) from 64 random bits: the top 52 bits as the mantissa of [1, 2), minus 1
-__device__ __forceinline__ double unit_from_bits(uint32_t hi, uint32_t lo) {
-    const uint64_t bits = ((uint64_t)hi << 32) | lo;
-    return __longlong_as_double((long long)((bits >> 12) | 0x3FF0000000000000ull)) - 1.0;
-}
 
 // sqrt(2) erfinv(u) for u in (-1, 1): closed-form start (|rel err| < 4e-3)
 // and two Halley steps in fp32 (~1e-7 -> below fp32 rounding), then one
@@ -134,9 +97,7 @@ __global__ __launch_bounds__(256) void fields_grouped_kernel(int count,
 }
 
 __global__ void uniform_kernel(const uint32_t* __restrict__ key, double* __restrict__ out) {
-    uint32_t a = 0, b = 1;
-    threefry2x32(key[0], key[1], a, b);
-    out[0] = fmax(0.0, unit_from_bits(a, b));
+    out[0] = uniform_value(key[0], key[1]);
 }
 
 __global__ void threefry_kernel(uint32_t k0, uint32_t k1, long long n, const uint32_t* __restrict__ c0,

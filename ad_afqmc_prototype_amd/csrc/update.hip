@@ -5,29 +5,9 @@
 
 #include "afqmc_hip.h"
 #include "common.h"
+#include "device_util.h"
 
 namespace {
-
-template <int T>
-__device__ __forceinline__ void block_sum(double (&v)[4], double* sh) {
-    // sh: 4 * (T / 64) doubles
-    const int lane = threadIdx.x & 63;
-    const int wave = threadIdx.x >> 6;
-#pragma unroll
-    for (int k = 0; k < 4; ++k) v[k] = afq::wave_sum(v[k]);
-    if (lane == 0)
-#pragma unroll
-        for (int k = 0; k < 4; ++k) sh[wave * 4 + k] = v[k];
-    __syncthreads();
-    if (threadIdx.x == 0) {
-#pragma unroll
-        for (int k = 0; k < 4; ++k) {
-            double s = 0.0;
-            for (int q = 0; q < T / 64; ++q) s += sh[q * 4 + k];
-            v[k] = s;
-        }
-    }
-}
 
 // One walker: row w of the (nw, naux) arrays (planes nw * naux apart).
 // SPLIT: also the split-bf16 planes of the shifted fields, rows w (real) and
@@ -181,21 +161,7 @@ __global__ __launch_bounds__(1024) void block_energy_kernel(
     int nw, const double2* __restrict__ e, const double* __restrict__ w, double e_ref,
     double thresh, double alpha, const double* __restrict__ pop, double* __restrict__ out) {
     __shared__ double sh[64];
-    double acc[4] = {0.0, 0.0, 0.0, 0.0};
-    for (int k = threadIdx.x; k < nw; k += 1024) {
-        double er = e[k].x;
-        if (fabs(er - e_ref) > thresh) er = e_ref;
-        acc[0] += w[k] * er;
-        acc[1] += w[k];
-    }
-    block_sum<1024>(acc, sh);
-    if (threadIdx.x == 0) {
-        const double wsum = acc[1];
-        const double eb = (wsum == 0.0) ? e_ref : acc[0] / wsum;
-        out[0] = eb;
-        out[1] = wsum;
-        out[2] = (1.0 - alpha) * (*pop) + alpha * eb;
-    }
+    block_energy_body(nw, e, w, e_ref, thresh, alpha, pop, out, out + 1, out + 2, sh);
 }
 
 }  // namespace

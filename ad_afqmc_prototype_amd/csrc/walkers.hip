@@ -6,6 +6,7 @@
 
 #include "afqmc_hip.h"
 #include "common.h"
+#include "device_util.h"
 
 namespace {
 
@@ -280,51 +281,15 @@ void launch_cholqr_grouped(int count, int total, const afq_system_step* sys, hip
     hipLaunchKernelGGL(cholqr_grouped_kernel<MAXO>, grid, dim3(THREADS), 0, st, count, total, 1, sys);
 }
 
-// ---------------------------------------------------------------------------
-// Comb indices.  The cumulative sum is one sequential left-to-right fp64 pass
-// (numpy.cumsum order) and z_k = (tot * (k + zeta)) / nw is evaluated without
-// contraction, so the indices are reproducible bit for bit.  The search uses
-// searchsorted's order, in which NaN sorts above every number: with a NaN
-// weight the total and every tooth are NaN, and each tooth lands on the first
-// NaN of the cumulative sum (the walker whose weight is NaN).  An index past
-// the end is clamped to nw - 1, as an out-of-range gather index is.
-// ---------------------------------------------------------------------------
+// Comb indices of one walker batch (comb_body): zeta from the argument, or
+// from the device when zeta_p is set.
 __global__ __launch_bounds__(1024) void comb_kernel(int nw, const double* __restrict__ weights,
                                                      double zeta_v, const double* __restrict__ zeta_p,
                                                      double* __restrict__ cw,
                                                      int* __restrict__ idx,
                                                      double* __restrict__ wnew) {
-#pragma clang fp contract(off)
     __shared__ double tot_s;
-    if (threadIdx.x == 0) {
-        double s = 0.0;
-        for (int i = 0; i < nw; ++i) {
-            s = s + fabs(weights[i]);
-            cw[i] = s;
-        }
-        tot_s = s;
-        __threadfence();
-    }
-    __syncthreads();
-    const double tot = tot_s;
-    const double avg = tot / (double)nw;
-    const double zeta = zeta_p ? zeta_p[0] : zeta_v;
-    for (int k = threadIdx.x; k < nw; k += blockDim.x) {
-        const double kz = (double)k + zeta;
-        const double prod = tot * kz;
-        const double z = prod / (double)nw;
-        int lo = 0, hi = nw;
-        while (lo < hi) {
-            const int mid = (lo + hi) >> 1;
-            const double c = cw[mid];
-            if (c < z || (z != z && c == c))
-                lo = mid + 1;
-            else
-                hi = mid;
-        }
-        idx[k] = lo < nw ? lo : nw - 1;
-        wnew[k] = avg;
-    }
+    comb_body(nw, weights, zeta_p ? zeta_p[0] : zeta_v, cw, idx, wnew, &tot_s);
 }
 
 __global__ void gather_kernel(int nrows, long long row_len, int nplanes, long long sps,

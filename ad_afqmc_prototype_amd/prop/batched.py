@@ -173,7 +173,7 @@ class system_batch:
                  sr_every: int = 20, n_exp_terms: int = 6, weight_floor: float = 1.0e-3,
                  weight_cap: float = 100.0, damping: float = 0.1, shift_ema: float = 0.1,
                  gemm_shapes: tuple[int, int] | None = None, precision: str = "bf16x3",
-                 merge_one_body_halves: bool = True):
+                 merge_one_body_halves: bool = True, grouped_block_end: bool = True):
         if not systems:
             raise ValueError("empty batch")
         if precision not in ("bf16x3", "fp32"):
@@ -184,6 +184,9 @@ class system_batch:
         # walkers are E^-1 W, their Green's functions are taken with the trial
         # E C and the force bias with C^T L E
         self.merge = bool(merge_one_body_halves) and precision == "bf16x3"
+        # the block end of every system in two launches (afq_block_end_grouped,
+        # afq_reconfigure_gather_grouped) instead of a dozen per system
+        self.grouped_block_end = bool(grouped_block_end)
         self.deferred = False
         if qr_every % 2 or sr_every % 2 or qr_every <= 0 or sr_every <= 0:
             raise ValueError("qr_every and sr_every must be positive and even")
@@ -312,6 +315,10 @@ class system_batch:
         self.weights = [P([m["nw"] for m in ms], f64) for _ in range(2)]
         self.pop = [torch.zeros((self.count,), dtype=f64, device=dev) for _ in range(2)]
         self.keys = [torch.zeros((self.count, 2), dtype=torch.int32, device=dev) for _ in range(2)]
+        if self.grouped_block_end:
+            self.comb_cw = torch.empty((sum(m["nw"] for m in ms),), dtype=f64, device=dev)
+            self.comb_idx = torch.empty((sum(m["nw"] for m in ms),), dtype=torch.int32, device=dev)
+            self.greens_tmp = torch.zeros_like(self.greens.buf)  # the gaps between systems stay zero
         self.mf = P([m["g"] for m in ms], c128)
         self.trials = P([m["n"] * m["o"] for m in ms], f64)
         for i, m in enumerate(ms):
@@ -615,7 +622,10 @@ class system_batch:
     def _measure_reconfigure(self):
         """Per system: clipped local energies, block energy and shift EMA
         (prop/blocks.py:59-79), comb with a uniform from the system's key
-        (prop/blocks.py:81-92).  Called right after _orthonormalize."""
+        (prop/blocks.py:81-92).  Called right after _orthonormalize.  With
+        grouped_block_end every system's block end is one launch and the
+        gather another; otherwise a dozen launches per system (the same
+        results bit for bit)."""
         thresh = (2.0 / self.dt) ** 0.5
         if self.quadratic:
             # c (the force bias of the measured walkers), Z = G Y, then the
@@ -635,6 +645,17 @@ class system_batch:
             else:
                 kernels._call("afq_energy_f32_grouped", t, bk["count"], t.data_ptr(), bk["walkers"],
                               bk["max_n"], bk["nt"], kernels._stream(t))
+        if self.grouped_block_end:
+            d = self.desc[0]
+            st = kernels._stream(d)
+            kernels._call("afq_block_end_grouped", d, self.count, d.data_ptr(), thresh, float(self.shift_ema),
+                          self.block_stats.data_ptr(), self.comb_cw.data_ptr(), self.comb_idx.data_ptr(), st)
+            kernels._call("afq_reconfigure_gather_grouped", d, self.count, d.data_ptr(), self.total_walkers,
+                          self.comb_idx.data_ptr(), self.greens.buf.data_ptr(), self.greens_tmp.data_ptr(), st)
+            self.greens.buf.copy_(self.greens_tmp)
+            if self.precision == "bf16x3":
+                self._split("fb")
+            return
         for i, m in enumerate(self.members):
             nw, n, o = m["nw"], m["n"], m["o"]
             g = self.greens.part(i, (2, nw, o * n))

@@ -661,6 +661,33 @@ int afq_overlap_greens_bf16x3_grouped(int count, const afq_system_step* sys, int
 int afq_phaseless_update_grouped(int count, const afq_system_step* sys, void* stream);
 
 /*
+ * The block end of every system up to the comb, one workgroup each
+ * (prop/blocks.py:59-92; per system afq_block_energy, afq_rng_split,
+ * afq_rng_uniform and afq_comb_indices_dev, with the same results bit for
+ * bit): from energy (real parts), weights_in, pop_in, e_estimate and key_in,
+ * stats[2 i] = the clipped block energy of system i (local energies farther
+ * than thresh from e_estimate count as e_estimate), stats[2 i + 1] = its weight
+ * sum; in place pop_in = (1 - shift_ema) pop_in + shift_ema stats[2 i],
+ * (key_in, sub) = split(key_in), weights_in = the comb's weights for
+ * zeta = uniform(sub).  idx (int32) and cw (float64, caller scratch): one
+ * entry per walker of the batch at walker0; idx gets the comb indices.
+ */
+int afq_block_end_grouped(int count, const afq_system_step* sys, double thresh, double shift_ema, double* stats,
+                          double* cw, int* idx, void* stream);
+
+/*
+ * The reconfiguration gather of every system, one workgroup per walker
+ * (prop/blocks.py:81-92, walkers.py:233-265 per system; afq_gather_rows per
+ * array): with j = idx[walker0 + k], walkers_in[k] = walkers_out[j],
+ * log_overlaps_in[k] = 2 logdet[j], and both planes of the Green's-function
+ * rows greens[p, j] go to row k of the same planes at greens_dst +
+ * (greens - greens_src) (a buffer laid out like the one greens_src starts,
+ * not overlapping it).  idx: afq_block_end_grouped's.
+ */
+int afq_reconfigure_gather_grouped(int count, const afq_system_step* sys, int total_walkers, const int* idx,
+                                   const double* greens_src, double* greens_dst, void* stream);
+
+/*
  * afq_energy_f32 (restricted walkers) for every system: local energies of the
  * Green's functions in `greens` into `energy` (meas/rhf.py:36-50).  Every
  * system of the table must have ceil(o / 16) == nt (1 or 2); max_n the largest n.

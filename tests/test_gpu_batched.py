@@ -6,6 +6,8 @@
   system run alone with afqmc_step / orthonormalize_state / measure /
   reconfigure in fp32 precision: walkers, weights, log-overlaps, population
   shifts, keys and block energies, bit for bit (the same kernel bodies);
+* the block end in two grouped launches equals the per-system launches bit
+  for bit, in both precisions;
 * a block captured as a HIP graph and replayed equals the eager batch;
 * the grouped bf16x3 GEMM and operand split equal their single-problem
   entry points bit for bit, and a batch in bf16x3 precision (contractions and
@@ -139,6 +141,39 @@ def test_batch_matches_systems_run_alone(gpu_device, shapes):
     # the run is live: weights moved and the comb copied walkers
     w = batch.system_state(3)["weights"]
     assert float(w.min()) > 0.0 and np.isfinite(per_block[-1]).all()
+
+
+@pytest.mark.parametrize("precision", ["bf16x3", "fp32"])
+def test_grouped_block_end_matches_per_system(gpu_device, precision):
+    """The block end in two grouped launches (afq_block_end_grouped,
+    afq_reconfigure_gather_grouped) against the per-system launches: after
+    each of two blocks the block energies and weight sums, walkers, weights,
+    log-overlaps, population shifts, keys, the Green's-function rows (and in
+    bf16x3 their split planes) are equal bit for bit."""
+    import torch
+    from ad_afqmc_prototype_amd.prop.batched import system_batch
+    print(f"grouped vs per-system block end ({precision})", flush=True)
+    a = system_batch(_members(gpu_device), dt=DT, qr_every=QR, sr_every=SR, precision=precision)
+    b = system_batch(_members(gpu_device), dt=DT, qr_every=QR, sr_every=SR, precision=precision,
+                     grouped_block_end=False)
+    assert a.grouped_block_end and not b.grouped_block_end
+    for blk in range(2):
+        a.advance(SR)
+        b.advance(SR)
+        torch.cuda.synchronize()
+        assert torch.equal(a.block_stats, b.block_stats), f"block {blk}: block stats"
+        for i in range(len(SYSTEMS)):
+            x, y = a.system_state(i), b.system_state(i)
+            for k in x:
+                assert torch.equal(x[k], y[k]), f"block {blk}, system {i}: {k}"
+        assert torch.equal(a.greens.buf, b.greens.buf), f"block {blk}: Green's-function rows"
+        if precision == "bf16x3":
+            for p in range(2):
+                assert torch.equal(a.greens_hl[p].buf, b.greens_hl[p].buf), f"block {blk}: split plane {p}"
+    # the run is live and the comb ran (it leaves every weight of a system equal)
+    assert float(a.block_stats[:, 1].min()) > 0.0
+    w = [a.system_state(i)["weights"] for i in range(len(SYSTEMS))]
+    assert all(torch.all(x == x[0]) for x in w)     # the comb leaves equal weights
 
 
 @pytest.mark.parametrize("precision", ["bf16x3", "fp32"])
