@@ -239,6 +239,9 @@ __device__ __forceinline__ rsrc_t make_rsrc(const void* p, unsigned long long by
 // per workgroup of 2 WM waves (WM x 2 waves of 64 x 128 each).  A k-step
 // moves the four operand planes (A hi, A lo: BM rows; B hi, B lo: 256 rows;
 // 64 bytes per row) in 1 KB blocks of 16 rows, BPW blocks per wave.
+// With WM = 4 each wave's blocks lie in one plane (waves 0..3 move A, 4..7 B)
+// and the B planes run three k-steps deep (2 x 32 + 3 x 32 KB = the CU's
+// 160 KB of LDS): the B movers issue two k-steps ahead, the A movers one.
 template <int WM>
 struct dma_shape {
     static constexpr int BM = 64 * WM;
@@ -246,14 +249,19 @@ struct dma_shape {
     static constexpr int ABLK = BM / 16;            // blocks per A plane
     static constexpr int NBLK = 2 * ABLK + 2 * 16;  // blocks per k-step
     static constexpr int BPW = NBLK / (2 * WM);     // blocks per wave
-    static constexpr int STAGE = NBLK * 64;         // uint4 per k-step
-    __host__ __device__ static constexpr int plane_off(int p) {
-        return p < 2 ? p * ABLK * 64 : 2 * ABLK * 64 + (p - 2) * 16 * 64;
+    static constexpr int NB = WM == 4 ? 3 : 2;      // B stages
+    static constexpr int A_STAGE = 2 * ABLK * 64;   // uint4 of one k-step's A planes
+    static constexpr int B_STAGE = 2 * 16 * 64;     // uint4 of one k-step's B planes
+    static constexpr int SIZE = 2 * A_STAGE + NB * B_STAGE;
+    // first uint4 of plane p (A hi, A lo, B hi, B lo) of k-step kt
+    __host__ __device__ static constexpr int plane_at(int p, int kt) {
+        return p < 2 ? (kt & 1) * A_STAGE + p * ABLK * 64
+                     : 2 * A_STAGE + (kt % NB) * B_STAGE + (p - 2) * 16 * 64;
     }
 };
 
 // acc[i][j] (rows wm*64 + 16 i, columns wn*128 + 16 j of the tile at m0, n0)
-// = A B^T over the whole K, with S the double-buffered stages (2 STAGE uint4).
+// = A B^T over the whole K, with S the staged planes (SIZE uint4).
 template <int WM>
 __device__ __forceinline__ void dma_mainloop(int M, int N, int K, const uint16_t* __restrict__ Ahi,
                                              const uint16_t* __restrict__ Alo, long long lda,
@@ -289,7 +297,7 @@ __device__ __forceinline__ void dma_mainloop(int M, int N, int K, const uint16_t
     const rsrc_t R1 = WM == 4 ? R4 : plane_rsrc(1);
     const rsrc_t R2 = WM == 4 ? R4 : plane_rsrc(2);
     const rsrc_t R3 = WM == 4 ? R4 : plane_rsrc(3);
-    auto issue_stage = [&](int kt, int buf) {
+    auto issue_stage = [&](int kt) {
         const int k = kt * 32 + chunk * 8;
         // k >= K sets the top bit: the offset lands outside the descriptor
         const unsigned past = (unsigned)((K - 1 - k) >> 31) & kOutside;
@@ -310,8 +318,8 @@ __device__ __forceinline__ void dma_mainloop(int M, int N, int K, const uint16_t
             const long long ld = plane < 2 ? lda : ldb;
             const int row = blk * 16 + r_loc;
             const unsigned off = (unsigned)(((long long)row * ld + k) * 2) | past;
-            __builtin_amdgcn_raw_ptr_buffer_load_lds(
-                R, (lds_ptr_t)&S[buf * D::STAGE + D::plane_off(plane) + blk * 64], 16, off, 0, 0, 0);
+            __builtin_amdgcn_raw_ptr_buffer_load_lds(R, (lds_ptr_t)&S[D::plane_at(plane, kt) + blk * 64], 16, off,
+                                                     0, 0, 0);
         }
     };
 
@@ -326,19 +334,31 @@ __device__ __forceinline__ void dma_mainloop(int M, int N, int K, const uint16_t
     const int lane_slot = lr * 4 + (q ^ ((lr >> 1) & 3));
 
     const int nk = afq::ceil_div(K, 32);
-    issue_stage(0, 0);
-    asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+    // a wave that moves B with three B stages runs two k-steps ahead: it keeps
+    // its eight loads of the later k-step in flight at each wait
+    const bool deep = D::NB == 3 && plane4 >= 2;
+    issue_stage(0);
+    if (deep && nk > 1) {
+        issue_stage(1);
+        asm volatile("s_waitcnt vmcnt(8)" ::: "memory");
+    } else {
+        asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+    }
     __syncthreads();
     for (int kt = 0; kt < nk; ++kt) {
-        const int buf = kt & 1;
-        if (kt + 1 < nk) issue_stage(kt + 1, buf ^ 1);
-        const uint4* Sb = S + buf * D::STAGE;
+        const int ka = deep ? kt + 2 : kt + 1;  // the k-step this wave issues now
+        const bool ahead = ka < nk;
+        if (ahead) issue_stage(ka);
+        const uint4* A0 = S + D::plane_at(0, kt);
+        const uint4* A1 = S + D::plane_at(1, kt);
+        const uint4* B0 = S + D::plane_at(2, kt);
+        const uint4* B1 = S + D::plane_at(3, kt);
         bf16x8 ah[4], al[4];
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
             const int slot = (wm * 64 + i * 16) * 4 + lane_slot;
-            ah[i] = __builtin_bit_cast(bf16x8, Sb[D::plane_off(0) + slot]);
-            al[i] = __builtin_bit_cast(bf16x8, Sb[D::plane_off(1) + slot]);
+            ah[i] = __builtin_bit_cast(bf16x8, A0[slot]);
+            al[i] = __builtin_bit_cast(bf16x8, A1[slot]);
         }
 #pragma unroll
         for (int jh = 0; jh < 2; ++jh) {
@@ -346,8 +366,8 @@ __device__ __forceinline__ void dma_mainloop(int M, int N, int K, const uint16_t
 #pragma unroll
             for (int j = 0; j < 4; ++j) {
                 const int slot = (wn * 128 + (jh * 4 + j) * 16) * 4 + lane_slot;
-                bh[j] = __builtin_bit_cast(bf16x8, Sb[D::plane_off(2) + slot]);
-                bl[j] = __builtin_bit_cast(bf16x8, Sb[D::plane_off(3) + slot]);
+                bh[j] = __builtin_bit_cast(bf16x8, B0[slot]);
+                bl[j] = __builtin_bit_cast(bf16x8, B1[slot]);
             }
 #pragma unroll
             for (int i = 0; i < 4; ++i)
@@ -359,8 +379,11 @@ __device__ __forceinline__ void dma_mainloop(int M, int N, int K, const uint16_t
                     c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bh[j], c, 0, 0, 0);
                 }
         }
-        // the next stage has landed in LDS and every wave is done with this one
-        asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+        // k-step kt + 1 has landed in LDS and every wave is done with this one
+        if (deep && ahead)
+            asm volatile("s_waitcnt vmcnt(8)" ::: "memory");
+        else
+            asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
         __syncthreads();
     }
 }
@@ -392,8 +415,11 @@ __global__ __launch_bounds__(THREADS2, 1) void gemm_bf16x3_nt_256_kernel(
     const uint16_t* __restrict__ Alo, long long lda, const uint16_t* __restrict__ Bhi,
     const uint16_t* __restrict__ Blo, long long ldb, TOut* __restrict__ C, long long ldc,
     int tiles_m, int tiles_n, afq::sym_dst sd = afq::sym_dst{0, 0, 0}) {
-    // [stage][plane: A hi, A lo, B hi, B lo][256 rows x 64 bytes]
-    __shared__ __attribute__((aligned(1024))) uint4 S[2][4][BM2 * 4];
+    // [stage][plane: hi, lo][256 rows x 64 bytes]: A two k-steps deep, B three
+    // (the CU's 160 KB of LDS), so the B movers (waves 4..7) issue two k-steps
+    // ahead and the A movers (waves 0..3) one
+    __shared__ __attribute__((aligned(1024))) uint4 SA[2][2][BM2 * 4];
+    __shared__ __attribute__((aligned(1024))) uint4 SB[3][2][BN2 * 4];
 
     const int total = tiles_m * tiles_n;
     const int t = afq::xcd_contiguous(blockIdx.x, total);
@@ -425,8 +451,10 @@ __global__ __launch_bounds__(THREADS2, 1) void gemm_bf16x3_nt_256_kernel(
     const rsrc_t R = make_rsrc(src, (unsigned long long)rows_left * ld * 2);
     const int r_loc = lane >> 2;
     const int chunk = (lane & 3) ^ ((lane >> 3) & 3);
-    auto issue_stage = [&](int kt, int buf) {
+    const bool deep = plane >= 2;
+    auto issue_stage = [&](int kt) {
         const int k = kt * 32 + chunk * 8;
+        uint4* dst = deep ? &SB[kt % 3][plane - 2][0] : &SA[kt & 1][plane][0];
 #pragma unroll
         for (int i = 0; i < 8; ++i) {
             const int blk = (wave & 1) * 8 + i;
@@ -434,7 +462,7 @@ __global__ __launch_bounds__(THREADS2, 1) void gemm_bf16x3_nt_256_kernel(
             // k >= K sets the top bit: the offset lands outside the descriptor
             const unsigned past = (unsigned)((K - 1 - k) >> 31) & kOutside;
             const unsigned off = (unsigned)(((long long)row * ld + k) * 2) | past;
-            __builtin_amdgcn_raw_ptr_buffer_load_lds(R, (lds_ptr_t)&S[buf][plane][blk * 64], 16, off, 0, 0, 0);
+            __builtin_amdgcn_raw_ptr_buffer_load_lds(R, (lds_ptr_t)(dst + blk * 64), 16, off, 0, 0, 0);
         }
     };
 
@@ -450,18 +478,28 @@ __global__ __launch_bounds__(THREADS2, 1) void gemm_bf16x3_nt_256_kernel(
     const int lane_slot = lr * 4 + (q ^ ((lr >> 1) & 3));
 
     const int nk = afq::ceil_div(K, 32);
-    issue_stage(0, 0);
-    asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+    issue_stage(0);
+    if (deep && nk > 1) {
+        issue_stage(1);
+        asm volatile("s_waitcnt vmcnt(8)" ::: "memory");
+    } else {
+        asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+    }
     __syncthreads();
     for (int kt = 0; kt < nk; ++kt) {
-        const int buf = kt & 1;
-        if (kt + 1 < nk) issue_stage(kt + 1, buf ^ 1);
+        const int ka = deep ? kt + 2 : kt + 1;  // the k-step this wave issues now
+        const bool ahead = ka < nk;
+        if (ahead) issue_stage(ka);
+        const uint4* A0 = SA[kt & 1][0];
+        const uint4* A1 = SA[kt & 1][1];
+        const uint4* B0 = SB[kt % 3][0];
+        const uint4* B1 = SB[kt % 3][1];
         bf16x8 ah[4], al[4];
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
             const int slot = (wm * 64 + i * 16) * 4 + lane_slot;
-            ah[i] = __builtin_bit_cast(bf16x8, S[buf][0][slot]);
-            al[i] = __builtin_bit_cast(bf16x8, S[buf][1][slot]);
+            ah[i] = __builtin_bit_cast(bf16x8, A0[slot]);
+            al[i] = __builtin_bit_cast(bf16x8, A1[slot]);
         }
 #pragma unroll
         for (int jh = 0; jh < 2; ++jh) {
@@ -469,8 +507,8 @@ __global__ __launch_bounds__(THREADS2, 1) void gemm_bf16x3_nt_256_kernel(
 #pragma unroll
             for (int j = 0; j < 4; ++j) {
                 const int slot = (wn * 128 + (jh * 4 + j) * 16) * 4 + lane_slot;
-                bh[j] = __builtin_bit_cast(bf16x8, S[buf][2][slot]);
-                bl[j] = __builtin_bit_cast(bf16x8, S[buf][3][slot]);
+                bh[j] = __builtin_bit_cast(bf16x8, B0[slot]);
+                bl[j] = __builtin_bit_cast(bf16x8, B1[slot]);
             }
 #pragma unroll
             for (int i = 0; i < 4; ++i)
@@ -482,8 +520,12 @@ __global__ __launch_bounds__(THREADS2, 1) void gemm_bf16x3_nt_256_kernel(
                     c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bh[j], c, 0, 0, 0);
                 }
         }
-        // the next stage has landed in LDS and every wave is done with this one
-        asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+        // k-step kt + 1 has landed in LDS and every wave is done with this one
+        // (a B mover keeps its eight loads of k-step kt + 2 in flight)
+        if (deep && ahead)
+            asm volatile("s_waitcnt vmcnt(8)" ::: "memory");
+        else
+            asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
         __syncthreads();
     }
 
@@ -496,7 +538,7 @@ __global__ __launch_bounds__(THREADS2, 1) void gemm_bf16x3_nt_256_kernel(
         // row pi at pi * PP (both padded: the fills and the column reads hit 64
         // distinct banks).
         constexpr int RP = 324, PP = 20;
-        float* L = reinterpret_cast<float*>(&S[0][0][0]);
+        float* L = reinterpret_cast<float*>(&SB[0][0][0]);  // 64 x RP floats fit in B's 96 KB
         float* Cf = reinterpret_cast<float*>(C);
         int bi, bj;
         afq::sym_block(n0 >> 8, sd.nb, bi, bj);
@@ -553,7 +595,7 @@ template <int WM>
 __global__ __launch_bounds__(128 * WM, 1) void gemm_bf16x3_grouped_dma_kernel(
     int count, int total, const afq_gemm_problem_bf16x3* __restrict__ P) {
     using D = dma_shape<WM>;
-    __shared__ __attribute__((aligned(1024))) uint4 S[2 * D::STAGE];
+    __shared__ __attribute__((aligned(1024))) uint4 S[D::SIZE];
     const int t = afq::xcd_contiguous(blockIdx.x, total);
     const int pi = afq::find_group(count, t, [&](int i) { return P[i].tile_start; });
     const afq_gemm_problem_bf16x3& p = P[pi];
