@@ -409,12 +409,23 @@ __device__ __forceinline__ void dma_store(int M, int N, float alpha, TOut* __res
         }
 }
 
+// Split K as in gemm_bf16x3_nt_kernel: blockIdx.y = kz takes k in
+// [kz kc, (kz + 1) kc) and writes its partial product to C + kz c_stride.
 template <typename TOut, bool SYM = false>
 __global__ __launch_bounds__(THREADS2, 1) void gemm_bf16x3_nt_256_kernel(
     int M, int N, int K, float alpha, const uint16_t* __restrict__ Ahi,
     const uint16_t* __restrict__ Alo, long long lda, const uint16_t* __restrict__ Bhi,
     const uint16_t* __restrict__ Blo, long long ldb, TOut* __restrict__ C, long long ldc,
-    int tiles_m, int tiles_n, afq::sym_dst sd = afq::sym_dst{0, 0, 0}) {
+    int tiles_m, int tiles_n, afq::sym_dst sd = afq::sym_dst{0, 0, 0}, int kc = 0, long long c_stride = 0) {
+    if (gridDim.y > 1) {
+        const int k_begin = blockIdx.y * kc;
+        K = min(K - k_begin, kc);
+        Ahi += k_begin;
+        Alo += k_begin;
+        Bhi += k_begin;
+        Blo += k_begin;
+        C += blockIdx.y * c_stride;
+    }
     // [stage][plane: hi, lo][256 rows x 64 bytes]: A two k-steps deep, B three
     // (the CU's 160 KB of LDS), so the B movers (waves 4..7) issue two k-steps
     // ahead and the A movers (waves 0..3) one
@@ -766,9 +777,23 @@ extern "C" int afq_gemm_bf16x3_nt_splitk(int M, int N, int K, float alpha, const
     const int s = K == 0 ? 1 : afq::ceil_div(K, kc);
     const long long stride = (long long)M * N;
     hipStream_t st = (hipStream_t)stream;
-    hipLaunchKernelGGL((gemm_bf16x3_nt_kernel<float, false>), dim3((unsigned)(tiles_m * tiles_n), (unsigned)s),
-                       dim3(THREADS), 0, st, M, N, K, 1.0f, A_hi, A_lo, lda, B_hi, B_lo, ldb, workspace, (long long)N,
-                       tiles_m, tiles_n, afq::sym_dst{0, 0, 0}, kc, stride);
+    // 256 x 256 tiles (one workgroup per CU) when their chunks fill the CUs in
+    // rounds at least three quarters full; each chunk's partial product is the
+    // same in either tiling (same k order, same three products per k-step)
+    const long long tiles2 = (long long)afq::ceil_div(M, BM2) * afq::ceil_div(N, BN2);
+    const long long wg2 = tiles2 * s;
+    const long long cus = afq::cu_count();
+    const long long rounds = (wg2 + cus - 1) / cus;
+    if (tiles2 < (1LL << 31) && wg2 >= cus && 4 * wg2 >= 3 * rounds * cus) {
+        hipLaunchKernelGGL((gemm_bf16x3_nt_256_kernel<float, false>), dim3((unsigned)tiles2, (unsigned)s),
+                           dim3(THREADS2), 0, st, M, N, K, 1.0f, A_hi, A_lo, lda, B_hi, B_lo, ldb, workspace,
+                           (long long)N, afq::ceil_div(M, BM2), afq::ceil_div(N, BN2), afq::sym_dst{0, 0, 0}, kc,
+                           stride);
+    } else {
+        hipLaunchKernelGGL((gemm_bf16x3_nt_kernel<float, false>), dim3((unsigned)(tiles_m * tiles_n), (unsigned)s),
+                           dim3(THREADS), 0, st, M, N, K, 1.0f, A_hi, A_lo, lda, B_hi, B_lo, ldb, workspace,
+                           (long long)N, tiles_m, tiles_n, afq::sym_dst{0, 0, 0}, kc, stride);
+    }
     hipLaunchKernelGGL(splitk_reduce_kernel, dim3(grid_for(stride)), dim3(256), 0, st, M, N, s, workspace, stride,
                        alpha, C, ldc, c_fp64);
     return afq::check_launch("afq_gemm_bf16x3_nt_splitk");

@@ -369,6 +369,37 @@ def test_bf16x3_gemm_split_k(gpu_device, M, N, K):
         assert float((c1 - ref).abs().max()) <= 2e-5 * scale
 
 
+@pytest.mark.parametrize("c_dtype", ["float64", "float32"])
+def test_bf16x3_split_k_chunks_equal_unsplit_chunk_products(gpu_device, c_dtype):
+    """The force-bias shape (M 4096, N 1000, K 6400, 4 chunks of 1600) runs its
+    chunks on the 256 x 256 tiles; each chunk's fp32 partial equals the
+    128 x 128 kernel's unsplit product over the chunk's K range, so the result
+    equals those partials summed in fp64 in chunk order, times alpha, bit for
+    bit."""
+    import torch
+    from ad_afqmc_prototype_amd import kernels
+    M, N, K, s, alpha = 4096, 1000, 6400, 4, 0.5
+    print(f"split-K chunk partials M={M} N={N} K={K} splits={s} ({c_dtype})", flush=True)
+    g = torch.Generator(device="cpu").manual_seed(11)
+    A = torch.randn((M, K), dtype=torch.float64, generator=g).to(gpu_device)
+    B = torch.randn((N, K), dtype=torch.float64, generator=g).to(gpu_device)
+    ah, al = kernels.split_bf16(A)
+    bh, bl = kernels.split_bf16(B)
+    dt = getattr(torch, c_dtype)
+    out = torch.empty((M, N), dtype=dt, device=gpu_device)
+    kernels.gemm_bf16x3_nt(ah, al, bh, bl, out, alpha=alpha, splits=s)
+    kc = 1600
+    acc = None
+    for z in range(s):
+        part = torch.empty((M, N), dtype=torch.float32, device=gpu_device)
+        sl = slice(z * kc, (z + 1) * kc)
+        kernels.gemm_bf16x3_nt(ah[:, sl].contiguous(), al[:, sl].contiguous(), bh[:, sl].contiguous(),
+                               bl[:, sl].contiguous(), part, alpha=1.0, splits=1)
+        acc = part.double() if acc is None else acc + part.double()
+    ref = (alpha * acc).to(dt)
+    assert torch.equal(out, ref)
+
+
 @pytest.mark.parametrize("nw,n,o,nspin", [(8, 24, 6, 1), (5, 45, 11, 2), (3, 17, 5, 1), (16, 200, 32, 1),
                                           (4, 224, 16, 2)])
 def test_trotter_bf16x3_upper_block_vhs_equals_full_rows(gpu_device, nw, n, o, nspin):
