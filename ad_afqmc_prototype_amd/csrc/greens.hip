@@ -763,11 +763,9 @@ __device__ __forceinline__ void greens_gj_split_body(int n, int o, const double2
             l[e] = (__bf16)(x[e] - (float)hh);
         }
     };
-    const int npt = (n + 15) >> 4;
-    float wre[NKB][8], wim[NKB][8], nre[NKB][8], nim[NKB][8];
-    load_w(0, wre, wim);
-    for (int pt = 0; pt < npt; ++pt) {
-        load_w(pt + 1, nre, nim);  // past the end: all zeros, nothing read
+    // G rows of w rows 16 pt .. 16 pt + 15 (w as fp32, the lane's fragment
+    // entries): split, the product on bf16 MFMA, the bf16 planes stored
+    auto row_block = [&](int pt, const float (&wre)[NKB][8], const float (&wim)[NKB][8]) {
         bf16x8_t brh[NKB], brl[NKB], bih[NKB], bil[NKB];
 #pragma unroll
         for (int kb = 0; kb < NKB; ++kb) {
@@ -811,13 +809,69 @@ __device__ __forceinline__ void greens_gj_split_body(int n, int o, const double2
                 }
             }
         }
+    };
+    const int npt = (n + 15) >> 4;
+    if constexpr (NT == 2) {
+        // 16 rows of w per step: one contiguous block of 16 o entries, read
+        // coalesced into registers one step ahead and passed to the fragment
+        // layout through the A^T region (free once A's fragments are loaded):
+        // each load instruction then covers 8 cache lines, not 64
+        constexpr int PW = O + 1;        // padded row pitch (double2)
+        constexpr int NV = 16 * O / 64;  // entries per lane per block
+        static_assert(16 * PW * sizeof(double2) <= sizeof(S.at), "w block exceeds the A^T region");
+        double2* Wt = reinterpret_cast<double2*>(&S.at[0][0][0]);
+        wave_lds_sync();  // A's fragments and the permutation are read before the region is reused
+        int soff[NV];
 #pragma unroll
-        for (int kb = 0; kb < NKB; ++kb)
+        for (int i = 0; i < NV; ++i) {
+            const int u = i * 64 + lane;
+            const int row = u / o;
+            soff[i] = u < 16 * o ? row * PW + (u - row * o) : -1;
+        }
+        double2 wv[NV];
+        auto fetch = [&](int pt) {
+            const int base = pt * 16 * o;
+            const int lim = min(16, n - pt * 16) * o;  // <= 0 past the end: nothing read
 #pragma unroll
-            for (int e = 0; e < 8; ++e) {
-                wre[kb][e] = nre[kb][e];
-                wim[kb][e] = nim[kb][e];
+            for (int i = 0; i < NV; ++i) {
+                const int u = i * 64 + lane;
+                wv[i] = u < lim ? W[base + u] : make_double2(0.0, 0.0);
             }
+        };
+        fetch(0);
+        for (int pt = 0; pt < npt; ++pt) {
+#pragma unroll
+            for (int i = 0; i < NV; ++i)
+                if (soff[i] >= 0) Wt[soff[i]] = wv[i];
+            fetch(pt + 1);
+            wave_lds_sync();
+            float wre[NKB][8], wim[NKB][8];
+#pragma unroll
+            for (int kb = 0; kb < NKB; ++kb)
+#pragma unroll
+                for (int e = 0; e < 8; ++e) {
+                    const int kk = kb * 32 + 8 * q + e;
+                    const double2 v = kk < o ? Wt[lr * PW + kk] : make_double2(0.0, 0.0);
+                    wre[kb][e] = (float)v.x;
+                    wim[kb][e] = (float)v.y;
+                }
+            wave_lds_sync();  // the block is read before the next one is written
+            row_block(pt, wre, wim);
+        }
+    } else {
+        float wre[NKB][8], wim[NKB][8], nre[NKB][8], nim[NKB][8];
+        load_w(0, wre, wim);
+        for (int pt = 0; pt < npt; ++pt) {
+            load_w(pt + 1, nre, nim);  // past the end: all zeros, nothing read
+            row_block(pt, wre, wim);
+#pragma unroll
+            for (int kb = 0; kb < NKB; ++kb)
+#pragma unroll
+                for (int e = 0; e < 8; ++e) {
+                    wre[kb][e] = nre[kb][e];
+                    wim[kb][e] = nim[kb][e];
+                }
+        }
     }
 }
 
