@@ -21,6 +21,10 @@ c_vp = ctypes.c_void_p
 # name -> argtypes (restype is int for every compute entry point)
 _SIGNATURES = {
     "afq_dgemm_nt": [c_int, c_int, c_int, c_double, c_vp, c_ll, c_vp, c_ll, c_double, c_vp, c_ll, c_vp],
+    "afq_dgemm_nt_lower": [c_int, c_int, c_int, c_double, c_vp, c_ll, c_vp, c_ll, c_vp, c_ll, c_vp],
+    "afq_sgemm_nt_lower": [c_int, c_int, c_int, ctypes.c_float, c_vp, c_ll, c_vp, c_ll, c_vp, c_ll, c_int, c_vp],
+    "afq_gemm_bf16x3_nt_lower": [c_int, c_int, c_int, ctypes.c_float, c_vp, c_vp, c_ll, c_vp, c_vp, c_ll, c_vp,
+                                 c_ll, c_int, c_vp],
     "afq_dgemm_nt_splitk": [c_int, c_int, c_int, c_double, c_vp, c_ll, c_vp, c_ll, c_vp, c_ll, c_int, c_vp, c_vp],
     "afq_sgemm_nt": [c_int, c_int, c_int, ctypes.c_float, c_vp, c_ll, c_vp, c_ll, ctypes.c_float, c_vp,
                      c_ll, c_int, c_vp],
@@ -100,7 +104,7 @@ _SIGNATURES = {
     "afq_unpack_pairs": [c_int, c_int, c_vp, c_ll, c_vp, c_vp],
 }
 
-ABI_VERSION = 21
+ABI_VERSION = 22
 
 
 class HipLibraryError(RuntimeError):

@@ -38,6 +38,12 @@ __device__ inline int xcd_contiguous(int b, int total) {
     return offset + local;
 }
 
+// Column tile of position nn in the order 0, nt - 1, 1, nt - 2, ... (a
+// bijection of [0, nt)): with a B whose rows are zero past the diagonal, tile
+// j's k-loop runs to its last column, and any contiguous range of positions
+// (one XCD's share) then holds as much work as any other.
+__device__ inline int zigzag(int nn, int nt) { return (nn & 1) ? nt - 1 - (nn >> 1) : nn >> 1; }
+
 // v_mfma_f64_16x16x4_f64.  Lane l supplies A[l%16][l/16] and B[l/16][l%16];
 // the accumulator holds D[4*r + l/16][l%16] in element r (measured on gfx950).
 __device__ __forceinline__ f64x4 mfma_f64(double a, double b, f64x4 c) {

@@ -195,17 +195,25 @@ __device__ __forceinline__ void tile_origin(int tiles_m, int tiles_n, int bm, in
     n0 = (in_group / gsize) * bn;
 }
 
+// lower_b: row j of B is zero past column j, so the tile of columns n0.. stops
+// its k-loop at k = n0 + BN (the zeros inside that range are still read); the
+// column tiles are then taken in afq::zigzag order, so that every XCD's
+// contiguous share of tiles holds the same work
 template <typename T, typename TOut, bool SYM = false>
 __global__ __launch_bounds__(THREADS, 2) void gemm_nt_kernel(
     int M, int N, int K, T alpha, const T* __restrict__ A, long long lda,
     const T* __restrict__ B, long long ldb, T beta, TOut* __restrict__ C,
-    long long ldc, int tiles_m, int tiles_n, afq::sym_dst sd = afq::sym_dst{0, 0, 0}) {
+    long long ldc, int tiles_m, int tiles_n, afq::sym_dst sd = afq::sym_dst{0, 0, 0}, int lower_b = 0) {
     using CF = tile_cfg<T, 4, 4>;
     __shared__ T As[2 * CF::BM * PITCH];
     __shared__ T Bs[2 * CF::BN * PITCH];
 
     int m0, n0;
     tile_origin(tiles_m, tiles_n, CF::BM, CF::BN, m0, n0);
+    if (lower_b) {
+        n0 = afq::zigzag(n0 / CF::BN, tiles_n) * CF::BN;
+        K = min(K, n0 + CF::BN);
+    }
 
     typename CF::acc_t acc[4][4];
     gemm_nt_tile<T, 4, 4>(M, N, K, A, lda, B, ldb, m0, n0, As, Bs, acc);
@@ -282,7 +290,7 @@ namespace afq {
 
 template <typename T, typename TOut>
 int gemm_nt(const char* name, int M, int N, int K, T alpha, const T* A, long long lda, const T* B,
-            long long ldb, T beta, TOut* C, long long ldc, hipStream_t stream) {
+            long long ldb, T beta, TOut* C, long long ldc, hipStream_t stream, int lower_b = 0) {
     AFQ_REQUIRE(M >= 0 && N >= 0 && K >= 0, "%s: negative size (M=%d N=%d K=%d)", name, M, N, K);
     if (M == 0 || N == 0) return 0;
     AFQ_REQUIRE(A && B && C, "%s: null pointer", name);
@@ -292,7 +300,7 @@ int gemm_nt(const char* name, int M, int N, int K, T alpha, const T* A, long lon
     const long long total = (long long)tiles_m * tiles_n;
     AFQ_REQUIRE(total < (1LL << 31), "%s: problem too large", name);
     hipLaunchKernelGGL((gemm_nt_kernel<T, TOut>), dim3((unsigned)total), dim3(THREADS), 0, stream, M, N, K,
-                       alpha, A, lda, B, ldb, beta, C, ldc, tiles_m, tiles_n);
+                       alpha, A, lda, B, ldb, beta, C, ldc, tiles_m, tiles_n, sym_dst{0, 0, 0}, lower_b);
     return check_launch(name);
 }
 
@@ -338,6 +346,22 @@ extern "C" int afq_dgemm_nt(int M, int N, int K, double alpha, const double* A, 
                             const double* B, long long ldb, double beta, double* C, long long ldc,
                             void* stream) {
     return afq::dgemm_nt(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, (hipStream_t)stream);
+}
+
+extern "C" int afq_dgemm_nt_lower(int M, int N, int K, double alpha, const double* A, long long lda,
+                                  const double* B, long long ldb, double* C, long long ldc, void* stream) {
+    return afq::gemm_nt<double, double>("afq_dgemm_nt_lower", M, N, K, alpha, A, lda, B, ldb, 0.0, C, ldc,
+                                        (hipStream_t)stream, 1);
+}
+
+extern "C" int afq_sgemm_nt_lower(int M, int N, int K, float alpha, const float* A, long long lda,
+                                  const float* B, long long ldb, void* C, long long ldc, int c_fp64,
+                                  void* stream) {
+    if (c_fp64)
+        return afq::gemm_nt<float, double>("afq_sgemm_nt_lower", M, N, K, alpha, A, lda, B, ldb, 0.0f,
+                                           (double*)C, ldc, (hipStream_t)stream, 1);
+    return afq::gemm_nt<float, float>("afq_sgemm_nt_lower", M, N, K, alpha, A, lda, B, ldb, 0.0f, (float*)C,
+                                      ldc, (hipStream_t)stream, 1);
 }
 
 extern "C" int afq_dgemm_nt_splitk(int M, int N, int K, double alpha, const double* A, long long lda,

@@ -154,12 +154,17 @@ __device__ __forceinline__ void bf16x3_tile(int M, int N, int K, float alpha, co
         }
 }
 
+// lower_b (whole K only): row j of B is zero past column j, so the tile of
+// columns n0.. stops its k-loop at k = n0 + BN (the zeros inside that range
+// are still read); the column tiles are then taken in afq::zigzag order, so
+// that every XCD's contiguous share of tiles holds the same work
 template <typename TOut, bool SYM = false>
 __global__ __launch_bounds__(THREADS, 2) void gemm_bf16x3_nt_kernel(
     int M, int N, int K, float alpha, const uint16_t* __restrict__ Ahi,
     const uint16_t* __restrict__ Alo, long long lda, const uint16_t* __restrict__ Bhi,
     const uint16_t* __restrict__ Blo, long long ldb, TOut* __restrict__ C, long long ldc,
-    int tiles_m, int tiles_n, afq::sym_dst sd = afq::sym_dst{0, 0, 0}, int kc = 0, long long c_stride = 0) {
+    int tiles_m, int tiles_n, afq::sym_dst sd = afq::sym_dst{0, 0, 0}, int kc = 0, long long c_stride = 0,
+    int lower_b = 0) {
     if (gridDim.y > 1) {
         const int k_begin = blockIdx.y * kc;
         K = min(K - k_begin, kc);
@@ -179,7 +184,8 @@ __global__ __launch_bounds__(THREADS, 2) void gemm_bf16x3_nt_kernel(
     const int gsize = min(tiles_m - first_m, GROUP_M);
     const int in_group = t - group * per_group;
     const int m0 = (first_m + in_group % gsize) * BM;
-    const int n0 = (in_group / gsize) * BN;
+    const int n0 = (lower_b ? afq::zigzag(in_group / gsize, tiles_n) : in_group / gsize) * BN;
+    if (lower_b) K = min(K, n0 + BN);
 
     bf16x3_tile<TOut, SYM>(M, N, K, alpha, Ahi, Alo, lda, Bhi, Blo, ldb, C, ldc, m0, n0, sd, S);
 }
@@ -409,14 +415,15 @@ __device__ __forceinline__ void dma_store(int M, int N, float alpha, TOut* __res
         }
 }
 
-// Split K as in gemm_bf16x3_nt_kernel: blockIdx.y = kz takes k in
+// Split K and lower_b as in gemm_bf16x3_nt_kernel: blockIdx.y = kz takes k in
 // [kz kc, (kz + 1) kc) and writes its partial product to C + kz c_stride.
 template <typename TOut, bool SYM = false>
 __global__ __launch_bounds__(THREADS2, 1) void gemm_bf16x3_nt_256_kernel(
     int M, int N, int K, float alpha, const uint16_t* __restrict__ Ahi,
     const uint16_t* __restrict__ Alo, long long lda, const uint16_t* __restrict__ Bhi,
     const uint16_t* __restrict__ Blo, long long ldb, TOut* __restrict__ C, long long ldc,
-    int tiles_m, int tiles_n, afq::sym_dst sd = afq::sym_dst{0, 0, 0}, int kc = 0, long long c_stride = 0) {
+    int tiles_m, int tiles_n, afq::sym_dst sd = afq::sym_dst{0, 0, 0}, int kc = 0, long long c_stride = 0,
+    int lower_b = 0) {
     if (gridDim.y > 1) {
         const int k_begin = blockIdx.y * kc;
         K = min(K - k_begin, kc);
@@ -440,7 +447,8 @@ __global__ __launch_bounds__(THREADS2, 1) void gemm_bf16x3_nt_256_kernel(
     const int gsize = min(tiles_m - first_m, GROUP_M2);
     const int in_group = t - group * per_group;
     const int m0 = (first_m + in_group % gsize) * BM2;
-    const int n0 = (in_group / gsize) * BN2;
+    const int n0 = (lower_b ? afq::zigzag(in_group / gsize, tiles_n) : in_group / gsize) * BN2;
+    if (lower_b) K = min(K, n0 + BN2);
 
     const int tid = threadIdx.x;
     const int lane = tid & 63;
@@ -707,54 +715,72 @@ unsigned grid_for(long long total) {
 
 }  // namespace
 
-extern "C" int afq_gemm_bf16x3_nt(int M, int N, int K, float alpha, const unsigned short* A_hi,
-                                  const unsigned short* A_lo, long long lda,
-                                  const unsigned short* B_hi, const unsigned short* B_lo,
-                                  long long ldb, void* C, long long ldc, int c_fp64, void* stream) {
-    AFQ_REQUIRE(M >= 0 && N >= 0 && K >= 0, "afq_gemm_bf16x3_nt: negative size");
+namespace {
+
+// afq_gemm_bf16x3_nt and afq_gemm_bf16x3_nt_lower: 256 x 256 tiles once they
+// give at least two workgroups per CU, else 128 x 128
+int bf16x3_nt(const char* name, int M, int N, int K, float alpha, const unsigned short* A_hi,
+              const unsigned short* A_lo, long long lda, const unsigned short* B_hi, const unsigned short* B_lo,
+              long long ldb, void* C, long long ldc, int c_fp64, int lower_b, hipStream_t st) {
+    AFQ_REQUIRE(M >= 0 && N >= 0 && K >= 0, "%s: negative size", name);
     if (M == 0 || N == 0) return 0;
-    AFQ_REQUIRE(A_hi && A_lo && B_hi && B_lo && C, "afq_gemm_bf16x3_nt: null pointer");
-    AFQ_REQUIRE(K % 8 == 0 && lda % 8 == 0 && ldb % 8 == 0,
-                "afq_gemm_bf16x3_nt: K=%d, lda and ldb must be multiples of 8", K);
-    AFQ_REQUIRE(lda >= K && ldb >= K && ldc >= N, "afq_gemm_bf16x3_nt: leading dimension too small");
-    hipStream_t st = (hipStream_t)stream;
-    // 256 x 256 tiles once they give at least two workgroups per CU
+    AFQ_REQUIRE(A_hi && A_lo && B_hi && B_lo && C, "%s: null pointer", name);
+    AFQ_REQUIRE(K % 8 == 0 && lda % 8 == 0 && ldb % 8 == 0, "%s: K=%d, lda and ldb must be multiples of 8", name, K);
+    AFQ_REQUIRE(lda >= K && ldb >= K && ldc >= N, "%s: leading dimension too small", name);
+    AFQ_REQUIRE(c_fp64 >= 0 && c_fp64 <= 2, "%s: c_fp64=%d not in 0..2", name, c_fp64);
+    const afq::sym_dst none{0, 0, 0};
     const int tiles_m2 = afq::ceil_div(M, BM2);
     const int tiles_n2 = afq::ceil_div(N, BN2);
     const long long total2 = (long long)tiles_m2 * tiles_n2;
-    AFQ_REQUIRE(c_fp64 >= 0 && c_fp64 <= 2, "afq_gemm_bf16x3_nt: c_fp64=%d not in 0..2", c_fp64);
     if (total2 >= 512 && total2 < (1LL << 31)) {
+        const dim3 g((unsigned)total2);
         if (c_fp64 == 2)
-            hipLaunchKernelGGL(gemm_bf16x3_nt_256_kernel<afq::bf16_pair>, dim3((unsigned)total2), dim3(THREADS2), 0,
-                               st, M, N, K, alpha, A_hi, A_lo, lda, B_hi, B_lo, ldb, (afq::bf16_pair*)C, ldc,
-                               tiles_m2, tiles_n2);
+            hipLaunchKernelGGL(gemm_bf16x3_nt_256_kernel<afq::bf16_pair>, g, dim3(THREADS2), 0, st, M, N, K, alpha,
+                               A_hi, A_lo, lda, B_hi, B_lo, ldb, (afq::bf16_pair*)C, ldc, tiles_m2, tiles_n2, none, 0,
+                               0LL, lower_b);
         else if (c_fp64)
-            hipLaunchKernelGGL(gemm_bf16x3_nt_256_kernel<double>, dim3((unsigned)total2), dim3(THREADS2), 0, st,
-                               M, N, K, alpha, A_hi, A_lo, lda, B_hi, B_lo, ldb, (double*)C, ldc, tiles_m2,
-                               tiles_n2);
+            hipLaunchKernelGGL(gemm_bf16x3_nt_256_kernel<double>, g, dim3(THREADS2), 0, st, M, N, K, alpha, A_hi,
+                               A_lo, lda, B_hi, B_lo, ldb, (double*)C, ldc, tiles_m2, tiles_n2, none, 0, 0LL,
+                               lower_b);
         else
-            hipLaunchKernelGGL(gemm_bf16x3_nt_256_kernel<float>, dim3((unsigned)total2), dim3(THREADS2), 0, st,
-                               M, N, K, alpha, A_hi, A_lo, lda, B_hi, B_lo, ldb, (float*)C, ldc, tiles_m2,
-                               tiles_n2);
-        return afq::check_launch("afq_gemm_bf16x3_nt");
+            hipLaunchKernelGGL(gemm_bf16x3_nt_256_kernel<float>, g, dim3(THREADS2), 0, st, M, N, K, alpha, A_hi,
+                               A_lo, lda, B_hi, B_lo, ldb, (float*)C, ldc, tiles_m2, tiles_n2, none, 0, 0LL,
+                               lower_b);
+        return afq::check_launch(name);
     }
     const int tiles_m = afq::ceil_div(M, BM);
     const int tiles_n = afq::ceil_div(N, BN);
     const long long total = (long long)tiles_m * tiles_n;
-    AFQ_REQUIRE(total < (1LL << 31), "afq_gemm_bf16x3_nt: problem too large");
+    AFQ_REQUIRE(total < (1LL << 31), "%s: problem too large", name);
+    const dim3 g((unsigned)total);
     if (c_fp64 == 2)
-        hipLaunchKernelGGL(gemm_bf16x3_nt_kernel<afq::bf16_pair>, dim3((unsigned)total), dim3(THREADS), 0, st,
-                           M, N, K, alpha, A_hi, A_lo, lda, B_hi, B_lo, ldb, (afq::bf16_pair*)C, ldc,
-                           tiles_m, tiles_n);
+        hipLaunchKernelGGL(gemm_bf16x3_nt_kernel<afq::bf16_pair>, g, dim3(THREADS), 0, st, M, N, K, alpha, A_hi, A_lo,
+                           lda, B_hi, B_lo, ldb, (afq::bf16_pair*)C, ldc, tiles_m, tiles_n, none, 0, 0LL, lower_b);
     else if (c_fp64)
-        hipLaunchKernelGGL(gemm_bf16x3_nt_kernel<double>, dim3((unsigned)total), dim3(THREADS), 0, st,
-                           M, N, K, alpha, A_hi, A_lo, lda, B_hi, B_lo, ldb, (double*)C, ldc,
-                           tiles_m, tiles_n);
+        hipLaunchKernelGGL(gemm_bf16x3_nt_kernel<double>, g, dim3(THREADS), 0, st, M, N, K, alpha, A_hi, A_lo, lda,
+                           B_hi, B_lo, ldb, (double*)C, ldc, tiles_m, tiles_n, none, 0, 0LL, lower_b);
     else
-        hipLaunchKernelGGL(gemm_bf16x3_nt_kernel<float>, dim3((unsigned)total), dim3(THREADS), 0, st,
-                           M, N, K, alpha, A_hi, A_lo, lda, B_hi, B_lo, ldb, (float*)C, ldc,
-                           tiles_m, tiles_n);
-    return afq::check_launch("afq_gemm_bf16x3_nt");
+        hipLaunchKernelGGL(gemm_bf16x3_nt_kernel<float>, g, dim3(THREADS), 0, st, M, N, K, alpha, A_hi, A_lo, lda,
+                           B_hi, B_lo, ldb, (float*)C, ldc, tiles_m, tiles_n, none, 0, 0LL, lower_b);
+    return afq::check_launch(name);
+}
+
+}  // namespace
+
+extern "C" int afq_gemm_bf16x3_nt(int M, int N, int K, float alpha, const unsigned short* A_hi,
+                                  const unsigned short* A_lo, long long lda,
+                                  const unsigned short* B_hi, const unsigned short* B_lo,
+                                  long long ldb, void* C, long long ldc, int c_fp64, void* stream) {
+    return bf16x3_nt("afq_gemm_bf16x3_nt", M, N, K, alpha, A_hi, A_lo, lda, B_hi, B_lo, ldb, C, ldc, c_fp64, 0,
+                     (hipStream_t)stream);
+}
+
+extern "C" int afq_gemm_bf16x3_nt_lower(int M, int N, int K, float alpha, const unsigned short* A_hi,
+                                        const unsigned short* A_lo, long long lda, const unsigned short* B_hi,
+                                        const unsigned short* B_lo, long long ldb, void* C, long long ldc,
+                                        int c_fp64, void* stream) {
+    return bf16x3_nt("afq_gemm_bf16x3_nt_lower", M, N, K, alpha, A_hi, A_lo, lda, B_hi, B_lo, ldb, C, ldc, c_fp64,
+                     1, (hipStream_t)stream);
 }
 
 extern "C" int afq_gemm_bf16x3_nt_splitk(int M, int N, int K, float alpha, const unsigned short* A_hi,

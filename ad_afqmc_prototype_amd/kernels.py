@@ -122,6 +122,43 @@ def dgemm_nt(A, B, C, alpha=1.0, beta=0.0, tag=None):
     return C
 
 
+def gemm_nt_lower(A, B, C, alpha=1.0, tag=None):
+    """C = alpha * A @ B.T for a B whose row j is zero past column j (every
+    column tile's k-loop stops at its last column; same result as the full
+    GEMM on the same B).  A, B float64 (fp64, C float64), float32 (fp32, C
+    float32 or float64) or (hi, lo) bf16 plane pairs from split_bf16 (C
+    float32 or float64)."""
+    if isinstance(A, tuple):
+        (ah, al), (bh, bl) = A, B
+        M, K = ah.shape
+        N = bh.shape[0]
+        for t, nm, sh in ((ah, "a_hi", (M, K)), (al, "a_lo", (M, K)), (bh, "b_hi", (N, K)), (bl, "b_lo", (N, K))):
+            _check(t, torch.int16, sh, nm, ah.device)
+        if C.dtype not in (f32, f64):
+            raise TypeError("C must be float32 or float64")
+        _check(C, C.dtype, (M, N), "C", ah.device)
+        _call("afq_gemm_bf16x3_nt_lower", ah, M, N, K, float(alpha), ah.data_ptr(), al.data_ptr(), K,
+              bh.data_ptr(), bl.data_ptr(), K, C.data_ptr(), N, int(C.dtype == f64), _stream(ah), tag=tag)
+        return C
+    M, K = A.shape
+    N = B.shape[0]
+    if A.dtype == f64:
+        _check(A, f64, (M, K), "A")
+        _check(B, f64, (N, K), "B", A.device)
+        _check(C, f64, (M, N), "C", A.device)
+        _call("afq_dgemm_nt_lower", A, M, N, K, float(alpha), A.data_ptr(), K, B.data_ptr(), K, C.data_ptr(), N,
+              _stream(A), tag=tag)
+        return C
+    _check(A, f32, (M, K), "A")
+    _check(B, f32, (N, K), "B", A.device)
+    if C.dtype not in (f32, f64):
+        raise TypeError("C must be float32 or float64")
+    _check(C, C.dtype, (M, N), "C", A.device)
+    _call("afq_sgemm_nt_lower", A, M, N, K, float(alpha), A.data_ptr(), K, B.data_ptr(), K, C.data_ptr(), N,
+          int(C.dtype == f64), _stream(A), tag=tag)
+    return C
+
+
 def sym_blocks(n: int) -> int:
     """Number of upper 16 x 16 blocks of an n x n symmetric matrix."""
     nb = (n + 15) // 16

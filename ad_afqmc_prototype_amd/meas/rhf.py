@@ -67,15 +67,16 @@ class rhf_meas_ctx:
         return self.rot_chol.is_complex()
 
     def exchange_operand(self, s: int):
-        """Y of spin sector s, (o n, o n) with Y[(a,q),(b,p)] = sum_g R[g,a,p] R[g,b,q]
-        (R = rot_chol[s], real): the exchange sum_g tr(f_g f_g) of a walker with
-        f_g = R_g G^T is the quadratic form vec(G)^T Y vec(G).  Y is symmetric.
-        float64; float32 in fp32 mode; (hi, lo) bf16 planes in bf16x3 mode."""
+        """L of spin sector s, (o n, o n): the lower triangle of
+        Y[(a,q),(b,p)] = sum_g R[g,a,p] R[g,b,q] (R = rot_chol[s], real) with its
+        diagonal halved, so Y = L + L^T.  The exchange sum_g tr(f_g f_g) of a
+        walker with f_g = R_g G^T is the quadratic form vec(G)^T Y vec(G) =
+        2 vec(G)^T L^T vec(G) (Y is symmetric), one GEMM Z = 2 G L^T whose
+        column tiles stop at the diagonal (kernels.gemm_nt_lower).  float64;
+        float32 in fp32 mode; (hi, lo) bf16 planes in bf16x3 mode."""
         key = ("exchange", s)
         if key not in self._cache:
-            _, g, o, n = self.rot_chol.shape
-            r = self.rot_chol[s].reshape(g, o * n)
-            y = (r.T @ r).view(o, n, o, n).permute(0, 3, 2, 1).reshape(o * n, o * n).contiguous()
+            y = exchange_lower(self.rot_chol[s])
             if self.precision == "fp32":
                 y = kernels.convert_f32(y)
             elif self.precision == "bf16x3":
@@ -105,6 +106,18 @@ class rhf_meas_ctx:
                 op[:, :, :o, :n] = self.rot_chol[:nspin]
             self._cache[key] = (op, self.rot_h1[:nspin].contiguous())
         return self._cache[key]
+
+
+def exchange_lower(rot: torch.Tensor) -> torch.Tensor:
+    """L (o n, o n) for the half-rotated Cholesky rot (naux, o, n), real: the
+    lower triangle of Y[(a,q),(b,p)] = sum_g rot[g,a,p] rot[g,b,q] with its
+    diagonal halved, so Y = L + L^T and the exchange vec(G)^T Y vec(G) =
+    2 vec(G)^T L^T vec(G)."""
+    g, o, n = rot.shape
+    r = rot.reshape(g, o * n)
+    y = torch.tril((r.T @ r).view(o, n, o, n).permute(0, 3, 2, 1).reshape(o * n, o * n))
+    y.diagonal().mul_(0.5)
+    return y
 
 
 def build_meas_ctx(ham_data: ham_chol, trial_data, precision: str = "fp64",
@@ -320,7 +333,8 @@ def _energy_quadratic(greens, nw: int, n: int, o: int, nspin: int, meas_ctx: rhf
       c_g (summed over spin sectors) = the force-bias contraction with alpha 1;
       Coulomb: restricted 2 sum_g c_g^2, unrestricted (sum_g (c_up + c_dn)_g^2) / 2;
       exchange of sector s: sum_g tr(f_g f_g) = vec(G_s)^T Y_s vec(G_s), one GEMM
-      Z = G_s Y_s over all walkers (the ctx's precision, fp64 output);
+      Z = 2 G_s L_s^T = G_s Y_s over all walkers (L_s: Y_s's lower triangle,
+      diagonal halved; the ctx's precision, fp64 output);
       one-body: tr(rot_h1_s G_s);
     the dot products and sums on afq_energy_assemble (fp64)."""
     g = greens.greens
@@ -331,14 +345,14 @@ def _energy_quadratic(greens, nw: int, n: int, o: int, nspin: int, meas_ctx: rhf
         rows = g.reshape(2 * nw, nspin * on)[:, sec * on:(sec + 1) * on]
         if nspin > 1:
             rows = rows.contiguous()
-        y = meas_ctx.exchange_operand(sec)
+        lo = meas_ctx.exchange_operand(sec)
         if meas_ctx.precision == "fp64":
-            kernels.dgemm_nt(rows, y, z[sec], tag="exchange")
+            kernels.gemm_nt_lower(rows, lo, z[sec], alpha=2.0, tag="exchange")
         elif meas_ctx.precision == "fp32":
-            kernels.sgemm_nt(kernels.convert_f32(rows), y, z[sec], tag="exchange")
+            kernels.gemm_nt_lower(kernels.convert_f32(rows), lo, z[sec], alpha=2.0, tag="exchange")
         else:
             gh, gl = greens.split if nspin == 1 and greens.split is not None else kernels.split_bf16(rows)
-            kernels.gemm_bf16x3_nt(gh, gl, y[0], y[1], z[sec], tag="exchange")
+            kernels.gemm_nt_lower((gh, gl), lo, z[sec], alpha=2.0, tag="exchange")
     return kernels.energy_assemble(g, z, c, meas_ctx.rot_h1[:nspin].contiguous(), nw, o, n, nspin, h0)
 
 

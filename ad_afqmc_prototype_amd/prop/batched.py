@@ -44,7 +44,7 @@ from .. import _lib
 from .. import kernels
 from ..core.system import system
 from ..ham.chol import ham_chol
-from ..meas.rhf import deferred_view, make_rhf_meas_ops, quadratic_exchange
+from ..meas.rhf import deferred_view, exchange_lower, make_rhf_meas_ops, quadratic_exchange
 from ..trial.rhf import make_rhf_trial_ops, rhf_trial
 from .afqmc import init_prop_state
 from .chol_afqmc_ops import chol_sym_rows, make_chol_afqmc_ops
@@ -261,9 +261,9 @@ class system_batch:
         else:
             ops = dict(chol_s32=kernels.convert_f32(chol_sym_rows(s.ham.chol)), rot32=m_ctx.fb_operand(1))
         if quadratic_exchange(n, o, g):
-            # Y of the exchange as a quadratic form, in the batch's precision
-            r = m_ctx.rot_chol[0].reshape(g, o * n)
-            y = (r.T @ r).view(o, n, o, n).permute(0, 3, 2, 1).reshape(o * n, o * n).contiguous()
+            # the exchange as a quadratic form: L, Y's lower triangle with its
+            # diagonal halved (Z = 2 G L^T), in the batch's precision
+            y = exchange_lower(m_ctx.rot_chol[0])
             if self.precision == "bf16x3":
                 ops["y_hl"] = kernels.split_bf16(y)
             else:
@@ -393,7 +393,7 @@ class system_batch:
                 p.M, p.N, p.K = 2 * m["nw"], k, k
                 p.lda, p.ldb, p.ldc = k, k, k
                 p.A, p.B, p.C = self.greens32.ptr(i), m["y32"].data_ptr(), self.exch_z.ptr(i)
-                p.alpha, p.c_fp64 = 1.0, 1
+                p.alpha, p.c_fp64 = 2.0, 1  # Z = 2 G L^T
             else:
                 # symmetric VHS: the upper 16 x 16 blocks, each entry stored twice
                 # into the (n, ldv) rows the Trotter reads
@@ -422,7 +422,7 @@ class system_batch:
                 k = pad8(m["o"] * m["n"])
                 (bh, bl), (ah, al) = m["y_hl"], self.greens_hl
                 p.M, p.N, p.K = 2 * m["nw"], m["o"] * m["n"], k
-                p.ldc, p.C, p.alpha, p.c_fp64 = m["o"] * m["n"], self.exch_z.ptr(i), 1.0, 1
+                p.ldc, p.C, p.alpha, p.c_fp64 = m["o"] * m["n"], self.exch_z.ptr(i), 2.0, 1  # Z = 2 G L^T
             else:
                 # V's upper blocks as pair words, stored as they come
                 k = pad8(m["g"])

@@ -463,6 +463,11 @@ def kernel_table(ktimes, n, o, g, nw, nspin, rhf, precision):
         "afq_dgemm_nt:exchange": (4.0 * nw * (o * n) ** 2, "fp64"),
         "afq_sgemm_nt:exchange": (4.0 * nw * (o * n) ** 2, "fp32"),
         "afq_gemm_bf16x3_nt:exchange": (4.0 * nw * (o * n) ** 2, "bf16x3"),
+        # the same with Y's lower triangle (Z = 2 G L^T, column tiles stopping at
+        # the diagonal): 2 nw rows x the on (on + 1) / 2 entries of L, 2 flops each
+        "afq_dgemm_nt_lower:exchange": (2.0 * nw * o * n * (o * n + 1), "fp64"),
+        "afq_sgemm_nt_lower:exchange": (2.0 * nw * o * n * (o * n + 1), "fp32"),
+        "afq_gemm_bf16x3_nt_lower:exchange": (2.0 * nw * o * n * (o * n + 1), "bf16x3"),
     }
     out = {}
     for name, t in ktimes.items():

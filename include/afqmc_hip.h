@@ -76,6 +76,26 @@ int afq_gemm_bf16x3_nt(int M, int N, int K, float alpha, const unsigned short* A
                        int c_fp64, void* stream);
 
 /*
+ * The GEMMs above for a B whose row j is zero past column j (B[j][k] = 0 for
+ * k > j; those entries must hold zeros, the tile on the diagonal still reads
+ * them): every tile of output columns n0 .. n0 + bn - 1 stops its k-loop at
+ * k = n0 + bn, which skips about half the products when N = K.  beta = 0.
+ * Same results as the full-K call on the same B, bit for bit (the skipped
+ * products are zeros).  The exchange's quadratic form vec(G)^T Y vec(G)
+ * (meas/rhf.py:36-50, 75-105 regrouped) with Y symmetric equals
+ * 2 vec(G)^T L^T vec(G) for L the lower triangle of Y with its diagonal halved:
+ * Z = 2 G L^T on these entry points, half the products of Z = G Y.
+ */
+int afq_dgemm_nt_lower(int M, int N, int K, double alpha, const double* A, long long lda, const double* B,
+                       long long ldb, double* C, long long ldc, void* stream);
+int afq_sgemm_nt_lower(int M, int N, int K, float alpha, const float* A, long long lda, const float* B,
+                       long long ldb, void* C, long long ldc, int c_fp64, void* stream);
+int afq_gemm_bf16x3_nt_lower(int M, int N, int K, float alpha, const unsigned short* A_hi,
+                             const unsigned short* A_lo, long long lda, const unsigned short* B_hi,
+                             const unsigned short* B_lo, long long ldb, void* C, long long ldc, int c_fp64,
+                             void* stream);
+
+/*
  * afq_gemm_bf16x3_nt with the K range cut into `splits` chunks (whole 32-wide
  * k-steps) that run side by side: for problems of few output tiles, which then
  * fill the GPU.  Each chunk writes its fp32 partial product to workspace

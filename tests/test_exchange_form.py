@@ -2,12 +2,14 @@
 
 energy_kernel_r/u (meas/rhf.py:45-47) sums tr(f_g f_g) over the Cholesky
 index with f_g = R_g G^T (o x o).  The measurement regroups that sum as
-vec(G)^T Y vec(G) with Y built once per context from R^T R
-(rhf_meas_ctx.exchange_operand), so the step runs one GEMM Z = G Y over all
-walkers.  Y is checked here against the oracle's per-g sum on complex Green's
-functions (where a transposed regroup or a conjugation would show), for odd
-sizes and both spin sectors; its symmetry, and the cost rule that picks the
-form."""
+vec(G)^T Y vec(G) with Y built once per context from R^T R; the context
+keeps L, Y's lower triangle with its diagonal halved (Y = L + L^T,
+rhf_meas_ctx.exchange_operand), so the step runs one GEMM Z = 2 G L^T over all
+walkers whose column tiles stop at the diagonal.  The form is checked here
+against the oracle's per-g sum on complex Green's functions (where a
+transposed regroup or a conjugation would show), for odd sizes and both spin
+sectors, through both the quadratic form and the assembly's row dots of G and
+Z; L's shape, and the cost rule that picks the form."""
 import numpy as np
 import pytest
 import torch
@@ -31,13 +33,37 @@ def test_quadratic_form_equals_per_g_exchange(o, n, g):
     nw = 5
     gr = rng.standard_normal((nw, o, n)) + 1j * rng.standard_normal((nw, o, n))
     for s in range(2):
-        y = ctx.exchange_operand(s).numpy()
-        assert y.shape == (o * n, o * n)
+        lo = ctx.exchange_operand(s).numpy()
+        assert lo.shape == (o * n, o * n)
+        np.testing.assert_array_equal(lo, np.tril(lo))        # nothing above the diagonal
+        y = lo + lo.T
         np.testing.assert_array_equal(y, y.T)
         v = gr.reshape(nw, o * n)
-        quad = np.einsum("wi,ij,wj->w", v, y, v)
         _, exc = orc._coulomb_exchange(gr, rot[s])
-        np.testing.assert_allclose(quad, exc, rtol=1e-12, atol=1e-12 * np.abs(exc).max())
+        tol = dict(rtol=1e-12, atol=1e-12 * np.abs(exc).max())
+        np.testing.assert_allclose(np.einsum("wi,ij,wj->w", v, y, v), exc, **tol)
+        # the assembly's form: Z = 2 G L^T on the real and imaginary rows, then
+        # re <a, Za> - <b, Zb>, im <a, Zb> + <b, Za>
+        a, b = v.real, v.imag
+        za, zb = 2.0 * a @ lo.T, 2.0 * b @ lo.T
+        asm = (np.einsum("wi,wi->w", a, za) - np.einsum("wi,wi->w", b, zb)
+               + 1j * (np.einsum("wi,wi->w", a, zb) + np.einsum("wi,wi->w", b, za)))
+        np.testing.assert_allclose(asm, exc, **tol)
+
+
+def test_lower_triangle_without_halved_diagonal_fails():
+    """A planted mistake: L without the halved diagonal counts the diagonal of
+    Y twice (so the check above is not vacuous)."""
+    rng = np.random.default_rng(5)
+    o, n, g = 3, 5, 7
+    rot = rng.standard_normal((1, g, o, n))
+    ctx = _ctx(np.concatenate([rot, rot]))
+    lo = ctx.exchange_operand(0).numpy()
+    wrong = np.tril(lo + lo.T)
+    gr = rng.standard_normal((4, o, n)) + 1j * rng.standard_normal((4, o, n))
+    v = gr.reshape(4, o * n)
+    _, exc = orc._coulomb_exchange(gr, rot[0])
+    assert np.abs(2.0 * np.einsum("wi,ij,wj->w", v, wrong, v) - exc).max() > 1e-3 * np.abs(exc).max()
 
 
 def test_quadratic_form_transposed_regroup_fails():
