@@ -104,7 +104,7 @@ _SIGNATURES = {
     "afq_unpack_pairs": [c_int, c_int, c_vp, c_ll, c_vp, c_vp],
 }
 
-ABI_VERSION = 22
+ABI_VERSION = 23
 
 
 class HipLibraryError(RuntimeError):

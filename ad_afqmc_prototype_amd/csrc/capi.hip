@@ -63,4 +63,4 @@ int cu_count() {
 
 extern "C" const char* afq_last_error(void) { return g_last_error; }
 
-extern "C" int afq_abi_version(void) { return 22; }
+extern "C" int afq_abi_version(void) { return 23; }

@@ -270,9 +270,12 @@ __global__ __launch_bounds__(THREADS, 2) void sgemm_grouped_kernel(int count, in
     const int tiles_m = afq::ceil_div(M, CF::BM);
     const int local = t - p.tile_start;
     const int m0 = (local % tiles_m) * CF::BM;
-    const int n0 = (local / tiles_m) * CF::BN;
+    const int nn = local / tiles_m;
+    // b_lower: column tiles in zigzag order, each k-loop stopping at its last column
+    const int n0 = (p.b_lower ? afq::zigzag(nn, afq::ceil_div(N, CF::BN)) : nn) * CF::BN;
+    const int K = p.b_lower ? min(p.K, n0 + CF::BN) : p.K;
     typename CF::acc_t acc[TM][TN];
-    gemm_nt_tile<float, TM, TN>(M, N, p.K, p.A, p.lda, p.B, p.ldb, m0, n0, As, Bs, acc);
+    gemm_nt_tile<float, TM, TN>(M, N, K, p.A, p.lda, p.B, p.ldb, m0, n0, As, Bs, acc);
     if (p.sym_n > 0)
         gemm_store_sym<float, TM, TN>(M, p.alpha, (float*)p.C,
                                       afq::sym_dst{p.sym_n, p.sym_ldv, afq::ceil_div(p.sym_n, 16)}, m0, n0, acc);

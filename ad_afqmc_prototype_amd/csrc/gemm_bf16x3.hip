@@ -203,18 +203,21 @@ __global__ __launch_bounds__(THREADS, 2) void gemm_bf16x3_grouped_kernel(int cou
     const int tiles_m = afq::ceil_div(p.M, BM);
     const int local = t - p.tile_start;
     const int m0 = (local % tiles_m) * BM;
-    const int n0 = (local / tiles_m) * BN;
+    const int nn = local / tiles_m;
+    // b_lower: column tiles in zigzag order, each k-loop stopping at its last column
+    const int n0 = (p.b_lower ? afq::zigzag(nn, afq::ceil_div(p.N, BN)) : nn) * BN;
+    const int K = p.b_lower ? min(p.K, n0 + BN) : p.K;
     if (p.sym_n > 0)
-        bf16x3_tile<float, true>(p.M, p.N, p.K, p.alpha, p.A_hi, p.A_lo, p.lda, p.B_hi, p.B_lo, p.ldb, (float*)p.C,
+        bf16x3_tile<float, true>(p.M, p.N, K, p.alpha, p.A_hi, p.A_lo, p.lda, p.B_hi, p.B_lo, p.ldb, (float*)p.C,
                                  0LL, m0, n0, afq::sym_dst{p.sym_n, p.sym_ldv, afq::ceil_div(p.sym_n, 16)}, S);
     else if (p.c_fp64 == 2)
-        bf16x3_tile<afq::bf16_pair, false>(p.M, p.N, p.K, p.alpha, p.A_hi, p.A_lo, p.lda, p.B_hi, p.B_lo, p.ldb,
+        bf16x3_tile<afq::bf16_pair, false>(p.M, p.N, K, p.alpha, p.A_hi, p.A_lo, p.lda, p.B_hi, p.B_lo, p.ldb,
                                            (afq::bf16_pair*)p.C, p.ldc, m0, n0, afq::sym_dst{0, 0, 0}, S);
     else if (p.c_fp64)
-        bf16x3_tile<double, false>(p.M, p.N, p.K, p.alpha, p.A_hi, p.A_lo, p.lda, p.B_hi, p.B_lo, p.ldb,
+        bf16x3_tile<double, false>(p.M, p.N, K, p.alpha, p.A_hi, p.A_lo, p.lda, p.B_hi, p.B_lo, p.ldb,
                                    (double*)p.C, p.ldc, m0, n0, afq::sym_dst{0, 0, 0}, S);
     else
-        bf16x3_tile<float, false>(p.M, p.N, p.K, p.alpha, p.A_hi, p.A_lo, p.lda, p.B_hi, p.B_lo, p.ldb,
+        bf16x3_tile<float, false>(p.M, p.N, K, p.alpha, p.A_hi, p.A_lo, p.lda, p.B_hi, p.B_lo, p.ldb,
                                   (float*)p.C, p.ldc, m0, n0, afq::sym_dst{0, 0, 0}, S);
 }
 
@@ -621,9 +624,11 @@ __global__ __launch_bounds__(128 * WM, 1) void gemm_bf16x3_grouped_dma_kernel(
     const int tiles_m = afq::ceil_div(p.M, D::BM);
     const int local = t - p.tile_start;
     const int m0 = (local % tiles_m) * D::BM;
-    const int n0 = (local / tiles_m) * 256;
+    const int nn = local / tiles_m;
+    const int n0 = (p.b_lower ? afq::zigzag(nn, afq::ceil_div(p.N, 256)) : nn) * 256;
+    const int K = p.b_lower ? min(p.K, n0 + 256) : p.K;
     f32x4 acc[4][8];
-    dma_mainloop<WM>(p.M, p.N, p.K, p.A_hi, p.A_lo, p.lda, p.B_hi, p.B_lo, p.ldb, m0, n0, S, acc);
+    dma_mainloop<WM>(p.M, p.N, K, p.A_hi, p.A_lo, p.lda, p.B_hi, p.B_lo, p.ldb, m0, n0, S, acc);
     if (p.sym_n > 0) {
         const afq::sym_dst sd{p.sym_n, p.sym_ldv, afq::ceil_div(p.sym_n, 16)};
         int bi, bj;

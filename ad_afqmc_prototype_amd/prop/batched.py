@@ -80,7 +80,7 @@ class _gemm_problem(ctypes.Structure):
     _fields_ = [("M", ctypes.c_int), ("N", ctypes.c_int), ("K", ctypes.c_int), ("tile_start", ctypes.c_int),
                 ("lda", ctypes.c_longlong), ("ldb", ctypes.c_longlong), ("ldc", ctypes.c_longlong),
                 ("A", _vp), ("B", _vp), ("C", _vp), ("alpha", ctypes.c_float), ("c_fp64", ctypes.c_int),
-                ("sym_n", ctypes.c_int), ("sym_ldv", ctypes.c_int)]
+                ("sym_n", ctypes.c_int), ("sym_ldv", ctypes.c_int), ("b_lower", ctypes.c_int)]
 
 
 class _gemm_problem_bf16x3(ctypes.Structure):
@@ -90,7 +90,7 @@ class _gemm_problem_bf16x3(ctypes.Structure):
                 ("lda", ctypes.c_longlong), ("ldb", ctypes.c_longlong), ("ldc", ctypes.c_longlong),
                 ("A_hi", _vp), ("A_lo", _vp), ("B_hi", _vp), ("B_lo", _vp), ("C", _vp),
                 ("alpha", ctypes.c_float), ("c_fp64", ctypes.c_int), ("sym_n", ctypes.c_int),
-                ("sym_ldv", ctypes.c_int)]
+                ("sym_ldv", ctypes.c_int), ("b_lower", ctypes.c_int)]
 
 
 class _split_problem(ctypes.Structure):
@@ -101,8 +101,8 @@ class _split_problem(ctypes.Structure):
 
 
 assert ctypes.sizeof(_system_step) == 8 * 4 + 8 * 8 + 26 * 8 + 8 + 2 * 8 + 8 + 8 + 8
-assert ctypes.sizeof(_gemm_problem) == 80
-assert ctypes.sizeof(_gemm_problem_bf16x3) == 96
+assert ctypes.sizeof(_gemm_problem) == 88
+assert ctypes.sizeof(_gemm_problem_bf16x3) == 104
 assert ctypes.sizeof(_split_problem) == 40
 GEMM_SHAPES = (0, 1, 2, 3)        # grouped fp32 GEMM tile shapes
 BF16X3_SHAPES = (0, 1, 2)         # grouped bf16x3 GEMM tile shapes
@@ -393,7 +393,7 @@ class system_batch:
                 p.M, p.N, p.K = 2 * m["nw"], k, k
                 p.lda, p.ldb, p.ldc = k, k, k
                 p.A, p.B, p.C = self.greens32.ptr(i), m["y32"].data_ptr(), self.exch_z.ptr(i)
-                p.alpha, p.c_fp64 = 2.0, 1  # Z = 2 G L^T
+                p.alpha, p.c_fp64, p.b_lower = 2.0, 1, 1  # Z = 2 G L^T, L lower triangular
             else:
                 # symmetric VHS: the upper 16 x 16 blocks, each entry stored twice
                 # into the (n, ldv) rows the Trotter reads
@@ -422,7 +422,8 @@ class system_batch:
                 k = pad8(m["o"] * m["n"])
                 (bh, bl), (ah, al) = m["y_hl"], self.greens_hl
                 p.M, p.N, p.K = 2 * m["nw"], m["o"] * m["n"], k
-                p.ldc, p.C, p.alpha, p.c_fp64 = m["o"] * m["n"], self.exch_z.ptr(i), 2.0, 1  # Z = 2 G L^T
+                p.ldc, p.C, p.alpha, p.c_fp64 = m["o"] * m["n"], self.exch_z.ptr(i), 2.0, 1
+                p.b_lower = 1  # Z = 2 G L^T, L lower triangular
             else:
                 # V's upper blocks as pair words, stored as they come
                 k = pad8(m["g"])

@@ -543,6 +543,8 @@ typedef struct afq_gemm_problem {
      * in C's rows of sym_n x sym_ldv floats (sym_ldv a multiple of 4, C float32
      * and 16-byte aligned; ldc unused) */
     int sym_n, sym_ldv;
+    int b_lower;                 /* 1: row j of B is zero past column j, and every column tile's k-loop stops
+                                    at its last column (as afq_sgemm_nt_lower) */
 } afq_gemm_problem;
 
 /*
@@ -576,6 +578,7 @@ typedef struct afq_gemm_problem_bf16x3 {
     float alpha;
     int c_fp64;
     int sym_n, sym_ldv;          /* sym_n > 0: the symmetric VHS store of afq_gemm_problem */
+    int b_lower;                 /* as afq_gemm_problem's (afq_gemm_bf16x3_nt_lower) */
 } afq_gemm_problem_bf16x3;
 
 /*

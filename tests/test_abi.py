@@ -114,6 +114,9 @@ int main(void) {
   O(afq_system_step, rot_chol); O(afq_system_step, energy); O(afq_system_step, exch_z); O(afq_system_step, exp_h1);
   O(afq_gemm_problem, tile_start); O(afq_gemm_problem, ldc);
   O(afq_gemm_problem, C); O(afq_gemm_problem, alpha); O(afq_gemm_problem, c_fp64);
+  O(afq_gemm_problem, b_lower);
+  printf("size_gemm3 %zu\n", sizeof(afq_gemm_problem_bf16x3));
+  printf("b_lower3 %zu\n", offsetof(afq_gemm_problem_bf16x3, b_lower));
   return 0;
 }
 '''
@@ -130,5 +133,7 @@ int main(void) {
     for f in ("tblock0", "sqrt_dt", "damping", "trial", "vhs", "pop_out", "key_out", "ldr", "h0",
               "rot_chol", "energy", "exch_z", "exp_h1"):
         assert int(out[f]) == getattr(B._system_step, f).offset, f
-    for f in ("tile_start", "ldc", "C", "alpha", "c_fp64"):
+    for f in ("tile_start", "ldc", "C", "alpha", "c_fp64", "b_lower"):
         assert int(out[f]) == getattr(B._gemm_problem, f).offset, f
+    assert int(out["size_gemm3"]) == ctypes.sizeof(B._gemm_problem_bf16x3)
+    assert int(out["b_lower3"]) == B._gemm_problem_bf16x3.b_lower.offset

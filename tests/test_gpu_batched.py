@@ -281,6 +281,62 @@ def test_grouped_bf16x3_gemm_and_split_match_single(gpu_device, shape):
         assert torch.equal(out, ref), f"problem {i}: grouped bf16x3 GEMM (shape {shape}) differs"
 
 
+@pytest.mark.parametrize("precision,shape", [("fp32", 0), ("fp32", 3), ("bf16x3", 0), ("bf16x3", 2)])
+def test_grouped_lower_b_matches_single(gpu_device, precision, shape):
+    """Grouped problems flagged b_lower (B zero past its diagonal, column tiles
+    stopping there in zigzag order) equal afq_sgemm_nt_lower /
+    afq_gemm_bf16x3_nt_lower on each problem bit for bit, and problems without
+    the flag in the same launch equal the full GEMM."""
+    import torch
+    from ad_afqmc_prototype_amd import _lib, kernels
+    from ad_afqmc_prototype_amd.prop import batched as B
+    print(f"grouped lower-B GEMM {precision} shape {shape}", flush=True)
+    g = torch.Generator(device="cpu").manual_seed(17)
+    dims = [(96, 120, 120), (300, 264, 264), (37, 40, 40), (130, 520, 520), (64, 200, 96)]
+    st = torch.cuda.current_stream().cuda_stream
+    bm, bn = B.gemm_tile_shape(shape, precision)
+    bf = precision == "bf16x3"
+    arr = ((B._gemm_problem_bf16x3 if bf else B._gemm_problem) * len(dims))()
+    keep, outs, refs, start = [], [], [], 0
+    for i, (p, (M, N, K)) in enumerate(zip(arr, dims)):
+        lower = i != len(dims) - 1
+        a = torch.randn((M, K), generator=g, dtype=torch.float64).to(gpu_device)
+        b = torch.randn((N, K), generator=g, dtype=torch.float64)
+        b = (torch.tril(b) if lower else b).to(gpu_device)
+        ref = torch.empty((M, N), dtype=torch.float64, device=gpu_device)
+        out = torch.full_like(ref, float("nan"))
+        if bf:
+            (ah, al), (bh, bl) = kernels.split_bf16(a), kernels.split_bf16(b)
+            kp = ah.shape[1]
+            if lower:
+                kernels.gemm_nt_lower((ah, al), (bh, bl), ref, alpha=2.0)
+            else:
+                kernels.gemm_bf16x3_nt(ah, al, bh, bl, ref, alpha=2.0, splits=1)
+            p.A_hi, p.A_lo, p.B_hi, p.B_lo = ah.data_ptr(), al.data_ptr(), bh.data_ptr(), bl.data_ptr()
+            keep += [ah, al, bh, bl]
+        else:
+            a32, b32 = kernels.convert_f32(a), kernels.convert_f32(b)
+            kp = K
+            if lower:
+                kernels.gemm_nt_lower(a32, b32, ref, alpha=2.0)
+            else:
+                kernels.sgemm_nt(a32, b32, ref, alpha=2.0)
+            p.A, p.B = a32.data_ptr(), b32.data_ptr()
+            keep += [a32, b32]
+        p.M, p.N, p.K, p.tile_start = M, N, kp, start
+        p.lda, p.ldb, p.ldc = kp, kp, N
+        p.C, p.alpha, p.c_fp64, p.b_lower = out.data_ptr(), 2.0, 1, int(lower)
+        start += -(-M // bm) * -(-N // bn)
+        outs.append(out)
+        refs.append(ref)
+    tab = B._upload(arr).to(gpu_device)
+    entry = "afq_gemm_bf16x3_nt_grouped" if bf else "afq_sgemm_nt_grouped"
+    _lib.call(entry, len(dims), tab.data_ptr(), start, shape, st)
+    torch.cuda.synchronize()
+    for i, (out, ref) in enumerate(zip(outs, refs)):
+        assert torch.equal(out, ref), f"problem {i}: grouped GEMM differs"
+
+
 def test_grouped_bf16x3_energy_matches_single(gpu_device):
     """The grouped bf16x3 energy (one launch per occupied-tile bucket, two
     walkers per workgroup where they fit) equals afq_energy_bf16x3 on each
