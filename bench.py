@@ -579,7 +579,7 @@ def main():
 
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); the walker "
-              "step shards over walkers, which is described in DESIGN.md but not run here.",
+              "step would shard over walkers (one process per GPU), which is not run here.",
               file=sys.stderr)
         sys.exit(2)
 
