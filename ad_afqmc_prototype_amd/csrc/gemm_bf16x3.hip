@@ -723,7 +723,8 @@ unsigned grid_for(long long total) {
 namespace {
 
 // afq_gemm_bf16x3_nt and afq_gemm_bf16x3_nt_lower: 256 x 256 tiles once they
-// give at least two workgroups per CU, else 128 x 128
+// give at least two workgroups per CU (one with a lower-triangular B), else
+// 128 x 128 (the same sums in either tiling)
 int bf16x3_nt(const char* name, int M, int N, int K, float alpha, const unsigned short* A_hi,
               const unsigned short* A_lo, long long lda, const unsigned short* B_hi, const unsigned short* B_lo,
               long long ldb, void* C, long long ldc, int c_fp64, int lower_b, hipStream_t st) {
@@ -737,7 +738,10 @@ int bf16x3_nt(const char* name, int M, int N, int K, float alpha, const unsigned
     const int tiles_m2 = afq::ceil_div(M, BM2);
     const int tiles_n2 = afq::ceil_div(N, BN2);
     const long long total2 = (long long)tiles_m2 * tiles_n2;
-    if (total2 >= 512 && total2 < (1LL << 31)) {
+    // a lower-triangular B halves and evens out the work per tile: one round of
+    // 256 x 256 tiles already fills the CUs
+    const long long min2 = lower_b ? afq::cu_count() : 512;
+    if (total2 >= min2 && total2 < (1LL << 31)) {
         const dim3 g((unsigned)total2);
         if (c_fp64 == 2)
             hipLaunchKernelGGL(gemm_bf16x3_nt_256_kernel<afq::bf16_pair>, g, dim3(THREADS2), 0, st, M, N, K, alpha,
